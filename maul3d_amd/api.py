@@ -1,0 +1,248 @@
+"""ctypes mirror of the Maul3D public C API.
+
+The same bindings drive any shared library that exports the ``m3*`` surface:
+this package's ``libmaul3d_amd.so`` (the MI355X build) and, in tests only,
+the reference engine compiled as the parity checker.  Struct layouts follow
+the reference public headers field for field:
+
+* ``m3Vec3``/``m3Pos3``/``m3Quat``/``m3Transform``  include/maul3d/math.h:24-52
+* ``m3WorldId``/``m3BodyId``/``m3ShapeId``          include/maul3d/base.h:64-104
+* ``m3WorldDef``                                     include/maul3d/world.h:48-107
+* ``m3BodyDef``                                      include/maul3d/body.h:18-40
+* ``m3ShapeDef``                                     include/maul3d/shape.h:46-80
+* ``m3Counters``/``m3Profile``                       include/maul3d/world.h:531-593
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+c_f = C.c_float
+c_d = C.c_double
+c_i32 = C.c_int32
+c_u16 = C.c_uint16
+c_u64 = C.c_uint64
+
+
+class Vec3(C.Structure):
+    _fields_ = [("x", c_f), ("y", c_f), ("z", c_f)]
+
+
+class Pos3(C.Structure):
+    _fields_ = [("x", c_d), ("y", c_d), ("z", c_d)]
+
+
+class Quat(C.Structure):
+    _fields_ = [("x", c_f), ("y", c_f), ("z", c_f), ("w", c_f)]
+
+
+class Transform(C.Structure):
+    _fields_ = [("p", Pos3), ("q", Quat)]
+
+
+class WorldId(C.Structure):
+    _fields_ = [("index1", c_i32), ("generation", c_u16)]
+
+
+class BodyId(C.Structure):
+    _fields_ = [("index1", c_i32), ("world0", c_u16), ("generation", c_u16)]
+
+
+class ShapeId(C.Structure):
+    _fields_ = [("index1", c_i32), ("world0", c_u16), ("generation", c_u16)]
+
+
+class JointId(C.Structure):
+    _fields_ = [("index1", c_i32), ("world0", c_u16), ("generation", c_u16)]
+
+
+class WorldDef(C.Structure):
+    _fields_ = [
+        ("gravity", Vec3),
+        ("bodyCapacity", c_i32),
+        ("shapeCapacity", c_i32),
+        ("meshCapacity", c_i32),
+        ("jointCapacity", c_i32),
+        ("voxelCapacity", c_i32),
+        ("characterCapacity", c_i32),
+        ("vehicleCapacity", c_i32),
+        ("softBodyCapacity", c_i32),
+        ("workerCount", c_i32),
+        ("enqueueTask", C.c_void_p),
+        ("finishTask", C.c_void_p),
+        ("userTaskContext", C.c_void_p),
+        ("contactHertz", c_f),
+        ("contactDampingRatio", c_f),
+        ("contactPushMaxSpeed", c_f),
+        ("restitutionThreshold", c_f),
+        ("maximumLinearSpeed", c_f),
+        ("enableSleeping", c_i32),
+        ("enableContinuous", c_i32),
+        ("hitEventThreshold", c_f),
+        ("internalValue", c_i32),
+    ]
+
+
+class BodyDef(C.Structure):
+    _fields_ = [
+        ("type", c_i32),
+        ("position", Pos3),
+        ("rotation", Quat),
+        ("linearVelocity", Vec3),
+        ("angularVelocity", Vec3),
+        ("gravityScale", c_f),
+        ("linearDamping", c_f),
+        ("angularDamping", c_f),
+        ("userData", c_u64),
+        ("isBullet", C.c_bool),
+        ("internalValue", c_i32),
+    ]
+
+
+class ShapeDef(C.Structure):
+    _fields_ = [
+        ("density", c_f),
+        ("friction", c_f),
+        ("restitution", c_f),
+        ("categoryBits", c_u64),
+        ("maskBits", c_u64),
+        ("groupIndex", c_i32),
+        ("rollingResistance", c_f),
+        ("userData", c_u64),
+        ("isSensor", C.c_bool),
+        ("enableHitEvents", C.c_bool),
+        ("enablePreSolveEvents", C.c_bool),
+        ("localPosition", Vec3),
+        ("localRotation", Quat),
+        ("internalValue", c_i32),
+    ]
+
+
+class Sphere(C.Structure):
+    _fields_ = [("center", Vec3), ("radius", c_f)]
+
+
+class Plane(C.Structure):
+    _fields_ = [("normal", Vec3), ("offset", c_f)]
+
+
+class Capsule(C.Structure):
+    _fields_ = [("point1", Vec3), ("point2", Vec3), ("radius", c_f)]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("bodyCount", c_i32),
+        ("shapeCount", c_i32),
+        ("jointCount", c_i32),
+        ("contactCount", c_i32),
+        ("awakeCount", c_i32),
+        ("islandCount", c_i32),
+        ("colorCount", c_i32),
+        ("characterCount", c_i32),
+        ("vehicleCount", c_i32),
+        ("softBodyCount", c_i32),
+        ("voxelChunkCount", c_i32),
+        ("hullCount", c_i32),
+        ("meshCount", c_i32),
+        ("treeHeight", c_i32),
+        ("scratchPeak", c_i32),
+        ("scratchCapacity", c_i32),
+        ("snapshotBytes", c_i32),
+        ("allocCalls", C.c_int64),
+        ("freeCalls", C.c_int64),
+    ]
+
+
+class Profile(C.Structure):
+    _fields_ = [
+        (n, c_f)
+        for n in (
+            "step",
+            "vehicles",
+            "broadphase",
+            "narrowphase",
+            "events",
+            "prepare",
+            "solve",
+            "continuous",
+            "sleep",
+            "characters",
+            "softBodies",
+        )
+    ]
+
+
+STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "m3DefaultWorldDef": (WorldDef, []),
+    "m3CreateWorld": (WorldId, [C.POINTER(WorldDef)]),
+    "m3DestroyWorld": (None, [WorldId]),
+    "m3World_IsValid": (C.c_bool, [WorldId]),
+    "m3World_Step": (None, [WorldId, c_f, c_i32]),
+    "m3World_Hash": (c_u64, [WorldId]),
+    "m3World_SnapshotSize": (c_i32, [WorldId]),
+    "m3World_Snapshot": (c_i32, [WorldId, C.c_void_p, c_i32]),
+    "m3World_Restore": (C.c_bool, [WorldId, C.c_void_p, c_i32]),
+    "m3World_GetCounters": (Counters, [WorldId]),
+    "m3World_GetProfile": (Profile, [WorldId]),
+    "m3World_GetGravity": (Vec3, [WorldId]),
+    "m3World_SetGravity": (None, [WorldId, Vec3]),
+    "m3World_EnableSleeping": (None, [WorldId, C.c_bool]),
+    "m3World_EnableContinuous": (None, [WorldId, C.c_bool]),
+    "m3DefaultBodyDef": (BodyDef, []),
+    "m3CreateBody": (BodyId, [WorldId, C.POINTER(BodyDef)]),
+    "m3DestroyBody": (None, [BodyId]),
+    "m3Body_IsValid": (C.c_bool, [BodyId]),
+    "m3Body_GetPosition": (Pos3, [BodyId]),
+    "m3Body_GetRotation": (Quat, [BodyId]),
+    "m3Body_GetLinearVelocity": (Vec3, [BodyId]),
+    "m3Body_GetAngularVelocity": (Vec3, [BodyId]),
+    "m3Body_SetLinearVelocity": (None, [BodyId, Vec3]),
+    "m3Body_SetAngularVelocity": (None, [BodyId, Vec3]),
+    "m3Body_IsAwake": (C.c_bool, [BodyId]),
+    "m3Body_SetAwake": (None, [BodyId, C.c_bool]),
+    "m3Body_ApplyForce": (None, [BodyId, Vec3]),
+    "m3Body_ApplyTorque": (None, [BodyId, Vec3]),
+    "m3DefaultShapeDef": (ShapeDef, []),
+    "m3CreateSphereShape": (ShapeId, [BodyId, C.POINTER(ShapeDef), C.POINTER(Sphere)]),
+    "m3CreatePlaneShape": (ShapeId, [BodyId, C.POINTER(ShapeDef), C.POINTER(Plane)]),
+    "m3CreateBoxShape": (ShapeId, [BodyId, C.POINTER(ShapeDef), Vec3]),
+    "m3CreateCapsuleShape": (ShapeId, [BodyId, C.POINTER(ShapeDef), C.POINTER(Capsule)]),
+    "m3CreateHullShape": (ShapeId, [BodyId, C.POINTER(ShapeDef), C.POINTER(Vec3), c_i32]),
+    "m3Hash64": (c_u64, [c_u64, C.c_void_p, c_i32]),
+}
+
+
+class Api:
+    """Typed view of one library's ``m3*`` exports."""
+
+    def __init__(self, lib: C.CDLL, optional: tuple[str, ...] = ()):
+        self.lib = lib
+        for name, (res, args) in _SIGNATURES.items():
+            try:
+                fn = getattr(lib, name)
+            except AttributeError:
+                if name in optional:
+                    continue
+                raise
+            fn.restype = res
+            fn.argtypes = args
+            setattr(self, name, fn)
+
+    # convenience wrappers -------------------------------------------------
+    def snapshot(self, world: WorldId) -> bytes:
+        size = self.m3World_SnapshotSize(world)
+        if size < 0:
+            raise RuntimeError("m3World_SnapshotSize refused the world")
+        buf = C.create_string_buffer(size)
+        written = self.m3World_Snapshot(world, buf, size)
+        if written != size:
+            raise RuntimeError(f"m3World_Snapshot wrote {written} of {size} bytes")
+        return buf.raw
+
+    def restore(self, world: WorldId, data: bytes) -> bool:
+        buf = C.create_string_buffer(data, len(data))
+        return bool(self.m3World_Restore(world, buf, len(data)))

@@ -1,0 +1,254 @@
+"""Seeded scene builders over the public ``m3*`` API.
+
+Every builder takes an :class:`maul3d_amd.api.Api` and returns the world id,
+so one scene definition feeds the MI355X library and the CPU checker with
+bit-identical inputs.  Randomness is SplitMix64 with the 53-bit uniform draw
+(restated from bench/bench_main.c:43-55); every float handed to the API is the
+round-to-nearest cast of that double, as the C harness does.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+from .api import (
+    DYNAMIC,
+    Api,
+    BodyDef,
+    Capsule,
+    Plane,
+    Pos3,
+    Quat,
+    ShapeDef,
+    Sphere,
+    Vec3,
+    WorldId,
+)
+
+_MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 stream and its [lo, hi) double draw (bench/bench_main.c:43-55)."""
+
+    def __init__(self, seed: int):
+        self.state = seed & _MASK64
+
+    def next(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _MASK64
+        return z ^ (z >> 31)
+
+    def range(self, lo: float, hi: float) -> float:
+        return lo + (hi - lo) * (float(self.next() >> 11) / 9007199254740992.0)
+
+    def unit_quat(self) -> tuple[float, float, float, float]:
+        """Uniform rotation by rejection from the 4-cube, normalized in double."""
+        while True:
+            x = self.range(-1.0, 1.0)
+            y = self.range(-1.0, 1.0)
+            z = self.range(-1.0, 1.0)
+            w = self.range(-1.0, 1.0)
+            s = x * x + y * y + z * z + w * w
+            if 1.0e-4 < s <= 1.0:
+                inv = 1.0 / math.sqrt(s)
+                return (x * inv, y * inv, z * inv, w * inv)
+
+
+def _world(api: Api, bodies: int, shapes: int, joints: int | None = None) -> WorldId:
+    d = api.m3DefaultWorldDef()
+    d.bodyCapacity = bodies
+    d.shapeCapacity = shapes
+    if joints is not None:
+        d.jointCapacity = joints
+    w = api.m3CreateWorld(C.byref(d))
+    if w.index1 == 0:
+        raise RuntimeError("m3CreateWorld refused the def")
+    return w
+
+
+def _ground(api: Api, world: WorldId, friction: float = 0.6) -> None:
+    gd = api.m3DefaultBodyDef()
+    ground = api.m3CreateBody(world, C.byref(gd))
+    sd = api.m3DefaultShapeDef()
+    sd.friction = friction
+    plane = Plane(Vec3(0.0, 1.0, 0.0), 0.0)
+    if api.m3CreatePlaneShape(ground, C.byref(sd), C.byref(plane)).index1 == 0:
+        raise RuntimeError("plane refused")
+
+
+def _check(shape_id, what: str) -> None:
+    if shape_id.index1 == 0:
+        raise RuntimeError(f"{what} refused")
+
+
+def bench_pyramid(api: Api) -> WorldId:
+    """The reference bench's sphere pyramid (bench/bench_main.c:68-95).
+
+    Its hash after 300 steps of (1/60, 4) is pinned at 975f319f2f44cc5d in
+    bench/pins.txt:1.
+    """
+    world = _world(api, 256, 256)
+    _ground(api, world)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    sd = api.m3DefaultShapeDef()
+    sd.friction = 0.6
+    for layer in range(6):
+        n = 6 - layer
+        for i in range(n):
+            for j in range(n):
+                bd.position = Pos3(
+                    float(i) - 0.5 * float(n) + 0.05 * float(layer),
+                    0.5 + 1.02 * float(layer),
+                    float(j) - 0.5 * float(n),
+                )
+                body = api.m3CreateBody(world, C.byref(bd))
+                ball = Sphere(Vec3(0.0, 0.0, 0.0), 0.5)
+                _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+    return world
+
+
+def smoke10k(api: Api) -> WorldId:
+    """The reference bench's 10k mixed rain (bench/bench_main.c:612-650).
+
+    Hash after 150 steps pinned at bc84e0074315a0b3 (bench/pins.txt:7).
+    """
+    world = _world(api, 16384, 16384)
+    _ground(api, world)
+    sd = api.m3DefaultShapeDef()
+    sd.friction = 0.6
+    sd.rollingResistance = 0.05
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    rng = SplitMix64(0x10A11)
+    for k in range(10000):
+        x = rng.range(-46.0, 46.0)
+        y = rng.range(8.0, 60.0)
+        z = rng.range(-46.0, 46.0)
+        bd.position = Pos3(x, y, z)
+        body = api.m3CreateBody(world, C.byref(bd))
+        if k % 3 == 0:
+            ball = Sphere(Vec3(0.0, 0.0, 0.0), rng.range(0.2, 0.4))
+            _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+        elif k % 3 == 1:
+            hx = rng.range(0.15, 0.35)
+            hy = rng.range(0.15, 0.35)
+            hz = rng.range(0.15, 0.35)
+            _check(api.m3CreateBoxShape(body, C.byref(sd), Vec3(hx, hy, hz)), "box")
+        else:
+            cap = Capsule(Vec3(-0.2, 0.0, 0.0), Vec3(0.2, 0.0, 0.0), rng.range(0.12, 0.2))
+            _check(api.m3CreateCapsuleShape(body, C.byref(sd), C.byref(cap)), "capsule")
+    return world
+
+
+def box_pyramid(api: Api, rows: int = 20) -> WorldId:
+    """Classic pyramid: ``rows`` rows of unit cubes on a static plane.
+
+    Row r holds rows-r boxes of half-extent 0.5 (density 1000, friction 0.6,
+    restitution 0) centred at x = i - (rows-1-r)/2, y = 0.5 + r, z = 0.
+    20 rows = 210 dynamic boxes.
+    """
+    n_boxes = rows * (rows + 1) // 2
+    cap = max(256, n_boxes + 8)
+    world = _world(api, cap, cap)
+    _ground(api, world)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    sd.restitution = 0.0
+    for r in range(rows):
+        count = rows - r
+        for i in range(count):
+            bd.position = Pos3(float(i) - 0.5 * float(rows - 1 - r), 0.5 + float(r), 0.0)
+            body = api.m3CreateBody(world, C.byref(bd))
+            _check(api.m3CreateBoxShape(body, C.byref(sd), Vec3(0.5, 0.5, 0.5)), "box")
+    return world
+
+
+def rain(api: Api, count: int = 10000, seed: int = 0x5EED10C, extent: float = 50.0) -> WorldId:
+    """Mixed rain: ``count`` dynamic bodies dropped onto a plane.
+
+    Body k is a box when k % 5 < 2 (half-extents U[0.25,1] each), a sphere when
+    k % 5 is 2 or 3 (radius U[0.25,0.75]) and a capsule when k % 5 == 4
+    (radius U[0.2,0.5], half-height U[0.25,1] along local x).  Positions are
+    uniform in [-extent,extent] x [2,52] x [-extent,extent], rotations uniform,
+    velocities zero, density 1000, friction 0.6.  Draw order per body:
+    position x, y, z, rotation, then the shape's sizes.
+    """
+    cap = count + 64
+    world = _world(api, cap, cap)
+    _ground(api, world)
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    rng = SplitMix64(seed)
+    for k in range(count):
+        x = rng.range(-extent, extent)
+        y = rng.range(2.0, 52.0)
+        z = rng.range(-extent, extent)
+        qx, qy, qz, qw = rng.unit_quat()
+        bd.position = Pos3(x, y, z)
+        bd.rotation = Quat(qx, qy, qz, qw)
+        body = api.m3CreateBody(world, C.byref(bd))
+        kind = k % 5
+        if kind < 2:
+            hx = rng.range(0.25, 1.0)
+            hy = rng.range(0.25, 1.0)
+            hz = rng.range(0.25, 1.0)
+            _check(api.m3CreateBoxShape(body, C.byref(sd), Vec3(hx, hy, hz)), "box")
+        elif kind < 4:
+            ball = Sphere(Vec3(0.0, 0.0, 0.0), rng.range(0.25, 0.75))
+            _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+        else:
+            r = rng.range(0.2, 0.5)
+            hh = rng.range(0.25, 1.0)
+            cap_ = Capsule(Vec3(-hh, 0.0, 0.0), Vec3(hh, 0.0, 0.0), r)
+            _check(api.m3CreateCapsuleShape(body, C.byref(sd), C.byref(cap_)), "capsule")
+    return world
+
+
+def toss(api: Api, seed: int, boxes: int = 32) -> WorldId:
+    """One differentiable-toss world: ``boxes`` boxes launched onto a plane.
+
+    Half-extents U[0.2,0.6], mass U[0.5,5] kg (density = m / (8 hx hy hz)),
+    linear velocity U[-5,5]^3, angular velocity U[-3,3]^3, friction U[0.3,0.9],
+    positions on a 4-wide grid 1.5 m apart at heights U[1,3].
+    """
+    world = _world(api, boxes + 8, boxes + 8)
+    _ground(api, world)
+    rng = SplitMix64(seed)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    for k in range(boxes):
+        hx = rng.range(0.2, 0.6)
+        hy = rng.range(0.2, 0.6)
+        hz = rng.range(0.2, 0.6)
+        mass = rng.range(0.5, 5.0)
+        fx = C.c_float(hx).value
+        fy = C.c_float(hy).value
+        fz = C.c_float(hz).value
+        sd = api.m3DefaultShapeDef()
+        sd.density = mass / (8.0 * fx * fy * fz)
+        sd.friction = rng.range(0.3, 0.9)
+        bd.position = Pos3(1.5 * float(k % 4) - 2.25, rng.range(1.0, 3.0), 1.5 * float(k // 4) - 5.25)
+        bd.linearVelocity = Vec3(rng.range(-5, 5), rng.range(-5, 5), rng.range(-5, 5))
+        bd.angularVelocity = Vec3(rng.range(-3, 3), rng.range(-3, 3), rng.range(-3, 3))
+        body = api.m3CreateBody(world, C.byref(bd))
+        _check(api.m3CreateBoxShape(body, C.byref(sd), Vec3(hx, hy, hz)), "box")
+    return world
+
+
+SCENES = {
+    "bench_pyramid": bench_pyramid,
+    "smoke10k": smoke10k,
+    "box_pyramid": box_pyramid,
+    "rain": rain,
+}
