@@ -1,0 +1,173 @@
+// m3gpu: the thin C ABI between the C17 host world and the gfx950 kernels.
+//
+// The host world (maul3d_amd/csrc/world.c) owns creation, ids, hashing and
+// snapshots; this layer owns the device mirror of the world state and runs
+// the whole step on the GPU: broadphase pair list, narrowphase manifolds and
+// warm-start carry, island wake, contact prepare, graph coloring, the
+// substep loop (integrate / warm start / solve / integrate positions /
+// relax), restitution, impulse store, continuous collision against planes
+// and the island sleep pass.  It replaces the body of m3StepInternal
+// (src/solver.c:3662-4320) behind m3World_Step (src/solver.c:4322).
+//
+// Every entry point takes plain pointers and sizes.  Return codes: 0 = ok,
+// M3GPU_STALL = the step refused for capacity (the world did not advance),
+// negative = a HIP error (the message is available from m3gpu_last_error).
+#ifndef M3GPU_H
+#define M3GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M3GPU_API __attribute__((visibility("default"))) extern
+
+#define M3GPU_STALL 1
+
+// Field groups for upload/download.
+#define M3GPU_FIELDS_STATIC  0x1u // mass, shapes, hulls, filters, materials
+#define M3GPU_FIELDS_DYNAMIC 0x2u // transforms, velocities, awake, sleep timers, forces
+#define M3GPU_FIELDS_PAIRS   0x4u // pair keys, manifolds, frozen keys, leaf boxes
+#define M3GPU_FIELDS_ALL     0x7u
+
+typedef struct m3gpuWorld m3gpuWorld;
+
+typedef struct m3gpuCaps
+{
+    int32_t bodyCapacity;
+    int32_t shapeCapacity;
+    int32_t pairCapacity;
+    int32_t hullCapacity;
+} m3gpuCaps;
+
+// Host arrays of one world.  Element types are the records of m3_core.h
+// (sizes noted); counts are the id-pool high-water marks.
+typedef struct m3gpuHostState
+{
+    int32_t maxBody;
+    int32_t maxShape;
+    int32_t maxHull;
+    // bodies (maxBody each)
+    void* transforms;        // m3Transform, 40 B
+    void* linearVelocities;  // m3Vec3, 12 B
+    void* angularVelocities; // m3Vec3
+    float* invMass;
+    void* invInertiaLocal; // m3Mat3, 36 B
+    void* inertiaLocal;    // m3Mat3
+    void* localCenters;    // m3Vec3
+    float* gravityScales;
+    float* linearDamping;
+    float* angularDamping;
+    uint8_t* bodyAlive;
+    uint8_t* types;
+    uint8_t* awake;
+    float* sleepTimes;
+    uint8_t* bulletFlags;
+    float* minExtents;
+    float* maxExtents;
+    void* bodyForce;  // m3Vec3
+    void* bodyTorque; // m3Vec3
+    uint8_t* bodyEnabled;
+    uint8_t* bodyLocks;
+    float* bodySleepThreshold;
+    uint8_t* bodyCanSleep;
+    uint8_t* bodyHasTarget;
+    void* bodyTarget; // m3Transform
+    int32_t* bodyShapeHead;
+    // shapes (maxShape each)
+    uint8_t* shapeAlive;
+    int32_t* shapeBody;
+    uint8_t* shapeType;
+    void* shapeGeom; // m3ShapeGeom, 32 B
+    float* shapeFriction;
+    float* shapeRestitution;
+    float* shapeRollingResistance;
+    uint8_t* shapeSensor;
+    uint8_t* shapeHasOffset;
+    void* shapeLocalPos;   // m3Vec3
+    void* shapeLocalRot;   // m3Quat
+    void* shapeSurfaceVel; // m3Vec3
+    uint64_t* shapeCategory;
+    uint64_t* shapeMask;
+    int32_t* shapeGroup;
+    int32_t* shapeNext;
+    int32_t* shapeHullIndex;
+    double* leafBoxes; // 6 doubles per shape: lo xyz, hi xyz (the proxy's stored box)
+    // hulls (maxHull)
+    void* hullData; // m3HullData, 5808 B
+    // pairs
+    int32_t* pairCount;
+    uint64_t* pairKeys;  // pairCapacity
+    void* manifolds;     // m3Manifold, 188 B, pairCapacity
+    int32_t* sleepingPairCount;
+    uint64_t* sleepingPairKeys; // pairCapacity
+    uint8_t* frozenDirty;
+    uint8_t* pairsFullQuery;
+    uint64_t* stepCount; // advanced on the device by every step that completes
+} m3gpuHostState;
+
+typedef struct m3gpuStepParams
+{
+    float dt;
+    int32_t substeps;
+    float gravity[3];
+    float contactHertz;
+    float contactDampingRatio;
+    float contactPushMaxSpeed;
+    float restitutionThreshold;
+    float maximumLinearSpeed;
+    float maximumAngularSpeed;
+    int32_t sleepEnabled;
+    int32_t continuousEnabled;
+} m3gpuStepParams;
+
+typedef struct m3gpuStepStats
+{
+    int32_t pairCount;
+    int32_t constraintCount;
+    int32_t colorCount;
+    int32_t islandCount;
+    int32_t awakeCount;
+    float msBroadphase;
+    float msNarrowphase;
+    float msPrepare;
+    float msSolve;
+    float msContinuous;
+    float msSleep;
+    float msStep;
+} m3gpuStepStats;
+
+M3GPU_API int m3gpu_device_count(void);
+M3GPU_API const char* m3gpu_last_error(void);
+
+M3GPU_API m3gpuWorld* m3gpu_world_create(const m3gpuCaps* caps);
+M3GPU_API void m3gpu_world_destroy(m3gpuWorld* world);
+
+// Copies the selected field groups host -> device (upload) or device -> host
+// (download).  Both block until the copy is complete.
+M3GPU_API int m3gpu_upload(m3gpuWorld* world, const m3gpuHostState* host, uint32_t fields);
+M3GPU_API int m3gpu_download(m3gpuWorld* world, m3gpuHostState* host, uint32_t fields);
+
+// One full step on the device.  When stats is non-null the call waits for
+// the step and fills it (counts, per-stage device milliseconds); with null
+// stats the step is only enqueued and the caller may return at once.
+M3GPU_API int m3gpu_step(m3gpuWorld* world, const m3gpuStepParams* params,
+                         m3gpuStepStats* stats);
+
+// Steps several worlds back to back on the device, each with its own
+// params; returns the number of worlds that stalled.
+M3GPU_API int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,
+                              int32_t count);
+
+// Waits for all queued device work of this world.
+M3GPU_API int m3gpu_sync(m3gpuWorld* world);
+
+// Waits for the world's last step and reports its counts and stage times.
+M3GPU_API int m3gpu_last_stats(m3gpuWorld* world, m3gpuStepStats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif // M3GPU_H

@@ -1,0 +1,222 @@
+// maul3d_amd: the Maul3D public C surface for the MI355X build.
+//
+// This header declares the subset of the reference API that the soft-step
+// contact-solver path needs, with the same names, layouts and error
+// behaviour, so a host that links the reference can link
+// libmaul3d_amd.so instead.  Each declaration names the reference
+// declaration it replaces.  The step runs on one MI355X through the
+// m3gpu_* layer (include/m3gpu.h); everything here is plain C.
+#ifndef MAUL3D_AMD_H
+#define MAUL3D_AMD_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M3_API __attribute__((visibility("default"))) extern
+
+// ---- math types (include/maul3d/math.h:24-52) -----------------------------
+#ifndef M3_CORE_H
+typedef float m3real;
+typedef struct m3Vec3 { m3real x, y, z; } m3Vec3;
+typedef struct m3Pos3 { double x, y, z; } m3Pos3;
+typedef struct m3Quat { m3real x, y, z, w; } m3Quat;
+typedef struct m3Transform { m3Pos3 p; m3Quat q; } m3Transform;
+typedef struct m3Mat3 { m3Vec3 cx, cy, cz; } m3Mat3;
+#endif
+
+// ---- ids (include/maul3d/base.h:64-104) ------------------------------------
+typedef struct m3WorldId { int32_t index1; uint16_t generation; } m3WorldId;
+typedef struct m3BodyId { int32_t index1; uint16_t world0; uint16_t generation; } m3BodyId;
+typedef struct m3ShapeId { int32_t index1; uint16_t world0; uint16_t generation; } m3ShapeId;
+typedef struct m3JointId { int32_t index1; uint16_t world0; uint16_t generation; } m3JointId;
+
+typedef enum m3Result
+{
+    m3_success = 0,
+    m3_errorInvalid = 1,
+    m3_errorCapacity = 2,
+    m3_errorConfig = 3,
+} m3Result;
+
+typedef enum m3BodyType
+{
+    m3_staticBody = 0,
+    m3_kinematicBody = 1,
+    m3_dynamicBody = 2,
+} m3BodyType;
+
+typedef enum m3ShapeType
+{
+    m3_sphereShape = 0,
+    m3_planeShape = 1,
+    m3_hullShape = 2,
+    m3_capsuleShape = 3,
+} m3ShapeType;
+
+// ---- defs (include/maul3d/world.h:48-107, body.h:18-40, shape.h:46-80) ----
+typedef void m3TaskFn(int32_t startIndex, int32_t endIndex, void* taskContext);
+typedef void* m3EnqueueTaskFn(m3TaskFn* task, int32_t itemCount, int32_t minRange,
+                              void* taskContext, void* userContext);
+typedef void m3FinishTaskFn(void* userTask, void* userContext);
+
+typedef struct m3WorldDef
+{
+    m3Vec3 gravity;
+    int32_t bodyCapacity;
+    int32_t shapeCapacity;
+    int32_t meshCapacity;
+    int32_t jointCapacity;
+    int32_t voxelCapacity;
+    int32_t characterCapacity;
+    int32_t vehicleCapacity;
+    int32_t softBodyCapacity;
+    int32_t workerCount;
+    m3EnqueueTaskFn* enqueueTask; // accepted for source compatibility; the
+    m3FinishTaskFn* finishTask;   // device step never calls host task hooks
+    void* userTaskContext;
+    float contactHertz;
+    float contactDampingRatio;
+    float contactPushMaxSpeed;
+    float restitutionThreshold;
+    float maximumLinearSpeed;
+    int32_t enableSleeping;
+    int32_t enableContinuous;
+    float hitEventThreshold;
+    int32_t internalValue;
+} m3WorldDef;
+
+typedef struct m3BodyDef
+{
+    int32_t type;
+    m3Pos3 position;
+    m3Quat rotation;
+    m3Vec3 linearVelocity;
+    m3Vec3 angularVelocity;
+    float gravityScale;
+    float linearDamping;
+    float angularDamping;
+    uint64_t userData;
+    bool isBullet;
+    int32_t internalValue;
+} m3BodyDef;
+
+typedef struct m3ShapeDef
+{
+    float density;
+    float friction;
+    float restitution;
+    uint64_t categoryBits;
+    uint64_t maskBits;
+    int32_t groupIndex;
+    float rollingResistance;
+    uint64_t userData;
+    bool isSensor;
+    bool enableHitEvents;
+    bool enablePreSolveEvents;
+    m3Vec3 localPosition;
+    m3Quat localRotation;
+    int32_t internalValue;
+} m3ShapeDef;
+
+typedef struct m3Sphere { m3Vec3 center; m3real radius; } m3Sphere;
+typedef struct m3Plane { m3Vec3 normal; m3real offset; } m3Plane;
+typedef struct m3Capsule { m3Vec3 point1; m3Vec3 point2; m3real radius; } m3Capsule;
+
+// include/maul3d/world.h:531-554
+typedef struct m3Counters
+{
+    int32_t bodyCount;
+    int32_t shapeCount;
+    int32_t jointCount;
+    int32_t contactCount;
+    int32_t awakeCount;
+    int32_t islandCount;
+    int32_t colorCount;
+    int32_t characterCount;
+    int32_t vehicleCount;
+    int32_t softBodyCount;
+    int32_t voxelChunkCount;
+    int32_t hullCount;
+    int32_t meshCount;
+    int32_t treeHeight;
+    int32_t scratchPeak;
+    int32_t scratchCapacity;
+    int32_t snapshotBytes;
+    int64_t allocCalls;
+    int64_t freeCalls;
+} m3Counters;
+
+// include/maul3d/world.h:578-593 (device-timed per stage)
+typedef struct m3Profile
+{
+    float step;
+    float vehicles;
+    float broadphase;
+    float narrowphase;
+    float events;
+    float prepare;
+    float solve;
+    float continuous;
+    float sleep;
+    float characters;
+    float softBodies;
+} m3Profile;
+
+// ---- API -------------------------------------------------------------------
+M3_API int m3GetVersion(void);                                   // base.h:34
+M3_API uint64_t m3Hash64(uint64_t h, const void* bytes, int32_t count); // base.h:110
+
+M3_API m3WorldDef m3DefaultWorldDef(void);                       // world.h:109
+M3_API m3WorldId m3CreateWorld(const m3WorldDef* def);           // world.h:117
+M3_API void m3DestroyWorld(m3WorldId worldId);                   // world.h:118
+M3_API bool m3World_IsValid(m3WorldId worldId);                  // world.h:119
+M3_API void m3World_SetGravity(m3WorldId worldId, m3Vec3 gravity);
+M3_API m3Vec3 m3World_GetGravity(m3WorldId worldId);
+M3_API void m3World_EnableSleeping(m3WorldId worldId, bool flag);
+M3_API void m3World_EnableContinuous(m3WorldId worldId, bool flag);
+
+// The drop-in boundary: world.h:195 (Step), :203-205 (Snapshot/Restore),
+// :510 (Hash).  Step is void and silently refuses a bad id, dt <= 0 or
+// substeps outside [1, 256]; a capacity failure stalls the world without
+// advancing stepCount.  Snapshot returns the size written or -1; Restore is
+// atomic and returns false without touching the world on bad bytes.
+M3_API void m3World_Step(m3WorldId worldId, float dt, int32_t substeps);
+M3_API int32_t m3World_SnapshotSize(m3WorldId worldId);
+M3_API int32_t m3World_Snapshot(m3WorldId worldId, void* out, int32_t capacity);
+M3_API bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size);
+M3_API uint64_t m3World_Hash(m3WorldId worldId);
+M3_API m3Counters m3World_GetCounters(m3WorldId worldId);       // world.h:556
+M3_API m3Profile m3World_GetProfile(m3WorldId worldId);         // world.h:595
+
+M3_API m3BodyDef m3DefaultBodyDef(void);                        // body.h:42
+M3_API m3BodyId m3CreateBody(m3WorldId worldId, const m3BodyDef* def); // body.h:43
+M3_API bool m3Body_IsValid(m3BodyId bodyId);
+M3_API m3Pos3 m3Body_GetPosition(m3BodyId bodyId);
+M3_API m3Quat m3Body_GetRotation(m3BodyId bodyId);
+M3_API m3Vec3 m3Body_GetLinearVelocity(m3BodyId bodyId);
+M3_API m3Vec3 m3Body_GetAngularVelocity(m3BodyId bodyId);
+M3_API void m3Body_SetLinearVelocity(m3BodyId bodyId, m3Vec3 velocity);
+M3_API void m3Body_SetAngularVelocity(m3BodyId bodyId, m3Vec3 velocity);
+M3_API bool m3Body_IsAwake(m3BodyId bodyId);
+M3_API void m3Body_ApplyForce(m3BodyId bodyId, m3Vec3 force);
+M3_API void m3Body_ApplyTorque(m3BodyId bodyId, m3Vec3 torque);
+
+M3_API m3ShapeDef m3DefaultShapeDef(void);                      // shape.h:82
+M3_API m3ShapeId m3CreateSphereShape(m3BodyId bodyId, const m3ShapeDef* def,
+                                     const m3Sphere* sphere);   // shape.h:86
+M3_API m3ShapeId m3CreatePlaneShape(m3BodyId bodyId, const m3ShapeDef* def,
+                                    const m3Plane* plane);      // shape.h:90
+M3_API m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def,
+                                  m3Vec3 halfExtents);          // shape.h:93
+M3_API m3ShapeId m3CreateCapsuleShape(m3BodyId bodyId, const m3ShapeDef* def,
+                                      const m3Capsule* capsule); // shape.h:104
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif // MAUL3D_AMD_H

@@ -219,7 +219,16 @@ _SIGNATURES = {
 class Api:
     """Typed view of one library's ``m3*`` exports."""
 
-    def __init__(self, lib: C.CDLL, optional: tuple[str, ...] = ()):
+    def __init__(
+        self,
+        lib: C.CDLL,
+        optional: tuple[str, ...] = (
+            "m3DestroyBody",
+            "m3Body_SetAwake",
+            "m3CreateHullShape",
+            "m3World_GetProfile",
+        ),
+    ):
         self.lib = lib
         for name, (res, args) in _SIGNATURES.items():
             try:

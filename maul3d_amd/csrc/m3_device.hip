@@ -1,0 +1,1487 @@
+// gfx950 step for the soft-step contact solver: the m3gpu_* C ABI
+// (include/m3gpu.h) over a device-resident, structure-of-arrays world.
+//
+// One step is a fixed sequence of kernels on the world's stream with no
+// host round trip: every count the later stages need (pairs, constraints,
+// movers, colors) stays in device memory (m3Scalars) and kernels read it.
+// Stage order and semantics follow m3StepInternal (src/solver.c:3662-4320).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "m3gpu.h"
+#include "m3_step.h"
+
+#define M3_SENTINEL32 0xFFFFFFFFu
+#define M3_SENTINEL64 0xFFFFFFFFFFFFFFFFull
+
+static char s_lastError[512];
+
+static int Fail(const char* what, hipError_t e)
+{
+    snprintf(s_lastError, sizeof(s_lastError), "%s: %s", what, hipGetErrorString(e));
+    return -1;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do                                                                                             \
+    {                                                                                              \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+        {                                                                                          \
+            return Fail(#expr, e_);                                                                \
+        }                                                                                          \
+    } while (0)
+
+struct m3gpuWorld
+{
+    m3gpuCaps caps;
+    hipStream_t stream;
+    m3View view;          // device pointers, counts filled at upload
+    uint64_t* keysBuf[2]; // current pair list and the previous step's
+    m3Manifold* mansBuf[2];
+    int cur;
+    int fullQuery; // rebuild the frozen-pair buffer before the next step
+    int ccdUnsupported;
+    // scratch for the broadphase sorts and compactions
+    uint64_t* sapKeysIn;
+    uint64_t* sapKeysOut;
+    int32_t* sapValsIn;
+    int32_t* sapValsOut;
+    uint32_t* emitSorted;
+    uint32_t* emitUnique;
+    uint8_t* pairFlags;
+    uint8_t* bodyFlags;
+    int32_t* root;
+    uint8_t* frozenNow;
+    uint32_t* bodyMasks;
+    void* cubTemp;
+    size_t cubBytes;
+    m3Scalars* hostScalars; // pinned mirror for stats
+};
+
+// ---------------------------------------------------------------------------
+// Small helpers
+// ---------------------------------------------------------------------------
+
+static inline unsigned Blocks(int64_t n, int threads)
+{
+    int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b > 0 ? b : 1);
+}
+
+__device__ static inline uint64_t SortableDouble(double d)
+{
+    uint64_t u = (uint64_t)__double_as_longlong(d);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+__device__ static inline int32_t LoadRelaxed(const int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ static inline void Emit(const m3View& v, uint32_t key)
+{
+    int32_t slot = atomicAdd(&v.sc->emitCount, 1);
+    if (slot < v.emitCapacity)
+    {
+        v.emit[slot] = key;
+    }
+    else
+    {
+        v.sc->stall = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Broadphase (src/broad_phase.c:391-637)
+// ---------------------------------------------------------------------------
+
+__global__ void k_begin(m3View v)
+{
+    m3Scalars* sc = v.sc;
+    sc->oldPairCount = sc->pairCount;
+    sc->emitCount = 0;
+    sc->stall = 0;
+    sc->anyRestitution = 0;
+    sc->islandCount = 0;
+    sc->awakeCount = 0;
+    sc->discoverCount = 0;
+}
+
+__global__ void k_fill32(uint32_t* p, int32_t n, uint32_t value)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+    {
+        p[i] = value;
+    }
+}
+
+// Fresh bounds of every alive non-plane shape and its sweep key.
+__global__ void k_fresh(m3View v, uint64_t* sapKeys, int32_t* sapVals)
+{
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.maxShape)
+    {
+        return;
+    }
+    uint64_t key = M3_SENTINEL64;
+    if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE)
+    {
+        double box[6];
+        m3FreshBox(&v, s, box);
+        for (int k = 0; k < 6; ++k)
+        {
+            v.fresh[6 * s + k] = box[k];
+        }
+        key = SortableDouble(box[0]);
+        if (key == M3_SENTINEL64)
+        {
+            key -= 1;
+        }
+    }
+    if (sapKeys != nullptr)
+    {
+        sapKeys[s] = key;
+        sapVals[s] = s;
+    }
+}
+
+// Frozen-pair discovery for one shape: every tree shape whose stored box
+// and fresh box both overlap this shape's fresh box, plus every allowed
+// plane (src/broad_phase.c:351-428).
+__device__ static void DiscoverShape(const m3View& v, int32_t s)
+{
+    if (v.shapeAlive[s] == 0)
+    {
+        return;
+    }
+    if (v.shapeType[s] != M3_PLANE)
+    {
+        const double* self = &v.fresh[6 * s];
+        for (int32_t o = 0; o < v.maxShape; ++o)
+        {
+            if (o == s || v.shapeAlive[o] == 0 || v.shapeType[o] == M3_PLANE)
+            {
+                continue;
+            }
+            if (!m3BoxOverlap(&v.leaf[6 * o], self) || !m3PairAllowed(&v, s, o) ||
+                !m3BoxOverlap(self, &v.fresh[6 * o]))
+            {
+                continue;
+            }
+            int32_t slot = atomicAdd(&v.sc->sleepCount, 1);
+            if (slot < v.pairCapacity)
+            {
+                v.sleepKeys[slot] = m3ExpandKey(m3CompactKey(s, o));
+            }
+        }
+    }
+    for (int32_t pi = 0; pi < v.planeCount; ++pi)
+    {
+        int32_t p = v.planes[pi];
+        if (!m3PairAllowed(&v, p, s))
+        {
+            continue;
+        }
+        int32_t slot = atomicAdd(&v.sc->sleepCount, 1);
+        if (slot < v.pairCapacity)
+        {
+            v.sleepKeys[slot] = m3ExpandKey(m3CompactKey(p, s));
+        }
+    }
+}
+
+__global__ void k_reset_sleep(m3View v) { v.sc->sleepCount = 0; }
+
+// After a restore: rebuild the frozen buffer from every sleeping body.
+__global__ void k_discover_all(m3View v)
+{
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.maxShape || v.shapeAlive[s] == 0)
+    {
+        return;
+    }
+    int32_t b = v.shapeBody[s];
+    if (v.bodyAlive[b] != 0 && v.type[b] != M3_STATIC && v.awake[b] == 0)
+    {
+        DiscoverShape(v, s);
+    }
+}
+
+// A proxy moves only when its fresh box escapes the stored box; it is then
+// re-stored inflated by the margin (src/broad_phase.c:505-539).
+__global__ void k_refresh(m3View v)
+{
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.maxShape || v.shapeAlive[s] == 0 || v.shapeType[s] == M3_PLANE ||
+        v.awake[v.shapeBody[s]] == 0)
+    {
+        return;
+    }
+    const double* f = &v.fresh[6 * s];
+    double* leaf = &v.leaf[6 * s];
+    if (leaf[0] <= f[0] && leaf[1] <= f[1] && leaf[2] <= f[2] && f[3] <= leaf[3] &&
+        f[4] <= leaf[4] && f[5] <= leaf[5])
+    {
+        return;
+    }
+    for (int k = 0; k < 3; ++k)
+    {
+        leaf[k] = f[k] - (double)M3_AABB_MARGIN;
+        leaf[3 + k] = f[3 + k] + (double)M3_AABB_MARGIN;
+    }
+}
+
+// Sweep along x over the sorted fresh boxes: every overlapping pair is
+// tested once and emitted when the tree pass would emit it.
+__global__ void k_sap(m3View v, const uint64_t* keys, const int32_t* vals)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxShape || keys[i] == M3_SENTINEL64)
+    {
+        return;
+    }
+    int32_t s = vals[i];
+    const double* a = &v.fresh[6 * s];
+    double hx = a[3];
+    for (int32_t j = i + 1; j < v.maxShape; ++j)
+    {
+        if (keys[j] == M3_SENTINEL64)
+        {
+            break;
+        }
+        int32_t o = vals[j];
+        const double* b = &v.fresh[6 * o];
+        if (b[0] > hx)
+        {
+            break;
+        }
+        if (m3BoxOverlap(a, b) && m3TreePairEmits(&v, s, o))
+        {
+            Emit(v, m3CompactKey(s, o));
+        }
+    }
+}
+
+// Planes pair with every allowed hot shape (src/broad_phase.c:541-564).
+__global__ void k_planes(m3View v)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)v.maxShape * v.planeCount)
+    {
+        return;
+    }
+    int32_t s = (int32_t)(idx % v.maxShape);
+    int32_t p = v.planes[idx / v.maxShape];
+    if (v.shapeAlive[s] == 0 || v.shapeType[s] == M3_PLANE || !m3PairAllowed(&v, p, s) ||
+        !m3PairHot(&v, p, s))
+    {
+        return;
+    }
+    Emit(v, m3CompactKey(p, s));
+}
+
+// Still-cold frozen pairs ride along (src/broad_phase.c:607-632).
+__global__ void k_frozen(m3View v)
+{
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t n = v.sc->sleepCount < v.pairCapacity ? v.sc->sleepCount : v.pairCapacity;
+    if (k >= n)
+    {
+        return;
+    }
+    uint64_t key = v.sleepKeys[k];
+    int32_t i = (int32_t)(key >> 32);
+    int32_t j = (int32_t)(key & 0xFFFFFFFFu);
+    if (v.shapeAlive[i] == 0 || v.shapeAlive[j] == 0 || m3PairHot(&v, i, j))
+    {
+        return;
+    }
+    Emit(v, m3CompactKey(i, j));
+}
+
+// The unique sorted keys become this step's pair list; the cold ones are
+// the frozen buffer carried to the next step.
+__global__ void k_pair_count(m3View v, const uint32_t* unique)
+{
+    m3Scalars* sc = v.sc;
+    int32_t n = sc->uniqueCount;
+    if (n > 0 && unique[n - 1] == M3_SENTINEL32)
+    {
+        n -= 1;
+    }
+    if (sc->emitCount > v.emitCapacity || n > v.pairCapacity)
+    {
+        sc->stall = 1;
+    }
+    if (sc->stall)
+    {
+        sc->pairCount = sc->oldPairCount;
+        return;
+    }
+    sc->pairCount = n;
+}
+
+__global__ void k_expand(m3View v, const uint32_t* unique, uint8_t* coldFlags)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.pairCapacity)
+    {
+        return;
+    }
+    uint8_t cold = 0;
+    if (v.sc->stall == 0 && i < v.sc->pairCount)
+    {
+        uint64_t key = m3ExpandKey(unique[i]);
+        v.keys[i] = key;
+        cold = m3PairHot(&v, (int32_t)(key >> 32), (int32_t)(key & 0xFFFFFFFFu)) ? 0 : 1;
+    }
+    coldFlags[i] = cold;
+}
+
+// On a capacity stall the previous pair list stays the world's list.
+__global__ void k_stall_restore(m3View v)
+{
+    if (v.sc->stall == 0)
+    {
+        return;
+    }
+    int32_t n = v.sc->oldPairCount;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        v.keys[i] = v.oldKeys[i];
+        v.manifolds[i] = v.oldManifolds[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Narrowphase (src/manifold.c:2357-2944)
+// ---------------------------------------------------------------------------
+
+__global__ void k_narrow(m3View v)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || i >= v.sc->pairCount)
+    {
+        return;
+    }
+    m3NarrowPair(&v, i, v.sc->oldPairCount);
+}
+
+// ---------------------------------------------------------------------------
+// Sweep capture and islands (src/solver.c:3437-3580, 3851-3881)
+// ---------------------------------------------------------------------------
+
+__global__ void k_capture(m3View v)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0)
+    {
+        return;
+    }
+    if (v.bodyAlive[i] != 0)
+    {
+        m3Transform xf = v.xf[i];
+        m3Vec3 rlc = m3RotateVec3(xf.q, v.lc[i]);
+        m3Pos3 c;
+        c.x = xf.p.x + (double)rlc.x;
+        c.y = xf.p.y + (double)rlc.y;
+        c.z = xf.p.z + (double)rlc.z;
+        v.com0[i] = c;
+        v.rot0[i] = xf.q;
+    }
+    v.parent[i] = i;
+    v.forced[i] = 0;
+    v.flag[i] = 0;
+    v.deltaPos[i] = m3V3(0.0f, 0.0f, 0.0f);
+    v.deltaRot[i] = m3IdentityQuat();
+}
+
+__device__ static int32_t Find(const int32_t* parent, int32_t i)
+{
+    while (true)
+    {
+        int32_t p = LoadRelaxed(&parent[i]);
+        if (p == i)
+        {
+            return i;
+        }
+        i = p;
+    }
+}
+
+// Lock-free union that always hooks the larger root under the smaller, so
+// each component's root is its lowest body index: the reference's
+// "lower root wins" labels.
+__device__ static void Unite(int32_t* parent, int32_t a, int32_t b)
+{
+    while (true)
+    {
+        a = Find(parent, a);
+        b = Find(parent, b);
+        if (a == b)
+        {
+            return;
+        }
+        int32_t lo = a < b ? a : b;
+        int32_t hi = a < b ? b : a;
+        if (atomicCAS(&parent[hi], hi, lo) == hi)
+        {
+            return;
+        }
+    }
+}
+
+__global__ void k_union(m3View v)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || i >= v.sc->pairCount || v.manifolds[i].pointCount == 0)
+    {
+        return;
+    }
+    uint64_t key = v.keys[i];
+    int32_t sA = (int32_t)(key >> 32);
+    int32_t sB = (int32_t)(key & 0xFFFFFFFFu);
+    if (v.sensor[sA] != 0 || v.sensor[sB] != 0)
+    {
+        return;
+    }
+    int32_t bA = v.shapeBody[sA];
+    int32_t bB = v.shapeBody[sB];
+    int dynA = v.type[bA] == M3_DYNAMIC;
+    int dynB = v.type[bB] == M3_DYNAMIC;
+    if (dynA && dynB)
+    {
+        Unite(v.parent, bA, bB);
+    }
+    else if (dynA || dynB)
+    {
+        int32_t kin = dynA ? bB : bA;
+        int32_t dyn = dynA ? bA : bB;
+        if (v.type[kin] == M3_KINEMATIC)
+        {
+            m3Vec3 lin = v.lv[kin];
+            m3Vec3 ang = v.av[kin];
+            if (m3Dot3(lin, lin) > 0.0f || m3Dot3(ang, ang) > 0.0f)
+            {
+                v.forced[dyn] = 1;
+            }
+        }
+    }
+}
+
+__global__ void k_root(m3View v, int32_t* root)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0)
+    {
+        return;
+    }
+    root[i] = Find(v.parent, i);
+}
+
+__global__ void k_wake_mark(m3View v, const int32_t* root)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC)
+    {
+        return;
+    }
+    if (v.awake[i] != 0 || v.forced[i] != 0)
+    {
+        v.flag[root[i]] = 1;
+    }
+}
+
+__global__ void k_wake_apply(m3View v, const int32_t* root)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC)
+    {
+        return;
+    }
+    if (v.awake[i] == 0 && v.flag[root[i]] != 0)
+    {
+        v.awake[i] = 1;
+        v.sleepTime[i] = 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Prepare and coloring (src/solver.c:156-404, 2741-2822)
+// ---------------------------------------------------------------------------
+
+__global__ void k_constraint_flags(m3View v, uint8_t* flags)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.pairCapacity)
+    {
+        return;
+    }
+    flags[i] = (v.sc->stall == 0 && i < v.sc->pairCount && m3PairIsConstraint(&v, i)) ? 1 : 0;
+}
+
+__global__ void k_prepare(m3View v, float hertz, float zeta, float h)
+{
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || c >= v.sc->constraintCount)
+    {
+        return;
+    }
+    m3Softness soft = m3MakeSoft(hertz, zeta, h);
+    m3Softness staticSoft = m3MakeSoft(2.0f * hertz, zeta, h);
+    m3Constraint* con = &v.cons[c];
+    m3PrepareContact(&v, v.consPair[c], con, soft, staticSoft);
+    if (con->restitution > 0.0f)
+    {
+        v.sc->anyRestitution = 1;
+    }
+}
+
+// Greedy first-fit coloring in constraint order, reproduced exactly by one
+// workgroup: constraints are taken 256 at a time, and a constraint colors
+// as soon as every earlier constraint in its chunk that shares an awake
+// dynamic body has colored (those are the only ones whose masks it reads).
+// Then a stable placement lists each color in ascending constraint order.
+#define M3_COLOR_CHUNK 256
+__global__ void __launch_bounds__(M3_COLOR_CHUNK) k_color(m3View v, uint32_t* masks)
+{
+    __shared__ int32_t sA[M3_COLOR_CHUNK];
+    __shared__ int32_t sB[M3_COLOR_CHUNK];
+    __shared__ uint8_t sColor[M3_COLOR_CHUNK];
+    __shared__ uint8_t sDone[M3_COLOR_CHUNK];
+    __shared__ int32_t counts[M3_GRAPH_COLORS + 1];
+    __shared__ int32_t running[M3_GRAPH_COLORS + 1];
+    const int32_t t = threadIdx.x;
+    m3Scalars* sc = v.sc;
+    if (sc->stall != 0)
+    {
+        return;
+    }
+    const int32_t n = sc->constraintCount;
+    for (int32_t i = t; i < v.maxBody; i += M3_COLOR_CHUNK)
+    {
+        masks[i] = 0u;
+    }
+    if (t <= M3_GRAPH_COLORS)
+    {
+        counts[t] = 0;
+    }
+    __syncthreads();
+    for (int32_t base = 0; base < n; base += M3_COLOR_CHUNK)
+    {
+        int32_t c = base + t;
+        int32_t A = -1;
+        int32_t B = -1;
+        if (c < n)
+        {
+            const m3Constraint* con = &v.cons[c];
+            int32_t a = con->bodyA;
+            int32_t b = con->bodyB;
+            if (v.type[a] == M3_DYNAMIC && v.awake[a] != 0)
+            {
+                A = a;
+            }
+            if (v.type[b] == M3_DYNAMIC && v.awake[b] != 0)
+            {
+                B = b;
+            }
+        }
+        sA[t] = A;
+        sB[t] = B;
+        sDone[t] = c < n ? 0 : 1;
+        __syncthreads();
+        int32_t pa = -1;
+        int32_t pb = -1;
+        for (int32_t u = t - 1; u >= 0 && ((A >= 0 && pa < 0) || (B >= 0 && pb < 0)); --u)
+        {
+            int32_t ua = sA[u];
+            int32_t ub = sB[u];
+            if (A >= 0 && pa < 0 && (ua == A || ub == A))
+            {
+                pa = u;
+            }
+            if (B >= 0 && pb < 0 && (ua == B || ub == B))
+            {
+                pb = u;
+            }
+        }
+        while (true)
+        {
+            int ready = sDone[t] == 0 && (pa < 0 || sDone[pa] != 0) && (pb < 0 || sDone[pb] != 0);
+            // Barrier: every read of sDone above precedes any write below.
+            if (!__syncthreads_or(sDone[t] == 0))
+            {
+                break;
+            }
+            if (ready)
+            {
+                uint32_t mask = (A >= 0 ? masks[A] : 0u) | (B >= 0 ? masks[B] : 0u);
+                int32_t color = M3_GRAPH_COLORS;
+                for (int32_t bit = 0; bit < M3_GRAPH_COLORS; ++bit)
+                {
+                    if ((mask & (1u << bit)) == 0u)
+                    {
+                        color = bit;
+                        break;
+                    }
+                }
+                if (color < M3_GRAPH_COLORS)
+                {
+                    if (A >= 0)
+                    {
+                        masks[A] |= 1u << color;
+                    }
+                    if (B >= 0)
+                    {
+                        masks[B] |= 1u << color;
+                    }
+                }
+                sColor[t] = (uint8_t)color;
+                v.colors[c] = (uint8_t)color;
+                atomicAdd(&counts[color], 1);
+            }
+            __syncthreads();
+            if (ready)
+            {
+                sDone[t] = 1;
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (t == 0)
+    {
+        int32_t cursor = 0;
+        int32_t used = 0;
+        for (int32_t k = 0; k <= M3_GRAPH_COLORS; ++k)
+        {
+            sc->colorStarts[k] = cursor;
+            running[k] = cursor;
+            cursor += counts[k];
+            used += counts[k] > 0 ? 1 : 0;
+        }
+        sc->colorStarts[M3_GRAPH_COLORS + 1] = cursor;
+        sc->colorCount = used;
+    }
+    __syncthreads();
+    // Stable placement: rank inside the chunk among equal colors.
+    for (int32_t base = 0; base < n; base += M3_COLOR_CHUNK)
+    {
+        int32_t c = base + t;
+        uint8_t mine = c < n ? v.colors[c] : (uint8_t)0xFF;
+        sColor[t] = mine;
+        __syncthreads();
+        if (c < n)
+        {
+            int32_t rank = 0;
+            for (int32_t u = 0; u < t; ++u)
+            {
+                rank += sColor[u] == mine ? 1 : 0;
+            }
+            v.colorLists[running[mine] + rank] = c;
+        }
+        __syncthreads();
+        if (t <= M3_GRAPH_COLORS)
+        {
+            int32_t add = 0;
+            for (int32_t u = 0; u < M3_COLOR_CHUNK; ++u)
+            {
+                add += sColor[u] == (uint8_t)t ? 1 : 0;
+            }
+            running[t] += add;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Movers and the substep loop (src/solver.c:3938-4203)
+// ---------------------------------------------------------------------------
+
+__global__ void k_mover_flags(m3View v, uint8_t* flags, float dt)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody)
+    {
+        return;
+    }
+    uint8_t f = 0;
+    if (v.sc->stall == 0 && v.bodyAlive[i] != 0 && v.enabled[i] != 0)
+    {
+        uint8_t type = v.type[i];
+        if (type == M3_KINEMATIC || (type == M3_DYNAMIC && v.awake[i] != 0))
+        {
+            f = 1;
+        }
+        if (type == M3_KINEMATIC && v.hasTarget[i] != 0)
+        {
+            m3KinematicServo(&v, i, dt);
+        }
+    }
+    flags[i] = f;
+}
+
+__global__ void k_integrate_v(m3View v, float h, m3Vec3 gravity, float maxLin, float maxAng)
+{
+    int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || m >= v.sc->moverCount)
+    {
+        return;
+    }
+    m3IntegrateVelocity(&v, v.movers[m], h, gravity, maxLin, maxAng);
+}
+
+__global__ void k_integrate_x(m3View v, float h)
+{
+    int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || m >= v.sc->moverCount)
+    {
+        return;
+    }
+    m3IntegratePosition(&v, v.movers[m], h);
+}
+
+// One graph color of one pass: constraints in a color share no awake
+// dynamic body, so they run concurrently; mode 0 warm start, 1 solve with
+// bias, 2 relax.
+__global__ void k_color_pass(m3View v, int32_t color, int32_t mode, float invH, float pushMax)
+{
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    int32_t start = v.sc->colorStarts[color];
+    int32_t end = v.sc->colorStarts[color + 1];
+    for (int32_t k = start + blockIdx.x * blockDim.x + threadIdx.x; k < end;
+         k += gridDim.x * blockDim.x)
+    {
+        m3Constraint* c = &v.cons[v.colorLists[k]];
+        if (mode == 0)
+        {
+            m3WarmStartOne(&v, c);
+        }
+        else
+        {
+            m3SolveContact(&v, c, invH, pushMax, mode == 1);
+        }
+    }
+}
+
+// The overflow color runs serially in ascending constraint order.
+__global__ void k_overflow_pass(m3View v, int32_t mode, float invH, float pushMax)
+{
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    int32_t start = v.sc->colorStarts[M3_GRAPH_COLORS];
+    int32_t end = v.sc->colorStarts[M3_GRAPH_COLORS + 1];
+    for (int32_t k = start; k < end; ++k)
+    {
+        m3Constraint* c = &v.cons[v.colorLists[k]];
+        if (mode == 0)
+        {
+            m3WarmStartOne(&v, c);
+        }
+        else
+        {
+            m3SolveContact(&v, c, invH, pushMax, mode == 1);
+        }
+    }
+}
+
+__global__ void k_clear_forces(m3View v)
+{
+    int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || m >= v.sc->moverCount)
+    {
+        return;
+    }
+    int32_t i = v.movers[m];
+    v.force[i] = m3V3(0.0f, 0.0f, 0.0f);
+    v.torque[i] = m3V3(0.0f, 0.0f, 0.0f);
+}
+
+__global__ void k_restitution(m3View v, float threshold)
+{
+    if (v.sc->stall != 0 || v.sc->anyRestitution == 0)
+    {
+        return;
+    }
+    int32_t n = v.sc->constraintCount;
+    for (int32_t c = 0; c < n; ++c)
+    {
+        m3RestitutionOne(&v, &v.cons[c], threshold);
+    }
+}
+
+__global__ void k_store(m3View v)
+{
+    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || c >= v.sc->constraintCount)
+    {
+        return;
+    }
+    m3StoreOne(&v, &v.cons[c]);
+}
+
+// ---------------------------------------------------------------------------
+// Continuous collision and sleep (src/solver.c:3349-3430, 3582-3660)
+// ---------------------------------------------------------------------------
+
+__global__ void k_ccd(m3View v)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0)
+    {
+        return;
+    }
+    m3ContinuousBody(&v, i);
+}
+
+__global__ void k_sleep_timers(m3View v, const int32_t* root, float dt, int sleepEnabled)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0)
+    {
+        return;
+    }
+    v.flag[i] = 1; // island readiness, cleared below by any slow member
+    if (v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC || v.awake[i] == 0)
+    {
+        return;
+    }
+    if (root[i] == i)
+    {
+        atomicAdd(&v.sc->islandCount, 1);
+    }
+    if (sleepEnabled)
+    {
+        m3SleepTimer(&v, i, dt);
+    }
+}
+
+__global__ void k_sleep_ready(m3View v, const int32_t* root)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
+        v.awake[i] == 0)
+    {
+        return;
+    }
+    if (v.sleepTime[i] < 0.5f)
+    {
+        v.flag[root[i]] = 0;
+    }
+}
+
+__global__ void k_sleep_freeze(m3View v, const int32_t* root, uint8_t* frozenNow)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody)
+    {
+        return;
+    }
+    frozenNow[i] = 0;
+    if (v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC || v.awake[i] == 0)
+    {
+        return;
+    }
+    if (v.flag[root[i]] != 0)
+    {
+        v.awake[i] = 0;
+        v.lv[i] = m3V3(0.0f, 0.0f, 0.0f);
+        v.av[i] = m3V3(0.0f, 0.0f, 0.0f);
+        frozenNow[i] = 1;
+        atomicAdd(&v.sc->discoverCount, 1);
+    }
+}
+
+__global__ void k_fresh_if_frozen(m3View v)
+{
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.maxShape || v.sc->stall != 0 || v.sc->discoverCount == 0)
+    {
+        return;
+    }
+    if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE)
+    {
+        double box[6];
+        m3FreshBox(&v, s, box);
+        for (int k = 0; k < 6; ++k)
+        {
+            v.fresh[6 * s + k] = box[k];
+        }
+    }
+}
+
+__global__ void k_discover_frozen(m3View v, const uint8_t* frozenNow)
+{
+    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= v.maxShape || v.sc->stall != 0 || v.sc->discoverCount == 0 || v.shapeAlive[s] == 0)
+    {
+        return;
+    }
+    if (frozenNow[v.shapeBody[s]] != 0)
+    {
+        DiscoverShape(v, s);
+    }
+}
+
+__global__ void k_end(m3View v)
+{
+    m3Scalars* sc = v.sc;
+    if (sc->stall == 0)
+    {
+        sc->stepCount += 1;
+    }
+    if (sc->sleepCount > v.pairCapacity)
+    {
+        sc->sleepCount = v.pairCapacity;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+template <typename T>
+static int DevAlloc(T** p, int64_t count)
+{
+    int64_t bytes = count > 0 ? count * (int64_t)sizeof(T) : (int64_t)sizeof(T);
+    HIP_TRY(hipMalloc((void**)p, (size_t)bytes));
+    HIP_TRY(hipMemset(*p, 0, (size_t)bytes));
+    return 0;
+}
+
+extern "C" int m3gpu_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+    {
+        return 0;
+    }
+    return n;
+}
+
+extern "C" const char* m3gpu_last_error(void) { return s_lastError; }
+
+static int CreateImpl(m3gpuWorld* w)
+{
+    const m3gpuCaps& c = w->caps;
+    m3View& v = w->view;
+    memset(&v, 0, sizeof(v));
+    v.pairCapacity = c.pairCapacity;
+    v.emitCapacity = 2 * c.pairCapacity;
+    int32_t nb = c.bodyCapacity;
+    int32_t ns = c.shapeCapacity;
+    int32_t np = c.pairCapacity;
+    HIP_TRY(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+    if (DevAlloc(&v.xf, nb) || DevAlloc(&v.lv, nb) || DevAlloc(&v.av, nb) ||
+        DevAlloc(&v.invMass, nb) || DevAlloc(&v.invI, nb) || DevAlloc(&v.inertia, nb) ||
+        DevAlloc(&v.lc, nb) || DevAlloc(&v.gravityScale, nb) || DevAlloc(&v.linDamp, nb) ||
+        DevAlloc(&v.angDamp, nb) || DevAlloc(&v.bodyAlive, nb) || DevAlloc(&v.type, nb) ||
+        DevAlloc(&v.awake, nb) || DevAlloc(&v.sleepTime, nb) || DevAlloc(&v.bullet, nb) ||
+        DevAlloc(&v.minExt, nb) || DevAlloc(&v.maxExt, nb) || DevAlloc(&v.force, nb) ||
+        DevAlloc(&v.torque, nb) || DevAlloc(&v.enabled, nb) || DevAlloc(&v.locks, nb) ||
+        DevAlloc(&v.sleepThreshold, nb) || DevAlloc(&v.canSleep, nb) ||
+        DevAlloc(&v.hasTarget, nb) || DevAlloc(&v.target, nb) || DevAlloc(&v.shapeHead, nb))
+    {
+        return -1;
+    }
+    if (DevAlloc(&v.shapeAlive, ns) || DevAlloc(&v.shapeBody, ns) || DevAlloc(&v.shapeType, ns) ||
+        DevAlloc(&v.geom, ns) || DevAlloc(&v.friction, ns) || DevAlloc(&v.restitution, ns) ||
+        DevAlloc(&v.rolling, ns) || DevAlloc(&v.sensor, ns) || DevAlloc(&v.hasOffset, ns) ||
+        DevAlloc(&v.localPos, ns) || DevAlloc(&v.localRot, ns) || DevAlloc(&v.surfaceVel, ns) ||
+        DevAlloc(&v.category, ns) || DevAlloc(&v.mask, ns) || DevAlloc(&v.group, ns) ||
+        DevAlloc(&v.shapeNext, ns) || DevAlloc(&v.hullIndex, ns) || DevAlloc(&v.leaf, 6 * (int64_t)ns) ||
+        DevAlloc(&v.fresh, 6 * (int64_t)ns) || DevAlloc(&v.planes, ns))
+    {
+        return -1;
+    }
+    m3HullData* hulls = nullptr;
+    if (DevAlloc(&hulls, c.hullCapacity))
+    {
+        return -1;
+    }
+    v.hulls = hulls;
+    for (int k = 0; k < 2; ++k)
+    {
+        if (DevAlloc(&w->keysBuf[k], np) || DevAlloc(&w->mansBuf[k], np))
+        {
+            return -1;
+        }
+    }
+    if (DevAlloc(&v.sleepKeys, np) || DevAlloc(&v.emit, v.emitCapacity) ||
+        DevAlloc(&w->emitSorted, v.emitCapacity) || DevAlloc(&w->emitUnique, v.emitCapacity) ||
+        DevAlloc(&v.cons, np) || DevAlloc(&v.consPair, np) || DevAlloc(&v.colors, np) ||
+        DevAlloc(&v.colorLists, np) || DevAlloc(&v.movers, nb) || DevAlloc(&v.deltaPos, nb) ||
+        DevAlloc(&v.deltaRot, nb) || DevAlloc(&v.com0, nb) || DevAlloc(&v.rot0, nb) ||
+        DevAlloc(&v.parent, nb) || DevAlloc(&v.forced, nb) || DevAlloc(&v.flag, nb) ||
+        DevAlloc(&v.sc, 1) || DevAlloc(&w->sapKeysIn, ns) || DevAlloc(&w->sapKeysOut, ns) ||
+        DevAlloc(&w->sapValsIn, ns) || DevAlloc(&w->sapValsOut, ns) ||
+        DevAlloc(&w->pairFlags, np) || DevAlloc(&w->bodyFlags, nb) || DevAlloc(&w->root, nb) ||
+        DevAlloc(&w->frozenNow, nb) || DevAlloc(&w->bodyMasks, nb))
+    {
+        return -1;
+    }
+    HIP_TRY(hipHostMalloc((void**)&w->hostScalars, sizeof(m3Scalars), hipHostMallocDefault));
+    memset(w->hostScalars, 0, sizeof(m3Scalars));
+
+    // One temp buffer sized for the largest primitive call.
+    size_t need = 0;
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, w->sapKeysIn, w->sapKeysOut,
+                                               w->sapValsIn, w->sapValsOut, ns, 0, 64, w->stream));
+    need = bytes > need ? bytes : need;
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, v.emit, w->emitSorted,
+                                              v.emitCapacity, 0, 32, w->stream));
+    need = bytes > need ? bytes : need;
+    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, bytes, w->emitSorted, w->emitUnique,
+                                         &v.sc->uniqueCount, v.emitCapacity, w->stream));
+    need = bytes > need ? bytes : need;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, v.keys, w->pairFlags, v.sleepKeys,
+                                          &v.sc->sleepCount, np, w->stream));
+    need = bytes > need ? bytes : need;
+    hipcub::CountingInputIterator<int32_t> counting(0);
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->pairFlags, v.consPair,
+                                          &v.sc->constraintCount, np, w->stream));
+    need = bytes > need ? bytes : need;
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->bodyFlags, v.movers,
+                                          &v.sc->moverCount, nb, w->stream));
+    need = bytes > need ? bytes : need;
+    w->cubBytes = need > 0 ? need : 1;
+    HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
+    w->cur = 0;
+    v.keys = w->keysBuf[0];
+    v.manifolds = w->mansBuf[0];
+    v.oldKeys = w->keysBuf[1];
+    v.oldManifolds = w->mansBuf[1];
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* caps)
+{
+    if (caps == nullptr || caps->bodyCapacity <= 0 || caps->shapeCapacity <= 0 ||
+        caps->pairCapacity <= 0 || caps->shapeCapacity > 65535)
+    {
+        snprintf(s_lastError, sizeof(s_lastError),
+                 "m3gpu_world_create: capacities must be positive and shapeCapacity <= 65535");
+        return nullptr;
+    }
+    m3gpuWorld* w = (m3gpuWorld*)calloc(1, sizeof(m3gpuWorld));
+    if (w == nullptr)
+    {
+        return nullptr;
+    }
+    w->caps = *caps;
+    if (CreateImpl(w) != 0)
+    {
+        free(w);
+        return nullptr;
+    }
+    return w;
+}
+
+extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
+{
+    if (w == nullptr)
+    {
+        return;
+    }
+    hipStreamSynchronize(w->stream);
+    m3View& v = w->view;
+    void* ptrs[] = {v.xf, v.lv, v.av, v.invMass, v.invI, v.inertia, v.lc, v.gravityScale,
+                    v.linDamp, v.angDamp, v.bodyAlive, v.type, v.awake, v.sleepTime, v.bullet,
+                    v.minExt, v.maxExt, v.force, v.torque, v.enabled, v.locks, v.sleepThreshold,
+                    v.canSleep, v.hasTarget, v.target, v.shapeHead, v.shapeAlive, v.shapeBody,
+                    v.shapeType, v.geom, v.friction, v.restitution, v.rolling, v.sensor,
+                    v.hasOffset, v.localPos, v.localRot, v.surfaceVel, v.category, v.mask,
+                    v.group, v.shapeNext, v.hullIndex, v.leaf, v.fresh, v.planes,
+                    (void*)v.hulls, w->keysBuf[0], w->keysBuf[1], w->mansBuf[0], w->mansBuf[1],
+                    v.sleepKeys, v.emit, w->emitSorted, w->emitUnique, v.cons, v.consPair,
+                    v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
+                    v.parent, v.forced, v.flag, v.sc, w->sapKeysIn, w->sapKeysOut, w->sapValsIn,
+                    w->sapValsOut, w->pairFlags, w->bodyFlags, w->root, w->frozenNow,
+                    w->bodyMasks, w->cubTemp};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
+    {
+        if (ptrs[i] != nullptr)
+        {
+            hipFree(ptrs[i]);
+        }
+    }
+    hipHostFree(w->hostScalars);
+    hipStreamDestroy(w->stream);
+    free(w);
+}
+
+static int Copy(m3gpuWorld* w, void* dst, const void* src, int64_t bytes, hipMemcpyKind kind)
+{
+    if (bytes <= 0)
+    {
+        return 0;
+    }
+    HIP_TRY(hipMemcpyAsync(dst, src, (size_t)bytes, kind, w->stream));
+    return 0;
+}
+
+#define UP(dev, host, count, type) \
+    if (Copy(w, (dev), (host), (int64_t)(count) * (int64_t)sizeof(type), hipMemcpyHostToDevice)) return -1
+#define DOWN(host, dev, count, type) \
+    if (Copy(w, (host), (dev), (int64_t)(count) * (int64_t)sizeof(type), hipMemcpyDeviceToHost)) return -1
+
+extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fields)
+{
+    if (w == nullptr || h == nullptr)
+    {
+        return -1;
+    }
+    if (h->maxBody > w->caps.bodyCapacity || h->maxShape > w->caps.shapeCapacity ||
+        h->maxHull > w->caps.hullCapacity)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_upload: counts exceed capacities");
+        return -1;
+    }
+    m3View& v = w->view;
+    v.maxBody = h->maxBody;
+    v.maxShape = h->maxShape;
+    int32_t nb = h->maxBody;
+    int32_t ns = h->maxShape;
+    if (fields & M3GPU_FIELDS_STATIC)
+    {
+        UP(v.invMass, h->invMass, nb, float);
+        UP(v.invI, h->invInertiaLocal, nb, m3Mat3);
+        UP(v.inertia, h->inertiaLocal, nb, m3Mat3);
+        UP(v.lc, h->localCenters, nb, m3Vec3);
+        UP(v.gravityScale, h->gravityScales, nb, float);
+        UP(v.linDamp, h->linearDamping, nb, float);
+        UP(v.angDamp, h->angularDamping, nb, float);
+        UP(v.bodyAlive, h->bodyAlive, nb, uint8_t);
+        UP(v.type, h->types, nb, uint8_t);
+        UP(v.bullet, h->bulletFlags, nb, uint8_t);
+        UP(v.minExt, h->minExtents, nb, float);
+        UP(v.maxExt, h->maxExtents, nb, float);
+        UP(v.enabled, h->bodyEnabled, nb, uint8_t);
+        UP(v.locks, h->bodyLocks, nb, uint8_t);
+        UP(v.sleepThreshold, h->bodySleepThreshold, nb, float);
+        UP(v.canSleep, h->bodyCanSleep, nb, uint8_t);
+        UP(v.target, h->bodyTarget, nb, m3Transform);
+        UP(v.shapeHead, h->bodyShapeHead, nb, int32_t);
+        UP(v.shapeAlive, h->shapeAlive, ns, uint8_t);
+        UP(v.shapeBody, h->shapeBody, ns, int32_t);
+        UP(v.shapeType, h->shapeType, ns, uint8_t);
+        UP(v.geom, h->shapeGeom, ns, m3ShapeGeom);
+        UP(v.friction, h->shapeFriction, ns, float);
+        UP(v.restitution, h->shapeRestitution, ns, float);
+        UP(v.rolling, h->shapeRollingResistance, ns, float);
+        UP(v.sensor, h->shapeSensor, ns, uint8_t);
+        UP(v.hasOffset, h->shapeHasOffset, ns, uint8_t);
+        UP(v.localPos, h->shapeLocalPos, ns, m3Vec3);
+        UP(v.localRot, h->shapeLocalRot, ns, m3Quat);
+        UP(v.surfaceVel, h->shapeSurfaceVel, ns, m3Vec3);
+        UP(v.category, h->shapeCategory, ns, uint64_t);
+        UP(v.mask, h->shapeMask, ns, uint64_t);
+        UP(v.group, h->shapeGroup, ns, int32_t);
+        UP(v.shapeNext, h->shapeNext, ns, int32_t);
+        UP(v.hullIndex, h->shapeHullIndex, ns, int32_t);
+        UP((void*)v.hulls, h->hullData, h->maxHull, m3HullData);
+        // The plane list and the unsupported-target census are host facts.
+        const uint8_t* alive = h->shapeAlive;
+        const uint8_t* stype = h->shapeType;
+        int32_t* planes = (int32_t*)malloc((size_t)(ns > 0 ? ns : 1) * sizeof(int32_t));
+        int32_t pc = 0;
+        int unsupported = 0;
+        for (int32_t s = 0; s < ns; ++s)
+        {
+            if (alive[s] == 0)
+            {
+                continue;
+            }
+            if (stype[s] == M3_PLANE)
+            {
+                planes[pc++] = s;
+            }
+            else if (stype[s] != M3_SPHERE && stype[s] != M3_HULL && stype[s] != M3_CAPSULE)
+            {
+                unsupported = 1;
+            }
+            else if (h->types[h->shapeBody[s]] == M3_STATIC)
+            {
+                unsupported = 1; // static tree targets for continuous sweeps
+            }
+        }
+        for (int32_t b = 0; b < nb; ++b)
+        {
+            if (h->bodyAlive[b] != 0 && h->bulletFlags[b] != 0)
+            {
+                unsupported = 1;
+            }
+        }
+        v.planeCount = pc;
+        w->ccdUnsupported = unsupported;
+        int rc = Copy(w, v.planes, planes, (int64_t)pc * 4, hipMemcpyHostToDevice);
+        HIP_TRY(hipStreamSynchronize(w->stream));
+        free(planes);
+        if (rc)
+        {
+            return -1;
+        }
+    }
+    if (fields & M3GPU_FIELDS_DYNAMIC)
+    {
+        UP(v.xf, h->transforms, nb, m3Transform);
+        UP(v.lv, h->linearVelocities, nb, m3Vec3);
+        UP(v.av, h->angularVelocities, nb, m3Vec3);
+        UP(v.awake, h->awake, nb, uint8_t);
+        UP(v.sleepTime, h->sleepTimes, nb, float);
+        UP(v.force, h->bodyForce, nb, m3Vec3);
+        UP(v.torque, h->bodyTorque, nb, m3Vec3);
+        UP(v.hasTarget, h->bodyHasTarget, nb, uint8_t);
+        UP(&v.sc->stepCount, h->stepCount, 1, uint64_t);
+    }
+    if (fields & M3GPU_FIELDS_PAIRS)
+    {
+        int32_t pc = *h->pairCount;
+        int32_t zc = *h->sleepingPairCount;
+        UP(v.keys, h->pairKeys, pc, uint64_t);
+        UP(v.manifolds, h->manifolds, pc, m3Manifold);
+        UP(&v.sc->pairCount, h->pairCount, 1, int32_t);
+        UP(v.sleepKeys, h->sleepingPairKeys, zc, uint64_t);
+        UP(&v.sc->sleepCount, h->sleepingPairCount, 1, int32_t);
+        UP(v.leaf, h->leafBoxes, 6 * (int64_t)ns, double);
+        w->fullQuery = *h->pairsFullQuery != 0;
+    }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
+{
+    if (w == nullptr || h == nullptr)
+    {
+        return -1;
+    }
+    m3View& v = w->view;
+    int32_t nb = v.maxBody;
+    int32_t ns = v.maxShape;
+    if (fields & M3GPU_FIELDS_DYNAMIC)
+    {
+        DOWN(h->transforms, v.xf, nb, m3Transform);
+        DOWN(h->linearVelocities, v.lv, nb, m3Vec3);
+        DOWN(h->angularVelocities, v.av, nb, m3Vec3);
+        DOWN(h->awake, v.awake, nb, uint8_t);
+        DOWN(h->sleepTimes, v.sleepTime, nb, float);
+        DOWN(h->bodyForce, v.force, nb, m3Vec3);
+        DOWN(h->bodyTorque, v.torque, nb, m3Vec3);
+        DOWN(h->bodyHasTarget, v.hasTarget, nb, uint8_t);
+        DOWN(h->stepCount, &v.sc->stepCount, 1, uint64_t);
+    }
+    if (fields & M3GPU_FIELDS_PAIRS)
+    {
+        DOWN(h->pairCount, &v.sc->pairCount, 1, int32_t);
+        DOWN(h->sleepingPairCount, &v.sc->sleepCount, 1, int32_t);
+        HIP_TRY(hipStreamSynchronize(w->stream));
+        int32_t pc = *h->pairCount;
+        int32_t zc = *h->sleepingPairCount;
+        DOWN(h->pairKeys, v.keys, pc, uint64_t);
+        DOWN(h->manifolds, v.manifolds, pc, m3Manifold);
+        DOWN(h->sleepingPairKeys, v.sleepKeys, zc, uint64_t);
+        DOWN(h->leafBoxes, v.leaf, 6 * (int64_t)ns, double);
+        *h->frozenDirty = 0;
+    }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+// Enqueue one step.  Launch geometry only depends on host-known capacities
+// and high-water marks; every data-dependent count is read on the device.
+static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
+{
+    hipStream_t st = w->stream;
+    // Swap the pair buffers: last step's list becomes the carry source.
+    w->cur ^= 1;
+    m3View& v = w->view;
+    v.keys = w->keysBuf[w->cur];
+    v.manifolds = w->mansBuf[w->cur];
+    v.oldKeys = w->keysBuf[w->cur ^ 1];
+    v.oldManifolds = w->mansBuf[w->cur ^ 1];
+    const int T = 256;
+    int32_t nb = v.maxBody;
+    int32_t ns = v.maxShape;
+    int32_t np = v.pairCapacity;
+    int32_t ne = v.emitCapacity;
+    size_t tb = w->cubBytes;
+
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, st, v);
+    hipLaunchKernelGGL(k_fill32, dim3(Blocks(ne, T)), dim3(T), 0, st, v.emit, ne, M3_SENTINEL32);
+    // Broadphase.
+    hipLaunchKernelGGL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysIn, w->sapValsIn);
+    if (w->fullQuery)
+    {
+        hipLaunchKernelGGL(k_reset_sleep, dim3(1), dim3(1), 0, st, v);
+        hipLaunchKernelGGL(k_discover_all, dim3(Blocks(ns, T)), dim3(T), 0, st, v);
+        w->fullQuery = 0;
+    }
+    hipLaunchKernelGGL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v);
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(w->cubTemp, tb, w->sapKeysIn, w->sapKeysOut,
+                                               w->sapValsIn, w->sapValsOut, ns, 0, 64, st));
+    hipLaunchKernelGGL(k_sap, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysOut, w->sapValsOut);
+    if (v.planeCount > 0)
+    {
+        hipLaunchKernelGGL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, v);
+    }
+    hipLaunchKernelGGL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, v);
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(w->cubTemp, tb, v.emit, w->emitSorted, ne, 0, 32, st));
+    HIP_TRY(hipcub::DeviceSelect::Unique(w->cubTemp, tb, w->emitSorted, w->emitUnique,
+                                         &v.sc->uniqueCount, ne, st));
+    hipLaunchKernelGGL(k_pair_count, dim3(1), dim3(1), 0, st, v, w->emitUnique);
+    hipLaunchKernelGGL(k_expand, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->emitUnique, w->pairFlags);
+    hipLaunchKernelGGL(k_stall_restore, dim3(Blocks(np, T)), dim3(T), 0, st, v);
+    HIP_TRY(hipcub::DeviceSelect::Flagged(w->cubTemp, tb, v.keys, w->pairFlags, v.sleepKeys,
+                                          &v.sc->sleepCount, np, st));
+    // Narrowphase.
+    hipLaunchKernelGGL(k_narrow, dim3(Blocks(np, 64)), dim3(64), 0, st, v);
+    // Sweeps, islands, wake.
+    hipLaunchKernelGGL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, v);
+    hipLaunchKernelGGL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, v);
+    hipLaunchKernelGGL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
+    hipLaunchKernelGGL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
+    hipLaunchKernelGGL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
+    // Prepare and color.
+    float h = p->dt / (float)p->substeps;
+    float invH = h > 0.0f ? 1.0f / h : 0.0f;
+    hipLaunchKernelGGL(k_constraint_flags, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->pairFlags);
+    hipcub::CountingInputIterator<int32_t> counting(0);
+    HIP_TRY(hipcub::DeviceSelect::Flagged(w->cubTemp, tb, counting, w->pairFlags, v.consPair,
+                                          &v.sc->constraintCount, np, st));
+    hipLaunchKernelGGL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, v, p->contactHertz,
+                       p->contactDampingRatio, h);
+    hipLaunchKernelGGL(k_color, dim3(1), dim3(M3_COLOR_CHUNK), 0, st, v, w->bodyMasks);
+    hipLaunchKernelGGL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->bodyFlags, p->dt);
+    HIP_TRY(hipcub::DeviceSelect::Flagged(w->cubTemp, tb, counting, w->bodyFlags, v.movers,
+                                          &v.sc->moverCount, nb, st));
+    // Substeps.
+    m3Vec3 g = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
+    unsigned passBlocks = Blocks(np, 128) < 512 ? Blocks(np, 128) : 512;
+    for (int32_t sub = 0; sub < p->substeps; ++sub)
+    {
+        hipLaunchKernelGGL(k_integrate_v, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h, g,
+                           p->maximumLinearSpeed, p->maximumAngularSpeed);
+        for (int mode = 0; mode < 2; ++mode)
+        {
+            for (int32_t color = 0; color < M3_GRAPH_COLORS; ++color)
+            {
+                hipLaunchKernelGGL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, mode,
+                                   invH, p->contactPushMaxSpeed);
+            }
+            hipLaunchKernelGGL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, mode, invH,
+                               p->contactPushMaxSpeed);
+        }
+        hipLaunchKernelGGL(k_integrate_x, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h);
+        for (int32_t color = 0; color < M3_GRAPH_COLORS; ++color)
+        {
+            hipLaunchKernelGGL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, 2, invH,
+                               p->contactPushMaxSpeed);
+        }
+        hipLaunchKernelGGL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, 2, invH,
+                           p->contactPushMaxSpeed);
+    }
+    hipLaunchKernelGGL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, v);
+    hipLaunchKernelGGL(k_restitution, dim3(1), dim3(1), 0, st, v, p->restitutionThreshold);
+    hipLaunchKernelGGL(k_store, dim3(Blocks(np, T)), dim3(T), 0, st, v);
+    if (p->continuousEnabled)
+    {
+        hipLaunchKernelGGL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, v);
+    }
+    hipLaunchKernelGGL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root, p->dt,
+                       p->sleepEnabled);
+    if (p->sleepEnabled)
+    {
+        hipLaunchKernelGGL(k_sleep_ready, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
+        hipLaunchKernelGGL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root,
+                           w->frozenNow);
+        hipLaunchKernelGGL(k_fresh_if_frozen, dim3(Blocks(ns, T)), dim3(T), 0, st, v);
+        hipLaunchKernelGGL(k_discover_frozen, dim3(Blocks(ns, 64)), dim3(64), 0, st, v,
+                           w->frozenNow);
+    }
+    hipLaunchKernelGGL(k_end, dim3(1), dim3(1), 0, st, v);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStats* stats)
+{
+    if (w == nullptr || p == nullptr || !(p->dt > 0.0f) || p->substeps < 1 ||
+        p->substeps > M3_MAX_SUBSTEPS)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step: bad arguments");
+        return -1;
+    }
+    if (p->continuousEnabled && w->ccdUnsupported)
+    {
+        snprintf(s_lastError, sizeof(s_lastError),
+                 "m3gpu_step: continuous collision against static non-plane shapes or bullets "
+                 "is not implemented on the device path");
+        return -1;
+    }
+    if (EnqueueStep(w, p) != 0)
+    {
+        return -1;
+    }
+    if (stats != nullptr)
+    {
+        return m3gpu_last_stats(w, stats) != 0 ? -1 : (w->hostScalars->stall ? M3GPU_STALL : 0);
+    }
+    return 0;
+}
+
+extern "C" int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,
+                               int32_t count)
+{
+    for (int32_t k = 0; k < count; ++k)
+    {
+        if (m3gpu_step(worlds[k], &params[k], nullptr) < 0)
+        {
+            return -1;
+        }
+    }
+    return 0;
+}
+
+extern "C" int m3gpu_sync(m3gpuWorld* w)
+{
+    if (w == nullptr)
+    {
+        return -1;
+    }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    return 0;
+}
+
+extern "C" int m3gpu_last_stats(m3gpuWorld* w, m3gpuStepStats* stats)
+{
+    if (w == nullptr || stats == nullptr)
+    {
+        return -1;
+    }
+    HIP_TRY(hipMemcpyAsync(w->hostScalars, w->view.sc, sizeof(m3Scalars), hipMemcpyDeviceToHost,
+                           w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    const m3Scalars* s = w->hostScalars;
+    memset(stats, 0, sizeof(*stats));
+    stats->pairCount = s->pairCount;
+    stats->constraintCount = s->constraintCount;
+    stats->colorCount = s->colorCount;
+    stats->islandCount = s->islandCount;
+    stats->awakeCount = s->moverCount;
+    return 0;
+}

@@ -1,0 +1,1596 @@
+// Per-item stage bodies of the device step.  Each function handles one
+// shape, pair, constraint or body and is called by exactly one GPU thread;
+// the orchestration (ordering, compaction, graph-color scheduling) lives in
+// m3_device.hip.  Arithmetic follows the reference stage it names, in the
+// same operation order, so the device reproduces the CPU engine's bits.
+#ifndef M3_STEP_H
+#define M3_STEP_H
+
+#include "m3_collide.h"
+
+// ---------------------------------------------------------------------------
+// Device-resident world: structure-of-arrays in HBM.  Body and shape records
+// keep the reference byte layouts so uploads and downloads are plain copies.
+// ---------------------------------------------------------------------------
+
+typedef struct m3Softness
+{
+    m3real biasRate;
+    m3real massScale;
+    m3real impulseScale;
+} m3Softness;
+
+typedef struct m3CPoint
+{
+    m3Vec3 rA;
+    m3Vec3 rB;
+    m3real baseSeparation;
+    m3real normalMass;
+    m3real leverArm;
+    m3real relativeVelocity;
+    m3real normalImpulse;
+    m3real totalNormalImpulse;
+} m3CPoint;
+
+// One contact constraint (the prepared form of one touching manifold).
+typedef struct m3Constraint
+{
+    int32_t bodyA;
+    int32_t bodyB;
+    int32_t manifoldIndex;
+    int32_t pointCount;
+    m3Vec3 normal;
+    m3Vec3 t1;
+    m3Vec3 t2;
+    m3Vec3 originA;
+    m3Vec3 originB;
+    m3real frictionK11;
+    m3real frictionK12;
+    m3real frictionK22;
+    m3real frictionImpulse1;
+    m3real frictionImpulse2;
+    m3real twistMass;
+    m3real twistImpulse;
+    m3real tangentVelocity1;
+    m3real tangentVelocity2;
+    m3real friction;
+    m3real restitution;
+    m3real invMassA;
+    m3real invMassB;
+    m3Mat3 invIA;
+    m3Mat3 invIB;
+    m3Softness softness;
+    m3real rollingResistance;
+    m3Vec3 rollingImpulse;
+    m3Mat3 rollingK;
+    uint8_t dynA; // body A is dynamic (impulses land on it)
+    uint8_t dynB;
+    uint8_t pad[2];
+    m3CPoint points[M3_MANIFOLD_MAX_POINTS];
+} m3Constraint;
+
+// Step-wide scalars that live on the device so no stage needs a host
+// round trip.
+typedef struct m3Scalars
+{
+    int32_t pairCount;
+    int32_t oldPairCount;
+    int32_t sleepCount;
+    int32_t emitCount;
+    int32_t uniqueCount;
+    int32_t constraintCount;
+    int32_t moverCount;
+    int32_t stall;
+    int32_t anyRestitution;
+    int32_t islandCount;
+    int32_t awakeCount;
+    int32_t colorCount;
+    int32_t discoverCount;
+    int32_t pad;
+    int32_t colorStarts[M3_GRAPH_COLORS + 2];
+    uint64_t stepCount;
+} m3Scalars;
+
+typedef struct m3View
+{
+    int32_t maxBody;
+    int32_t maxShape;
+    int32_t pairCapacity;
+    int32_t emitCapacity;
+    int32_t planeCount;
+    // bodies
+    m3Transform* xf;
+    m3Vec3* lv;
+    m3Vec3* av;
+    float* invMass;
+    m3Mat3* invI;
+    m3Mat3* inertia;
+    m3Vec3* lc;
+    float* gravityScale;
+    float* linDamp;
+    float* angDamp;
+    uint8_t* bodyAlive;
+    uint8_t* type;
+    uint8_t* awake;
+    float* sleepTime;
+    uint8_t* bullet;
+    float* minExt;
+    float* maxExt;
+    m3Vec3* force;
+    m3Vec3* torque;
+    uint8_t* enabled;
+    uint8_t* locks;
+    float* sleepThreshold;
+    uint8_t* canSleep;
+    uint8_t* hasTarget;
+    m3Transform* target;
+    int32_t* shapeHead;
+    // shapes
+    uint8_t* shapeAlive;
+    int32_t* shapeBody;
+    uint8_t* shapeType;
+    m3ShapeGeom* geom;
+    float* friction;
+    float* restitution;
+    float* rolling;
+    uint8_t* sensor;
+    uint8_t* hasOffset;
+    m3Vec3* localPos;
+    m3Quat* localRot;
+    m3Vec3* surfaceVel;
+    uint64_t* category;
+    uint64_t* mask;
+    int32_t* group;
+    int32_t* shapeNext;
+    int32_t* hullIndex;
+    double* leaf;   // 6 per shape
+    double* fresh;  // 6 per shape, this step's tight bounds
+    int32_t* planes; // alive plane shapes, ascending
+    const m3HullData* hulls;
+    // pairs
+    uint64_t* keys;
+    m3Manifold* manifolds;
+    const uint64_t* oldKeys;
+    const m3Manifold* oldManifolds;
+    uint64_t* sleepKeys;
+    uint32_t* emit;
+    // solver scratch
+    m3Constraint* cons;
+    int32_t* consPair;  // pair index of each constraint
+    uint8_t* colors;
+    int32_t* colorLists;
+    int32_t* movers;
+    m3Vec3* deltaPos;
+    m3Quat* deltaRot;
+    m3Pos3* com0;
+    m3Quat* rot0;
+    int32_t* parent;
+    uint8_t* forced;
+    uint8_t* flag;
+    m3Scalars* sc;
+} m3View;
+
+// ---------------------------------------------------------------------------
+// Shared helpers
+// ---------------------------------------------------------------------------
+
+// src/world_internal.h:1069-1083
+M3_HD m3Transform m3ShapeXf(const m3View* v, int32_t s)
+{
+    m3Transform xf = v->xf[v->shapeBody[s]];
+    if (v->hasOffset[s] != 0)
+    {
+        m3Vec3 r = m3RotateVec3(xf.q, v->localPos[s]);
+        xf.p.x += (double)r.x;
+        xf.p.y += (double)r.y;
+        xf.p.z += (double)r.z;
+        xf.q = m3MulQuat(xf.q, v->localRot[s]);
+    }
+    return xf;
+}
+
+// src/broad_phase.c:24-148: the margin-inflated tight box of one shape.
+M3_HD void m3FreshBox(const m3View* v, int32_t s, double box[6])
+{
+    m3Transform xf = m3ShapeXf(v, s);
+    uint8_t type = v->shapeType[s];
+    if (type == M3_HULL)
+    {
+        const m3HullData* hull = &v->hulls[v->hullIndex[s]];
+        double lo0 = 1.0e30, lo1 = 1.0e30, lo2 = 1.0e30;
+        double hi0 = -1.0e30, hi1 = -1.0e30, hi2 = -1.0e30;
+        for (int32_t k = 0; k < hull->vertexCount; ++k)
+        {
+            m3Vec3 r = m3RotateVec3(xf.q, hull->vertices[k]);
+            double p0 = xf.p.x + (double)r.x;
+            double p1 = xf.p.y + (double)r.y;
+            double p2 = xf.p.z + (double)r.z;
+            lo0 = p0 < lo0 ? p0 : lo0;
+            hi0 = p0 > hi0 ? p0 : hi0;
+            lo1 = p1 < lo1 ? p1 : lo1;
+            hi1 = p1 > hi1 ? p1 : hi1;
+            lo2 = p2 < lo2 ? p2 : lo2;
+            hi2 = p2 > hi2 ? p2 : hi2;
+        }
+        box[0] = lo0 - (double)M3_AABB_MARGIN;
+        box[1] = lo1 - (double)M3_AABB_MARGIN;
+        box[2] = lo2 - (double)M3_AABB_MARGIN;
+        box[3] = hi0 + (double)M3_AABB_MARGIN;
+        box[4] = hi1 + (double)M3_AABB_MARGIN;
+        box[5] = hi2 + (double)M3_AABB_MARGIN;
+        return;
+    }
+    if (type == M3_CAPSULE)
+    {
+        m3Vec3 r1 = m3RotateVec3(xf.q, v->geom[s].v);
+        m3Vec3 r2 = m3RotateVec3(xf.q, v->geom[s].v2);
+        double c1[3] = {xf.p.x + (double)r1.x, xf.p.y + (double)r1.y, xf.p.z + (double)r1.z};
+        double c2[3] = {xf.p.x + (double)r2.x, xf.p.y + (double)r2.y, xf.p.z + (double)r2.z};
+        double fat = (double)(v->geom[s].s + M3_AABB_MARGIN);
+        for (int32_t k = 0; k < 3; ++k)
+        {
+            box[k] = (c1[k] < c2[k] ? c1[k] : c2[k]) - fat;
+            box[3 + k] = (c1[k] > c2[k] ? c1[k] : c2[k]) + fat;
+        }
+        return;
+    }
+    m3Vec3 r = m3RotateVec3(xf.q, v->geom[s].v);
+    double cx = xf.p.x + (double)r.x;
+    double cy = xf.p.y + (double)r.y;
+    double cz = xf.p.z + (double)r.z;
+    double fat = (double)(v->geom[s].s + M3_AABB_MARGIN);
+    box[0] = cx - fat;
+    box[1] = cy - fat;
+    box[2] = cz - fat;
+    box[3] = cx + fat;
+    box[4] = cy + fat;
+    box[5] = cz + fat;
+}
+
+M3_HD int m3BoxOverlap(const double* a, const double* b)
+{
+    return a[0] <= b[3] && b[0] <= a[3] && a[1] <= b[4] && b[1] <= a[4] && a[2] <= b[5] &&
+           b[2] <= a[5];
+}
+
+// src/broad_phase.c:182-232 (joint lists are empty on this path).
+M3_HD int m3PairAllowed(const m3View* v, int32_t i, int32_t j)
+{
+    int32_t bi = v->shapeBody[i];
+    int32_t bj = v->shapeBody[j];
+    if (bi == bj)
+    {
+        return 0;
+    }
+    if (v->enabled[bi] == 0 || v->enabled[bj] == 0)
+    {
+        return 0;
+    }
+    if (v->type[bi] == M3_STATIC && v->type[bj] == M3_STATIC)
+    {
+        return 0;
+    }
+    if (v->sensor[i] != 0 && v->sensor[j] != 0)
+    {
+        return 0;
+    }
+    int32_t gi = v->group[i];
+    if (gi != 0 && gi == v->group[j])
+    {
+        if (gi < 0)
+        {
+            return 0;
+        }
+    }
+    else if (!((v->category[i] & v->mask[j]) != 0 && (v->category[j] & v->mask[i]) != 0))
+    {
+        return 0;
+    }
+    return 1;
+}
+
+// src/broad_phase.c:238-251
+M3_HD int m3PairHot(const m3View* v, int32_t i, int32_t j)
+{
+    int32_t bi = v->shapeBody[i];
+    int32_t bj = v->shapeBody[j];
+    if (v->type[bi] != M3_STATIC && v->awake[bi] != 0)
+    {
+        return 1;
+    }
+    return v->type[bj] != M3_STATIC && v->awake[bj] != 0;
+}
+
+// A shape runs its own tree query when its body is static or awake.
+M3_HD int m3ShapeQueries(const m3View* v, int32_t s)
+{
+    int32_t b = v->shapeBody[s];
+    return v->type[b] == M3_STATIC || v->awake[b] != 0;
+}
+
+// Would the reference's tree pass emit the overlapping pair (a, b)?  The
+// query runs from the lower index when both sides query, otherwise from the
+// querying side; the hit needs the other side's stored proxy box to
+// overlap the querier's fresh box (src/broad_phase.c:285-349,
+// src/dynamic_tree.c:367-406), then the fresh re-test.
+M3_HD int m3TreePairEmits(const m3View* v, int32_t a, int32_t b)
+{
+    int qa = m3ShapeQueries(v, a);
+    int qb = m3ShapeQueries(v, b);
+    if (!qa && !qb)
+    {
+        return 0;
+    }
+    if (!m3PairAllowed(v, a, b) || !m3PairHot(v, a, b))
+    {
+        return 0;
+    }
+    int32_t self;
+    int32_t other;
+    if (qa && qb)
+    {
+        self = a < b ? a : b;
+        other = a < b ? b : a;
+    }
+    else
+    {
+        self = qa ? a : b;
+        other = qa ? b : a;
+    }
+    if (!m3BoxOverlap(&v->leaf[6 * other], &v->fresh[6 * self]))
+    {
+        return 0;
+    }
+    return m3BoxOverlap(&v->fresh[6 * self], &v->fresh[6 * other]);
+}
+
+// Compact 32-bit pair key: low shape in the high half.
+M3_HD uint32_t m3CompactKey(int32_t i, int32_t j)
+{
+    return i < j ? (((uint32_t)i << 16) | (uint32_t)j) : (((uint32_t)j << 16) | (uint32_t)i);
+}
+
+M3_HD uint64_t m3ExpandKey(uint32_t k)
+{
+    return ((uint64_t)(k >> 16) << 32) | (uint64_t)(k & 0xFFFFu);
+}
+
+// ---------------------------------------------------------------------------
+// Narrowphase (src/manifold.c:2357-2920)
+// ---------------------------------------------------------------------------
+
+M3_HD m3Vec3 m3FromCom(const m3View* v, int32_t body, double px, double py, double pz)
+{
+    const m3Transform xf = v->xf[body];
+    m3Vec3 rlc = m3RotateVec3(xf.q, v->lc[body]);
+    return m3V3((m3real)(px - xf.p.x - (double)rlc.x), (m3real)(py - xf.p.y - (double)rlc.y),
+                (m3real)(pz - xf.p.z - (double)rlc.z));
+}
+
+M3_HD m3Proxy m3MakeProxy(const m3View* v, int32_t s, m3Vec3 scratch[2])
+{
+    m3Proxy proxy;
+    uint8_t type = v->shapeType[s];
+    if (type == M3_HULL)
+    {
+        const m3HullData* hull = &v->hulls[v->hullIndex[s]];
+        proxy.points = hull->vertices;
+        proxy.count = hull->vertexCount;
+        proxy.radius = 0.0f;
+        return proxy;
+    }
+    if (type == M3_CAPSULE)
+    {
+        scratch[0] = v->geom[s].v;
+        scratch[1] = v->geom[s].v2;
+        proxy.points = scratch;
+        proxy.count = 2;
+        proxy.radius = v->geom[s].s;
+        return proxy;
+    }
+    scratch[0] = v->geom[s].v;
+    proxy.points = scratch;
+    proxy.count = 1;
+    proxy.radius = v->geom[s].s;
+    return proxy;
+}
+
+M3_HD void m3SwapAnchors(m3Manifold* m, int32_t count)
+{
+    m->normal = m3Neg3(m->normal);
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3Vec3 tmp = m->points[k].anchorA;
+        m->points[k].anchorA = m->points[k].anchorB;
+        m->points[k].anchorB = tmp;
+    }
+}
+
+M3_HD m3Manifold m3CollidePlaneHull(const m3View* v, int32_t shapeA, int32_t planeShape,
+                                    int32_t hullShape)
+{
+    m3Manifold fresh = m3ZeroManifold();
+    const m3HullData* hull = &v->hulls[v->hullIndex[hullShape]];
+    int32_t planeBody = v->shapeBody[planeShape];
+    int32_t hullBody = v->shapeBody[hullShape];
+    m3Transform xf = m3ShapeXf(v, hullShape);
+    m3Vec3 n = v->geom[planeShape].v;
+    m3real offset = v->geom[planeShape].s;
+    // Deepest four below the margin (ties to the lower vertex), kept in
+    // ascending vertex order: a running top-4 is the same selection as the
+    // reference's repeated min scan over all candidates.
+    int32_t kIdx[M3_MANIFOLD_MAX_POINTS];
+    m3real kSep[M3_MANIFOLD_MAX_POINTS];
+    double kW[M3_MANIFOLD_MAX_POINTS][3];
+    int32_t kc = 0;
+    for (int32_t vi = 0; vi < hull->vertexCount; ++vi)
+    {
+        m3Vec3 r = m3RotateVec3(xf.q, hull->vertices[vi]);
+        double wx = xf.p.x + (double)r.x;
+        double wy = xf.p.y + (double)r.y;
+        double wz = xf.p.z + (double)r.z;
+        double sep = (double)n.x * wx + (double)n.y * wy + (double)n.z * wz - (double)offset;
+        m3real fs = (m3real)sep;
+        if (!(fs < M3_SPECULATIVE_DISTANCE))
+        {
+            continue;
+        }
+        // Insert if among the four deepest so far: candidates arrive in
+        // ascending index, so a strictly smaller separation wins and ties
+        // keep the earlier (lower) index.
+        int32_t pos = kc;
+        while (pos > 0 && fs < kSep[pos - 1])
+        {
+            pos -= 1;
+        }
+        if (pos >= M3_MANIFOLD_MAX_POINTS)
+        {
+            continue;
+        }
+        int32_t last = kc < M3_MANIFOLD_MAX_POINTS ? kc : M3_MANIFOLD_MAX_POINTS - 1;
+        for (int32_t q = last; q > pos; --q)
+        {
+            kIdx[q] = kIdx[q - 1];
+            kSep[q] = kSep[q - 1];
+            kW[q][0] = kW[q - 1][0];
+            kW[q][1] = kW[q - 1][1];
+            kW[q][2] = kW[q - 1][2];
+        }
+        kIdx[pos] = vi;
+        kSep[pos] = fs;
+        kW[pos][0] = wx;
+        kW[pos][1] = wy;
+        kW[pos][2] = wz;
+        if (kc < M3_MANIFOLD_MAX_POINTS)
+        {
+            kc += 1;
+        }
+    }
+    // Ascending vertex index among the kept.
+    for (int32_t a = 0; a < kc; ++a)
+    {
+        for (int32_t b = a + 1; b < kc; ++b)
+        {
+            if (kIdx[b] < kIdx[a])
+            {
+                int32_t ti = kIdx[a];
+                kIdx[a] = kIdx[b];
+                kIdx[b] = ti;
+                m3real ts = kSep[a];
+                kSep[a] = kSep[b];
+                kSep[b] = ts;
+                for (int32_t c = 0; c < 3; ++c)
+                {
+                    double tw = kW[a][c];
+                    kW[a][c] = kW[b][c];
+                    kW[b][c] = tw;
+                }
+            }
+        }
+    }
+    fresh.normal = n;
+    fresh.pointCount = kc;
+    for (int32_t k = 0; k < kc; ++k)
+    {
+        double sepD = (double)kSep[k];
+        double px = kW[k][0] - (double)n.x * sepD;
+        double py = kW[k][1] - (double)n.y * sepD;
+        double pz = kW[k][2] - (double)n.z * sepD;
+        fresh.points[k].anchorA = m3FromCom(v, planeBody, px, py, pz);
+        fresh.points[k].anchorB = m3FromCom(v, hullBody, kW[k][0], kW[k][1], kW[k][2]);
+        fresh.points[k].separation = kSep[k];
+        fresh.points[k].id = (uint16_t)kIdx[k];
+    }
+    if (planeShape != shapeA)
+    {
+        m3SwapAnchors(&fresh, kc);
+    }
+    return fresh;
+}
+
+M3_HD m3Manifold m3CollidePairHulls(const m3View* v, int32_t shapeA, int32_t shapeB)
+{
+    int32_t bodyA = v->shapeBody[shapeA];
+    int32_t bodyB = v->shapeBody[shapeB];
+    m3Transform xfA = m3ShapeXf(v, shapeA);
+    m3Transform xfB = m3ShapeXf(v, shapeB);
+    m3Quat qRel = m3MulQuat(m3Conj(xfA.q), xfB.q);
+    m3Vec3 dp = m3V3((m3real)(xfB.p.x - xfA.p.x), (m3real)(xfB.p.y - xfA.p.y),
+                     (m3real)(xfB.p.z - xfA.p.z));
+    m3Vec3 pRel = m3InvRotateVec3(xfA.q, dp);
+    m3Manifold fresh = m3CollideHulls(&v->hulls[v->hullIndex[shapeA]],
+                                      &v->hulls[v->hullIndex[shapeB]], qRel, pRel);
+    if (fresh.pointCount > 0)
+    {
+        fresh.normal = m3RotateVec3(xfA.q, fresh.normal);
+        for (int32_t k = 0; k < fresh.pointCount; ++k)
+        {
+            m3Vec3 rA = m3RotateVec3(xfA.q, fresh.points[k].anchorA);
+            m3Vec3 rB = m3RotateVec3(xfA.q, fresh.points[k].anchorB);
+            fresh.points[k].anchorA = m3FromCom(v, bodyA, xfA.p.x + (double)rA.x,
+                                                xfA.p.y + (double)rA.y, xfA.p.z + (double)rA.z);
+            fresh.points[k].anchorB = m3FromCom(v, bodyB, xfA.p.x + (double)rB.x,
+                                                xfA.p.y + (double)rB.y, xfA.p.z + (double)rB.z);
+        }
+    }
+    return fresh;
+}
+
+M3_HD m3Manifold m3CollidePlaneCapsule(const m3View* v, int32_t shapeA, int32_t planeShape,
+                                       int32_t capShape)
+{
+    m3Manifold fresh = m3ZeroManifold();
+    int32_t planeBody = v->shapeBody[planeShape];
+    int32_t capBody = v->shapeBody[capShape];
+    m3Transform xf = m3ShapeXf(v, capShape);
+    m3Vec3 n = v->geom[planeShape].v;
+    m3real offset = v->geom[planeShape].s;
+    m3real radius = v->geom[capShape].s;
+    m3Vec3 caps[2] = {v->geom[capShape].v, v->geom[capShape].v2};
+    int32_t count = 0;
+    for (int32_t k = 0; k < 2; ++k)
+    {
+        m3Vec3 r = m3RotateVec3(xf.q, caps[k]);
+        double wx = xf.p.x + (double)r.x;
+        double wy = xf.p.y + (double)r.y;
+        double wz = xf.p.z + (double)r.z;
+        double centerDist = (double)n.x * wx + (double)n.y * wy + (double)n.z * wz - (double)offset;
+        m3real sep = (m3real)centerDist - radius;
+        if (sep > M3_SPECULATIVE_DISTANCE)
+        {
+            continue;
+        }
+        double px = wx - (double)n.x * centerDist;
+        double py = wy - (double)n.y * centerDist;
+        double pz = wz - (double)n.z * centerDist;
+        fresh.points[count].anchorA = m3FromCom(v, planeBody, px, py, pz);
+        fresh.points[count].anchorB =
+            m3FromCom(v, capBody, wx - (double)(n.x * radius), wy - (double)(n.y * radius),
+                      wz - (double)(n.z * radius));
+        fresh.points[count].separation = sep;
+        fresh.points[count].id = (uint16_t)k;
+        count += 1;
+    }
+    fresh.normal = n;
+    fresh.pointCount = count;
+    if (count > 0 && planeShape != shapeA)
+    {
+        m3SwapAnchors(&fresh, count);
+    }
+    return fresh;
+}
+
+M3_HD m3Manifold m3CollideCapsuleHull(const m3View* v, int32_t shapeA, int32_t hullShape,
+                                      int32_t capShape)
+{
+    m3Manifold fresh = m3ZeroManifold();
+    int32_t hullBody = v->shapeBody[hullShape];
+    int32_t capBody = v->shapeBody[capShape];
+    m3Transform xfH = m3ShapeXf(v, hullShape);
+    m3Transform xfC = m3ShapeXf(v, capShape);
+    m3Quat qRel = m3MulQuat(m3Conj(xfH.q), xfC.q);
+    m3Vec3 dp = m3V3((m3real)(xfC.p.x - xfH.p.x), (m3real)(xfC.p.y - xfH.p.y),
+                     (m3real)(xfC.p.z - xfH.p.z));
+    m3Vec3 pRel = m3InvRotateVec3(xfH.q, dp);
+    m3Vec3 s1 = m3Add3(m3RotateVec3(qRel, v->geom[capShape].v), pRel);
+    m3Vec3 s2 = m3Add3(m3RotateVec3(qRel, v->geom[capShape].v2), pRel);
+    m3Manifold local =
+        m3CollideSegmentHull(&v->hulls[v->hullIndex[hullShape]], s1, s2, v->geom[capShape].s);
+    if (local.pointCount > 0)
+    {
+        m3Vec3 nWorld = m3RotateVec3(xfH.q, local.normal);
+        fresh.normal = hullShape == shapeA ? nWorld : m3Neg3(nWorld);
+        fresh.pointCount = local.pointCount;
+        for (int32_t k = 0; k < local.pointCount; ++k)
+        {
+            m3Vec3 rH = m3RotateVec3(xfH.q, local.points[k].anchorA);
+            m3Vec3 rC = m3RotateVec3(xfH.q, local.points[k].anchorB);
+            m3Vec3 aHull = m3FromCom(v, hullBody, xfH.p.x + (double)rH.x, xfH.p.y + (double)rH.y,
+                                     xfH.p.z + (double)rH.z);
+            m3Vec3 aCap = m3FromCom(v, capBody, xfH.p.x + (double)rC.x, xfH.p.y + (double)rC.y,
+                                    xfH.p.z + (double)rC.z);
+            fresh.points[k].anchorA = hullShape == shapeA ? aHull : aCap;
+            fresh.points[k].anchorB = hullShape == shapeA ? aCap : aHull;
+            fresh.points[k].separation = local.points[k].separation;
+            fresh.points[k].id = local.points[k].id;
+        }
+    }
+    return fresh;
+}
+
+// Hull-sphere, capsule-sphere and capsule-capsule: GJK on the cores, radii
+// analytic, one point (src/manifold.c:2666-2808).
+M3_HD m3Manifold m3CollideGjkPair(const m3View* v, int32_t shapeA, int32_t shapeB, int hullPair)
+{
+    m3Manifold fresh = m3ZeroManifold();
+    uint8_t typeA = v->shapeType[shapeA];
+    int32_t bodyA = v->shapeBody[shapeA];
+    int32_t bodyB = v->shapeBody[shapeB];
+    m3Transform xfA = m3ShapeXf(v, shapeA);
+    m3Transform xfB = m3ShapeXf(v, shapeB);
+    m3Quat q = m3MulQuat(m3Conj(xfA.q), xfB.q);
+    m3Vec3 dp = m3V3((m3real)(xfB.p.x - xfA.p.x), (m3real)(xfB.p.y - xfA.p.y),
+                     (m3real)(xfB.p.z - xfA.p.z));
+    m3Vec3 p = m3InvRotateVec3(xfA.q, dp);
+    m3Vec3 pointsA[2];
+    m3Vec3 pointsB[2];
+    m3Proxy proxyA = m3MakeProxy(v, shapeA, pointsA);
+    m3Proxy proxyB = m3MakeProxy(v, shapeB, pointsB);
+    m3GjkOut out = m3ShapeDistance(&proxyA, &proxyB, q, p);
+    m3real rA = proxyA.radius;
+    m3real rB = proxyB.radius;
+    m3real sep = out.distance - rA - rB;
+    if (!(sep <= M3_SPECULATIVE_DISTANCE))
+    {
+        return fresh;
+    }
+    m3Vec3 nLocal;
+    m3Vec3 pALocal;
+    m3Vec3 pBLocal;
+    if (out.distance > 0.0f)
+    {
+        nLocal = out.normal;
+        pALocal = m3Add3(out.pointA, m3MulSV3(rA, nLocal));
+        pBLocal = m3Sub3(out.pointB, m3MulSV3(rB, nLocal));
+    }
+    else if (hullPair)
+    {
+        int hullIsA = typeA == M3_HULL;
+        const m3HullData* hull = &v->hulls[v->hullIndex[hullIsA ? shapeA : shapeB]];
+        const m3Proxy* round = hullIsA ? &proxyB : &proxyA;
+        m3Vec3 core = hullIsA ? m3Add3(m3RotateVec3(q, round->points[0]), p)
+                              : m3InvRotateVec3(q, m3Sub3(round->points[0], p));
+        m3Vec3 nHull;
+        m3real coreSep;
+        m3Vec3 onHull;
+        m3DeepPointInHull(hull, core, &nHull, &coreSep, &onHull);
+        m3real rRound = round->radius;
+        sep = coreSep - rRound;
+        if (hullIsA)
+        {
+            nLocal = nHull;
+            pALocal = onHull;
+            pBLocal = m3Sub3(core, m3MulSV3(rRound, nHull));
+        }
+        else
+        {
+            m3Vec3 nA = m3RotateVec3(q, nHull);
+            nLocal = m3Neg3(nA);
+            pALocal = m3Sub3(m3Add3(m3RotateVec3(q, core), p), m3MulSV3(rRound, nA));
+            pBLocal = m3Add3(m3RotateVec3(q, onHull), p);
+        }
+    }
+    else
+    {
+        m3Vec3 axis;
+        m3Vec3 cA;
+        m3Vec3 cB;
+        if (proxyA.count == 2 && proxyB.count == 2)
+        {
+            m3Vec3 dirA = m3Sub3(pointsA[1], pointsA[0]);
+            m3Vec3 dirB = m3RotateVec3(q, m3Sub3(pointsB[1], pointsB[0]));
+            axis = m3Cross3(dirA, dirB);
+            cA = m3MulSV3(0.5f, m3Add3(pointsA[0], pointsA[1]));
+            cB = m3Add3(m3RotateVec3(q, m3MulSV3(0.5f, m3Add3(pointsB[0], pointsB[1]))), p);
+            if (m3Dot3(axis, axis) < 1.0e-10f)
+            {
+                axis = m3Sub3(cB, cA);
+            }
+        }
+        else
+        {
+            m3Vec3 dir = proxyA.count == 2 ? m3Sub3(pointsA[1], pointsA[0])
+                                           : m3RotateVec3(q, m3Sub3(pointsB[1], pointsB[0]));
+            m3Vec3 t1;
+            m3Vec3 t2;
+            m3TangentBasis(m3Normalize3(dir), &t1, &t2);
+            axis = t1;
+            cA = proxyA.count == 1 ? pointsA[0] : m3MulSV3(0.5f, m3Add3(pointsA[0], pointsA[1]));
+            cB = proxyB.count == 1
+                     ? m3Add3(m3RotateVec3(q, pointsB[0]), p)
+                     : m3Add3(m3RotateVec3(q, m3MulSV3(0.5f, m3Add3(pointsB[0], pointsB[1]))), p);
+        }
+        nLocal = m3Normalize3(axis);
+        if (m3Dot3(nLocal, m3Sub3(cB, cA)) < 0.0f)
+        {
+            nLocal = m3Neg3(nLocal);
+        }
+        m3Vec3 mid = m3MulSV3(0.5f, m3Add3(cA, cB));
+        pALocal = mid;
+        pBLocal = mid;
+        sep = -(rA + rB);
+    }
+    fresh.normal = m3RotateVec3(xfA.q, nLocal);
+    m3Vec3 wA = m3RotateVec3(xfA.q, pALocal);
+    m3Vec3 wB = m3RotateVec3(xfA.q, pBLocal);
+    fresh.points[0].anchorA =
+        m3FromCom(v, bodyA, xfA.p.x + (double)wA.x, xfA.p.y + (double)wA.y, xfA.p.z + (double)wA.z);
+    fresh.points[0].anchorB =
+        m3FromCom(v, bodyB, xfA.p.x + (double)wB.x, xfA.p.y + (double)wB.y, xfA.p.z + (double)wB.z);
+    fresh.points[0].separation = sep;
+    fresh.points[0].id = 0;
+    fresh.pointCount = 1;
+    return fresh;
+}
+
+M3_HD void m3SphereCenter(const m3View* v, int32_t s, double* cx, double* cy, double* cz)
+{
+    m3Transform xf = m3ShapeXf(v, s);
+    m3Vec3 r = m3RotateVec3(xf.q, v->geom[s].v);
+    *cx = xf.p.x + (double)r.x;
+    *cy = xf.p.y + (double)r.y;
+    *cz = xf.p.z + (double)r.z;
+}
+
+M3_HD m3Manifold m3CollidePlaneSpherePair(const m3View* v, int32_t shapeA, int32_t shapeB)
+{
+    uint8_t typeA = v->shapeType[shapeA];
+    int32_t planeShape = typeA == M3_PLANE ? shapeA : shapeB;
+    int32_t sphereShape = planeShape == shapeA ? shapeB : shapeA;
+    m3Vec3 n = v->geom[planeShape].v;
+    m3real offset = v->geom[planeShape].s;
+    double cx, cy, cz;
+    m3SphereCenter(v, sphereShape, &cx, &cy, &cz);
+    double distD = (double)n.x * cx + (double)n.y * cy + (double)n.z * cz - (double)offset;
+    m3Manifold fresh = m3CollidePlaneSphere(n, (m3real)distD, v->geom[sphereShape].s);
+    if (fresh.pointCount > 0)
+    {
+        int32_t planeBody = v->shapeBody[planeShape];
+        int32_t ballBody = v->shapeBody[sphereShape];
+        double px = cx - (double)(fresh.normal.x * (m3real)distD);
+        double py = cy - (double)(fresh.normal.y * (m3real)distD);
+        double pz = cz - (double)(fresh.normal.z * (m3real)distD);
+        fresh.points[0].anchorA = m3FromCom(v, planeBody, px, py, pz);
+        fresh.points[0].anchorB =
+            m3Add3(m3FromCom(v, ballBody, cx, cy, cz), fresh.points[0].anchorB);
+        if (planeShape != shapeA)
+        {
+            m3Vec3 tmp = fresh.points[0].anchorA;
+            fresh.points[0].anchorA = fresh.points[0].anchorB;
+            fresh.points[0].anchorB = tmp;
+            fresh.normal = m3Neg3(fresh.normal);
+        }
+    }
+    return fresh;
+}
+
+M3_HD m3Manifold m3CollideSpherePair(const m3View* v, int32_t shapeA, int32_t shapeB)
+{
+    double ax, ay, az, bx, by, bz;
+    m3SphereCenter(v, shapeA, &ax, &ay, &az);
+    m3SphereCenter(v, shapeB, &bx, &by, &bz);
+    m3Vec3 d = m3V3((m3real)(bx - ax), (m3real)(by - ay), (m3real)(bz - az));
+    m3Manifold fresh = m3CollideSpheres(d, v->geom[shapeA].s, v->geom[shapeB].s);
+    if (fresh.pointCount > 0)
+    {
+        fresh.points[0].anchorA =
+            m3Add3(m3FromCom(v, v->shapeBody[shapeA], ax, ay, az), fresh.points[0].anchorA);
+        fresh.points[0].anchorB =
+            m3Add3(m3FromCom(v, v->shapeBody[shapeB], bx, by, bz), fresh.points[0].anchorB);
+    }
+    return fresh;
+}
+
+// One pair: fresh manifold by shape family, then the warm-start carry by
+// feature id from the previous step's sorted list.
+M3_HD void m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
+{
+    uint64_t key = v->keys[i];
+    int32_t shapeA = (int32_t)(key >> 32);
+    int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+    uint8_t typeA = v->shapeType[shapeA];
+    uint8_t typeB = v->shapeType[shapeB];
+    int planePair = typeA == M3_PLANE || typeB == M3_PLANE;
+    int hullPair = typeA == M3_HULL || typeB == M3_HULL;
+    int capsulePair = typeA == M3_CAPSULE || typeB == M3_CAPSULE;
+    m3Manifold fresh;
+    if (planePair && hullPair)
+    {
+        int32_t planeShape = typeA == M3_PLANE ? shapeA : shapeB;
+        int32_t hullShape = planeShape == shapeA ? shapeB : shapeA;
+        fresh = m3CollidePlaneHull(v, shapeA, planeShape, hullShape);
+    }
+    else if (typeA == M3_HULL && typeB == M3_HULL)
+    {
+        fresh = m3CollidePairHulls(v, shapeA, shapeB);
+    }
+    else if (planePair && capsulePair)
+    {
+        int32_t planeShape = typeA == M3_PLANE ? shapeA : shapeB;
+        int32_t capShape = planeShape == shapeA ? shapeB : shapeA;
+        fresh = m3CollidePlaneCapsule(v, shapeA, planeShape, capShape);
+    }
+    else if (hullPair && capsulePair)
+    {
+        int32_t hullShape = typeA == M3_HULL ? shapeA : shapeB;
+        int32_t capShape = hullShape == shapeA ? shapeB : shapeA;
+        fresh = m3CollideCapsuleHull(v, shapeA, hullShape, capShape);
+    }
+    else if (hullPair || capsulePair)
+    {
+        fresh = m3CollideGjkPair(v, shapeA, shapeB, hullPair);
+    }
+    else if (planePair)
+    {
+        fresh = m3CollidePlaneSpherePair(v, shapeA, shapeB);
+    }
+    else
+    {
+        fresh = m3CollideSpherePair(v, shapeA, shapeB);
+    }
+
+    if (fresh.pointCount > 0 && oldCount > 0)
+    {
+        int32_t lo = 0;
+        int32_t hi = oldCount - 1;
+        while (lo <= hi)
+        {
+            int32_t mid = (lo + hi) / 2;
+            uint64_t k = v->oldKeys[mid];
+            if (k == key)
+            {
+                const m3Manifold* prev = &v->oldManifolds[mid];
+                fresh.frictionImpulse = prev->frictionImpulse;
+                fresh.twistImpulse = prev->twistImpulse;
+                fresh.rollingImpulse = prev->rollingImpulse;
+                int32_t prevCount = prev->pointCount;
+                for (int32_t k2 = 0; k2 < fresh.pointCount; ++k2)
+                {
+                    for (int32_t o = 0; o < prevCount; ++o)
+                    {
+                        if (prev->points[o].id == fresh.points[k2].id)
+                        {
+                            fresh.points[k2].normalImpulse = prev->points[o].normalImpulse;
+                            fresh.points[k2].flags |= 1;
+                            break;
+                        }
+                    }
+                }
+                break;
+            }
+            if (k < key)
+            {
+                lo = mid + 1;
+            }
+            else
+            {
+                hi = mid - 1;
+            }
+        }
+    }
+    v->manifolds[i] = fresh;
+}
+
+// ---------------------------------------------------------------------------
+// Contact solver (src/solver.c:30-600)
+// ---------------------------------------------------------------------------
+
+M3_HD m3Softness m3MakeSoft(m3real hertz, m3real zeta, m3real h)
+{
+    m3Softness s;
+    if (hertz == 0.0f)
+    {
+        s.biasRate = 0.0f;
+        s.massScale = 0.0f;
+        s.impulseScale = 0.0f;
+        return s;
+    }
+    m3real omega = 2.0f * M3_PI * hertz;
+    m3real a1 = 2.0f * zeta + h * omega;
+    m3real a2 = h * omega * a1;
+    m3real a3 = 1.0f / (1.0f + a2);
+    s.biasRate = omega / a1;
+    s.massScale = a2 * a3;
+    s.impulseScale = a3;
+    return s;
+}
+
+// I_w^-1 = R I_l^-1 R^T built column by column on the world basis.
+M3_HD m3Mat3 m3WorldInvInertia(const m3View* v, int32_t body)
+{
+    if (v->type[body] != M3_DYNAMIC)
+    {
+        return m3ZeroMat3();
+    }
+    m3Quat q = v->xf[body].q;
+    m3Mat3 il = v->invI[body];
+    m3Mat3 r;
+    r.cx = m3RotateVec3(q, m3MulMV3(il, m3InvRotateVec3(q, m3V3(1.0f, 0.0f, 0.0f))));
+    r.cy = m3RotateVec3(q, m3MulMV3(il, m3InvRotateVec3(q, m3V3(0.0f, 1.0f, 0.0f))));
+    r.cz = m3RotateVec3(q, m3MulMV3(il, m3InvRotateVec3(q, m3V3(0.0f, 0.0f, 1.0f))));
+    return r;
+}
+
+M3_HD m3real m3EffectiveMass(const m3Constraint* c, m3Vec3 rA, m3Vec3 rB, m3Vec3 dir)
+{
+    m3Vec3 arm1 = m3Cross3(rA, dir);
+    m3Vec3 arm2 = m3Cross3(rB, dir);
+    m3real k = c->invMassA + c->invMassB + m3Dot3(arm1, m3MulMV3(c->invIA, arm1)) +
+               m3Dot3(arm2, m3MulMV3(c->invIB, arm2));
+    return k > 0.0f ? 1.0f / k : 0.0f;
+}
+
+M3_HD m3Vec3 m3VelocityAt(const m3View* v, int32_t body, m3Vec3 arm)
+{
+    return m3Add3(v->lv[body], m3Cross3(v->av[body], arm));
+}
+
+// Does pair i become a constraint this step (src/solver.c:156-186)?
+M3_HD int m3PairIsConstraint(const m3View* v, int32_t i)
+{
+    const m3Manifold* m = &v->manifolds[i];
+    if (m->pointCount == 0)
+    {
+        return 0;
+    }
+    uint64_t key = v->keys[i];
+    int32_t shapeA = (int32_t)(key >> 32);
+    int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+    int32_t bodyA = v->shapeBody[shapeA];
+    int32_t bodyB = v->shapeBody[shapeB];
+    int awakeDynA = v->type[bodyA] == M3_DYNAMIC && v->awake[bodyA] != 0;
+    int awakeDynB = v->type[bodyB] == M3_DYNAMIC && v->awake[bodyB] != 0;
+    if (!awakeDynA && !awakeDynB)
+    {
+        return 0;
+    }
+    if (v->sensor[shapeA] != 0 || v->sensor[shapeB] != 0)
+    {
+        return 0;
+    }
+    return 1;
+}
+
+// src/solver.c:249-403 for one constraint.
+M3_HD void m3PrepareContact(const m3View* v, int32_t pairIndex, m3Constraint* c, m3Softness soft,
+                            m3Softness staticSoft)
+{
+    const m3Manifold* manifold = &v->manifolds[pairIndex];
+    uint64_t key = v->keys[pairIndex];
+    int32_t shapeA = (int32_t)(key >> 32);
+    int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+    int32_t bodyA = v->shapeBody[shapeA];
+    int32_t bodyB = v->shapeBody[shapeB];
+    memset(c, 0, sizeof(*c));
+    c->bodyA = bodyA;
+    c->bodyB = bodyB;
+    c->manifoldIndex = pairIndex;
+    c->pointCount = manifold->pointCount;
+    c->normal = manifold->normal;
+    m3TangentBasis(c->normal, &c->t1, &c->t2);
+    c->dynA = v->type[bodyA] == M3_DYNAMIC ? 1 : 0;
+    c->dynB = v->type[bodyB] == M3_DYNAMIC ? 1 : 0;
+    c->invMassA = c->dynA ? v->invMass[bodyA] : 0.0f;
+    c->invMassB = c->dynB ? v->invMass[bodyB] : 0.0f;
+    c->invIA = m3WorldInvInertia(v, bodyA);
+    c->invIB = m3WorldInvInertia(v, bodyB);
+    float fricA = v->friction[shapeA];
+    float fricB = v->friction[shapeB];
+    float restA = v->restitution[shapeA];
+    float restB = v->restitution[shapeB];
+    float rollA = v->rolling[shapeA];
+    float rollB = v->rolling[shapeB];
+    m3Vec3 surfA = v->surfaceVel[shapeA];
+    m3Vec3 surfB = v->surfaceVel[shapeB];
+    c->friction = sqrtf(fricA * fricB);
+    c->restitution = m3MaxF(restA, restB);
+    c->rollingResistance = m3MaxF(rollA, rollB) * m3MaxF(v->maxExt[bodyA], v->maxExt[bodyB]);
+    c->rollingImpulse = manifold->rollingImpulse;
+    if (c->rollingResistance > 0.0f)
+    {
+        m3Mat3 sum = c->invIA;
+        sum.cx = m3Add3(sum.cx, c->invIB.cx);
+        sum.cy = m3Add3(sum.cy, c->invIB.cy);
+        sum.cz = m3Add3(sum.cz, c->invIB.cz);
+        c->rollingK = sum;
+    }
+    c->softness = (c->invMassA == 0.0f || c->invMassB == 0.0f) ? staticSoft : soft;
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        m3CPoint* cp = &c->points[k];
+        cp->rA = manifold->points[k].anchorA;
+        cp->rB = manifold->points[k].anchorB;
+        cp->baseSeparation = manifold->points[k].separation -
+                             (m3Dot3(cp->rB, c->normal) - m3Dot3(cp->rA, c->normal));
+        cp->normalMass = m3EffectiveMass(c, cp->rA, cp->rB, c->normal);
+        cp->relativeVelocity =
+            m3Dot3(m3Sub3(m3VelocityAt(v, bodyB, cp->rB), m3VelocityAt(v, bodyA, cp->rA)),
+                   c->normal);
+        cp->normalImpulse = manifold->points[k].normalImpulse;
+        cp->totalNormalImpulse = 0.0f;
+    }
+    m3Vec3 originA = m3V3(0.0f, 0.0f, 0.0f);
+    m3Vec3 originB = m3V3(0.0f, 0.0f, 0.0f);
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        originA = m3Add3(originA, c->points[k].rA);
+        originB = m3Add3(originB, c->points[k].rB);
+    }
+    m3real invCount = 1.0f / (m3real)c->pointCount;
+    c->originA = m3MulSV3(invCount, originA);
+    c->originB = m3MulSV3(invCount, originB);
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        c->points[k].leverArm = m3Length3(m3Sub3(c->points[k].rA, c->originA));
+    }
+    m3Vec3 rtA1 = m3Cross3(c->originA, c->t1);
+    m3Vec3 rtA2 = m3Cross3(c->originA, c->t2);
+    m3Vec3 rtB1 = m3Cross3(c->originB, c->t1);
+    m3Vec3 rtB2 = m3Cross3(c->originB, c->t2);
+    m3real k11 = c->invMassA + c->invMassB + m3Dot3(rtA1, m3MulMV3(c->invIA, rtA1)) +
+                 m3Dot3(rtB1, m3MulMV3(c->invIB, rtB1));
+    m3real k22 = c->invMassA + c->invMassB + m3Dot3(rtA2, m3MulMV3(c->invIA, rtA2)) +
+                 m3Dot3(rtB2, m3MulMV3(c->invIB, rtB2));
+    m3real k12 = m3Dot3(rtA1, m3MulMV3(c->invIA, rtA2)) + m3Dot3(rtB1, m3MulMV3(c->invIB, rtB2));
+    m3real det = k11 * k22 - k12 * k12;
+    if (det > 0.0f)
+    {
+        m3real invDet = 1.0f / det;
+        c->frictionK11 = k22 * invDet;
+        c->frictionK22 = k11 * invDet;
+        c->frictionK12 = -k12 * invDet;
+    }
+    else
+    {
+        c->frictionK11 = 0.0f;
+        c->frictionK22 = 0.0f;
+        c->frictionK12 = 0.0f;
+    }
+    c->frictionImpulse1 = m3Dot3(manifold->frictionImpulse, c->t1);
+    c->frictionImpulse2 = m3Dot3(manifold->frictionImpulse, c->t2);
+    m3Vec3 surf = m3Sub3(surfA, surfB);
+    c->tangentVelocity1 = m3Dot3(surf, c->t1);
+    c->tangentVelocity2 = m3Dot3(surf, c->t2);
+    m3Vec3 iSumN = m3Add3(m3MulMV3(c->invIA, c->normal), m3MulMV3(c->invIB, c->normal));
+    m3real twistK = m3Dot3(c->normal, iSumN);
+    c->twistMass = twistK > 0.0f ? 1.0f / twistK : 0.0f;
+    c->twistImpulse = manifold->twistImpulse;
+}
+
+M3_HD void m3ApplyImpulse(const m3View* v, const m3Constraint* c, m3Vec3 impulse, m3Vec3 rsA,
+                          m3Vec3 rsB)
+{
+    if (c->dynA)
+    {
+        int32_t a = c->bodyA;
+        v->lv[a] = m3Sub3(v->lv[a], m3MulSV3(c->invMassA, impulse));
+        v->av[a] = m3Sub3(v->av[a], m3MulMV3(c->invIA, m3Cross3(rsA, impulse)));
+    }
+    if (c->dynB)
+    {
+        int32_t b = c->bodyB;
+        v->lv[b] = m3Add3(v->lv[b], m3MulSV3(c->invMassB, impulse));
+        v->av[b] = m3Add3(v->av[b], m3MulMV3(c->invIB, m3Cross3(rsB, impulse)));
+    }
+}
+
+// src/solver.c:408-429
+M3_HD void m3WarmStartOne(const m3View* v, m3Constraint* c)
+{
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        m3CPoint* cp = &c->points[k];
+        m3ApplyImpulse(v, c, m3MulSV3(cp->normalImpulse, c->normal), cp->rA, cp->rB);
+    }
+    m3Vec3 f = m3Add3(m3MulSV3(c->frictionImpulse1, c->t1), m3MulSV3(c->frictionImpulse2, c->t2));
+    m3ApplyImpulse(v, c, f, c->originA, c->originB);
+    m3Vec3 tw = m3Add3(m3MulSV3(c->twistImpulse, c->normal), c->rollingImpulse);
+    if (c->dynA)
+    {
+        v->av[c->bodyA] = m3Sub3(v->av[c->bodyA], m3MulMV3(c->invIA, tw));
+    }
+    if (c->dynB)
+    {
+        v->av[c->bodyB] = m3Add3(v->av[c->bodyB], m3MulMV3(c->invIB, tw));
+    }
+}
+
+// Cramer's rule; singular returns zero (src/solver.c:2909-2926).
+M3_HD m3Vec3 m3Solve3(const m3Mat3* J, m3Vec3 b)
+{
+    m3Vec3 cxy = m3Cross3(J->cy, J->cz);
+    m3real det = m3Dot3(J->cx, cxy);
+    if (det == 0.0f)
+    {
+        return m3V3(0.0f, 0.0f, 0.0f);
+    }
+    m3real inv = 1.0f / det;
+    m3Vec3 x;
+    x.x = inv * m3Dot3(b, cxy);
+    x.y = inv * m3Dot3(J->cx, m3Cross3(b, J->cz));
+    x.z = inv * m3Dot3(J->cx, m3Cross3(J->cy, b));
+    return x;
+}
+
+// src/solver.c:430-581
+M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real pushMax,
+                          int useBias)
+{
+    m3real passNormal = 0.0f;
+    m3real twistLimit = 0.0f;
+    m3Quat dqA = v->deltaRot[c->bodyA];
+    m3Quat dqB = v->deltaRot[c->bodyB];
+    m3Vec3 dpA = v->deltaPos[c->bodyA];
+    m3Vec3 dpB = v->deltaPos[c->bodyB];
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        m3CPoint* cp = &c->points[k];
+        m3Vec3 rsA = m3RotateVec3(dqA, cp->rA);
+        m3Vec3 rsB = m3RotateVec3(dqB, cp->rB);
+        m3Vec3 ds = m3Add3(m3Sub3(dpB, dpA), m3Sub3(rsB, rsA));
+        m3real s = cp->baseSeparation + m3Dot3(ds, c->normal);
+        m3real bias = 0.0f;
+        m3real massScale = 1.0f;
+        m3real impulseScale = 0.0f;
+        if (s > 0.0f)
+        {
+            bias = s * invH;
+        }
+        else if (useBias)
+        {
+            bias = m3MaxF(c->softness.biasRate * s, -pushMax);
+            massScale = c->softness.massScale;
+            impulseScale = c->softness.impulseScale;
+        }
+        m3real vn = m3Dot3(
+            m3Sub3(m3VelocityAt(v, c->bodyB, cp->rB), m3VelocityAt(v, c->bodyA, cp->rA)), c->normal);
+        m3real impulse = -cp->normalMass * massScale * (vn + bias) - impulseScale * cp->normalImpulse;
+        m3real newImpulse = m3MaxF(cp->normalImpulse + impulse, 0.0f);
+        impulse = newImpulse - cp->normalImpulse;
+        cp->normalImpulse = newImpulse;
+        cp->totalNormalImpulse += newImpulse;
+        passNormal += newImpulse;
+        twistLimit += cp->leverArm * newImpulse;
+        m3ApplyImpulse(v, c, m3MulSV3(impulse, c->normal), cp->rA, cp->rB);
+    }
+    if (useBias)
+    {
+        return;
+    }
+    {
+        m3Vec3 wRel = m3Sub3(v->av[c->bodyB], v->av[c->bodyA]);
+        m3real twistSpeed = m3Dot3(c->normal, wRel);
+        m3real maxTwist = c->friction * twistLimit;
+        m3real delta = -c->twistMass * twistSpeed;
+        m3real oldTwist = c->twistImpulse;
+        m3real newTwist = oldTwist + delta;
+        if (newTwist < -maxTwist)
+        {
+            newTwist = -maxTwist;
+        }
+        else if (newTwist > maxTwist)
+        {
+            newTwist = maxTwist;
+        }
+        delta = newTwist - oldTwist;
+        c->twistImpulse = newTwist;
+        m3Vec3 tw = m3MulSV3(delta, c->normal);
+        if (c->dynA)
+        {
+            v->av[c->bodyA] = m3Sub3(v->av[c->bodyA], m3MulMV3(c->invIA, tw));
+        }
+        if (c->dynB)
+        {
+            v->av[c->bodyB] = m3Add3(v->av[c->bodyB], m3MulMV3(c->invIB, tw));
+        }
+    }
+    if (c->rollingResistance > 0.0f)
+    {
+        m3Vec3 wRel = m3Sub3(v->av[c->bodyB], v->av[c->bodyA]);
+        m3Vec3 delta = m3MulSV3(-1.0f, m3Solve3(&c->rollingK, wRel));
+        m3Vec3 accum = m3Add3(c->rollingImpulse, delta);
+        m3real maxRoll = c->rollingResistance * passNormal;
+        m3real mag2 = m3Dot3(accum, accum);
+        if (mag2 > maxRoll * maxRoll && mag2 > 0.0f)
+        {
+            accum = m3MulSV3(maxRoll / sqrtf(mag2), accum);
+        }
+        delta = m3Sub3(accum, c->rollingImpulse);
+        c->rollingImpulse = accum;
+        // The reference writes both sides unconditionally here; a body
+        // without inverse inertia receives a zero change.
+        v->av[c->bodyA] = m3Sub3(v->av[c->bodyA], m3MulMV3(c->invIA, delta));
+        v->av[c->bodyB] = m3Add3(v->av[c->bodyB], m3MulMV3(c->invIB, delta));
+    }
+    {
+        m3Vec3 vrel = m3Sub3(m3VelocityAt(v, c->bodyB, c->originB),
+                             m3VelocityAt(v, c->bodyA, c->originA));
+        m3real vt1 = m3Dot3(vrel, c->t1) - c->tangentVelocity1;
+        m3real vt2 = m3Dot3(vrel, c->t2) - c->tangentVelocity2;
+        m3real f1 = c->frictionImpulse1 - (c->frictionK11 * vt1 + c->frictionK12 * vt2);
+        m3real f2 = c->frictionImpulse2 - (c->frictionK12 * vt1 + c->frictionK22 * vt2);
+        m3real maxFriction = c->friction * passNormal;
+        m3real mag2 = f1 * f1 + f2 * f2;
+        if (mag2 > maxFriction * maxFriction)
+        {
+            m3real mag = sqrtf(mag2);
+            m3real scale = mag > 0.0f ? maxFriction / mag : 0.0f;
+            f1 *= scale;
+            f2 *= scale;
+        }
+        m3real d1 = f1 - c->frictionImpulse1;
+        m3real d2 = f2 - c->frictionImpulse2;
+        c->frictionImpulse1 = f1;
+        c->frictionImpulse2 = f2;
+        m3ApplyImpulse(v, c, m3Add3(m3MulSV3(d1, c->t1), m3MulSV3(d2, c->t2)), c->originA,
+                       c->originB);
+    }
+}
+
+// src/solver.c:2879-2907 for one constraint.
+M3_HD void m3RestitutionOne(const m3View* v, m3Constraint* c, m3real threshold)
+{
+    if (c->restitution == 0.0f)
+    {
+        return;
+    }
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        m3CPoint* cp = &c->points[k];
+        if (cp->relativeVelocity > -threshold || cp->totalNormalImpulse == 0.0f)
+        {
+            continue;
+        }
+        m3real vn = m3Dot3(
+            m3Sub3(m3VelocityAt(v, c->bodyB, cp->rB), m3VelocityAt(v, c->bodyA, cp->rA)), c->normal);
+        m3real impulse = -cp->normalMass * (vn + c->restitution * cp->relativeVelocity);
+        m3real newImpulse = m3MaxF(cp->normalImpulse + impulse, 0.0f);
+        impulse = newImpulse - cp->normalImpulse;
+        cp->normalImpulse = newImpulse;
+        m3ApplyImpulse(v, c, m3MulSV3(impulse, c->normal), cp->rA, cp->rB);
+    }
+}
+
+// src/solver.c:2979-3048 without the observer hit events.
+M3_HD void m3StoreOne(const m3View* v, const m3Constraint* c)
+{
+    m3Manifold* m = &v->manifolds[c->manifoldIndex];
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        m->points[k].normalImpulse = c->points[k].normalImpulse;
+    }
+    m->frictionImpulse =
+        m3Add3(m3MulSV3(c->frictionImpulse1, c->t1), m3MulSV3(c->frictionImpulse2, c->t2));
+    m->twistImpulse = c->twistImpulse;
+    m->rollingImpulse = c->rollingImpulse;
+}
+
+// ---------------------------------------------------------------------------
+// Integration (src/solver.c:2927-2977, 4087-4199)
+// ---------------------------------------------------------------------------
+
+// One implicit Newton step of the gyroscopic term in body coordinates;
+// exactly isotropic tensors skip it.
+M3_HD m3Vec3 m3GyroscopicOmega(const m3View* v, int32_t body, m3Vec3 w, m3real h)
+{
+    const m3Mat3 inertia = v->inertia[body];
+    const m3real i00 = inertia.cx.x;
+    const m3real i01 = inertia.cy.x;
+    const m3real i02 = inertia.cz.x;
+    const m3real i11 = inertia.cy.y;
+    const m3real i12 = inertia.cz.y;
+    const m3real i22 = inertia.cz.z;
+    if (i01 == 0.0f && i02 == 0.0f && i12 == 0.0f && i00 == i11 && i11 == i22)
+    {
+        return w;
+    }
+    m3Quat q = v->xf[body].q;
+    m3Vec3 omega2 = m3InvRotateVec3(q, w);
+    const m3real w1 = omega2.x;
+    const m3real w2 = omega2.y;
+    const m3real w3 = omega2.z;
+    const m3real Iw1 = i00 * w1 + i01 * w2 + i02 * w3;
+    const m3real Iw2 = i01 * w1 + i11 * w2 + i12 * w3;
+    const m3real Iw3 = i02 * w1 + i12 * w2 + i22 * w3;
+    m3Vec3 b = m3V3(h * (w2 * Iw3 - w3 * Iw2), h * (w3 * Iw1 - w1 * Iw3), h * (w1 * Iw2 - w2 * Iw1));
+    m3Mat3 J;
+    J.cx = m3V3(i00 + h * (w2 * i02 - w3 * i01), i01 + h * (w3 * i00 - w1 * i02 - Iw3),
+                i02 + h * (w1 * i01 - w2 * i00 + Iw2));
+    J.cy = m3V3(i01 + h * (w2 * i12 - w3 * i11 + Iw3), i11 + h * (w3 * i01 - w1 * i12),
+                i12 + h * (w1 * i11 - w2 * i01 - Iw1));
+    J.cz = m3V3(i02 + h * (w2 * i22 - w3 * i12 - Iw2), i12 + h * (w3 * i02 - w1 * i22 + Iw1),
+                i22 + h * (w1 * i12 - w2 * i02));
+    omega2 = m3Sub3(omega2, m3Solve3(&J, b));
+    return m3RotateVec3(q, omega2);
+}
+
+M3_HD void m3IntegrateVelocity(const m3View* v, int32_t i, m3real h, m3Vec3 gravity,
+                               m3real maxLinear, m3real maxAngular)
+{
+    if (v->type[i] != M3_DYNAMIC)
+    {
+        return;
+    }
+    m3Vec3 lin = v->lv[i];
+    m3Vec3 ang = v->av[i];
+    lin = m3Add3(lin, m3MulSV3(h * v->gravityScale[i], gravity));
+    lin = m3Add3(lin, m3MulSV3(h * v->invMass[i], v->force[i]));
+    m3Vec3 tq = v->torque[i];
+    if (tq.x != 0.0f || tq.y != 0.0f || tq.z != 0.0f)
+    {
+        ang = m3Add3(ang, m3MulSV3(h, m3MulMV3(m3WorldInvInertia(v, i), tq)));
+    }
+    lin = m3MulSV3(1.0f / (1.0f + h * v->linDamp[i]), lin);
+    ang = m3MulSV3(1.0f / (1.0f + h * v->angDamp[i]), ang);
+    ang = m3GyroscopicOmega(v, i, ang, h);
+    m3real v2 = m3Dot3(lin, lin);
+    if (v2 > maxLinear * maxLinear)
+    {
+        lin = m3MulSV3(maxLinear / sqrtf(v2), lin);
+    }
+    if ((v->locks[i] & 0x40u) == 0)
+    {
+        m3real w2 = m3Dot3(ang, ang);
+        if (w2 > maxAngular * maxAngular)
+        {
+            ang = m3MulSV3(maxAngular / sqrtf(w2), ang);
+        }
+    }
+    v->lv[i] = lin;
+    v->av[i] = ang;
+}
+
+M3_HD void m3IntegratePosition(const m3View* v, int32_t i, m3real h)
+{
+    m3Vec3 lin = v->lv[i];
+    m3Vec3 ang = v->av[i];
+    uint8_t locks = v->locks[i];
+    if (locks != 0)
+    {
+        if (locks & 1u)
+            lin.x = 0.0f;
+        if (locks & 2u)
+            lin.y = 0.0f;
+        if (locks & 4u)
+            lin.z = 0.0f;
+        if (locks & 8u)
+            ang.x = 0.0f;
+        if (locks & 16u)
+            ang.y = 0.0f;
+        if (locks & 32u)
+            ang.z = 0.0f;
+        v->lv[i] = lin;
+        v->av[i] = ang;
+    }
+    m3Vec3 lc = v->lc[i];
+    m3Transform xf = v->xf[i];
+    m3Vec3 rlcOld = m3RotateVec3(xf.q, lc);
+    double cx = xf.p.x + (double)rlcOld.x + (double)(h * lin.x);
+    double cy = xf.p.y + (double)rlcOld.y + (double)(h * lin.y);
+    double cz = xf.p.z + (double)rlcOld.z + (double)(h * lin.z);
+    m3Vec3 dw = m3MulSV3(h, ang);
+    xf.q = m3IntegrateRotation(xf.q, dw);
+    m3Vec3 rlcNew = m3RotateVec3(xf.q, lc);
+    xf.p.x = cx - (double)rlcNew.x;
+    xf.p.y = cy - (double)rlcNew.y;
+    xf.p.z = cz - (double)rlcNew.z;
+    v->xf[i] = xf;
+    v->deltaPos[i] = m3Add3(v->deltaPos[i], m3MulSV3(h, lin));
+    v->deltaRot[i] = m3IntegrateRotation(v->deltaRot[i], dw);
+}
+
+// Kinematic servo: velocities that land the body on its target this step
+// (src/solver.c:3955-3981).
+M3_HD void m3KinematicServo(const m3View* v, int32_t i, m3real dt)
+{
+    m3real servoInvDt = 1.0f / dt;
+    const m3Transform now = v->xf[i];
+    const m3Transform want = v->target[i];
+    v->lv[i] = m3V3((m3real)(want.p.x - now.p.x) * servoInvDt, (m3real)(want.p.y - now.p.y) * servoInvDt,
+                    (m3real)(want.p.z - now.p.z) * servoInvDt);
+    m3Quat dq = m3MulQuat(want.q, m3Q(-now.q.x, -now.q.y, -now.q.z, now.q.w));
+    if (dq.w < 0.0f)
+    {
+        dq = m3Q(-dq.x, -dq.y, -dq.z, -dq.w);
+    }
+    v->av[i] = m3MulSV3(2.0f * servoInvDt, m3V3(dq.x, dq.y, dq.z));
+    v->hasTarget[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Continuous collision against planes (src/solver.c:3065-3079, 3280-3430;
+// sweep transform src/distance.c:712-732)
+// ---------------------------------------------------------------------------
+
+typedef struct m3Sweep
+{
+    m3Vec3 localCenter;
+    m3Vec3 c1;
+    m3Vec3 c2;
+    m3Quat q1;
+    m3Quat q2;
+} m3Sweep;
+
+M3_HD m3Transform m3SweepTransform(const m3Sweep* sweep, m3real time)
+{
+    m3Transform transform;
+    m3Quat q2 = sweep->q2;
+    if (sweep->q1.x * q2.x + sweep->q1.y * q2.y + sweep->q1.z * q2.z + sweep->q1.w * q2.w < 0.0f)
+    {
+        q2 = m3Q(-q2.x, -q2.y, -q2.z, -q2.w);
+    }
+    m3Quat q = m3Q(sweep->q1.x + time * (q2.x - sweep->q1.x), sweep->q1.y + time * (q2.y - sweep->q1.y),
+                   sweep->q1.z + time * (q2.z - sweep->q1.z), sweep->q1.w + time * (q2.w - sweep->q1.w));
+    transform.q = m3NormalizeQuat(q);
+    m3Vec3 c = m3Add3(sweep->c1, m3MulSV3(time, m3Sub3(sweep->c2, sweep->c1)));
+    m3Vec3 r = m3RotateVec3(transform.q, sweep->localCenter);
+    transform.p.x = (double)(c.x - r.x);
+    transform.p.y = (double)(c.y - r.y);
+    transform.p.z = (double)(c.z - r.z);
+    return transform;
+}
+
+M3_HD m3Sweep m3RelativeSweep(const m3View* v, int32_t body, m3Pos3 base)
+{
+    m3Sweep sweep;
+    sweep.localCenter = v->lc[body];
+    sweep.c1 = m3V3((m3real)(v->com0[body].x - base.x), (m3real)(v->com0[body].y - base.y),
+                    (m3real)(v->com0[body].z - base.z));
+    m3Vec3 rlc = m3RotateVec3(v->xf[body].q, v->lc[body]);
+    sweep.c2 = m3V3((m3real)(v->xf[body].p.x + (double)rlc.x - base.x),
+                    (m3real)(v->xf[body].p.y + (double)rlc.y - base.y),
+                    (m3real)(v->xf[body].p.z + (double)rlc.z - base.z));
+    sweep.q1 = v->rot0[body];
+    sweep.q2 = v->xf[body].q;
+    return sweep;
+}
+
+M3_HD m3real m3ContinuousVersusPlane(const m3View* v, int32_t fastBody, int32_t fastShape,
+                                     const m3Sweep* sweep, m3Pos3 base, int32_t planeShape,
+                                     m3real fraction)
+{
+    m3Vec3 n = v->geom[planeShape].v;
+    m3real offset = v->geom[planeShape].s -
+                    (m3real)((double)n.x * base.x + (double)n.y * base.y + (double)n.z * base.z);
+    m3Vec3 scratch[2];
+    m3Proxy proxy = m3MakeProxy(v, fastShape, scratch);
+    const m3real linearSlop = M3_LINEAR_SLOP;
+    m3real target = m3MaxF(linearSlop, proxy.radius - linearSlop);
+    m3real tolerance = 0.25f * linearSlop;
+    m3Vec3 travel = m3Sub3(sweep->c2, sweep->c1);
+    m3Quat dq = m3MulQuat(sweep->q2, m3Q(-sweep->q1.x, -sweep->q1.y, -sweep->q1.z, sweep->q1.w));
+    m3real arc = 2.0f * sqrtf(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[fastBody];
+    m3real rate = sqrtf(m3Dot3(travel, travel)) + arc;
+    if (!(rate > 0.0f))
+    {
+        return fraction;
+    }
+    m3real t = 0.0f;
+    for (int32_t iter = 0; iter < 25; ++iter)
+    {
+        m3Transform xf = m3SweepTransform(sweep, t);
+        m3real minD = 3.4e38f;
+        for (int32_t k = 0; k < proxy.count; ++k)
+        {
+            m3Vec3 r = m3RotateVec3(xf.q, proxy.points[k]);
+            m3real d = n.x * ((m3real)xf.p.x + r.x) + n.y * ((m3real)xf.p.y + r.y) +
+                       n.z * ((m3real)xf.p.z + r.z);
+            minD = m3MinF(minD, d);
+        }
+        m3real sep = minD - offset;
+        if (sep <= 0.0f)
+        {
+            return fraction;
+        }
+        if (sep <= target + tolerance)
+        {
+            if (t > 0.0f && t < fraction)
+            {
+                fraction = t;
+            }
+            return fraction;
+        }
+        t += (sep - target) / rate;
+        if (t >= fraction)
+        {
+            return fraction;
+        }
+    }
+    return fraction;
+}
+
+// The fast test and the plane sweeps for one body; returns 1 when the body
+// was pulled back.  Static non-plane targets are not supported on the
+// device path (none exist in this build's scenes): see the orchestrator.
+M3_HD int m3ContinuousBody(const m3View* v, int32_t i)
+{
+    if (v->bodyAlive[i] == 0 || v->type[i] != M3_DYNAMIC)
+    {
+        return 0;
+    }
+    m3Transform xfi = v->xf[i];
+    m3Vec3 rlc = m3RotateVec3(xfi.q, v->lc[i]);
+    double cx = xfi.p.x + (double)rlc.x;
+    double cy = xfi.p.y + (double)rlc.y;
+    double cz = xfi.p.z + (double)rlc.z;
+    m3Pos3 c0 = v->com0[i];
+    m3Vec3 dc = m3V3((m3real)(cx - c0.x), (m3real)(cy - c0.y), (m3real)(cz - c0.z));
+    m3Quat q0 = v->rot0[i];
+    m3Quat dq = m3MulQuat(xfi.q, m3Q(-q0.x, -q0.y, -q0.z, q0.w));
+    m3real arc = 2.0f * sqrtf(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[i];
+    m3real maxMotion = sqrtf(m3Dot3(dc, dc)) + arc;
+    if (!(maxMotion > 0.5f * v->minExt[i]))
+    {
+        return 0;
+    }
+    m3Pos3 base = c0;
+    m3real fraction = 1.0f;
+    m3Sweep sweep = m3RelativeSweep(v, i, base);
+    for (int32_t s = v->shapeHead[i]; s != -1; s = v->shapeNext[s])
+    {
+        if (v->sensor[s] != 0)
+        {
+            continue;
+        }
+        for (int32_t pi = 0; pi < v->planeCount; ++pi)
+        {
+            fraction = m3ContinuousVersusPlane(v, i, s, &sweep, base, v->planes[pi], fraction);
+        }
+    }
+    if (fraction < 1.0f)
+    {
+        m3Transform xf = m3SweepTransform(&sweep, fraction);
+        v->xf[i].q = xf.q;
+        v->xf[i].p.x = base.x + xf.p.x;
+        v->xf[i].p.y = base.y + xf.p.y;
+        v->xf[i].p.z = base.z + xf.p.z;
+        return 1;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Sleep (src/solver.c:3582-3625): the per-body timer update.
+// ---------------------------------------------------------------------------
+
+M3_HD void m3SleepTimer(const m3View* v, int32_t i, m3real dt)
+{
+    m3real invDt = dt > 0.0f ? 1.0f / dt : 0.0f;
+    m3Vec3 lin = v->lv[i];
+    m3Vec3 ang = v->av[i];
+    m3real velocity = sqrtf(m3Dot3(lin, lin)) + sqrtf(m3Dot3(ang, ang)) * v->maxExt[i];
+    m3Transform xf = v->xf[i];
+    m3Vec3 rlc = m3RotateVec3(xf.q, v->lc[i]);
+    m3Pos3 c0 = v->com0[i];
+    m3Vec3 dc = m3V3((m3real)(xf.p.x + (double)rlc.x - c0.x), (m3real)(xf.p.y + (double)rlc.y - c0.y),
+                     (m3real)(xf.p.z + (double)rlc.z - c0.z));
+    m3Quat q0 = v->rot0[i];
+    m3Quat dq = m3MulQuat(xf.q, m3Q(-q0.x, -q0.y, -q0.z, q0.w));
+    m3real motion = sqrtf(m3Dot3(dc, dc)) +
+                    2.0f * sqrtf(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[i];
+    m3real sleepVelocity = m3MaxF(velocity, 0.5f * invDt * motion);
+    if (v->canSleep[i] == 0)
+    {
+        sleepVelocity = 3.4e38f;
+    }
+    if (sleepVelocity < v->sleepThreshold[i])
+    {
+        v->sleepTime[i] += dt;
+    }
+    else
+    {
+        v->sleepTime[i] = 0.0f;
+    }
+}
+
+#endif // M3_STEP_H

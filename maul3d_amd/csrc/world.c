@@ -1,0 +1,1979 @@
+// The host world of the MI355X build: id pools, world registry, creation
+// with closed-form mass properties and canonical box hulls, the curated
+// world hash, snapshot/restore, and the hand-off of every step to the
+// device layer (include/m3gpu.h).  C17, compiled under the reference FP law
+// (-ffp-contract=off, no fast-math) so creation-time floats are the
+// reference's bits.
+//
+// Host state is authoritative for everything the step never writes (mass,
+// shapes, hulls, filters).  The device is authoritative for what the step
+// writes (transforms, velocities, sleep state, pairs, manifolds); the host
+// copy is refreshed lazily on the first read after a step.
+
+#include "m3_core.h"
+
+#include "maul3d_amd.h"
+#include "m3gpu.h"
+
+#include <stddef.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+// Larger than the reference's 64-slot table (src/world_internal.h:37) so one
+// process can hold a 256-world rollback batch.
+#define M3_MAX_WORLDS 1024
+
+#define M3_COOKIE 0x4D33u
+#define M3_WORLD_COOKIE ((int32_t)(M3_COOKIE ^ ((int32_t)sizeof(m3WorldDef) << 8)))
+#define M3_BODY_COOKIE ((int32_t)(M3_COOKIE ^ ((int32_t)sizeof(m3BodyDef) << 8) ^ 1))
+#define M3_SHAPE_COOKIE ((int32_t)(M3_COOKIE ^ ((int32_t)sizeof(m3ShapeDef) << 8) ^ 2))
+
+// Tuning defaults (src/world_internal.h:1024-1046).
+#define M3_CONTACT_HERTZ_DEFAULT 30.0f
+#define M3_CONTACT_DAMPING_RATIO_DEFAULT 10.0f
+#define M3_CONTACT_PUSH_MAX_SPEED_DEFAULT 3.0f
+#define M3_RESTITUTION_THRESHOLD_DEFAULT 1.0f
+#define M3_MAX_LINEAR_SPEED_DEFAULT 400.0f
+#define M3_MAX_ANGULAR_SPEED_DEFAULT 800.0f
+#define M3_HIT_EVENT_THRESHOLD_DEFAULT 1.0f
+#define M3_SLEEP_VELOCITY_DEFAULT 0.05f
+
+// ---------------------------------------------------------------------------
+// FIFO id pools (src/allocator.c:178-218): recycled slots first in free
+// order, then fresh slots; slot order defines pair keys.
+// ---------------------------------------------------------------------------
+
+typedef struct m3IdPool
+{
+    int32_t capacity;
+    int32_t maxIndex;
+    int32_t freeHead;
+    int32_t freeCount;
+    int32_t retiredCount;
+    uint16_t* generations;
+    uint8_t* alive;
+    int32_t* freeQueue;
+} m3IdPool;
+
+static void* Zalloc(size_t n)
+{
+    void* p = calloc(n > 0 ? n : 1, 1);
+    if (p == NULL)
+    {
+        fprintf(stderr, "maul3d_amd: out of host memory (%zu bytes)\n", n);
+        abort();
+    }
+    return p;
+}
+
+static m3IdPool PoolCreate(int32_t capacity)
+{
+    m3IdPool p;
+    memset(&p, 0, sizeof(p));
+    p.capacity = capacity;
+    p.generations = (uint16_t*)Zalloc((size_t)capacity * sizeof(uint16_t));
+    p.alive = (uint8_t*)Zalloc((size_t)capacity);
+    p.freeQueue = (int32_t*)Zalloc((size_t)capacity * sizeof(int32_t));
+    return p;
+}
+
+static void PoolDestroy(m3IdPool* p)
+{
+    free(p->generations);
+    free(p->alive);
+    free(p->freeQueue);
+    memset(p, 0, sizeof(*p));
+}
+
+static int32_t PoolAlloc(m3IdPool* p)
+{
+    if (p->freeCount > 0)
+    {
+        int32_t index = p->freeQueue[p->freeHead];
+        p->freeHead = (p->freeHead + 1) % p->capacity;
+        p->freeCount -= 1;
+        p->alive[index] = 1;
+        return index;
+    }
+    if (p->maxIndex < p->capacity)
+    {
+        int32_t index = p->maxIndex;
+        p->maxIndex += 1;
+        p->alive[index] = 1;
+        return index;
+    }
+    return -1;
+}
+
+static void PoolFree(m3IdPool* p, int32_t index)
+{
+    if (index < 0 || index >= p->capacity || p->alive[index] == 0)
+    {
+        return;
+    }
+    p->alive[index] = 0;
+    if (p->generations[index] == 0xFFFF)
+    {
+        p->retiredCount += 1;
+        return;
+    }
+    p->generations[index] += 1;
+    p->freeQueue[(p->freeHead + p->freeCount) % p->capacity] = index;
+    p->freeCount += 1;
+}
+
+static int PoolValid(const m3IdPool* p, int32_t index, uint16_t generation)
+{
+    return index >= 0 && index < p->capacity && p->alive[index] != 0 &&
+           p->generations[index] == generation;
+}
+
+// ---------------------------------------------------------------------------
+// World
+// ---------------------------------------------------------------------------
+
+typedef struct m3World
+{
+    m3Vec3 gravity;
+    float contactHertz;
+    float contactDampingRatio;
+    float contactPushMaxSpeed;
+    float restitutionThreshold;
+    float maximumLinearSpeed;
+    float maximumAngularSpeed;
+    uint8_t sleepEnabled;
+    uint8_t continuousEnabled;
+    float hitEventThreshold;
+    uint64_t stepCount;
+    int32_t bodyCapacity;
+    int32_t shapeCapacity;
+    int32_t jointCapacity;
+    uint16_t generation;
+    uint16_t worldIndex0;
+
+    m3IdPool bodyPool;
+    m3Transform* transforms;
+    m3Vec3* linearVelocities;
+    m3Vec3* angularVelocities;
+    float* invMass;
+    m3Mat3* invInertiaLocal;
+    m3Mat3* inertiaLocal;
+    m3Vec3* localCenters;
+    float* gravityScales;
+    float* linearDamping;
+    float* angularDamping;
+    uint8_t* types;
+    uint8_t* awake;
+    float* sleepTimes;
+    uint8_t* bulletFlags;
+    float* minExtents;
+    float* maxExtents;
+    uint64_t* userData;
+    m3Vec3* bodyForce;
+    m3Vec3* bodyTorque;
+    uint8_t* bodyEnabled;
+    uint8_t* bodyLocks;
+    float* bodySleepThreshold;
+    uint8_t* bodyCanSleep;
+    uint8_t* bodyHasTarget;
+    m3Transform* bodyTarget;
+    int32_t* bodyShapeHead;
+
+    m3IdPool shapePool;
+    int32_t* shapeBody;
+    uint8_t* shapeType;
+    m3ShapeGeom* shapeGeom;
+    float* shapeDensity;
+    float* shapeFriction;
+    float* shapeRestitution;
+    uint8_t* shapeHitEvents;
+    m3Vec3* shapeLocalPos;
+    m3Quat* shapeLocalRot;
+    uint8_t* shapeHasOffset;
+    m3Vec3* shapeSurfaceVel;
+    uint8_t* shapePreSolve;
+    float* shapeRollingResistance;
+    uint64_t* shapeCategory;
+    uint64_t* shapeMask;
+    int32_t* shapeGroup;
+    uint64_t* shapeUserData;
+    int32_t* shapeNext;
+    int32_t* shapeHullIndex;
+    int32_t* shapeMeshIndex;
+    uint8_t* shapeSensor;
+    double* leafBoxes; // the stored proxy box of every tree shape
+
+    m3IdPool hullPool;
+    m3HullData* hullData;
+    int32_t* hullRefCounts;
+
+    m3IdPool jointPool; // joints are not on the device path yet: always empty
+
+    uint64_t* pairKeys;
+    m3Manifold* manifolds;
+    int32_t pairCount;
+    int32_t pairCapacity;
+    uint64_t* sleepingPairKeys;
+    int32_t sleepingPairCount;
+    uint8_t frozenDirty;
+    uint8_t pairsFullQuery;
+
+    // device mirror
+    m3gpuWorld* dev;
+    uint32_t devDirty;  // field groups the host changed since the last upload
+    uint32_t hostStale; // field groups the device changed since the last download
+
+    m3Profile profile;
+    int32_t lastIslandCount;
+    int32_t lastColorCount;
+    int32_t lastAwakeCount;
+    int32_t lastContactCount;
+} m3World;
+
+static m3World* s_worlds[M3_MAX_WORLDS];
+static uint16_t s_worldGenerations[M3_MAX_WORLDS];
+
+static m3World* WorldFromId(m3WorldId id)
+{
+    int32_t index = id.index1 - 1;
+    if (index < 0 || index >= M3_MAX_WORLDS || s_worlds[index] == NULL ||
+        s_worldGenerations[index] != id.generation)
+    {
+        return NULL;
+    }
+    return s_worlds[index];
+}
+
+static m3World* WorldFromIndex0(uint16_t index0)
+{
+    return index0 < M3_MAX_WORLDS ? s_worlds[index0] : NULL;
+}
+
+static int FiniteF(float f) { return f == f && f - f == 0.0f; }
+static int FiniteV3(m3Vec3 v) { return FiniteF(v.x) && FiniteF(v.y) && FiniteF(v.z); }
+static int FiniteQ(m3Quat q) { return FiniteF(q.x) && FiniteF(q.y) && FiniteF(q.z) && FiniteF(q.w); }
+static int FiniteD(double d) { return d == d && d - d == 0.0; }
+static int FiniteP3(m3Pos3 p) { return FiniteD(p.x) && FiniteD(p.y) && FiniteD(p.z); }
+
+static void HostView(m3World* w, m3gpuHostState* h)
+{
+    memset(h, 0, sizeof(*h));
+    h->maxBody = w->bodyPool.maxIndex;
+    h->maxShape = w->shapePool.maxIndex;
+    h->maxHull = w->hullPool.maxIndex;
+    h->transforms = w->transforms;
+    h->linearVelocities = w->linearVelocities;
+    h->angularVelocities = w->angularVelocities;
+    h->invMass = w->invMass;
+    h->invInertiaLocal = w->invInertiaLocal;
+    h->inertiaLocal = w->inertiaLocal;
+    h->localCenters = w->localCenters;
+    h->gravityScales = w->gravityScales;
+    h->linearDamping = w->linearDamping;
+    h->angularDamping = w->angularDamping;
+    h->bodyAlive = w->bodyPool.alive;
+    h->types = w->types;
+    h->awake = w->awake;
+    h->sleepTimes = w->sleepTimes;
+    h->bulletFlags = w->bulletFlags;
+    h->minExtents = w->minExtents;
+    h->maxExtents = w->maxExtents;
+    h->bodyForce = w->bodyForce;
+    h->bodyTorque = w->bodyTorque;
+    h->bodyEnabled = w->bodyEnabled;
+    h->bodyLocks = w->bodyLocks;
+    h->bodySleepThreshold = w->bodySleepThreshold;
+    h->bodyCanSleep = w->bodyCanSleep;
+    h->bodyHasTarget = w->bodyHasTarget;
+    h->bodyTarget = w->bodyTarget;
+    h->bodyShapeHead = w->bodyShapeHead;
+    h->shapeAlive = w->shapePool.alive;
+    h->shapeBody = w->shapeBody;
+    h->shapeType = w->shapeType;
+    h->shapeGeom = w->shapeGeom;
+    h->shapeFriction = w->shapeFriction;
+    h->shapeRestitution = w->shapeRestitution;
+    h->shapeRollingResistance = w->shapeRollingResistance;
+    h->shapeSensor = w->shapeSensor;
+    h->shapeHasOffset = w->shapeHasOffset;
+    h->shapeLocalPos = w->shapeLocalPos;
+    h->shapeLocalRot = w->shapeLocalRot;
+    h->shapeSurfaceVel = w->shapeSurfaceVel;
+    h->shapeCategory = w->shapeCategory;
+    h->shapeMask = w->shapeMask;
+    h->shapeGroup = w->shapeGroup;
+    h->shapeNext = w->shapeNext;
+    h->shapeHullIndex = w->shapeHullIndex;
+    h->leafBoxes = w->leafBoxes;
+    h->hullData = w->hullData;
+    h->pairCount = &w->pairCount;
+    h->pairKeys = w->pairKeys;
+    h->manifolds = w->manifolds;
+    h->sleepingPairCount = &w->sleepingPairCount;
+    h->sleepingPairKeys = w->sleepingPairKeys;
+    h->frozenDirty = &w->frozenDirty;
+    h->pairsFullQuery = &w->pairsFullQuery;
+    h->stepCount = &w->stepCount;
+}
+
+static void DeviceFail(const char* what)
+{
+    fprintf(stderr, "maul3d_amd: %s failed: %s\n", what, m3gpu_last_error());
+    abort();
+}
+
+// Bring the host copy up to date with the device (after a step).
+static void SyncHost(m3World* w)
+{
+    if (w->hostStale == 0 || w->dev == NULL)
+    {
+        return;
+    }
+    m3gpuHostState h;
+    HostView(w, &h);
+    if (m3gpu_download(w->dev, &h, w->hostStale) != 0)
+    {
+        DeviceFail("m3gpu_download");
+    }
+    w->hostStale = 0;
+}
+
+// Every mutation goes through here: read the device state back first, then
+// mark the groups the mutation touches for upload before the next step.
+static void Touch(m3World* w, uint32_t groups)
+{
+    SyncHost(w);
+    w->devDirty |= groups;
+}
+
+// ---------------------------------------------------------------------------
+// Defaults and world lifetime (src/world.c:64-480)
+// ---------------------------------------------------------------------------
+
+int m3GetVersion(void) { return 1 * 10000 + 22 * 100 + 0; }
+
+uint64_t m3Hash64(uint64_t h, const void* bytes, int32_t count)
+{
+    return m3HashBytes(h, bytes, count);
+}
+
+m3WorldDef m3DefaultWorldDef(void)
+{
+    m3WorldDef def;
+    memset(&def, 0, sizeof(def));
+    def.gravity = m3V3(0.0f, -10.0f, 0.0f);
+    def.bodyCapacity = 1024;
+    def.shapeCapacity = 2048;
+    def.meshCapacity = 4;
+    def.jointCapacity = 64;
+    def.voxelCapacity = 4;
+    def.characterCapacity = 4;
+    def.vehicleCapacity = 2;
+    def.softBodyCapacity = 2;
+    def.workerCount = 1;
+    def.contactHertz = M3_CONTACT_HERTZ_DEFAULT;
+    def.contactDampingRatio = M3_CONTACT_DAMPING_RATIO_DEFAULT;
+    def.contactPushMaxSpeed = M3_CONTACT_PUSH_MAX_SPEED_DEFAULT;
+    def.restitutionThreshold = M3_RESTITUTION_THRESHOLD_DEFAULT;
+    def.maximumLinearSpeed = M3_MAX_LINEAR_SPEED_DEFAULT;
+    def.enableSleeping = 1;
+    def.enableContinuous = 1;
+    def.hitEventThreshold = M3_HIT_EVENT_THRESHOLD_DEFAULT;
+    def.internalValue = M3_WORLD_COOKIE;
+    return def;
+}
+
+#define ALLOC(field, n, type) (field) = (type*)Zalloc((size_t)(n) * sizeof(type))
+
+m3WorldId m3CreateWorld(const m3WorldDef* def)
+{
+    m3WorldId nullId = {0, 0};
+    if (def == NULL || def->internalValue != M3_WORLD_COOKIE || def->bodyCapacity <= 0 ||
+        def->shapeCapacity <= 0 || def->meshCapacity <= 0 || def->jointCapacity <= 0 ||
+        def->voxelCapacity <= 0 || def->characterCapacity <= 0 || def->vehicleCapacity <= 0 ||
+        def->softBodyCapacity <= 0 || def->workerCount <= 0 ||
+        (def->enqueueTask == NULL) != (def->finishTask == NULL) || !FiniteV3(def->gravity) ||
+        !FiniteF(def->contactHertz) || def->contactHertz <= 0.0f ||
+        !FiniteF(def->contactDampingRatio) || def->contactDampingRatio <= 0.0f ||
+        !FiniteF(def->contactPushMaxSpeed) || def->contactPushMaxSpeed <= 0.0f ||
+        !FiniteF(def->restitutionThreshold) || def->restitutionThreshold < 0.0f ||
+        !FiniteF(def->maximumLinearSpeed) || def->maximumLinearSpeed <= 0.0f ||
+        !FiniteF(def->hitEventThreshold) || def->hitEventThreshold < 0.0f)
+    {
+        return nullId;
+    }
+    int32_t slot = -1;
+    for (int32_t i = 0; i < M3_MAX_WORLDS; ++i)
+    {
+        if (s_worlds[i] == NULL)
+        {
+            slot = i;
+            break;
+        }
+    }
+    if (slot < 0)
+    {
+        return nullId;
+    }
+    m3World* w = (m3World*)Zalloc(sizeof(m3World));
+    w->gravity = def->gravity;
+    w->contactHertz = def->contactHertz;
+    w->contactDampingRatio = def->contactDampingRatio;
+    w->contactPushMaxSpeed = def->contactPushMaxSpeed;
+    w->restitutionThreshold = def->restitutionThreshold;
+    w->maximumLinearSpeed = def->maximumLinearSpeed;
+    w->maximumAngularSpeed = M3_MAX_ANGULAR_SPEED_DEFAULT;
+    w->sleepEnabled = def->enableSleeping != 0 ? 1 : 0;
+    w->continuousEnabled = def->enableContinuous != 0 ? 1 : 0;
+    w->hitEventThreshold = def->hitEventThreshold;
+    w->bodyCapacity = def->bodyCapacity;
+    w->shapeCapacity = def->shapeCapacity;
+    w->jointCapacity = def->jointCapacity;
+    w->generation = s_worldGenerations[slot];
+    w->worldIndex0 = (uint16_t)slot;
+
+    int32_t cap = def->bodyCapacity;
+    w->bodyPool = PoolCreate(cap);
+    ALLOC(w->transforms, cap, m3Transform);
+    ALLOC(w->linearVelocities, cap, m3Vec3);
+    ALLOC(w->angularVelocities, cap, m3Vec3);
+    ALLOC(w->invMass, cap, float);
+    ALLOC(w->invInertiaLocal, cap, m3Mat3);
+    ALLOC(w->inertiaLocal, cap, m3Mat3);
+    ALLOC(w->localCenters, cap, m3Vec3);
+    ALLOC(w->gravityScales, cap, float);
+    ALLOC(w->linearDamping, cap, float);
+    ALLOC(w->angularDamping, cap, float);
+    ALLOC(w->types, cap, uint8_t);
+    ALLOC(w->awake, cap, uint8_t);
+    ALLOC(w->sleepTimes, cap, float);
+    ALLOC(w->bulletFlags, cap, uint8_t);
+    ALLOC(w->minExtents, cap, float);
+    ALLOC(w->maxExtents, cap, float);
+    ALLOC(w->userData, cap, uint64_t);
+    ALLOC(w->bodyForce, cap, m3Vec3);
+    ALLOC(w->bodyTorque, cap, m3Vec3);
+    ALLOC(w->bodyEnabled, cap, uint8_t);
+    ALLOC(w->bodyLocks, cap, uint8_t);
+    ALLOC(w->bodySleepThreshold, cap, float);
+    ALLOC(w->bodyCanSleep, cap, uint8_t);
+    ALLOC(w->bodyHasTarget, cap, uint8_t);
+    ALLOC(w->bodyTarget, cap, m3Transform);
+    ALLOC(w->bodyShapeHead, cap, int32_t);
+    for (int32_t i = 0; i < cap; ++i)
+    {
+        w->bodyShapeHead[i] = -1;
+    }
+
+    int32_t shapeCap = def->shapeCapacity;
+    w->shapePool = PoolCreate(shapeCap);
+    ALLOC(w->shapeBody, shapeCap, int32_t);
+    ALLOC(w->shapeType, shapeCap, uint8_t);
+    ALLOC(w->shapeGeom, shapeCap, m3ShapeGeom);
+    ALLOC(w->shapeDensity, shapeCap, float);
+    ALLOC(w->shapeFriction, shapeCap, float);
+    ALLOC(w->shapeRestitution, shapeCap, float);
+    ALLOC(w->shapeHitEvents, shapeCap, uint8_t);
+    ALLOC(w->shapeLocalPos, shapeCap, m3Vec3);
+    ALLOC(w->shapeLocalRot, shapeCap, m3Quat);
+    ALLOC(w->shapeHasOffset, shapeCap, uint8_t);
+    ALLOC(w->shapeSurfaceVel, shapeCap, m3Vec3);
+    ALLOC(w->shapePreSolve, shapeCap, uint8_t);
+    ALLOC(w->shapeRollingResistance, shapeCap, float);
+    ALLOC(w->shapeCategory, shapeCap, uint64_t);
+    ALLOC(w->shapeMask, shapeCap, uint64_t);
+    ALLOC(w->shapeGroup, shapeCap, int32_t);
+    ALLOC(w->shapeUserData, shapeCap, uint64_t);
+    ALLOC(w->shapeNext, shapeCap, int32_t);
+    ALLOC(w->shapeHullIndex, shapeCap, int32_t);
+    ALLOC(w->shapeMeshIndex, shapeCap, int32_t);
+    ALLOC(w->shapeSensor, shapeCap, uint8_t);
+    ALLOC(w->leafBoxes, (size_t)shapeCap * 6, double);
+    for (int32_t i = 0; i < shapeCap; ++i)
+    {
+        w->shapeBody[i] = -1;
+        w->shapeNext[i] = -1;
+        w->shapeHullIndex[i] = -1;
+        w->shapeMeshIndex[i] = -1;
+    }
+    w->hullPool = PoolCreate(shapeCap);
+    ALLOC(w->hullData, shapeCap, m3HullData);
+    ALLOC(w->hullRefCounts, shapeCap, int32_t);
+    w->jointPool = PoolCreate(def->jointCapacity);
+
+    w->pairCapacity = 8 * shapeCap;
+    ALLOC(w->pairKeys, w->pairCapacity, uint64_t);
+    ALLOC(w->manifolds, w->pairCapacity, m3Manifold);
+    ALLOC(w->sleepingPairKeys, w->pairCapacity, uint64_t);
+    w->devDirty = M3GPU_FIELDS_ALL;
+
+    s_worlds[slot] = w;
+    m3WorldId id = {slot + 1, w->generation};
+    return id;
+}
+
+void m3DestroyWorld(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return;
+    }
+    int32_t slot = w->worldIndex0;
+    if (w->dev != NULL)
+    {
+        m3gpu_world_destroy(w->dev);
+    }
+    void* arrays[] = {
+        w->transforms, w->linearVelocities, w->angularVelocities, w->invMass,
+        w->invInertiaLocal, w->inertiaLocal, w->localCenters, w->gravityScales,
+        w->linearDamping, w->angularDamping, w->types, w->awake, w->sleepTimes,
+        w->bulletFlags, w->minExtents, w->maxExtents, w->userData, w->bodyForce,
+        w->bodyTorque, w->bodyEnabled, w->bodyLocks, w->bodySleepThreshold,
+        w->bodyCanSleep, w->bodyHasTarget, w->bodyTarget, w->bodyShapeHead,
+        w->shapeBody, w->shapeType, w->shapeGeom, w->shapeDensity, w->shapeFriction,
+        w->shapeRestitution, w->shapeHitEvents, w->shapeLocalPos, w->shapeLocalRot,
+        w->shapeHasOffset, w->shapeSurfaceVel, w->shapePreSolve, w->shapeRollingResistance,
+        w->shapeCategory, w->shapeMask, w->shapeGroup, w->shapeUserData, w->shapeNext,
+        w->shapeHullIndex, w->shapeMeshIndex, w->shapeSensor, w->leafBoxes, w->hullData,
+        w->hullRefCounts, w->pairKeys, w->manifolds, w->sleepingPairKeys,
+    };
+    for (size_t i = 0; i < sizeof(arrays) / sizeof(arrays[0]); ++i)
+    {
+        free(arrays[i]);
+    }
+    PoolDestroy(&w->bodyPool);
+    PoolDestroy(&w->shapePool);
+    PoolDestroy(&w->hullPool);
+    PoolDestroy(&w->jointPool);
+    free(w);
+    s_worlds[slot] = NULL;
+    s_worldGenerations[slot] += 1;
+}
+
+bool m3World_IsValid(m3WorldId worldId) { return WorldFromId(worldId) != NULL; }
+
+void m3World_SetGravity(m3WorldId worldId, m3Vec3 gravity)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || !FiniteV3(gravity))
+    {
+        return;
+    }
+    w->gravity = gravity;
+}
+
+m3Vec3 m3World_GetGravity(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    return w != NULL ? w->gravity : m3V3(0.0f, 0.0f, 0.0f);
+}
+
+void m3World_EnableContinuous(m3WorldId worldId, bool flag)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w != NULL)
+    {
+        w->continuousEnabled = flag ? 1 : 0;
+    }
+}
+
+void m3World_EnableSleeping(m3WorldId worldId, bool flag)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return;
+    }
+    w->sleepEnabled = flag ? 1 : 0;
+    if (!flag)
+    {
+        // Turning sleep off wakes everyone (the reference toggle).
+        Touch(w, M3GPU_FIELDS_DYNAMIC);
+        for (int32_t i = 0; i < w->bodyPool.maxIndex; ++i)
+        {
+            if (w->bodyPool.alive[i] != 0 && w->types[i] == M3_DYNAMIC && w->awake[i] == 0)
+            {
+                w->awake[i] = 1;
+                w->sleepTimes[i] = 0.0f;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bodies (src/body.c:35-101)
+// ---------------------------------------------------------------------------
+
+m3BodyDef m3DefaultBodyDef(void)
+{
+    m3BodyDef def;
+    memset(&def, 0, sizeof(def));
+    def.type = m3_staticBody;
+    def.rotation = m3IdentityQuat();
+    def.gravityScale = 1.0f;
+    def.internalValue = M3_BODY_COOKIE;
+    return def;
+}
+
+static int32_t BodySlot(const m3World* w, m3BodyId id)
+{
+    int32_t index = id.index1 - 1;
+    if (w == NULL || id.world0 != w->worldIndex0 || !PoolValid(&w->bodyPool, index, id.generation))
+    {
+        return -1;
+    }
+    return index;
+}
+
+static m3World* ResolveBody(m3BodyId id, int32_t* out)
+{
+    m3World* w = WorldFromIndex0(id.world0);
+    int32_t index = w != NULL ? BodySlot(w, id) : -1;
+    *out = index;
+    return index >= 0 ? w : NULL;
+}
+
+m3BodyId m3CreateBody(m3WorldId worldId, const m3BodyDef* def)
+{
+    m3BodyId nullId = {0, 0, 0};
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || def == NULL || def->internalValue != M3_BODY_COOKIE ||
+        (def->type != m3_staticBody && def->type != m3_kinematicBody &&
+         def->type != m3_dynamicBody))
+    {
+        return nullId;
+    }
+    m3real qq = def->rotation.x * def->rotation.x + def->rotation.y * def->rotation.y +
+                def->rotation.z * def->rotation.z + def->rotation.w * def->rotation.w;
+    if (!FiniteP3(def->position) || !FiniteQ(def->rotation) || !FiniteV3(def->linearVelocity) ||
+        !FiniteV3(def->angularVelocity) || !FiniteF(def->gravityScale) ||
+        !FiniteF(def->linearDamping) || !FiniteF(def->angularDamping) ||
+        def->linearDamping < 0.0f || def->angularDamping < 0.0f || !(qq > 0.98f) ||
+        !(qq < 1.02f))
+    {
+        return nullId;
+    }
+    Touch(w, M3GPU_FIELDS_ALL);
+    int32_t i = PoolAlloc(&w->bodyPool);
+    if (i < 0)
+    {
+        return nullId;
+    }
+    w->transforms[i].p = def->position;
+    w->transforms[i].q = m3NormalizeQuat(def->rotation);
+    w->linearVelocities[i] = def->linearVelocity;
+    w->angularVelocities[i] = def->angularVelocity;
+    w->invMass[i] = def->type == m3_dynamicBody ? 1.0f : 0.0f;
+    w->invInertiaLocal[i] = m3ZeroMat3();
+    w->inertiaLocal[i] = m3ZeroMat3();
+    w->awake[i] = 1;
+    w->sleepTimes[i] = 0.0f;
+    w->minExtents[i] = 1.0e30f;
+    w->maxExtents[i] = 0.0f;
+    w->localCenters[i] = m3V3(0.0f, 0.0f, 0.0f);
+    w->gravityScales[i] = def->gravityScale;
+    w->linearDamping[i] = def->linearDamping;
+    w->angularDamping[i] = def->angularDamping;
+    w->types[i] = (uint8_t)def->type;
+    w->bulletFlags[i] = def->isBullet ? 1 : 0;
+    w->userData[i] = def->userData;
+    w->bodyForce[i] = m3V3(0.0f, 0.0f, 0.0f);
+    w->bodyTorque[i] = m3V3(0.0f, 0.0f, 0.0f);
+    w->bodyEnabled[i] = 1;
+    w->bodyLocks[i] = 0;
+    w->bodySleepThreshold[i] = M3_SLEEP_VELOCITY_DEFAULT;
+    w->bodyCanSleep[i] = 1;
+    w->bodyHasTarget[i] = 0;
+    memset(&w->bodyTarget[i], 0, sizeof(m3Transform));
+    w->bodyTarget[i].q.w = 1.0f;
+    w->bodyShapeHead[i] = -1;
+    m3BodyId id = {i + 1, w->worldIndex0, w->bodyPool.generations[i]};
+    return id;
+}
+
+bool m3Body_IsValid(m3BodyId bodyId)
+{
+    int32_t i;
+    return ResolveBody(bodyId, &i) != NULL;
+}
+
+m3Pos3 m3Body_GetPosition(m3BodyId bodyId)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    m3Pos3 zero = {0.0, 0.0, 0.0};
+    if (w == NULL)
+    {
+        return zero;
+    }
+    SyncHost(w);
+    return w->transforms[i].p;
+}
+
+m3Quat m3Body_GetRotation(m3BodyId bodyId)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL)
+    {
+        return m3IdentityQuat();
+    }
+    SyncHost(w);
+    return w->transforms[i].q;
+}
+
+m3Vec3 m3Body_GetLinearVelocity(m3BodyId bodyId)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL)
+    {
+        return m3V3(0.0f, 0.0f, 0.0f);
+    }
+    SyncHost(w);
+    return w->linearVelocities[i];
+}
+
+m3Vec3 m3Body_GetAngularVelocity(m3BodyId bodyId)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL)
+    {
+        return m3V3(0.0f, 0.0f, 0.0f);
+    }
+    SyncHost(w);
+    return w->angularVelocities[i];
+}
+
+// src/body.c:406-418: a commanded velocity always wakes.
+void m3Body_SetLinearVelocity(m3BodyId bodyId, m3Vec3 velocity)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL || !FiniteV3(velocity))
+    {
+        return;
+    }
+    Touch(w, M3GPU_FIELDS_DYNAMIC);
+    w->awake[i] = 1;
+    w->sleepTimes[i] = 0.0f;
+    w->linearVelocities[i] = velocity;
+}
+
+void m3Body_SetAngularVelocity(m3BodyId bodyId, m3Vec3 velocity)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL || !FiniteV3(velocity))
+    {
+        return;
+    }
+    Touch(w, M3GPU_FIELDS_DYNAMIC);
+    w->awake[i] = 1;
+    w->sleepTimes[i] = 0.0f;
+    w->angularVelocities[i] = velocity;
+}
+
+bool m3Body_IsAwake(m3BodyId bodyId)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL)
+    {
+        return false;
+    }
+    SyncHost(w);
+    return w->awake[i] != 0;
+}
+
+// src/body.c:337-361: forces accumulate for one step and wake a body that
+// can respond.
+static void ApplyForceLike(m3BodyId bodyId, m3Vec3 value, int torque)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL || !FiniteV3(value))
+    {
+        return;
+    }
+    if (!(w->types[i] == M3_DYNAMIC && w->invMass[i] > 0.0f))
+    {
+        return;
+    }
+    Touch(w, M3GPU_FIELDS_DYNAMIC);
+    if (torque)
+    {
+        w->bodyTorque[i] = m3Add3(w->bodyTorque[i], value);
+    }
+    else
+    {
+        w->bodyForce[i] = m3Add3(w->bodyForce[i], value);
+    }
+    if (value.x != 0.0f || value.y != 0.0f || value.z != 0.0f)
+    {
+        w->awake[i] = 1;
+        w->sleepTimes[i] = 0.0f;
+    }
+}
+
+void m3Body_ApplyForce(m3BodyId bodyId, m3Vec3 force) { ApplyForceLike(bodyId, force, 0); }
+void m3Body_ApplyTorque(m3BodyId bodyId, m3Vec3 torque) { ApplyForceLike(bodyId, torque, 1); }
+
+// ---------------------------------------------------------------------------
+// Hulls: the canonical box and its half-edge adjacency (src/hull.c:14-139)
+// ---------------------------------------------------------------------------
+
+static void HullBuildHalfEdges(m3HullData* hull)
+{
+    int32_t count = 0;
+    int32_t dirKey[M3_HULL_MAX_HALF_EDGES];
+    int32_t dirSlot[M3_HULL_MAX_HALF_EDGES];
+    int32_t dirCount = 0;
+    int32_t faceEdgeSlot[M3_HULL_MAX_FACE_INDICES];
+    for (int32_t f = 0; f < hull->faceCount; ++f)
+    {
+        int32_t n = hull->faceVertCounts[f];
+        int32_t start = hull->faceVertStart[f];
+        for (int32_t k = 0; k < n; ++k)
+        {
+            int32_t a = hull->faceIndices[start + k];
+            int32_t b = hull->faceIndices[start + (k + 1) % n];
+            int32_t reverseKey = b * 256 + a;
+            int32_t slot = -1;
+            for (int32_t d = 0; d < dirCount; ++d)
+            {
+                if (dirKey[d] == reverseKey)
+                {
+                    slot = dirSlot[d] ^ 1;
+                    break;
+                }
+            }
+            if (slot < 0)
+            {
+                slot = count;
+                count += 2;
+                dirKey[dirCount] = a * 256 + b;
+                dirSlot[dirCount] = slot;
+                dirCount += 1;
+            }
+            hull->edges[slot].origin = (uint8_t)a;
+            hull->edges[slot].twin = (uint16_t)(slot ^ 1);
+            hull->edges[slot].face = (uint8_t)f;
+            faceEdgeSlot[start + k] = slot;
+        }
+    }
+    for (int32_t f = 0; f < hull->faceCount; ++f)
+    {
+        int32_t n = hull->faceVertCounts[f];
+        int32_t start = hull->faceVertStart[f];
+        for (int32_t k = 0; k < n; ++k)
+        {
+            hull->edges[faceEdgeSlot[start + k]].next = (uint16_t)faceEdgeSlot[start + (k + 1) % n];
+        }
+    }
+    hull->edgeCount = count;
+    m3Vec3 c = m3V3(0.0f, 0.0f, 0.0f);
+    for (int32_t v = 0; v < hull->vertexCount; ++v)
+    {
+        c = m3Add3(c, hull->vertices[v]);
+    }
+    hull->center = m3MulSV3(1.0f / (m3real)hull->vertexCount, c);
+}
+
+static void BuildBoxHull(m3HullData* out, m3Vec3 he)
+{
+    memset(out, 0, sizeof(*out));
+    m3real hx = he.x;
+    m3real hy = he.y;
+    m3real hz = he.z;
+    // Canonical vertex order: bit0 of the index pattern walks -x,+x around
+    // the -z face then the +z face.
+    const m3real sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    const m3real sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    const m3real sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+    for (int32_t v = 0; v < 8; ++v)
+    {
+        out->vertices[v] = m3V3(sx[v] < 0 ? -hx : hx, sy[v] < 0 ? -hy : hy, sz[v] < 0 ? -hz : hz);
+    }
+    const m3real nrm[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    const m3real off[6] = {hx, hx, hy, hy, hz, hz};
+    // Counter-clockwise loops seen from outside: +x, -x, +y, -y, +z, -z.
+    const uint8_t loops[6][4] = {{1, 2, 6, 5}, {0, 4, 7, 3}, {2, 3, 7, 6},
+                                 {0, 1, 5, 4}, {4, 5, 6, 7}, {0, 3, 2, 1}};
+    out->vertexCount = 8;
+    out->faceCount = 6;
+    out->indexCount = 24;
+    for (int32_t f = 0; f < 6; ++f)
+    {
+        out->faceNormals[f] = m3V3(nrm[f][0], nrm[f][1], nrm[f][2]);
+        out->faceOffsets[f] = off[f];
+        out->faceVertCounts[f] = 4;
+        out->faceVertStart[f] = (uint16_t)(4 * f);
+        for (int32_t k = 0; k < 4; ++k)
+        {
+            out->faceIndices[4 * f + k] = loops[f][k];
+        }
+    }
+    // Closed-form unit-density box: m = 8 hx hy hz, I = m/3 diag(...).
+    float m = 8.0f * hx * hy * hz;
+    out->unitMass = m;
+    out->unitCom = m3V3(0.0f, 0.0f, 0.0f);
+    out->unitInertiaCom = m3ZeroMat3();
+    out->unitInertiaCom.cx.x = (m / 3.0f) * (hy * hy + hz * hz);
+    out->unitInertiaCom.cy.y = (m / 3.0f) * (hx * hx + hz * hz);
+    out->unitInertiaCom.cz.z = (m / 3.0f) * (hx * hx + hy * hy);
+    HullBuildHalfEdges(out);
+}
+
+// Content dedupe in ascending slot order (src/hull.c:141-163).
+static int32_t InternHull(m3World* w, const m3HullData* data)
+{
+    for (int32_t i = 0; i < w->hullPool.maxIndex; ++i)
+    {
+        if (w->hullPool.alive[i] != 0 && memcmp(&w->hullData[i], data, sizeof(m3HullData)) == 0)
+        {
+            w->hullRefCounts[i] += 1;
+            return i;
+        }
+    }
+    int32_t index = PoolAlloc(&w->hullPool);
+    if (index < 0)
+    {
+        return -1;
+    }
+    w->hullData[index] = *data;
+    w->hullRefCounts[index] = 1;
+    return index;
+}
+
+// ---------------------------------------------------------------------------
+// Mass properties (src/shape.c:49-301)
+// ---------------------------------------------------------------------------
+
+static m3Mat3 InvertSymmetric(m3Mat3 m)
+{
+    m3real a = m.cx.x;
+    m3real b = m.cy.x;
+    m3real c = m.cz.x;
+    m3real d = m.cy.y;
+    m3real e = m.cz.y;
+    m3real f = m.cz.z;
+    m3real co00 = d * f - e * e;
+    m3real co01 = c * e - b * f;
+    m3real co02 = b * e - c * d;
+    m3real det = a * co00 + b * co01 + c * co02;
+    if (!(det > 0.0f))
+    {
+        return m3ZeroMat3();
+    }
+    m3real inv = 1.0f / det;
+    m3Mat3 r;
+    r.cx = m3V3(co00 * inv, co01 * inv, co02 * inv);
+    r.cy = m3V3(co01 * inv, (a * f - c * c) * inv, (b * c - a * e) * inv);
+    r.cz = m3V3(co02 * inv, (b * c - a * e) * inv, (a * d - b * b) * inv);
+    return r;
+}
+
+// c' = p + R c, I' = R I R^T for a shape on a compound offset.
+static void ComposeMassProps(const m3World* w, int32_t s, m3Vec3* com, m3Mat3* I)
+{
+    if (w->shapeHasOffset[s] == 0)
+    {
+        return;
+    }
+    m3Quat q = w->shapeLocalRot[s];
+    *com = m3Add3(w->shapeLocalPos[s], m3RotateVec3(q, *com));
+    m3Mat3 r;
+    r.cx = m3RotateVec3(q, m3V3(1.0f, 0.0f, 0.0f));
+    r.cy = m3RotateVec3(q, m3V3(0.0f, 1.0f, 0.0f));
+    r.cz = m3RotateVec3(q, m3V3(0.0f, 0.0f, 1.0f));
+    m3Mat3 t;
+    t.cx = m3V3(I->cx.x * r.cx.x + I->cy.x * r.cx.y + I->cz.x * r.cx.z,
+                I->cx.y * r.cx.x + I->cy.y * r.cx.y + I->cz.y * r.cx.z,
+                I->cx.z * r.cx.x + I->cy.z * r.cx.y + I->cz.z * r.cx.z);
+    t.cy = m3V3(I->cx.x * r.cy.x + I->cy.x * r.cy.y + I->cz.x * r.cy.z,
+                I->cx.y * r.cy.x + I->cy.y * r.cy.y + I->cz.y * r.cy.z,
+                I->cx.z * r.cy.x + I->cy.z * r.cy.y + I->cz.z * r.cy.z);
+    t.cz = m3V3(I->cx.x * r.cz.x + I->cy.x * r.cz.y + I->cz.x * r.cz.z,
+                I->cx.y * r.cz.x + I->cy.y * r.cz.y + I->cz.y * r.cz.z,
+                I->cx.z * r.cz.x + I->cy.z * r.cz.y + I->cz.z * r.cz.z);
+    I->cx = m3V3(r.cx.x * t.cx.x + r.cy.x * t.cx.y + r.cz.x * t.cx.z,
+                 r.cx.y * t.cx.x + r.cy.y * t.cx.y + r.cz.y * t.cx.z,
+                 r.cx.z * t.cx.x + r.cy.z * t.cx.y + r.cz.z * t.cx.z);
+    I->cy = m3V3(r.cx.x * t.cy.x + r.cy.x * t.cy.y + r.cz.x * t.cy.z,
+                 r.cx.y * t.cy.x + r.cy.y * t.cy.y + r.cz.y * t.cy.z,
+                 r.cx.z * t.cy.x + r.cy.z * t.cy.y + r.cz.z * t.cy.z);
+    I->cz = m3V3(r.cx.x * t.cz.x + r.cy.x * t.cz.y + r.cz.x * t.cz.z,
+                 r.cx.y * t.cz.x + r.cy.y * t.cz.y + r.cz.y * t.cz.z,
+                 r.cx.z * t.cz.x + r.cy.z * t.cz.y + r.cz.z * t.cz.z);
+}
+
+static int ShapeMassProps(const m3World* w, int32_t s, float* mOut, m3Vec3* cOut, m3Mat3* iOut)
+{
+    uint8_t type = w->shapeType[s];
+    if (type == M3_SPHERE)
+    {
+        float r = w->shapeGeom[s].s;
+        float m = w->shapeDensity[s] * (4.0f / 3.0f) * M3_PI * r * r * r;
+        float ic = 0.4f * m * r * r;
+        *mOut = m;
+        *cOut = w->shapeGeom[s].v;
+        *iOut = m3ZeroMat3();
+        iOut->cx.x = ic;
+        iOut->cy.y = ic;
+        iOut->cz.z = ic;
+        return 1;
+    }
+    if (type == M3_CAPSULE)
+    {
+        // Cylinder plus two hemispheres about the segment midpoint:
+        // I = Iperp * Id + (Iaxial - Iperp) * (u u^T).
+        m3Vec3 p1 = w->shapeGeom[s].v;
+        m3Vec3 p2 = w->shapeGeom[s].v2;
+        float r = w->shapeGeom[s].s;
+        m3Vec3 axis = m3Sub3(p2, p1);
+        float length = sqrtf(m3Dot3(axis, axis));
+        m3Vec3 u = length > 0.0f ? m3MulSV3(1.0f / length, axis) : m3V3(1.0f, 0.0f, 0.0f);
+        float density = w->shapeDensity[s];
+        float mCyl = density * M3_PI * r * r * length;
+        float mSph = density * (4.0f / 3.0f) * M3_PI * r * r * r;
+        float axial = 0.5f * mCyl * r * r + 0.4f * mSph * r * r;
+        float perp = mCyl * (length * length / 12.0f + 0.25f * r * r) +
+                     mSph * (0.4f * r * r + 0.25f * length * length + 0.375f * length * r);
+        *mOut = mCyl + mSph;
+        *cOut = m3MulSV3(0.5f, m3Add3(p1, p2));
+        float d = axial - perp;
+        iOut->cx = m3V3(perp + d * u.x * u.x, d * u.x * u.y, d * u.x * u.z);
+        iOut->cy = m3V3(d * u.x * u.y, perp + d * u.y * u.y, d * u.y * u.z);
+        iOut->cz = m3V3(d * u.x * u.z, d * u.y * u.z, perp + d * u.z * u.z);
+        return 1;
+    }
+    if (type == M3_HULL)
+    {
+        const m3HullData* hull = &w->hullData[w->shapeHullIndex[s]];
+        float density = w->shapeDensity[s];
+        *mOut = density * hull->unitMass;
+        *cOut = hull->unitCom;
+        m3Mat3 ic = hull->unitInertiaCom;
+        ic.cx = m3MulSV3(density, ic.cx);
+        ic.cy = m3MulSV3(density, ic.cy);
+        ic.cz = m3MulSV3(density, ic.cz);
+        *iOut = ic;
+        return 1;
+    }
+    return 0;
+}
+
+static void RecomputeMass(m3World* w, int32_t b)
+{
+    if (w->types[b] != M3_DYNAMIC)
+    {
+        w->invMass[b] = 0.0f;
+        w->invInertiaLocal[b] = m3ZeroMat3();
+        w->inertiaLocal[b] = m3ZeroMat3();
+        w->localCenters[b] = m3V3(0.0f, 0.0f, 0.0f);
+        w->minExtents[b] = 1.0e30f;
+        w->maxExtents[b] = 0.0f;
+        return;
+    }
+    float mass = 0.0f;
+    m3Vec3 center = m3V3(0.0f, 0.0f, 0.0f);
+    for (int32_t s = w->bodyShapeHead[b]; s != -1; s = w->shapeNext[s])
+    {
+        float m;
+        m3Vec3 c;
+        m3Mat3 ic;
+        if (!ShapeMassProps(w, s, &m, &c, &ic))
+        {
+            continue;
+        }
+        ComposeMassProps(w, s, &c, &ic);
+        mass += m;
+        center = m3Add3(center, m3MulSV3(m, c));
+    }
+    if (!(mass > 0.0f))
+    {
+        w->invMass[b] = 1.0f;
+        w->invInertiaLocal[b] = m3ZeroMat3();
+        w->inertiaLocal[b] = m3ZeroMat3();
+        w->localCenters[b] = m3V3(0.0f, 0.0f, 0.0f);
+        w->minExtents[b] = 1.0e30f;
+        w->maxExtents[b] = 0.0f;
+        return;
+    }
+    center = m3MulSV3(1.0f / mass, center);
+    m3Mat3 I = m3ZeroMat3();
+    for (int32_t s = w->bodyShapeHead[b]; s != -1; s = w->shapeNext[s])
+    {
+        float m;
+        m3Vec3 c;
+        m3Mat3 ic;
+        if (!ShapeMassProps(w, s, &m, &c, &ic))
+        {
+            continue;
+        }
+        ComposeMassProps(w, s, &c, &ic);
+        m3Vec3 d = m3Sub3(c, center);
+        float d2 = m3Dot3(d, d);
+        // Parallel axis theorem, term by term.
+        I.cx.x += ic.cx.x + m * (d2 - d.x * d.x);
+        I.cy.y += ic.cy.y + m * (d2 - d.y * d.y);
+        I.cz.z += ic.cz.z + m * (d2 - d.z * d.z);
+        I.cy.x += ic.cy.x - m * d.x * d.y;
+        I.cx.y += ic.cx.y - m * d.x * d.y;
+        I.cz.x += ic.cz.x - m * d.x * d.z;
+        I.cx.z += ic.cx.z - m * d.x * d.z;
+        I.cz.y += ic.cz.y - m * d.y * d.z;
+        I.cy.z += ic.cy.z - m * d.y * d.z;
+    }
+    w->invMass[b] = 1.0f / mass;
+    w->inertiaLocal[b] = I;
+    w->invInertiaLocal[b] = InvertSymmetric(I);
+
+    float minExtent = 1.0e30f;
+    float maxExtent = 0.0f;
+    for (int32_t s = w->bodyShapeHead[b]; s != -1; s = w->shapeNext[s])
+    {
+        uint8_t type = w->shapeType[s];
+        if (type == M3_SPHERE)
+        {
+            float r = w->shapeGeom[s].s;
+            m3Vec3 d = m3Sub3(w->shapeGeom[s].v, center);
+            minExtent = m3MinF(minExtent, r);
+            maxExtent = m3MaxF(maxExtent, sqrtf(m3Dot3(d, d)) + r);
+        }
+        else if (type == M3_CAPSULE)
+        {
+            float r = w->shapeGeom[s].s;
+            m3Vec3 d1 = m3Sub3(w->shapeGeom[s].v, center);
+            m3Vec3 d2 = m3Sub3(w->shapeGeom[s].v2, center);
+            minExtent = m3MinF(minExtent, r);
+            maxExtent = m3MaxF(maxExtent, sqrtf(m3MaxF(m3Dot3(d1, d1), m3Dot3(d2, d2))) + r);
+        }
+        else if (type == M3_HULL)
+        {
+            const m3HullData* hull = &w->hullData[w->shapeHullIndex[s]];
+            for (int32_t f = 0; f < hull->faceCount; ++f)
+            {
+                float dist = hull->faceOffsets[f] - m3Dot3(hull->faceNormals[f], center);
+                minExtent = m3MinF(minExtent, dist);
+            }
+            for (int32_t v = 0; v < hull->vertexCount; ++v)
+            {
+                m3Vec3 d = m3Sub3(hull->vertices[v], center);
+                maxExtent = m3MaxF(maxExtent, sqrtf(m3Dot3(d, d)));
+            }
+        }
+    }
+    w->minExtents[b] = minExtent;
+    w->maxExtents[b] = maxExtent;
+    w->localCenters[b] = center;
+}
+
+// ---------------------------------------------------------------------------
+// Shape bounds on the host (creation-time proxy box; the device recomputes
+// them every step with the same arithmetic).  src/broad_phase.c:24-148.
+// ---------------------------------------------------------------------------
+
+static m3Transform ShapeWorldTransform(const m3World* w, int32_t s)
+{
+    m3Transform xf = w->transforms[w->shapeBody[s]];
+    if (w->shapeHasOffset[s] != 0)
+    {
+        m3Vec3 r = m3RotateVec3(xf.q, w->shapeLocalPos[s]);
+        xf.p.x += (double)r.x;
+        xf.p.y += (double)r.y;
+        xf.p.z += (double)r.z;
+        xf.q = m3MulQuat(xf.q, w->shapeLocalRot[s]);
+    }
+    return xf;
+}
+
+static void ShapeBounds(const m3World* w, int32_t s, double box[6])
+{
+    m3Transform xf = ShapeWorldTransform(w, s);
+    uint8_t type = w->shapeType[s];
+    if (type == M3_HULL)
+    {
+        const m3HullData* hull = &w->hullData[w->shapeHullIndex[s]];
+        double lo[3] = {1.0e30, 1.0e30, 1.0e30};
+        double hi[3] = {-1.0e30, -1.0e30, -1.0e30};
+        for (int32_t v = 0; v < hull->vertexCount; ++v)
+        {
+            m3Vec3 r = m3RotateVec3(xf.q, hull->vertices[v]);
+            double p[3] = {xf.p.x + (double)r.x, xf.p.y + (double)r.y, xf.p.z + (double)r.z};
+            for (int32_t k = 0; k < 3; ++k)
+            {
+                lo[k] = p[k] < lo[k] ? p[k] : lo[k];
+                hi[k] = p[k] > hi[k] ? p[k] : hi[k];
+            }
+        }
+        for (int32_t k = 0; k < 3; ++k)
+        {
+            box[k] = lo[k] - (double)M3_AABB_MARGIN;
+            box[3 + k] = hi[k] + (double)M3_AABB_MARGIN;
+        }
+        return;
+    }
+    if (type == M3_CAPSULE)
+    {
+        m3Vec3 r1 = m3RotateVec3(xf.q, w->shapeGeom[s].v);
+        m3Vec3 r2 = m3RotateVec3(xf.q, w->shapeGeom[s].v2);
+        double c1[3] = {xf.p.x + (double)r1.x, xf.p.y + (double)r1.y, xf.p.z + (double)r1.z};
+        double c2[3] = {xf.p.x + (double)r2.x, xf.p.y + (double)r2.y, xf.p.z + (double)r2.z};
+        double fat = (double)(w->shapeGeom[s].s + M3_AABB_MARGIN);
+        for (int32_t k = 0; k < 3; ++k)
+        {
+            box[k] = (c1[k] < c2[k] ? c1[k] : c2[k]) - fat;
+            box[3 + k] = (c1[k] > c2[k] ? c1[k] : c2[k]) + fat;
+        }
+        return;
+    }
+    m3Vec3 r = m3RotateVec3(xf.q, w->shapeGeom[s].v);
+    double c[3] = {xf.p.x + (double)r.x, xf.p.y + (double)r.y, xf.p.z + (double)r.z};
+    double fat = (double)(w->shapeGeom[s].s + M3_AABB_MARGIN);
+    for (int32_t k = 0; k < 3; ++k)
+    {
+        box[k] = c[k] - fat;
+        box[3 + k] = c[k] + fat;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shapes (src/shape.c:411-625, 716-823, 1334-1344)
+// ---------------------------------------------------------------------------
+
+m3ShapeDef m3DefaultShapeDef(void)
+{
+    m3ShapeDef def;
+    memset(&def, 0, sizeof(def));
+    def.density = 1.0f;
+    def.friction = 0.6f;
+    def.restitution = 0.0f;
+    def.categoryBits = 1ull;
+    def.maskBits = ~0ull;
+    def.groupIndex = 0;
+    def.localRotation = m3IdentityQuat();
+    def.internalValue = M3_SHAPE_COOKIE;
+    return def;
+}
+
+static m3ShapeId CreateShape(m3BodyId bodyId, const m3ShapeDef* def, uint8_t type,
+                             const m3ShapeGeom* geom)
+{
+    m3ShapeId nullId = {0, 0, 0};
+    m3World* w = WorldFromIndex0(bodyId.world0);
+    int32_t b = w != NULL ? BodySlot(w, bodyId) : -1;
+    if (b < 0 || def == NULL || def->internalValue != M3_SHAPE_COOKIE)
+    {
+        return nullId;
+    }
+    float rotLen2 = def->localRotation.x * def->localRotation.x +
+                    def->localRotation.y * def->localRotation.y +
+                    def->localRotation.z * def->localRotation.z +
+                    def->localRotation.w * def->localRotation.w;
+    if (!FiniteV3(def->localPosition) || !FiniteQ(def->localRotation) || rotLen2 < 0.99f ||
+        rotLen2 > 1.01f || !FiniteF(def->density) || !(def->density > 0.0f) ||
+        !FiniteF(def->friction) || def->friction < 0.0f || !FiniteF(def->restitution) ||
+        def->restitution < 0.0f || !FiniteF(def->rollingResistance) ||
+        def->rollingResistance < 0.0f)
+    {
+        return nullId;
+    }
+    Touch(w, M3GPU_FIELDS_ALL);
+    int32_t s = PoolAlloc(&w->shapePool);
+    if (s < 0)
+    {
+        return nullId;
+    }
+    w->shapeBody[s] = b;
+    w->shapeType[s] = type;
+    w->shapeGeom[s] = *geom;
+    w->shapeDensity[s] = def->density;
+    w->shapeFriction[s] = def->friction;
+    w->shapeRestitution[s] = def->restitution;
+    w->shapeRollingResistance[s] = def->rollingResistance;
+    w->shapeCategory[s] = def->categoryBits;
+    w->shapeMask[s] = def->maskBits;
+    w->shapeGroup[s] = def->groupIndex;
+    w->shapeUserData[s] = def->userData;
+    w->shapeSensor[s] = def->isSensor ? 1 : 0;
+    w->shapeHitEvents[s] = def->enableHitEvents ? 1 : 0;
+    w->shapePreSolve[s] = def->enablePreSolveEvents ? 1 : 0;
+    w->shapeSurfaceVel[s] = m3V3(0.0f, 0.0f, 0.0f);
+    w->shapeLocalPos[s] = def->localPosition;
+    w->shapeLocalRot[s] = def->localRotation;
+    w->shapeHasOffset[s] =
+        (def->localPosition.x != 0.0f || def->localPosition.y != 0.0f ||
+         def->localPosition.z != 0.0f || def->localRotation.x != 0.0f ||
+         def->localRotation.y != 0.0f || def->localRotation.z != 0.0f ||
+         def->localRotation.w != 1.0f)
+            ? 1
+            : 0;
+    w->shapeNext[s] = w->bodyShapeHead[b];
+    w->bodyShapeHead[b] = s;
+    w->shapeHullIndex[s] = -1;
+    w->shapeMeshIndex[s] = -1;
+    if (type == M3_HULL)
+    {
+        m3HullData data;
+        BuildBoxHull(&data, geom->v);
+        w->shapeHullIndex[s] = InternHull(w, &data);
+        if (w->shapeHullIndex[s] < 0)
+        {
+            w->bodyShapeHead[b] = w->shapeNext[s];
+            w->shapeNext[s] = -1;
+            w->shapeBody[s] = -1;
+            PoolFree(&w->shapePool, s);
+            return nullId;
+        }
+    }
+    // Non-plane shapes get a proxy whose stored box starts as the tight
+    // (margin-inflated) bounds, like a fresh tree leaf.
+    if (type != M3_PLANE)
+    {
+        ShapeBounds(w, s, &w->leafBoxes[6 * s]);
+    }
+    RecomputeMass(w, b);
+    m3ShapeId id = {s + 1, w->worldIndex0, w->shapePool.generations[s]};
+    return id;
+}
+
+m3ShapeId m3CreateSphereShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Sphere* sphere)
+{
+    m3ShapeId nullId = {0, 0, 0};
+    if (sphere == NULL || !(sphere->radius > 0.0f) || !FiniteF(sphere->radius) ||
+        !FiniteV3(sphere->center))
+    {
+        return nullId;
+    }
+    m3ShapeGeom geom;
+    memset(&geom, 0, sizeof(geom));
+    geom.v = sphere->center;
+    geom.s = sphere->radius;
+    return CreateShape(bodyId, def, M3_SPHERE, &geom);
+}
+
+m3ShapeId m3CreatePlaneShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Plane* plane)
+{
+    m3ShapeId nullId = {0, 0, 0};
+    if (plane == NULL || !FiniteV3(plane->normal) || !FiniteF(plane->offset) ||
+        !(m3Dot3(plane->normal, plane->normal) > 1.0e-12f))
+    {
+        return nullId;
+    }
+    m3World* w = WorldFromIndex0(bodyId.world0);
+    int32_t b = w != NULL ? BodySlot(w, bodyId) : -1;
+    if (b < 0 || w->types[b] != M3_STATIC)
+    {
+        return nullId;
+    }
+    m3ShapeGeom geom;
+    memset(&geom, 0, sizeof(geom));
+    geom.v = m3Normalize3(plane->normal);
+    geom.s = plane->offset;
+    return CreateShape(bodyId, def, M3_PLANE, &geom);
+}
+
+m3ShapeId m3CreateCapsuleShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Capsule* capsule)
+{
+    m3ShapeId nullId = {0, 0, 0};
+    if (capsule == NULL || !(capsule->radius > 0.0f) || !FiniteF(capsule->radius) ||
+        !FiniteV3(capsule->point1) || !FiniteV3(capsule->point2))
+    {
+        return nullId;
+    }
+    m3Vec3 axis = m3Sub3(capsule->point2, capsule->point1);
+    if (!(m3Dot3(axis, axis) > 0.0f))
+    {
+        return nullId;
+    }
+    m3ShapeGeom geom;
+    memset(&geom, 0, sizeof(geom));
+    geom.v = capsule->point1;
+    geom.s = capsule->radius;
+    geom.v2 = capsule->point2;
+    return CreateShape(bodyId, def, M3_CAPSULE, &geom);
+}
+
+m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def, m3Vec3 halfExtents)
+{
+    m3ShapeId nullId = {0, 0, 0};
+    if (!(halfExtents.x > 0.0f) || !(halfExtents.y > 0.0f) || !(halfExtents.z > 0.0f) ||
+        !FiniteV3(halfExtents))
+    {
+        return nullId;
+    }
+    m3ShapeGeom geom;
+    memset(&geom, 0, sizeof(geom));
+    geom.v = halfExtents;
+    return CreateShape(bodyId, def, M3_HULL, &geom);
+}
+
+// ---------------------------------------------------------------------------
+// Step (src/solver.c:4322-4330 wall; the body runs on the device)
+// ---------------------------------------------------------------------------
+
+static int EnsureDevice(m3World* w)
+{
+    if (w->dev == NULL)
+    {
+        m3gpuCaps caps;
+        caps.bodyCapacity = w->bodyCapacity;
+        caps.shapeCapacity = w->shapeCapacity;
+        caps.pairCapacity = w->pairCapacity;
+        caps.hullCapacity = w->shapeCapacity;
+        w->dev = m3gpu_world_create(&caps);
+        if (w->dev == NULL)
+        {
+            DeviceFail("m3gpu_world_create");
+        }
+        w->devDirty = M3GPU_FIELDS_ALL;
+    }
+    if (w->devDirty != 0)
+    {
+        m3gpuHostState h;
+        HostView(w, &h);
+        if (m3gpu_upload(w->dev, &h, w->devDirty) != 0)
+        {
+            DeviceFail("m3gpu_upload");
+        }
+        w->devDirty = 0;
+    }
+    return 1;
+}
+
+static void FillParams(const m3World* w, float dt, int32_t substeps, m3gpuStepParams* p)
+{
+    memset(p, 0, sizeof(*p));
+    p->dt = dt;
+    p->substeps = substeps;
+    p->gravity[0] = w->gravity.x;
+    p->gravity[1] = w->gravity.y;
+    p->gravity[2] = w->gravity.z;
+    p->contactHertz = w->contactHertz;
+    p->contactDampingRatio = w->contactDampingRatio;
+    p->contactPushMaxSpeed = w->contactPushMaxSpeed;
+    p->restitutionThreshold = w->restitutionThreshold;
+    p->maximumLinearSpeed = w->maximumLinearSpeed;
+    p->maximumAngularSpeed = w->maximumAngularSpeed;
+    p->sleepEnabled = w->sleepEnabled;
+    p->continuousEnabled = w->continuousEnabled;
+}
+
+void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || !(dt > 0.0f) || substeps < 1 || substeps > M3_MAX_SUBSTEPS)
+    {
+        return;
+    }
+    EnsureDevice(w);
+    m3gpuStepParams p;
+    FillParams(w, dt, substeps, &p);
+    int rc = m3gpu_step(w->dev, &p, NULL);
+    if (rc < 0)
+    {
+        DeviceFail("m3gpu_step");
+    }
+    w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
+}
+
+// Batched stepping of many worlds with one call: the rollback and toss
+// workloads.  Not in the reference API (its worlds step one by one).
+M3_API int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt,
+                                int32_t substeps);
+int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32_t substeps)
+{
+    if (worlds == NULL || count <= 0 || !(dt > 0.0f) || substeps < 1 ||
+        substeps > M3_MAX_SUBSTEPS)
+    {
+        return -1;
+    }
+    enum { CHUNK = 1024 };
+    m3gpuWorld* devs[CHUNK];
+    m3gpuStepParams params[CHUNK];
+    int32_t n = 0;
+    for (int32_t k = 0; k < count && n < CHUNK; ++k)
+    {
+        m3World* w = WorldFromId(worlds[k]);
+        if (w == NULL)
+        {
+            continue;
+        }
+        EnsureDevice(w);
+        devs[n] = w->dev;
+        FillParams(w, dt, substeps, &params[n]);
+        n += 1;
+        w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
+    }
+    int rc = m3gpu_step_many(devs, params, n);
+    if (rc < 0)
+    {
+        DeviceFail("m3gpu_step_many");
+    }
+    return n;
+}
+
+// Waits for the world's queued device work (for timing harnesses).
+M3_API void m3World_Sync(m3WorldId worldId);
+void m3World_Sync(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w != NULL && w->dev != NULL)
+    {
+        m3gpu_sync(w->dev);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hash (src/snapshot.c:932-1390, the branches live for this build's state)
+// ---------------------------------------------------------------------------
+
+uint64_t m3World_Hash(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return 0;
+    }
+    SyncHost(w);
+    uint64_t h = M3_HASH_INIT;
+    h = m3HashBytes(h, &w->stepCount, 8);
+    h = m3HashBytes(h, &w->gravity, (int32_t)sizeof(m3Vec3));
+    if (w->contactHertz != M3_CONTACT_HERTZ_DEFAULT ||
+        w->contactDampingRatio != M3_CONTACT_DAMPING_RATIO_DEFAULT ||
+        w->contactPushMaxSpeed != M3_CONTACT_PUSH_MAX_SPEED_DEFAULT ||
+        w->restitutionThreshold != M3_RESTITUTION_THRESHOLD_DEFAULT ||
+        w->maximumLinearSpeed != M3_MAX_LINEAR_SPEED_DEFAULT || w->sleepEnabled == 0 ||
+        w->continuousEnabled == 0 || w->hitEventThreshold != M3_HIT_EVENT_THRESHOLD_DEFAULT)
+    {
+        h = m3HashBytes(h, &w->contactHertz, 4);
+        h = m3HashBytes(h, &w->contactDampingRatio, 4);
+        h = m3HashBytes(h, &w->contactPushMaxSpeed, 4);
+        h = m3HashBytes(h, &w->restitutionThreshold, 4);
+        h = m3HashBytes(h, &w->maximumLinearSpeed, 4);
+        h = m3HashBytes(h, &w->sleepEnabled, 1);
+        h = m3HashBytes(h, &w->continuousEnabled, 1);
+        h = m3HashBytes(h, &w->hitEventThreshold, 4);
+    }
+    if (w->maximumAngularSpeed != M3_MAX_ANGULAR_SPEED_DEFAULT)
+    {
+        h = m3HashBytes(h, &w->maximumAngularSpeed, 4);
+    }
+    for (int32_t i = 0; i < w->bodyPool.maxIndex; ++i)
+    {
+        uint8_t alive = w->bodyPool.alive[i];
+        h = m3HashBytes(h, &alive, 1);
+        if (alive == 0)
+        {
+            continue;
+        }
+        h = m3HashBytes(h, &w->transforms[i], (int32_t)sizeof(m3Transform));
+        h = m3HashBytes(h, &w->linearVelocities[i], (int32_t)sizeof(m3Vec3));
+        if (w->bodyForce[i].x != 0.0f || w->bodyForce[i].y != 0.0f || w->bodyForce[i].z != 0.0f ||
+            w->bodyTorque[i].x != 0.0f || w->bodyTorque[i].y != 0.0f ||
+            w->bodyTorque[i].z != 0.0f)
+        {
+            h = m3HashBytes(h, &w->bodyForce[i], (int32_t)sizeof(m3Vec3));
+            h = m3HashBytes(h, &w->bodyTorque[i], (int32_t)sizeof(m3Vec3));
+        }
+        if (w->bodyEnabled[i] == 0 || w->bodyLocks[i] != 0 ||
+            w->bodySleepThreshold[i] != M3_SLEEP_VELOCITY_DEFAULT || w->bodyCanSleep[i] == 0 ||
+            w->bodyHasTarget[i] != 0)
+        {
+            h = m3HashBytes(h, &w->bodyEnabled[i], 1);
+            h = m3HashBytes(h, &w->bodyLocks[i], 1);
+            h = m3HashBytes(h, &w->bodySleepThreshold[i], 4);
+            h = m3HashBytes(h, &w->bodyCanSleep[i], 1);
+            h = m3HashBytes(h, &w->bodyHasTarget[i], 1);
+            h = m3HashBytes(h, &w->bodyTarget[i], (int32_t)sizeof(m3Transform));
+        }
+        h = m3HashBytes(h, &w->angularVelocities[i], (int32_t)sizeof(m3Vec3));
+        h = m3HashBytes(h, &w->invMass[i], 4);
+        h = m3HashBytes(h, &w->invInertiaLocal[i], (int32_t)sizeof(m3Mat3));
+        h = m3HashBytes(h, &w->localCenters[i], (int32_t)sizeof(m3Vec3));
+        h = m3HashBytes(h, &w->types[i], 1);
+        h = m3HashBytes(h, &w->bulletFlags[i], 1);
+        h = m3HashBytes(h, &w->awake[i], 1);
+        h = m3HashBytes(h, &w->sleepTimes[i], 4);
+    }
+    for (int32_t i = 0; i < w->shapePool.maxIndex; ++i)
+    {
+        uint8_t alive = w->shapePool.alive[i];
+        h = m3HashBytes(h, &alive, 1);
+        if (alive == 0)
+        {
+            continue;
+        }
+        h = m3HashBytes(h, &w->shapeBody[i], 4);
+        h = m3HashBytes(h, &w->shapeType[i], 1);
+        h = m3HashBytes(h, &w->shapeGeom[i], (int32_t)sizeof(m3ShapeGeom));
+        h = m3HashBytes(h, &w->shapeDensity[i], 4);
+        h = m3HashBytes(h, &w->shapeFriction[i], 4);
+        h = m3HashBytes(h, &w->shapeRestitution[i], 4);
+        if (w->shapeHitEvents[i] != 0 || w->shapePreSolve[i] != 0)
+        {
+            h = m3HashBytes(h, &w->shapeHitEvents[i], 1);
+            h = m3HashBytes(h, &w->shapePreSolve[i], 1);
+        }
+        if (w->shapeSurfaceVel[i].x != 0.0f || w->shapeSurfaceVel[i].y != 0.0f ||
+            w->shapeSurfaceVel[i].z != 0.0f)
+        {
+            h = m3HashBytes(h, &w->shapeSurfaceVel[i], (int32_t)sizeof(m3Vec3));
+        }
+        if (w->shapeHasOffset[i] != 0)
+        {
+            h = m3HashBytes(h, &w->shapeLocalPos[i], (int32_t)sizeof(m3Vec3));
+            h = m3HashBytes(h, &w->shapeLocalRot[i], (int32_t)sizeof(m3Quat));
+        }
+        if (w->shapeRollingResistance[i] != 0.0f)
+        {
+            h = m3HashBytes(h, &w->shapeRollingResistance[i], 4);
+        }
+        if (w->shapeCategory[i] != 1ull || w->shapeMask[i] != ~0ull || w->shapeGroup[i] != 0)
+        {
+            h = m3HashBytes(h, &w->shapeCategory[i], 8);
+            h = m3HashBytes(h, &w->shapeMask[i], 8);
+            h = m3HashBytes(h, &w->shapeGroup[i], 4);
+        }
+        h = m3HashBytes(h, &w->shapeHullIndex[i], 4);
+        h = m3HashBytes(h, &w->shapeMeshIndex[i], 4);
+        h = m3HashBytes(h, &w->shapeSensor[i], 1);
+    }
+    // Character, soft-body, vehicle and voxel pools are empty here, so
+    // their blocks contribute nothing; joints likewise while the pool is
+    // empty.
+    h = m3HashBytes(h, &w->pairCount, 4);
+    for (int32_t i = 0; i < w->pairCount; ++i)
+    {
+        h = m3HashBytes(h, &w->pairKeys[i], 8);
+        h = m3HashBytes(h, &w->manifolds[i], (int32_t)sizeof(m3Manifold));
+    }
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Counters and profile
+// ---------------------------------------------------------------------------
+
+static void PullStats(m3World* w)
+{
+    if (w->dev == NULL)
+    {
+        return;
+    }
+    m3gpuStepStats st;
+    if (m3gpu_last_stats(w->dev, &st) != 0)
+    {
+        return;
+    }
+    w->lastIslandCount = st.islandCount;
+    w->lastColorCount = st.colorCount;
+    w->lastAwakeCount = st.awakeCount;
+    w->lastContactCount = st.constraintCount;
+    memset(&w->profile, 0, sizeof(w->profile));
+    w->profile.step = st.msStep;
+    w->profile.broadphase = st.msBroadphase;
+    w->profile.narrowphase = st.msNarrowphase;
+    w->profile.prepare = st.msPrepare;
+    w->profile.solve = st.msSolve;
+    w->profile.continuous = st.msContinuous;
+    w->profile.sleep = st.msSleep;
+}
+
+m3Counters m3World_GetCounters(m3WorldId worldId)
+{
+    m3Counters c;
+    memset(&c, 0, sizeof(c));
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return c;
+    }
+    SyncHost(w);
+    PullStats(w);
+    for (int32_t i = 0; i < w->bodyPool.maxIndex; ++i)
+    {
+        c.bodyCount += w->bodyPool.alive[i];
+        if (w->bodyPool.alive[i] != 0 && w->types[i] == M3_DYNAMIC && w->awake[i] != 0)
+        {
+            c.awakeCount += 1;
+        }
+    }
+    for (int32_t i = 0; i < w->shapePool.maxIndex; ++i)
+    {
+        c.shapeCount += w->shapePool.alive[i];
+    }
+    for (int32_t i = 0; i < w->hullPool.maxIndex; ++i)
+    {
+        c.hullCount += w->hullPool.alive[i];
+    }
+    for (int32_t i = 0; i < w->pairCount; ++i)
+    {
+        c.contactCount += w->manifolds[i].pointCount > 0 ? 1 : 0;
+    }
+    c.islandCount = w->lastIslandCount;
+    c.colorCount = w->lastColorCount;
+    c.snapshotBytes = m3World_SnapshotSize(worldId);
+    return c;
+}
+
+m3Profile m3World_GetProfile(m3WorldId worldId)
+{
+    m3Profile p;
+    memset(&p, 0, sizeof(p));
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return p;
+    }
+    PullStats(w);
+    return w->profile;
+}
+
+// ---------------------------------------------------------------------------
+// Snapshot / restore.  This build's own block format (the reference's
+// includes its dynamic tree, which this build replaces by per-shape proxy
+// boxes): a header, then fixed-order blocks sized by the pool high-water
+// marks.  Restore validates every size before it writes anything.
+// ---------------------------------------------------------------------------
+
+#define M3_SNAP_MAGIC 0x31414D33u // "3MA1"
+#define M3_SNAP_VERSION 1u
+
+typedef struct m3SnapHeader
+{
+    uint32_t magic;
+    uint32_t version;
+    uint32_t recordSizes; // sizeof(m3Manifold) << 16 | sizeof(m3Transform)
+    uint32_t hullSize;
+    int32_t bodyCapacity;
+    int32_t shapeCapacity;
+    int32_t pairCapacity;
+    int32_t bodyMax;
+    int32_t shapeMax;
+    int32_t hullMax;
+    int32_t pairCount;
+    int32_t sleepingPairCount;
+    uint64_t stepCount;
+    uint64_t payloadHash;
+} m3SnapHeader;
+
+typedef struct m3Walker
+{
+    uint8_t* out;       // write pass when non-null
+    const uint8_t* in;  // read pass when non-null
+    int64_t cursor;
+    int64_t limit;
+    int failed;
+} m3Walker;
+
+static void Walk(m3Walker* k, void* field, int64_t bytes)
+{
+    if (k->failed || bytes <= 0)
+    {
+        return;
+    }
+    if (k->limit >= 0 && k->cursor + bytes > k->limit)
+    {
+        k->failed = 1;
+        return;
+    }
+    if (k->out != NULL)
+    {
+        memcpy(k->out + k->cursor, field, (size_t)bytes);
+    }
+    else if (k->in != NULL && field != NULL)
+    {
+        memcpy(field, k->in + k->cursor, (size_t)bytes);
+    }
+    k->cursor += bytes;
+}
+
+static void WalkPool(m3Walker* k, m3IdPool* p)
+{
+    Walk(k, &p->maxIndex, 4);
+    Walk(k, &p->freeHead, 4);
+    Walk(k, &p->freeCount, 4);
+    Walk(k, &p->retiredCount, 4);
+    Walk(k, p->generations, (int64_t)p->capacity * 2);
+    Walk(k, p->alive, p->capacity);
+    Walk(k, p->freeQueue, (int64_t)p->capacity * 4);
+}
+
+// One walk for measure, write and read.  Counts come from the world (write)
+// or the validated header (read); either way the same order.
+static void WalkWorld(m3Walker* k, m3World* w, int32_t nb, int32_t ns, int32_t nh, int32_t np,
+                      int32_t nz)
+{
+    Walk(k, &w->gravity, sizeof(m3Vec3));
+    Walk(k, &w->contactHertz, 4);
+    Walk(k, &w->contactDampingRatio, 4);
+    Walk(k, &w->contactPushMaxSpeed, 4);
+    Walk(k, &w->restitutionThreshold, 4);
+    Walk(k, &w->maximumLinearSpeed, 4);
+    Walk(k, &w->maximumAngularSpeed, 4);
+    Walk(k, &w->sleepEnabled, 1);
+    Walk(k, &w->continuousEnabled, 1);
+    Walk(k, &w->hitEventThreshold, 4);
+    WalkPool(k, &w->bodyPool);
+    WalkPool(k, &w->shapePool);
+    WalkPool(k, &w->hullPool);
+    Walk(k, w->transforms, (int64_t)nb * sizeof(m3Transform));
+    Walk(k, w->linearVelocities, (int64_t)nb * sizeof(m3Vec3));
+    Walk(k, w->angularVelocities, (int64_t)nb * sizeof(m3Vec3));
+    Walk(k, w->invMass, (int64_t)nb * 4);
+    Walk(k, w->invInertiaLocal, (int64_t)nb * sizeof(m3Mat3));
+    Walk(k, w->inertiaLocal, (int64_t)nb * sizeof(m3Mat3));
+    Walk(k, w->localCenters, (int64_t)nb * sizeof(m3Vec3));
+    Walk(k, w->gravityScales, (int64_t)nb * 4);
+    Walk(k, w->linearDamping, (int64_t)nb * 4);
+    Walk(k, w->angularDamping, (int64_t)nb * 4);
+    Walk(k, w->types, nb);
+    Walk(k, w->awake, nb);
+    Walk(k, w->sleepTimes, (int64_t)nb * 4);
+    Walk(k, w->bulletFlags, nb);
+    Walk(k, w->minExtents, (int64_t)nb * 4);
+    Walk(k, w->maxExtents, (int64_t)nb * 4);
+    Walk(k, w->userData, (int64_t)nb * 8);
+    Walk(k, w->bodyForce, (int64_t)nb * sizeof(m3Vec3));
+    Walk(k, w->bodyTorque, (int64_t)nb * sizeof(m3Vec3));
+    Walk(k, w->bodyEnabled, nb);
+    Walk(k, w->bodyLocks, nb);
+    Walk(k, w->bodySleepThreshold, (int64_t)nb * 4);
+    Walk(k, w->bodyCanSleep, nb);
+    Walk(k, w->bodyHasTarget, nb);
+    Walk(k, w->bodyTarget, (int64_t)nb * sizeof(m3Transform));
+    Walk(k, w->bodyShapeHead, (int64_t)nb * 4);
+    Walk(k, w->shapeBody, (int64_t)ns * 4);
+    Walk(k, w->shapeType, ns);
+    Walk(k, w->shapeGeom, (int64_t)ns * sizeof(m3ShapeGeom));
+    Walk(k, w->shapeDensity, (int64_t)ns * 4);
+    Walk(k, w->shapeFriction, (int64_t)ns * 4);
+    Walk(k, w->shapeRestitution, (int64_t)ns * 4);
+    Walk(k, w->shapeHitEvents, ns);
+    Walk(k, w->shapeLocalPos, (int64_t)ns * sizeof(m3Vec3));
+    Walk(k, w->shapeLocalRot, (int64_t)ns * sizeof(m3Quat));
+    Walk(k, w->shapeHasOffset, ns);
+    Walk(k, w->shapeSurfaceVel, (int64_t)ns * sizeof(m3Vec3));
+    Walk(k, w->shapePreSolve, ns);
+    Walk(k, w->shapeRollingResistance, (int64_t)ns * 4);
+    Walk(k, w->shapeCategory, (int64_t)ns * 8);
+    Walk(k, w->shapeMask, (int64_t)ns * 8);
+    Walk(k, w->shapeGroup, (int64_t)ns * 4);
+    Walk(k, w->shapeUserData, (int64_t)ns * 8);
+    Walk(k, w->shapeNext, (int64_t)ns * 4);
+    Walk(k, w->shapeHullIndex, (int64_t)ns * 4);
+    Walk(k, w->shapeMeshIndex, (int64_t)ns * 4);
+    Walk(k, w->shapeSensor, ns);
+    Walk(k, w->leafBoxes, (int64_t)ns * 6 * 8);
+    Walk(k, w->hullData, (int64_t)nh * sizeof(m3HullData));
+    Walk(k, w->hullRefCounts, (int64_t)nh * 4);
+    Walk(k, w->pairKeys, (int64_t)np * 8);
+    Walk(k, w->manifolds, (int64_t)np * sizeof(m3Manifold));
+    Walk(k, w->sleepingPairKeys, (int64_t)nz * 8);
+    Walk(k, &w->frozenDirty, 1);
+}
+
+int32_t m3World_SnapshotSize(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return -1;
+    }
+    SyncHost(w);
+    m3Walker k;
+    memset(&k, 0, sizeof(k));
+    k.limit = -1;
+    WalkWorld(&k, w, w->bodyPool.maxIndex, w->shapePool.maxIndex, w->hullPool.maxIndex,
+              w->pairCount, w->sleepingPairCount);
+    int64_t total = (int64_t)sizeof(m3SnapHeader) + k.cursor;
+    return total > 0x7FFFFFFF ? -1 : (int32_t)total;
+}
+
+int32_t m3World_Snapshot(m3WorldId worldId, void* out, int32_t capacity)
+{
+    m3World* w = WorldFromId(worldId);
+    int32_t size = m3World_SnapshotSize(worldId);
+    if (w == NULL || out == NULL || size < 0 || capacity < size)
+    {
+        return -1;
+    }
+    m3SnapHeader hdr;
+    memset(&hdr, 0, sizeof(hdr));
+    hdr.magic = M3_SNAP_MAGIC;
+    hdr.version = M3_SNAP_VERSION;
+    hdr.recordSizes = ((uint32_t)sizeof(m3Manifold) << 16) | (uint32_t)sizeof(m3Transform);
+    hdr.hullSize = (uint32_t)sizeof(m3HullData);
+    hdr.bodyCapacity = w->bodyCapacity;
+    hdr.shapeCapacity = w->shapeCapacity;
+    hdr.pairCapacity = w->pairCapacity;
+    hdr.bodyMax = w->bodyPool.maxIndex;
+    hdr.shapeMax = w->shapePool.maxIndex;
+    hdr.hullMax = w->hullPool.maxIndex;
+    hdr.pairCount = w->pairCount;
+    hdr.sleepingPairCount = w->sleepingPairCount;
+    hdr.stepCount = w->stepCount;
+    m3Walker k;
+    memset(&k, 0, sizeof(k));
+    k.out = (uint8_t*)out + sizeof(hdr);
+    k.limit = size - (int64_t)sizeof(hdr);
+    WalkWorld(&k, w, hdr.bodyMax, hdr.shapeMax, hdr.hullMax, hdr.pairCount, hdr.sleepingPairCount);
+    hdr.payloadHash = m3HashBytes(M3_HASH_INIT, (uint8_t*)out + sizeof(hdr), size - (int32_t)sizeof(hdr));
+    memcpy(out, &hdr, sizeof(hdr));
+    return size;
+}
+
+bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || data == NULL || size < (int32_t)sizeof(m3SnapHeader))
+    {
+        return false;
+    }
+    m3SnapHeader hdr;
+    memcpy(&hdr, data, sizeof(hdr));
+    if (hdr.magic != M3_SNAP_MAGIC || hdr.version != M3_SNAP_VERSION ||
+        hdr.recordSizes != (((uint32_t)sizeof(m3Manifold) << 16) | (uint32_t)sizeof(m3Transform)) ||
+        hdr.hullSize != (uint32_t)sizeof(m3HullData) || hdr.bodyCapacity != w->bodyCapacity ||
+        hdr.shapeCapacity != w->shapeCapacity || hdr.pairCapacity != w->pairCapacity ||
+        hdr.bodyMax < 0 || hdr.bodyMax > w->bodyCapacity || hdr.shapeMax < 0 ||
+        hdr.shapeMax > w->shapeCapacity || hdr.hullMax < 0 || hdr.hullMax > w->shapeCapacity ||
+        hdr.pairCount < 0 || hdr.pairCount > w->pairCapacity || hdr.sleepingPairCount < 0 ||
+        hdr.sleepingPairCount > w->pairCapacity)
+    {
+        return false;
+    }
+    if (m3HashBytes(M3_HASH_INIT, (const uint8_t*)data + sizeof(hdr),
+                    size - (int32_t)sizeof(hdr)) != hdr.payloadHash)
+    {
+        return false;
+    }
+    // Measure with the header's counts: the payload must be exactly that.
+    m3Walker k;
+    memset(&k, 0, sizeof(k));
+    k.limit = -1;
+    WalkWorld(&k, w, hdr.bodyMax, hdr.shapeMax, hdr.hullMax, hdr.pairCount, hdr.sleepingPairCount);
+    if (k.cursor != (int64_t)size - (int64_t)sizeof(hdr))
+    {
+        return false;
+    }
+    // Validation passed: now write.  Pending device results are dropped.
+    w->hostStale = 0;
+    memset(&k, 0, sizeof(k));
+    k.in = (const uint8_t*)data + sizeof(hdr);
+    k.limit = size - (int64_t)sizeof(hdr);
+    WalkWorld(&k, w, hdr.bodyMax, hdr.shapeMax, hdr.hullMax, hdr.pairCount, hdr.sleepingPairCount);
+    w->pairCount = hdr.pairCount;
+    w->sleepingPairCount = hdr.sleepingPairCount;
+    w->stepCount = hdr.stepCount;
+    // The frozen buffer is rebuilt from the sleeping state on the next step
+    // (src/snapshot.c:893-894).
+    w->pairsFullQuery = 1;
+    w->devDirty = M3GPU_FIELDS_ALL;
+    return true;
+}
