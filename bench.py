@@ -1,0 +1,163 @@
+"""Benchmark: body-substeps/s of the soft-step contact solver on one MI355X.
+
+Workload: the 10k-body mixed rain, one world (10,000 dynamic bodies: 40%
+boxes, 40% spheres, 20% capsules, seeded), m3World_Step(1/60, 4).  A "step"
+is one m3World_Step.  The default run times the scene's full 600-step
+trajectory (fall, pile-up, settle) with the world resident in HBM; the
+final world hash is checked against the committed reference fixture.
+
+Prints one JSON line.  ``roofline`` reports the dominant kernel's
+algorithmic HBM bytes over its event-timed average duration; ``cpu_baseline``
+times the reference engine (built from its C sources into oracle/_ref) on
+the host cores over a bounded prefix of the same trajectory.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DT = 1.0 / 60.0
+SUBSTEPS = 4
+BODIES = 10000
+HBM_PEAK_GBS = 8000.0
+
+
+def parse() -> argparse.Namespace:
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=600)
+    p.add_argument("--warmup", type=int, default=0)
+    p.add_argument("--cpu-steps", type=int, default=150, help="reference steps timed on the host")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-roofline", action="store_true")
+    return p.parse_args()
+
+
+def golden_hash(step: int) -> str | None:
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "hashes.json")) as f:
+            data = json.load(f)
+        return data["scenes"]["rain"]["hashes"].get(str(step))
+    except OSError:
+        return None
+
+
+def run_gpu(steps: int, warmup: int) -> dict:
+    import maul3d_amd
+    from maul3d_amd import scenes
+
+    gpu = maul3d_amd.get_api()
+    lib = maul3d_amd.library()
+    world = scenes.rain(gpu, count=BODIES)
+    lib.m3World_Sync(world)  # upload: the world is resident before timing
+    for _ in range(warmup):
+        gpu.m3World_Step(world, DT, SUBSTEPS)
+    lib.m3World_Sync(world)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpu.m3World_Step(world, DT, SUBSTEPS)
+    lib.m3World_Sync(world)
+    wall = time.perf_counter() - t0
+    h = "%016x" % gpu.m3World_Hash(world)
+    want = golden_hash(warmup + steps)
+    gpu.m3DestroyWorld(world)
+    return {"wall": wall, "hash": h, "hash_expected": want}
+
+
+def run_roofline(steps: int, warmup: int) -> dict | None:
+    """Event-timed per-kernel averages over a second pass of the same run."""
+    import maul3d_amd
+    from maul3d_amd import profile, scenes
+
+    gpu = maul3d_amd.get_api()
+    lib = maul3d_amd.library()
+    world = scenes.rain(gpu, count=BODIES)
+    lib.m3World_Sync(world)
+    for _ in range(warmup):
+        gpu.m3World_Step(world, DT, SUBSTEPS)
+    stats = profile.profile_steps(world, steps)
+    gpu.m3DestroyWorld(world)
+    return stats
+
+
+def run_cpu(steps: int) -> dict | None:
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libmaul3d_ref.so")
+    if not os.path.exists(ref_path):
+        return None
+    from maul3d_amd import api, scenes
+
+    ref = api.Api(C.CDLL(ref_path))
+    world = scenes.rain(ref, count=BODIES)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ref.m3World_Step(world, DT, SUBSTEPS)
+    wall = time.perf_counter() - t0
+    ref.m3DestroyWorld(world)
+    return {
+        "value": BODIES * SUBSTEPS * steps / wall,
+        "unit": "body-substeps/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": f"reference engine (oracle/_ref, serial, no task hooks) on steps 1-{steps} "
+        f"of the same 10k rain; {wall:.1f} s",
+    }
+
+
+def main() -> None:
+    args = parse()
+    if args.gpus != 1:
+        print(
+            "bench.py: this build runs on exactly one GPU (worlds shard across GPUs "
+            "with no exchange; no multi-GPU run is part of this build)",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+    res = run_gpu(args.steps, args.warmup)
+    value = BODIES * SUBSTEPS * args.steps / res["wall"]
+    line = {
+        "metric": "body-substeps/sec (10k rain, single world)",
+        "value": value,
+        "unit": "body-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * res["wall"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic (seeded 10k rain scene)",
+        "config": {
+            "workload": "10k-body rain, single world: 10,000 dynamic bodies (40% boxes, "
+            "40% spheres, 20% capsules), dt=1/60, 4 substeps",
+            "bodies": BODIES,
+            "substeps": SUBSTEPS,
+            "trajectory_steps": args.warmup + args.steps,
+        },
+        "world_hash": res["hash"],
+        "hash_bit_exact": (res["hash"] == res["hash_expected"]) if res["hash_expected"] else None,
+    }
+    if not args.no_roofline:
+        try:
+            prof = run_roofline(args.steps, args.warmup)
+        except Exception as exc:  # reported, never silently dropped
+            prof = None
+            line["roofline_error"] = repr(exc)
+        if prof is not None:
+            line["roofline"] = prof["roofline"]
+            line["kernels"] = prof["kernels"]
+    if not args.no_cpu:
+        line["cpu_baseline"] = run_cpu(args.cpu_steps)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

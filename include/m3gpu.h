@@ -166,6 +166,36 @@ M3GPU_API int m3gpu_sync(m3gpuWorld* world);
 // Waits for the world's last step and reports its counts and stage times.
 M3GPU_API int m3gpu_last_stats(m3gpuWorld* world, m3gpuStepStats* stats);
 
+// Stage profile: between begin and end every step of the world waits for
+// its own completion and adds each stage's HIP-event time (events recorded
+// on the world's stream around each launch) and its work counts.
+#define M3GPU_MAX_STAGES 64
+typedef struct m3gpuStageTime
+{
+    char name[32];
+    int64_t launches;
+    double ms;
+} m3gpuStageTime;
+
+typedef struct m3gpuProfile
+{
+    int32_t stageCount;
+    m3gpuStageTime stage[M3GPU_MAX_STAGES];
+    int64_t steps;
+    int64_t sumPairs;       // pair list entries, summed over profiled steps
+    int64_t sumOldPairs;    // previous-step pairs (the warm-start carry source)
+    int64_t sumConstraints; // contact constraints
+    int64_t sumPoints;      // contact points over all constraints
+    int64_t sumMovers;      // awake dynamic and kinematic bodies
+    int64_t sumSubsteps;
+    int64_t sumShapes; // shape high-water mark
+    int64_t sumBodies; // body high-water mark
+    int64_t sumOverflow; // constraints in the serial overflow color
+} m3gpuProfile;
+
+M3GPU_API int m3gpu_profile_begin(m3gpuWorld* world);
+M3GPU_API int m3gpu_profile_end(m3gpuWorld* world, m3gpuProfile* out);
+
 #ifdef __cplusplus
 }
 #endif

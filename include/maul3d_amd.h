@@ -192,6 +192,18 @@ M3_API uint64_t m3World_Hash(m3WorldId worldId);
 M3_API m3Counters m3World_GetCounters(m3WorldId worldId);       // world.h:556
 M3_API m3Profile m3World_GetProfile(m3WorldId worldId);         // world.h:595
 
+// Extensions of this build (no reference counterpart): step many worlds with
+// one call (the batched rollback and toss workloads; returns the number of
+// valid worlds stepped or -1 on bad arguments), and wait for a world's
+// queued device work after uploading pending host changes.
+M3_API int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt,
+                                int32_t substeps);
+M3_API void m3World_Sync(m3WorldId worldId);
+// Stage profile of the steps between Begin and End (struct in m3gpu.h).
+struct m3gpuProfile;
+M3_API int32_t m3World_ProfileBegin(m3WorldId worldId);
+M3_API int32_t m3World_ProfileEnd(m3WorldId worldId, struct m3gpuProfile* out);
+
 M3_API m3BodyDef m3DefaultBodyDef(void);                        // body.h:42
 M3_API m3BodyId m3CreateBody(m3WorldId worldId, const m3BodyDef* def); // body.h:43
 M3_API bool m3Body_IsValid(m3BodyId bodyId);

@@ -61,6 +61,7 @@ struct m3gpuWorld
     void* cubTemp;
     size_t cubBytes;
     m3Scalars* hostScalars; // pinned mirror for stats
+    struct m3Profiler* prof;
 };
 
 // ---------------------------------------------------------------------------
@@ -111,6 +112,7 @@ __global__ void k_begin(m3View v)
     sc->islandCount = 0;
     sc->awakeCount = 0;
     sc->discoverCount = 0;
+    sc->pointCount = 0;
 }
 
 __global__ void k_fill32(uint32_t* p, int32_t n, uint32_t value)
@@ -538,6 +540,7 @@ __global__ void k_prepare(m3View v, float hertz, float zeta, float h)
     m3Softness staticSoft = m3MakeSoft(2.0f * hertz, zeta, h);
     m3Constraint* con = &v.cons[c];
     m3PrepareContact(&v, v.consPair[c], con, soft, staticSoft);
+    atomicAdd(&v.sc->pointCount, con->pointCount);
     if (con->restitution > 0.0f)
     {
         v.sc->anyRestitution = 1;
@@ -1091,13 +1094,15 @@ extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* caps)
     return w;
 }
 
+static void ProfFree(m3gpuWorld* w);
+
 extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
 {
     if (w == nullptr)
     {
         return;
     }
-    hipStreamSynchronize(w->stream);
+    (void)hipStreamSynchronize(w->stream);
     m3View& v = w->view;
     void* ptrs[] = {v.xf, v.lv, v.av, v.invMass, v.invI, v.inertia, v.lc, v.gravityScale,
                     v.linDamp, v.angDamp, v.bodyAlive, v.type, v.awake, v.sleepTime, v.bullet,
@@ -1116,11 +1121,12 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
     {
         if (ptrs[i] != nullptr)
         {
-            hipFree(ptrs[i]);
+            (void)hipFree(ptrs[i]);
         }
     }
-    hipHostFree(w->hostScalars);
-    hipStreamDestroy(w->stream);
+    (void)hipHostFree(w->hostScalars);
+    ProfFree(w);
+    (void)hipStreamDestroy(w->stream);
     free(w);
 }
 
@@ -1302,6 +1308,66 @@ extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
     return 0;
 }
 
+// Stage names for the profile (event-timed on the world's stream).
+enum m3Stage
+{
+    ST_BEGIN, ST_FILL, ST_FRESH, ST_DISCOVER_ALL, ST_REFRESH, ST_SAP_SORT, ST_SAP, ST_PLANES,
+    ST_FROZEN, ST_EMIT_SORT, ST_UNIQUE, ST_PAIR_COUNT, ST_EXPAND, ST_STALL_RESTORE,
+    ST_SLEEP_SELECT, ST_NARROW, ST_CAPTURE, ST_UNION, ST_ROOT, ST_WAKE_MARK, ST_WAKE_APPLY,
+    ST_CONS_FLAGS, ST_CONS_SELECT, ST_PREPARE, ST_COLOR, ST_MOVER_FLAGS, ST_MOVER_SELECT,
+    ST_INTEGRATE_V, ST_WARM, ST_WARM_OVERFLOW, ST_SOLVE, ST_SOLVE_OVERFLOW, ST_INTEGRATE_X,
+    ST_RELAX, ST_RELAX_OVERFLOW, ST_CLEAR_FORCES, ST_RESTITUTION, ST_STORE, ST_CCD,
+    ST_SLEEP_TIMERS, ST_SLEEP_READY, ST_SLEEP_FREEZE, ST_FRESH_FROZEN, ST_DISCOVER_FROZEN, ST_END,
+    ST_COUNT
+};
+
+static const char* const kStageNames[ST_COUNT] = {
+    "begin", "fill_emit", "fresh_bounds", "discover_all", "refresh_proxies", "sap_sort", "sap_sweep",
+    "plane_pairs", "frozen_merge", "pair_sort", "pair_unique", "pair_count", "pair_expand",
+    "stall_restore", "frozen_select", "narrowphase", "capture", "island_union", "island_root",
+    "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "prepare", "coloring",
+    "mover_flags", "mover_select", "integrate_velocities", "warm_start", "warm_start_overflow",
+    "solve", "solve_overflow", "integrate_positions", "relax", "relax_overflow", "clear_forces",
+    "restitution", "store_impulses", "continuous", "sleep_timers", "sleep_ready", "sleep_freeze",
+    "fresh_bounds_frozen", "discover_frozen", "end"};
+
+struct m3Profiler
+{
+    int on;
+    int pending; // events recorded in the current step
+    hipEvent_t ev[2 * 1024];
+    int stage[1024];
+    double ms[ST_COUNT];
+    int64_t launches[ST_COUNT];
+    int64_t steps;
+    // per-step counts summed over profiled steps (for the byte models)
+    int64_t sumPairs, sumOldPairs, sumConstraints, sumPoints, sumMovers, sumSubsteps;
+    int64_t sumShapes, sumBodies, sumOverflow;
+};
+
+static m3Profiler* Prof(m3gpuWorld* w);
+
+// Records a launch between two events when the world is being profiled.
+#define STAGE(id, launch)                                                                          \
+    do                                                                                             \
+    {                                                                                              \
+        m3Profiler* pr_ = Prof(w);                                                                 \
+        int slot_ = -1;                                                                            \
+        if (pr_ != nullptr && pr_->pending < 1024)                                                 \
+        {                                                                                          \
+            slot_ = pr_->pending++;                                                                \
+            pr_->stage[slot_] = (id);                                                              \
+            HIP_TRY(hipEventRecord(pr_->ev[2 * slot_], st));                                       \
+        }                                                                                          \
+        launch;                                                                                    \
+        if (slot_ >= 0)                                                                            \
+        {                                                                                          \
+            HIP_TRY(hipEventRecord(pr_->ev[2 * slot_ + 1], st));                                   \
+        }                                                                                          \
+    } while (0)
+
+#define KL(...) hipLaunchKernelGGL(__VA_ARGS__)
+
 // Enqueue one step.  Launch geometry only depends on host-known capacities
 // and high-water marks; every data-dependent count is read on the device.
 static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
@@ -1320,101 +1386,143 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     int32_t np = v.pairCapacity;
     int32_t ne = v.emitCapacity;
     size_t tb = w->cubBytes;
+    void* ct = w->cubTemp;
 
-    hipLaunchKernelGGL(k_begin, dim3(1), dim3(1), 0, st, v);
-    hipLaunchKernelGGL(k_fill32, dim3(Blocks(ne, T)), dim3(T), 0, st, v.emit, ne, M3_SENTINEL32);
+    STAGE(ST_BEGIN, KL(k_begin, dim3(1), dim3(1), 0, st, v));
+    STAGE(ST_FILL, KL(k_fill32, dim3(Blocks(ne, T)), dim3(T), 0, st, v.emit, ne, M3_SENTINEL32));
     // Broadphase.
-    hipLaunchKernelGGL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysIn, w->sapValsIn);
+    STAGE(ST_FRESH, KL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysIn, w->sapValsIn));
     if (w->fullQuery)
     {
-        hipLaunchKernelGGL(k_reset_sleep, dim3(1), dim3(1), 0, st, v);
-        hipLaunchKernelGGL(k_discover_all, dim3(Blocks(ns, T)), dim3(T), 0, st, v);
+        KL(k_reset_sleep, dim3(1), dim3(1), 0, st, v);
+        STAGE(ST_DISCOVER_ALL, KL(k_discover_all, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
         w->fullQuery = 0;
     }
-    hipLaunchKernelGGL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v);
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(w->cubTemp, tb, w->sapKeysIn, w->sapKeysOut,
-                                               w->sapValsIn, w->sapValsOut, ns, 0, 64, st));
-    hipLaunchKernelGGL(k_sap, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysOut, w->sapValsOut);
+    STAGE(ST_REFRESH, KL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
+    STAGE(ST_SAP_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+                           ct, tb, w->sapKeysIn, w->sapKeysOut, w->sapValsIn, w->sapValsOut, ns, 0,
+                           64, st)));
+    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysOut, w->sapValsOut));
     if (v.planeCount > 0)
     {
-        hipLaunchKernelGGL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, v);
+        STAGE(ST_PLANES,
+              KL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, v));
     }
-    hipLaunchKernelGGL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, v);
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(w->cubTemp, tb, v.emit, w->emitSorted, ne, 0, 32, st));
-    HIP_TRY(hipcub::DeviceSelect::Unique(w->cubTemp, tb, w->emitSorted, w->emitUnique,
-                                         &v.sc->uniqueCount, ne, st));
-    hipLaunchKernelGGL(k_pair_count, dim3(1), dim3(1), 0, st, v, w->emitUnique);
-    hipLaunchKernelGGL(k_expand, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->emitUnique, w->pairFlags);
-    hipLaunchKernelGGL(k_stall_restore, dim3(Blocks(np, T)), dim3(T), 0, st, v);
-    HIP_TRY(hipcub::DeviceSelect::Flagged(w->cubTemp, tb, v.keys, w->pairFlags, v.sleepKeys,
-                                          &v.sc->sleepCount, np, st));
+    STAGE(ST_FROZEN, KL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, v));
+    STAGE(ST_EMIT_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ct, tb, v.emit, w->emitSorted,
+                                                                   ne, 0, 32, st)));
+    STAGE(ST_UNIQUE, HIP_TRY(hipcub::DeviceSelect::Unique(ct, tb, w->emitSorted, w->emitUnique,
+                                                          &v.sc->uniqueCount, ne, st)));
+    STAGE(ST_PAIR_COUNT, KL(k_pair_count, dim3(1), dim3(1), 0, st, v, w->emitUnique));
+    STAGE(ST_EXPAND,
+          KL(k_expand, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->emitUnique, w->pairFlags));
+    STAGE(ST_STALL_RESTORE, KL(k_stall_restore, dim3(Blocks(np, T)), dim3(T), 0, st, v));
+    STAGE(ST_SLEEP_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, v.keys, w->pairFlags,
+                                                                 v.sleepKeys, &v.sc->sleepCount,
+                                                                 np, st)));
     // Narrowphase.
-    hipLaunchKernelGGL(k_narrow, dim3(Blocks(np, 64)), dim3(64), 0, st, v);
+    STAGE(ST_NARROW, KL(k_narrow, dim3(Blocks(np, 64)), dim3(64), 0, st, v));
     // Sweeps, islands, wake.
-    hipLaunchKernelGGL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, v);
-    hipLaunchKernelGGL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, v);
-    hipLaunchKernelGGL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
-    hipLaunchKernelGGL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
-    hipLaunchKernelGGL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
+    STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
+    STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, v));
+    STAGE(ST_ROOT, KL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
+    STAGE(ST_WAKE_MARK, KL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
+    STAGE(ST_WAKE_APPLY, KL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
     // Prepare and color.
     float h = p->dt / (float)p->substeps;
     float invH = h > 0.0f ? 1.0f / h : 0.0f;
-    hipLaunchKernelGGL(k_constraint_flags, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->pairFlags);
+    STAGE(ST_CONS_FLAGS,
+          KL(k_constraint_flags, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->pairFlags));
     hipcub::CountingInputIterator<int32_t> counting(0);
-    HIP_TRY(hipcub::DeviceSelect::Flagged(w->cubTemp, tb, counting, w->pairFlags, v.consPair,
-                                          &v.sc->constraintCount, np, st));
-    hipLaunchKernelGGL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, v, p->contactHertz,
-                       p->contactDampingRatio, h);
-    hipLaunchKernelGGL(k_color, dim3(1), dim3(M3_COLOR_CHUNK), 0, st, v, w->bodyMasks);
-    hipLaunchKernelGGL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->bodyFlags, p->dt);
-    HIP_TRY(hipcub::DeviceSelect::Flagged(w->cubTemp, tb, counting, w->bodyFlags, v.movers,
-                                          &v.sc->moverCount, nb, st));
+    STAGE(ST_CONS_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, w->pairFlags,
+                                                                v.consPair,
+                                                                &v.sc->constraintCount, np, st)));
+    STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, v, p->contactHertz,
+                         p->contactDampingRatio, h));
+    STAGE(ST_COLOR, KL(k_color, dim3(1), dim3(M3_COLOR_CHUNK), 0, st, v, w->bodyMasks));
+    STAGE(ST_MOVER_FLAGS,
+          KL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->bodyFlags, p->dt));
+    STAGE(ST_MOVER_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, w->bodyFlags,
+                                                                 v.movers, &v.sc->moverCount, nb,
+                                                                 st)));
     // Substeps.
     m3Vec3 g = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
     unsigned passBlocks = Blocks(np, 128) < 512 ? Blocks(np, 128) : 512;
+    float push = p->contactPushMaxSpeed;
     for (int32_t sub = 0; sub < p->substeps; ++sub)
     {
-        hipLaunchKernelGGL(k_integrate_v, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h, g,
-                           p->maximumLinearSpeed, p->maximumAngularSpeed);
+        STAGE(ST_INTEGRATE_V, KL(k_integrate_v, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h, g,
+                                 p->maximumLinearSpeed, p->maximumAngularSpeed));
         for (int mode = 0; mode < 2; ++mode)
         {
             for (int32_t color = 0; color < M3_GRAPH_COLORS; ++color)
             {
-                hipLaunchKernelGGL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, mode,
-                                   invH, p->contactPushMaxSpeed);
+                STAGE(mode == 0 ? ST_WARM : ST_SOLVE,
+                      KL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, mode, invH, push));
             }
-            hipLaunchKernelGGL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, mode, invH,
-                               p->contactPushMaxSpeed);
+            STAGE(mode == 0 ? ST_WARM_OVERFLOW : ST_SOLVE_OVERFLOW,
+                  KL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, mode, invH, push));
         }
-        hipLaunchKernelGGL(k_integrate_x, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h);
+        STAGE(ST_INTEGRATE_X, KL(k_integrate_x, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h));
         for (int32_t color = 0; color < M3_GRAPH_COLORS; ++color)
         {
-            hipLaunchKernelGGL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, 2, invH,
-                               p->contactPushMaxSpeed);
+            STAGE(ST_RELAX,
+                  KL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, 2, invH, push));
         }
-        hipLaunchKernelGGL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, 2, invH,
-                           p->contactPushMaxSpeed);
+        STAGE(ST_RELAX_OVERFLOW, KL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, 2, invH, push));
     }
-    hipLaunchKernelGGL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, v);
-    hipLaunchKernelGGL(k_restitution, dim3(1), dim3(1), 0, st, v, p->restitutionThreshold);
-    hipLaunchKernelGGL(k_store, dim3(Blocks(np, T)), dim3(T), 0, st, v);
+    STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
+    STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, v, p->restitutionThreshold));
+    STAGE(ST_STORE, KL(k_store, dim3(Blocks(np, T)), dim3(T), 0, st, v));
     if (p->continuousEnabled)
     {
-        hipLaunchKernelGGL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, v);
+        STAGE(ST_CCD, KL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, v));
     }
-    hipLaunchKernelGGL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root, p->dt,
-                       p->sleepEnabled);
+    STAGE(ST_SLEEP_TIMERS, KL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root,
+                              p->dt, p->sleepEnabled));
     if (p->sleepEnabled)
     {
-        hipLaunchKernelGGL(k_sleep_ready, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root);
-        hipLaunchKernelGGL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root,
-                           w->frozenNow);
-        hipLaunchKernelGGL(k_fresh_if_frozen, dim3(Blocks(ns, T)), dim3(T), 0, st, v);
-        hipLaunchKernelGGL(k_discover_frozen, dim3(Blocks(ns, 64)), dim3(64), 0, st, v,
-                           w->frozenNow);
+        STAGE(ST_SLEEP_READY, KL(k_sleep_ready, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
+        STAGE(ST_SLEEP_FREEZE, KL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root,
+                                  w->frozenNow));
+        STAGE(ST_FRESH_FROZEN, KL(k_fresh_if_frozen, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
+        STAGE(ST_DISCOVER_FROZEN, KL(k_discover_frozen, dim3(Blocks(ns, 64)), dim3(64), 0, st, v,
+                                     w->frozenNow));
     }
-    hipLaunchKernelGGL(k_end, dim3(1), dim3(1), 0, st, v);
+    STAGE(ST_END, KL(k_end, dim3(1), dim3(1), 0, st, v));
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Waits for the profiled step, folds its event times and counts in.
+static int ProfileCollect(m3gpuWorld* w, int32_t substeps)
+{
+    m3Profiler* pr = Prof(w);
+    if (pr == nullptr)
+    {
+        return 0;
+    }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    for (int k = 0; k < pr->pending; ++k)
+    {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, pr->ev[2 * k], pr->ev[2 * k + 1]));
+        pr->ms[pr->stage[k]] += ms;
+        pr->launches[pr->stage[k]] += 1;
+    }
+    pr->pending = 0;
+    HIP_TRY(hipMemcpy(w->hostScalars, w->view.sc, sizeof(m3Scalars), hipMemcpyDeviceToHost));
+    const m3Scalars* s = w->hostScalars;
+    pr->steps += 1;
+    pr->sumPairs += s->pairCount;
+    pr->sumOldPairs += s->oldPairCount;
+    pr->sumConstraints += s->constraintCount;
+    pr->sumPoints += s->pointCount;
+    pr->sumMovers += s->moverCount;
+    pr->sumSubsteps += substeps;
+    pr->sumShapes += w->view.maxShape;
+    pr->sumBodies += w->view.maxBody;
+    pr->sumOverflow += s->colorStarts[M3_GRAPH_COLORS + 1] - s->colorStarts[M3_GRAPH_COLORS];
     return 0;
 }
 
@@ -1434,6 +1542,10 @@ extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStat
         return -1;
     }
     if (EnqueueStep(w, p) != 0)
+    {
+        return -1;
+    }
+    if (ProfileCollect(w, p->substeps) != 0)
     {
         return -1;
     }
@@ -1484,4 +1596,83 @@ extern "C" int m3gpu_last_stats(m3gpuWorld* w, m3gpuStepStats* stats)
     stats->islandCount = s->islandCount;
     stats->awakeCount = s->moverCount;
     return 0;
+}
+
+static m3Profiler* Prof(m3gpuWorld* w) { return (w->prof != nullptr && w->prof->on) ? w->prof : nullptr; }
+
+extern "C" int m3gpu_profile_begin(m3gpuWorld* w)
+{
+    if (w == nullptr)
+    {
+        return -1;
+    }
+    if (w->prof == nullptr)
+    {
+        w->prof = (m3Profiler*)calloc(1, sizeof(m3Profiler));
+        if (w->prof == nullptr)
+        {
+            return -1;
+        }
+        for (int k = 0; k < 2 * 1024; ++k)
+        {
+            HIP_TRY(hipEventCreate(&w->prof->ev[k]));
+        }
+    }
+    m3Profiler* pr = w->prof;
+    memset(pr->ms, 0, sizeof(pr->ms));
+    memset(pr->launches, 0, sizeof(pr->launches));
+    pr->steps = pr->sumPairs = pr->sumOldPairs = pr->sumConstraints = pr->sumPoints = 0;
+    pr->sumMovers = pr->sumSubsteps = pr->sumShapes = pr->sumBodies = pr->sumOverflow = 0;
+    pr->pending = 0;
+    pr->on = 1;
+    return 0;
+}
+
+extern "C" int m3gpu_profile_end(m3gpuWorld* w, m3gpuProfile* out)
+{
+    if (w == nullptr || out == nullptr || w->prof == nullptr)
+    {
+        return -1;
+    }
+    m3Profiler* pr = w->prof;
+    pr->on = 0;
+    memset(out, 0, sizeof(*out));
+    int n = 0;
+    for (int k = 0; k < ST_COUNT && n < M3GPU_MAX_STAGES; ++k)
+    {
+        if (pr->launches[k] == 0)
+        {
+            continue;
+        }
+        strncpy(out->stage[n].name, kStageNames[k], sizeof(out->stage[n].name) - 1);
+        out->stage[n].launches = pr->launches[k];
+        out->stage[n].ms = pr->ms[k];
+        n += 1;
+    }
+    out->stageCount = n;
+    out->steps = pr->steps;
+    out->sumPairs = pr->sumPairs;
+    out->sumOldPairs = pr->sumOldPairs;
+    out->sumConstraints = pr->sumConstraints;
+    out->sumPoints = pr->sumPoints;
+    out->sumMovers = pr->sumMovers;
+    out->sumSubsteps = pr->sumSubsteps;
+    out->sumShapes = pr->sumShapes;
+    out->sumBodies = pr->sumBodies;
+    out->sumOverflow = pr->sumOverflow;
+    return 0;
+}
+
+static void ProfFree(m3gpuWorld* w)
+{
+    if (w->prof == nullptr)
+    {
+        return;
+    }
+    for (int k = 0; k < 2 * 1024; ++k)
+    {
+        (void)hipEventDestroy(w->prof->ev[k]);
+    }
+    free(w->prof);
+    w->prof = nullptr;
 }

@@ -86,7 +86,7 @@ typedef struct m3Scalars
     int32_t awakeCount;
     int32_t colorCount;
     int32_t discoverCount;
-    int32_t pad;
+    int32_t pointCount; // contact points over all constraints
     int32_t colorStarts[M3_GRAPH_COLORS + 2];
     uint64_t stepCount;
 } m3Scalars;

@@ -1441,6 +1441,7 @@ static int EnsureDevice(m3World* w)
             DeviceFail("m3gpu_upload");
         }
         w->devDirty = 0;
+        w->pairsFullQuery = 0; // consumed by the device's next step
     }
     return 1;
 }
@@ -1483,8 +1484,6 @@ void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
 
 // Batched stepping of many worlds with one call: the rollback and toss
 // workloads.  Not in the reference API (its worlds step one by one).
-M3_API int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt,
-                                int32_t substeps);
 int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32_t substeps)
 {
     if (worlds == NULL || count <= 0 || !(dt > 0.0f) || substeps < 1 ||
@@ -1517,15 +1516,42 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
     return n;
 }
 
-// Waits for the world's queued device work (for timing harnesses).
-M3_API void m3World_Sync(m3WorldId worldId);
+// Uploads pending host changes and waits for the world's queued device work
+// (timing harnesses call it before and after a timed region).
 void m3World_Sync(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);
-    if (w != NULL && w->dev != NULL)
+    if (w == NULL)
     {
-        m3gpu_sync(w->dev);
+        return;
     }
+    EnsureDevice(w);
+    if (m3gpu_sync(w->dev) != 0)
+    {
+        DeviceFail("m3gpu_sync");
+    }
+}
+
+// Stage profile of the following steps (device event times + counts).
+int32_t m3World_ProfileBegin(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return -1;
+    }
+    EnsureDevice(w);
+    return m3gpu_profile_begin(w->dev);
+}
+
+int32_t m3World_ProfileEnd(m3WorldId worldId, struct m3gpuProfile* out)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || w->dev == NULL)
+    {
+        return -1;
+    }
+    return m3gpu_profile_end(w->dev, out);
 }
 
 // ---------------------------------------------------------------------------
