@@ -18,7 +18,7 @@ extern "C" {
 
 #define M3_API __attribute__((visibility("default"))) extern
 
-// ---- math types (include/maul3d/math.h:24-52) -----------------------------
+// ---- math types (include/maul3d/math.h:27-55) -----------------------------
 #ifndef M3_CORE_H
 typedef float m3real;
 typedef struct m3Vec3 { m3real x, y, z; } m3Vec3;
@@ -28,7 +28,7 @@ typedef struct m3Transform { m3Pos3 p; m3Quat q; } m3Transform;
 typedef struct m3Mat3 { m3Vec3 cx, cy, cz; } m3Mat3;
 #endif
 
-// ---- ids (include/maul3d/base.h:64-104) ------------------------------------
+// ---- ids (include/maul3d/base.h:69-104) ------------------------------------
 typedef struct m3WorldId { int32_t index1; uint16_t generation; } m3WorldId;
 typedef struct m3BodyId { int32_t index1; uint16_t world0; uint16_t generation; } m3BodyId;
 typedef struct m3ShapeId { int32_t index1; uint16_t world0; uint16_t generation; } m3ShapeId;
@@ -57,7 +57,7 @@ typedef enum m3ShapeType
     m3_capsuleShape = 3,
 } m3ShapeType;
 
-// ---- defs (include/maul3d/world.h:48-107, body.h:18-40, shape.h:46-80) ----
+// ---- defs (include/maul3d/world.h:43-76, body.h:21-39, shape.h:40-74) ----
 typedef void m3TaskFn(int32_t startIndex, int32_t endIndex, void* taskContext);
 typedef void* m3EnqueueTaskFn(m3TaskFn* task, int32_t itemCount, int32_t minRange,
                               void* taskContext, void* userContext);
@@ -126,7 +126,7 @@ typedef struct m3Sphere { m3Vec3 center; m3real radius; } m3Sphere;
 typedef struct m3Plane { m3Vec3 normal; m3real offset; } m3Plane;
 typedef struct m3Capsule { m3Vec3 point1; m3Vec3 point2; m3real radius; } m3Capsule;
 
-// include/maul3d/world.h:531-554
+// include/maul3d/world.h:531-552
 typedef struct m3Counters
 {
     int32_t bodyCount;
@@ -150,7 +150,7 @@ typedef struct m3Counters
     int64_t freeCalls;
 } m3Counters;
 
-// include/maul3d/world.h:578-593 (device-timed per stage)
+// include/maul3d/world.h:578-591 (device-timed per stage)
 typedef struct m3Profile
 {
     float step;
@@ -167,17 +167,18 @@ typedef struct m3Profile
 } m3Profile;
 
 // ---- API -------------------------------------------------------------------
-M3_API int m3GetVersion(void);                                   // base.h:34
-M3_API uint64_t m3Hash64(uint64_t h, const void* bytes, int32_t count); // base.h:110
+// Citations are include/maul3d/<file>:<line> of the reference declaration.
+M3_API int m3GetVersion(void);                                          // base.h:39
+M3_API uint64_t m3Hash64(uint64_t h, const void* bytes, int32_t count); // base.h:112
 
-M3_API m3WorldDef m3DefaultWorldDef(void);                       // world.h:109
-M3_API m3WorldId m3CreateWorld(const m3WorldDef* def);           // world.h:117
-M3_API void m3DestroyWorld(m3WorldId worldId);                   // world.h:118
-M3_API bool m3World_IsValid(m3WorldId worldId);                  // world.h:119
-M3_API void m3World_SetGravity(m3WorldId worldId, m3Vec3 gravity);
-M3_API m3Vec3 m3World_GetGravity(m3WorldId worldId);
-M3_API void m3World_EnableSleeping(m3WorldId worldId, bool flag);
-M3_API void m3World_EnableContinuous(m3WorldId worldId, bool flag);
+M3_API m3WorldDef m3DefaultWorldDef(void);                         // world.h:82
+M3_API m3WorldId m3CreateWorld(const m3WorldDef* def);             // world.h:86
+M3_API void m3DestroyWorld(m3WorldId worldId);                     // world.h:87
+M3_API bool m3World_IsValid(m3WorldId worldId);                    // world.h:88
+M3_API void m3World_SetGravity(m3WorldId worldId, m3Vec3 gravity); // world.h:92
+M3_API m3Vec3 m3World_GetGravity(m3WorldId worldId);               // world.h:134
+M3_API void m3World_EnableSleeping(m3WorldId worldId, bool flag);  // world.h:157
+M3_API void m3World_EnableContinuous(m3WorldId worldId, bool flag); // world.h:163
 
 // The drop-in boundary: world.h:195 (Step), :203-205 (Snapshot/Restore),
 // :510 (Hash).  Step is void and silently refuses a bad id, dt <= 0 or
@@ -189,8 +190,8 @@ M3_API int32_t m3World_SnapshotSize(m3WorldId worldId);
 M3_API int32_t m3World_Snapshot(m3WorldId worldId, void* out, int32_t capacity);
 M3_API bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size);
 M3_API uint64_t m3World_Hash(m3WorldId worldId);
-M3_API m3Counters m3World_GetCounters(m3WorldId worldId);       // world.h:556
-M3_API m3Profile m3World_GetProfile(m3WorldId worldId);         // world.h:595
+M3_API m3Counters m3World_GetCounters(m3WorldId worldId);       // world.h:554
+M3_API m3Profile m3World_GetProfile(m3WorldId worldId);         // world.h:593
 
 // Extensions of this build (no reference counterpart): step many worlds with
 // one call (the batched rollback and toss workloads; returns the number of
@@ -204,28 +205,28 @@ struct m3gpuProfile;
 M3_API int32_t m3World_ProfileBegin(m3WorldId worldId);
 M3_API int32_t m3World_ProfileEnd(m3WorldId worldId, struct m3gpuProfile* out);
 
-M3_API m3BodyDef m3DefaultBodyDef(void);                        // body.h:42
-M3_API m3BodyId m3CreateBody(m3WorldId worldId, const m3BodyDef* def); // body.h:43
-M3_API bool m3Body_IsValid(m3BodyId bodyId);
-M3_API m3Pos3 m3Body_GetPosition(m3BodyId bodyId);
-M3_API m3Quat m3Body_GetRotation(m3BodyId bodyId);
-M3_API m3Vec3 m3Body_GetLinearVelocity(m3BodyId bodyId);
-M3_API m3Vec3 m3Body_GetAngularVelocity(m3BodyId bodyId);
-M3_API void m3Body_SetLinearVelocity(m3BodyId bodyId, m3Vec3 velocity);
-M3_API void m3Body_SetAngularVelocity(m3BodyId bodyId, m3Vec3 velocity);
-M3_API bool m3Body_IsAwake(m3BodyId bodyId);
-M3_API void m3Body_ApplyForce(m3BodyId bodyId, m3Vec3 force);
-M3_API void m3Body_ApplyTorque(m3BodyId bodyId, m3Vec3 torque);
+M3_API m3BodyDef m3DefaultBodyDef(void);                               // body.h:43
+M3_API m3BodyId m3CreateBody(m3WorldId worldId, const m3BodyDef* def); // body.h:49
+M3_API bool m3Body_IsValid(m3BodyId bodyId);                           // body.h:55
+M3_API m3Pos3 m3Body_GetPosition(m3BodyId bodyId);                     // body.h:56
+M3_API m3Quat m3Body_GetRotation(m3BodyId bodyId);                     // body.h:57
+M3_API m3Vec3 m3Body_GetLinearVelocity(m3BodyId bodyId);               // body.h:58
+M3_API m3Vec3 m3Body_GetAngularVelocity(m3BodyId bodyId);              // body.h:59
+M3_API void m3Body_SetLinearVelocity(m3BodyId bodyId, m3Vec3 velocity); // body.h:122
+M3_API void m3Body_SetAngularVelocity(m3BodyId bodyId, m3Vec3 velocity); // body.h:123
+M3_API bool m3Body_IsAwake(m3BodyId bodyId);                           // body.h:103
+M3_API void m3Body_ApplyForce(m3BodyId bodyId, m3Vec3 force);          // body.h:115
+M3_API void m3Body_ApplyTorque(m3BodyId bodyId, m3Vec3 torque);        // body.h:116
 
-M3_API m3ShapeDef m3DefaultShapeDef(void);                      // shape.h:82
+M3_API m3ShapeDef m3DefaultShapeDef(void);                      // shape.h:78
 M3_API m3ShapeId m3CreateSphereShape(m3BodyId bodyId, const m3ShapeDef* def,
                                      const m3Sphere* sphere);   // shape.h:86
 M3_API m3ShapeId m3CreatePlaneShape(m3BodyId bodyId, const m3ShapeDef* def,
-                                    const m3Plane* plane);      // shape.h:90
+                                    const m3Plane* plane);      // shape.h:91
 M3_API m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def,
-                                  m3Vec3 halfExtents);          // shape.h:93
+                                  m3Vec3 halfExtents);          // shape.h:98
 M3_API m3ShapeId m3CreateCapsuleShape(m3BodyId bodyId, const m3ShapeDef* def,
-                                      const m3Capsule* capsule); // shape.h:104
+                                      const m3Capsule* capsule); // shape.h:112
 
 #ifdef __cplusplus
 }

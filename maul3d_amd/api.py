@@ -5,12 +5,12 @@ this package's ``libmaul3d_amd.so`` (the MI355X build) and, in tests only,
 the reference engine compiled as the parity checker.  Struct layouts follow
 the reference public headers field for field:
 
-* ``m3Vec3``/``m3Pos3``/``m3Quat``/``m3Transform``  include/maul3d/math.h:24-52
-* ``m3WorldId``/``m3BodyId``/``m3ShapeId``          include/maul3d/base.h:64-104
-* ``m3WorldDef``                                     include/maul3d/world.h:48-107
-* ``m3BodyDef``                                      include/maul3d/body.h:18-40
-* ``m3ShapeDef``                                     include/maul3d/shape.h:46-80
-* ``m3Counters``/``m3Profile``                       include/maul3d/world.h:531-593
+* ``m3Vec3``/``m3Pos3``/``m3Quat``/``m3Transform``  include/maul3d/math.h:27-55
+* ``m3WorldId``/``m3BodyId``/``m3ShapeId``          include/maul3d/base.h:69-104
+* ``m3WorldDef``                                     include/maul3d/world.h:43-76
+* ``m3BodyDef``                                      include/maul3d/body.h:21-39
+* ``m3ShapeDef``                                     include/maul3d/shape.h:40-74
+* ``m3Counters``/``m3Profile``                       include/maul3d/world.h:531-591
 """
 
 from __future__ import annotations

@@ -5,6 +5,7 @@
 // host round trip: every count the later stages need (pairs, constraints,
 // movers, colors) stays in device memory (m3Scalars) and kernels read it.
 // Stage order and semantics follow m3StepInternal (src/solver.c:3662-4320).
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -51,17 +52,24 @@ struct m3gpuWorld
     uint64_t* sapKeysOut;
     int32_t* sapValsIn;
     int32_t* sapValsOut;
+    float4* sapXY;
+    float2* sapZ;
+    int4* sapAttr;
     uint32_t* emitSorted;
     uint32_t* emitUnique;
     uint8_t* pairFlags;
     uint8_t* bodyFlags;
     int32_t* root;
     uint8_t* frozenNow;
-    uint32_t* bodyMasks;
+    uint16_t* bodyMasks;
+    int32_t* bodyOwner;
+    int32_t* iota;
+    uint8_t* colorsSorted;
     void* cubTemp;
     size_t cubBytes;
     m3Scalars* hostScalars; // pinned mirror for stats
     struct m3Profiler* prof;
+    int coopBlocks; // co-resident blocks of the substep kernel
 };
 
 // ---------------------------------------------------------------------------
@@ -95,6 +103,75 @@ __device__ static inline void Emit(const m3View& v, uint32_t key)
     else
     {
         v.sc->stall = 1;
+    }
+}
+
+// Per-lane emission buffer: one counter atomic per M3_EMIT_BUF keys instead
+// of one per key (tens of thousands of pairs contend on one counter).
+#define M3_EMIT_BUF 16
+struct m3EmitBuf
+{
+    uint32_t key[M3_EMIT_BUF];
+    int32_t n;
+};
+
+__device__ static inline void EmitFlush(const m3View& v, m3EmitBuf& b)
+{
+    if (b.n == 0)
+    {
+        return;
+    }
+    int32_t base = atomicAdd(&v.sc->emitCount, b.n);
+    for (int32_t k = 0; k < b.n; ++k)
+    {
+        if (base + k < v.emitCapacity)
+        {
+            v.emit[base + k] = b.key[k];
+        }
+        else
+        {
+            v.sc->stall = 1;
+        }
+    }
+    b.n = 0;
+}
+
+__device__ static inline void EmitBuffered(const m3View& v, m3EmitBuf& b, uint32_t key)
+{
+    b.key[b.n++] = key;
+    if (b.n == M3_EMIT_BUF)
+    {
+        EmitFlush(v, b);
+    }
+}
+
+// One key per lane at most: a single atomic per wavefront.
+__device__ static inline void EmitWave(const m3View& v, bool valid, uint32_t key)
+{
+    uint64_t mask = __ballot(valid);
+    if (mask == 0)
+    {
+        return;
+    }
+    int32_t lane = __lane_id();
+    int32_t leader = __ffsll((unsigned long long)mask) - 1;
+    int32_t base = 0;
+    if (lane == leader)
+    {
+        base = atomicAdd(&v.sc->emitCount, __popcll(mask));
+    }
+    base = __shfl(base, leader);
+    if (valid)
+    {
+        int32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+        if (base + rank < v.emitCapacity)
+        {
+            v.emit[base + rank] = key;
+        }
+        else
+        {
+            v.sc->stall = 1;
+        }
     }
 }
 
@@ -154,65 +231,25 @@ __global__ void k_fresh(m3View v, uint64_t* sapKeys, int32_t* sapVals)
     }
 }
 
-// Frozen-pair discovery for one shape: every tree shape whose stored box
-// and fresh box both overlap this shape's fresh box, plus every allowed
-// plane (src/broad_phase.c:351-428).
-__device__ static void DiscoverShape(const m3View& v, int32_t s)
-{
-    if (v.shapeAlive[s] == 0)
-    {
-        return;
-    }
-    if (v.shapeType[s] != M3_PLANE)
-    {
-        const double* self = &v.fresh[6 * s];
-        for (int32_t o = 0; o < v.maxShape; ++o)
-        {
-            if (o == s || v.shapeAlive[o] == 0 || v.shapeType[o] == M3_PLANE)
-            {
-                continue;
-            }
-            if (!m3BoxOverlap(&v.leaf[6 * o], self) || !m3PairAllowed(&v, s, o) ||
-                !m3BoxOverlap(self, &v.fresh[6 * o]))
-            {
-                continue;
-            }
-            int32_t slot = atomicAdd(&v.sc->sleepCount, 1);
-            if (slot < v.pairCapacity)
-            {
-                v.sleepKeys[slot] = m3ExpandKey(m3CompactKey(s, o));
-            }
-        }
-    }
-    for (int32_t pi = 0; pi < v.planeCount; ++pi)
-    {
-        int32_t p = v.planes[pi];
-        if (!m3PairAllowed(&v, p, s))
-        {
-            continue;
-        }
-        int32_t slot = atomicAdd(&v.sc->sleepCount, 1);
-        if (slot < v.pairCapacity)
-        {
-            v.sleepKeys[slot] = m3ExpandKey(m3CompactKey(p, s));
-        }
-    }
-}
-
 __global__ void k_reset_sleep(m3View v) { v.sc->sleepCount = 0; }
 
-// After a restore: rebuild the frozen buffer from every sleeping body.
-__global__ void k_discover_all(m3View v)
+// After a restore every sleeping body rediscovers its frozen pairs.
+__global__ void k_pending_all(m3View v)
 {
-    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= v.maxShape || v.shapeAlive[s] == 0)
+    int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= v.maxBody)
     {
         return;
     }
-    int32_t b = v.shapeBody[s];
-    if (v.bodyAlive[b] != 0 && v.type[b] != M3_STATIC && v.awake[b] == 0)
+    v.pending[b] = (v.bodyAlive[b] != 0 && v.type[b] != M3_STATIC && v.awake[b] == 0) ? 1 : 0;
+}
+
+__device__ static inline void AppendFrozen(const m3View& v, int32_t i, int32_t j)
+{
+    int32_t slot = atomicAdd(&v.sc->sleepCount, 1);
+    if (slot < v.pairCapacity)
     {
-        DiscoverShape(v, s);
+        v.sleepKeys[slot] = m3ExpandKey(m3CompactKey(i, j));
     }
 }
 
@@ -240,53 +277,183 @@ __global__ void k_refresh(m3View v)
     }
 }
 
-// Sweep along x over the sorted fresh boxes: every overlapping pair is
-// tested once and emitted when the tree pass would emit it.
-__global__ void k_sap(m3View v, const uint64_t* keys, const int32_t* vals)
+// Outward-rounded float copies of the sorted fresh boxes: a float test that
+// fails implies the exact double test fails, so the sweep filters on floats
+// and confirms survivors in double.
+__device__ static inline float RoundDown(double d)
+{
+    float f = (float)d;
+    return (double)f > d ? nextafterf(f, -INFINITY) : f;
+}
+
+__device__ static inline float RoundUp(double d)
+{
+    float f = (float)d;
+    return (double)f < d ? nextafterf(f, INFINITY) : f;
+}
+
+// Per-shape pair facts packed at gather time so the sweep's filters are
+// register operations (src/broad_phase.c:182-251 read these per pair).
+enum
+{
+    SA_QUERIES = 1,  // static or awake: runs its own tree query
+    SA_HOT = 2,      // awake and not static
+    SA_STATIC = 4,
+    SA_ENABLED = 8,
+    SA_SENSOR = 16,
+    SA_PENDING = 32, // its body froze last step
+    SA_DEFAULT_FILTER = 64,
+};
+
+__global__ void k_sap_gather(m3View v, const uint64_t* keys, const int32_t* vals, float4* xy,
+                             float2* zz, int4* attr)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxShape || keys[i] == M3_SENTINEL64)
+    if (i >= v.maxShape)
     {
         return;
     }
+    if (keys[i] == M3_SENTINEL64)
+    {
+        xy[i] = make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+        zz[i] = make_float2(INFINITY, -INFINITY);
+        attr[i] = make_int4(-1, -1, 0, 0);
+        return;
+    }
     int32_t s = vals[i];
-    const double* a = &v.fresh[6 * s];
-    double hx = a[3];
+    const double* f = &v.fresh[6 * s];
+    xy[i] = make_float4(RoundDown(f[0]), RoundUp(f[3]), RoundDown(f[1]), RoundUp(f[4]));
+    zz[i] = make_float2(RoundDown(f[2]), RoundUp(f[5]));
+    int32_t b = v.shapeBody[s];
+    int32_t fl = 0;
+    bool isStatic = v.type[b] == M3_STATIC;
+    fl |= (isStatic || v.awake[b] != 0) ? SA_QUERIES : 0;
+    fl |= (!isStatic && v.awake[b] != 0) ? SA_HOT : 0;
+    fl |= isStatic ? SA_STATIC : 0;
+    fl |= v.enabled[b] != 0 ? SA_ENABLED : 0;
+    fl |= v.sensor[s] != 0 ? SA_SENSOR : 0;
+    fl |= v.pending[b] != 0 ? SA_PENDING : 0;
+    fl |= (v.category[s] == 1ull && v.mask[s] == ~0ull && v.group[s] == 0) ? SA_DEFAULT_FILTER : 0;
+    attr[i] = make_int4(s, b, fl, 0);
+}
+
+// m3PairAllowed from packed facts; shapes with custom filters take the
+// full test.
+__device__ static inline bool AllowedPacked(const m3View& v, int4 a, int4 b)
+{
+    if (a.y == b.y || !(a.z & SA_ENABLED) || !(b.z & SA_ENABLED) ||
+        ((a.z & SA_STATIC) && (b.z & SA_STATIC)) || ((a.z & SA_SENSOR) && (b.z & SA_SENSOR)))
+    {
+        return false;
+    }
+    if ((a.z & SA_DEFAULT_FILTER) && (b.z & SA_DEFAULT_FILTER))
+    {
+        return true;
+    }
+    return m3PairAllowed(&v, a.x, b.x) != 0;
+}
+
+// Sweep along x over the sorted fresh boxes: every overlapping pair is
+// visited once, emitted when the tree pass would emit it (m3TreePairEmits),
+// and recorded in the frozen buffer when one side belongs to a body that
+// froze last step (m3DiscoverHit).  Runs before the proxy refresh: the
+// frozen discovery reads last step's stored boxes, as the reference's
+// end-of-step discovery does.
+__global__ void k_sap(m3View v, const float4* xy, const float2* zz, const int4* attr)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxShape)
+    {
+        return;
+    }
+    const int4 me = attr[i];
+    if (me.x < 0)
+    {
+        return;
+    }
+    const float4 mxy = xy[i];
+    const float2 mz = zz[i];
+    const double* a = &v.fresh[6 * me.x];
+    m3EmitBuf buf;
+    buf.n = 0;
     for (int32_t j = i + 1; j < v.maxShape; ++j)
     {
-        if (keys[j] == M3_SENTINEL64)
+        float4 oxy = xy[j];
+        if (oxy.x > mxy.y) // sentinels have +inf lower x: the sweep ends
         {
             break;
         }
-        int32_t o = vals[j];
-        const double* b = &v.fresh[6 * o];
-        if (b[0] > hx)
+        if (oxy.z > mxy.w || mxy.z > oxy.w)
         {
-            break;
+            continue;
         }
-        if (m3BoxOverlap(a, b) && m3TreePairEmits(&v, s, o))
+        float2 oz = zz[j];
+        if (oz.x > mz.y || mz.x > oz.y)
         {
-            Emit(v, m3CompactKey(s, o));
+            continue;
+        }
+        const int4 ot = attr[j];
+        const double* b = &v.fresh[6 * ot.x];
+        if (!m3BoxOverlap(a, b))
+        {
+            continue;
+        }
+        bool pend = ((me.z | ot.z) & SA_PENDING) != 0;
+        bool hot = ((me.z | ot.z) & SA_HOT) != 0;
+        bool queries = ((me.z | ot.z) & SA_QUERIES) != 0;
+        if (!pend && !(hot && queries))
+        {
+            continue;
+        }
+        if (!AllowedPacked(v, me, ot))
+        {
+            continue;
+        }
+        if ((me.z & SA_PENDING) && m3BoxOverlap(&v.leaf[6 * ot.x], a))
+        {
+            AppendFrozen(v, me.x, ot.x);
+        }
+        if ((ot.z & SA_PENDING) && m3BoxOverlap(&v.leaf[6 * me.x], b))
+        {
+            AppendFrozen(v, ot.x, me.x);
+        }
+        if (!hot || !queries)
+        {
+            continue;
+        }
+        bool qa = (me.z & SA_QUERIES) != 0;
+        bool qb = (ot.z & SA_QUERIES) != 0;
+        bool emit = (qa && qb) || (qa ? m3BoxOverlap(&v.leaf[6 * ot.x], a)
+                                      : m3BoxOverlap(&v.leaf[6 * me.x], b));
+        if (emit)
+        {
+            EmitBuffered(v, buf, m3CompactKey(me.x, ot.x));
         }
     }
+    EmitFlush(v, buf);
 }
 
 // Planes pair with every allowed hot shape (src/broad_phase.c:541-564).
 __global__ void k_planes(m3View v)
 {
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)v.maxShape * v.planeCount)
+    bool hot = false;
+    uint32_t key = 0;
+    if (idx < (int64_t)v.maxShape * v.planeCount)
     {
-        return;
+        int32_t s = (int32_t)(idx % v.maxShape);
+        int32_t p = v.planes[idx / v.maxShape];
+        if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE && m3PairAllowed(&v, p, s))
+        {
+            if (v.pending[v.shapeBody[s]])
+            {
+                AppendFrozen(v, p, s); // planes pair unconditionally with a frozen body
+            }
+            hot = m3PairHot(&v, p, s) != 0;
+            key = m3CompactKey(p, s);
+        }
     }
-    int32_t s = (int32_t)(idx % v.maxShape);
-    int32_t p = v.planes[idx / v.maxShape];
-    if (v.shapeAlive[s] == 0 || v.shapeType[s] == M3_PLANE || !m3PairAllowed(&v, p, s) ||
-        !m3PairHot(&v, p, s))
-    {
-        return;
-    }
-    Emit(v, m3CompactKey(p, s));
+    EmitWave(v, hot, key);
 }
 
 // Still-cold frozen pairs ride along (src/broad_phase.c:607-632).
@@ -294,18 +461,17 @@ __global__ void k_frozen(m3View v)
 {
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t n = v.sc->sleepCount < v.pairCapacity ? v.sc->sleepCount : v.pairCapacity;
-    if (k >= n)
+    bool cold = false;
+    uint32_t key = 0;
+    if (k < n)
     {
-        return;
+        uint64_t full = v.sleepKeys[k];
+        int32_t i = (int32_t)(full >> 32);
+        int32_t j = (int32_t)(full & 0xFFFFFFFFu);
+        cold = v.shapeAlive[i] != 0 && v.shapeAlive[j] != 0 && !m3PairHot(&v, i, j);
+        key = m3CompactKey(i, j);
     }
-    uint64_t key = v.sleepKeys[k];
-    int32_t i = (int32_t)(key >> 32);
-    int32_t j = (int32_t)(key & 0xFFFFFFFFu);
-    if (v.shapeAlive[i] == 0 || v.shapeAlive[j] == 0 || m3PairHot(&v, i, j))
-    {
-        return;
-    }
-    Emit(v, m3CompactKey(i, j));
+    EmitWave(v, cold, key);
 }
 
 // The unique sorted keys become this step's pair list; the cold ones are
@@ -401,6 +567,7 @@ __global__ void k_capture(m3View v)
     v.parent[i] = i;
     v.forced[i] = 0;
     v.flag[i] = 0;
+    v.pending[i] = 0; // the sweep has recorded last step's freezes
     v.deltaPos[i] = m3V3(0.0f, 0.0f, 0.0f);
     v.deltaRot[i] = m3IdentityQuat();
 }
@@ -547,42 +714,50 @@ __global__ void k_prepare(m3View v, float hertz, float zeta, float h)
     }
 }
 
-// Greedy first-fit coloring in constraint order, reproduced exactly by one
-// workgroup: constraints are taken 256 at a time, and a constraint colors
-// as soon as every earlier constraint in its chunk that shares an awake
-// dynamic body has colored (those are the only ones whose masks it reads).
-// Then a stable placement lists each color in ascending constraint order.
-#define M3_COLOR_CHUNK 256
-__global__ void __launch_bounds__(M3_COLOR_CHUNK) k_color(m3View v, uint32_t* masks)
+// Greedy first-fit coloring in constraint order (src/solver.c:2741-2822),
+// reproduced exactly by one workgroup.  Constraints are taken 1024 at a
+// time; inside a chunk, a constraint colors in the first round in which it
+// is the lowest pending constraint on each of its awake dynamic bodies
+// (an owner table claimed with atomic min), so every earlier constraint that
+// shares a body has already updated that body's mask.  Body masks and
+// owners live in LDS for worlds up to M3_COLOR_LDS_BODIES bodies.
+#define M3_COLOR_THREADS 1024
+#define M3_COLOR_LDS_BODIES 16384
+
+template <bool kLds>
+__global__ void __launch_bounds__(M3_COLOR_THREADS)
+    k_color(m3View v, uint16_t* gMasks, int32_t* gOwner)
 {
-    __shared__ int32_t sA[M3_COLOR_CHUNK];
-    __shared__ int32_t sB[M3_COLOR_CHUNK];
-    __shared__ uint8_t sColor[M3_COLOR_CHUNK];
-    __shared__ uint8_t sDone[M3_COLOR_CHUNK];
+    __shared__ uint16_t sMask[kLds ? M3_COLOR_LDS_BODIES : 1];
+    __shared__ int32_t sOwner[kLds ? M3_COLOR_LDS_BODIES : 1];
     __shared__ int32_t counts[M3_GRAPH_COLORS + 1];
-    __shared__ int32_t running[M3_GRAPH_COLORS + 1];
+    uint16_t* masks = kLds ? sMask : gMasks;
+    int32_t* owner = kLds ? sOwner : gOwner;
     const int32_t t = threadIdx.x;
     m3Scalars* sc = v.sc;
-    if (sc->stall != 0)
+    const int32_t n = sc->stall != 0 ? 0 : sc->constraintCount;
+    for (int32_t i = t; i < v.maxBody; i += M3_COLOR_THREADS)
     {
-        return;
+        masks[i] = 0;
+        owner[i] = 0x7FFFFFFF;
     }
-    const int32_t n = sc->constraintCount;
-    for (int32_t i = t; i < v.maxBody; i += M3_COLOR_CHUNK)
+    // Unused slots sort after every color in the list build.
+    for (int32_t c = n + t; c < v.pairCapacity; c += M3_COLOR_THREADS)
     {
-        masks[i] = 0u;
+        v.colors[c] = 0xFF;
     }
     if (t <= M3_GRAPH_COLORS)
     {
         counts[t] = 0;
     }
     __syncthreads();
-    for (int32_t base = 0; base < n; base += M3_COLOR_CHUNK)
+    for (int32_t base = 0; base < n; base += M3_COLOR_THREADS)
     {
         int32_t c = base + t;
         int32_t A = -1;
         int32_t B = -1;
-        if (c < n)
+        int pend = c < n;
+        if (pend)
         {
             const m3Constraint* con = &v.cons[c];
             int32_t a = con->bodyA;
@@ -596,36 +771,27 @@ __global__ void __launch_bounds__(M3_COLOR_CHUNK) k_color(m3View v, uint32_t* ma
                 B = b;
             }
         }
-        sA[t] = A;
-        sB[t] = B;
-        sDone[t] = c < n ? 0 : 1;
-        __syncthreads();
-        int32_t pa = -1;
-        int32_t pb = -1;
-        for (int32_t u = t - 1; u >= 0 && ((A >= 0 && pa < 0) || (B >= 0 && pb < 0)); --u)
-        {
-            int32_t ua = sA[u];
-            int32_t ub = sB[u];
-            if (A >= 0 && pa < 0 && (ua == A || ub == A))
-            {
-                pa = u;
-            }
-            if (B >= 0 && pb < 0 && (ua == B || ub == B))
-            {
-                pb = u;
-            }
-        }
         while (true)
         {
-            int ready = sDone[t] == 0 && (pa < 0 || sDone[pa] != 0) && (pb < 0 || sDone[pb] != 0);
-            // Barrier: every read of sDone above precedes any write below.
-            if (!__syncthreads_or(sDone[t] == 0))
+            if (pend)
+            {
+                if (A >= 0)
+                {
+                    atomicMin(&owner[A], t);
+                }
+                if (B >= 0)
+                {
+                    atomicMin(&owner[B], t);
+                }
+            }
+            if (!__syncthreads_or(pend))
             {
                 break;
             }
+            int ready = pend && (A < 0 || owner[A] == t) && (B < 0 || owner[B] == t);
             if (ready)
             {
-                uint32_t mask = (A >= 0 ? masks[A] : 0u) | (B >= 0 ? masks[B] : 0u);
+                uint32_t mask = (A >= 0 ? (uint32_t)masks[A] : 0u) | (B >= 0 ? (uint32_t)masks[B] : 0u);
                 int32_t color = M3_GRAPH_COLORS;
                 for (int32_t bit = 0; bit < M3_GRAPH_COLORS; ++bit)
                 {
@@ -639,26 +805,32 @@ __global__ void __launch_bounds__(M3_COLOR_CHUNK) k_color(m3View v, uint32_t* ma
                 {
                     if (A >= 0)
                     {
-                        masks[A] |= 1u << color;
+                        masks[A] = (uint16_t)(masks[A] | (1u << color));
                     }
                     if (B >= 0)
                     {
-                        masks[B] |= 1u << color;
+                        masks[B] = (uint16_t)(masks[B] | (1u << color));
                     }
                 }
-                sColor[t] = (uint8_t)color;
                 v.colors[c] = (uint8_t)color;
                 atomicAdd(&counts[color], 1);
             }
             __syncthreads();
             if (ready)
             {
-                sDone[t] = 1;
+                pend = 0;
+                if (A >= 0)
+                {
+                    owner[A] = 0x7FFFFFFF;
+                }
+                if (B >= 0)
+                {
+                    owner[B] = 0x7FFFFFFF;
+                }
             }
             __syncthreads();
         }
     }
-    __syncthreads();
     if (t == 0)
     {
         int32_t cursor = 0;
@@ -666,41 +838,20 @@ __global__ void __launch_bounds__(M3_COLOR_CHUNK) k_color(m3View v, uint32_t* ma
         for (int32_t k = 0; k <= M3_GRAPH_COLORS; ++k)
         {
             sc->colorStarts[k] = cursor;
-            running[k] = cursor;
             cursor += counts[k];
             used += counts[k] > 0 ? 1 : 0;
         }
         sc->colorStarts[M3_GRAPH_COLORS + 1] = cursor;
         sc->colorCount = used;
     }
-    __syncthreads();
-    // Stable placement: rank inside the chunk among equal colors.
-    for (int32_t base = 0; base < n; base += M3_COLOR_CHUNK)
+}
+
+__global__ void k_iota(int32_t* p, int32_t n)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
     {
-        int32_t c = base + t;
-        uint8_t mine = c < n ? v.colors[c] : (uint8_t)0xFF;
-        sColor[t] = mine;
-        __syncthreads();
-        if (c < n)
-        {
-            int32_t rank = 0;
-            for (int32_t u = 0; u < t; ++u)
-            {
-                rank += sColor[u] == mine ? 1 : 0;
-            }
-            v.colorLists[running[mine] + rank] = c;
-        }
-        __syncthreads();
-        if (t <= M3_GRAPH_COLORS)
-        {
-            int32_t add = 0;
-            for (int32_t u = 0; u < M3_COLOR_CHUNK; ++u)
-            {
-                add += sColor[u] == (uint8_t)t ? 1 : 0;
-            }
-            running[t] += add;
-        }
-        __syncthreads();
+        p[i] = i;
     }
 }
 
@@ -800,6 +951,109 @@ __global__ void k_overflow_pass(m3View v, int32_t mode, float invH, float pushMa
     }
 }
 
+struct m3SubstepParams
+{
+    int32_t substeps;
+    float h;
+    float invH;
+    float push;
+    float maxLinear;
+    float maxAngular;
+    m3Vec3 gravity;
+};
+
+// One graph color of one pass over the grid (mode 0 warm start, 1 solve
+// with bias, 2 relax); the overflow color runs serially on one thread.
+__device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t end, int mode,
+                                         const m3SubstepParams& sp, int32_t gid, int32_t gsz,
+                                         bool serial)
+{
+    if (serial)
+    {
+        if (gid == 0)
+        {
+            for (int32_t k = start; k < end; ++k)
+            {
+                m3Constraint* c = &v.cons[v.colorLists[k]];
+                if (mode == 0)
+                {
+                    m3WarmStartOne(&v, c);
+                }
+                else
+                {
+                    m3SolveContact(&v, c, sp.invH, sp.push, mode == 1);
+                }
+            }
+        }
+        return;
+    }
+    for (int32_t k = start + gid; k < end; k += gsz)
+    {
+        m3Constraint* c = &v.cons[v.colorLists[k]];
+        if (mode == 0)
+        {
+            m3WarmStartOne(&v, c);
+        }
+        else
+        {
+            m3SolveContact(&v, c, sp.invH, sp.push, mode == 1);
+        }
+    }
+}
+
+// The whole substep loop of src/solver.c:4085-4203 in one cooperative
+// launch: integrate velocities, warm start by color, solve by color,
+// integrate positions, relax by color, with a grid barrier after each phase.
+// Every branch depends only on counts written before the launch, so all
+// blocks reach the same barriers.
+__global__ void __launch_bounds__(256) k_substeps(m3View v, m3SubstepParams sp)
+{
+    namespace cg = cooperative_groups;
+    cg::grid_group grid = cg::this_grid();
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    int32_t starts[M3_GRAPH_COLORS + 2];
+    for (int32_t k = 0; k < M3_GRAPH_COLORS + 2; ++k)
+    {
+        starts[k] = v.sc->colorStarts[k];
+    }
+    const int32_t movers = v.sc->moverCount;
+    const int32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t gsz = gridDim.x * blockDim.x;
+    for (int32_t sub = 0; sub < sp.substeps; ++sub)
+    {
+        for (int32_t m = gid; m < movers; m += gsz)
+        {
+            m3IntegrateVelocity(&v, v.movers[m], sp.h, sp.gravity, sp.maxLinear, sp.maxAngular);
+        }
+        grid.sync();
+        for (int mode = 0; mode < 3; ++mode)
+        {
+            if (mode == 2)
+            {
+                for (int32_t m = gid; m < movers; m += gsz)
+                {
+                    m3IntegratePosition(&v, v.movers[m], sp.h);
+                }
+                grid.sync();
+            }
+            for (int32_t color = 0; color <= M3_GRAPH_COLORS; ++color)
+            {
+                int32_t start = starts[color];
+                int32_t end = starts[color + 1];
+                if (end == start)
+                {
+                    continue;
+                }
+                ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS);
+                grid.sync();
+            }
+        }
+    }
+}
+
 __global__ void k_clear_forces(m3View v)
 {
     int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -885,15 +1139,11 @@ __global__ void k_sleep_ready(m3View v, const int32_t* root)
     }
 }
 
-__global__ void k_sleep_freeze(m3View v, const int32_t* root, uint8_t* frozenNow)
+__global__ void k_sleep_freeze(m3View v, const int32_t* root)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody)
-    {
-        return;
-    }
-    frozenNow[i] = 0;
-    if (v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC || v.awake[i] == 0)
+    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
+        v.awake[i] == 0)
     {
         return;
     }
@@ -902,39 +1152,8 @@ __global__ void k_sleep_freeze(m3View v, const int32_t* root, uint8_t* frozenNow
         v.awake[i] = 0;
         v.lv[i] = m3V3(0.0f, 0.0f, 0.0f);
         v.av[i] = m3V3(0.0f, 0.0f, 0.0f);
-        frozenNow[i] = 1;
+        v.pending[i] = 1; // frozen pairs are recorded by the next sweep
         atomicAdd(&v.sc->discoverCount, 1);
-    }
-}
-
-__global__ void k_fresh_if_frozen(m3View v)
-{
-    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= v.maxShape || v.sc->stall != 0 || v.sc->discoverCount == 0)
-    {
-        return;
-    }
-    if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE)
-    {
-        double box[6];
-        m3FreshBox(&v, s, box);
-        for (int k = 0; k < 6; ++k)
-        {
-            v.fresh[6 * s + k] = box[k];
-        }
-    }
-}
-
-__global__ void k_discover_frozen(m3View v, const uint8_t* frozenNow)
-{
-    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= v.maxShape || v.sc->stall != 0 || v.sc->discoverCount == 0 || v.shapeAlive[s] == 0)
-    {
-        return;
-    }
-    if (frozenNow[v.shapeBody[s]] != 0)
-    {
-        DiscoverShape(v, s);
     }
 }
 
@@ -1028,10 +1247,13 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.colorLists, np) || DevAlloc(&v.movers, nb) || DevAlloc(&v.deltaPos, nb) ||
         DevAlloc(&v.deltaRot, nb) || DevAlloc(&v.com0, nb) || DevAlloc(&v.rot0, nb) ||
         DevAlloc(&v.parent, nb) || DevAlloc(&v.forced, nb) || DevAlloc(&v.flag, nb) ||
+        DevAlloc(&v.pending, nb) ||
         DevAlloc(&v.sc, 1) || DevAlloc(&w->sapKeysIn, ns) || DevAlloc(&w->sapKeysOut, ns) ||
         DevAlloc(&w->sapValsIn, ns) || DevAlloc(&w->sapValsOut, ns) ||
+        DevAlloc(&w->sapXY, ns) || DevAlloc(&w->sapZ, ns) || DevAlloc(&w->sapAttr, ns) ||
         DevAlloc(&w->pairFlags, np) || DevAlloc(&w->bodyFlags, nb) || DevAlloc(&w->root, nb) ||
-        DevAlloc(&w->frozenNow, nb) || DevAlloc(&w->bodyMasks, nb))
+        DevAlloc(&w->frozenNow, nb) || DevAlloc(&w->bodyMasks, nb) ||
+        DevAlloc(&w->bodyOwner, nb) || DevAlloc(&w->iota, np) || DevAlloc(&w->colorsSorted, np))
     {
         return -1;
     }
@@ -1060,7 +1282,28 @@ static int CreateImpl(m3gpuWorld* w)
     HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->bodyFlags, v.movers,
                                           &v.sc->moverCount, nb, w->stream));
     need = bytes > need ? bytes : need;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v.colors, w->colorsSorted, w->iota,
+                                               v.colorLists, np, 0, 5, w->stream));
+    need = bytes > need ? bytes : need;
+    hipLaunchKernelGGL(k_iota, dim3(Blocks(np, 256)), dim3(256), 0, w->stream, w->iota, np);
     w->cubBytes = need > 0 ? need : 1;
+    {
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        if (!prop.cooperativeLaunch)
+        {
+            snprintf(s_lastError, sizeof(s_lastError), "device lacks cooperative launch");
+            return -1;
+        }
+        int perCu = 0;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_substeps, 256, 0));
+        int maxBlocks = perCu * prop.multiProcessorCount;
+        const char* env = getenv("MAUL3D_AMD_SUBSTEP_BLOCKS");
+        int want = env != nullptr ? atoi(env) : 16;
+        w->coopBlocks = want < 1 ? 1 : (want > maxBlocks ? maxBlocks : want);
+    }
     HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
     w->cur = 0;
     v.keys = w->keysBuf[0];
@@ -1114,9 +1357,9 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     (void*)v.hulls, w->keysBuf[0], w->keysBuf[1], w->mansBuf[0], w->mansBuf[1],
                     v.sleepKeys, v.emit, w->emitSorted, w->emitUnique, v.cons, v.consPair,
                     v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
-                    v.parent, v.forced, v.flag, v.sc, w->sapKeysIn, w->sapKeysOut, w->sapValsIn,
-                    w->sapValsOut, w->pairFlags, w->bodyFlags, w->root, w->frozenNow,
-                    w->bodyMasks, w->cubTemp};
+                    v.parent, v.forced, v.flag, v.pending, v.sc, w->sapKeysIn, w->sapKeysOut, w->sapValsIn,
+                    w->sapValsOut, w->sapXY, w->sapZ, w->sapAttr, w->pairFlags, w->bodyFlags, w->root, w->frozenNow,
+                    w->bodyMasks, w->bodyOwner, w->iota, w->colorsSorted, w->cubTemp};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     {
         if (ptrs[i] != nullptr)
@@ -1311,23 +1554,23 @@ extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
 // Stage names for the profile (event-timed on the world's stream).
 enum m3Stage
 {
-    ST_BEGIN, ST_FILL, ST_FRESH, ST_DISCOVER_ALL, ST_REFRESH, ST_SAP_SORT, ST_SAP, ST_PLANES,
+    ST_BEGIN, ST_FILL, ST_FRESH, ST_DISCOVER_ALL, ST_REFRESH, ST_SAP_SORT, ST_SAP_GATHER, ST_SAP, ST_PLANES,
     ST_FROZEN, ST_EMIT_SORT, ST_UNIQUE, ST_PAIR_COUNT, ST_EXPAND, ST_STALL_RESTORE,
     ST_SLEEP_SELECT, ST_NARROW, ST_CAPTURE, ST_UNION, ST_ROOT, ST_WAKE_MARK, ST_WAKE_APPLY,
-    ST_CONS_FLAGS, ST_CONS_SELECT, ST_PREPARE, ST_COLOR, ST_MOVER_FLAGS, ST_MOVER_SELECT,
+    ST_CONS_FLAGS, ST_CONS_SELECT, ST_PREPARE, ST_COLOR, ST_COLOR_LISTS, ST_MOVER_FLAGS, ST_MOVER_SELECT,
     ST_INTEGRATE_V, ST_WARM, ST_WARM_OVERFLOW, ST_SOLVE, ST_SOLVE_OVERFLOW, ST_INTEGRATE_X,
-    ST_RELAX, ST_RELAX_OVERFLOW, ST_CLEAR_FORCES, ST_RESTITUTION, ST_STORE, ST_CCD,
+    ST_RELAX, ST_RELAX_OVERFLOW, ST_CLEAR_FORCES, ST_SUBSTEPS, ST_RESTITUTION, ST_STORE, ST_CCD,
     ST_SLEEP_TIMERS, ST_SLEEP_READY, ST_SLEEP_FREEZE, ST_FRESH_FROZEN, ST_DISCOVER_FROZEN, ST_END,
     ST_COUNT
 };
 
 static const char* const kStageNames[ST_COUNT] = {
-    "begin", "fill_emit", "fresh_bounds", "discover_all", "refresh_proxies", "sap_sort", "sap_sweep",
+    "begin", "fill_emit", "fresh_bounds", "discover_all", "refresh_proxies", "sap_sort", "sap_gather", "sap_sweep",
     "plane_pairs", "frozen_merge", "pair_sort", "pair_unique", "pair_count", "pair_expand",
     "stall_restore", "frozen_select", "narrowphase", "capture", "island_union", "island_root",
-    "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "prepare", "coloring",
+    "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "prepare", "coloring", "color_lists",
     "mover_flags", "mover_select", "integrate_velocities", "warm_start", "warm_start_overflow",
-    "solve", "solve_overflow", "integrate_positions", "relax", "relax_overflow", "clear_forces",
+    "solve", "solve_overflow", "integrate_positions", "relax", "relax_overflow", "clear_forces", "substeps",
     "restitution", "store_impulses", "continuous", "sleep_timers", "sleep_ready", "sleep_freeze",
     "fresh_bounds_frozen", "discover_frozen", "end"};
 
@@ -1394,20 +1637,25 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     STAGE(ST_FRESH, KL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysIn, w->sapValsIn));
     if (w->fullQuery)
     {
+        // After a restore the frozen buffer is rebuilt from every sleeping
+        // body (src/broad_phase.c:432-448).
         KL(k_reset_sleep, dim3(1), dim3(1), 0, st, v);
-        STAGE(ST_DISCOVER_ALL, KL(k_discover_all, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
+        STAGE(ST_DISCOVER_ALL, KL(k_pending_all, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
         w->fullQuery = 0;
     }
-    STAGE(ST_REFRESH, KL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
     STAGE(ST_SAP_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
                            ct, tb, w->sapKeysIn, w->sapKeysOut, w->sapValsIn, w->sapValsOut, ns, 0,
                            64, st)));
-    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysOut, w->sapValsOut));
+    STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysOut,
+                            w->sapValsOut, w->sapXY, w->sapZ, w->sapAttr));
+    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, 64)), dim3(64), 0, st, v, w->sapXY, w->sapZ,
+                     w->sapAttr));
     if (v.planeCount > 0)
     {
         STAGE(ST_PLANES,
               KL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, v));
     }
+    STAGE(ST_REFRESH, KL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
     STAGE(ST_FROZEN, KL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, v));
     STAGE(ST_EMIT_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ct, tb, v.emit, w->emitSorted,
                                                                    ne, 0, 32, st)));
@@ -1439,37 +1687,39 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
                                                                 &v.sc->constraintCount, np, st)));
     STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, v, p->contactHertz,
                          p->contactDampingRatio, h));
-    STAGE(ST_COLOR, KL(k_color, dim3(1), dim3(M3_COLOR_CHUNK), 0, st, v, w->bodyMasks));
+    if (nb <= M3_COLOR_LDS_BODIES)
+    {
+        STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, v, w->bodyMasks,
+                           w->bodyOwner));
+    }
+    else
+    {
+        STAGE(ST_COLOR, KL(k_color<false>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, v,
+                           w->bodyMasks, w->bodyOwner));
+    }
+    // Stable radix sort on the color byte: each color's list ascends.
+    STAGE(ST_COLOR_LISTS, HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
+                              ct, tb, v.colors, w->colorsSorted, w->iota, v.colorLists, np, 0, 5,
+                              st)));
     STAGE(ST_MOVER_FLAGS,
           KL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->bodyFlags, p->dt));
     STAGE(ST_MOVER_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, w->bodyFlags,
                                                                  v.movers, &v.sc->moverCount, nb,
                                                                  st)));
-    // Substeps.
-    m3Vec3 g = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
-    unsigned passBlocks = Blocks(np, 128) < 512 ? Blocks(np, 128) : 512;
-    float push = p->contactPushMaxSpeed;
-    for (int32_t sub = 0; sub < p->substeps; ++sub)
+    // Substeps: one cooperative launch.
+    m3SubstepParams sp;
+    sp.substeps = p->substeps;
+    sp.h = h;
+    sp.invH = invH;
+    sp.push = p->contactPushMaxSpeed;
+    sp.maxLinear = p->maximumLinearSpeed;
+    sp.maxAngular = p->maximumAngularSpeed;
+    sp.gravity = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
     {
-        STAGE(ST_INTEGRATE_V, KL(k_integrate_v, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h, g,
-                                 p->maximumLinearSpeed, p->maximumAngularSpeed));
-        for (int mode = 0; mode < 2; ++mode)
-        {
-            for (int32_t color = 0; color < M3_GRAPH_COLORS; ++color)
-            {
-                STAGE(mode == 0 ? ST_WARM : ST_SOLVE,
-                      KL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, mode, invH, push));
-            }
-            STAGE(mode == 0 ? ST_WARM_OVERFLOW : ST_SOLVE_OVERFLOW,
-                  KL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, mode, invH, push));
-        }
-        STAGE(ST_INTEGRATE_X, KL(k_integrate_x, dim3(Blocks(nb, T)), dim3(T), 0, st, v, h));
-        for (int32_t color = 0; color < M3_GRAPH_COLORS; ++color)
-        {
-            STAGE(ST_RELAX,
-                  KL(k_color_pass, dim3(passBlocks), dim3(128), 0, st, v, color, 2, invH, push));
-        }
-        STAGE(ST_RELAX_OVERFLOW, KL(k_overflow_pass, dim3(1), dim3(1), 0, st, v, 2, invH, push));
+        void* args[] = {(void*)&v, (void*)&sp};
+        STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps,
+                                                              dim3(w->coopBlocks), dim3(256), args,
+                                                              0, st)));
     }
     STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
     STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, v, p->restitutionThreshold));
@@ -1483,11 +1733,7 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     if (p->sleepEnabled)
     {
         STAGE(ST_SLEEP_READY, KL(k_sleep_ready, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
-        STAGE(ST_SLEEP_FREEZE, KL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root,
-                                  w->frozenNow));
-        STAGE(ST_FRESH_FROZEN, KL(k_fresh_if_frozen, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
-        STAGE(ST_DISCOVER_FROZEN, KL(k_discover_frozen, dim3(Blocks(ns, 64)), dim3(64), 0, st, v,
-                                     w->frozenNow));
+        STAGE(ST_SLEEP_FREEZE, KL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
     }
     STAGE(ST_END, KL(k_end, dim3(1), dim3(1), 0, st, v));
     HIP_TRY(hipGetLastError());

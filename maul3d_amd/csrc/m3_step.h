@@ -167,6 +167,7 @@ typedef struct m3View
     int32_t* parent;
     uint8_t* forced;
     uint8_t* flag;
+    uint8_t* pending; // bodies whose frozen pairs the next sweep records
     m3Scalars* sc;
 } m3View;
 
@@ -308,11 +309,15 @@ M3_HD int m3ShapeQueries(const m3View* v, int32_t s)
     return v->type[b] == M3_STATIC || v->awake[b] != 0;
 }
 
-// Would the reference's tree pass emit the overlapping pair (a, b)?  The
-// query runs from the lower index when both sides query, otherwise from the
-// querying side; the hit needs the other side's stored proxy box to
-// overlap the querier's fresh box (src/broad_phase.c:285-349,
-// src/dynamic_tree.c:367-406), then the fresh re-test.
+// Would the reference's tree pass emit the pair (a, b), whose fresh boxes
+// overlap?  The query runs from the querying side (from the lower index
+// when both query) and hits the other shape when that shape's stored proxy
+// box overlaps the querier's fresh box, then re-tests fresh boxes
+// (src/broad_phase.c:285-349, src/dynamic_tree.c:367-406).  A querying
+// shape's proxy is refreshed to contain its fresh box before the queries
+// run, so for two queriers the proxy test is implied by the fresh overlap;
+// a non-querying shape's proxy is never refreshed, so its stored box is
+// tested as it stands.  This lets the test run before the refresh.
 M3_HD int m3TreePairEmits(const m3View* v, int32_t a, int32_t b)
 {
     int qa = m3ShapeQueries(v, a);
@@ -325,23 +330,23 @@ M3_HD int m3TreePairEmits(const m3View* v, int32_t a, int32_t b)
     {
         return 0;
     }
-    int32_t self;
-    int32_t other;
     if (qa && qb)
     {
-        self = a < b ? a : b;
-        other = a < b ? b : a;
+        return 1;
     }
-    else
-    {
-        self = qa ? a : b;
-        other = qa ? b : a;
-    }
-    if (!m3BoxOverlap(&v->leaf[6 * other], &v->fresh[6 * self]))
-    {
-        return 0;
-    }
-    return m3BoxOverlap(&v->fresh[6 * self], &v->fresh[6 * other]);
+    int32_t self = qa ? a : b;
+    int32_t other = qa ? b : a;
+    return m3BoxOverlap(&v->leaf[6 * other], &v->fresh[6 * self]);
+}
+
+// Frozen-pair discovery of `self` (a shape of a body that froze): the tree
+// hit needs the other proxy's stored box to overlap self's fresh box, plus
+// the pair filter (src/broad_phase.c:351-383); the fresh overlap is the
+// caller's.
+M3_HD int m3DiscoverHit(const m3View* v, int32_t self, int32_t other)
+{
+    return m3BoxOverlap(&v->leaf[6 * other], &v->fresh[6 * self]) &&
+           m3PairAllowed(v, self, other);
 }
 
 // Compact 32-bit pair key: low shape in the high half.
@@ -1103,6 +1108,55 @@ M3_HD void m3WarmStartOne(const m3View* v, m3Constraint* c)
     {
         v->av[c->bodyB] = m3Add3(v->av[c->bodyB], m3MulMV3(c->invIB, tw));
     }
+}
+
+// The warm-start updates of constraint c applied to one of its bodies only
+// (side 0 = A, 1 = B).  WarmStartOne's impulses depend on stored
+// accumulators, never on velocities, so replaying each body's constraints
+// in color order with this function performs exactly the operations the
+// colored pass performs on that body, in the same order.
+M3_HD void m3WarmStartSide(const m3View* v, const m3Constraint* c, int side)
+{
+    int32_t b = side == 0 ? c->bodyA : c->bodyB;
+    m3Vec3 lin = v->lv[b];
+    m3Vec3 ang = v->av[b];
+    for (int32_t k = 0; k < c->pointCount; ++k)
+    {
+        const m3CPoint* cp = &c->points[k];
+        m3Vec3 impulse = m3MulSV3(cp->normalImpulse, c->normal);
+        if (side == 0)
+        {
+            lin = m3Sub3(lin, m3MulSV3(c->invMassA, impulse));
+            ang = m3Sub3(ang, m3MulMV3(c->invIA, m3Cross3(cp->rA, impulse)));
+        }
+        else
+        {
+            lin = m3Add3(lin, m3MulSV3(c->invMassB, impulse));
+            ang = m3Add3(ang, m3MulMV3(c->invIB, m3Cross3(cp->rB, impulse)));
+        }
+    }
+    m3Vec3 f = m3Add3(m3MulSV3(c->frictionImpulse1, c->t1), m3MulSV3(c->frictionImpulse2, c->t2));
+    if (side == 0)
+    {
+        lin = m3Sub3(lin, m3MulSV3(c->invMassA, f));
+        ang = m3Sub3(ang, m3MulMV3(c->invIA, m3Cross3(c->originA, f)));
+    }
+    else
+    {
+        lin = m3Add3(lin, m3MulSV3(c->invMassB, f));
+        ang = m3Add3(ang, m3MulMV3(c->invIB, m3Cross3(c->originB, f)));
+    }
+    m3Vec3 tw = m3Add3(m3MulSV3(c->twistImpulse, c->normal), c->rollingImpulse);
+    if (side == 0)
+    {
+        ang = m3Sub3(ang, m3MulMV3(c->invIA, tw));
+    }
+    else
+    {
+        ang = m3Add3(ang, m3MulMV3(c->invIB, tw));
+    }
+    v->lv[b] = lin;
+    v->av[b] = ang;
 }
 
 // Cramer's rule; singular returns zero (src/solver.c:2909-2926).

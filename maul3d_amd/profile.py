@@ -14,6 +14,7 @@ per unit:
 * narrowphase, per pair: 572 B (key, two transforms, centers of mass and
   shape records read, the previous manifold read, the 188 B manifold written)
 * prepare, per constraint: 454 B read + the constraint record written
+* substeps (the fused substep loop): the sum of the five substep stages
 """
 
 from __future__ import annotations
@@ -58,6 +59,8 @@ def stage_bytes(name: str, p: GpuProfile) -> float | None:
         return sub * 142.0 * m
     if name == "integrate_positions":
         return sub * 173.0 * m
+    if name == "substeps":
+        return sub * ((370.0 * 2 + 270.0) * c + (48.0 * 2 + 28.0) * q + (142.0 + 173.0) * m)
     if name == "narrowphase":
         return 572.0 * p.sumPairs
     if name == "prepare":
