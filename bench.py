@@ -35,9 +35,14 @@ def parse() -> argparse.Namespace:
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=600)
     p.add_argument("--warmup", type=int, default=0)
-    p.add_argument("--cpu-steps", type=int, default=150, help="reference steps timed on the host")
+    p.add_argument("--cpu-steps", type=int, default=300, help="reference steps timed on the host (steps 1..N of the same trajectory)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
+    p.add_argument(
+        "--copy-calibration",
+        action="store_true",
+        help="run only a plain 256 MiB device copy (calibrates HBM byte counters)",
+    )
     return p.parse_args()
 
 
@@ -111,8 +116,25 @@ def run_cpu(steps: int) -> dict | None:
     }
 
 
+def copy_calibration() -> None:
+    """A plain elementwise copy of known size, the scale for FETCH_SIZE and
+    WRITE_SIZE counter readings of the step kernels."""
+    import torch
+
+    n = 64 << 20
+    src = torch.ones(n, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+
+
 def main() -> None:
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(
             "bench.py: this build runs on exactly one GPU (worlds shard across GPUs "

@@ -5,7 +5,6 @@
 // host round trip: every count the later stages need (pairs, constraints,
 // movers, colors) stays in device memory (m3Scalars) and kernels read it.
 // Stage order and semantics follow m3StepInternal (src/solver.c:3662-4320).
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -60,16 +59,16 @@ struct m3gpuWorld
     uint8_t* pairFlags;
     uint8_t* bodyFlags;
     int32_t* root;
-    uint8_t* frozenNow;
     uint16_t* bodyMasks;
     int32_t* bodyOwner;
     int32_t* iota;
-    uint8_t* colorsSorted;
     void* cubTemp;
     size_t cubBytes;
     m3Scalars* hostScalars; // pinned mirror for stats
     struct m3Profiler* prof;
-    int coopBlocks; // co-resident blocks of the substep kernel
+    int coopBlocks;    // co-resident blocks of the substep kernel
+    int coopThreads;   // threads per block of the substep kernel
+    uint32_t* barrier; // substep grid barrier: arrival count, generation
 };
 
 // ---------------------------------------------------------------------------
@@ -653,6 +652,7 @@ __global__ void k_root(m3View v, int32_t* root)
         return;
     }
     root[i] = Find(v.parent, i);
+    v.slotMask[i] = 0u;
 }
 
 __global__ void k_wake_mark(m3View v, const int32_t* root)
@@ -696,17 +696,64 @@ __global__ void k_constraint_flags(m3View v, uint8_t* flags)
     flags[i] = (v.sc->stall == 0 && i < v.sc->pairCount && m3PairIsConstraint(&v, i)) ? 1 : 0;
 }
 
-__global__ void k_prepare(m3View v, float hertz, float zeta, float h)
+// The awake dynamic bodies of each constraint, the only bodies coloring
+// tracks (src/solver.c:2741-2822).
+__global__ void k_cons_bodies(m3View v)
 {
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0 || c >= v.sc->constraintCount)
     {
         return;
     }
+    uint64_t key = v.keys[v.consPair[c]];
+    int32_t a = v.shapeBody[(int32_t)(key >> 32)];
+    int32_t b = v.shapeBody[(int32_t)(key & 0xFFFFFFFFu)];
+    v.consBodies[2 * c] = (v.type[a] == M3_DYNAMIC && v.awake[a] != 0) ? a : -1;
+    v.consBodies[2 * c + 1] = (v.type[b] == M3_DYNAMIC && v.awake[b] != 0) ? b : -1;
+}
+
+__device__ static inline void RegisterSlot(const m3View& v, int32_t body, int32_t slot, int side,
+                                           int32_t color)
+{
+    if (color < M3_GRAPH_COLORS)
+    {
+        v.slotTable[(int64_t)body * M3_GRAPH_COLORS + color] = (slot << 1) | side;
+        atomicOr(&v.slotMask[body], 1u << color);
+    }
+    else
+    {
+        atomicOr(&v.slotMask[body], 0x10000u);
+    }
+}
+
+// Prepares constraints straight into color-slot order (the substep passes
+// then read slot k without an index hop) and records each dynamic body's
+// slots.  A dynamic body in a constraint is awake: it touches an awake
+// dynamic body or a static/kinematic one, and touching dynamic bodies share
+// an island, which the wake pass woke whole.  So coloring gave it at most
+// one slot per color.
+__global__ void k_prepare(m3View v, float hertz, float zeta, float h)
+{
+    int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || k >= v.sc->constraintCount)
+    {
+        return;
+    }
     m3Softness soft = m3MakeSoft(hertz, zeta, h);
     m3Softness staticSoft = m3MakeSoft(2.0f * hertz, zeta, h);
-    m3Constraint* con = &v.cons[c];
+    int32_t c = v.colorLists[k];
+    m3Constraint* con = &v.cons[k];
     m3PrepareContact(&v, v.consPair[c], con, soft, staticSoft);
+    v.slotOf[c] = k;
+    int32_t color = v.slotColors[k];
+    if (con->dynA)
+    {
+        RegisterSlot(v, con->bodyA, k, 0, color);
+    }
+    if (con->dynB)
+    {
+        RegisterSlot(v, con->bodyB, k, 1, color);
+    }
     atomicAdd(&v.sc->pointCount, con->pointCount);
     if (con->restitution > 0.0f)
     {
@@ -759,17 +806,8 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
         int pend = c < n;
         if (pend)
         {
-            const m3Constraint* con = &v.cons[c];
-            int32_t a = con->bodyA;
-            int32_t b = con->bodyB;
-            if (v.type[a] == M3_DYNAMIC && v.awake[a] != 0)
-            {
-                A = a;
-            }
-            if (v.type[b] == M3_DYNAMIC && v.awake[b] != 0)
-            {
-                B = b;
-            }
+            A = v.consBodies[2 * c];
+            B = v.consBodies[2 * c + 1];
         }
         while (true)
         {
@@ -882,75 +920,6 @@ __global__ void k_mover_flags(m3View v, uint8_t* flags, float dt)
     flags[i] = f;
 }
 
-__global__ void k_integrate_v(m3View v, float h, m3Vec3 gravity, float maxLin, float maxAng)
-{
-    int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || m >= v.sc->moverCount)
-    {
-        return;
-    }
-    m3IntegrateVelocity(&v, v.movers[m], h, gravity, maxLin, maxAng);
-}
-
-__global__ void k_integrate_x(m3View v, float h)
-{
-    int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || m >= v.sc->moverCount)
-    {
-        return;
-    }
-    m3IntegratePosition(&v, v.movers[m], h);
-}
-
-// One graph color of one pass: constraints in a color share no awake
-// dynamic body, so they run concurrently; mode 0 warm start, 1 solve with
-// bias, 2 relax.
-__global__ void k_color_pass(m3View v, int32_t color, int32_t mode, float invH, float pushMax)
-{
-    if (v.sc->stall != 0)
-    {
-        return;
-    }
-    int32_t start = v.sc->colorStarts[color];
-    int32_t end = v.sc->colorStarts[color + 1];
-    for (int32_t k = start + blockIdx.x * blockDim.x + threadIdx.x; k < end;
-         k += gridDim.x * blockDim.x)
-    {
-        m3Constraint* c = &v.cons[v.colorLists[k]];
-        if (mode == 0)
-        {
-            m3WarmStartOne(&v, c);
-        }
-        else
-        {
-            m3SolveContact(&v, c, invH, pushMax, mode == 1);
-        }
-    }
-}
-
-// The overflow color runs serially in ascending constraint order.
-__global__ void k_overflow_pass(m3View v, int32_t mode, float invH, float pushMax)
-{
-    if (v.sc->stall != 0)
-    {
-        return;
-    }
-    int32_t start = v.sc->colorStarts[M3_GRAPH_COLORS];
-    int32_t end = v.sc->colorStarts[M3_GRAPH_COLORS + 1];
-    for (int32_t k = start; k < end; ++k)
-    {
-        m3Constraint* c = &v.cons[v.colorLists[k]];
-        if (mode == 0)
-        {
-            m3WarmStartOne(&v, c);
-        }
-        else
-        {
-            m3SolveContact(&v, c, invH, pushMax, mode == 1);
-        }
-    }
-}
-
 struct m3SubstepParams
 {
     int32_t substeps;
@@ -962,8 +931,52 @@ struct m3SubstepParams
     m3Vec3 gravity;
 };
 
-// One graph color of one pass over the grid (mode 0 warm start, 1 solve
-// with bias, 2 relax); the overflow color runs serially on one thread.
+// Grid-wide barrier of the substep kernel.  The fences are those of a
+// cooperative-groups grid sync (agent-scope release before arriving,
+// acquire once the generation flips); arrival is one counter atomic per
+// block.  All blocks are co-resident (cooperative launch).  The wait is
+// bounded: a barrier that times out marks the world faulted (the host
+// reports it at the next synchronization) and later barriers of a faulted
+// world pass at once, so every block still runs to completion.
+#define M3_BARRIER_SPINS (1u << 19)
+
+__device__ static inline void GridBarrier(uint32_t* bar, int32_t* fault)
+{
+    __syncthreads();
+    if (threadIdx.x == 0 &&
+        __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    {
+        uint32_t gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t prev =
+            __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == gridDim.x - 1u)
+        {
+            __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        else
+        {
+            // Relaxed polling reads the coherent copy without invalidating
+            // this XCD's L2 on every poll; one acquire fence follows.
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+            {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins == M3_BARRIER_SPINS ||
+                    __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                {
+                    __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+    }
+    __syncthreads();
+}
+
+// One graph color of one pass over the grid (mode 1 solve with bias,
+// 2 relax); the overflow color runs serially on one thread.
 __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t end, int mode,
                                          const m3SubstepParams& sp, int32_t gid, int32_t gsz,
                                          bool serial)
@@ -974,42 +987,30 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
         {
             for (int32_t k = start; k < end; ++k)
             {
-                m3Constraint* c = &v.cons[v.colorLists[k]];
-                if (mode == 0)
-                {
-                    m3WarmStartOne(&v, c);
-                }
-                else
-                {
-                    m3SolveContact(&v, c, sp.invH, sp.push, mode == 1);
-                }
+                m3SolveContact(&v, &v.cons[k], sp.invH, sp.push, mode == 1);
             }
         }
         return;
     }
     for (int32_t k = start + gid; k < end; k += gsz)
     {
-        m3Constraint* c = &v.cons[v.colorLists[k]];
-        if (mode == 0)
-        {
-            m3WarmStartOne(&v, c);
-        }
-        else
-        {
-            m3SolveContact(&v, c, sp.invH, sp.push, mode == 1);
-        }
+        m3SolveContact(&v, &v.cons[k], sp.invH, sp.push, mode == 1);
     }
 }
 
+#define M3_SUBSTEP_THREADS 256
+
 // The whole substep loop of src/solver.c:4085-4203 in one cooperative
-// launch: integrate velocities, warm start by color, solve by color,
-// integrate positions, relax by color, with a grid barrier after each phase.
+// launch, with a grid barrier after each phase:
+//   integrate velocities and warm start (one phase: a warm-start impulse
+//   depends only on stored accumulators, so each body replays its own
+//   constraints in color order, the exact operation sequence of the colored
+//   pass on that body), solve by color, integrate positions, relax by color.
 // Every branch depends only on counts written before the launch, so all
 // blocks reach the same barriers.
-__global__ void __launch_bounds__(256) k_substeps(m3View v, m3SubstepParams sp)
+__global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
+    k_substeps(m3View v, m3SubstepParams sp, uint32_t* bar)
 {
-    namespace cg = cooperative_groups;
-    cg::grid_group grid = cg::this_grid();
     if (v.sc->stall != 0)
     {
         return;
@@ -1026,10 +1027,12 @@ __global__ void __launch_bounds__(256) k_substeps(m3View v, m3SubstepParams sp)
     {
         for (int32_t m = gid; m < movers; m += gsz)
         {
-            m3IntegrateVelocity(&v, v.movers[m], sp.h, sp.gravity, sp.maxLinear, sp.maxAngular);
+            int32_t b = v.movers[m];
+            m3IntegrateVelocity(&v, b, sp.h, sp.gravity, sp.maxLinear, sp.maxAngular);
+            m3WarmStartBody(&v, b, starts[M3_GRAPH_COLORS], starts[M3_GRAPH_COLORS + 1]);
         }
-        grid.sync();
-        for (int mode = 0; mode < 3; ++mode)
+        GridBarrier(bar, &v.sc->fault);
+        for (int mode = 1; mode < 3; ++mode)
         {
             if (mode == 2)
             {
@@ -1037,7 +1040,7 @@ __global__ void __launch_bounds__(256) k_substeps(m3View v, m3SubstepParams sp)
                 {
                     m3IntegratePosition(&v, v.movers[m], sp.h);
                 }
-                grid.sync();
+                GridBarrier(bar, &v.sc->fault);
             }
             for (int32_t color = 0; color <= M3_GRAPH_COLORS; ++color)
             {
@@ -1048,7 +1051,7 @@ __global__ void __launch_bounds__(256) k_substeps(m3View v, m3SubstepParams sp)
                     continue;
                 }
                 ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS);
-                grid.sync();
+                GridBarrier(bar, &v.sc->fault);
             }
         }
     }
@@ -1075,7 +1078,7 @@ __global__ void k_restitution(m3View v, float threshold)
     int32_t n = v.sc->constraintCount;
     for (int32_t c = 0; c < n; ++c)
     {
-        m3RestitutionOne(&v, &v.cons[c], threshold);
+        m3RestitutionOne(&v, &v.cons[v.slotOf[c]], threshold);
     }
 }
 
@@ -1252,8 +1255,11 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&w->sapValsIn, ns) || DevAlloc(&w->sapValsOut, ns) ||
         DevAlloc(&w->sapXY, ns) || DevAlloc(&w->sapZ, ns) || DevAlloc(&w->sapAttr, ns) ||
         DevAlloc(&w->pairFlags, np) || DevAlloc(&w->bodyFlags, nb) || DevAlloc(&w->root, nb) ||
-        DevAlloc(&w->frozenNow, nb) || DevAlloc(&w->bodyMasks, nb) ||
-        DevAlloc(&w->bodyOwner, nb) || DevAlloc(&w->iota, np) || DevAlloc(&w->colorsSorted, np))
+        DevAlloc(&w->bodyMasks, nb) || DevAlloc(&w->bodyOwner, nb) ||
+        DevAlloc(&w->iota, np) || DevAlloc(&v.slotColors, np) ||
+        DevAlloc(&v.consBodies, 2 * (int64_t)np) || DevAlloc(&v.slotOf, np) ||
+        DevAlloc(&v.slotTable, (int64_t)M3_GRAPH_COLORS * nb) || DevAlloc(&v.slotMask, nb) ||
+        DevAlloc(&w->barrier, 4))
     {
         return -1;
     }
@@ -1282,7 +1288,7 @@ static int CreateImpl(m3gpuWorld* w)
     HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->bodyFlags, v.movers,
                                           &v.sc->moverCount, nb, w->stream));
     need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v.colors, w->colorsSorted, w->iota,
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v.colors, v.slotColors, w->iota,
                                                v.colorLists, np, 0, 5, w->stream));
     need = bytes > need ? bytes : need;
     hipLaunchKernelGGL(k_iota, dim3(Blocks(np, 256)), dim3(256), 0, w->stream, w->iota, np);
@@ -1298,10 +1304,11 @@ static int CreateImpl(m3gpuWorld* w)
             return -1;
         }
         int perCu = 0;
-        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_substeps, 256, 0));
+        w->coopThreads = M3_SUBSTEP_THREADS;
+        HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_substeps, w->coopThreads, 0));
         int maxBlocks = perCu * prop.multiProcessorCount;
         const char* env = getenv("MAUL3D_AMD_SUBSTEP_BLOCKS");
-        int want = env != nullptr ? atoi(env) : 16;
+        int want = env != nullptr ? atoi(env) : 32;
         w->coopBlocks = want < 1 ? 1 : (want > maxBlocks ? maxBlocks : want);
     }
     HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
@@ -1358,8 +1365,9 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.sleepKeys, v.emit, w->emitSorted, w->emitUnique, v.cons, v.consPair,
                     v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
                     v.parent, v.forced, v.flag, v.pending, v.sc, w->sapKeysIn, w->sapKeysOut, w->sapValsIn,
-                    w->sapValsOut, w->sapXY, w->sapZ, w->sapAttr, w->pairFlags, w->bodyFlags, w->root, w->frozenNow,
-                    w->bodyMasks, w->bodyOwner, w->iota, w->colorsSorted, w->cubTemp};
+                    w->sapValsOut, w->sapXY, w->sapZ, w->sapAttr, w->pairFlags, w->bodyFlags, w->root,
+                    w->bodyMasks, w->bodyOwner, w->iota, v.slotColors, v.consBodies, v.slotOf,
+                    v.slotTable, v.slotMask, w->barrier, w->cubTemp};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     {
         if (ptrs[i] != nullptr)
@@ -1513,6 +1521,20 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
     return 0;
 }
 
+// A world whose substep barrier timed out holds no valid state.
+static int CheckFault(m3gpuWorld* w)
+{
+    HIP_TRY(hipMemcpyAsync(&w->hostScalars->fault, &w->view.sc->fault, sizeof(int32_t),
+                           hipMemcpyDeviceToHost, w->stream));
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    if (w->hostScalars->fault != 0)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "substep grid barrier timed out");
+        return -1;
+    }
+    return 0;
+}
+
 extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
 {
     if (w == nullptr || h == nullptr)
@@ -1548,7 +1570,7 @@ extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
         *h->frozenDirty = 0;
     }
     HIP_TRY(hipStreamSynchronize(w->stream));
-    return 0;
+    return CheckFault(w);
 }
 
 // Stage names for the profile (event-timed on the world's stream).
@@ -1557,22 +1579,21 @@ enum m3Stage
     ST_BEGIN, ST_FILL, ST_FRESH, ST_DISCOVER_ALL, ST_REFRESH, ST_SAP_SORT, ST_SAP_GATHER, ST_SAP, ST_PLANES,
     ST_FROZEN, ST_EMIT_SORT, ST_UNIQUE, ST_PAIR_COUNT, ST_EXPAND, ST_STALL_RESTORE,
     ST_SLEEP_SELECT, ST_NARROW, ST_CAPTURE, ST_UNION, ST_ROOT, ST_WAKE_MARK, ST_WAKE_APPLY,
-    ST_CONS_FLAGS, ST_CONS_SELECT, ST_PREPARE, ST_COLOR, ST_COLOR_LISTS, ST_MOVER_FLAGS, ST_MOVER_SELECT,
-    ST_INTEGRATE_V, ST_WARM, ST_WARM_OVERFLOW, ST_SOLVE, ST_SOLVE_OVERFLOW, ST_INTEGRATE_X,
-    ST_RELAX, ST_RELAX_OVERFLOW, ST_CLEAR_FORCES, ST_SUBSTEPS, ST_RESTITUTION, ST_STORE, ST_CCD,
-    ST_SLEEP_TIMERS, ST_SLEEP_READY, ST_SLEEP_FREEZE, ST_FRESH_FROZEN, ST_DISCOVER_FROZEN, ST_END,
+    ST_CONS_FLAGS, ST_CONS_SELECT, ST_CONS_BODIES, ST_COLOR, ST_COLOR_LISTS, ST_PREPARE,
+    ST_MOVER_FLAGS, ST_MOVER_SELECT, ST_SUBSTEPS, ST_CLEAR_FORCES, ST_RESTITUTION, ST_STORE, ST_CCD,
+    ST_SLEEP_TIMERS, ST_SLEEP_READY, ST_SLEEP_FREEZE, ST_END,
     ST_COUNT
 };
 
-static const char* const kStageNames[ST_COUNT] = {
+static const char* const kStageNames[] = {
     "begin", "fill_emit", "fresh_bounds", "discover_all", "refresh_proxies", "sap_sort", "sap_gather", "sap_sweep",
     "plane_pairs", "frozen_merge", "pair_sort", "pair_unique", "pair_count", "pair_expand",
     "stall_restore", "frozen_select", "narrowphase", "capture", "island_union", "island_root",
-    "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "prepare", "coloring", "color_lists",
-    "mover_flags", "mover_select", "integrate_velocities", "warm_start", "warm_start_overflow",
-    "solve", "solve_overflow", "integrate_positions", "relax", "relax_overflow", "clear_forces", "substeps",
+    "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "constraint_bodies",
+    "coloring", "color_lists", "prepare", "mover_flags", "mover_select", "substeps", "clear_forces",
     "restitution", "store_impulses", "continuous", "sleep_timers", "sleep_ready", "sleep_freeze",
-    "fresh_bounds_frozen", "discover_frozen", "end"};
+    "end"};
+static_assert(sizeof(kStageNames) / sizeof(kStageNames[0]) == ST_COUNT, "stage names out of sync");
 
 struct m3Profiler
 {
@@ -1685,8 +1706,7 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     STAGE(ST_CONS_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, w->pairFlags,
                                                                 v.consPair,
                                                                 &v.sc->constraintCount, np, st)));
-    STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, v, p->contactHertz,
-                         p->contactDampingRatio, h));
+    STAGE(ST_CONS_BODIES, KL(k_cons_bodies, dim3(Blocks(np, T)), dim3(T), 0, st, v));
     if (nb <= M3_COLOR_LDS_BODIES)
     {
         STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, v, w->bodyMasks,
@@ -1699,8 +1719,10 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     }
     // Stable radix sort on the color byte: each color's list ascends.
     STAGE(ST_COLOR_LISTS, HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
-                              ct, tb, v.colors, w->colorsSorted, w->iota, v.colorLists, np, 0, 5,
+                              ct, tb, v.colors, v.slotColors, w->iota, v.colorLists, np, 0, 5,
                               st)));
+    STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, v, p->contactHertz,
+                         p->contactDampingRatio, h));
     STAGE(ST_MOVER_FLAGS,
           KL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->bodyFlags, p->dt));
     STAGE(ST_MOVER_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, w->bodyFlags,
@@ -1716,9 +1738,9 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     sp.maxAngular = p->maximumAngularSpeed;
     sp.gravity = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
     {
-        void* args[] = {(void*)&v, (void*)&sp};
+        void* args[] = {(void*)&v, (void*)&sp, (void*)&w->barrier};
         STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps,
-                                                              dim3(w->coopBlocks), dim3(256), args,
+                                                              dim3(w->coopBlocks), dim3(w->coopThreads), args,
                                                               0, st)));
     }
     STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
@@ -1822,7 +1844,7 @@ extern "C" int m3gpu_sync(m3gpuWorld* w)
         return -1;
     }
     HIP_TRY(hipStreamSynchronize(w->stream));
-    return 0;
+    return CheckFault(w);
 }
 
 extern "C" int m3gpu_last_stats(m3gpuWorld* w, m3gpuStepStats* stats)

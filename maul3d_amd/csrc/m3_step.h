@@ -88,6 +88,7 @@ typedef struct m3Scalars
     int32_t discoverCount;
     int32_t pointCount; // contact points over all constraints
     int32_t colorStarts[M3_GRAPH_COLORS + 2];
+    int32_t fault; // sticky: a substep grid barrier timed out
     uint64_t stepCount;
 } m3Scalars;
 
@@ -155,10 +156,15 @@ typedef struct m3View
     uint64_t* sleepKeys;
     uint32_t* emit;
     // solver scratch
-    m3Constraint* cons;
-    int32_t* consPair;  // pair index of each constraint
-    uint8_t* colors;
-    int32_t* colorLists;
+    m3Constraint* cons;   // prepared constraints in color-slot order
+    int32_t* consPair;    // pair index of each constraint (constraint order)
+    int32_t* consBodies;  // awake dynamic bodies of each constraint (-1 = none), 2 per constraint
+    uint8_t* colors;      // color of each constraint (constraint order)
+    uint8_t* slotColors;  // color of each slot (sorted)
+    int32_t* colorLists;  // constraint index of each slot
+    int32_t* slotOf;      // slot of each constraint
+    int32_t* slotTable;   // per body and color < 16: slot << 1 | side
+    uint32_t* slotMask;   // per body: bit c = a slot in color c; bit 16 = overflow slots
     int32_t* movers;
     m3Vec3* deltaPos;
     m3Quat* deltaRot;
@@ -1157,6 +1163,37 @@ M3_HD void m3WarmStartSide(const m3View* v, const m3Constraint* c, int side)
     }
     v->lv[b] = lin;
     v->av[b] = ang;
+}
+
+// Warm start of every constraint on dynamic body b in the order the
+// colored pass reaches them: one slot per color 0..15 from the body's slot
+// table, then the overflow color's slots [ovStart, ovEnd) in list order.
+M3_HD void m3WarmStartBody(const m3View* v, int32_t b, int32_t ovStart, int32_t ovEnd)
+{
+    uint32_t mask = v->slotMask[b];
+    uint32_t colors = mask & 0xFFFFu;
+    while (colors != 0u)
+    {
+        int32_t color = __builtin_ctz(colors);
+        colors &= colors - 1u;
+        int32_t e = v->slotTable[(int64_t)b * M3_GRAPH_COLORS + color];
+        m3WarmStartSide(v, &v->cons[e >> 1], e & 1);
+    }
+    if ((mask & 0x10000u) != 0u)
+    {
+        for (int32_t k = ovStart; k < ovEnd; ++k)
+        {
+            const m3Constraint* c = &v->cons[k];
+            if (c->dynA && c->bodyA == b)
+            {
+                m3WarmStartSide(v, c, 0);
+            }
+            if (c->dynB && c->bodyB == b)
+            {
+                m3WarmStartSide(v, c, 1);
+            }
+        }
+    }
 }
 
 // Cramer's rule; singular returns zero (src/solver.c:2909-2926).
