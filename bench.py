@@ -38,6 +38,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--cpu-steps", type=int, default=300, help="reference steps timed on the host (steps 1..N of the same trajectory)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched-rollback measurement")
+    p.add_argument("--batch-worlds", type=int, default=256)
     p.add_argument(
         "--copy-calibration",
         action="store_true",
@@ -116,6 +118,84 @@ def run_cpu(steps: int) -> dict | None:
     }
 
 
+def batch_fixture() -> dict | None:
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "batch_rollback.json")) as f:
+            return json.load(f)
+    except OSError:
+        return None
+
+
+def run_batched(worlds: int) -> dict:
+    """Batched rollback: ``worlds`` independent worlds of 50-2000 bodies
+    (pyramids, box stacks, sphere/box rain) warmed 100 steps, snapshotted,
+    restored and resimulated 8 steps.  Timed: the 8 resimulated steps, once
+    through m3World_StepMany (every launch shared by all worlds) and once
+    world by world through m3World_Step, waiting for each world before the
+    next (the sequential single-world path).  Restores happen before each
+    timed region; hashes are checked against the reference fixture."""
+    import maul3d_amd
+    from maul3d_amd import api, scenes
+
+    gpu = maul3d_amd.get_api()
+    lib = maul3d_amd.library()
+    fx = batch_fixture()
+    ws = [scenes.batch_world(gpu, k) for k in range(worlds)]
+    dyn = sum(gpu.m3World_GetCounters(w).bodyCount - 1 for w in ws)  # minus each ground
+    ids = (api.WorldId * worlds)(*ws)
+    warm = 100
+    resim = 8
+    for _ in range(warm):
+        lib.m3World_StepMany(ids, worlds, C.c_float(DT), SUBSTEPS)
+    snaps = [gpu.snapshot(w) for w in ws]
+
+    def check() -> bool | None:
+        if fx is None or worlds > len(fx["worlds"]):
+            return None
+        return all(
+            "%016x" % gpu.m3World_Hash(w) == fx["worlds"][str(k)]["resim"][resim - 1]
+            for k, w in enumerate(ws)
+        )
+
+    def restore_all() -> None:
+        for w, snap in zip(ws, snaps):
+            if not gpu.restore(w, snap):
+                raise RuntimeError("restore refused")
+            lib.m3World_Sync(w)
+
+    restore_all()
+    t0 = time.perf_counter()
+    for _ in range(resim):
+        lib.m3World_StepMany(ids, worlds, C.c_float(DT), SUBSTEPS)
+    for w in ws:
+        lib.m3World_Sync(w)
+    t_batch = time.perf_counter() - t0
+    ok_batch = check()
+
+    restore_all()
+    t0 = time.perf_counter()
+    for w in ws:
+        for _ in range(resim):
+            gpu.m3World_Step(w, DT, SUBSTEPS)
+        lib.m3World_Sync(w)
+    t_seq = time.perf_counter() - t0
+    ok_seq = check()
+    for w in ws:
+        gpu.m3DestroyWorld(w)
+    work = dyn * SUBSTEPS * resim
+    return {
+        "workload": f"{worlds} worlds, {dyn} dynamic bodies; restore at step {warm}, "
+        f"{resim} steps resimulated",
+        "value": work / t_batch,
+        "unit": "body-substeps/s",
+        "ms_per_step": 1000.0 * t_batch / resim,
+        "sequential_value": work / t_seq,
+        "sequential_ms_per_step": 1000.0 * t_seq / resim,
+        "speedup_vs_sequential": t_seq / t_batch,
+        "hash_bit_exact": (ok_batch and ok_seq) if ok_batch is not None else None,
+    }
+
+
 def copy_calibration() -> None:
     """A plain elementwise copy of known size, the scale for FETCH_SIZE and
     WRITE_SIZE counter readings of the step kernels."""
@@ -176,6 +256,8 @@ def main() -> None:
         if prof is not None:
             line["roofline"] = prof["roofline"]
             line["kernels"] = prof["kernels"]
+    if not args.no_batch:
+        line["batched"] = run_batched(args.batch_worlds)
     if not args.no_cpu:
         line["cpu_baseline"] = run_cpu(args.cpu_steps)
     print(json.dumps(line))

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -36,17 +37,13 @@ static int Fail(const char* what, hipError_t e)
         }                                                                                          \
     } while (0)
 
-struct m3gpuWorld
+
+// What a kernel sees of one world: the device view plus the step's scratch.
+// Kernels take an array of these and work on entry blockIdx.y, so one
+// launch serves one world or a batch of independent worlds alike.
+struct m3Dev
 {
-    m3gpuCaps caps;
-    hipStream_t stream;
-    m3View view;          // device pointers, counts filled at upload
-    uint64_t* keysBuf[2]; // current pair list and the previous step's
-    m3Manifold* mansBuf[2];
-    int cur;
-    int fullQuery; // rebuild the frozen-pair buffer before the next step
-    int ccdUnsupported;
-    // scratch for the broadphase sorts and compactions
+    m3View v;
     uint64_t* sapKeysIn;
     uint64_t* sapKeysOut;
     int32_t* sapValsIn;
@@ -61,15 +58,50 @@ struct m3gpuWorld
     int32_t* root;
     uint16_t* bodyMasks;
     int32_t* bodyOwner;
-    int32_t* iota;
+    uint32_t* barrier; // substep grid barrier: arrival count, generation
+};
+
+struct m3gpuWorld
+{
+    m3gpuCaps caps;
+    hipStream_t stream;
+    m3View view;          // device pointers, counts filled at upload
+    uint64_t* keysBuf[2]; // current pair list and the previous step's
+    m3Manifold* mansBuf[2];
+    int cur;
+    int fullQuery; // rebuild the frozen-pair buffer before the next step
+    int ccdUnsupported;
+    m3Dev dev;      // host copy of what kernels see (scratch pointers, view)
+    m3Dev* devDev;  // device copies: [parity] = pair buffers for a step with cur = parity
+    int devDirty;   // the device copies need refreshing
+    int64_t layout; // bumped whenever the device copies change
+    int streamBusy; // work enqueued on the world's stream since it was last idle
+    int batched;    // the world's latest device work ran in the batch stream
+    int64_t stamp;  // duplicate detection in batched calls
     void* cubTemp;
     size_t cubBytes;
     m3Scalars* hostScalars; // pinned mirror for stats
     struct m3Profiler* prof;
-    int coopBlocks;    // co-resident blocks of the substep kernel
-    int coopThreads;   // threads per block of the substep kernel
-    uint32_t* barrier; // substep grid barrier: arrival count, generation
+    int coopBlocks;  // co-resident blocks of the substep kernel
+    int coopThreads; // threads per block of the substep kernel
 };
+
+// The entry is read through the constant address space: its fields load as
+// invariant scalars and the pointers in it are known to address global
+// memory (as for a kernel argument), so accesses through them compile to
+// global rather than flat memory operations.
+// (The host compilation pass of kernel bodies takes the plain read.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) m3Dev m3ConstDev;
+#define M3_WORLD_ENTRY (((m3ConstDev*)ds)[blockIdx.y])
+#else
+#define M3_WORLD_ENTRY (ds[blockIdx.y])
+#endif
+
+#define M3_WORLD                                                                                   \
+    const m3Dev d = M3_WORLD_ENTRY;                                                                \
+    const m3View& v = d.v;                                                                         \
+    (void)d
 
 // ---------------------------------------------------------------------------
 // Small helpers
@@ -178,8 +210,9 @@ __device__ static inline void EmitWave(const m3View& v, bool valid, uint32_t key
 // Broadphase (src/broad_phase.c:391-637)
 // ---------------------------------------------------------------------------
 
-__global__ void k_begin(m3View v)
+__global__ void k_begin(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     m3Scalars* sc = v.sc;
     sc->oldPairCount = sc->pairCount;
     sc->emitCount = 0;
@@ -191,18 +224,20 @@ __global__ void k_begin(m3View v)
     sc->pointCount = 0;
 }
 
-__global__ void k_fill32(uint32_t* p, int32_t n, uint32_t value)
+__global__ void k_fill_emit(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
+    if (i < v.emitCapacity)
     {
-        p[i] = value;
+        v.emit[i] = M3_SENTINEL32;
     }
 }
 
 // Fresh bounds of every alive non-plane shape and its sweep key.
-__global__ void k_fresh(m3View v, uint64_t* sapKeys, int32_t* sapVals)
+__global__ void k_fresh(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= v.maxShape)
     {
@@ -223,20 +258,25 @@ __global__ void k_fresh(m3View v, uint64_t* sapKeys, int32_t* sapVals)
             key -= 1;
         }
     }
-    if (sapKeys != nullptr)
+    d.sapKeysIn[s] = key;
+    d.sapValsIn[s] = s;
+}
+
+__global__ void k_reset_sleep(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    if (v.sc->fullQuery != 0)
     {
-        sapKeys[s] = key;
-        sapVals[s] = s;
+        v.sc->sleepCount = 0;
     }
 }
 
-__global__ void k_reset_sleep(m3View v) { v.sc->sleepCount = 0; }
-
 // After a restore every sleeping body rediscovers its frozen pairs.
-__global__ void k_pending_all(m3View v)
+__global__ void k_pending_all(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= v.maxBody)
+    if (b >= v.maxBody || v.sc->fullQuery == 0)
     {
         return;
     }
@@ -254,8 +294,9 @@ __device__ static inline void AppendFrozen(const m3View& v, int32_t i, int32_t j
 
 // A proxy moves only when its fresh box escapes the stored box; it is then
 // re-stored inflated by the margin (src/broad_phase.c:505-539).
-__global__ void k_refresh(m3View v)
+__global__ void k_refresh(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= v.maxShape || v.shapeAlive[s] == 0 || v.shapeType[s] == M3_PLANE ||
         v.awake[v.shapeBody[s]] == 0)
@@ -304,9 +345,14 @@ enum
     SA_DEFAULT_FILTER = 64,
 };
 
-__global__ void k_sap_gather(m3View v, const uint64_t* keys, const int32_t* vals, float4* xy,
-                             float2* zz, int4* attr)
+__global__ void k_sap_gather(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const uint64_t* keys = d.sapKeysOut;
+    const int32_t* vals = d.sapValsOut;
+    float4* xy = d.sapXY;
+    float2* zz = d.sapZ;
+    int4* attr = d.sapAttr;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxShape)
     {
@@ -358,8 +404,12 @@ __device__ static inline bool AllowedPacked(const m3View& v, int4 a, int4 b)
 // froze last step (m3DiscoverHit).  Runs before the proxy refresh: the
 // frozen discovery reads last step's stored boxes, as the reference's
 // end-of-step discovery does.
-__global__ void k_sap(m3View v, const float4* xy, const float2* zz, const int4* attr)
+__global__ void k_sap(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const float4* xy = d.sapXY;
+    const float2* zz = d.sapZ;
+    const int4* attr = d.sapAttr;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxShape)
     {
@@ -433,8 +483,9 @@ __global__ void k_sap(m3View v, const float4* xy, const float2* zz, const int4* 
 }
 
 // Planes pair with every allowed hot shape (src/broad_phase.c:541-564).
-__global__ void k_planes(m3View v)
+__global__ void k_planes(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool hot = false;
     uint32_t key = 0;
@@ -456,8 +507,9 @@ __global__ void k_planes(m3View v)
 }
 
 // Still-cold frozen pairs ride along (src/broad_phase.c:607-632).
-__global__ void k_frozen(m3View v)
+__global__ void k_frozen(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     int32_t n = v.sc->sleepCount < v.pairCapacity ? v.sc->sleepCount : v.pairCapacity;
     bool cold = false;
@@ -475,8 +527,10 @@ __global__ void k_frozen(m3View v)
 
 // The unique sorted keys become this step's pair list; the cold ones are
 // the frozen buffer carried to the next step.
-__global__ void k_pair_count(m3View v, const uint32_t* unique)
+__global__ void k_pair_count(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const uint32_t* unique = d.emitUnique;
     m3Scalars* sc = v.sc;
     int32_t n = sc->uniqueCount;
     if (n > 0 && unique[n - 1] == M3_SENTINEL32)
@@ -495,8 +549,11 @@ __global__ void k_pair_count(m3View v, const uint32_t* unique)
     sc->pairCount = n;
 }
 
-__global__ void k_expand(m3View v, const uint32_t* unique, uint8_t* coldFlags)
+__global__ void k_expand(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const uint32_t* unique = d.emitUnique;
+    uint8_t* coldFlags = d.pairFlags;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.pairCapacity)
     {
@@ -513,8 +570,9 @@ __global__ void k_expand(m3View v, const uint32_t* unique, uint8_t* coldFlags)
 }
 
 // On a capacity stall the previous pair list stays the world's list.
-__global__ void k_stall_restore(m3View v)
+__global__ void k_stall_restore(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     if (v.sc->stall == 0)
     {
         return;
@@ -531,8 +589,9 @@ __global__ void k_stall_restore(m3View v)
 // Narrowphase (src/manifold.c:2357-2944)
 // ---------------------------------------------------------------------------
 
-__global__ void k_narrow(m3View v)
+__global__ void k_narrow(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0 || i >= v.sc->pairCount)
     {
@@ -545,8 +604,9 @@ __global__ void k_narrow(m3View v)
 // Sweep capture and islands (src/solver.c:3437-3580, 3851-3881)
 // ---------------------------------------------------------------------------
 
-__global__ void k_capture(m3View v)
+__global__ void k_capture(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0)
     {
@@ -606,8 +666,9 @@ __device__ static void Unite(int32_t* parent, int32_t a, int32_t b)
     }
 }
 
-__global__ void k_union(m3View v)
+__global__ void k_union(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0 || i >= v.sc->pairCount || v.manifolds[i].pointCount == 0)
     {
@@ -644,8 +705,10 @@ __global__ void k_union(m3View v)
     }
 }
 
-__global__ void k_root(m3View v, int32_t* root)
+__global__ void k_root(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0)
     {
@@ -655,8 +718,10 @@ __global__ void k_root(m3View v, int32_t* root)
     v.slotMask[i] = 0u;
 }
 
-__global__ void k_wake_mark(m3View v, const int32_t* root)
+__global__ void k_wake_mark(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC)
     {
@@ -668,8 +733,10 @@ __global__ void k_wake_mark(m3View v, const int32_t* root)
     }
 }
 
-__global__ void k_wake_apply(m3View v, const int32_t* root)
+__global__ void k_wake_apply(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC)
     {
@@ -686,8 +753,10 @@ __global__ void k_wake_apply(m3View v, const int32_t* root)
 // Prepare and coloring (src/solver.c:156-404, 2741-2822)
 // ---------------------------------------------------------------------------
 
-__global__ void k_constraint_flags(m3View v, uint8_t* flags)
+__global__ void k_constraint_flags(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    uint8_t* flags = d.pairFlags;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.pairCapacity)
     {
@@ -698,8 +767,9 @@ __global__ void k_constraint_flags(m3View v, uint8_t* flags)
 
 // The awake dynamic bodies of each constraint, the only bodies coloring
 // tracks (src/solver.c:2741-2822).
-__global__ void k_cons_bodies(m3View v)
+__global__ void k_cons_bodies(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0 || c >= v.sc->constraintCount)
     {
@@ -732,8 +802,9 @@ __device__ static inline void RegisterSlot(const m3View& v, int32_t body, int32_
 // dynamic body or a static/kinematic one, and touching dynamic bodies share
 // an island, which the wake pass woke whole.  So coloring gave it at most
 // one slot per color.
-__global__ void k_prepare(m3View v, float hertz, float zeta, float h)
+__global__ void k_prepare(const m3Dev* __restrict__ ds, float hertz, float zeta, float h)
 {
+    M3_WORLD;
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0 || k >= v.sc->constraintCount)
     {
@@ -773,8 +844,11 @@ __global__ void k_prepare(m3View v, float hertz, float zeta, float h)
 
 template <bool kLds>
 __global__ void __launch_bounds__(M3_COLOR_THREADS)
-    k_color(m3View v, uint16_t* gMasks, int32_t* gOwner)
+    k_color(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    uint16_t* gMasks = d.bodyMasks;
+    int32_t* gOwner = d.bodyOwner;
     __shared__ uint16_t sMask[kLds ? M3_COLOR_LDS_BODIES : 1];
     __shared__ int32_t sOwner[kLds ? M3_COLOR_LDS_BODIES : 1];
     __shared__ int32_t counts[M3_GRAPH_COLORS + 1];
@@ -869,12 +943,14 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
             __syncthreads();
         }
     }
+    __shared__ int32_t starts[M3_GRAPH_COLORS + 2];
     if (t == 0)
     {
         int32_t cursor = 0;
         int32_t used = 0;
         for (int32_t k = 0; k <= M3_GRAPH_COLORS; ++k)
         {
+            starts[k] = cursor;
             sc->colorStarts[k] = cursor;
             cursor += counts[k];
             used += counts[k] > 0 ? 1 : 0;
@@ -882,23 +958,67 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
         sc->colorStarts[M3_GRAPH_COLORS + 1] = cursor;
         sc->colorCount = used;
     }
-}
-
-__global__ void k_iota(int32_t* p, int32_t n)
-{
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
+    // Color lists: each color's slots hold its constraints in ascending
+    // order (a stable counting sort; in-wave ranks from ballots).
+    __shared__ int32_t waveBase[M3_GRAPH_COLORS + 1][M3_COLOR_THREADS / 64];
+    __shared__ int32_t running[M3_GRAPH_COLORS + 1];
+    if (t <= M3_GRAPH_COLORS)
     {
-        p[i] = i;
+        running[t] = 0;
+    }
+    __syncthreads();
+    const int32_t lane = t & 63;
+    const int32_t wave = t >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int32_t base = 0; base < n; base += M3_COLOR_THREADS)
+    {
+        int32_t c = base + t;
+        int32_t color = c < n ? (int32_t)v.colors[c] : -1;
+        int32_t rank = 0;
+        for (int32_t k = 0; k <= M3_GRAPH_COLORS; ++k)
+        {
+            uint64_t m = __ballot(color == k);
+            if (color == k)
+            {
+                rank = __popcll(m & below);
+            }
+            if (lane == 0)
+            {
+                waveBase[k][wave] = __popcll(m);
+            }
+        }
+        __syncthreads();
+        if (t <= M3_GRAPH_COLORS)
+        {
+            int32_t acc = running[t];
+            for (int32_t w = 0; w < M3_COLOR_THREADS / 64; ++w)
+            {
+                int32_t x = waveBase[t][w];
+                waveBase[t][w] = acc;
+                acc += x;
+            }
+            running[t] = acc;
+        }
+        __syncthreads();
+        if (color >= 0)
+        {
+            int32_t slot = starts[color] + waveBase[color][wave] + rank;
+            v.colorLists[slot] = c;
+            v.slotColors[slot] = (uint8_t)color;
+        }
+        __syncthreads();
     }
 }
+
 
 // ---------------------------------------------------------------------------
 // Movers and the substep loop (src/solver.c:3938-4203)
 // ---------------------------------------------------------------------------
 
-__global__ void k_mover_flags(m3View v, uint8_t* flags, float dt)
+__global__ void k_mover_flags(const m3Dev* __restrict__ ds, float dt)
 {
+    M3_WORLD;
+    uint8_t* flags = d.bodyFlags;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody)
     {
@@ -1000,29 +1120,36 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
 
 #define M3_SUBSTEP_THREADS 256
 
-// The whole substep loop of src/solver.c:4085-4203 in one cooperative
-// launch, with a grid barrier after each phase:
+// The whole substep loop of src/solver.c:4085-4203, with a barrier after
+// each phase:
 //   integrate velocities and warm start (one phase: a warm-start impulse
 //   depends only on stored accumulators, so each body replays its own
 //   constraints in color order, the exact operation sequence of the colored
 //   pass on that body), solve by color, integrate positions, relax by color.
-// Every branch depends only on counts written before the launch, so all
-// blocks reach the same barriers.
-__global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
-    k_substeps(m3View v, m3SubstepParams sp, uint32_t* bar)
+// Every branch depends only on counts written before the loop starts, so
+// all participants reach the same barriers.
+struct m3GridSync
 {
-    if (v.sc->stall != 0)
-    {
-        return;
-    }
+    uint32_t* bar;
+    int32_t* fault;
+    __device__ void operator()() const { GridBarrier(bar, fault); }
+};
+
+struct m3BlockSync
+{
+    __device__ void operator()() const { __syncthreads(); }
+};
+
+template <typename Sync>
+__device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams& sp, int32_t gid,
+                                          int32_t gsz, Sync sync)
+{
     int32_t starts[M3_GRAPH_COLORS + 2];
     for (int32_t k = 0; k < M3_GRAPH_COLORS + 2; ++k)
     {
         starts[k] = v.sc->colorStarts[k];
     }
     const int32_t movers = v.sc->moverCount;
-    const int32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t gsz = gridDim.x * blockDim.x;
     for (int32_t sub = 0; sub < sp.substeps; ++sub)
     {
         for (int32_t m = gid; m < movers; m += gsz)
@@ -1031,7 +1158,7 @@ __global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
             m3IntegrateVelocity(&v, b, sp.h, sp.gravity, sp.maxLinear, sp.maxAngular);
             m3WarmStartBody(&v, b, starts[M3_GRAPH_COLORS], starts[M3_GRAPH_COLORS + 1]);
         }
-        GridBarrier(bar, &v.sc->fault);
+        sync();
         for (int mode = 1; mode < 3; ++mode)
         {
             if (mode == 2)
@@ -1040,7 +1167,7 @@ __global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
                 {
                     m3IntegratePosition(&v, v.movers[m], sp.h);
                 }
-                GridBarrier(bar, &v.sc->fault);
+                sync();
             }
             for (int32_t color = 0; color <= M3_GRAPH_COLORS; ++color)
             {
@@ -1051,14 +1178,43 @@ __global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
                     continue;
                 }
                 ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS);
-                GridBarrier(bar, &v.sc->fault);
+                sync();
             }
         }
     }
 }
 
-__global__ void k_clear_forces(m3View v)
+// One world over a cooperative grid.
+__global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
+    k_substeps(const m3Dev* __restrict__ ds, m3SubstepParams sp)
 {
+    M3_WORLD;
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    SubstepLoop(v, sp, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x,
+                m3GridSync{d.barrier, &v.sc->fault});
+}
+
+// A batch: worlds are independent, so each workgroup runs its own world's
+// whole substep loop with workgroup barriers.
+#define M3_BLOCK_MAX_THREADS 1024
+
+__global__ void __launch_bounds__(M3_BLOCK_MAX_THREADS)
+    k_substeps_block(const m3Dev* __restrict__ ds, m3SubstepParams sp)
+{
+    M3_WORLD;
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    SubstepLoop(v, sp, (int32_t)threadIdx.x, (int32_t)blockDim.x, m3BlockSync{});
+}
+
+__global__ void k_clear_forces(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
     int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0 || m >= v.sc->moverCount)
     {
@@ -1069,8 +1225,9 @@ __global__ void k_clear_forces(m3View v)
     v.torque[i] = m3V3(0.0f, 0.0f, 0.0f);
 }
 
-__global__ void k_restitution(m3View v, float threshold)
+__global__ void k_restitution(const m3Dev* __restrict__ ds, float threshold)
 {
+    M3_WORLD;
     if (v.sc->stall != 0 || v.sc->anyRestitution == 0)
     {
         return;
@@ -1082,8 +1239,9 @@ __global__ void k_restitution(m3View v, float threshold)
     }
 }
 
-__global__ void k_store(m3View v)
+__global__ void k_store(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0 || c >= v.sc->constraintCount)
     {
@@ -1096,8 +1254,9 @@ __global__ void k_store(m3View v)
 // Continuous collision and sleep (src/solver.c:3349-3430, 3582-3660)
 // ---------------------------------------------------------------------------
 
-__global__ void k_ccd(m3View v)
+__global__ void k_ccd(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0)
     {
@@ -1106,8 +1265,10 @@ __global__ void k_ccd(m3View v)
     m3ContinuousBody(&v, i);
 }
 
-__global__ void k_sleep_timers(m3View v, const int32_t* root, float dt, int sleepEnabled)
+__global__ void k_sleep_timers(const m3Dev* __restrict__ ds, float dt, int sleepEnabled)
 {
+    M3_WORLD;
+    const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0)
     {
@@ -1128,8 +1289,10 @@ __global__ void k_sleep_timers(m3View v, const int32_t* root, float dt, int slee
     }
 }
 
-__global__ void k_sleep_ready(m3View v, const int32_t* root)
+__global__ void k_sleep_ready(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
         v.awake[i] == 0)
@@ -1142,8 +1305,10 @@ __global__ void k_sleep_ready(m3View v, const int32_t* root)
     }
 }
 
-__global__ void k_sleep_freeze(m3View v, const int32_t* root)
+__global__ void k_sleep_freeze(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
+    const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
         v.awake[i] == 0)
@@ -1160,13 +1325,15 @@ __global__ void k_sleep_freeze(m3View v, const int32_t* root)
     }
 }
 
-__global__ void k_end(m3View v)
+__global__ void k_end(const m3Dev* __restrict__ ds)
 {
+    M3_WORLD;
     m3Scalars* sc = v.sc;
     if (sc->stall == 0)
     {
         sc->stepCount += 1;
     }
+    sc->fullQuery = 0;
     if (sc->sleepCount > v.pairCapacity)
     {
         sc->sleepCount = v.pairCapacity;
@@ -1174,8 +1341,206 @@ __global__ void k_end(m3View v)
 }
 
 // ---------------------------------------------------------------------------
+// Batches of small worlds: one workgroup per world replaces each
+// device-wide sort and compaction, with the world's data staged in LDS.
+// ---------------------------------------------------------------------------
+
+#define M3_BATCH_THREADS 1024
+#define M3_BATCH_EMIT_MAX 32768  // emitted pair keys a world sorts in LDS
+#define M3_BATCH_SHAPES_MAX 4096 // shapes a world sweep-sorts in LDS
+
+// Exclusive prefix sum over the block (blockDim.x a multiple of 64, at most
+// 1024); *total receives the block sum.  Every thread must call it.
+__device__ static inline int32_t BlockExclusiveScan(int32_t x, int32_t* total)
+{
+    __shared__ int32_t waveSum[M3_BATCH_THREADS / 64 + 1];
+    const int32_t lane = threadIdx.x & 63;
+    const int32_t wave = threadIdx.x >> 6;
+    const int32_t waves = blockDim.x >> 6;
+    int32_t incl = x;
+    for (int32_t off = 1; off < 64; off <<= 1)
+    {
+        int32_t y = __shfl_up(incl, off);
+        if (lane >= off)
+        {
+            incl += y;
+        }
+    }
+    if (lane == 63)
+    {
+        waveSum[wave] = incl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int32_t acc = 0;
+        for (int32_t w = 0; w < waves; ++w)
+        {
+            int32_t t = waveSum[w];
+            waveSum[w] = acc;
+            acc += t;
+        }
+        waveSum[M3_BATCH_THREADS / 64] = acc;
+    }
+    __syncthreads();
+    int32_t result = waveSum[wave] + incl - x;
+    *total = waveSum[M3_BATCH_THREADS / 64];
+    __syncthreads();
+    return result;
+}
+
+// Bitonic sort of a[0, n) in LDS, n a power of two.
+template <typename K>
+__device__ static inline void BitonicSortLds(K* a, int32_t n)
+{
+    for (int32_t k = 2; k <= n; k <<= 1)
+    {
+        for (int32_t j = k >> 1; j > 0; j >>= 1)
+        {
+            for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
+            {
+                int32_t l = i ^ j;
+                if (l > i)
+                {
+                    K x = a[i];
+                    K y = a[l];
+                    bool ascending = (i & k) == 0;
+                    if ((y < x) == ascending)
+                    {
+                        a[i] = y;
+                        a[l] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ static inline int32_t NextPow2(int32_t n)
+{
+    int32_t p = 1;
+    while (p < n)
+    {
+        p <<= 1;
+    }
+    return p;
+}
+
+// Sweep keys ordered by (key, shape): the order a stable sort by key gives.
+struct m3SweepItem
+{
+    uint64_t key;
+    int32_t shape;
+    int32_t pad;
+    __device__ bool operator<(const m3SweepItem& o) const
+    {
+        return key < o.key || (key == o.key && shape < o.shape);
+    }
+};
+
+__global__ void __launch_bounds__(M3_BATCH_THREADS) k_sap_sort_block(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    __shared__ m3SweepItem items[M3_BATCH_SHAPES_MAX];
+    const int32_t n = v.maxShape;
+    const int32_t n2 = NextPow2(n);
+    for (int32_t i = threadIdx.x; i < n2; i += blockDim.x)
+    {
+        m3SweepItem it;
+        it.key = i < n ? d.sapKeysIn[i] : M3_SENTINEL64;
+        it.shape = i < n ? d.sapValsIn[i] : 0x7FFFFFFF;
+        it.pad = 0;
+        items[i] = it;
+    }
+    __syncthreads();
+    BitonicSortLds(items, n2);
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
+    {
+        d.sapKeysOut[i] = items[i].key;
+        d.sapValsOut[i] = items[i].shape;
+    }
+}
+
+// The emitted keys sorted and made unique (the pair list candidates).
+__global__ void __launch_bounds__(M3_BATCH_THREADS) k_emit_sort_block(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    __shared__ uint32_t keys[M3_BATCH_EMIT_MAX];
+    int32_t n = v.sc->emitCount;
+    n = n < v.emitCapacity ? n : v.emitCapacity;
+    const int32_t n2 = NextPow2(n);
+    for (int32_t i = threadIdx.x; i < n2; i += blockDim.x)
+    {
+        keys[i] = i < n ? v.emit[i] : M3_SENTINEL32;
+    }
+    __syncthreads();
+    BitonicSortLds(keys, n2);
+    int32_t written = 0;
+    for (int32_t base = 0; base < n; base += blockDim.x)
+    {
+        int32_t i = base + threadIdx.x;
+        int32_t first = (i < n && (i == 0 || keys[i] != keys[i - 1])) ? 1 : 0;
+        int32_t total = 0;
+        int32_t pos = BlockExclusiveScan(first, &total);
+        if (first)
+        {
+            d.emitUnique[written + pos] = keys[i];
+        }
+        written += total;
+    }
+    if (threadIdx.x == 0)
+    {
+        v.sc->uniqueCount = written;
+    }
+}
+
+// Stable compactions: 0 = cold pair keys into the frozen buffer,
+// 1 = constraint pair indices, 2 = moving bodies.
+__global__ void __launch_bounds__(M3_BATCH_THREADS)
+    k_select_block(const m3Dev* __restrict__ ds, int mode)
+{
+    M3_WORLD;
+    const int32_t n = mode == 2 ? v.maxBody : v.pairCapacity;
+    const uint8_t* flags = mode == 2 ? d.bodyFlags : d.pairFlags;
+    int32_t written = 0;
+    for (int32_t base = 0; base < n; base += blockDim.x)
+    {
+        int32_t i = base + threadIdx.x;
+        int32_t f = (i < n && flags[i] != 0) ? 1 : 0;
+        int32_t total = 0;
+        int32_t pos = BlockExclusiveScan(f, &total);
+        if (f)
+        {
+            if (mode == 0)
+            {
+                v.sleepKeys[written + pos] = v.keys[i];
+            }
+            else if (mode == 1)
+            {
+                v.consPair[written + pos] = i;
+            }
+            else
+            {
+                v.movers[written + pos] = i;
+            }
+        }
+        written += total;
+    }
+    if (threadIdx.x == 0)
+    {
+        int32_t* out = mode == 0 ? &v.sc->sleepCount
+                                 : (mode == 1 ? &v.sc->constraintCount : &v.sc->moverCount);
+        *out = written;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+
+static int JoinBatch(m3gpuWorld* w);
+static void BatchForget(m3gpuWorld* w);
 
 template <typename T>
 static int DevAlloc(T** p, int64_t count)
@@ -1197,6 +1562,8 @@ extern "C" int m3gpu_device_count(void)
 }
 
 extern "C" const char* m3gpu_last_error(void) { return s_lastError; }
+
+static int RefreshDevCopies(m3gpuWorld* w);
 
 static int CreateImpl(m3gpuWorld* w)
 {
@@ -1245,21 +1612,21 @@ static int CreateImpl(m3gpuWorld* w)
         }
     }
     if (DevAlloc(&v.sleepKeys, np) || DevAlloc(&v.emit, v.emitCapacity) ||
-        DevAlloc(&w->emitSorted, v.emitCapacity) || DevAlloc(&w->emitUnique, v.emitCapacity) ||
+        DevAlloc(&w->dev.emitSorted, v.emitCapacity) || DevAlloc(&w->dev.emitUnique, v.emitCapacity) ||
         DevAlloc(&v.cons, np) || DevAlloc(&v.consPair, np) || DevAlloc(&v.colors, np) ||
         DevAlloc(&v.colorLists, np) || DevAlloc(&v.movers, nb) || DevAlloc(&v.deltaPos, nb) ||
         DevAlloc(&v.deltaRot, nb) || DevAlloc(&v.com0, nb) || DevAlloc(&v.rot0, nb) ||
         DevAlloc(&v.parent, nb) || DevAlloc(&v.forced, nb) || DevAlloc(&v.flag, nb) ||
         DevAlloc(&v.pending, nb) ||
-        DevAlloc(&v.sc, 1) || DevAlloc(&w->sapKeysIn, ns) || DevAlloc(&w->sapKeysOut, ns) ||
-        DevAlloc(&w->sapValsIn, ns) || DevAlloc(&w->sapValsOut, ns) ||
-        DevAlloc(&w->sapXY, ns) || DevAlloc(&w->sapZ, ns) || DevAlloc(&w->sapAttr, ns) ||
-        DevAlloc(&w->pairFlags, np) || DevAlloc(&w->bodyFlags, nb) || DevAlloc(&w->root, nb) ||
-        DevAlloc(&w->bodyMasks, nb) || DevAlloc(&w->bodyOwner, nb) ||
-        DevAlloc(&w->iota, np) || DevAlloc(&v.slotColors, np) ||
+        DevAlloc(&v.sc, 1) || DevAlloc(&w->dev.sapKeysIn, ns) || DevAlloc(&w->dev.sapKeysOut, ns) ||
+        DevAlloc(&w->dev.sapValsIn, ns) || DevAlloc(&w->dev.sapValsOut, ns) ||
+        DevAlloc(&w->dev.sapXY, ns) || DevAlloc(&w->dev.sapZ, ns) || DevAlloc(&w->dev.sapAttr, ns) ||
+        DevAlloc(&w->dev.pairFlags, np) || DevAlloc(&w->dev.bodyFlags, nb) || DevAlloc(&w->dev.root, nb) ||
+        DevAlloc(&w->dev.bodyMasks, nb) || DevAlloc(&w->dev.bodyOwner, nb) ||
+        DevAlloc(&v.slotColors, np) ||
         DevAlloc(&v.consBodies, 2 * (int64_t)np) || DevAlloc(&v.slotOf, np) ||
         DevAlloc(&v.slotTable, (int64_t)M3_GRAPH_COLORS * nb) || DevAlloc(&v.slotMask, nb) ||
-        DevAlloc(&w->barrier, 4))
+        DevAlloc(&w->dev.barrier, 4))
     {
         return -1;
     }
@@ -1269,29 +1636,25 @@ static int CreateImpl(m3gpuWorld* w)
     // One temp buffer sized for the largest primitive call.
     size_t need = 0;
     size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, w->sapKeysIn, w->sapKeysOut,
-                                               w->sapValsIn, w->sapValsOut, ns, 0, 64, w->stream));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, w->dev.sapKeysIn, w->dev.sapKeysOut,
+                                               w->dev.sapValsIn, w->dev.sapValsOut, ns, 0, 64, w->stream));
     need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, v.emit, w->emitSorted,
+    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, v.emit, w->dev.emitSorted,
                                               v.emitCapacity, 0, 32, w->stream));
     need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, bytes, w->emitSorted, w->emitUnique,
+    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, bytes, w->dev.emitSorted, w->dev.emitUnique,
                                          &v.sc->uniqueCount, v.emitCapacity, w->stream));
     need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, v.keys, w->pairFlags, v.sleepKeys,
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, v.keys, w->dev.pairFlags, v.sleepKeys,
                                           &v.sc->sleepCount, np, w->stream));
     need = bytes > need ? bytes : need;
     hipcub::CountingInputIterator<int32_t> counting(0);
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->pairFlags, v.consPair,
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->dev.pairFlags, v.consPair,
                                           &v.sc->constraintCount, np, w->stream));
     need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->bodyFlags, v.movers,
+    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->dev.bodyFlags, v.movers,
                                           &v.sc->moverCount, nb, w->stream));
     need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, v.colors, v.slotColors, w->iota,
-                                               v.colorLists, np, 0, 5, w->stream));
-    need = bytes > need ? bytes : need;
-    hipLaunchKernelGGL(k_iota, dim3(Blocks(np, 256)), dim3(256), 0, w->stream, w->iota, np);
     w->cubBytes = need > 0 ? need : 1;
     {
         int dev = 0;
@@ -1312,6 +1675,8 @@ static int CreateImpl(m3gpuWorld* w)
         w->coopBlocks = want < 1 ? 1 : (want > maxBlocks ? maxBlocks : want);
     }
     HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
+    HIP_TRY(hipMalloc(&w->devDev, 2 * sizeof(m3Dev)));
+    w->devDirty = 1;
     w->cur = 0;
     v.keys = w->keysBuf[0];
     v.manifolds = w->mansBuf[0];
@@ -1352,6 +1717,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
     {
         return;
     }
+    BatchForget(w);
     (void)hipStreamSynchronize(w->stream);
     m3View& v = w->view;
     void* ptrs[] = {v.xf, v.lv, v.av, v.invMass, v.invI, v.inertia, v.lc, v.gravityScale,
@@ -1362,12 +1728,12 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.hasOffset, v.localPos, v.localRot, v.surfaceVel, v.category, v.mask,
                     v.group, v.shapeNext, v.hullIndex, v.leaf, v.fresh, v.planes,
                     (void*)v.hulls, w->keysBuf[0], w->keysBuf[1], w->mansBuf[0], w->mansBuf[1],
-                    v.sleepKeys, v.emit, w->emitSorted, w->emitUnique, v.cons, v.consPair,
+                    v.sleepKeys, v.emit, w->dev.emitSorted, w->dev.emitUnique, v.cons, v.consPair,
                     v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
-                    v.parent, v.forced, v.flag, v.pending, v.sc, w->sapKeysIn, w->sapKeysOut, w->sapValsIn,
-                    w->sapValsOut, w->sapXY, w->sapZ, w->sapAttr, w->pairFlags, w->bodyFlags, w->root,
-                    w->bodyMasks, w->bodyOwner, w->iota, v.slotColors, v.consBodies, v.slotOf,
-                    v.slotTable, v.slotMask, w->barrier, w->cubTemp};
+                    v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut, w->dev.sapValsIn,
+                    w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.root,
+                    w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
+                    v.slotTable, v.slotMask, w->dev.barrier, w->cubTemp, w->devDev};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     {
         if (ptrs[i] != nullptr)
@@ -1398,7 +1764,7 @@ static int Copy(m3gpuWorld* w, void* dst, const void* src, int64_t bytes, hipMem
 
 extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fields)
 {
-    if (w == nullptr || h == nullptr)
+    if (w == nullptr || h == nullptr || JoinBatch(w) != 0)
     {
         return -1;
     }
@@ -1516,14 +1882,21 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(&v.sc->sleepCount, h->sleepingPairCount, 1, int32_t);
         UP(v.leaf, h->leafBoxes, 6 * (int64_t)ns, double);
         w->fullQuery = *h->pairsFullQuery != 0;
+        int32_t fullQuery = w->fullQuery;
+        UP(&v.sc->fullQuery, &fullQuery, 1, int32_t);
+        HIP_TRY(hipStreamSynchronize(w->stream));
     }
     HIP_TRY(hipStreamSynchronize(w->stream));
-    return 0;
+    return RefreshDevCopies(w);
 }
 
 // A world whose substep barrier timed out holds no valid state.
 static int CheckFault(m3gpuWorld* w)
 {
+    if (JoinBatch(w) != 0)
+    {
+        return -1;
+    }
     HIP_TRY(hipMemcpyAsync(&w->hostScalars->fault, &w->view.sc->fault, sizeof(int32_t),
                            hipMemcpyDeviceToHost, w->stream));
     HIP_TRY(hipStreamSynchronize(w->stream));
@@ -1537,7 +1910,7 @@ static int CheckFault(m3gpuWorld* w)
 
 extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
 {
-    if (w == nullptr || h == nullptr)
+    if (w == nullptr || h == nullptr || JoinBatch(w) != 0)
     {
         return -1;
     }
@@ -1579,7 +1952,7 @@ enum m3Stage
     ST_BEGIN, ST_FILL, ST_FRESH, ST_DISCOVER_ALL, ST_REFRESH, ST_SAP_SORT, ST_SAP_GATHER, ST_SAP, ST_PLANES,
     ST_FROZEN, ST_EMIT_SORT, ST_UNIQUE, ST_PAIR_COUNT, ST_EXPAND, ST_STALL_RESTORE,
     ST_SLEEP_SELECT, ST_NARROW, ST_CAPTURE, ST_UNION, ST_ROOT, ST_WAKE_MARK, ST_WAKE_APPLY,
-    ST_CONS_FLAGS, ST_CONS_SELECT, ST_CONS_BODIES, ST_COLOR, ST_COLOR_LISTS, ST_PREPARE,
+    ST_CONS_FLAGS, ST_CONS_SELECT, ST_CONS_BODIES, ST_COLOR, ST_PREPARE,
     ST_MOVER_FLAGS, ST_MOVER_SELECT, ST_SUBSTEPS, ST_CLEAR_FORCES, ST_RESTITUTION, ST_STORE, ST_CCD,
     ST_SLEEP_TIMERS, ST_SLEEP_READY, ST_SLEEP_FREEZE, ST_END,
     ST_COUNT
@@ -1590,7 +1963,7 @@ static const char* const kStageNames[] = {
     "plane_pairs", "frozen_merge", "pair_sort", "pair_unique", "pair_count", "pair_expand",
     "stall_restore", "frozen_select", "narrowphase", "capture", "island_union", "island_root",
     "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "constraint_bodies",
-    "coloring", "color_lists", "prepare", "mover_flags", "mover_select", "substeps", "clear_forces",
+    "coloring", "prepare", "mover_flags", "mover_select", "substeps", "clear_forces",
     "restitution", "store_impulses", "continuous", "sleep_timers", "sleep_ready", "sleep_freeze",
     "end"};
 static_assert(sizeof(kStageNames) / sizeof(kStageNames[0]) == ST_COUNT, "stage names out of sync");
@@ -1632,11 +2005,52 @@ static m3Profiler* Prof(m3gpuWorld* w);
 
 #define KL(...) hipLaunchKernelGGL(__VA_ARGS__)
 
+// Both device copies of a world's kernel view, one per pair-buffer parity
+// (a step with cur = p writes keysBuf[p] and carries from keysBuf[p ^ 1]).
+// Refreshed after creation and every upload, once the stream is idle.
+static int RefreshDevCopies(m3gpuWorld* w)
+{
+    m3Dev copies[2];
+    for (int parity = 0; parity < 2; ++parity)
+    {
+        copies[parity] = w->dev;
+        copies[parity].v = w->view;
+        copies[parity].v.keys = w->keysBuf[parity];
+        copies[parity].v.manifolds = w->mansBuf[parity];
+        copies[parity].v.oldKeys = w->keysBuf[parity ^ 1];
+        copies[parity].v.oldManifolds = w->mansBuf[parity ^ 1];
+    }
+    HIP_TRY(hipStreamSynchronize(w->stream));
+    HIP_TRY(hipMemcpy(w->devDev, copies, sizeof(copies), hipMemcpyHostToDevice));
+    w->devDirty = 0;
+    w->layout += 1;
+    w->streamBusy = 0;
+    return 0;
+}
+
+static m3SubstepParams SubstepParams(const m3gpuStepParams* p)
+{
+    float h = p->dt / (float)p->substeps;
+    m3SubstepParams sp;
+    sp.substeps = p->substeps;
+    sp.h = h;
+    sp.invH = h > 0.0f ? 1.0f / h : 0.0f;
+    sp.push = p->contactPushMaxSpeed;
+    sp.maxLinear = p->maximumLinearSpeed;
+    sp.maxAngular = p->maximumAngularSpeed;
+    sp.gravity = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
+    return sp;
+}
+
 // Enqueue one step.  Launch geometry only depends on host-known capacities
 // and high-water marks; every data-dependent count is read on the device.
 static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
 {
     hipStream_t st = w->stream;
+    if (w->devDirty && RefreshDevCopies(w) != 0)
+    {
+        return -1;
+    }
     // Swap the pair buffers: last step's list becomes the carry source.
     w->cur ^= 1;
     m3View& v = w->view;
@@ -1644,6 +2058,8 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     v.manifolds = w->mansBuf[w->cur];
     v.oldKeys = w->keysBuf[w->cur ^ 1];
     v.oldManifolds = w->mansBuf[w->cur ^ 1];
+    const m3Dev* D = w->devDev + w->cur;
+    const m3Dev& hd = w->dev;
     const int T = 256;
     int32_t nb = v.maxBody;
     int32_t ns = v.maxShape;
@@ -1652,112 +2068,93 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     size_t tb = w->cubBytes;
     void* ct = w->cubTemp;
 
-    STAGE(ST_BEGIN, KL(k_begin, dim3(1), dim3(1), 0, st, v));
-    STAGE(ST_FILL, KL(k_fill32, dim3(Blocks(ne, T)), dim3(T), 0, st, v.emit, ne, M3_SENTINEL32));
+    STAGE(ST_BEGIN, KL(k_begin, dim3(1), dim3(1), 0, st, D));
+    STAGE(ST_FILL, KL(k_fill_emit, dim3(Blocks(ne, T)), dim3(T), 0, st, D));
     // Broadphase.
-    STAGE(ST_FRESH, KL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysIn, w->sapValsIn));
+    STAGE(ST_FRESH, KL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
     if (w->fullQuery)
     {
         // After a restore the frozen buffer is rebuilt from every sleeping
         // body (src/broad_phase.c:432-448).
-        KL(k_reset_sleep, dim3(1), dim3(1), 0, st, v);
-        STAGE(ST_DISCOVER_ALL, KL(k_pending_all, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
+        KL(k_reset_sleep, dim3(1), dim3(1), 0, st, D);
+        STAGE(ST_DISCOVER_ALL, KL(k_pending_all, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
         w->fullQuery = 0;
     }
     STAGE(ST_SAP_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
-                           ct, tb, w->sapKeysIn, w->sapKeysOut, w->sapValsIn, w->sapValsOut, ns, 0,
+                           ct, tb, hd.sapKeysIn, hd.sapKeysOut, hd.sapValsIn, hd.sapValsOut, ns, 0,
                            64, st)));
-    STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, v, w->sapKeysOut,
-                            w->sapValsOut, w->sapXY, w->sapZ, w->sapAttr));
-    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, 64)), dim3(64), 0, st, v, w->sapXY, w->sapZ,
-                     w->sapAttr));
+    STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
+    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, 64)), dim3(64), 0, st, D));
     if (v.planeCount > 0)
     {
         STAGE(ST_PLANES,
-              KL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, v));
+              KL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, D));
     }
-    STAGE(ST_REFRESH, KL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, v));
-    STAGE(ST_FROZEN, KL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, v));
-    STAGE(ST_EMIT_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ct, tb, v.emit, w->emitSorted,
+    STAGE(ST_REFRESH, KL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
+    STAGE(ST_FROZEN, KL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    STAGE(ST_EMIT_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ct, tb, v.emit, hd.emitSorted,
                                                                    ne, 0, 32, st)));
-    STAGE(ST_UNIQUE, HIP_TRY(hipcub::DeviceSelect::Unique(ct, tb, w->emitSorted, w->emitUnique,
+    STAGE(ST_UNIQUE, HIP_TRY(hipcub::DeviceSelect::Unique(ct, tb, hd.emitSorted, hd.emitUnique,
                                                           &v.sc->uniqueCount, ne, st)));
-    STAGE(ST_PAIR_COUNT, KL(k_pair_count, dim3(1), dim3(1), 0, st, v, w->emitUnique));
-    STAGE(ST_EXPAND,
-          KL(k_expand, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->emitUnique, w->pairFlags));
-    STAGE(ST_STALL_RESTORE, KL(k_stall_restore, dim3(Blocks(np, T)), dim3(T), 0, st, v));
-    STAGE(ST_SLEEP_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, v.keys, w->pairFlags,
+    STAGE(ST_PAIR_COUNT, KL(k_pair_count, dim3(1), dim3(1), 0, st, D));
+    STAGE(ST_EXPAND, KL(k_expand, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    STAGE(ST_STALL_RESTORE, KL(k_stall_restore, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    STAGE(ST_SLEEP_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, v.keys, hd.pairFlags,
                                                                  v.sleepKeys, &v.sc->sleepCount,
                                                                  np, st)));
     // Narrowphase.
-    STAGE(ST_NARROW, KL(k_narrow, dim3(Blocks(np, 64)), dim3(64), 0, st, v));
+    STAGE(ST_NARROW, KL(k_narrow, dim3(Blocks(np, 64)), dim3(64), 0, st, D));
     // Sweeps, islands, wake.
-    STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
-    STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, v));
-    STAGE(ST_ROOT, KL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
-    STAGE(ST_WAKE_MARK, KL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
-    STAGE(ST_WAKE_APPLY, KL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
+    STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+    STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    STAGE(ST_ROOT, KL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+    STAGE(ST_WAKE_MARK, KL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+    STAGE(ST_WAKE_APPLY, KL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     // Prepare and color.
-    float h = p->dt / (float)p->substeps;
-    float invH = h > 0.0f ? 1.0f / h : 0.0f;
-    STAGE(ST_CONS_FLAGS,
-          KL(k_constraint_flags, dim3(Blocks(np, T)), dim3(T), 0, st, v, w->pairFlags));
+    const m3SubstepParams sp = SubstepParams(p);
+    STAGE(ST_CONS_FLAGS, KL(k_constraint_flags, dim3(Blocks(np, T)), dim3(T), 0, st, D));
     hipcub::CountingInputIterator<int32_t> counting(0);
-    STAGE(ST_CONS_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, w->pairFlags,
+    STAGE(ST_CONS_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, hd.pairFlags,
                                                                 v.consPair,
                                                                 &v.sc->constraintCount, np, st)));
-    STAGE(ST_CONS_BODIES, KL(k_cons_bodies, dim3(Blocks(np, T)), dim3(T), 0, st, v));
+    STAGE(ST_CONS_BODIES, KL(k_cons_bodies, dim3(Blocks(np, T)), dim3(T), 0, st, D));
     if (nb <= M3_COLOR_LDS_BODIES)
     {
-        STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, v, w->bodyMasks,
-                           w->bodyOwner));
+        STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D));
     }
     else
     {
-        STAGE(ST_COLOR, KL(k_color<false>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, v,
-                           w->bodyMasks, w->bodyOwner));
+        STAGE(ST_COLOR, KL(k_color<false>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D));
     }
-    // Stable radix sort on the color byte: each color's list ascends.
-    STAGE(ST_COLOR_LISTS, HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
-                              ct, tb, v.colors, v.slotColors, w->iota, v.colorLists, np, 0, 5,
-                              st)));
-    STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, v, p->contactHertz,
-                         p->contactDampingRatio, h));
-    STAGE(ST_MOVER_FLAGS,
-          KL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->bodyFlags, p->dt));
-    STAGE(ST_MOVER_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, w->bodyFlags,
+    STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, D, p->contactHertz,
+                         p->contactDampingRatio, sp.h));
+    STAGE(ST_MOVER_FLAGS, KL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt));
+    STAGE(ST_MOVER_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, hd.bodyFlags,
                                                                  v.movers, &v.sc->moverCount, nb,
                                                                  st)));
     // Substeps: one cooperative launch.
-    m3SubstepParams sp;
-    sp.substeps = p->substeps;
-    sp.h = h;
-    sp.invH = invH;
-    sp.push = p->contactPushMaxSpeed;
-    sp.maxLinear = p->maximumLinearSpeed;
-    sp.maxAngular = p->maximumAngularSpeed;
-    sp.gravity = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
     {
-        void* args[] = {(void*)&v, (void*)&sp, (void*)&w->barrier};
+        m3SubstepParams spArg = sp;
+        void* args[] = {(void*)&D, (void*)&spArg};
         STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps,
-                                                              dim3(w->coopBlocks), dim3(w->coopThreads), args,
-                                                              0, st)));
+                                                              dim3(w->coopBlocks),
+                                                              dim3(w->coopThreads), args, 0, st)));
     }
-    STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, v));
-    STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, v, p->restitutionThreshold));
-    STAGE(ST_STORE, KL(k_store, dim3(Blocks(np, T)), dim3(T), 0, st, v));
+    STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+    STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, D, p->restitutionThreshold));
+    STAGE(ST_STORE, KL(k_store, dim3(Blocks(np, T)), dim3(T), 0, st, D));
     if (p->continuousEnabled)
     {
-        STAGE(ST_CCD, KL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, v));
+        STAGE(ST_CCD, KL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, D));
     }
-    STAGE(ST_SLEEP_TIMERS, KL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root,
-                              p->dt, p->sleepEnabled));
+    STAGE(ST_SLEEP_TIMERS,
+          KL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt, p->sleepEnabled));
     if (p->sleepEnabled)
     {
-        STAGE(ST_SLEEP_READY, KL(k_sleep_ready, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
-        STAGE(ST_SLEEP_FREEZE, KL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, v, w->root));
+        STAGE(ST_SLEEP_READY, KL(k_sleep_ready, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+        STAGE(ST_SLEEP_FREEZE, KL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     }
-    STAGE(ST_END, KL(k_end, dim3(1), dim3(1), 0, st, v));
+    STAGE(ST_END, KL(k_end, dim3(1), dim3(1), 0, st, D));
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1809,10 +2206,11 @@ extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStat
                  "is not implemented on the device path");
         return -1;
     }
-    if (EnqueueStep(w, p) != 0)
+    if (JoinBatch(w) != 0 || EnqueueStep(w, p) != 0)
     {
         return -1;
     }
+    w->streamBusy = 1;
     if (ProfileCollect(w, p->substeps) != 0)
     {
         return -1;
@@ -1824,32 +2222,429 @@ extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStat
     return 0;
 }
 
-extern "C" int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,
-                               int32_t count)
+// ---------------------------------------------------------------------------
+// Batched stepping: independent worlds of one m3gpu_step_many call share
+// every launch of the step (grid.y = world), each world in its own device
+// buffers.  The world list of the last batched call stays cached with both
+// parities of the kernel views, so repeated calls launch straight away.
+// ---------------------------------------------------------------------------
+
+struct m3gpuBatch
 {
-    for (int32_t k = 0; k < count; ++k)
+    int32_t count;
+    int32_t capacity;
+    m3gpuWorld** worlds;
+    int* baseCur;        // each world's cur when the views were built
+    int64_t* layouts;    // each world's layout version then
+    int64_t steps;       // batch steps since the views were built
+    m3Dev* hostViews;    // [2][capacity] staging
+    m3Dev* devViews;     // [2][capacity]: entry q*capacity+k serves world k on steps s with s%2 == q
+    hipStream_t stream;
+    hipEvent_t done;
+    int32_t maxBody;
+    int32_t maxShape;
+    int32_t maxPairs;
+    int32_t maxEmit;
+    int64_t maxPlaneWork;
+    int threads; // workgroup size of the per-world substep loop
+    // The step's launch sequence captured once per parity and replayed.
+    int useGraphs;
+    hipGraphExec_t graph[2];
+    m3gpuStepParams graphParams[2];
+    int graphReady[2];
+};
+
+static m3gpuBatch s_batch;
+static pthread_mutex_t s_batchLock = PTHREAD_MUTEX_INITIALIZER;
+
+// A world whose latest work ran in the batch stream waits for it before
+// anything else touches the world.
+static int JoinBatch(m3gpuWorld* w)
+{
+    if (!w->batched)
     {
-        if (m3gpu_step(worlds[k], &params[k], nullptr) < 0)
-        {
-            return -1;
-        }
+        return 0;
+    }
+    pthread_mutex_lock(&s_batchLock);
+    hipError_t e = hipEventSynchronize(s_batch.done);
+    w->batched = 0;
+    pthread_mutex_unlock(&s_batchLock);
+    if (e != hipSuccess)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "batch wait: %s", hipGetErrorString(e));
+        return -1;
     }
     return 0;
 }
 
+static void BatchDropGraphs(void)
+{
+    for (int q = 0; q < 2; ++q)
+    {
+        if (s_batch.graphReady[q])
+        {
+            (void)hipGraphExecDestroy(s_batch.graph[q]);
+            s_batch.graphReady[q] = 0;
+        }
+    }
+}
+
+static void BatchDrop(void)
+{
+    BatchDropGraphs();
+    if (s_batch.count > 0)
+    {
+        (void)hipStreamSynchronize(s_batch.stream);
+        for (int32_t k = 0; k < s_batch.count; ++k)
+        {
+            s_batch.worlds[k]->batched = 0;
+        }
+    }
+    s_batch.count = 0;
+}
+
+static void BatchForget(m3gpuWorld* w)
+{
+    pthread_mutex_lock(&s_batchLock);
+    for (int32_t k = 0; k < s_batch.count; ++k)
+    {
+        if (s_batch.worlds[k] == w)
+        {
+            BatchDrop();
+            break;
+        }
+    }
+    pthread_mutex_unlock(&s_batchLock);
+}
+
+static bool BatchEligible(m3gpuWorld* w)
+{
+    const m3View& v = w->view;
+    return v.emitCapacity <= M3_BATCH_EMIT_MAX && v.maxShape <= M3_BATCH_SHAPES_MAX &&
+           v.maxBody <= M3_COLOR_LDS_BODIES && Prof(w) == nullptr;
+}
+
+static int BatchReserve(int32_t n)
+{
+    if (n <= s_batch.capacity)
+    {
+        return 0;
+    }
+    if (s_batch.capacity == 0)
+    {
+        HIP_TRY(hipStreamCreateWithFlags(&s_batch.stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&s_batch.done, hipEventDisableTiming));
+        const char* genv = getenv("MAUL3D_AMD_BATCH_GRAPHS");
+        s_batch.useGraphs = genv == nullptr || atoi(genv) != 0;
+        const char* env = getenv("MAUL3D_AMD_BATCH_THREADS");
+        int t = env != nullptr ? atoi(env) : 256;
+        t = t < 64 ? 64 : (t > M3_BLOCK_MAX_THREADS ? M3_BLOCK_MAX_THREADS : t);
+        s_batch.threads = t & ~63;
+    }
+    int32_t cap = n < 64 ? 64 : n;
+    free(s_batch.worlds);
+    free(s_batch.baseCur);
+    free(s_batch.layouts);
+    free(s_batch.hostViews);
+    if (s_batch.devViews != nullptr)
+    {
+        HIP_TRY(hipFree(s_batch.devViews));
+    }
+    s_batch.worlds = (m3gpuWorld**)calloc((size_t)cap, sizeof(m3gpuWorld*));
+    s_batch.baseCur = (int*)calloc((size_t)cap, sizeof(int));
+    s_batch.layouts = (int64_t*)calloc((size_t)cap, sizeof(int64_t));
+    s_batch.hostViews = (m3Dev*)calloc(2 * (size_t)cap, sizeof(m3Dev));
+    if (s_batch.worlds == nullptr || s_batch.baseCur == nullptr || s_batch.layouts == nullptr ||
+        s_batch.hostViews == nullptr)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "batch: out of host memory");
+        s_batch.capacity = 0;
+        return -1;
+    }
+    HIP_TRY(hipMalloc(&s_batch.devViews, 2 * (size_t)cap * sizeof(m3Dev)));
+    s_batch.capacity = cap;
+    return 0;
+}
+
+static bool BatchCurrent(m3gpuWorld* const* ws, int32_t n)
+{
+    if (s_batch.count != n)
+    {
+        return false;
+    }
+    for (int32_t k = 0; k < n; ++k)
+    {
+        m3gpuWorld* w = ws[k];
+        if (s_batch.worlds[k] != w || w->devDirty || w->layout != s_batch.layouts[k] ||
+            w->cur != (s_batch.baseCur[k] ^ (int)(s_batch.steps & 1)))
+        {
+            return false;
+        }
+    }
+    return true;
+}
+
+static int BatchBuild(m3gpuWorld* const* ws, int32_t n)
+{
+    BatchDrop();
+    if (BatchReserve(n) != 0)
+    {
+        return -1;
+    }
+    s_batch.maxBody = s_batch.maxShape = s_batch.maxPairs = s_batch.maxEmit = 0;
+    s_batch.maxPlaneWork = 0;
+    for (int32_t k = 0; k < n; ++k)
+    {
+        m3gpuWorld* w = ws[k];
+        if (w->devDirty && RefreshDevCopies(w) != 0)
+        {
+            return -1;
+        }
+        HIP_TRY(hipStreamSynchronize(w->stream));
+        w->streamBusy = 0;
+        for (int q = 0; q < 2; ++q)
+        {
+            // Step s of the batch runs world k with cur = baseCur ^ 1 ^ (s & 1).
+            int parity = w->cur ^ 1 ^ q;
+            m3Dev& e = s_batch.hostViews[(size_t)q * s_batch.capacity + k];
+            e = w->dev;
+            e.v = w->view;
+            e.v.keys = w->keysBuf[parity];
+            e.v.manifolds = w->mansBuf[parity];
+            e.v.oldKeys = w->keysBuf[parity ^ 1];
+            e.v.oldManifolds = w->mansBuf[parity ^ 1];
+        }
+        s_batch.worlds[k] = w;
+        s_batch.baseCur[k] = w->cur;
+        s_batch.layouts[k] = w->layout;
+        const m3View& v = w->view;
+        s_batch.maxBody = v.maxBody > s_batch.maxBody ? v.maxBody : s_batch.maxBody;
+        s_batch.maxShape = v.maxShape > s_batch.maxShape ? v.maxShape : s_batch.maxShape;
+        s_batch.maxPairs = v.pairCapacity > s_batch.maxPairs ? v.pairCapacity : s_batch.maxPairs;
+        s_batch.maxEmit = v.emitCapacity > s_batch.maxEmit ? v.emitCapacity : s_batch.maxEmit;
+        int64_t pw = (int64_t)v.maxShape * v.planeCount;
+        s_batch.maxPlaneWork = pw > s_batch.maxPlaneWork ? pw : s_batch.maxPlaneWork;
+    }
+    HIP_TRY(hipMemcpy(s_batch.devViews, s_batch.hostViews,
+                      2 * (size_t)s_batch.capacity * sizeof(m3Dev), hipMemcpyHostToDevice));
+    s_batch.count = n;
+    s_batch.steps = 0;
+    return 0;
+}
+
+// The launch sequence of one batch step: the same stage order as
+// EnqueueStep, each stage one launch over all worlds (grid.y = world).
+static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* p)
+{
+    const uint32_t W = (uint32_t)s_batch.count;
+    const int T = 256;
+    const int BT = M3_BATCH_THREADS;
+    const int32_t nb = s_batch.maxBody;
+    const int32_t ns = s_batch.maxShape;
+    const int32_t np = s_batch.maxPairs;
+    const int32_t ne = s_batch.maxEmit;
+    const m3SubstepParams sp = SubstepParams(p);
+    KL(k_begin, dim3(1, W), dim3(1), 0, st, D);
+    KL(k_fill_emit, dim3(Blocks(ne, T), W), dim3(T), 0, st, D);
+    KL(k_fresh, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
+    KL(k_reset_sleep, dim3(1, W), dim3(1), 0, st, D);
+    KL(k_pending_all, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    KL(k_sap_sort_block, dim3(1, W), dim3(BT), 0, st, D);
+    KL(k_sap_gather, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
+    KL(k_sap, dim3(Blocks(ns, 64), W), dim3(64), 0, st, D);
+    if (s_batch.maxPlaneWork > 0)
+    {
+        KL(k_planes, dim3(Blocks(s_batch.maxPlaneWork, T), W), dim3(T), 0, st, D);
+    }
+    KL(k_refresh, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
+    KL(k_frozen, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_emit_sort_block, dim3(1, W), dim3(BT), 0, st, D);
+    KL(k_pair_count, dim3(1, W), dim3(1), 0, st, D);
+    KL(k_expand, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_stall_restore, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 0);
+    KL(k_narrow, dim3(Blocks(np, 64), W), dim3(64), 0, st, D);
+    KL(k_capture, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    KL(k_union, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_root, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    KL(k_wake_mark, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    KL(k_wake_apply, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    KL(k_constraint_flags, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 1);
+    KL(k_cons_bodies, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_color<true>, dim3(1, W), dim3(M3_COLOR_THREADS), 0, st, D);
+    KL(k_prepare, dim3(Blocks(np, 64), W), dim3(64), 0, st, D, p->contactHertz,
+       p->contactDampingRatio, sp.h);
+    KL(k_mover_flags, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->dt);
+    KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 2);
+    KL(k_substeps_block, dim3(1, W), dim3(s_batch.threads), 0, st, D, sp);
+    KL(k_clear_forces, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    KL(k_restitution, dim3(1, W), dim3(1), 0, st, D, p->restitutionThreshold);
+    KL(k_store, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    if (p->continuousEnabled)
+    {
+        KL(k_ccd, dim3(Blocks(nb, 64), W), dim3(64), 0, st, D);
+    }
+    KL(k_sleep_timers, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->dt, p->sleepEnabled);
+    if (p->sleepEnabled)
+    {
+        KL(k_sleep_ready, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+        KL(k_sleep_freeze, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    }
+    KL(k_end, dim3(1, W), dim3(1), 0, st, D);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// One step of every world in the batch.
+static int BatchEnqueue(const m3gpuStepParams* p)
+{
+    hipStream_t st = s_batch.stream;
+    const int q = (int)(s_batch.steps & 1);
+    const m3Dev* D = s_batch.devViews + (size_t)q * s_batch.capacity;
+    for (int32_t k = 0; k < s_batch.count; ++k)
+    {
+        m3gpuWorld* w = s_batch.worlds[k];
+        w->cur ^= 1;
+        m3View& v = w->view;
+        v.keys = w->keysBuf[w->cur];
+        v.manifolds = w->mansBuf[w->cur];
+        v.oldKeys = w->keysBuf[w->cur ^ 1];
+        v.oldManifolds = w->mansBuf[w->cur ^ 1];
+        w->fullQuery = 0;
+        w->batched = 1;
+    }
+    if (s_batch.useGraphs)
+    {
+        if (s_batch.graphReady[q] && memcmp(&s_batch.graphParams[q], p, sizeof(*p)) != 0)
+        {
+            HIP_TRY(hipGraphExecDestroy(s_batch.graph[q]));
+            s_batch.graphReady[q] = 0;
+        }
+        if (!s_batch.graphReady[q])
+        {
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int rc = BatchLaunches(st, D, p);
+            hipError_t e = hipStreamEndCapture(st, &g);
+            if (rc != 0 || e != hipSuccess)
+            {
+                snprintf(s_lastError, sizeof(s_lastError), "batch graph capture failed: %s",
+                         hipGetErrorString(e));
+                return -1;
+            }
+            HIP_TRY(hipGraphInstantiate(&s_batch.graph[q], g, nullptr, nullptr, 0));
+            HIP_TRY(hipGraphDestroy(g));
+            s_batch.graphParams[q] = *p;
+            s_batch.graphReady[q] = 1;
+        }
+        HIP_TRY(hipGraphLaunch(s_batch.graph[q], st));
+    }
+    else if (BatchLaunches(st, D, p) != 0)
+    {
+        return -1;
+    }
+    HIP_TRY(hipEventRecord(s_batch.done, st));
+    s_batch.steps += 1;
+    return 0;
+}
+
+extern "C" int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,
+                               int32_t count)
+{
+    if (count <= 0)
+    {
+        return 0;
+    }
+    if (worlds == nullptr || params == nullptr || count > 65535)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step_many: bad arguments");
+        return -1;
+    }
+    // One launch sequence serves the call when every world fits the block
+    // kernels, all step with the same parameters and none appears twice.
+    bool batch = count >= 2;
+    for (int32_t k = 0; k < count && batch; ++k)
+    {
+        m3gpuWorld* w = worlds[k];
+        const m3gpuStepParams* p = &params[k];
+        if (w == nullptr || !(p->dt > 0.0f) || p->substeps < 1 || p->substeps > M3_MAX_SUBSTEPS ||
+            (p->continuousEnabled && w->ccdUnsupported) || !BatchEligible(w) ||
+            memcmp(p, &params[0], sizeof(*p)) != 0)
+        {
+            batch = false;
+        }
+    }
+    if (batch)
+    {
+        // A world listed twice steps twice, in order: no batch then.
+        static int64_t s_stamp = 0;
+        pthread_mutex_lock(&s_batchLock);
+        int64_t stamp = ++s_stamp;
+        for (int32_t k = 0; k < count; ++k)
+        {
+            if (worlds[k]->stamp == stamp)
+            {
+                batch = false;
+            }
+            worlds[k]->stamp = stamp;
+        }
+        pthread_mutex_unlock(&s_batchLock);
+    }
+    if (!batch)
+    {
+        for (int32_t k = 0; k < count; ++k)
+        {
+            if (m3gpu_step(worlds[k], &params[k], nullptr) < 0)
+            {
+                return -1;
+            }
+        }
+        return 0;
+    }
+    pthread_mutex_lock(&s_batchLock);
+    int rc = 0;
+    if (!BatchCurrent(worlds, count))
+    {
+        rc = BatchBuild(worlds, count);
+    }
+    else
+    {
+        // worlds stepped on their own streams since the last batch step
+        for (int32_t k = 0; k < count && rc == 0; ++k)
+        {
+            if (worlds[k]->streamBusy)
+            {
+                if (hipStreamSynchronize(worlds[k]->stream) != hipSuccess)
+                {
+                    rc = -1;
+                }
+                worlds[k]->streamBusy = 0;
+            }
+        }
+    }
+    if (rc == 0)
+    {
+        rc = BatchEnqueue(&params[0]);
+    }
+    pthread_mutex_unlock(&s_batchLock);
+    return rc;
+}
+
 extern "C" int m3gpu_sync(m3gpuWorld* w)
 {
-    if (w == nullptr)
+    if (w == nullptr || JoinBatch(w) != 0)
     {
         return -1;
     }
     HIP_TRY(hipStreamSynchronize(w->stream));
+    w->streamBusy = 0;
     return CheckFault(w);
 }
 
 extern "C" int m3gpu_last_stats(m3gpuWorld* w, m3gpuStepStats* stats)
 {
-    if (w == nullptr || stats == nullptr)
+    if (w == nullptr || stats == nullptr || JoinBatch(w) != 0)
     {
         return -1;
     }
@@ -1870,6 +2665,10 @@ static m3Profiler* Prof(m3gpuWorld* w) { return (w->prof != nullptr && w->prof->
 
 extern "C" int m3gpu_profile_begin(m3gpuWorld* w)
 {
+    if (w != nullptr && JoinBatch(w) != 0)
+    {
+        return -1;
+    }
     if (w == nullptr)
     {
         return -1;

@@ -88,7 +88,8 @@ typedef struct m3Scalars
     int32_t discoverCount;
     int32_t pointCount; // contact points over all constraints
     int32_t colorStarts[M3_GRAPH_COLORS + 2];
-    int32_t fault; // sticky: a substep grid barrier timed out
+    int32_t fault;     // sticky: a substep grid barrier timed out
+    int32_t fullQuery; // the next step rebuilds the frozen-pair buffer
     uint64_t stepCount;
 } m3Scalars;
 

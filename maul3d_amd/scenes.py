@@ -246,6 +246,73 @@ def toss(api: Api, seed: int, boxes: int = 32) -> WorldId:
     return world
 
 
+def batch_world(api: Api, index: int, seed: int = 0xB47C4) -> WorldId:
+    """World ``index`` of the batched-rollback set (uneven sizes and kinds).
+
+    Each world draws from its own stream SplitMix64(seed + 0x9E3779B9 * index):
+    a body budget N = 50 + floor(U[0,1951)) and a kind floor(U[0,3)):
+
+    * 0, pyramid: the largest r-row pyramid of unit cubes (half-extent 0.5,
+      centres x = i - (r-1-row)/2, y = 0.5 + row, z = 0) with r(r+1)/2 <= N;
+    * 1, box stacks: height h = 10 + floor(U[0,31)), N // h columns of unit
+      cubes on a square grid 1.5 m apart, y = 0.5 + level;
+    * 2, rain: N bodies, even k a sphere (radius U[0.25,0.5]), odd k a box
+      (half-extents U[0.25,0.5] each), positions U[-e,e] x U[1,11] x U[-e,e]
+      with e = max(3, 0.6 sqrt(N)), rotations uniform.
+
+    All dynamic, density 1000, friction 0.6, on a plane; capacities N + 8.
+    """
+    rng = SplitMix64(seed + 0x9E3779B9 * index)
+    n = 50 + int(rng.range(0.0, 1951.0))
+    kind = int(rng.range(0.0, 3.0))
+    world = _world(api, n + 8, n + 8)
+    _ground(api, world)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    unit = Vec3(0.5, 0.5, 0.5)
+    if kind == 0:
+        rows = int((math.sqrt(8.0 * n + 1.0) - 1.0) / 2.0)
+        while rows * (rows + 1) // 2 > n:
+            rows -= 1
+        for r in range(rows):
+            for i in range(rows - r):
+                bd.position = Pos3(float(i) - 0.5 * float(rows - 1 - r), 0.5 + float(r), 0.0)
+                body = api.m3CreateBody(world, C.byref(bd))
+                _check(api.m3CreateBoxShape(body, C.byref(sd), unit), "box")
+    elif kind == 1:
+        h = 10 + int(rng.range(0.0, 31.0))
+        columns = max(1, n // h)
+        side = int(math.ceil(math.sqrt(columns)))
+        for c in range(columns):
+            x = 1.5 * float(c % side) - 0.75 * float(side - 1)
+            z = 1.5 * float(c // side) - 0.75 * float(side - 1)
+            for level in range(h):
+                bd.position = Pos3(x, 0.5 + float(level), z)
+                body = api.m3CreateBody(world, C.byref(bd))
+                _check(api.m3CreateBoxShape(body, C.byref(sd), unit), "box")
+    else:
+        e = max(3.0, 0.6 * math.sqrt(float(n)))
+        for k in range(n):
+            x = rng.range(-e, e)
+            y = rng.range(1.0, 11.0)
+            z = rng.range(-e, e)
+            qx, qy, qz, qw = rng.unit_quat()
+            bd.position = Pos3(x, y, z)
+            bd.rotation = Quat(qx, qy, qz, qw)
+            body = api.m3CreateBody(world, C.byref(bd))
+            if k % 2 == 0:
+                ball = Sphere(Vec3(0.0, 0.0, 0.0), rng.range(0.25, 0.5))
+                _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+            else:
+                half = Vec3(rng.range(0.25, 0.5), rng.range(0.25, 0.5), rng.range(0.25, 0.5))
+                _check(api.m3CreateBoxShape(body, C.byref(sd), half), "box")
+        bd.rotation = Quat(0.0, 0.0, 0.0, 1.0)
+    return world
+
+
 SCENES = {
     "bench_pyramid": bench_pyramid,
     "smoke10k": smoke10k,

@@ -65,8 +65,38 @@ def toss_fixture(ref: api.Api, worlds: int, steps: int) -> dict:
     return out
 
 
+def batch_rollback_fixture(ref: api.Api, worlds: int, warm: int, resim: int) -> dict:
+    """Per world of the batched-rollback set: the hash after ``warm`` steps,
+    then snapshot, restore and ``resim`` more steps (hash after each)."""
+    out = {"warm": warm, "resim": resim, "worlds": {}}
+    for k in range(worlds):
+        w = scenes.batch_world(ref, k)
+        for _ in range(warm):
+            ref.m3World_Step(w, 1.0 / 60.0, 4)
+        snap = ref.snapshot(w)
+        rec = {"warm": "%016x" % ref.m3World_Hash(w), "bodies": ref.m3World_GetCounters(w).bodyCount}
+        if not ref.restore(w, snap):
+            raise RuntimeError(f"reference restore refused world {k}")
+        hashes = []
+        for _ in range(resim):
+            ref.m3World_Step(w, 1.0 / 60.0, 4)
+            hashes.append("%016x" % ref.m3World_Hash(w))
+        rec["resim"] = hashes
+        out["worlds"][str(k)] = rec
+        ref.m3DestroyWorld(w)
+        if k % 32 == 31:
+            print("batch world", k + 1, flush=True)
+    return out
+
+
 def main() -> None:
     ref = api.Api(C.CDLL(REF))
+    if len(sys.argv) > 1 and sys.argv[1] == "batch":
+        data = batch_rollback_fixture(ref, 256, 100, 8)
+        with open(os.path.join(HERE, "batch_rollback.json"), "w") as f:
+            json.dump(data, f, indent=0, sort_keys=True)
+        print("wrote batch_rollback.json")
+        return
     data = {"dt": "1/60", "substeps": 4, "scenes": {}}
     for name, (steps, every) in PLAN.items():
         print("scene", name, flush=True)
