@@ -40,6 +40,7 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-batch", action="store_true", help="skip the batched-rollback measurement")
     p.add_argument("--batch-worlds", type=int, default=256)
+    p.add_argument("--batch-only", action="store_true", help="run only the batched-rollback measurement")
     p.add_argument(
         "--copy-calibration",
         action="store_true",
@@ -222,6 +223,9 @@ def main() -> None:
             file=sys.stderr,
         )
         sys.exit(2)
+    if args.batch_only:
+        print(json.dumps({"batched": run_batched(args.batch_worlds)}))
+        return
     res = run_gpu(args.steps, args.warmup)
     value = BODIES * SUBSTEPS * args.steps / res["wall"]
     line = {
