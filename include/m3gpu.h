@@ -39,6 +39,7 @@ typedef struct m3gpuCaps
     int32_t shapeCapacity;
     int32_t pairCapacity;
     int32_t hullCapacity;
+    int32_t jointCapacity;
 } m3gpuCaps;
 
 // Host arrays of one world.  Element types are the records of m3_core.h
@@ -105,6 +106,33 @@ typedef struct m3gpuHostState
     uint8_t* frozenDirty;
     uint8_t* pairsFullQuery;
     uint64_t* stepCount; // advanced on the device by every step that completes
+    // joints (static group unless noted): slot arrays in the reference's
+    // layout, the body joint lists, and the joint components the host
+    // derives (joints linked through non-static bodies; component k owns
+    // jointCompList[jointCompStart[k] .. jointCompStart[k + 1]), ascending)
+    int32_t maxJoint;
+    int32_t jointCompCount;
+    uint8_t* jointAlive;
+    uint8_t* jointType;
+    int32_t* jointBodyA;
+    int32_t* jointBodyB;
+    void* jointLocalA; // m3Vec3
+    void* jointLocalB;
+    uint8_t* jointCollide;
+    void* jointFrameQA; // m3Quat
+    void* jointFrameQB;
+    uint8_t* jointFlags;
+    void* jointLimits;         // m3Vec3: twist lower, twist upper, cone
+    void* jointImpulse;        // m3Vec3, dynamic group
+    void* jointPerpImpulse;    // m3Vec3, dynamic group
+    void* jointLimitImpulse;   // m3Vec3, dynamic group
+    void* jointAngularImpulse; // m3Vec3, dynamic group
+    void* jointSpringImpulse;  // m3Vec3, dynamic group
+    int32_t* jointNextA;
+    int32_t* jointNextB;
+    int32_t* bodyJointHead;
+    int32_t* jointCompStart; // jointCompCount + 1
+    int32_t* jointCompList;  // maxJoint
 } m3gpuHostState;
 
 typedef struct m3gpuStepParams

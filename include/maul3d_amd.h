@@ -122,6 +122,59 @@ typedef struct m3ShapeDef
     int32_t internalValue;
 } m3ShapeDef;
 
+// Joint kinds (include/maul3d/joint.h:12-62).  This build steps the
+// spherical joint with its cone and twist limits; m3CreateJoint refuses the
+// other kinds.
+typedef enum m3JointType
+{
+    m3_sphericalJoint = 0,
+    m3_revoluteJoint = 1,
+    m3_prismaticJoint = 2,
+    m3_fixedJoint = 3,
+    m3_distanceJoint = 4,
+    m3_genericJoint = 5,
+    m3_wheelJoint = 6,
+    m3_filterJoint = 7,
+    m3_parallelJoint = 8,
+    m3_motorJoint = 9,
+    m3_gearJoint = 10,
+    m3_pulleyJoint = 11,
+} m3JointType;
+
+// The joint definition, field for field (include/maul3d/joint.h:80-163);
+// build it with m3DefaultJointDef.
+typedef struct m3JointDef
+{
+    int32_t type;
+    m3BodyId bodyA;
+    m3BodyId bodyB;
+    m3Vec3 localAnchorA;
+    m3Vec3 localAnchorB;
+    m3Vec3 localAxisA;
+    m3Vec3 localAxisB;
+    bool enableLimit; // spherical: twist range [lowerLimit, upperLimit]
+    m3real lowerLimit;
+    m3real upperLimit;
+    bool enableMotor;
+    m3real motorSpeed;
+    m3real maxMotorEffort;
+    bool enableCone; // spherical: swing within coneAngle of A's frame z-axis
+    m3real coneAngle;
+    bool collideConnected;
+    uint8_t genericLinear[3];
+    uint8_t genericAngular[3];
+    uint8_t genericMotorAxis;
+    uint8_t genericReserved;
+    m3real genericLinearLower[3];
+    m3real genericLinearUpper[3];
+    m3real genericAngularLower[3];
+    m3real genericAngularUpper[3];
+    m3real ratio;
+    m3Pos3 groundAnchorA;
+    m3Pos3 groundAnchorB;
+    int32_t internalValue;
+} m3JointDef;
+
 typedef struct m3Sphere { m3Vec3 center; m3real radius; } m3Sphere;
 typedef struct m3Plane { m3Vec3 normal; m3real offset; } m3Plane;
 typedef struct m3Capsule { m3Vec3 point1; m3Vec3 point2; m3real radius; } m3Capsule;
@@ -227,6 +280,11 @@ M3_API m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def,
                                   m3Vec3 halfExtents);          // shape.h:98
 M3_API m3ShapeId m3CreateCapsuleShape(m3BodyId bodyId, const m3ShapeDef* def,
                                       const m3Capsule* capsule); // shape.h:112
+
+M3_API m3JointDef m3DefaultJointDef(void);                 // joint.h:168
+M3_API m3JointId m3CreateJoint(const m3JointDef* def);    // joint.h:174 (spherical only)
+M3_API void m3DestroyJoint(m3JointId jointId);            // joint.h:175
+M3_API bool m3Joint_IsValid(m3JointId jointId);           // joint.h:176
 
 #ifdef __cplusplus
 }

@@ -118,6 +118,41 @@ class ShapeDef(C.Structure):
     ]
 
 
+class JointDef(C.Structure):
+    """m3JointDef field for field (include/maul3d/joint.h:80-163), 224 bytes."""
+
+    _fields_ = [
+        ("type", c_i32),
+        ("bodyA", BodyId),
+        ("bodyB", BodyId),
+        ("localAnchorA", Vec3),
+        ("localAnchorB", Vec3),
+        ("localAxisA", Vec3),
+        ("localAxisB", Vec3),
+        ("enableLimit", C.c_bool),
+        ("lowerLimit", c_f),
+        ("upperLimit", c_f),
+        ("enableMotor", C.c_bool),
+        ("motorSpeed", c_f),
+        ("maxMotorEffort", c_f),
+        ("enableCone", C.c_bool),
+        ("coneAngle", c_f),
+        ("collideConnected", C.c_bool),
+        ("genericLinear", C.c_uint8 * 3),
+        ("genericAngular", C.c_uint8 * 3),
+        ("genericMotorAxis", C.c_uint8),
+        ("genericReserved", C.c_uint8),
+        ("genericLinearLower", c_f * 3),
+        ("genericLinearUpper", c_f * 3),
+        ("genericAngularLower", c_f * 3),
+        ("genericAngularUpper", c_f * 3),
+        ("ratio", c_f),
+        ("groundAnchorA", Pos3),
+        ("groundAnchorB", Pos3),
+        ("internalValue", c_i32),
+    ]
+
+
 class Sphere(C.Structure):
     _fields_ = [("center", Vec3), ("radius", c_f)]
 
@@ -213,6 +248,10 @@ _SIGNATURES = {
     "m3CreateCapsuleShape": (ShapeId, [BodyId, C.POINTER(ShapeDef), C.POINTER(Capsule)]),
     "m3CreateHullShape": (ShapeId, [BodyId, C.POINTER(ShapeDef), C.POINTER(Vec3), c_i32]),
     "m3Hash64": (c_u64, [c_u64, C.c_void_p, c_i32]),
+    "m3DefaultJointDef": (JointDef, []),
+    "m3CreateJoint": (JointId, [C.POINTER(JointDef)]),
+    "m3DestroyJoint": (None, [JointId]),
+    "m3Joint_IsValid": (C.c_bool, [JointId]),
 }
 
 

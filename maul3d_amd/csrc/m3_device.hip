@@ -222,6 +222,7 @@ __global__ void k_begin(const m3Dev* __restrict__ ds)
     sc->awakeCount = 0;
     sc->discoverCount = 0;
     sc->pointCount = 0;
+    sc->jointActive = 0;
 }
 
 __global__ void k_fill_emit(const m3Dev* __restrict__ ds)
@@ -378,7 +379,10 @@ __global__ void k_sap_gather(const m3Dev* __restrict__ ds)
     fl |= v.enabled[b] != 0 ? SA_ENABLED : 0;
     fl |= v.sensor[s] != 0 ? SA_SENSOR : 0;
     fl |= v.pending[b] != 0 ? SA_PENDING : 0;
-    fl |= (v.category[s] == 1ull && v.mask[s] == ~0ull && v.group[s] == 0) ? SA_DEFAULT_FILTER : 0;
+    fl |= (v.category[s] == 1ull && v.mask[s] == ~0ull && v.group[s] == 0 &&
+           v.bodyJointHead[b] == -1)
+              ? SA_DEFAULT_FILTER
+              : 0;
     attr[i] = make_int4(s, b, fl, 0);
 }
 
@@ -705,6 +709,40 @@ __global__ void k_union(const m3Dev* __restrict__ ds)
     }
 }
 
+// Joints couple islands like touching contacts; a moving kinematic
+// partner is a wake source through a joint too (src/solver.c:3520-3547).
+__global__ void k_union_joints(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || j >= v.maxJoint || v.jointAlive[j] == 0)
+    {
+        return;
+    }
+    int32_t bA = v.jointBodyA[j];
+    int32_t bB = v.jointBodyB[j];
+    int dynA = v.type[bA] == M3_DYNAMIC;
+    int dynB = v.type[bB] == M3_DYNAMIC;
+    if (dynA && dynB)
+    {
+        Unite(v.parent, bA, bB);
+    }
+    else if (dynA || dynB)
+    {
+        int32_t kin = dynA ? bB : bA;
+        int32_t dyn = dynA ? bA : bB;
+        if (v.type[kin] == M3_KINEMATIC)
+        {
+            m3Vec3 lin = v.lv[kin];
+            m3Vec3 ang = v.av[kin];
+            if (m3Dot3(lin, lin) > 0.0f || m3Dot3(ang, ang) > 0.0f)
+            {
+                v.forced[dyn] = 1;
+            }
+        }
+    }
+}
+
 __global__ void k_root(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
@@ -830,6 +868,43 @@ __global__ void k_prepare(const m3Dev* __restrict__ ds, float hertz, float zeta,
     {
         v.sc->anyRestitution = 1;
     }
+}
+
+// Joints with an awake dynamic side are prepared in place by slot
+// (src/solver.c:699-811); filter joints have no rows.
+__global__ void k_prepare_joints(const m3Dev* __restrict__ ds, float h)
+{
+    M3_WORLD;
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || j >= v.maxJoint)
+    {
+        return;
+    }
+    uint8_t active = 0;
+    if (v.jointAlive[j] != 0 && v.jointType[j] != 7)
+    {
+        int32_t a = v.jointBodyA[j];
+        int32_t b = v.jointBodyB[j];
+        if ((v.type[a] == M3_DYNAMIC && v.awake[a] != 0) ||
+            (v.type[b] == M3_DYNAMIC && v.awake[b] != 0))
+        {
+            active = 1;
+            m3PrepareJoint(&v, j, &v.jcons[j], h);
+            atomicAdd(&v.sc->jointActive, 1);
+        }
+    }
+    v.jointActiveFlag[j] = active;
+}
+
+__global__ void k_store_joints(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || j >= v.maxJoint || v.jointActiveFlag[j] == 0)
+    {
+        return;
+    }
+    m3StoreJoint(&v, &v.jcons[j], j);
 }
 
 // Greedy first-fit coloring in constraint order (src/solver.c:2741-2822),
@@ -1150,15 +1225,37 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
         starts[k] = v.sc->colorStarts[k];
     }
     const int32_t movers = v.sc->moverCount;
+    // Joints run before the contacts in each pass, serially within a joint
+    // component (src/solver.c:4149-4201); with joints the warm-start gather
+    // waits for the joint warm start.
+    const bool joints = v.sc->jointActive > 0;
+    const int32_t comps = v.jointCompCount;
     for (int32_t sub = 0; sub < sp.substeps; ++sub)
     {
         for (int32_t m = gid; m < movers; m += gsz)
         {
             int32_t b = v.movers[m];
             m3IntegrateVelocity(&v, b, sp.h, sp.gravity, sp.maxLinear, sp.maxAngular);
-            m3WarmStartBody(&v, b, starts[M3_GRAPH_COLORS], starts[M3_GRAPH_COLORS + 1]);
+            if (!joints)
+            {
+                m3WarmStartBody(&v, b, starts[M3_GRAPH_COLORS], starts[M3_GRAPH_COLORS + 1]);
+            }
         }
         sync();
+        if (joints)
+        {
+            for (int32_t k = gid; k < comps; k += gsz)
+            {
+                m3JointComponentPass(&v, k, 0, sp.invH);
+            }
+            sync();
+            for (int32_t m = gid; m < movers; m += gsz)
+            {
+                m3WarmStartBody(&v, v.movers[m], starts[M3_GRAPH_COLORS],
+                                starts[M3_GRAPH_COLORS + 1]);
+            }
+            sync();
+        }
         for (int mode = 1; mode < 3; ++mode)
         {
             if (mode == 2)
@@ -1166,6 +1263,14 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 for (int32_t m = gid; m < movers; m += gsz)
                 {
                     m3IntegratePosition(&v, v.movers[m], sp.h);
+                }
+                sync();
+            }
+            if (joints)
+            {
+                for (int32_t k = gid; k < comps; k += gsz)
+                {
+                    m3JointComponentPass(&v, k, mode, sp.invH);
                 }
                 sync();
             }
@@ -1630,6 +1735,21 @@ static int CreateImpl(m3gpuWorld* w)
     {
         return -1;
     }
+    int32_t nj = c.jointCapacity > 0 ? c.jointCapacity : 1;
+    if (DevAlloc(&v.jointAlive, nj) || DevAlloc(&v.jointType, nj) || DevAlloc(&v.jointBodyA, nj) ||
+        DevAlloc(&v.jointBodyB, nj) || DevAlloc(&v.jointLocalA, nj) ||
+        DevAlloc(&v.jointLocalB, nj) || DevAlloc(&v.jointCollide, nj) ||
+        DevAlloc(&v.jointFrameQA, nj) || DevAlloc(&v.jointFrameQB, nj) ||
+        DevAlloc(&v.jointFlags, nj) || DevAlloc(&v.jointLimits, nj) ||
+        DevAlloc(&v.jointImpulse, nj) || DevAlloc(&v.jointPerpImpulse, nj) ||
+        DevAlloc(&v.jointLimitImpulse, nj) || DevAlloc(&v.jointAngularImpulse, nj) ||
+        DevAlloc(&v.jointSpringImpulse, nj) || DevAlloc(&v.jointNextA, nj) ||
+        DevAlloc(&v.jointNextB, nj) || DevAlloc(&v.bodyJointHead, nb) ||
+        DevAlloc(&v.jointCompStart, nj + 1) || DevAlloc(&v.jointCompList, nj) ||
+        DevAlloc(&v.jcons, nj) || DevAlloc(&v.jointActiveFlag, nj))
+    {
+        return -1;
+    }
     HIP_TRY(hipHostMalloc((void**)&w->hostScalars, sizeof(m3Scalars), hipHostMallocDefault));
     memset(w->hostScalars, 0, sizeof(m3Scalars));
 
@@ -1733,7 +1853,12 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut, w->dev.sapValsIn,
                     w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.root,
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
-                    v.slotTable, v.slotMask, w->dev.barrier, w->cubTemp, w->devDev};
+                    v.slotTable, v.slotMask, w->dev.barrier, w->cubTemp, w->devDev,
+                    v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
+                    v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
+                    v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
+                    v.jointAngularImpulse, v.jointSpringImpulse, v.jointNextA, v.jointNextB,
+                    v.bodyJointHead, v.jointCompStart, v.jointCompList, v.jcons, v.jointActiveFlag};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     {
         if (ptrs[i] != nullptr)
@@ -1817,6 +1942,33 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(v.shapeNext, h->shapeNext, ns, int32_t);
         UP(v.hullIndex, h->shapeHullIndex, ns, int32_t);
         UP((void*)v.hulls, h->hullData, h->maxHull, m3HullData);
+        if (h->maxJoint > w->caps.jointCapacity)
+        {
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_upload: joints exceed capacity");
+            return -1;
+        }
+        int32_t mj = h->maxJoint;
+        v.maxJoint = mj;
+        v.jointCompCount = h->jointCompCount;
+        UP(v.bodyJointHead, h->bodyJointHead, nb, int32_t);
+        if (mj > 0)
+        {
+            UP(v.jointAlive, h->jointAlive, mj, uint8_t);
+            UP(v.jointType, h->jointType, mj, uint8_t);
+            UP(v.jointBodyA, h->jointBodyA, mj, int32_t);
+            UP(v.jointBodyB, h->jointBodyB, mj, int32_t);
+            UP(v.jointLocalA, h->jointLocalA, mj, m3Vec3);
+            UP(v.jointLocalB, h->jointLocalB, mj, m3Vec3);
+            UP(v.jointCollide, h->jointCollide, mj, uint8_t);
+            UP(v.jointFrameQA, h->jointFrameQA, mj, m3Quat);
+            UP(v.jointFrameQB, h->jointFrameQB, mj, m3Quat);
+            UP(v.jointFlags, h->jointFlags, mj, uint8_t);
+            UP(v.jointLimits, h->jointLimits, mj, m3Vec3);
+            UP(v.jointNextA, h->jointNextA, mj, int32_t);
+            UP(v.jointNextB, h->jointNextB, mj, int32_t);
+            UP(v.jointCompStart, h->jointCompStart, h->jointCompCount + 1, int32_t);
+            UP(v.jointCompList, h->jointCompList, mj, int32_t);
+        }
         // The plane list and the unsupported-target census are host facts.
         const uint8_t* alive = h->shapeAlive;
         const uint8_t* stype = h->shapeType;
@@ -1870,6 +2022,15 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(v.torque, h->bodyTorque, nb, m3Vec3);
         UP(v.hasTarget, h->bodyHasTarget, nb, uint8_t);
         UP(&v.sc->stepCount, h->stepCount, 1, uint64_t);
+        if (h->maxJoint > 0)
+        {
+            int32_t mj = h->maxJoint;
+            UP(v.jointImpulse, h->jointImpulse, mj, m3Vec3);
+            UP(v.jointPerpImpulse, h->jointPerpImpulse, mj, m3Vec3);
+            UP(v.jointLimitImpulse, h->jointLimitImpulse, mj, m3Vec3);
+            UP(v.jointAngularImpulse, h->jointAngularImpulse, mj, m3Vec3);
+            UP(v.jointSpringImpulse, h->jointSpringImpulse, mj, m3Vec3);
+        }
     }
     if (fields & M3GPU_FIELDS_PAIRS)
     {
@@ -1928,6 +2089,15 @@ extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
         DOWN(h->bodyTorque, v.torque, nb, m3Vec3);
         DOWN(h->bodyHasTarget, v.hasTarget, nb, uint8_t);
         DOWN(h->stepCount, &v.sc->stepCount, 1, uint64_t);
+        if (v.maxJoint > 0)
+        {
+            int32_t mj = v.maxJoint;
+            DOWN(h->jointImpulse, v.jointImpulse, mj, m3Vec3);
+            DOWN(h->jointPerpImpulse, v.jointPerpImpulse, mj, m3Vec3);
+            DOWN(h->jointLimitImpulse, v.jointLimitImpulse, mj, m3Vec3);
+            DOWN(h->jointAngularImpulse, v.jointAngularImpulse, mj, m3Vec3);
+            DOWN(h->jointSpringImpulse, v.jointSpringImpulse, mj, m3Vec3);
+        }
     }
     if (fields & M3GPU_FIELDS_PAIRS)
     {
@@ -2107,6 +2277,10 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     // Sweeps, islands, wake.
     STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    if (v.maxJoint > 0)
+    {
+        STAGE(ST_UNION, KL(k_union_joints, dim3(Blocks(v.maxJoint, T)), dim3(T), 0, st, D));
+    }
     STAGE(ST_ROOT, KL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_WAKE_MARK, KL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_WAKE_APPLY, KL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
@@ -2128,6 +2302,11 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     }
     STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, D, p->contactHertz,
                          p->contactDampingRatio, sp.h));
+    if (v.maxJoint > 0)
+    {
+        STAGE(ST_PREPARE,
+              KL(k_prepare_joints, dim3(Blocks(v.maxJoint, 64)), dim3(64), 0, st, D, sp.h));
+    }
     STAGE(ST_MOVER_FLAGS, KL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt));
     STAGE(ST_MOVER_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, hd.bodyFlags,
                                                                  v.movers, &v.sc->moverCount, nb,
@@ -2143,6 +2322,10 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, D, p->restitutionThreshold));
     STAGE(ST_STORE, KL(k_store, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    if (v.maxJoint > 0)
+    {
+        STAGE(ST_STORE, KL(k_store_joints, dim3(Blocks(v.maxJoint, T)), dim3(T), 0, st, D));
+    }
     if (p->continuousEnabled)
     {
         STAGE(ST_CCD, KL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, D));
@@ -2245,6 +2428,7 @@ struct m3gpuBatch
     int32_t maxShape;
     int32_t maxPairs;
     int32_t maxEmit;
+    int32_t maxJoint;
     int64_t maxPlaneWork;
     int threads; // workgroup size of the per-world substep loop
     // The step's launch sequence captured once per parity and replayed.
@@ -2392,6 +2576,7 @@ static int BatchBuild(m3gpuWorld* const* ws, int32_t n)
         return -1;
     }
     s_batch.maxBody = s_batch.maxShape = s_batch.maxPairs = s_batch.maxEmit = 0;
+    s_batch.maxJoint = 0;
     s_batch.maxPlaneWork = 0;
     for (int32_t k = 0; k < n; ++k)
     {
@@ -2422,6 +2607,7 @@ static int BatchBuild(m3gpuWorld* const* ws, int32_t n)
         s_batch.maxShape = v.maxShape > s_batch.maxShape ? v.maxShape : s_batch.maxShape;
         s_batch.maxPairs = v.pairCapacity > s_batch.maxPairs ? v.pairCapacity : s_batch.maxPairs;
         s_batch.maxEmit = v.emitCapacity > s_batch.maxEmit ? v.emitCapacity : s_batch.maxEmit;
+        s_batch.maxJoint = v.maxJoint > s_batch.maxJoint ? v.maxJoint : s_batch.maxJoint;
         int64_t pw = (int64_t)v.maxShape * v.planeCount;
         s_batch.maxPlaneWork = pw > s_batch.maxPlaneWork ? pw : s_batch.maxPlaneWork;
     }
@@ -2466,6 +2652,10 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_narrow, dim3(Blocks(np, 64), W), dim3(64), 0, st, D);
     KL(k_capture, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_union, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    if (s_batch.maxJoint > 0)
+    {
+        KL(k_union_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);
+    }
     KL(k_root, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_wake_mark, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_wake_apply, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
@@ -2475,12 +2665,20 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_color<true>, dim3(1, W), dim3(M3_COLOR_THREADS), 0, st, D);
     KL(k_prepare, dim3(Blocks(np, 64), W), dim3(64), 0, st, D, p->contactHertz,
        p->contactDampingRatio, sp.h);
+    if (s_batch.maxJoint > 0)
+    {
+        KL(k_prepare_joints, dim3(Blocks(s_batch.maxJoint, 64), W), dim3(64), 0, st, D, sp.h);
+    }
     KL(k_mover_flags, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->dt);
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 2);
     KL(k_substeps_block, dim3(1, W), dim3(s_batch.threads), 0, st, D, sp);
     KL(k_clear_forces, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_restitution, dim3(1, W), dim3(1), 0, st, D, p->restitutionThreshold);
     KL(k_store, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    if (s_batch.maxJoint > 0)
+    {
+        KL(k_store_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);
+    }
     if (p->continuousEnabled)
     {
         KL(k_ccd, dim3(Blocks(nb, 64), W), dim3(64), 0, st, D);

@@ -69,6 +69,38 @@ typedef struct m3Constraint
     m3CPoint points[M3_MANIFOLD_MAX_POINTS];
 } m3Constraint;
 
+// One prepared spherical joint (the fields of the reference's
+// m3JointConstraint that the spherical rows read, src/solver.c:584-639).
+typedef struct m3JointCon
+{
+    int32_t bodyA;
+    int32_t bodyB;
+    uint8_t flags; // bit0 twist limit, bit1 motor, bit2 cone, bit3 drive spring
+    uint8_t pad[3];
+    m3Vec3 rA; // COM-relative anchors, world-rotated at prepare
+    m3Vec3 rB;
+    m3Vec3 deltaCenter; // cB - cA at prepare
+    m3real invMassA;
+    m3real invMassB;
+    m3Mat3 invIA;
+    m3Mat3 invIB;
+    m3Softness softness;
+    m3Vec3 impulse;
+    m3Quat frameQA;
+    m3Quat frameQB;
+    m3Vec3 swingAxis;
+    m3Vec3 twistJacobian;
+    m3real swingMass;
+    m3real twistMass;
+    m3real coneAngle;
+    m3real lowerLimit;
+    m3real upperLimit;
+    m3real lowerImpulse;
+    m3real upperImpulse;
+    m3real swingImpulse;
+    m3Vec3 springImpulseV;
+} m3JointCon;
+
 // Step-wide scalars that live on the device so no stage needs a host
 // round trip.
 typedef struct m3Scalars
@@ -90,6 +122,7 @@ typedef struct m3Scalars
     int32_t colorStarts[M3_GRAPH_COLORS + 2];
     int32_t fault;     // sticky: a substep grid barrier timed out
     int32_t fullQuery; // the next step rebuilds the frozen-pair buffer
+    int32_t jointActive; // joints prepared this step
     uint64_t stepCount;
 } m3Scalars;
 
@@ -176,6 +209,34 @@ typedef struct m3View
     uint8_t* flag;
     uint8_t* pending; // bodies whose frozen pairs the next sweep records
     m3Scalars* sc;
+    // joints: slot arrays (the reference's world arrays), the joint
+    // components (joints linked through non-static bodies, each a run of
+    // ascending slots), and this step's prepared records
+    int32_t maxJoint;
+    int32_t jointCompCount;
+    uint8_t* jointAlive;
+    uint8_t* jointType;
+    int32_t* jointBodyA;
+    int32_t* jointBodyB;
+    m3Vec3* jointLocalA;
+    m3Vec3* jointLocalB;
+    uint8_t* jointCollide;
+    m3Quat* jointFrameQA;
+    m3Quat* jointFrameQB;
+    uint8_t* jointFlags;
+    m3Vec3* jointLimits;
+    m3Vec3* jointImpulse;
+    m3Vec3* jointPerpImpulse;
+    m3Vec3* jointLimitImpulse;
+    m3Vec3* jointAngularImpulse;
+    m3Vec3* jointSpringImpulse;
+    int32_t* jointNextA;
+    int32_t* jointNextB;
+    int32_t* bodyJointHead;
+    int32_t* jointCompStart;
+    int32_t* jointCompList;
+    m3JointCon* jcons;
+    uint8_t* jointActiveFlag;
 } m3View;
 
 // ---------------------------------------------------------------------------
@@ -293,6 +354,20 @@ M3_HD int m3PairAllowed(const m3View* v, int32_t i, int32_t j)
     else if (!((v->category[i] & v->mask[j]) != 0 && (v->category[j] & v->mask[i]) != 0))
     {
         return 0;
+    }
+    // Jointed bodies skip contact unless the joint allows it
+    // (src/broad_phase.c:221-231).
+    if (v->bodyJointHead != 0)
+    {
+        for (int32_t jt = v->bodyJointHead[bi]; jt != -1;
+             jt = v->jointBodyA[jt] == bi ? v->jointNextA[jt] : v->jointNextB[jt])
+        {
+            int32_t other = v->jointBodyA[jt] == bi ? v->jointBodyB[jt] : v->jointBodyA[jt];
+            if (other == bj && v->jointCollide[jt] == 0)
+            {
+                return 0;
+            }
+        }
     }
     return 1;
 }
@@ -1212,6 +1287,257 @@ M3_HD m3Vec3 m3Solve3(const m3Mat3* J, m3Vec3 b)
     x.y = inv * m3Dot3(J->cx, m3Cross3(b, J->cz));
     x.z = inv * m3Dot3(J->cx, m3Cross3(J->cy, b));
     return x;
+}
+
+// ---------------------------------------------------------------------------
+// Spherical joints (src/solver.c:577-811, 1044-1066, 1195-1337, 2657-2730)
+// ---------------------------------------------------------------------------
+
+M3_HD m3real m3TwistAngle(m3Quat q)
+{
+    m3real twist = q.w < 0.0f ? m3Atan2(-q.z, -q.w) : m3Atan2(q.z, q.w);
+    return 2.0f * twist;
+}
+
+M3_HD m3real m3SwingAngle(m3Quat q)
+{
+    m3real x = sqrtf(q.z * q.z + q.w * q.w);
+    m3real y = sqrtf(q.x * q.x + q.y * q.y);
+    return 2.0f * m3Atan2(y, x);
+}
+
+// Joint j prepared at the step's start transforms (src/solver.c:699-811,
+// the spherical branch).
+M3_HD void m3PrepareJoint(const m3View* v, int32_t j, m3JointCon* c, m3real h)
+{
+    m3JointCon z;
+    memset(&z, 0, sizeof(z));
+    *c = z;
+    int32_t a = v->jointBodyA[j];
+    int32_t b = v->jointBodyB[j];
+    c->bodyA = a;
+    c->bodyB = b;
+    m3Transform xfA = v->xf[a];
+    m3Transform xfB = v->xf[b];
+    c->rA = m3RotateVec3(xfA.q, m3Sub3(v->jointLocalA[j], v->lc[a]));
+    c->rB = m3RotateVec3(xfB.q, m3Sub3(v->jointLocalB[j], v->lc[b]));
+    m3Vec3 rlcA = m3RotateVec3(xfA.q, v->lc[a]);
+    m3Vec3 rlcB = m3RotateVec3(xfB.q, v->lc[b]);
+    c->deltaCenter = m3V3((m3real)(xfB.p.x + (double)rlcB.x - xfA.p.x - (double)rlcA.x),
+                          (m3real)(xfB.p.y + (double)rlcB.y - xfA.p.y - (double)rlcA.y),
+                          (m3real)(xfB.p.z + (double)rlcB.z - xfA.p.z - (double)rlcA.z));
+    c->invMassA = v->type[a] == M3_DYNAMIC ? v->invMass[a] : 0.0f;
+    c->invMassB = v->type[b] == M3_DYNAMIC ? v->invMass[b] : 0.0f;
+    c->invIA = m3WorldInvInertia(v, a);
+    c->invIB = m3WorldInvInertia(v, b);
+    c->softness = m3MakeSoft(60.0f, 2.0f, h);
+    c->impulse = v->jointImpulse[j];
+    c->flags = v->jointFlags[j];
+    c->springImpulseV = v->jointSpringImpulse[j];
+    if ((c->flags & (1 | 4 | 8)) != 0)
+    {
+        c->frameQA = m3MulQuat(xfA.q, v->jointFrameQA[j]);
+        c->frameQB = m3MulQuat(xfB.q, v->jointFrameQB[j]);
+        m3Vec3 coneAxis = m3RotateVec3(c->frameQA, m3V3(0.0f, 0.0f, 1.0f));
+        m3Vec3 twistAxis = m3RotateVec3(c->frameQB, m3V3(0.0f, 0.0f, 1.0f));
+        m3Vec3 swing = m3Cross3(coneAxis, twistAxis);
+        m3real swingLen2 = m3Dot3(swing, swing);
+        if (swingLen2 > 1.0e-12f)
+        {
+            swing = m3MulSV3(1.0f / sqrtf(swingLen2), swing);
+        }
+        else
+        {
+            m3Vec3 t2;
+            m3TangentBasis(coneAxis, &swing, &t2);
+        }
+        c->swingAxis = swing;
+        m3Vec3 sumSwing = m3Add3(m3MulMV3(c->invIA, swing), m3MulMV3(c->invIB, swing));
+        m3real kSwing = m3Dot3(swing, sumSwing);
+        c->swingMass = kSwing > 0.0f ? 1.0f / kSwing : 0.0f;
+        m3Quat conjA = m3Q(-c->frameQA.x, -c->frameQA.y, -c->frameQA.z, c->frameQA.w);
+        m3Quat relQ = m3MulQuat(conjA, c->frameQB);
+        m3real denom = relQ.z * relQ.z + relQ.w * relQ.w;
+        m3real tanHalf =
+            denom > 1.0e-12f ? sqrtf((relQ.x * relQ.x + relQ.y * relQ.y) / denom) : 0.0f;
+        m3Vec3 perp = m3Cross3(swing, coneAxis);
+        c->twistJacobian = m3Add3(coneAxis, m3MulSV3(tanHalf, perp));
+        m3Vec3 sumTwist =
+            m3Add3(m3MulMV3(c->invIA, c->twistJacobian), m3MulMV3(c->invIB, c->twistJacobian));
+        m3real kTwist = m3Dot3(c->twistJacobian, sumTwist);
+        c->twistMass = kTwist > 0.0f ? 1.0f / kTwist : 0.0f;
+        c->coneAngle = v->jointLimits[j].z;
+        c->lowerLimit = v->jointLimits[j].x;
+        c->upperLimit = v->jointLimits[j].y;
+        c->lowerImpulse = v->jointLimitImpulse[j].x;
+        c->upperImpulse = v->jointLimitImpulse[j].y;
+        c->swingImpulse = v->jointLimitImpulse[j].z;
+    }
+}
+
+M3_HD void m3WarmStartJoint(const m3View* v, const m3JointCon* c)
+{
+    m3Vec3 rA = m3RotateVec3(v->deltaRot[c->bodyA], c->rA);
+    m3Vec3 rB = m3RotateVec3(v->deltaRot[c->bodyB], c->rB);
+    m3Vec3 angularImpulse = m3V3(0.0f, 0.0f, 0.0f);
+    m3Vec3 linearExtra = m3V3(0.0f, 0.0f, 0.0f);
+    if ((c->flags & (1 | 4 | 8)) != 0)
+    {
+        angularImpulse = m3MulSV3(-c->swingImpulse, c->swingAxis);
+        angularImpulse =
+            m3Add3(angularImpulse, m3MulSV3(c->lowerImpulse - c->upperImpulse, c->twistJacobian));
+        if ((c->flags & 8) != 0)
+        {
+            angularImpulse = m3Add3(angularImpulse, c->springImpulseV);
+        }
+    }
+    m3Vec3 totalLinear = m3Add3(c->impulse, linearExtra);
+    v->lv[c->bodyA] = m3Sub3(v->lv[c->bodyA], m3MulSV3(c->invMassA, totalLinear));
+    v->av[c->bodyA] = m3Sub3(v->av[c->bodyA],
+                             m3MulMV3(c->invIA, m3Add3(m3Cross3(rA, totalLinear), angularImpulse)));
+    v->lv[c->bodyB] = m3Add3(v->lv[c->bodyB], m3MulSV3(c->invMassB, totalLinear));
+    v->av[c->bodyB] = m3Add3(v->av[c->bodyB],
+                             m3MulMV3(c->invIB, m3Add3(m3Cross3(rB, totalLinear), angularImpulse)));
+}
+
+// One limit row: impulse delta along jac, accumulated and clamped at 0.
+M3_HD m3real m3JointLimitDelta(const m3JointCon* c, m3real cc, m3real cdot, m3real mass,
+                               m3real* acc, m3real invH, int useBias)
+{
+    m3real bias = 0.0f;
+    m3real massScale = 1.0f;
+    m3real impulseScale = 0.0f;
+    if (cc > 0.0f)
+    {
+        bias = cc * invH;
+    }
+    else if (useBias)
+    {
+        bias = c->softness.biasRate * cc;
+        massScale = c->softness.massScale;
+        impulseScale = c->softness.impulseScale;
+    }
+    m3real old = *acc;
+    m3real delta = -massScale * mass * (cdot + bias) - impulseScale * old;
+    *acc = m3MaxF(old + delta, 0.0f);
+    return *acc - old;
+}
+
+// Twist range, cone, then the point constraint (src/solver.c:1211-1337,
+// 2657-2700); the drive spring (flag 8) has no setter in this build.
+M3_HD void m3SolveJoint(const m3View* v, m3JointCon* c, m3real invH, int useBias)
+{
+    m3Vec3 rA = m3RotateVec3(v->deltaRot[c->bodyA], c->rA);
+    m3Vec3 rB = m3RotateVec3(v->deltaRot[c->bodyB], c->rB);
+    m3Vec3 vA = v->lv[c->bodyA];
+    m3Vec3 wA = v->av[c->bodyA];
+    m3Vec3 vB = v->lv[c->bodyB];
+    m3Vec3 wB = v->av[c->bodyB];
+    if ((c->flags & (1 | 4 | 8)) != 0)
+    {
+        m3Quat quatA = m3MulQuat(v->deltaRot[c->bodyA], c->frameQA);
+        m3Quat quatB = m3MulQuat(v->deltaRot[c->bodyB], c->frameQB);
+        m3Quat conjA = m3Q(-quatA.x, -quatA.y, -quatA.z, quatA.w);
+        m3Quat relQ = m3MulQuat(conjA, quatB);
+        if ((c->flags & 1) != 0)
+        {
+            m3real angle = m3TwistAngle(relQ);
+            m3Vec3 jac = c->twistJacobian;
+            m3real delta = m3JointLimitDelta(c, angle - c->lowerLimit, m3Dot3(m3Sub3(wB, wA), jac),
+                                             c->twistMass, &c->lowerImpulse, invH, useBias);
+            wA = m3Sub3(wA, m3MulSV3(delta, m3MulMV3(c->invIA, jac)));
+            wB = m3Add3(wB, m3MulSV3(delta, m3MulMV3(c->invIB, jac)));
+            delta = m3JointLimitDelta(c, c->upperLimit - angle, m3Dot3(m3Sub3(wA, wB), jac),
+                                      c->twistMass, &c->upperImpulse, invH, useBias);
+            wA = m3Add3(wA, m3MulSV3(delta, m3MulMV3(c->invIA, jac)));
+            wB = m3Sub3(wB, m3MulSV3(delta, m3MulMV3(c->invIB, jac)));
+        }
+        if ((c->flags & 4) != 0)
+        {
+            m3real swingAngle = m3SwingAngle(relQ);
+            m3Vec3 axis = c->swingAxis;
+            m3real delta = m3JointLimitDelta(c, c->coneAngle - swingAngle,
+                                             m3Dot3(m3Sub3(wA, wB), axis), c->swingMass,
+                                             &c->swingImpulse, invH, useBias);
+            wA = m3Add3(wA, m3MulSV3(delta, m3MulMV3(c->invIA, axis)));
+            wB = m3Sub3(wB, m3MulSV3(delta, m3MulMV3(c->invIB, axis)));
+        }
+        v->av[c->bodyA] = wA;
+        v->av[c->bodyB] = wB;
+    }
+    m3Vec3 cdot = m3Sub3(m3Add3(vB, m3Cross3(wB, rB)), m3Add3(vA, m3Cross3(wA, rA)));
+    m3Vec3 bias = m3V3(0.0f, 0.0f, 0.0f);
+    m3real massScale = 1.0f;
+    m3real impulseScale = 0.0f;
+    if (useBias)
+    {
+        m3Vec3 separation = m3Add3(
+            m3Add3(m3Sub3(v->deltaPos[c->bodyB], v->deltaPos[c->bodyA]), m3Sub3(rB, rA)),
+            c->deltaCenter);
+        bias = m3MulSV3(c->softness.biasRate, separation);
+        massScale = c->softness.massScale;
+        impulseScale = c->softness.impulseScale;
+    }
+    m3Mat3 k;
+    for (int32_t axisIndex = 0; axisIndex < 3; ++axisIndex)
+    {
+        m3Vec3 e = m3V3(axisIndex == 0 ? 1.0f : 0.0f, axisIndex == 1 ? 1.0f : 0.0f,
+                        axisIndex == 2 ? 1.0f : 0.0f);
+        m3Vec3 tA = m3Cross3(rA, m3MulMV3(c->invIA, m3Cross3(rA, e)));
+        m3Vec3 tB = m3Cross3(rB, m3MulMV3(c->invIB, m3Cross3(rB, e)));
+        m3Vec3 col = m3Sub3(m3MulSV3(c->invMassA + c->invMassB, e), m3Add3(tA, tB));
+        if (axisIndex == 0)
+        {
+            k.cx = col;
+        }
+        else if (axisIndex == 1)
+        {
+            k.cy = col;
+        }
+        else
+        {
+            k.cz = col;
+        }
+    }
+    m3Vec3 sol = m3Solve3(&k, m3Add3(cdot, bias));
+    m3Vec3 impulse = m3Sub3(m3MulSV3(-massScale, sol), m3MulSV3(impulseScale, c->impulse));
+    c->impulse = m3Add3(c->impulse, impulse);
+    v->lv[c->bodyA] = m3Sub3(vA, m3MulSV3(c->invMassA, impulse));
+    v->av[c->bodyA] = m3Sub3(wA, m3MulMV3(c->invIA, m3Cross3(rA, impulse)));
+    v->lv[c->bodyB] = m3Add3(vB, m3MulSV3(c->invMassB, impulse));
+    v->av[c->bodyB] = m3Add3(wB, m3MulMV3(c->invIB, m3Cross3(rB, impulse)));
+}
+
+// src/solver.c:2704-2730 for a spherical joint.
+M3_HD void m3StoreJoint(const m3View* v, const m3JointCon* c, int32_t j)
+{
+    v->jointImpulse[j] = c->impulse;
+    v->jointPerpImpulse[j] = m3V3(0.0f, 0.0f, 0.0f);
+    v->jointLimitImpulse[j] = m3V3(c->lowerImpulse, c->upperImpulse, c->swingImpulse);
+    v->jointAngularImpulse[j] = m3V3(0.0f, 0.0f, 0.0f);
+    v->jointSpringImpulse[j] = c->springImpulseV;
+}
+
+// Every joint of component k, serially in ascending slot order (the
+// reference's order restricted to joints that share non-static bodies).
+M3_HD void m3JointComponentPass(const m3View* v, int32_t k, int mode, m3real invH)
+{
+    for (int32_t e = v->jointCompStart[k]; e < v->jointCompStart[k + 1]; ++e)
+    {
+        int32_t j = v->jointCompList[e];
+        if (v->jointActiveFlag[j] == 0)
+        {
+            continue;
+        }
+        if (mode == 0)
+        {
+            m3WarmStartJoint(v, &v->jcons[j]);
+        }
+        else
+        {
+            m3SolveJoint(v, &v->jcons[j], invH, mode == 1);
+        }
+    }
 }
 
 // src/solver.c:430-581

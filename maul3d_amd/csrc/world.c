@@ -27,6 +27,8 @@
 #define M3_WORLD_COOKIE ((int32_t)(M3_COOKIE ^ ((int32_t)sizeof(m3WorldDef) << 8)))
 #define M3_BODY_COOKIE ((int32_t)(M3_COOKIE ^ ((int32_t)sizeof(m3BodyDef) << 8) ^ 1))
 #define M3_SHAPE_COOKIE ((int32_t)(M3_COOKIE ^ ((int32_t)sizeof(m3ShapeDef) << 8) ^ 2))
+#define M3_JOINT_COOKIE ((int32_t)(M3_COOKIE ^ ((int32_t)sizeof(m3JointDef) << 8) ^ 3))
+_Static_assert(sizeof(m3JointDef) == 224, "m3JointDef layout");
 
 // Tuning defaults (src/world_internal.h:1024-1046).
 #define M3_CONTACT_HERTZ_DEFAULT 30.0f
@@ -207,7 +209,37 @@ typedef struct m3World
     m3HullData* hullData;
     int32_t* hullRefCounts;
 
-    m3IdPool jointPool; // joints are not on the device path yet: always empty
+    // Joints (src/world_internal.h joint arrays; spherical on this build).
+    m3IdPool jointPool;
+    uint8_t* jointType;
+    int32_t* jointBodyA;
+    int32_t* jointBodyB;
+    m3Vec3* jointLocalA;
+    m3Vec3* jointLocalB;
+    uint8_t* jointCollide;
+    m3Vec3* jointImpulse;
+    m3Vec3* jointPerpImpulse;
+    m3Vec3* jointLimitImpulse;
+    m3Vec3* jointAngularImpulse;
+    m3Quat* jointFrameQA;
+    m3Quat* jointFrameQB;
+    uint8_t* jointFlags;
+    m3Vec3* jointMotor;
+    m3Vec3* jointBreak;
+    m3Vec3* jointSpring;
+    float* jointTargetScalar;
+    m3Quat* jointTargetQ;
+    m3Vec3* jointSpringImpulse;
+    m3Vec3* jointLimits;
+    int32_t* jointNextA;
+    int32_t* jointNextB;
+    int32_t* bodyJointHead;
+    // the device's joint schedule: components of joints linked through
+    // non-static bodies, each a run of ascending slots
+    int32_t jointCompCount;
+    int32_t* jointCompStart;
+    int32_t* jointCompList;
+    int32_t* jointScratch;
 
     uint64_t* pairKeys;
     m3Manifold* manifolds;
@@ -314,6 +346,127 @@ static void HostView(m3World* w, m3gpuHostState* h)
     h->frozenDirty = &w->frozenDirty;
     h->pairsFullQuery = &w->pairsFullQuery;
     h->stepCount = &w->stepCount;
+    h->maxJoint = w->jointPool.maxIndex;
+    h->jointCompCount = w->jointCompCount;
+    h->jointAlive = w->jointPool.alive;
+    h->jointType = w->jointType;
+    h->jointBodyA = w->jointBodyA;
+    h->jointBodyB = w->jointBodyB;
+    h->jointLocalA = w->jointLocalA;
+    h->jointLocalB = w->jointLocalB;
+    h->jointCollide = w->jointCollide;
+    h->jointFrameQA = w->jointFrameQA;
+    h->jointFrameQB = w->jointFrameQB;
+    h->jointFlags = w->jointFlags;
+    h->jointLimits = w->jointLimits;
+    h->jointImpulse = w->jointImpulse;
+    h->jointPerpImpulse = w->jointPerpImpulse;
+    h->jointLimitImpulse = w->jointLimitImpulse;
+    h->jointAngularImpulse = w->jointAngularImpulse;
+    h->jointSpringImpulse = w->jointSpringImpulse;
+    h->jointNextA = w->jointNextA;
+    h->jointNextB = w->jointNextB;
+    h->bodyJointHead = w->bodyJointHead;
+    h->jointCompStart = w->jointCompStart;
+    h->jointCompList = w->jointCompList;
+}
+
+// The joint schedule: joints that share a non-static body (directly or
+// through other joints) form a component; the device runs each component
+// serially in ascending slot order and the components in parallel, which
+// is the reference's serial slot order restricted to joints that can see
+// each other's writes (joints never change static or kinematic velocities:
+// those bodies have zero inverse mass and inertia in the joint rows).
+static int32_t JointFind(int32_t* parent, int32_t x)
+{
+    while (parent[x] != x)
+    {
+        parent[x] = parent[parent[x]];
+        x = parent[x];
+    }
+    return x;
+}
+
+static void BuildJointComponents(m3World* w)
+{
+    int32_t nj = w->jointPool.maxIndex;
+    int32_t nb = w->bodyPool.maxIndex;
+    int32_t* parent = w->jointScratch;        // bodies
+    int32_t* compOf = w->jointScratch + nb;   // per joint: component root body, -1 none
+    for (int32_t b = 0; b < nb; ++b)
+    {
+        parent[b] = b;
+    }
+    for (int32_t j = 0; j < nj; ++j)
+    {
+        compOf[j] = -1;
+        if (w->jointPool.alive[j] == 0)
+        {
+            continue;
+        }
+        int32_t a = w->jointBodyA[j];
+        int32_t b = w->jointBodyB[j];
+        int movA = w->types[a] != (uint8_t)M3_STATIC;
+        int movB = w->types[b] != (uint8_t)M3_STATIC;
+        if (movA && movB)
+        {
+            int32_t ra = JointFind(parent, a);
+            int32_t rb = JointFind(parent, b);
+            if (ra != rb)
+            {
+                parent[ra < rb ? rb : ra] = ra < rb ? ra : rb;
+            }
+        }
+    }
+    // Components in order of their first joint; joints ascending within.
+    int32_t count = 0;
+    int32_t* label = (int32_t*)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(int32_t));
+    int32_t* sizes = (int32_t*)calloc((size_t)(nj > 0 ? nj : 1) + 1, sizeof(int32_t));
+    if (label == NULL || sizes == NULL)
+    {
+        fprintf(stderr, "maul3d_amd: out of host memory\n");
+        abort();
+    }
+    for (int32_t b = 0; b < nb; ++b)
+    {
+        label[b] = -1;
+    }
+    for (int32_t j = 0; j < nj; ++j)
+    {
+        if (w->jointPool.alive[j] == 0)
+        {
+            continue;
+        }
+        int32_t a = w->jointBodyA[j];
+        int32_t body = w->types[a] != (uint8_t)M3_STATIC ? a : w->jointBodyB[j];
+        int32_t root = JointFind(parent, body);
+        if (label[root] < 0)
+        {
+            label[root] = count;
+            count += 1;
+        }
+        compOf[j] = label[root];
+        sizes[label[root] + 1] += 1;
+    }
+    for (int32_t k = 0; k < count; ++k)
+    {
+        sizes[k + 1] += sizes[k];
+    }
+    for (int32_t k = 0; k <= count; ++k)
+    {
+        w->jointCompStart[k] = sizes[k];
+    }
+    for (int32_t j = 0; j < nj; ++j)
+    {
+        if (compOf[j] >= 0)
+        {
+            w->jointCompList[sizes[compOf[j]]] = j;
+            sizes[compOf[j]] += 1;
+        }
+    }
+    w->jointCompCount = count;
+    free(label);
+    free(sizes);
 }
 
 static void DeviceFail(const char* what)
@@ -460,9 +613,11 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
     ALLOC(w->bodyHasTarget, cap, uint8_t);
     ALLOC(w->bodyTarget, cap, m3Transform);
     ALLOC(w->bodyShapeHead, cap, int32_t);
+    ALLOC(w->bodyJointHead, cap, int32_t);
     for (int32_t i = 0; i < cap; ++i)
     {
         w->bodyShapeHead[i] = -1;
+        w->bodyJointHead[i] = -1;
     }
 
     int32_t shapeCap = def->shapeCapacity;
@@ -499,7 +654,43 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
     w->hullPool = PoolCreate(shapeCap);
     ALLOC(w->hullData, shapeCap, m3HullData);
     ALLOC(w->hullRefCounts, shapeCap, int32_t);
-    w->jointPool = PoolCreate(def->jointCapacity);
+    int32_t jointCap = def->jointCapacity;
+    w->jointPool = PoolCreate(jointCap);
+    ALLOC(w->jointType, jointCap, uint8_t);
+    ALLOC(w->jointBodyA, jointCap, int32_t);
+    ALLOC(w->jointBodyB, jointCap, int32_t);
+    ALLOC(w->jointLocalA, jointCap, m3Vec3);
+    ALLOC(w->jointLocalB, jointCap, m3Vec3);
+    ALLOC(w->jointCollide, jointCap, uint8_t);
+    ALLOC(w->jointImpulse, jointCap, m3Vec3);
+    ALLOC(w->jointPerpImpulse, jointCap, m3Vec3);
+    ALLOC(w->jointLimitImpulse, jointCap, m3Vec3);
+    ALLOC(w->jointAngularImpulse, jointCap, m3Vec3);
+    ALLOC(w->jointFrameQA, jointCap, m3Quat);
+    ALLOC(w->jointFrameQB, jointCap, m3Quat);
+    ALLOC(w->jointFlags, jointCap, uint8_t);
+    ALLOC(w->jointMotor, jointCap, m3Vec3);
+    ALLOC(w->jointBreak, jointCap, m3Vec3);
+    ALLOC(w->jointSpring, jointCap, m3Vec3);
+    ALLOC(w->jointTargetScalar, jointCap, float);
+    ALLOC(w->jointTargetQ, jointCap, m3Quat);
+    ALLOC(w->jointSpringImpulse, jointCap, m3Vec3);
+    ALLOC(w->jointLimits, jointCap, m3Vec3);
+    ALLOC(w->jointNextA, jointCap, int32_t);
+    ALLOC(w->jointNextB, jointCap, int32_t);
+    ALLOC(w->jointCompStart, (size_t)jointCap + 1, int32_t);
+    ALLOC(w->jointCompList, jointCap, int32_t);
+    ALLOC(w->jointScratch, (size_t)jointCap + (size_t)cap, int32_t);
+    for (int32_t j = 0; j < jointCap; ++j)
+    {
+        w->jointBodyA[j] = -1;
+        w->jointBodyB[j] = -1;
+        w->jointNextA[j] = -1;
+        w->jointNextB[j] = -1;
+        w->jointFrameQA[j].w = 1.0f;
+        w->jointFrameQB[j].w = 1.0f;
+        w->jointTargetQ[j].w = 1.0f;
+    }
 
     w->pairCapacity = 8 * shapeCap;
     ALLOC(w->pairKeys, w->pairCapacity, uint64_t);
@@ -537,6 +728,12 @@ void m3DestroyWorld(m3WorldId worldId)
         w->shapeCategory, w->shapeMask, w->shapeGroup, w->shapeUserData, w->shapeNext,
         w->shapeHullIndex, w->shapeMeshIndex, w->shapeSensor, w->leafBoxes, w->hullData,
         w->hullRefCounts, w->pairKeys, w->manifolds, w->sleepingPairKeys,
+        w->jointType, w->jointBodyA, w->jointBodyB, w->jointLocalA, w->jointLocalB,
+        w->jointCollide, w->jointImpulse, w->jointPerpImpulse, w->jointLimitImpulse,
+        w->jointAngularImpulse, w->jointFrameQA, w->jointFrameQB, w->jointFlags, w->jointMotor,
+        w->jointBreak, w->jointSpring, w->jointTargetScalar, w->jointTargetQ,
+        w->jointSpringImpulse, w->jointLimits, w->jointNextA, w->jointNextB, w->bodyJointHead,
+        w->jointCompStart, w->jointCompList, w->jointScratch,
     };
     for (size_t i = 0; i < sizeof(arrays) / sizeof(arrays[0]); ++i)
     {
@@ -1416,6 +1613,236 @@ m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def, m3Vec3 halfEx
 // Step (src/solver.c:4322-4330 wall; the body runs on the device)
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+// Joints (src/joint.c:15-520): the spherical joint with its cone and twist
+// limits.  Other kinds are refused (m3_nullJointId).
+// ---------------------------------------------------------------------------
+
+m3JointDef m3DefaultJointDef(void)
+{
+    m3JointDef def;
+    memset(&def, 0, sizeof(def));
+    def.type = m3_sphericalJoint;
+    def.localAxisA = m3V3(0.0f, 0.0f, 1.0f);
+    def.localAxisB = m3V3(0.0f, 0.0f, 1.0f);
+    def.genericMotorAxis = 255;
+    def.ratio = 1.0f;
+    def.internalValue = M3_JOINT_COOKIE;
+    return def;
+}
+
+// Quaternion of an orthonormal right-handed column basis, branching on the
+// largest diagonal term (src/joint.c:30-75).
+static m3Quat QuatFromBasis(m3Vec3 t1, m3Vec3 t2, m3Vec3 axis)
+{
+    float m00 = t1.x, m01 = t2.x, m02 = axis.x;
+    float m10 = t1.y, m11 = t2.y, m12 = axis.y;
+    float m20 = t1.z, m21 = t2.z, m22 = axis.z;
+    m3Quat q;
+    float trace = m00 + m11 + m22;
+    if (trace > 0.0f)
+    {
+        float s = sqrtf(trace + 1.0f) * 2.0f;
+        q.w = 0.25f * s;
+        q.x = (m21 - m12) / s;
+        q.y = (m02 - m20) / s;
+        q.z = (m10 - m01) / s;
+    }
+    else if (m00 > m11 && m00 > m22)
+    {
+        float s = sqrtf(1.0f + m00 - m11 - m22) * 2.0f;
+        q.w = (m21 - m12) / s;
+        q.x = 0.25f * s;
+        q.y = (m01 + m10) / s;
+        q.z = (m02 + m20) / s;
+    }
+    else if (m11 > m22)
+    {
+        float s = sqrtf(1.0f + m11 - m00 - m22) * 2.0f;
+        q.w = (m02 - m20) / s;
+        q.x = (m01 + m10) / s;
+        q.y = 0.25f * s;
+        q.z = (m12 + m21) / s;
+    }
+    else
+    {
+        float s = sqrtf(1.0f + m22 - m00 - m11) * 2.0f;
+        q.w = (m10 - m01) / s;
+        q.x = (m02 + m20) / s;
+        q.y = (m12 + m21) / s;
+        q.z = 0.25f * s;
+    }
+    return m3NormalizeQuat(q);
+}
+
+// Frame whose z-axis is the given unit axis; the tangent basis fixes x, y.
+static m3Quat QuatFromAxisZ(m3Vec3 axis)
+{
+    m3Vec3 t1;
+    m3Vec3 t2;
+    m3TangentBasis(axis, &t1, &t2);
+    return QuatFromBasis(t1, t2, axis);
+}
+
+static m3World* ResolveJoint(m3JointId id, int32_t* out)
+{
+    m3World* w = WorldFromIndex0(id.world0);
+    int32_t index = id.index1 - 1;
+    if (w == NULL || !PoolValid(&w->jointPool, index, id.generation))
+    {
+        return NULL;
+    }
+    *out = index;
+    return w;
+}
+
+m3JointId m3CreateJoint(const m3JointDef* def)
+{
+    m3JointId none = {0, 0, 0};
+    if (def == NULL || def->internalValue != M3_JOINT_COOKIE || def->type != m3_sphericalJoint)
+    {
+        return none;
+    }
+    if (!FiniteV3(def->localAnchorA) || !FiniteV3(def->localAnchorB) ||
+        !FiniteV3(def->localAxisA) || !FiniteV3(def->localAxisB) || !FiniteF(def->lowerLimit) ||
+        !FiniteF(def->upperLimit) || !FiniteF(def->motorSpeed) || !FiniteF(def->maxMotorEffort) ||
+        !FiniteF(def->coneAngle) || def->maxMotorEffort < 0.0f ||
+        (def->enableLimit && def->lowerLimit > def->upperLimit) ||
+        (def->enableCone && def->coneAngle < 0.0f))
+    {
+        return none;
+    }
+    m3World* w = WorldFromIndex0(def->bodyA.world0);
+    if (w == NULL || def->bodyB.world0 != def->bodyA.world0)
+    {
+        return none;
+    }
+    int32_t a = BodySlot(w, def->bodyA);
+    int32_t b = BodySlot(w, def->bodyB);
+    if (a < 0 || b < 0 || a == b)
+    {
+        return none;
+    }
+    if (w->types[a] != (uint8_t)M3_DYNAMIC && w->types[b] != (uint8_t)M3_DYNAMIC)
+    {
+        return none;
+    }
+    Touch(w, M3GPU_FIELDS_STATIC | M3GPU_FIELDS_DYNAMIC);
+    int32_t index = PoolAlloc(&w->jointPool);
+    if (index < 0)
+    {
+        return none;
+    }
+    w->jointType[index] = (uint8_t)def->type;
+    w->jointBodyA[index] = a;
+    w->jointBodyB[index] = b;
+    w->jointLocalA[index] = def->localAnchorA;
+    w->jointLocalB[index] = def->localAnchorB;
+    w->jointCollide[index] = def->collideConnected ? 1 : 0;
+    w->jointImpulse[index] = m3V3(0.0f, 0.0f, 0.0f);
+    w->jointPerpImpulse[index] = m3V3(0.0f, 0.0f, 0.0f);
+    w->jointLimitImpulse[index] = m3V3(0.0f, 0.0f, 0.0f);
+    w->jointAngularImpulse[index] = m3V3(0.0f, 0.0f, 0.0f);
+    w->jointFrameQA[index] = QuatFromAxisZ(m3Normalize3(def->localAxisA));
+    w->jointFrameQB[index] = QuatFromAxisZ(m3Normalize3(def->localAxisB));
+    w->jointFlags[index] = (uint8_t)((def->enableLimit ? 1 : 0) | (def->enableMotor ? 2 : 0) |
+                                     (def->enableCone ? 4 : 0));
+    w->jointMotor[index] = m3V3(def->motorSpeed, def->maxMotorEffort, 0.0f);
+    w->jointBreak[index] = m3V3(0.0f, 0.0f, 0.0f);
+    w->jointSpring[index] = m3V3(0.0f, 0.0f, 0.0f);
+    w->jointTargetScalar[index] = 0.0f;
+    w->jointTargetQ[index] = m3Q(0.0f, 0.0f, 0.0f, 1.0f);
+    w->jointSpringImpulse[index] = m3V3(0.0f, 0.0f, 0.0f);
+    w->jointLimits[index] = m3V3(def->lowerLimit, def->upperLimit, def->coneAngle);
+    w->jointNextA[index] = w->bodyJointHead[a];
+    w->bodyJointHead[a] = index;
+    w->jointNextB[index] = w->bodyJointHead[b];
+    w->bodyJointHead[b] = index;
+    if (w->types[a] == (uint8_t)M3_DYNAMIC)
+    {
+        w->awake[a] = 1;
+        w->sleepTimes[a] = 0.0f;
+    }
+    if (w->types[b] == (uint8_t)M3_DYNAMIC)
+    {
+        w->awake[b] = 1;
+        w->sleepTimes[b] = 0.0f;
+    }
+    m3JointId id = {index + 1, w->worldIndex0, w->jointPool.generations[index]};
+    return id;
+}
+
+static void UnlinkJoint(m3World* w, int32_t body, int32_t index)
+{
+    int32_t* cursor = &w->bodyJointHead[body];
+    while (*cursor != -1)
+    {
+        int32_t j = *cursor;
+        if (j == index)
+        {
+            *cursor = w->jointBodyA[j] == body ? w->jointNextA[j] : w->jointNextB[j];
+            return;
+        }
+        cursor = w->jointBodyA[j] == body ? &w->jointNextA[j] : &w->jointNextB[j];
+    }
+}
+
+void m3DestroyJoint(m3JointId jointId)
+{
+    int32_t index;
+    m3World* w = ResolveJoint(jointId, &index);
+    if (w == NULL)
+    {
+        return;
+    }
+    Touch(w, M3GPU_FIELDS_STATIC | M3GPU_FIELDS_DYNAMIC);
+    int32_t a = w->jointBodyA[index];
+    int32_t b = w->jointBodyB[index];
+    UnlinkJoint(w, a, index);
+    UnlinkJoint(w, b, index);
+    if (w->types[a] == (uint8_t)M3_DYNAMIC && w->bodyPool.alive[a] != 0)
+    {
+        w->awake[a] = 1;
+        w->sleepTimes[a] = 0.0f;
+    }
+    if (w->types[b] == (uint8_t)M3_DYNAMIC && w->bodyPool.alive[b] != 0)
+    {
+        w->awake[b] = 1;
+        w->sleepTimes[b] = 0.0f;
+    }
+    m3Vec3 zero = m3V3(0.0f, 0.0f, 0.0f);
+    m3Quat identity = m3Q(0.0f, 0.0f, 0.0f, 1.0f);
+    w->jointType[index] = 0;
+    w->jointBodyA[index] = -1;
+    w->jointBodyB[index] = -1;
+    w->jointLocalA[index] = zero;
+    w->jointLocalB[index] = zero;
+    w->jointCollide[index] = 0;
+    w->jointImpulse[index] = zero;
+    w->jointPerpImpulse[index] = zero;
+    w->jointLimitImpulse[index] = zero;
+    w->jointAngularImpulse[index] = zero;
+    w->jointFrameQA[index] = identity;
+    w->jointFrameQB[index] = identity;
+    w->jointFlags[index] = 0;
+    w->jointMotor[index] = zero;
+    w->jointBreak[index] = zero;
+    w->jointSpring[index] = zero;
+    w->jointTargetScalar[index] = 0.0f;
+    w->jointTargetQ[index] = identity;
+    w->jointSpringImpulse[index] = zero;
+    w->jointLimits[index] = zero;
+    w->jointNextA[index] = -1;
+    w->jointNextB[index] = -1;
+    PoolFree(&w->jointPool, index);
+}
+
+bool m3Joint_IsValid(m3JointId jointId)
+{
+    int32_t index;
+    return ResolveJoint(jointId, &index) != NULL;
+}
+
 static int EnsureDevice(m3World* w)
 {
     if (w->dev == NULL)
@@ -1425,6 +1852,7 @@ static int EnsureDevice(m3World* w)
         caps.shapeCapacity = w->shapeCapacity;
         caps.pairCapacity = w->pairCapacity;
         caps.hullCapacity = w->shapeCapacity;
+        caps.jointCapacity = w->jointCapacity;
         w->dev = m3gpu_world_create(&caps);
         if (w->dev == NULL)
         {
@@ -1434,6 +1862,10 @@ static int EnsureDevice(m3World* w)
     }
     if (w->devDirty != 0)
     {
+        if (w->devDirty & M3GPU_FIELDS_STATIC)
+        {
+            BuildJointComponents(w);
+        }
         m3gpuHostState h;
         HostView(w, &h);
         if (m3gpu_upload(w->dev, &h, w->devDirty) != 0)
@@ -1670,8 +2102,45 @@ uint64_t m3World_Hash(m3WorldId worldId)
         h = m3HashBytes(h, &w->shapeSensor[i], 1);
     }
     // Character, soft-body, vehicle and voxel pools are empty here, so
-    // their blocks contribute nothing; joints likewise while the pool is
-    // empty.
+    // their blocks contribute nothing.  Joints (src/snapshot.c:1300-1357;
+    // the generic and pulley extensions never apply: only spherical joints
+    // exist here).
+    for (int32_t i = 0; i < w->jointPool.maxIndex; ++i)
+    {
+        uint8_t alive = w->jointPool.alive[i];
+        h = m3HashBytes(h, &alive, 1);
+        if (alive == 0)
+        {
+            continue;
+        }
+        h = m3HashBytes(h, &w->jointType[i], 1);
+        h = m3HashBytes(h, &w->jointBodyA[i], 4);
+        h = m3HashBytes(h, &w->jointBodyB[i], 4);
+        h = m3HashBytes(h, &w->jointLocalA[i], (int32_t)sizeof(m3Vec3));
+        h = m3HashBytes(h, &w->jointLocalB[i], (int32_t)sizeof(m3Vec3));
+        h = m3HashBytes(h, &w->jointImpulse[i], (int32_t)sizeof(m3Vec3));
+        if (w->jointBreak[i].x != 0.0f || w->jointBreak[i].y != 0.0f)
+        {
+            h = m3HashBytes(h, &w->jointBreak[i], (int32_t)sizeof(m3Vec3));
+        }
+        if ((w->jointFlags[i] & 8) != 0 || w->jointTargetScalar[i] != 0.0f ||
+            w->jointTargetQ[i].x != 0.0f || w->jointTargetQ[i].y != 0.0f ||
+            w->jointTargetQ[i].z != 0.0f || w->jointTargetQ[i].w != 1.0f ||
+            w->jointSpringImpulse[i].x != 0.0f || w->jointSpringImpulse[i].y != 0.0f ||
+            w->jointSpringImpulse[i].z != 0.0f)
+        {
+            h = m3HashBytes(h, &w->jointSpring[i], (int32_t)sizeof(m3Vec3));
+            h = m3HashBytes(h, &w->jointTargetScalar[i], 4);
+            h = m3HashBytes(h, &w->jointTargetQ[i], (int32_t)sizeof(m3Quat));
+            h = m3HashBytes(h, &w->jointSpringImpulse[i], (int32_t)sizeof(m3Vec3));
+        }
+        h = m3HashBytes(h, &w->jointPerpImpulse[i], (int32_t)sizeof(m3Vec3));
+        h = m3HashBytes(h, &w->jointLimitImpulse[i], (int32_t)sizeof(m3Vec3));
+        h = m3HashBytes(h, &w->jointAngularImpulse[i], (int32_t)sizeof(m3Vec3));
+        h = m3HashBytes(h, &w->jointFrameQA[i], (int32_t)sizeof(m3Quat));
+        h = m3HashBytes(h, &w->jointFrameQB[i], (int32_t)sizeof(m3Quat));
+        h = m3HashBytes(h, &w->jointFlags[i], 1);
+    }
     h = m3HashBytes(h, &w->pairCount, 4);
     for (int32_t i = 0; i < w->pairCount; ++i)
     {
@@ -1768,7 +2237,7 @@ m3Profile m3World_GetProfile(m3WorldId worldId)
 // ---------------------------------------------------------------------------
 
 #define M3_SNAP_MAGIC 0x31414D33u // "3MA1"
-#define M3_SNAP_VERSION 1u
+#define M3_SNAP_VERSION 2u
 
 typedef struct m3SnapHeader
 {
@@ -1784,6 +2253,8 @@ typedef struct m3SnapHeader
     int32_t hullMax;
     int32_t pairCount;
     int32_t sleepingPairCount;
+    int32_t jointCapacity;
+    int32_t reserved;
     uint64_t stepCount;
     uint64_t payloadHash;
 } m3SnapHeader;
@@ -1902,6 +2373,31 @@ static void WalkWorld(m3Walker* k, m3World* w, int32_t nb, int32_t ns, int32_t n
     Walk(k, w->manifolds, (int64_t)np * sizeof(m3Manifold));
     Walk(k, w->sleepingPairKeys, (int64_t)nz * 8);
     Walk(k, &w->frozenDirty, 1);
+    int64_t nj = w->jointCapacity;
+    WalkPool(k, &w->jointPool);
+    Walk(k, w->jointType, nj);
+    Walk(k, w->jointBodyA, nj * 4);
+    Walk(k, w->jointBodyB, nj * 4);
+    Walk(k, w->jointLocalA, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointLocalB, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointCollide, nj);
+    Walk(k, w->jointImpulse, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointPerpImpulse, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointLimitImpulse, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointAngularImpulse, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointFrameQA, nj * (int64_t)sizeof(m3Quat));
+    Walk(k, w->jointFrameQB, nj * (int64_t)sizeof(m3Quat));
+    Walk(k, w->jointFlags, nj);
+    Walk(k, w->jointMotor, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointBreak, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointSpring, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointTargetScalar, nj * 4);
+    Walk(k, w->jointTargetQ, nj * (int64_t)sizeof(m3Quat));
+    Walk(k, w->jointSpringImpulse, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointLimits, nj * (int64_t)sizeof(m3Vec3));
+    Walk(k, w->jointNextA, nj * 4);
+    Walk(k, w->jointNextB, nj * 4);
+    Walk(k, w->bodyJointHead, (int64_t)nb * 4);
 }
 
 int32_t m3World_SnapshotSize(m3WorldId worldId)
@@ -1943,6 +2439,7 @@ int32_t m3World_Snapshot(m3WorldId worldId, void* out, int32_t capacity)
     hdr.hullMax = w->hullPool.maxIndex;
     hdr.pairCount = w->pairCount;
     hdr.sleepingPairCount = w->sleepingPairCount;
+    hdr.jointCapacity = w->jointCapacity;
     hdr.stepCount = w->stepCount;
     m3Walker k;
     memset(&k, 0, sizeof(k));
@@ -1970,7 +2467,7 @@ bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size)
         hdr.bodyMax < 0 || hdr.bodyMax > w->bodyCapacity || hdr.shapeMax < 0 ||
         hdr.shapeMax > w->shapeCapacity || hdr.hullMax < 0 || hdr.hullMax > w->shapeCapacity ||
         hdr.pairCount < 0 || hdr.pairCount > w->pairCapacity || hdr.sleepingPairCount < 0 ||
-        hdr.sleepingPairCount > w->pairCapacity)
+        hdr.sleepingPairCount > w->pairCapacity || hdr.jointCapacity != w->jointCapacity)
     {
         return false;
     }

@@ -14,6 +14,7 @@ import math
 
 from .api import (
     DYNAMIC,
+    STATIC,
     Api,
     BodyDef,
     Capsule,
@@ -313,9 +314,83 @@ def batch_world(api: Api, index: int, seed: int = 0xB47C4) -> WorldId:
     return world
 
 
+def chains(
+    api: Api,
+    count: int = 500,
+    links: int = 20,
+    boxes: int = 4000,
+    seed: int = 0xC4A1,
+    height: float = 14.0,
+) -> WorldId:
+    """Capsule chains on spherical joints with 45-degree cone limits, plus
+    optional falling boxes.
+
+    Chain c hangs from a static anchor body at (12 (c % 10) - 54, height,
+    1.5 (c // 10) - 37.5).  Its ``links`` capsules (radius 0.1, segment from
+    -0.25 to 0.25 along local x, density 1000, friction 0.6) start
+    horizontal along +x, link k centred 0.5 k + 0.25 from the anchor.  Joint
+    k pins the previous body (the anchor for k = 0) to link k at their shared
+    point; both frame z-axes are local +x (the chain direction), the cone
+    half-angle is 0.7853982 rad and jointed bodies do not collide.  Creation
+    order: the plane, then per chain the anchor, its links, then its joints.
+    ``boxes`` boxes (half-extents U[0.25,0.75]) then fall from
+    U[-55,55] x U[20,40] x U[-40,40] with uniform rotations (stream
+    SplitMix64(seed)).
+    """
+    n_bodies = count * (links + 1) + boxes + 1
+    world = _world(api, n_bodies + 8, n_bodies + 8, joints=count * links + 8)
+    _ground(api, world)
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    capsule = Capsule(Vec3(-0.25, 0.0, 0.0), Vec3(0.25, 0.0, 0.0), 0.1)
+    for c in range(count):
+        x0 = 12.0 * float(c % 10) - 54.0
+        z0 = 1.5 * float(c // 10) - 37.5
+        ad = api.m3DefaultBodyDef()
+        ad.type = STATIC
+        ad.position = Pos3(x0, height, z0)
+        anchor = api.m3CreateBody(world, C.byref(ad))
+        bodies = []
+        bd = api.m3DefaultBodyDef()
+        bd.type = DYNAMIC
+        for k in range(links):
+            bd.position = Pos3(x0 + 0.5 * float(k) + 0.25, height, z0)
+            body = api.m3CreateBody(world, C.byref(bd))
+            _check(api.m3CreateCapsuleShape(body, C.byref(sd), C.byref(capsule)), "capsule")
+            bodies.append(body)
+        jd = api.m3DefaultJointDef()
+        jd.enableCone = True
+        jd.coneAngle = 0.7853982
+        jd.localAxisA = Vec3(1.0, 0.0, 0.0)
+        jd.localAxisB = Vec3(1.0, 0.0, 0.0)
+        for k in range(links):
+            jd.bodyA = anchor if k == 0 else bodies[k - 1]
+            jd.bodyB = bodies[k]
+            jd.localAnchorA = Vec3(0.0, 0.0, 0.0) if k == 0 else Vec3(0.25, 0.0, 0.0)
+            jd.localAnchorB = Vec3(-0.25, 0.0, 0.0)
+            if api.m3CreateJoint(C.byref(jd)).index1 == 0:
+                raise RuntimeError("joint refused")
+    rng = SplitMix64(seed)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    for _ in range(boxes):
+        x = rng.range(-55.0, 55.0)
+        y = rng.range(20.0, 40.0)
+        z = rng.range(-40.0, 40.0)
+        qx, qy, qz, qw = rng.unit_quat()
+        bd.position = Pos3(x, y, z)
+        bd.rotation = Quat(qx, qy, qz, qw)
+        body = api.m3CreateBody(world, C.byref(bd))
+        half = Vec3(rng.range(0.25, 0.75), rng.range(0.25, 0.75), rng.range(0.25, 0.75))
+        _check(api.m3CreateBoxShape(body, C.byref(sd), half), "box")
+    return world
+
+
 SCENES = {
     "bench_pyramid": bench_pyramid,
     "smoke10k": smoke10k,
     "box_pyramid": box_pyramid,
     "rain": rain,
+    "chains": chains,
 }

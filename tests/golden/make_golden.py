@@ -30,6 +30,7 @@ PLAN = {
     "box_pyramid": (300, 1),
     "smoke10k": (150, 10),
     "rain": (600, 10),
+    "chains": (600, 10),
 }
 
 
@@ -91,6 +92,18 @@ def batch_rollback_fixture(ref: api.Api, worlds: int, warm: int, resim: int) -> 
 
 def main() -> None:
     ref = api.Api(C.CDLL(REF))
+    if len(sys.argv) > 2 and sys.argv[1] == "scene":
+        # Refresh one scene's entry in hashes.json.
+        path = os.path.join(HERE, "hashes.json")
+        with open(path) as f:
+            data = json.load(f)
+        name = sys.argv[2]
+        steps, every = PLAN[name]
+        data["scenes"][name] = run_scene(ref, name, steps, every)
+        with open(path, "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        print("updated", name)
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "batch":
         data = batch_rollback_fixture(ref, 256, 100, 8)
         with open(os.path.join(HERE, "batch_rollback.json"), "w") as f:

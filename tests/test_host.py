@@ -130,3 +130,78 @@ def test_hash64_is_fnv1a():
     assert gpu.m3Hash64(init, b"", 0) == 0xCBF29CE484222325
     assert gpu.m3Hash64(init, b"a", 1) == 0xAF63DC4C8601EC8C
     assert gpu.m3Hash64(init, b"foobar", 6) == 0x85944171F73967E8
+
+
+def test_joint_creation_matches_reference():
+    """Joint frames (quaternion from the axis basis), the joint lists, the
+    wake-on-create rule and the hash's joint block agree before stepping;
+    the default def is byte-identical, cookie included."""
+    ref = ref_api()
+    gpu = product_api()
+    assert bytes(ref.m3DefaultJointDef()) == bytes(gpu.m3DefaultJointDef())
+    for kw in (dict(count=4, links=6, boxes=0), dict(count=30, links=20, boxes=200)):
+        wr = scenes.chains(ref, **kw)
+        wg = scenes.chains(gpu, **kw)
+        assert hexh(gpu.m3World_Hash(wg)) == hexh(ref.m3World_Hash(wr))
+        ref.m3DestroyWorld(wr)
+        gpu.m3DestroyWorld(wg)
+
+
+def test_joint_refusals_and_destroy():
+    ref = ref_api()
+    gpu = product_api()
+    for lib in (ref, gpu):
+        w = scenes.box_pyramid(lib, rows=3)
+        sbd = lib.m3DefaultBodyDef()
+        static = lib.m3CreateBody(w, C.byref(sbd))
+        static2 = lib.m3CreateBody(w, C.byref(sbd))
+        dbd = lib.m3DefaultBodyDef()
+        dbd.type = api.DYNAMIC
+        dbd.position = api.Pos3(0.0, 5.0, 0.0)
+        dyn = lib.m3CreateBody(w, C.byref(dbd))
+        jd = lib.m3DefaultJointDef()
+        jd.bodyA, jd.bodyB = static, static2
+        assert lib.m3CreateJoint(C.byref(jd)).index1 == 0  # two immovables
+        jd.bodyA, jd.bodyB = dyn, dyn
+        assert lib.m3CreateJoint(C.byref(jd)).index1 == 0  # same body
+        jd.bodyA, jd.bodyB = static, dyn
+        bad = lib.m3DefaultJointDef()
+        bad.bodyA, bad.bodyB = static, dyn
+        bad.internalValue ^= 4
+        assert lib.m3CreateJoint(C.byref(bad)).index1 == 0  # cookie
+        bad = lib.m3DefaultJointDef()
+        bad.bodyA, bad.bodyB = static, dyn
+        bad.enableCone = True
+        bad.coneAngle = -0.5
+        assert lib.m3CreateJoint(C.byref(bad)).index1 == 0  # a negative cone
+        j = lib.m3CreateJoint(C.byref(jd))
+        assert j.index1 != 0 and lib.m3Joint_IsValid(j)
+        lib.m3DestroyJoint(j)
+        assert not lib.m3Joint_IsValid(j)
+        j2 = lib.m3CreateJoint(C.byref(jd))
+        assert j2.index1 == j.index1 and j2.generation == j.generation + 1
+        lib.m3DestroyWorld(w)
+    # Kinds other than spherical are refused on this build.
+    w = scenes.box_pyramid(gpu, rows=2)
+    s = gpu.m3CreateBody(w, C.byref(gpu.m3DefaultBodyDef()))
+    dbd = gpu.m3DefaultBodyDef()
+    dbd.type = api.DYNAMIC
+    d = gpu.m3CreateBody(w, C.byref(dbd))
+    jd = gpu.m3DefaultJointDef()
+    jd.bodyA, jd.bodyB = s, d
+    jd.type = 3  # fixed
+    assert gpu.m3CreateJoint(C.byref(jd)).index1 == 0
+    gpu.m3DestroyWorld(w)
+
+
+def test_joint_world_snapshot_roundtrip():
+    gpu = product_api()
+    w = scenes.chains(gpu, count=10, links=10, boxes=20)
+    h0 = gpu.m3World_Hash(w)
+    snap = gpu.snapshot(w)
+    w2 = scenes.chains(gpu, count=10, links=10, boxes=20)
+    jd = gpu.m3DefaultJointDef()
+    assert gpu.restore(w2, snap)
+    assert gpu.m3World_Hash(w2) == h0
+    gpu.m3DestroyWorld(w)
+    gpu.m3DestroyWorld(w2)
