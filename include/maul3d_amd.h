@@ -281,6 +281,9 @@ M3_API m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def,
 M3_API m3ShapeId m3CreateCapsuleShape(m3BodyId bodyId, const m3ShapeDef* def,
                                       const m3Capsule* capsule); // shape.h:112
 
+M3_API m3ShapeId m3CreateHullShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Vec3* points,
+                                   int32_t count); // shape.h:148, 4..256 points
+
 M3_API m3JointDef m3DefaultJointDef(void);                 // joint.h:168
 M3_API m3JointId m3CreateJoint(const m3JointDef* def);    // joint.h:174 (spherical only)
 M3_API void m3DestroyJoint(m3JointId jointId);            // joint.h:175

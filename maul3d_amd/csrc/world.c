@@ -14,6 +14,7 @@
 
 #include "maul3d_amd.h"
 #include "m3gpu.h"
+#include "m3_hull.h"
 
 #include <stddef.h>
 #include <stdio.h>
@@ -1019,65 +1020,9 @@ void m3Body_ApplyForce(m3BodyId bodyId, m3Vec3 force) { ApplyForceLike(bodyId, f
 void m3Body_ApplyTorque(m3BodyId bodyId, m3Vec3 torque) { ApplyForceLike(bodyId, torque, 1); }
 
 // ---------------------------------------------------------------------------
-// Hulls: the canonical box and its half-edge adjacency (src/hull.c:14-139)
+// Hulls: the canonical box (src/hull.c:14-75); the half-edge adjacency and
+// point-cloud hulls are in hull_build.c
 // ---------------------------------------------------------------------------
-
-static void HullBuildHalfEdges(m3HullData* hull)
-{
-    int32_t count = 0;
-    int32_t dirKey[M3_HULL_MAX_HALF_EDGES];
-    int32_t dirSlot[M3_HULL_MAX_HALF_EDGES];
-    int32_t dirCount = 0;
-    int32_t faceEdgeSlot[M3_HULL_MAX_FACE_INDICES];
-    for (int32_t f = 0; f < hull->faceCount; ++f)
-    {
-        int32_t n = hull->faceVertCounts[f];
-        int32_t start = hull->faceVertStart[f];
-        for (int32_t k = 0; k < n; ++k)
-        {
-            int32_t a = hull->faceIndices[start + k];
-            int32_t b = hull->faceIndices[start + (k + 1) % n];
-            int32_t reverseKey = b * 256 + a;
-            int32_t slot = -1;
-            for (int32_t d = 0; d < dirCount; ++d)
-            {
-                if (dirKey[d] == reverseKey)
-                {
-                    slot = dirSlot[d] ^ 1;
-                    break;
-                }
-            }
-            if (slot < 0)
-            {
-                slot = count;
-                count += 2;
-                dirKey[dirCount] = a * 256 + b;
-                dirSlot[dirCount] = slot;
-                dirCount += 1;
-            }
-            hull->edges[slot].origin = (uint8_t)a;
-            hull->edges[slot].twin = (uint16_t)(slot ^ 1);
-            hull->edges[slot].face = (uint8_t)f;
-            faceEdgeSlot[start + k] = slot;
-        }
-    }
-    for (int32_t f = 0; f < hull->faceCount; ++f)
-    {
-        int32_t n = hull->faceVertCounts[f];
-        int32_t start = hull->faceVertStart[f];
-        for (int32_t k = 0; k < n; ++k)
-        {
-            hull->edges[faceEdgeSlot[start + k]].next = (uint16_t)faceEdgeSlot[start + (k + 1) % n];
-        }
-    }
-    hull->edgeCount = count;
-    m3Vec3 c = m3V3(0.0f, 0.0f, 0.0f);
-    for (int32_t v = 0; v < hull->vertexCount; ++v)
-    {
-        c = m3Add3(c, hull->vertices[v]);
-    }
-    hull->center = m3MulSV3(1.0f / (m3real)hull->vertexCount, c);
-}
 
 static void BuildBoxHull(m3HullData* out, m3Vec3 he)
 {
@@ -1121,7 +1066,7 @@ static void BuildBoxHull(m3HullData* out, m3Vec3 he)
     out->unitInertiaCom.cx.x = (m / 3.0f) * (hy * hy + hz * hz);
     out->unitInertiaCom.cy.y = (m / 3.0f) * (hx * hx + hz * hz);
     out->unitInertiaCom.cz.z = (m / 3.0f) * (hx * hx + hy * hy);
-    HullBuildHalfEdges(out);
+    m3HostHullHalfEdges(out);
 }
 
 // Content dedupe in ascending slot order (src/hull.c:141-163).
@@ -1458,7 +1403,7 @@ m3ShapeDef m3DefaultShapeDef(void)
 }
 
 static m3ShapeId CreateShape(m3BodyId bodyId, const m3ShapeDef* def, uint8_t type,
-                             const m3ShapeGeom* geom)
+                             const m3ShapeGeom* geom, const m3HullData* hull)
 {
     m3ShapeId nullId = {0, 0, 0};
     m3World* w = WorldFromIndex0(bodyId.world0);
@@ -1516,7 +1461,14 @@ static m3ShapeId CreateShape(m3BodyId bodyId, const m3ShapeDef* def, uint8_t typ
     if (type == M3_HULL)
     {
         m3HullData data;
-        BuildBoxHull(&data, geom->v);
+        if (hull != NULL)
+        {
+            data = *hull;
+        }
+        else
+        {
+            BuildBoxHull(&data, geom->v);
+        }
         w->shapeHullIndex[s] = InternHull(w, &data);
         if (w->shapeHullIndex[s] < 0)
         {
@@ -1550,7 +1502,7 @@ m3ShapeId m3CreateSphereShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Sp
     memset(&geom, 0, sizeof(geom));
     geom.v = sphere->center;
     geom.s = sphere->radius;
-    return CreateShape(bodyId, def, M3_SPHERE, &geom);
+    return CreateShape(bodyId, def, M3_SPHERE, &geom, NULL);
 }
 
 m3ShapeId m3CreatePlaneShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Plane* plane)
@@ -1571,7 +1523,7 @@ m3ShapeId m3CreatePlaneShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Pla
     memset(&geom, 0, sizeof(geom));
     geom.v = m3Normalize3(plane->normal);
     geom.s = plane->offset;
-    return CreateShape(bodyId, def, M3_PLANE, &geom);
+    return CreateShape(bodyId, def, M3_PLANE, &geom, NULL);
 }
 
 m3ShapeId m3CreateCapsuleShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Capsule* capsule)
@@ -1592,7 +1544,7 @@ m3ShapeId m3CreateCapsuleShape(m3BodyId bodyId, const m3ShapeDef* def, const m3C
     geom.v = capsule->point1;
     geom.s = capsule->radius;
     geom.v2 = capsule->point2;
-    return CreateShape(bodyId, def, M3_CAPSULE, &geom);
+    return CreateShape(bodyId, def, M3_CAPSULE, &geom, NULL);
 }
 
 m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def, m3Vec3 halfExtents)
@@ -1606,12 +1558,43 @@ m3ShapeId m3CreateBoxShape(m3BodyId bodyId, const m3ShapeDef* def, m3Vec3 halfEx
     m3ShapeGeom geom;
     memset(&geom, 0, sizeof(geom));
     geom.v = halfExtents;
-    return CreateShape(bodyId, def, M3_HULL, &geom);
+    return CreateShape(bodyId, def, M3_HULL, &geom, NULL);
 }
 
 // ---------------------------------------------------------------------------
 // Step (src/solver.c:4322-4330 wall; the body runs on the device)
 // ---------------------------------------------------------------------------
+
+// A convex hull of 4..256 points (src/shape.c:988-1036): QuickHull output,
+// interned like every hull; degenerate clouds and clouds past the hull caps
+// give the null id.
+m3ShapeId m3CreateHullShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Vec3* points,
+                            int32_t count)
+{
+    m3ShapeId nullId = {0, 0, 0};
+    m3World* w = WorldFromIndex0(bodyId.world0);
+    int32_t b = w != NULL ? BodySlot(w, bodyId) : -1;
+    if (b < 0 || def == NULL || def->internalValue != M3_SHAPE_COOKIE || points == NULL ||
+        count <= 0)
+    {
+        return nullId;
+    }
+    for (int32_t i = 0; i < count; ++i)
+    {
+        if (!FiniteV3(points[i]))
+        {
+            return nullId;
+        }
+    }
+    m3HullData data;
+    if (!m3HostComputeHull(points, count, &data))
+    {
+        return nullId;
+    }
+    m3ShapeGeom geom;
+    memset(&geom, 0, sizeof(geom));
+    return CreateShape(bodyId, def, M3_HULL, &geom, &data);
+}
 
 // ---------------------------------------------------------------------------
 // Joints (src/joint.c:15-520): the spherical joint with its cone and twist

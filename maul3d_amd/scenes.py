@@ -314,32 +314,7 @@ def batch_world(api: Api, index: int, seed: int = 0xB47C4) -> WorldId:
     return world
 
 
-def chains(
-    api: Api,
-    count: int = 500,
-    links: int = 20,
-    boxes: int = 4000,
-    seed: int = 0xC4A1,
-    height: float = 14.0,
-) -> WorldId:
-    """Capsule chains on spherical joints with 45-degree cone limits, plus
-    optional falling boxes.
-
-    Chain c hangs from a static anchor body at (12 (c % 10) - 54, height,
-    1.5 (c // 10) - 37.5).  Its ``links`` capsules (radius 0.1, segment from
-    -0.25 to 0.25 along local x, density 1000, friction 0.6) start
-    horizontal along +x, link k centred 0.5 k + 0.25 from the anchor.  Joint
-    k pins the previous body (the anchor for k = 0) to link k at their shared
-    point; both frame z-axes are local +x (the chain direction), the cone
-    half-angle is 0.7853982 rad and jointed bodies do not collide.  Creation
-    order: the plane, then per chain the anchor, its links, then its joints.
-    ``boxes`` boxes (half-extents U[0.25,0.75]) then fall from
-    U[-55,55] x U[20,40] x U[-40,40] with uniform rotations (stream
-    SplitMix64(seed)).
-    """
-    n_bodies = count * (links + 1) + boxes + 1
-    world = _world(api, n_bodies + 8, n_bodies + 8, joints=count * links + 8)
-    _ground(api, world)
+def _add_chains(api: Api, world: WorldId, count: int, links: int, height: float) -> None:
     sd = api.m3DefaultShapeDef()
     sd.density = 1000.0
     sd.friction = 0.6
@@ -371,6 +346,38 @@ def chains(
             jd.localAnchorB = Vec3(-0.25, 0.0, 0.0)
             if api.m3CreateJoint(C.byref(jd)).index1 == 0:
                 raise RuntimeError("joint refused")
+
+
+def chains(
+    api: Api,
+    count: int = 500,
+    links: int = 20,
+    boxes: int = 4000,
+    seed: int = 0xC4A1,
+    height: float = 14.0,
+) -> WorldId:
+    """Capsule chains on spherical joints with 45-degree cone limits, plus
+    optional falling boxes.
+
+    Chain c hangs from a static anchor body at (12 (c % 10) - 54, height,
+    1.5 (c // 10) - 37.5).  Its ``links`` capsules (radius 0.1, segment from
+    -0.25 to 0.25 along local x, density 1000, friction 0.6) start
+    horizontal along +x, link k centred 0.5 k + 0.25 from the anchor.  Joint
+    k pins the previous body (the anchor for k = 0) to link k at their shared
+    point; both frame z-axes are local +x (the chain direction), the cone
+    half-angle is 0.7853982 rad and jointed bodies do not collide.  Creation
+    order: the plane, then per chain the anchor, its links, then its joints.
+    ``boxes`` boxes (half-extents U[0.25,0.75]) then fall from
+    U[-55,55] x U[20,40] x U[-40,40] with uniform rotations (stream
+    SplitMix64(seed)).
+    """
+    n_bodies = count * (links + 1) + boxes + 1
+    world = _world(api, n_bodies + 8, n_bodies + 8, joints=count * links + 8)
+    _ground(api, world)
+    _add_chains(api, world, count, links, height)
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
     rng = SplitMix64(seed)
     bd = api.m3DefaultBodyDef()
     bd.type = DYNAMIC
@@ -387,10 +394,101 @@ def chains(
     return world
 
 
+def ellipsoid_cloud(rng: SplitMix64) -> list[tuple[float, float, float]]:
+    """k = 8 + floor(U[0,57)) points on an ellipsoid with semi-axes U[0.3,1.2]:
+    directions by rejection from the unit cube (|d|^2 in (1e-4, 1]),
+    normalized in double, scaled per axis."""
+    k = 8 + int(rng.range(0.0, 57.0))
+    a = rng.range(0.3, 1.2)
+    b = rng.range(0.3, 1.2)
+    c = rng.range(0.3, 1.2)
+    pts = []
+    for _ in range(k):
+        while True:
+            x = rng.range(-1.0, 1.0)
+            y = rng.range(-1.0, 1.0)
+            z = rng.range(-1.0, 1.0)
+            r2 = x * x + y * y + z * z
+            if 1.0e-4 < r2 <= 1.0:
+                break
+        inv = 1.0 / math.sqrt(r2)
+        pts.append((a * x * inv, b * y * inv, c * z * inv))
+    return pts
+
+
+def hull_chains(
+    api: Api,
+    hulls: int = 4000,
+    count: int = 500,
+    links: int = 20,
+    seed: int = 0x4C4,
+) -> WorldId:
+    """Hull rain plus joint chains: ``count`` chains exactly as in
+    :func:`chains` (no boxes), then ``hulls`` convex hulls built by
+    m3CreateHullShape from :func:`ellipsoid_cloud` point sets (stream
+    SplitMix64(seed)), dropped with uniform rotations from
+    U[-50,50] x U[16,66] x U[-50,50] (stream SplitMix64(seed ^ 0xD40F)),
+    density 1000, friction 0.6.  Every cloud of the default seed is one the
+    reference's QuickHull builds inside its fixed work pools.
+    """
+    n_bodies = count * (links + 1) + hulls + 1
+    world = _world(api, n_bodies + 8, n_bodies + 8, joints=count * links + 8)
+    _ground(api, world)
+    _add_chains(api, world, count, links, 14.0)
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    clouds = SplitMix64(seed)
+    place = SplitMix64(seed ^ 0xD40F)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    for _ in range(hulls):
+        pts = ellipsoid_cloud(clouds)
+        arr = (Vec3 * len(pts))(*[Vec3(*p) for p in pts])
+        x = place.range(-50.0, 50.0)
+        y = place.range(16.0, 66.0)
+        z = place.range(-50.0, 50.0)
+        qx, qy, qz, qw = place.unit_quat()
+        bd.position = Pos3(x, y, z)
+        bd.rotation = Quat(qx, qy, qz, qw)
+        body = api.m3CreateBody(world, C.byref(bd))
+        _check(api.m3CreateHullShape(body, C.byref(sd), arr, len(pts)), "hull")
+    return world
+
+
+def hull_pile(api: Api, hulls: int = 200, seed: int = 0x4C4) -> WorldId:
+    """A dense hull column: the first ``hulls`` clouds of the hull-rain cloud
+    stream SplitMix64(seed), hull i dropped at U[-4,4] x (2 + 1.2 i) x
+    U[-4,4] with a uniform rotation (stream SplitMix64(seed ^ 0x9111)),
+    density 1000, friction 0.6.  Hull-on-hull manifolds dominate."""
+    world = _world(api, hulls + 8, hulls + 8)
+    _ground(api, world)
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    clouds = SplitMix64(seed)
+    place = SplitMix64(seed ^ 0x9111)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    for i in range(hulls):
+        pts = ellipsoid_cloud(clouds)
+        arr = (Vec3 * len(pts))(*[Vec3(*p) for p in pts])
+        x = place.range(-4.0, 4.0)
+        z = place.range(-4.0, 4.0)
+        qx, qy, qz, qw = place.unit_quat()
+        bd.position = Pos3(x, 2.0 + 1.2 * i, z)
+        bd.rotation = Quat(qx, qy, qz, qw)
+        body = api.m3CreateBody(world, C.byref(bd))
+        _check(api.m3CreateHullShape(body, C.byref(sd), arr, len(pts)), "hull")
+    return world
+
+
 SCENES = {
     "bench_pyramid": bench_pyramid,
     "smoke10k": smoke10k,
     "box_pyramid": box_pyramid,
     "rain": rain,
     "chains": chains,
+    "hull_chains": hull_chains,
+    "hull_pile": hull_pile,
 }

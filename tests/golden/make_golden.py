@@ -31,6 +31,7 @@ PLAN = {
     "smoke10k": (150, 10),
     "rain": (600, 10),
     "chains": (600, 10),
+    "hull_chains": (600, 10),
 }
 
 
