@@ -41,6 +41,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-batch", action="store_true", help="skip the batched-rollback measurement")
     p.add_argument("--batch-worlds", type=int, default=256)
     p.add_argument("--batch-only", action="store_true", help="run only the batched-rollback measurement")
+    p.add_argument("--no-hull-chains", action="store_true", help="skip the hull-rain-with-chains measurement")
+    p.add_argument(
+        "--hull-chains-only", action="store_true", help="run only the hull-rain-with-chains measurement"
+    )
     p.add_argument(
         "--copy-calibration",
         action="store_true",
@@ -197,6 +201,42 @@ def run_batched(worlds: int) -> dict:
     }
 
 
+def run_hull_chains(steps: int) -> dict:
+    """Hull rain plus joint chains: 4000 QuickHull hulls and 500 chains of 20
+    capsule links on spherical joints with 45-degree cones (14,000 dynamic
+    bodies, 10,000 joints), one world, ``steps`` steps timed with the world
+    resident; the final hash is checked against the reference fixture."""
+    import maul3d_amd
+    from maul3d_amd import scenes
+
+    gpu = maul3d_amd.get_api()
+    lib = maul3d_amd.library()
+    world = scenes.hull_chains(gpu)
+    dyn = gpu.m3World_GetCounters(world).bodyCount - 501  # ground and chain anchors are static
+    lib.m3World_Sync(world)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        gpu.m3World_Step(world, DT, SUBSTEPS)
+    lib.m3World_Sync(world)
+    wall = time.perf_counter() - t0
+    h = "%016x" % gpu.m3World_Hash(world)
+    gpu.m3DestroyWorld(world)
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "hashes.json")) as f:
+            want = json.load(f)["scenes"]["hull_chains"]["hashes"].get(str(steps))
+    except OSError:
+        want = None
+    return {
+        "workload": f"hull rain + chains: 4000 hulls, 500 chains x 20 links, {dyn} dynamic bodies, "
+        f"10,000 spherical joints; {steps} steps",
+        "value": dyn * SUBSTEPS * steps / wall,
+        "unit": "body-substeps/s",
+        "ms_per_step": 1000.0 * wall / steps,
+        "world_hash": h,
+        "hash_bit_exact": (h == want) if want else None,
+    }
+
+
 def copy_calibration() -> None:
     """A plain elementwise copy of known size, the scale for FETCH_SIZE and
     WRITE_SIZE counter readings of the step kernels."""
@@ -225,6 +265,9 @@ def main() -> None:
         sys.exit(2)
     if args.batch_only:
         print(json.dumps({"batched": run_batched(args.batch_worlds)}))
+        return
+    if args.hull_chains_only:
+        print(json.dumps({"hull_chains": run_hull_chains(args.steps)}))
         return
     res = run_gpu(args.steps, args.warmup)
     value = BODIES * SUBSTEPS * args.steps / res["wall"]
@@ -262,6 +305,8 @@ def main() -> None:
             line["kernels"] = prof["kernels"]
     if not args.no_batch:
         line["batched"] = run_batched(args.batch_worlds)
+    if not args.no_hull_chains:
+        line["hull_chains"] = run_hull_chains(600)
     if not args.no_cpu:
         line["cpu_baseline"] = run_cpu(args.cpu_steps)
     print(json.dumps(line))

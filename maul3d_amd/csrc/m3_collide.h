@@ -195,23 +195,44 @@ M3_HD void m3SegmentClosest(m3Vec3 p1, m3Vec3 d1, m3Vec3 p2, m3Vec3 d2, m3Vec3* 
     *c2 = m3Add3(p2, m3MulSV3(t, d2));
 }
 
+// The three separating-axis queries of a hull pair, computed ahead of the
+// manifold by a wavefront when the pair is wide (m3WideHullPair).
+typedef struct m3SatQuery
+{
+    m3FaceQuery faceA;
+    m3FaceQuery faceB;
+    m3EdgeQuery edge;
+} m3SatQuery;
+
+// Pairs whose edge query is larger than box against box (12 x 12 edge
+// pairs) take the wavefront query: its cost grows with the product of the
+// edge counts (up to 186 x 186 for two 64-vertex hulls).
+M3_HD int m3WideHullPair(const m3HullData* hullA, const m3HullData* hullB)
+{
+    return (hullA->edgeCount >> 1) * (hullB->edgeCount >> 1) > 144;
+}
+
 // Hull versus hull in A's frame; B arrives through (q, p).  Face queries
 // both ways, the edge query, then an edge contact or a Sutherland-Hodgman
-// clip of the incident face with the deepest-four reduction.
-M3_HD m3Manifold m3CollideHulls(const m3HullData* hullA, const m3HullData* hullB, m3Quat q, m3Vec3 p)
+// clip of the incident face with the deepest-four reduction.  `sat`, when
+// given, holds the three queries already evaluated (they are pure
+// functions of the hulls and (q, p), so evaluating them ahead and all three
+// changes nothing but the work done).
+M3_HD m3Manifold m3CollideHullsQ(const m3HullData* hullA, const m3HullData* hullB, m3Quat q,
+                                 m3Vec3 p, const m3SatQuery* sat)
 {
     m3Manifold manifold = m3ZeroManifold();
-    m3FaceQuery faceA = m3QueryFaces(hullA, hullB, q, p, 1);
+    m3FaceQuery faceA = sat != NULL ? sat->faceA : m3QueryFaces(hullA, hullB, q, p, 1);
     if (faceA.separation > M3_SPECULATIVE_DISTANCE)
     {
         return manifold;
     }
-    m3FaceQuery faceB = m3QueryFaces(hullB, hullA, q, p, 0);
+    m3FaceQuery faceB = sat != NULL ? sat->faceB : m3QueryFaces(hullB, hullA, q, p, 0);
     if (faceB.separation > M3_SPECULATIVE_DISTANCE)
     {
         return manifold;
     }
-    m3EdgeQuery edge = m3QueryEdges(hullA, hullB, q, p);
+    m3EdgeQuery edge = sat != NULL ? sat->edge : m3QueryEdges(hullA, hullB, q, p);
     if (edge.indexA >= 0 && edge.separation > M3_SPECULATIVE_DISTANCE)
     {
         return manifold;
@@ -383,6 +404,11 @@ M3_HD m3Manifold m3CollideHulls(const m3HullData* hullA, const m3HullData* hullB
                                            (polyId[candIdx[c]] & 0xFFu));
     }
     return manifold;
+}
+
+M3_HD m3Manifold m3CollideHulls(const m3HullData* hullA, const m3HullData* hullB, m3Quat q, m3Vec3 p)
+{
+    return m3CollideHullsQ(hullA, hullB, q, p, NULL);
 }
 
 // ---- point core inside a hull (src/manifold.c:503-520) ---------------------

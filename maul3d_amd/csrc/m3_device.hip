@@ -84,6 +84,7 @@ struct m3gpuWorld
     struct m3Profiler* prof;
     int coopBlocks;  // co-resident blocks of the substep kernel
     int coopThreads; // threads per block of the substep kernel
+    int wideHulls;   // some hull is larger than a box (wide hull pairs possible)
 };
 
 // The entry is read through the constant address space: its fields load as
@@ -223,6 +224,7 @@ __global__ void k_begin(const m3Dev* __restrict__ ds)
     sc->discoverCount = 0;
     sc->pointCount = 0;
     sc->jointActive = 0;
+    sc->satCount = 0;
 }
 
 __global__ void k_fill_emit(const m3Dev* __restrict__ ds)
@@ -592,6 +594,315 @@ __global__ void k_stall_restore(const m3Dev* __restrict__ ds)
 // ---------------------------------------------------------------------------
 // Narrowphase (src/manifold.c:2357-2944)
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Wide hull pairs: the separating-axis queries one wavefront per pair
+// ---------------------------------------------------------------------------
+//
+// A hull pair's face queries cost faces x vertices and its edge query
+// (edges of A) x (edges of B) tests, up to 186 x 186 for two 64-vertex hulls:
+// milliseconds for one thread.  Pairs above box-against-box size are listed
+// first, then each listed pair's three queries are evaluated by a 64-lane
+// workgroup: lanes split the faces and the edge pairs, and a reduction
+// picks what the serial scan with its early exits would return:
+//   * the first candidate (scan order) whose separation exceeds the
+//     speculative distance, where the scan returns at once, else
+//   * the first candidate attaining the largest separation (updates are
+//     strict), else the scan's initial value.
+// A face's inner scan may stop early in the serial form only when its value
+// could no longer be selected, so the full minimum over the vertices selects
+// identically.  Every candidate value comes from the same operations as the
+// serial scan, so the selected separation and axis are bit-identical.
+
+__global__ void k_sat_list(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || i >= v.sc->pairCount)
+    {
+        return;
+    }
+    uint64_t key = v.keys[i];
+    int32_t shapeA = (int32_t)(key >> 32);
+    int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+    if (v.shapeType[shapeA] != M3_HULL || v.shapeType[shapeB] != M3_HULL)
+    {
+        return;
+    }
+    if (m3WideHullPair(&v.hulls[v.hullIndex[shapeA]], &v.hulls[v.hullIndex[shapeB]]))
+    {
+        v.satList[atomicAdd(&v.sc->satCount, 1)] = i;
+    }
+}
+
+#define M3_HULL_MAX_EDGES (M3_HULL_MAX_HALF_EDGES / 2)
+
+struct SatLds
+{
+    m3Vec3 w[M3_HULL_MAX_VERTS]; // the other hull's vertices in the reference hull's frame
+    m3Vec3 aP[M3_HULL_MAX_EDGES]; // A's edges: origin, direction, the two face normals
+    m3Vec3 aE[M3_HULL_MAX_EDGES];
+    m3Vec3 aU[M3_HULL_MAX_EDGES];
+    m3Vec3 aV[M3_HULL_MAX_EDGES];
+    m3Vec3 bP[M3_HULL_MAX_EDGES]; // B's edges, moved into A's frame
+    m3Vec3 bE[M3_HULL_MAX_EDGES];
+    m3Vec3 bU[M3_HULL_MAX_EDGES];
+    m3Vec3 bV[M3_HULL_MAX_EDGES];
+};
+
+// (separation, index) of the serial scan's choice among this lane's and the
+// other lanes' candidates: first exceeding index if any, else first maximum.
+struct SatPick
+{
+    int32_t hit;  // smallest candidate index above the speculative distance
+    m3real hitSep;
+    int32_t best; // first candidate index of the largest separation
+    m3real bestSep;
+};
+
+__device__ inline void SatOffer(SatPick& s, int32_t idx, m3real sep)
+{
+    // Candidates arrive in ascending index per lane.
+    if (sep > M3_SPECULATIVE_DISTANCE && s.hit == INT32_MAX)
+    {
+        s.hit = idx;
+        s.hitSep = sep;
+    }
+    if (sep > s.bestSep)
+    {
+        s.bestSep = sep;
+        s.best = idx;
+    }
+}
+
+__device__ inline SatPick SatReduce(SatPick s)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        int32_t oHit = __shfl_xor(s.hit, off, 64);
+        m3real oHitSep = __shfl_xor(s.hitSep, off, 64);
+        int32_t oBest = __shfl_xor(s.best, off, 64);
+        m3real oBestSep = __shfl_xor(s.bestSep, off, 64);
+        if (oHit < s.hit)
+        {
+            s.hit = oHit;
+            s.hitSep = oHitSep;
+        }
+        if (oBestSep > s.bestSep || (oBestSep == s.bestSep && oBest < s.best))
+        {
+            s.bestSep = oBestSep;
+            s.best = oBest;
+        }
+    }
+    return s;
+}
+
+__device__ inline SatPick SatPickInit()
+{
+    SatPick s;
+    s.hit = INT32_MAX;
+    s.hitSep = 0.0f;
+    s.best = INT32_MAX;
+    s.bestSep = -3.4e38f;
+    return s;
+}
+
+// m3QueryFaces over the lanes (same values, same choice).
+__device__ m3FaceQuery WaveQueryFaces(const m3HullData* ref, const m3HullData* other, m3Quat q,
+                                      m3Vec3 p, int refIsA, SatLds& L, int lane)
+{
+    int32_t nv = other->vertexCount;
+    for (int32_t k = lane; k < nv; k += 64)
+    {
+        L.w[k] = refIsA ? m3Add3(m3RotateVec3(q, other->vertices[k]), p)
+                        : m3InvRotateVec3(q, m3Sub3(other->vertices[k], p));
+    }
+    __syncthreads();
+    SatPick s = SatPickInit();
+    int32_t nf = ref->faceCount;
+    for (int32_t f = lane; f < nf; f += 64)
+    {
+        m3Vec3 n = ref->faceNormals[f];
+        m3real off = ref->faceOffsets[f];
+        m3real best = 3.4e38f;
+        for (int32_t k = 0; k < nv; ++k)
+        {
+            best = m3MinF(best, m3Dot3(n, L.w[k]) - off);
+        }
+        SatOffer(s, f, best);
+    }
+    s = SatReduce(s);
+    __syncthreads();
+    m3FaceQuery out;
+    if (s.hit != INT32_MAX)
+    {
+        out.separation = s.hitSep;
+        out.faceIndex = s.hit;
+    }
+    else if (s.best != INT32_MAX)
+    {
+        out.separation = s.bestSep;
+        out.faceIndex = s.best;
+    }
+    else
+    {
+        out.separation = -3.4e38f;
+        out.faceIndex = 0;
+    }
+    return out;
+}
+
+// Separation of edge pair (ja, jb) as m3QueryEdges computes it; false when
+// the Gauss-map test or the degenerate-axis test rejects the pair.
+__device__ inline bool SatEdgePair(const SatLds& L, int32_t ja, int32_t jb, m3Vec3 centerA,
+                                   m3real* sepOut, m3Vec3* axisOut)
+{
+    m3Vec3 eA = L.aE[ja];
+    m3Vec3 eB = L.bE[jb];
+    m3real cba = m3Dot3(L.bU[jb], eA);
+    m3real dba = m3Dot3(L.bV[jb], eA);
+    m3real adc = -m3Dot3(L.aU[ja], eB);
+    m3real bdc = -m3Dot3(L.aV[ja], eB);
+    if (!(cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f))
+    {
+        return false;
+    }
+    m3Vec3 axis = m3Cross3(eA, eB);
+    m3real len = m3Length3(axis);
+    if (len < 1.0e-6f)
+    {
+        return false;
+    }
+    axis = m3MulSV3(1.0f / len, axis);
+    m3Vec3 pA = L.aP[ja];
+    if (m3Dot3(axis, m3Sub3(pA, centerA)) < 0.0f)
+    {
+        axis = m3Neg3(axis);
+    }
+    *sepOut = m3Dot3(axis, m3Sub3(L.bP[jb], pA));
+    *axisOut = axis;
+    return true;
+}
+
+// m3QueryEdges over the lanes: candidates in the serial scan order
+// t = (B edge) * (A edges) + (A edge).
+__device__ m3EdgeQuery WaveQueryEdges(const m3HullData* hullA, const m3HullData* hullB, m3Quat q,
+                                      m3Vec3 p, SatLds& L, int lane)
+{
+    int32_t nA = hullA->edgeCount >> 1;
+    int32_t nB = hullB->edgeCount >> 1;
+    for (int32_t k = lane; k < nA; k += 64)
+    {
+        m3HullHalfEdge edgeA = hullA->edges[2 * k];
+        m3HullHalfEdge twinA = hullA->edges[2 * k + 1];
+        m3Vec3 pA = hullA->vertices[edgeA.origin];
+        L.aP[k] = pA;
+        L.aE[k] = m3Sub3(hullA->vertices[twinA.origin], pA);
+        L.aU[k] = hullA->faceNormals[edgeA.face];
+        L.aV[k] = hullA->faceNormals[twinA.face];
+    }
+    for (int32_t k = lane; k < nB; k += 64)
+    {
+        m3HullHalfEdge edgeB = hullB->edges[2 * k];
+        m3HullHalfEdge twinB = hullB->edges[2 * k + 1];
+        m3Vec3 qB = m3Add3(m3RotateVec3(q, hullB->vertices[twinB.origin]), p);
+        m3Vec3 pB = m3Add3(m3RotateVec3(q, hullB->vertices[edgeB.origin]), p);
+        L.bP[k] = pB;
+        L.bE[k] = m3Sub3(qB, pB);
+        L.bU[k] = m3RotateVec3(q, hullB->faceNormals[edgeB.face]);
+        L.bV[k] = m3RotateVec3(q, hullB->faceNormals[twinB.face]);
+    }
+    __syncthreads();
+    m3Vec3 centerA = hullA->center;
+    SatPick s = SatPickInit();
+    int32_t total = nA * nB;
+    int32_t jb = lane / nA;
+    int32_t ja = lane - jb * nA;
+    for (int32_t t = lane; t < total; t += 64)
+    {
+        m3real sep;
+        m3Vec3 axis;
+        if (SatEdgePair(L, ja, jb, centerA, &sep, &axis))
+        {
+            SatOffer(s, t, sep);
+        }
+        ja += 64;
+        while (ja >= nA)
+        {
+            ja -= nA;
+            jb += 1;
+        }
+    }
+    s = SatReduce(s);
+    m3EdgeQuery out;
+    out.separation = -3.4e38f;
+    out.indexA = -1;
+    out.indexB = -1;
+    out.axis = m3V3(0.0f, 1.0f, 0.0f);
+    int32_t pick = s.hit != INT32_MAX ? s.hit : s.best;
+    if (pick != INT32_MAX)
+    {
+        int32_t pb = pick / nA;
+        int32_t pa = pick - pb * nA;
+        m3real sep;
+        m3Vec3 axis;
+        SatEdgePair(L, pa, pb, centerA, &sep, &axis);
+        out.separation = sep;
+        out.indexA = 2 * pa;
+        out.indexB = 2 * pb;
+        out.axis = axis;
+    }
+    __syncthreads();
+    return out;
+}
+
+// One 64-lane workgroup per listed pair, grid-stride over the list.
+__global__ __launch_bounds__(64) void k_sat(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    __shared__ SatLds L;
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    int lane = threadIdx.x;
+    int32_t count = v.sc->satCount;
+    for (int32_t k = blockIdx.x; k < count; k += gridDim.x)
+    {
+        int32_t i = v.satList[k];
+        uint64_t key = v.keys[i];
+        int32_t shapeA = (int32_t)(key >> 32);
+        int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+        const m3HullData* hullA = &v.hulls[v.hullIndex[shapeA]];
+        const m3HullData* hullB = &v.hulls[v.hullIndex[shapeB]];
+        m3Transform xfA;
+        m3Quat q;
+        m3Vec3 p;
+        m3HullPairFrame(&v, shapeA, shapeB, &xfA, &q, &p);
+        m3SatQuery out;
+        out.faceA = WaveQueryFaces(hullA, hullB, q, p, 1, L, lane);
+        out.faceB.separation = 0.0f;
+        out.faceB.faceIndex = 0;
+        out.edge.separation = 0.0f;
+        out.edge.indexA = -1;
+        out.edge.indexB = -1;
+        out.edge.axis = m3V3(0.0f, 1.0f, 0.0f);
+        // The manifold reads the later queries only when the earlier ones
+        // found no separation.
+        if (!(out.faceA.separation > M3_SPECULATIVE_DISTANCE))
+        {
+            out.faceB = WaveQueryFaces(hullB, hullA, q, p, 0, L, lane);
+            if (!(out.faceB.separation > M3_SPECULATIVE_DISTANCE))
+            {
+                out.edge = WaveQueryEdges(hullA, hullB, q, p, L, lane);
+            }
+        }
+        if (lane == 0)
+        {
+            v.satQueries[i] = out;
+        }
+    }
+}
 
 __global__ void k_narrow(const m3Dev* __restrict__ ds)
 {
@@ -1718,6 +2029,7 @@ static int CreateImpl(m3gpuWorld* w)
     }
     if (DevAlloc(&v.sleepKeys, np) || DevAlloc(&v.emit, v.emitCapacity) ||
         DevAlloc(&w->dev.emitSorted, v.emitCapacity) || DevAlloc(&w->dev.emitUnique, v.emitCapacity) ||
+        DevAlloc(&v.satQueries, np) || DevAlloc(&v.satList, np) ||
         DevAlloc(&v.cons, np) || DevAlloc(&v.consPair, np) || DevAlloc(&v.colors, np) ||
         DevAlloc(&v.colorLists, np) || DevAlloc(&v.movers, nb) || DevAlloc(&v.deltaPos, nb) ||
         DevAlloc(&v.deltaRot, nb) || DevAlloc(&v.com0, nb) || DevAlloc(&v.rot0, nb) ||
@@ -1848,7 +2160,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.hasOffset, v.localPos, v.localRot, v.surfaceVel, v.category, v.mask,
                     v.group, v.shapeNext, v.hullIndex, v.leaf, v.fresh, v.planes,
                     (void*)v.hulls, w->keysBuf[0], w->keysBuf[1], w->mansBuf[0], w->mansBuf[1],
-                    v.sleepKeys, v.emit, w->dev.emitSorted, w->dev.emitUnique, v.cons, v.consPair,
+                    v.sleepKeys, v.emit, v.satQueries, v.satList, w->dev.emitSorted, w->dev.emitUnique, v.cons, v.consPair,
                     v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
                     v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut, w->dev.sapValsIn,
                     w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.root,
@@ -1942,6 +2254,14 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(v.shapeNext, h->shapeNext, ns, int32_t);
         UP(v.hullIndex, h->shapeHullIndex, ns, int32_t);
         UP((void*)v.hulls, h->hullData, h->maxHull, m3HullData);
+        w->wideHulls = 0;
+        for (int32_t k = 0; k < h->maxHull; ++k)
+        {
+            if (((const m3HullData*)h->hullData)[k].edgeCount > 24)
+            {
+                w->wideHulls = 1;
+            }
+        }
         if (h->maxJoint > w->caps.jointCapacity)
         {
             snprintf(s_lastError, sizeof(s_lastError), "m3gpu_upload: joints exceed capacity");
@@ -2121,7 +2441,7 @@ enum m3Stage
 {
     ST_BEGIN, ST_FILL, ST_FRESH, ST_DISCOVER_ALL, ST_REFRESH, ST_SAP_SORT, ST_SAP_GATHER, ST_SAP, ST_PLANES,
     ST_FROZEN, ST_EMIT_SORT, ST_UNIQUE, ST_PAIR_COUNT, ST_EXPAND, ST_STALL_RESTORE,
-    ST_SLEEP_SELECT, ST_NARROW, ST_CAPTURE, ST_UNION, ST_ROOT, ST_WAKE_MARK, ST_WAKE_APPLY,
+    ST_SLEEP_SELECT, ST_SAT_LIST, ST_SAT, ST_NARROW, ST_CAPTURE, ST_UNION, ST_ROOT, ST_WAKE_MARK, ST_WAKE_APPLY,
     ST_CONS_FLAGS, ST_CONS_SELECT, ST_CONS_BODIES, ST_COLOR, ST_PREPARE,
     ST_MOVER_FLAGS, ST_MOVER_SELECT, ST_SUBSTEPS, ST_CLEAR_FORCES, ST_RESTITUTION, ST_STORE, ST_CCD,
     ST_SLEEP_TIMERS, ST_SLEEP_READY, ST_SLEEP_FREEZE, ST_END,
@@ -2131,7 +2451,7 @@ enum m3Stage
 static const char* const kStageNames[] = {
     "begin", "fill_emit", "fresh_bounds", "discover_all", "refresh_proxies", "sap_sort", "sap_gather", "sap_sweep",
     "plane_pairs", "frozen_merge", "pair_sort", "pair_unique", "pair_count", "pair_expand",
-    "stall_restore", "frozen_select", "narrowphase", "capture", "island_union", "island_root",
+    "stall_restore", "frozen_select", "hull_sat_list", "hull_sat", "narrowphase", "capture", "island_union", "island_root",
     "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "constraint_bodies",
     "coloring", "prepare", "mover_flags", "mover_select", "substeps", "clear_forces",
     "restitution", "store_impulses", "continuous", "sleep_timers", "sleep_ready", "sleep_freeze",
@@ -2196,6 +2516,16 @@ static int RefreshDevCopies(m3gpuWorld* w)
     w->layout += 1;
     w->streamBusy = 0;
     return 0;
+}
+
+// Workgroups of the wide-hull SAT pass: one per listed pair up to about
+// eight per CU over all worlds of the launch (the kernel strides over its
+// world's list).
+static int SatBlocks(int64_t pairs, int32_t worlds)
+{
+    int64_t cap = 2048 / (worlds > 0 ? worlds : 1);
+    int64_t b = pairs < cap ? pairs : cap;
+    return b < 1 ? 1 : (int)b;
 }
 
 static m3SubstepParams SubstepParams(const m3gpuStepParams* p)
@@ -2273,6 +2603,11 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
                                                                  v.sleepKeys, &v.sc->sleepCount,
                                                                  np, st)));
     // Narrowphase.
+    if (w->wideHulls)
+    {
+        STAGE(ST_SAT_LIST, KL(k_sat_list, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+        STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
+    }
     STAGE(ST_NARROW, KL(k_narrow, dim3(Blocks(np, 64)), dim3(64), 0, st, D));
     // Sweeps, islands, wake.
     STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
@@ -2429,6 +2764,7 @@ struct m3gpuBatch
     int32_t maxPairs;
     int32_t maxEmit;
     int32_t maxJoint;
+    int wideHulls;
     int64_t maxPlaneWork;
     int threads; // workgroup size of the per-world substep loop
     // The step's launch sequence captured once per parity and replayed.
@@ -2577,6 +2913,7 @@ static int BatchBuild(m3gpuWorld* const* ws, int32_t n)
     }
     s_batch.maxBody = s_batch.maxShape = s_batch.maxPairs = s_batch.maxEmit = 0;
     s_batch.maxJoint = 0;
+    s_batch.wideHulls = 0;
     s_batch.maxPlaneWork = 0;
     for (int32_t k = 0; k < n; ++k)
     {
@@ -2608,6 +2945,7 @@ static int BatchBuild(m3gpuWorld* const* ws, int32_t n)
         s_batch.maxPairs = v.pairCapacity > s_batch.maxPairs ? v.pairCapacity : s_batch.maxPairs;
         s_batch.maxEmit = v.emitCapacity > s_batch.maxEmit ? v.emitCapacity : s_batch.maxEmit;
         s_batch.maxJoint = v.maxJoint > s_batch.maxJoint ? v.maxJoint : s_batch.maxJoint;
+        s_batch.wideHulls |= w->wideHulls;
         int64_t pw = (int64_t)v.maxShape * v.planeCount;
         s_batch.maxPlaneWork = pw > s_batch.maxPlaneWork ? pw : s_batch.maxPlaneWork;
     }
@@ -2649,6 +2987,11 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_expand, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     KL(k_stall_restore, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 0);
+    if (s_batch.wideHulls)
+    {
+        KL(k_sat_list, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+        KL(k_sat, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
+    }
     KL(k_narrow, dim3(Blocks(np, 64), W), dim3(64), 0, st, D);
     KL(k_capture, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_union, dim3(Blocks(np, T), W), dim3(T), 0, st, D);

@@ -123,6 +123,7 @@ typedef struct m3Scalars
     int32_t fault;     // sticky: a substep grid barrier timed out
     int32_t fullQuery; // the next step rebuilds the frozen-pair buffer
     int32_t jointActive; // joints prepared this step
+    int32_t satCount;    // wide hull pairs listed for the wavefront SAT queries
     uint64_t stepCount;
 } m3Scalars;
 
@@ -189,6 +190,8 @@ typedef struct m3View
     const m3Manifold* oldManifolds;
     uint64_t* sleepKeys;
     uint32_t* emit;
+    m3SatQuery* satQueries; // per pair: SAT queries of wide hull pairs, evaluated ahead
+    int32_t* satList;       // pair indices of the wide hull pairs (any order)
     // solver scratch
     m3Constraint* cons;   // prepared constraints in color-slot order
     int32_t* consPair;    // pair index of each constraint (constraint order)
@@ -595,18 +598,29 @@ M3_HD m3Manifold m3CollidePlaneHull(const m3View* v, int32_t shapeA, int32_t pla
     return fresh;
 }
 
-M3_HD m3Manifold m3CollidePairHulls(const m3View* v, int32_t shapeA, int32_t shapeB)
+// B's pose in A's frame, as the hull pair is evaluated (src/manifold.c:2526-2536).
+M3_HD void m3HullPairFrame(const m3View* v, int32_t shapeA, int32_t shapeB, m3Transform* xfA,
+                           m3Quat* qRel, m3Vec3* pRel)
+{
+    *xfA = m3ShapeXf(v, shapeA);
+    m3Transform xfB = m3ShapeXf(v, shapeB);
+    *qRel = m3MulQuat(m3Conj(xfA->q), xfB.q);
+    m3Vec3 dp = m3V3((m3real)(xfB.p.x - xfA->p.x), (m3real)(xfB.p.y - xfA->p.y),
+                     (m3real)(xfB.p.z - xfA->p.z));
+    *pRel = m3InvRotateVec3(xfA->q, dp);
+}
+
+M3_HD m3Manifold m3CollidePairHulls(const m3View* v, int32_t shapeA, int32_t shapeB,
+                                    const m3SatQuery* sat)
 {
     int32_t bodyA = v->shapeBody[shapeA];
     int32_t bodyB = v->shapeBody[shapeB];
-    m3Transform xfA = m3ShapeXf(v, shapeA);
-    m3Transform xfB = m3ShapeXf(v, shapeB);
-    m3Quat qRel = m3MulQuat(m3Conj(xfA.q), xfB.q);
-    m3Vec3 dp = m3V3((m3real)(xfB.p.x - xfA.p.x), (m3real)(xfB.p.y - xfA.p.y),
-                     (m3real)(xfB.p.z - xfA.p.z));
-    m3Vec3 pRel = m3InvRotateVec3(xfA.q, dp);
-    m3Manifold fresh = m3CollideHulls(&v->hulls[v->hullIndex[shapeA]],
-                                      &v->hulls[v->hullIndex[shapeB]], qRel, pRel);
+    m3Transform xfA;
+    m3Quat qRel;
+    m3Vec3 pRel;
+    m3HullPairFrame(v, shapeA, shapeB, &xfA, &qRel, &pRel);
+    m3Manifold fresh = m3CollideHullsQ(&v->hulls[v->hullIndex[shapeA]],
+                                       &v->hulls[v->hullIndex[shapeB]], qRel, pRel, sat);
     if (fresh.pointCount > 0)
     {
         fresh.normal = m3RotateVec3(xfA.q, fresh.normal);
@@ -899,7 +913,11 @@ M3_HD void m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
     }
     else if (typeA == M3_HULL && typeB == M3_HULL)
     {
-        fresh = m3CollidePairHulls(v, shapeA, shapeB);
+        const m3HullData* hullA = &v->hulls[v->hullIndex[shapeA]];
+        const m3HullData* hullB = &v->hulls[v->hullIndex[shapeB]];
+        const m3SatQuery* sat =
+            v->satQueries != NULL && m3WideHullPair(hullA, hullB) ? &v->satQueries[i] : NULL;
+        fresh = m3CollidePairHulls(v, shapeA, shapeB, sat);
     }
     else if (planePair && capsulePair)
     {
