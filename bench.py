@@ -6,6 +6,12 @@ is one m3World_Step.  The default run times the scene's full 600-step
 trajectory (fall, pile-up, settle) with the world resident in HBM; the
 final world hash is checked against the committed reference fixture.
 
+The same line carries two more measurements: ``batched`` (256 independent
+worlds restored from a snapshot and resimulated 8 steps through
+m3World_StepMany, against the same worlds stepped one by one) and
+``hull_chains`` (4000 QuickHull hulls plus 500 chains of 20 capsule links on
+spherical joints, 600 steps).
+
 Prints one JSON line.  ``roofline`` reports the dominant kernel's
 algorithmic HBM bytes over its event-timed average duration; ``cpu_baseline``
 times the reference engine (built from its C sources into oracle/_ref) on
