@@ -465,7 +465,7 @@ M3_HD m3Manifold m3CollideSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 
         {
             continue;
         }
-        axis = m3MulSV3(1.0f / sqrtf(len2), axis);
+        axis = m3MulSV3(1.0f / m3Sqrt(len2), axis);
         if (m3Dot3(axis, m3Sub3(a, hull->center)) < 0.0f)
         {
             axis = m3Neg3(axis);

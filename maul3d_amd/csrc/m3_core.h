@@ -23,11 +23,19 @@
 #define M3_HD static inline
 #endif
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(M3_REAL)
 extern "C" {
 #endif
 
+// The scalar type of every local quantity: float for the engine.  A C++
+// translation unit may define M3_REAL (e.g. double) and include these
+// headers inside a namespace of its own to instantiate the same operation
+// sequences at another precision.
+#ifdef M3_REAL
+typedef M3_REAL m3real;
+#else
 typedef float m3real;
+#endif
 
 #define M3_PI 3.14159265358979323846f
 #define M3_AABB_MARGIN (4.0f * 0.005f)
@@ -107,7 +115,8 @@ typedef struct m3HullData
     m3Vec3 center;
 } m3HullData;
 
-#ifdef __cplusplus
+#if defined(M3_REAL)
+#elif defined(__cplusplus)
 static_assert(sizeof(m3Transform) == 40, "transform layout");
 static_assert(sizeof(m3ShapeGeom) == 32, "geom layout");
 static_assert(sizeof(m3ManifoldPoint) == 36, "manifold point layout");
@@ -180,7 +189,13 @@ M3_HD m3Vec3 m3Cross3(m3Vec3 a, m3Vec3 b)
     return m3V3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
-M3_HD m3real m3Length3(m3Vec3 v) { return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+#ifdef M3_REAL
+M3_HD m3real m3Sqrt(m3real x) { return sqrt(x); }
+#else
+M3_HD m3real m3Sqrt(m3real x) { return sqrtf(x); }
+#endif
+
+M3_HD m3real m3Length3(m3Vec3 v) { return m3Sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
 
 // Degenerate input falls back to +y.
 M3_HD m3Vec3 m3Normalize3(m3Vec3 v)
@@ -211,7 +226,7 @@ M3_HD m3Quat m3Conj(m3Quat q) { return m3Q(-q.x, -q.y, -q.z, q.w); }
 // Degenerate input falls back to identity.
 M3_HD m3Quat m3NormalizeQuat(m3Quat q)
 {
-    m3real mag = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+    m3real mag = m3Sqrt(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
     if (mag < 1.19209290e-7f)
     {
         return m3IdentityQuat();
@@ -313,7 +328,7 @@ M3_HD void m3TangentBasis(m3Vec3 normal, m3Vec3* t1, m3Vec3* t2)
     *t2 = m3Cross3(normal, *t1);
 }
 
-#ifdef __cplusplus
+#if defined(__cplusplus) && !defined(M3_REAL)
 }
 #endif
 

@@ -138,25 +138,25 @@ typedef struct m3View
     m3Transform* xf;
     m3Vec3* lv;
     m3Vec3* av;
-    float* invMass;
+    m3real* invMass;
     m3Mat3* invI;
     m3Mat3* inertia;
     m3Vec3* lc;
-    float* gravityScale;
-    float* linDamp;
-    float* angDamp;
+    m3real* gravityScale;
+    m3real* linDamp;
+    m3real* angDamp;
     uint8_t* bodyAlive;
     uint8_t* type;
     uint8_t* awake;
-    float* sleepTime;
+    m3real* sleepTime;
     uint8_t* bullet;
-    float* minExt;
-    float* maxExt;
+    m3real* minExt;
+    m3real* maxExt;
     m3Vec3* force;
     m3Vec3* torque;
     uint8_t* enabled;
     uint8_t* locks;
-    float* sleepThreshold;
+    m3real* sleepThreshold;
     uint8_t* canSleep;
     uint8_t* hasTarget;
     m3Transform* target;
@@ -166,9 +166,9 @@ typedef struct m3View
     int32_t* shapeBody;
     uint8_t* shapeType;
     m3ShapeGeom* geom;
-    float* friction;
-    float* restitution;
-    float* rolling;
+    m3real* friction;
+    m3real* restitution;
+    m3real* rolling;
     uint8_t* sensor;
     uint8_t* hasOffset;
     m3Vec3* localPos;
@@ -1089,15 +1089,15 @@ M3_HD void m3PrepareContact(const m3View* v, int32_t pairIndex, m3Constraint* c,
     c->invMassB = c->dynB ? v->invMass[bodyB] : 0.0f;
     c->invIA = m3WorldInvInertia(v, bodyA);
     c->invIB = m3WorldInvInertia(v, bodyB);
-    float fricA = v->friction[shapeA];
-    float fricB = v->friction[shapeB];
-    float restA = v->restitution[shapeA];
-    float restB = v->restitution[shapeB];
-    float rollA = v->rolling[shapeA];
-    float rollB = v->rolling[shapeB];
+    m3real fricA = v->friction[shapeA];
+    m3real fricB = v->friction[shapeB];
+    m3real restA = v->restitution[shapeA];
+    m3real restB = v->restitution[shapeB];
+    m3real rollA = v->rolling[shapeA];
+    m3real rollB = v->rolling[shapeB];
     m3Vec3 surfA = v->surfaceVel[shapeA];
     m3Vec3 surfB = v->surfaceVel[shapeB];
-    c->friction = sqrtf(fricA * fricB);
+    c->friction = m3Sqrt(fricA * fricB);
     c->restitution = m3MaxF(restA, restB);
     c->rollingResistance = m3MaxF(rollA, rollB) * m3MaxF(v->maxExt[bodyA], v->maxExt[bodyB]);
     c->rollingImpulse = manifold->rollingImpulse;
@@ -1319,8 +1319,8 @@ M3_HD m3real m3TwistAngle(m3Quat q)
 
 M3_HD m3real m3SwingAngle(m3Quat q)
 {
-    m3real x = sqrtf(q.z * q.z + q.w * q.w);
-    m3real y = sqrtf(q.x * q.x + q.y * q.y);
+    m3real x = m3Sqrt(q.z * q.z + q.w * q.w);
+    m3real y = m3Sqrt(q.x * q.x + q.y * q.y);
     return 2.0f * m3Atan2(y, x);
 }
 
@@ -1362,7 +1362,7 @@ M3_HD void m3PrepareJoint(const m3View* v, int32_t j, m3JointCon* c, m3real h)
         m3real swingLen2 = m3Dot3(swing, swing);
         if (swingLen2 > 1.0e-12f)
         {
-            swing = m3MulSV3(1.0f / sqrtf(swingLen2), swing);
+            swing = m3MulSV3(1.0f / m3Sqrt(swingLen2), swing);
         }
         else
         {
@@ -1377,7 +1377,7 @@ M3_HD void m3PrepareJoint(const m3View* v, int32_t j, m3JointCon* c, m3real h)
         m3Quat relQ = m3MulQuat(conjA, c->frameQB);
         m3real denom = relQ.z * relQ.z + relQ.w * relQ.w;
         m3real tanHalf =
-            denom > 1.0e-12f ? sqrtf((relQ.x * relQ.x + relQ.y * relQ.y) / denom) : 0.0f;
+            denom > 1.0e-12f ? m3Sqrt((relQ.x * relQ.x + relQ.y * relQ.y) / denom) : 0.0f;
         m3Vec3 perp = m3Cross3(swing, coneAxis);
         c->twistJacobian = m3Add3(coneAxis, m3MulSV3(tanHalf, perp));
         m3Vec3 sumTwist =
@@ -1639,7 +1639,7 @@ M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real 
         m3real mag2 = m3Dot3(accum, accum);
         if (mag2 > maxRoll * maxRoll && mag2 > 0.0f)
         {
-            accum = m3MulSV3(maxRoll / sqrtf(mag2), accum);
+            accum = m3MulSV3(maxRoll / m3Sqrt(mag2), accum);
         }
         delta = m3Sub3(accum, c->rollingImpulse);
         c->rollingImpulse = accum;
@@ -1659,7 +1659,7 @@ M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real 
         m3real mag2 = f1 * f1 + f2 * f2;
         if (mag2 > maxFriction * maxFriction)
         {
-            m3real mag = sqrtf(mag2);
+            m3real mag = m3Sqrt(mag2);
             m3real scale = mag > 0.0f ? maxFriction / mag : 0.0f;
             f1 *= scale;
             f2 *= scale;
@@ -1772,14 +1772,14 @@ M3_HD void m3IntegrateVelocity(const m3View* v, int32_t i, m3real h, m3Vec3 grav
     m3real v2 = m3Dot3(lin, lin);
     if (v2 > maxLinear * maxLinear)
     {
-        lin = m3MulSV3(maxLinear / sqrtf(v2), lin);
+        lin = m3MulSV3(maxLinear / m3Sqrt(v2), lin);
     }
     if ((v->locks[i] & 0x40u) == 0)
     {
         m3real w2 = m3Dot3(ang, ang);
         if (w2 > maxAngular * maxAngular)
         {
-            ang = m3MulSV3(maxAngular / sqrtf(w2), ang);
+            ang = m3MulSV3(maxAngular / m3Sqrt(w2), ang);
         }
     }
     v->lv[i] = lin;
@@ -1905,8 +1905,8 @@ M3_HD m3real m3ContinuousVersusPlane(const m3View* v, int32_t fastBody, int32_t 
     m3real tolerance = 0.25f * linearSlop;
     m3Vec3 travel = m3Sub3(sweep->c2, sweep->c1);
     m3Quat dq = m3MulQuat(sweep->q2, m3Q(-sweep->q1.x, -sweep->q1.y, -sweep->q1.z, sweep->q1.w));
-    m3real arc = 2.0f * sqrtf(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[fastBody];
-    m3real rate = sqrtf(m3Dot3(travel, travel)) + arc;
+    m3real arc = 2.0f * m3Sqrt(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[fastBody];
+    m3real rate = m3Sqrt(m3Dot3(travel, travel)) + arc;
     if (!(rate > 0.0f))
     {
         return fraction;
@@ -1963,8 +1963,8 @@ M3_HD int m3ContinuousBody(const m3View* v, int32_t i)
     m3Vec3 dc = m3V3((m3real)(cx - c0.x), (m3real)(cy - c0.y), (m3real)(cz - c0.z));
     m3Quat q0 = v->rot0[i];
     m3Quat dq = m3MulQuat(xfi.q, m3Q(-q0.x, -q0.y, -q0.z, q0.w));
-    m3real arc = 2.0f * sqrtf(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[i];
-    m3real maxMotion = sqrtf(m3Dot3(dc, dc)) + arc;
+    m3real arc = 2.0f * m3Sqrt(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[i];
+    m3real maxMotion = m3Sqrt(m3Dot3(dc, dc)) + arc;
     if (!(maxMotion > 0.5f * v->minExt[i]))
     {
         return 0;
@@ -2004,7 +2004,7 @@ M3_HD void m3SleepTimer(const m3View* v, int32_t i, m3real dt)
     m3real invDt = dt > 0.0f ? 1.0f / dt : 0.0f;
     m3Vec3 lin = v->lv[i];
     m3Vec3 ang = v->av[i];
-    m3real velocity = sqrtf(m3Dot3(lin, lin)) + sqrtf(m3Dot3(ang, ang)) * v->maxExt[i];
+    m3real velocity = m3Sqrt(m3Dot3(lin, lin)) + m3Sqrt(m3Dot3(ang, ang)) * v->maxExt[i];
     m3Transform xf = v->xf[i];
     m3Vec3 rlc = m3RotateVec3(xf.q, v->lc[i]);
     m3Pos3 c0 = v->com0[i];
@@ -2012,8 +2012,8 @@ M3_HD void m3SleepTimer(const m3View* v, int32_t i, m3real dt)
                      (m3real)(xf.p.z + (double)rlc.z - c0.z));
     m3Quat q0 = v->rot0[i];
     m3Quat dq = m3MulQuat(xf.q, m3Q(-q0.x, -q0.y, -q0.z, q0.w));
-    m3real motion = sqrtf(m3Dot3(dc, dc)) +
-                    2.0f * sqrtf(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[i];
+    m3real motion = m3Sqrt(m3Dot3(dc, dc)) +
+                    2.0f * m3Sqrt(dq.x * dq.x + dq.y * dq.y + dq.z * dq.z) * v->maxExt[i];
     m3real sleepVelocity = m3MaxF(velocity, 0.5f * invDt * motion);
     if (v->canSleep[i] == 0)
     {
