@@ -224,6 +224,43 @@ typedef struct m3gpuProfile
 M3GPU_API int m3gpu_profile_begin(m3gpuWorld* world);
 M3GPU_API int m3gpu_profile_end(m3gpuWorld* world, m3gpuProfile* out);
 
+// Trajectory gradients.  From a world's current state, `steps` steps with
+// `params` are simulated (every step applies `force` to each dynamic body),
+// checkpointing the state every `checkpointEvery` steps; the loss
+// |COM_T - target|^2 of the final mass-weighted centre of mass and its
+// gradients with respect to every body's initial linear and angular
+// velocity, mass, friction (of its first shape) and applied force are
+// returned by the reverse pass, which replays each segment from its
+// checkpoint.  The world's state is left as it was.  Scope: dynamic hulls
+// on static planes, no joints, restitution and rolling resistance zero.
+// `offsetKind` (0 none, 1 linear velocity, 2 angular velocity, 3 mass,
+// 4 friction, 5 force) shifts one parameter of body `offsetBody`, component
+// `offsetComp`, by `offsetValue` before the run (finite differences).
+typedef struct m3gpuAdjointRequest
+{
+    int32_t steps;
+    int32_t checkpointEvery;
+    int32_t forwardOnly; // loss only, no reverse pass
+    float target[3];
+    const float* force; // maxBody * 3, or null for none
+    int32_t offsetKind;
+    int32_t offsetBody;
+    int32_t offsetComp;
+    double offsetValue;
+    // outputs (maxBody entries; may be null when forwardOnly)
+    double* loss;
+    float* gradLinearVelocity;  // maxBody * 3
+    float* gradAngularVelocity; // maxBody * 3
+    float* gradMass;            // maxBody
+    float* gradFriction;        // maxBody
+    float* gradForce;           // maxBody * 3
+    int32_t* unsupported; // touching pairs or constraints outside the scope
+    int32_t* ccdPulled;   // continuous pull-backs passed through as identity
+} m3gpuAdjointRequest;
+
+M3GPU_API int m3gpu_adjoint_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,
+                                 const m3gpuAdjointRequest* requests, int32_t count);
+
 #ifdef __cplusplus
 }
 #endif

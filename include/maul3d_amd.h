@@ -253,6 +253,52 @@ M3_API m3Profile m3World_GetProfile(m3WorldId worldId);         // world.h:593
 M3_API int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt,
                                 int32_t substeps);
 M3_API void m3World_Sync(m3WorldId worldId);
+// Trajectory gradients (no reference counterpart; the reference has no
+// differentiable path).  For each world, from its current state: `steps`
+// steps of (timeStep, subStepCount), `force[b]` applied to dynamic body slot
+// b every step (null: none), loss = |COM_T - target|^2 of the final
+// mass-weighted centre of mass, and the loss's gradients with respect to
+// each body slot's initial linear and angular velocity, mass, friction (its
+// first shape's) and applied force, from hand-written adjoint kernels that
+// replay checkpointed segments of `checkpointEvery` steps.  Worlds are left
+// in their current state.  Gradients are taken on the piece of the step the
+// forward point lies in (clamps, contacts and sleep as the forward took
+// them).  Scope: dynamic hulls on static planes, no joints, restitution and
+// rolling resistance zero; `unsupported` counts what fell outside it and
+// `ccdPulled` the continuous pull-backs passed through as the identity.
+// `offset*` shift one parameter before the run (finite-difference checks).
+// Returns the number of worlds processed or -1 on bad arguments.
+typedef struct m3AdjointDef
+{
+    float timeStep;
+    int32_t subStepCount;
+    int32_t steps;
+    int32_t checkpointEvery;
+    m3Vec3 target;
+    const m3Vec3* force; // body slots
+    int32_t forwardOnly;
+    int32_t offsetKind; // 0 none, 1 linear velocity, 2 angular velocity, 3 mass, 4 friction, 5 force
+    int32_t offsetBody; // body slot
+    int32_t offsetComp;
+    double offsetValue;
+} m3AdjointDef;
+
+typedef struct m3AdjointResult
+{
+    double loss;
+    int32_t unsupported;
+    int32_t ccdPulled;
+    m3Vec3* gradLinearVelocity; // caller arrays of bodyCapacity entries (body slots)
+    m3Vec3* gradAngularVelocity;
+    float* gradMass;
+    float* gradFriction;
+    m3Vec3* gradForce;
+} m3AdjointResult;
+
+M3_API m3AdjointDef m3DefaultAdjointDef(void);
+M3_API int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointDef* defs,
+                               m3AdjointResult* results);
+
 // Stage profile of the steps between Begin and End (struct in m3gpu.h).
 struct m3gpuProfile;
 M3_API int32_t m3World_ProfileBegin(m3WorldId worldId);

@@ -255,6 +255,46 @@ _SIGNATURES = {
 }
 
 
+class AdjointDef(C.Structure):
+    """m3AdjointDef (this build's trajectory-gradient request)."""
+
+    _fields_ = [
+        ("timeStep", C.c_float),
+        ("subStepCount", c_i32),
+        ("steps", c_i32),
+        ("checkpointEvery", c_i32),
+        ("target", Vec3),
+        ("force", C.POINTER(Vec3)),
+        ("forwardOnly", c_i32),
+        ("offsetKind", c_i32),
+        ("offsetBody", c_i32),
+        ("offsetComp", c_i32),
+        ("offsetValue", C.c_double),
+    ]
+
+
+class AdjointResult(C.Structure):
+    """m3AdjointResult: loss, scope counters and caller-owned gradient arrays."""
+
+    _fields_ = [
+        ("loss", C.c_double),
+        ("unsupported", c_i32),
+        ("ccdPulled", c_i32),
+        ("gradLinearVelocity", C.POINTER(Vec3)),
+        ("gradAngularVelocity", C.POINTER(Vec3)),
+        ("gradMass", C.POINTER(C.c_float)),
+        ("gradFriction", C.POINTER(C.c_float)),
+        ("gradForce", C.POINTER(Vec3)),
+    ]
+
+
+_SIGNATURES["m3DefaultAdjointDef"] = (AdjointDef, [])
+_SIGNATURES["m3World_Adjoint"] = (
+    c_i32,
+    [C.POINTER(WorldId), c_i32, C.POINTER(AdjointDef), C.POINTER(AdjointResult)],
+)
+
+
 class Api:
     """Typed view of one library's ``m3*`` exports."""
 
@@ -266,6 +306,8 @@ class Api:
             "m3Body_SetAwake",
             "m3CreateHullShape",
             "m3World_GetProfile",
+            "m3DefaultAdjointDef",
+            "m3World_Adjoint",
         ),
     ):
         self.lib = lib

@@ -13,8 +13,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <vector>
+
 #include "m3gpu.h"
-#include "m3_step.h"
+#include "m3_adjoint.h"
 
 #define M3_SENTINEL32 0xFFFFFFFFu
 #define M3_SENTINEL64 0xFFFFFFFFFFFFFFFFull
@@ -3283,4 +3285,651 @@ static void ProfFree(m3gpuWorld* w)
     }
     free(w->prof);
     w->prof = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Trajectory gradients (m3gpu_adjoint_many)
+// ---------------------------------------------------------------------------
+//
+// One thread per world runs the serial step (m3_serial.h) over the world's
+// own device buffers: a forward pass that checkpoints the state every
+// `checkpointEvery` steps, then, segment by segment from the last, a replay
+// from the segment's checkpoint that records the tape and the reverse pass
+// of m3_adjoint.h over it.  The world's state is restored at the end.
+
+struct m3AdjCtx
+{
+    m3View v;
+    uint64_t* keysBuf[2];
+    m3Manifold* mansBuf[2];
+    m3SerialScratch scratch;
+    m3SerialParams sp;
+    int32_t steps;
+    int32_t every;
+    int32_t segs;
+    int32_t forwardOnly;
+    int32_t cur;
+    int32_t rc;
+    double target[3];
+    const m3Vec3* force;
+    int32_t offKind;
+    int32_t offBody;
+    int32_t offComp;
+    m3real offValue;
+    m3real saveInvMass;
+    m3Mat3 saveInvI;
+    m3Mat3 saveInertia;
+    m3real saveFriction;
+    uint8_t* ckpt;
+    int64_t ckptStride;
+    m3Tape* tapes;
+    m3AdjWorld aw;
+    m3real* comBar; // [maxBody]
+    double loss;
+    float* grads;   // [maxBody * 11]
+};
+
+__host__ __device__ static inline int64_t AdjAlign(int64_t x) { return (x + 15) & ~(int64_t)15; }
+
+__host__ static int64_t AdjCheckpointBytes(int32_t nb, int32_t ns, int32_t np)
+{
+    int64_t b = 32;
+    b += AdjAlign((int64_t)nb * sizeof(m3Transform));
+    b += 4 * AdjAlign((int64_t)nb * sizeof(m3Vec3));
+    b += AdjAlign((int64_t)nb * sizeof(m3real));
+    b += 2 * AdjAlign((int64_t)nb);
+    b += 2 * AdjAlign((int64_t)np * sizeof(uint64_t));
+    b += AdjAlign((int64_t)np * sizeof(m3Manifold));
+    b += AdjAlign((int64_t)ns * 6 * sizeof(double));
+    return b;
+}
+
+template <typename T> __device__ static inline void AdjCopy(T* dst, const T* src, int64_t n)
+{
+    for (int64_t i = 0; i < n; ++i)
+    {
+        dst[i] = src[i];
+    }
+}
+
+// Saves (save != 0) or loads the world's mutable state at checkpoint slot k.
+__device__ static void AdjCheckpoint(m3AdjCtx* c, int32_t k, int save)
+{
+    m3View& v = c->v;
+    int32_t nb = v.maxBody;
+    int32_t ns = v.maxShape;
+    int32_t np = v.pairCapacity;
+    uint8_t* p = c->ckpt + (int64_t)k * c->ckptStride;
+    int32_t* hdr = (int32_t*)p;
+    uint64_t* step = (uint64_t*)(p + 16);
+    uint8_t* q = p + 32;
+#define ADJ_FIELD(ptr, T, n)                                                                       \
+    T* ptr = (T*)q;                                                                                \
+    q += AdjAlign((int64_t)(n) * (int64_t)sizeof(T))
+    ADJ_FIELD(xf, m3Transform, nb);
+    ADJ_FIELD(lv, m3Vec3, nb);
+    ADJ_FIELD(av, m3Vec3, nb);
+    ADJ_FIELD(fo, m3Vec3, nb);
+    ADJ_FIELD(to, m3Vec3, nb);
+    ADJ_FIELD(st, m3real, nb);
+    ADJ_FIELD(aw, uint8_t, nb);
+    ADJ_FIELD(pe, uint8_t, nb);
+    ADJ_FIELD(keys, uint64_t, np);
+    ADJ_FIELD(sk, uint64_t, np);
+    ADJ_FIELD(mans, m3Manifold, np);
+    ADJ_FIELD(leaf, double, 6 * (int64_t)ns);
+#undef ADJ_FIELD
+    if (save)
+    {
+        hdr[0] = c->cur;
+        hdr[1] = v.sc->pairCount;
+        hdr[2] = v.sc->sleepCount;
+        hdr[3] = v.sc->fullQuery;
+        *step = v.sc->stepCount;
+        AdjCopy(xf, v.xf, nb);
+        AdjCopy(lv, v.lv, nb);
+        AdjCopy(av, v.av, nb);
+        AdjCopy(fo, v.force, nb);
+        AdjCopy(to, v.torque, nb);
+        AdjCopy(st, v.sleepTime, nb);
+        AdjCopy(aw, v.awake, nb);
+        AdjCopy(pe, v.pending, nb);
+        AdjCopy(keys, c->keysBuf[c->cur], hdr[1]);
+        AdjCopy(mans, c->mansBuf[c->cur], hdr[1]);
+        AdjCopy(sk, v.sleepKeys, hdr[2]);
+        AdjCopy(leaf, v.leaf, 6 * (int64_t)ns);
+    }
+    else
+    {
+        c->cur = hdr[0];
+        v.sc->pairCount = hdr[1];
+        v.sc->sleepCount = hdr[2];
+        v.sc->fullQuery = hdr[3];
+        v.sc->stepCount = *step;
+        AdjCopy(v.xf, xf, nb);
+        AdjCopy(v.lv, lv, nb);
+        AdjCopy(v.av, av, nb);
+        AdjCopy(v.force, fo, nb);
+        AdjCopy(v.torque, to, nb);
+        AdjCopy(v.sleepTime, st, nb);
+        AdjCopy(v.awake, aw, nb);
+        AdjCopy(v.pending, pe, nb);
+        AdjCopy(c->keysBuf[c->cur], keys, hdr[1]);
+        AdjCopy(c->mansBuf[c->cur], mans, hdr[1]);
+        AdjCopy(v.sleepKeys, sk, hdr[2]);
+        AdjCopy(v.leaf, leaf, 6 * (int64_t)ns);
+    }
+}
+
+__device__ static int AdjStepOnce(m3AdjCtx* c, m3Tape* tape)
+{
+    m3View& v = c->v;
+    for (int32_t b = 0; b < v.maxBody; ++b)
+    {
+        if (v.bodyAlive[b] && v.type[b] == M3_DYNAMIC)
+        {
+            v.force[b] = c->force[b];
+        }
+    }
+    c->cur ^= 1;
+    v.keys = c->keysBuf[c->cur];
+    v.manifolds = c->mansBuf[c->cur];
+    v.oldKeys = c->keysBuf[c->cur ^ 1];
+    v.oldManifolds = c->mansBuf[c->cur ^ 1];
+    v.sc->stall = 0;
+    if (m3SerialStep(&v, &c->scratch, &c->sp, tape) != 0)
+    {
+        c->cur ^= 1;
+        return 1;
+    }
+    return 0;
+}
+
+__device__ static void AdjOffset(m3AdjCtx* c)
+{
+    m3View& v = c->v;
+    int32_t b = c->offBody;
+    if (c->offKind == 0 || b < 0 || b >= v.maxBody)
+    {
+        return;
+    }
+    m3real d = c->offValue;
+    c->saveInvMass = v.invMass[b];
+    c->saveInvI = v.invI[b];
+    c->saveInertia = v.inertia[b];
+    int32_t shape = v.shapeHead[b];
+    c->saveFriction = shape >= 0 ? v.friction[shape] : 0.0f;
+    m3Vec3* vec = c->offKind == 1 ? &v.lv[b] : (c->offKind == 2 ? &v.av[b] : nullptr);
+    if (vec != nullptr)
+    {
+        if (c->offComp == 0)
+            vec->x += d;
+        else if (c->offComp == 1)
+            vec->y += d;
+        else
+            vec->z += d;
+    }
+    if (c->offKind == 3)
+    {
+        m3real m = 1.0f / v.invMass[b];
+        m3real m2 = m + d;
+        v.invMass[b] = 1.0f / m2;
+        m3real si = m / m2;
+        m3real sf = m2 / m;
+        v.invI[b].cx = m3MulSV3(si, v.invI[b].cx);
+        v.invI[b].cy = m3MulSV3(si, v.invI[b].cy);
+        v.invI[b].cz = m3MulSV3(si, v.invI[b].cz);
+        v.inertia[b].cx = m3MulSV3(sf, v.inertia[b].cx);
+        v.inertia[b].cy = m3MulSV3(sf, v.inertia[b].cy);
+        v.inertia[b].cz = m3MulSV3(sf, v.inertia[b].cz);
+    }
+    if (c->offKind == 4 && shape >= 0)
+    {
+        v.friction[shape] += d;
+    }
+}
+
+__global__ void k_adj_forward(m3AdjCtx* ctxs, int32_t count)
+{
+    int32_t k = blockIdx.x;
+    if (k >= count || threadIdx.x != 0)
+    {
+        return;
+    }
+    m3AdjCtx* c = &ctxs[k];
+    m3View& v = c->v;
+    AdjCheckpoint(c, c->segs, 1); // the original state
+    AdjOffset(c);
+    c->rc = 0;
+    for (int32_t s = 0; s < c->steps && c->rc == 0; ++s)
+    {
+        if (s % c->every == 0)
+        {
+            AdjCheckpoint(c, s / c->every, 1);
+        }
+        c->rc = AdjStepOnce(c, nullptr);
+    }
+    // Loss of the final mass-weighted centre of mass.
+    double M = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int32_t b = 0; b < v.maxBody; ++b)
+    {
+        if (!v.bodyAlive[b] || v.type[b] != M3_DYNAMIC || v.invMass[b] == 0.0f)
+        {
+            continue;
+        }
+        double m = 1.0 / (double)v.invMass[b];
+        m3Vec3 r = m3RotateVec3(v.xf[b].q, v.lc[b]);
+        cx += m * (v.xf[b].p.x + (double)r.x);
+        cy += m * (v.xf[b].p.y + (double)r.y);
+        cz += m * (v.xf[b].p.z + (double)r.z);
+        M += m;
+    }
+    double C[3] = {cx / M, cy / M, cz / M};
+    c->loss = 0.0;
+    for (int i = 0; i < 3; ++i)
+    {
+        c->loss += (C[i] - c->target[i]) * (C[i] - c->target[i]);
+    }
+    if (c->forwardOnly || c->rc != 0)
+    {
+        return;
+    }
+    m3AdjWorld* a = &c->aw;
+    m3real Cb[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        Cb[i] = (m3real)(2.0 * (C[i] - c->target[i]));
+    }
+    for (int32_t b = 0; b < v.maxBody; ++b)
+    {
+        memset(&a->body[b], 0, sizeof(m3BodyBar));
+        memset(&a->param[b], 0, sizeof(m3ParamBar));
+        c->comBar[b] = 0.0f;
+        if (!v.bodyAlive[b] || v.type[b] != M3_DYNAMIC)
+        {
+            continue;
+        }
+        m3real m = 1.0f / v.invMass[b];
+        m3Vec3 cbar = m3V3(Cb[0] * m / (m3real)M, Cb[1] * m / (m3real)M, Cb[2] * m / (m3real)M);
+        a->body[b].p = cbar;
+        m3AdjRotate(v.xf[b].q, v.lc[b], cbar, &a->body[b].q);
+        m3Vec3 r = m3RotateVec3(v.xf[b].q, v.lc[b]);
+        double dx = v.xf[b].p.x + (double)r.x - C[0];
+        double dy = v.xf[b].p.y + (double)r.y - C[1];
+        double dz = v.xf[b].p.z + (double)r.z - C[2];
+        c->comBar[b] = (m3real)((Cb[0] * dx + Cb[1] * dy + Cb[2] * dz) / M);
+    }
+    for (int32_t i = 0; i < v.pairCapacity; ++i)
+    {
+        memset(&a->store[i], 0, sizeof(m3ManBar));
+    }
+    a->unsupported = 0;
+    a->ccdPulled = 0;
+}
+
+// Replay of segment `seg` with its tape, then its reverse pass.
+__global__ void k_adj_segment(m3AdjCtx* ctxs, int32_t count, int32_t seg)
+{
+    int32_t k = blockIdx.x;
+    if (k >= count || threadIdx.x != 0)
+    {
+        return;
+    }
+    m3AdjCtx* c = &ctxs[k];
+    if (c->forwardOnly || c->rc != 0 || seg >= c->segs)
+    {
+        return;
+    }
+    AdjCheckpoint(c, seg, 0);
+    int32_t s0 = seg * c->every;
+    int32_t s1 = s0 + c->every < c->steps ? s0 + c->every : c->steps;
+    for (int32_t s = s0; s < s1; ++s)
+    {
+        AdjStepOnce(c, &c->tapes[s - s0]);
+    }
+    for (int32_t s = s1 - 1; s >= s0; --s)
+    {
+        m3AdjStep(&c->v, &c->sp, &c->tapes[s - s0], &c->aw);
+        m3ManBar* t = c->aw.store;
+        c->aw.store = c->aw.storePrev;
+        c->aw.storePrev = t;
+    }
+}
+
+__global__ void k_adj_finish(m3AdjCtx* ctxs, int32_t count)
+{
+    int32_t k = blockIdx.x;
+    if (k >= count || threadIdx.x != 0)
+    {
+        return;
+    }
+    m3AdjCtx* c = &ctxs[k];
+    m3View& v = c->v;
+    if (!c->forwardOnly && c->rc == 0)
+    {
+        for (int32_t b = 0; b < v.maxBody; ++b)
+        {
+            float* g = &c->grads[11 * b];
+            const m3BodyBar* bb = &c->aw.body[b];
+            const m3ParamBar* pb = &c->aw.param[b];
+            int dyn = v.bodyAlive[b] && v.type[b] == M3_DYNAMIC;
+            m3real im = v.invMass[b];
+            m3real m = dyn ? 1.0f / im : 0.0f;
+            g[0] = bb->v.x;
+            g[1] = bb->v.y;
+            g[2] = bb->v.z;
+            g[3] = bb->w.x;
+            g[4] = bb->w.y;
+            g[5] = bb->w.z;
+            g[6] = dyn ? pb->invMass * (-im * im) + pb->inertia / m - pb->invI / m + c->comBar[b] : 0.0f;
+            g[7] = pb->friction;
+            g[8] = pb->force.x;
+            g[9] = pb->force.y;
+            g[10] = pb->force.z;
+        }
+    }
+    // Original state and parameters back.
+    AdjCheckpoint(c, c->segs, 0);
+    if (c->offKind != 0 && c->offBody >= 0 && c->offBody < v.maxBody)
+    {
+        int32_t b = c->offBody;
+        v.invMass[b] = c->saveInvMass;
+        v.invI[b] = c->saveInvI;
+        v.inertia[b] = c->saveInertia;
+        int32_t shape = v.shapeHead[b];
+        if (shape >= 0)
+        {
+            v.friction[shape] = c->saveFriction;
+        }
+    }
+}
+
+// Device arena of the adjoint, kept between calls and grown as needed.
+static void* s_adjArena = nullptr;
+static size_t s_adjBytes = 0;
+
+static int AdjReserve(size_t bytes)
+{
+    if (bytes <= s_adjBytes)
+    {
+        return 0;
+    }
+    if (s_adjArena != nullptr)
+    {
+        HIP_TRY(hipFree(s_adjArena));
+        s_adjArena = nullptr;
+        s_adjBytes = 0;
+    }
+    HIP_TRY(hipMalloc(&s_adjArena, bytes));
+    s_adjBytes = bytes;
+    return 0;
+}
+
+// Per-world arena layout: tapes, bars and checkpoints.
+struct AdjLayout
+{
+    int64_t tapeStep;  // bytes of one step's tape
+    int64_t bars;      // bytes of the reverse scratch
+    int64_t ckptStride;
+    int64_t total;
+};
+
+static int64_t TapeStepBytes(int32_t nb, int32_t np, int32_t S)
+{
+    int64_t b = 0;
+    b += 2 * AdjAlign((int64_t)nb * sizeof(m3Transform)); // xf0, xfEnd
+    b += 3 * AdjAlign((int64_t)nb * sizeof(m3Vec3));      // lv0, av0, force0
+    b += 3 * AdjAlign((int64_t)nb);                       // slept, ccd, mover
+    b += AdjAlign((int64_t)np * sizeof(uint64_t));
+    b += AdjAlign((int64_t)np * sizeof(m3Manifold));
+    b += 2 * AdjAlign((int64_t)np * sizeof(int32_t));     // carryPair, consPair
+    b += AdjAlign((int64_t)np * 4);                       // carryPoint
+    b += AdjAlign(4 * sizeof(int32_t)) + AdjAlign((M3_GRAPH_COLORS + 2) * sizeof(int32_t));
+    int64_t sb = (int64_t)S * nb;
+    b += AdjAlign(sb * sizeof(m3Transform));
+    b += 8 * AdjAlign(sb * sizeof(m3Vec3)); // slv sav sdp ilv iav plv pav rdp
+    b += 2 * AdjAlign(sb * sizeof(m3Quat)); // sdq rdq
+    b += AdjAlign((int64_t)S * 2 * np * sizeof(m3TapePass));
+    b += AdjAlign((int64_t)np * sizeof(m3TapePass));
+    return b;
+}
+
+static int64_t BarsBytes(int32_t nb, int32_t np)
+{
+    return AdjAlign((int64_t)nb * sizeof(m3BodyBar)) + AdjAlign((int64_t)nb * sizeof(m3ParamBar)) +
+           AdjAlign((int64_t)np * sizeof(m3ConBar)) + 2 * AdjAlign((int64_t)np * sizeof(m3ManBar)) +
+           AdjAlign((int64_t)np * sizeof(m3Constraint)) + AdjAlign((int64_t)nb * sizeof(m3Vec3)) +
+           AdjAlign((int64_t)nb * sizeof(m3Quat)) + AdjAlign((int64_t)nb * sizeof(m3real)) +
+           AdjAlign((int64_t)nb * 11 * sizeof(float)) + AdjAlign((int64_t)nb * sizeof(m3Vec3)) +
+           2 * AdjAlign((int64_t)nb * sizeof(int32_t));
+}
+
+template <typename T> static T* Carve(uint8_t** p, int64_t n)
+{
+    T* r = (T*)*p;
+    *p += AdjAlign(n * (int64_t)sizeof(T));
+    return r;
+}
+
+static void CarveTape(m3Tape* t, uint8_t** p, int32_t nb, int32_t np, int32_t S)
+{
+    t->bodyCap = nb;
+    t->pairCap = np;
+    t->substeps = S;
+    t->xf0 = Carve<m3Transform>(p, nb);
+    t->xfEnd = Carve<m3Transform>(p, nb);
+    t->lv0 = Carve<m3Vec3>(p, nb);
+    t->av0 = Carve<m3Vec3>(p, nb);
+    t->force0 = Carve<m3Vec3>(p, nb);
+    t->slept = Carve<uint8_t>(p, nb);
+    t->ccd = Carve<uint8_t>(p, nb);
+    t->mover = Carve<uint8_t>(p, nb);
+    t->keys = Carve<uint64_t>(p, np);
+    t->man = Carve<m3Manifold>(p, np);
+    t->carryPair = Carve<int32_t>(p, np);
+    t->consPair = Carve<int32_t>(p, np);
+    t->carryPoint = Carve<int8_t>(p, 4 * (int64_t)np);
+    t->counts = Carve<int32_t>(p, 4);
+    t->colorStarts = Carve<int32_t>(p, M3_GRAPH_COLORS + 2);
+    int64_t sb = (int64_t)S * nb;
+    t->sxf = Carve<m3Transform>(p, sb);
+    t->slv = Carve<m3Vec3>(p, sb);
+    t->sav = Carve<m3Vec3>(p, sb);
+    t->sdp = Carve<m3Vec3>(p, sb);
+    t->ilv = Carve<m3Vec3>(p, sb);
+    t->iav = Carve<m3Vec3>(p, sb);
+    t->plv = Carve<m3Vec3>(p, sb);
+    t->pav = Carve<m3Vec3>(p, sb);
+    t->rdp = Carve<m3Vec3>(p, sb);
+    t->sdq = Carve<m3Quat>(p, sb);
+    t->rdq = Carve<m3Quat>(p, sb);
+    t->pass = Carve<m3TapePass>(p, (int64_t)S * 2 * np);
+    t->fin = Carve<m3TapePass>(p, np);
+}
+
+extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* ps,
+                                  const m3gpuAdjointRequest* rqs, int32_t count)
+{
+    if (ws == nullptr || ps == nullptr || rqs == nullptr || count <= 0)
+    {
+        return -1;
+    }
+    // Layouts and the arena.
+    std::vector<int64_t> offs(count + 1, 0);
+    int32_t maxSegs = 0;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3gpuWorld* w = ws[k];
+        if (w == nullptr || JoinBatch(w) != 0)
+        {
+            return -1;
+        }
+        HIP_TRY(hipStreamSynchronize(w->stream));
+        w->streamBusy = 0;
+        const m3View& v = w->view;
+        int32_t every = rqs[k].checkpointEvery > 0 ? rqs[k].checkpointEvery : 10;
+        int32_t segs = (rqs[k].steps + every - 1) / every;
+        maxSegs = segs > maxSegs ? segs : maxSegs;
+        int64_t bytes = (int64_t)(segs + 1) * AdjCheckpointBytes(v.maxBody, v.maxShape, v.pairCapacity);
+        if (!rqs[k].forwardOnly)
+        {
+            bytes += (int64_t)every * TapeStepBytes(v.maxBody, v.pairCapacity, ps[k].substeps);
+            bytes += (int64_t)every * sizeof(m3Tape) + 64;
+        }
+        bytes += BarsBytes(v.maxBody, v.pairCapacity) + AdjAlign((int64_t)v.maxBody * sizeof(m3Vec3));
+        offs[k + 1] = offs[k] + AdjAlign(bytes);
+    }
+    size_t ctxBytes = AdjAlign((int64_t)count * sizeof(m3AdjCtx));
+    if (AdjReserve((size_t)offs[count] + ctxBytes) != 0)
+    {
+        return -1;
+    }
+    uint8_t* arena = (uint8_t*)s_adjArena;
+    m3AdjCtx* dctx = (m3AdjCtx*)(arena + offs[count]);
+    std::vector<m3AdjCtx> ctx(count);
+    std::vector<std::vector<m3Tape>> tapes(count);
+    std::vector<float> hforce;
+    hipStream_t st = ws[0]->stream;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3gpuWorld* w = ws[k];
+        const m3gpuAdjointRequest* r = &rqs[k];
+        m3AdjCtx& c = ctx[k];
+        memset(&c, 0, sizeof(c));
+        const m3View& v = w->view;
+        int32_t nb = v.maxBody, ns = v.maxShape, np = v.pairCapacity;
+        c.v = v;
+        c.keysBuf[0] = w->keysBuf[0];
+        c.keysBuf[1] = w->keysBuf[1];
+        c.mansBuf[0] = w->mansBuf[0];
+        c.mansBuf[1] = w->mansBuf[1];
+        c.cur = w->cur;
+        const m3gpuStepParams* p = &ps[k];
+        c.sp.dt = p->dt;
+        c.sp.substeps = p->substeps;
+        c.sp.gravity = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
+        c.sp.contactHertz = p->contactHertz;
+        c.sp.contactDampingRatio = p->contactDampingRatio;
+        c.sp.contactPushMaxSpeed = p->contactPushMaxSpeed;
+        c.sp.restitutionThreshold = p->restitutionThreshold;
+        c.sp.maximumLinearSpeed = p->maximumLinearSpeed;
+        c.sp.maximumAngularSpeed = p->maximumAngularSpeed;
+        c.sp.sleepEnabled = p->sleepEnabled;
+        c.sp.continuousEnabled = p->continuousEnabled;
+        c.steps = r->steps;
+        c.every = r->checkpointEvery > 0 ? r->checkpointEvery : 10;
+        c.segs = (r->steps + c.every - 1) / c.every;
+        c.forwardOnly = r->forwardOnly;
+        for (int i = 0; i < 3; ++i)
+        {
+            c.target[i] = (double)r->target[i];
+        }
+        c.offKind = r->offsetKind;
+        c.offBody = r->offsetBody;
+        c.offComp = r->offsetComp;
+        c.offValue = (m3real)r->offsetValue;
+        uint8_t* q = arena + offs[k];
+        c.ckptStride = AdjCheckpointBytes(nb, ns, np);
+        c.ckpt = Carve<uint8_t>(&q, (int64_t)(c.segs + 1) * c.ckptStride);
+        m3Vec3* force = Carve<m3Vec3>(&q, nb);
+        c.force = force;
+        hforce.assign((size_t)nb * 3, 0.0f);
+        if (r->force != nullptr)
+        {
+            memcpy(hforce.data(), r->force, (size_t)nb * 3 * sizeof(float));
+        }
+        HIP_TRY(hipMemcpyAsync(force, hforce.data(), (size_t)nb * sizeof(m3Vec3), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        c.aw.body = Carve<m3BodyBar>(&q, nb);
+        c.aw.param = Carve<m3ParamBar>(&q, nb);
+        c.aw.cons = Carve<m3ConBar>(&q, np);
+        c.aw.store = Carve<m3ManBar>(&q, np);
+        c.aw.storePrev = Carve<m3ManBar>(&q, np);
+        c.aw.prep = Carve<m3Constraint>(&q, np);
+        c.aw.ccdP = Carve<m3Vec3>(&q, nb);
+        c.aw.ccdQ = Carve<m3Quat>(&q, nb);
+        c.comBar = Carve<m3real>(&q, nb);
+        c.grads = Carve<float>(&q, (int64_t)nb * 11);
+        c.scratch.root = Carve<int32_t>(&q, nb);
+        c.scratch.masks = Carve<uint32_t>(&q, nb);
+        if (!r->forwardOnly)
+        {
+            m3Tape* dt = Carve<m3Tape>(&q, c.every);
+            tapes[k].resize(c.every);
+            for (int32_t s = 0; s < c.every; ++s)
+            {
+                CarveTape(&tapes[k][s], &q, nb, np, p->substeps);
+            }
+            HIP_TRY(hipMemcpyAsync(dt, tapes[k].data(), c.every * sizeof(m3Tape), hipMemcpyHostToDevice, st));
+            HIP_TRY(hipStreamSynchronize(st));
+            c.tapes = dt;
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(dctx, ctx.data(), count * sizeof(m3AdjCtx), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_adj_forward, dim3(count), dim3(1), 0, st, dctx, count);
+    HIP_TRY(hipGetLastError());
+    for (int32_t seg = maxSegs - 1; seg >= 0; --seg)
+    {
+        hipLaunchKernelGGL(k_adj_segment, dim3(count), dim3(1), 0, st, dctx, count, seg);
+        HIP_TRY(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_adj_finish, dim3(count), dim3(1), 0, st, dctx, count);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(ctx.data(), dctx, count * sizeof(m3AdjCtx), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    int rc = 0;
+    std::vector<float> g;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        const m3AdjCtx& c = ctx[k];
+        const m3gpuAdjointRequest* r = &rqs[k];
+        int32_t nb = ws[k]->view.maxBody;
+        if (c.rc != 0)
+        {
+            rc = 1;
+        }
+        if (r->loss)
+        {
+            *r->loss = c.loss;
+        }
+        if (r->unsupported)
+        {
+            *r->unsupported = c.aw.unsupported;
+        }
+        if (r->ccdPulled)
+        {
+            *r->ccdPulled = c.aw.ccdPulled;
+        }
+        if (r->forwardOnly)
+        {
+            continue;
+        }
+        g.resize((size_t)nb * 11);
+        HIP_TRY(hipMemcpy(g.data(), c.grads, g.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int32_t b = 0; b < nb; ++b)
+        {
+            const float* e = &g[11 * (size_t)b];
+            if (r->gradLinearVelocity)
+            {
+                memcpy(&r->gradLinearVelocity[3 * b], e, 3 * sizeof(float));
+            }
+            if (r->gradAngularVelocity)
+            {
+                memcpy(&r->gradAngularVelocity[3 * b], e + 3, 3 * sizeof(float));
+            }
+            if (r->gradMass)
+            {
+                r->gradMass[b] = e[6];
+            }
+            if (r->gradFriction)
+            {
+                r->gradFriction[b] = e[7];
+            }
+            if (r->gradForce)
+            {
+                memcpy(&r->gradForce[3 * b], e + 8, 3 * sizeof(float));
+            }
+        }
+    }
+    return rc;
 }

@@ -40,6 +40,8 @@ typedef struct m3Tape
     m3Transform* xf0;  // [bodyCap] pose at step start
     m3Vec3* lv0;       // [bodyCap]
     m3Vec3* av0;       // [bodyCap]
+    m3Vec3* force0;    // [bodyCap] applied force of the step
+    m3Transform* xfEnd; // [bodyCap] pose after the substeps (continuous pass input)
     uint8_t* slept;    // [bodyCap] frozen by this step's sleep pass
     uint8_t* ccd;      // [bodyCap] pulled back by this step's continuous pass
     uint8_t* mover;    // [bodyCap] solved this step
@@ -58,9 +60,12 @@ typedef struct m3Tape
     m3Quat* sdq;
     m3Vec3* ilv;      // [substeps * bodyCap] after integrate-velocities
     m3Vec3* iav;
+    m3Vec3* plv;      // [substeps * bodyCap] after the solve pass (integrate-positions input)
+    m3Vec3* pav;
     m3Vec3* rdp;      // [substeps * bodyCap] after integrate-positions (relax pass input)
     m3Quat* rdq;
     m3TapePass* pass; // [substeps * 2 * pairCap] pre-solve records, slot order
+    m3TapePass* fin;  // [pairCap] accumulators after the last pass, slot order
 } m3Tape;
 
 M3_HD int32_t m3SerialFind(const int32_t* parent, int32_t i)
@@ -251,6 +256,7 @@ M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams
             tape->xf0[b] = v->xf[b];
             tape->lv0[b] = v->lv[b];
             tape->av0[b] = v->av[b];
+            tape->force0[b] = v->force[b];
             tape->slept[b] = 0;
             tape->ccd[b] = 0;
             tape->mover[b] = 0;
@@ -536,6 +542,14 @@ M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams
         {
             if (mode == 2)
             {
+                if (tape != NULL)
+                {
+                    for (int32_t b = 0; b < nb; ++b)
+                    {
+                        tape->plv[base + b] = v->lv[b];
+                        tape->pav[base + b] = v->av[b];
+                    }
+                }
                 for (int32_t m = 0; m < nm; ++m)
                 {
                     m3IntegratePosition(v, v->movers[m], h);
@@ -572,6 +586,25 @@ M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams
             }
         }
     }
+    if (tape != NULL)
+    {
+        for (int32_t k = 0; k < nc; ++k)
+        {
+            const m3Constraint* c = &v->cons[k];
+            m3TapePass* r = &tape->fin[k];
+            r->vA = v->lv[c->bodyA];
+            r->wA = v->av[c->bodyA];
+            r->vB = v->lv[c->bodyB];
+            r->wB = v->av[c->bodyB];
+            for (int32_t q = 0; q < M3_MANIFOLD_MAX_POINTS; ++q)
+            {
+                r->lambda[q] = q < c->pointCount ? c->points[q].normalImpulse : 0.0f;
+            }
+            r->f1 = c->frictionImpulse1;
+            r->f2 = c->frictionImpulse2;
+            r->twist = c->twistImpulse;
+        }
+    }
     for (int32_t m = 0; m < nm; ++m)
     {
         v->force[v->movers[m]] = m3V3(0.0f, 0.0f, 0.0f);
@@ -584,6 +617,13 @@ M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams
     for (int32_t k = 0; k < nc; ++k)
     {
         m3StoreOne(v, &v->cons[k]);
+    }
+    if (tape != NULL)
+    {
+        for (int32_t b = 0; b < nb; ++b)
+        {
+            tape->xfEnd[b] = v->xf[b];
+        }
     }
     if (p->continuousEnabled)
     {

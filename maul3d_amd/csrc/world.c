@@ -1931,6 +1931,83 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
     return n;
 }
 
+m3AdjointDef m3DefaultAdjointDef(void)
+{
+    m3AdjointDef d;
+    memset(&d, 0, sizeof(d));
+    d.timeStep = 1.0f / 60.0f;
+    d.subStepCount = 4;
+    d.steps = 120;
+    d.checkpointEvery = 10;
+    return d;
+}
+
+int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointDef* defs,
+                        m3AdjointResult* results)
+{
+    if (worlds == NULL || defs == NULL || results == NULL || count <= 0)
+    {
+        return -1;
+    }
+    enum { CHUNK = 1024 };
+    if (count > CHUNK)
+    {
+        return -1;
+    }
+    m3gpuWorld* devs[CHUNK];
+    m3gpuStepParams params[CHUNK];
+    m3gpuAdjointRequest reqs[CHUNK];
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3World* w = WorldFromId(worlds[k]);
+        const m3AdjointDef* d = &defs[k];
+        if (w == NULL || !(d->timeStep > 0.0f) || d->subStepCount < 1 ||
+            d->subStepCount > M3_MAX_SUBSTEPS || d->steps < 1 || d->checkpointEvery < 1)
+        {
+            return -1;
+        }
+        for (int32_t j = 0; j < w->jointPool.maxIndex; ++j)
+        {
+            if (w->jointPool.alive[j])
+            {
+                return -1; // joints are outside the adjoint's scope
+            }
+        }
+        EnsureDevice(w);
+        devs[k] = w->dev;
+        FillParams(w, d->timeStep, d->subStepCount, &params[k]);
+        m3gpuAdjointRequest* r = &reqs[k];
+        memset(r, 0, sizeof(*r));
+        r->steps = d->steps;
+        r->checkpointEvery = d->checkpointEvery;
+        r->forwardOnly = d->forwardOnly;
+        r->target[0] = d->target.x;
+        r->target[1] = d->target.y;
+        r->target[2] = d->target.z;
+        r->force = (const float*)d->force;
+        r->offsetKind = d->offsetKind;
+        r->offsetBody = d->offsetBody;
+        r->offsetComp = d->offsetComp;
+        r->offsetValue = d->offsetValue;
+        m3AdjointResult* o = &results[k];
+        r->loss = &o->loss;
+        r->gradLinearVelocity = (float*)o->gradLinearVelocity;
+        r->gradAngularVelocity = (float*)o->gradAngularVelocity;
+        r->gradMass = o->gradMass;
+        r->gradFriction = o->gradFriction;
+        r->gradForce = (float*)o->gradForce;
+        r->unsupported = &o->unsupported;
+        r->ccdPulled = &o->ccdPulled;
+        o->unsupported = 0;
+        o->ccdPulled = 0;
+    }
+    if (m3gpu_adjoint_many(devs, params, reqs, count) != 0)
+    {
+        DeviceFail("m3gpu_adjoint_many");
+    }
+    return count;
+}
+
 // Uploads pending host changes and waits for the world's queued device work
 // (timing harnesses call it before and after a timed region).
 void m3World_Sync(m3WorldId worldId)

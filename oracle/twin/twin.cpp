@@ -17,6 +17,7 @@
 #include <string.h>
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "m3gpu.h"
@@ -514,5 +515,410 @@ extern "C" int m3gpu_profile_begin(m3gpuWorld*) { return 0; }
 extern "C" int m3gpu_profile_end(m3gpuWorld*, m3gpuProfile* p)
 {
     memset(p, 0, sizeof(*p));
+    return 0;
+}
+
+// ---- trajectory gradients ---------------------------------------------------
+
+namespace
+{
+struct Checkpoint
+{
+    int cur;
+    std::vector<tw::m3Transform> xf;
+    std::vector<tw::m3Vec3> lv, av, force, torque;
+    std::vector<uint8_t> awake, pending;
+    std::vector<R> sleepTime;
+    std::vector<uint64_t> keys, sleepKeys;
+    std::vector<tw::m3Manifold> mans;
+    std::vector<double> leaf;
+    int32_t pairCount, sleepCount, fullQuery;
+    uint64_t stepCount;
+};
+
+void Save(m3gpuWorld* w, Checkpoint* c)
+{
+    tw::m3View& v = w->v;
+    int nb = v.maxBody, ns = v.maxShape;
+    c->cur = w->cur;
+    c->xf.assign(v.xf, v.xf + nb);
+    c->lv.assign(v.lv, v.lv + nb);
+    c->av.assign(v.av, v.av + nb);
+    c->force.assign(v.force, v.force + nb);
+    c->torque.assign(v.torque, v.torque + nb);
+    c->awake.assign(v.awake, v.awake + nb);
+    c->pending.assign(v.pending, v.pending + nb);
+    c->sleepTime.assign(v.sleepTime, v.sleepTime + nb);
+    c->pairCount = v.sc->pairCount;
+    c->keys.assign(w->keysBuf[w->cur].begin(), w->keysBuf[w->cur].begin() + c->pairCount);
+    c->mans.assign(w->mansBuf[w->cur].begin(), w->mansBuf[w->cur].begin() + c->pairCount);
+    c->sleepCount = v.sc->sleepCount;
+    c->sleepKeys.assign(v.sleepKeys, v.sleepKeys + c->sleepCount);
+    c->leaf.assign(v.leaf, v.leaf + 6 * (size_t)ns);
+    c->fullQuery = v.sc->fullQuery;
+    c->stepCount = v.sc->stepCount;
+}
+
+void Load(m3gpuWorld* w, const Checkpoint* c)
+{
+    tw::m3View& v = w->v;
+    w->cur = c->cur;
+    std::copy(c->xf.begin(), c->xf.end(), v.xf);
+    std::copy(c->lv.begin(), c->lv.end(), v.lv);
+    std::copy(c->av.begin(), c->av.end(), v.av);
+    std::copy(c->force.begin(), c->force.end(), v.force);
+    std::copy(c->torque.begin(), c->torque.end(), v.torque);
+    std::copy(c->awake.begin(), c->awake.end(), v.awake);
+    std::copy(c->pending.begin(), c->pending.end(), v.pending);
+    std::copy(c->sleepTime.begin(), c->sleepTime.end(), v.sleepTime);
+    v.sc->pairCount = c->pairCount;
+    std::copy(c->keys.begin(), c->keys.end(), w->keysBuf[w->cur].begin());
+    std::copy(c->mans.begin(), c->mans.end(), w->mansBuf[w->cur].begin());
+    v.keys = w->keysBuf[w->cur].data();
+    v.manifolds = w->mansBuf[w->cur].data();
+    v.oldKeys = w->keysBuf[w->cur ^ 1].data();
+    v.oldManifolds = w->mansBuf[w->cur ^ 1].data();
+    v.sc->sleepCount = c->sleepCount;
+    std::copy(c->sleepKeys.begin(), c->sleepKeys.end(), v.sleepKeys);
+    std::copy(c->leaf.begin(), c->leaf.end(), v.leaf);
+    v.sc->fullQuery = c->fullQuery;
+    v.sc->stepCount = c->stepCount;
+}
+
+// Tape storage of one step.
+struct TapeStore
+{
+    std::vector<tw::m3Transform> xf0, sxf, xfEnd;
+    std::vector<tw::m3Vec3> lv0, av0, force0, slv, sav, sdp, ilv, iav, plv, pav, rdp;
+    std::vector<tw::m3Quat> sdq, rdq;
+    std::vector<uint8_t> slept, ccd, mover;
+    std::vector<uint64_t> keys;
+    std::vector<tw::m3Manifold> man;
+    std::vector<int32_t> carryPair, consPair, counts, colorStarts;
+    std::vector<int8_t> carryPoint;
+    std::vector<tw::m3TapePass> pass, fin;
+    tw::m3Tape t;
+
+    void Init(int nb, int np, int substeps)
+    {
+        size_t sb = (size_t)substeps * nb;
+        xf0.resize(nb);
+        lv0.resize(nb);
+        av0.resize(nb);
+        force0.resize(nb);
+        xfEnd.resize(nb);
+        slept.resize(nb);
+        ccd.resize(nb);
+        mover.resize(nb);
+        keys.resize(np);
+        man.resize(np);
+        carryPair.resize(np);
+        carryPoint.resize(4 * (size_t)np);
+        consPair.resize(np);
+        counts.resize(4);
+        colorStarts.resize(M3_GRAPH_COLORS + 2);
+        sxf.resize(sb);
+        slv.resize(sb);
+        sav.resize(sb);
+        sdp.resize(sb);
+        sdq.resize(sb);
+        ilv.resize(sb);
+        iav.resize(sb);
+        plv.resize(sb);
+        pav.resize(sb);
+        rdp.resize(sb);
+        rdq.resize(sb);
+        pass.resize((size_t)substeps * 2 * np);
+        fin.resize(np);
+        t.bodyCap = nb;
+        t.pairCap = np;
+        t.substeps = substeps;
+        t.xf0 = xf0.data();
+        t.lv0 = lv0.data();
+        t.av0 = av0.data();
+        t.force0 = force0.data();
+        t.xfEnd = xfEnd.data();
+        t.slept = slept.data();
+        t.ccd = ccd.data();
+        t.mover = mover.data();
+        t.keys = keys.data();
+        t.man = man.data();
+        t.carryPair = carryPair.data();
+        t.carryPoint = carryPoint.data();
+        t.consPair = consPair.data();
+        t.counts = counts.data();
+        t.colorStarts = colorStarts.data();
+        t.sxf = sxf.data();
+        t.slv = slv.data();
+        t.sav = sav.data();
+        t.sdp = sdp.data();
+        t.sdq = sdq.data();
+        t.ilv = ilv.data();
+        t.iav = iav.data();
+        t.plv = plv.data();
+        t.pav = pav.data();
+        t.rdp = rdp.data();
+        t.rdq = rdq.data();
+        t.pass = pass.data();
+        t.fin = fin.data();
+    }
+};
+
+int FirstShape(const tw::m3View& v, int b) { return v.shapeHead[b]; }
+
+void ApplyForces(m3gpuWorld* w, const std::vector<tw::m3Vec3>& F)
+{
+    tw::m3View& v = w->v;
+    for (int b = 0; b < v.maxBody; ++b)
+    {
+        if (v.bodyAlive[b] && v.type[b] == M3_DYNAMIC)
+        {
+            v.force[b] = F[b];
+        }
+    }
+}
+
+// Mass-weighted centre of mass of the dynamic bodies (double).
+void Com(const tw::m3View& v, double out[3], double* totalMass)
+{
+    double M = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+    for (int b = 0; b < v.maxBody; ++b)
+    {
+        if (!v.bodyAlive[b] || v.type[b] != M3_DYNAMIC || v.invMass[b] == 0.0f)
+        {
+            continue;
+        }
+        double m = 1.0 / (double)v.invMass[b];
+        tw::m3Vec3 r = tw::m3RotateVec3(v.xf[b].q, v.lc[b]);
+        cx += m * (v.xf[b].p.x + (double)r.x);
+        cy += m * (v.xf[b].p.y + (double)r.y);
+        cz += m * (v.xf[b].p.z + (double)r.z);
+        M += m;
+    }
+    out[0] = cx / M;
+    out[1] = cy / M;
+    out[2] = cz / M;
+    *totalMass = M;
+}
+
+int Adjoint(m3gpuWorld* w, const m3gpuStepParams* p, const m3gpuAdjointRequest* rq)
+{
+    tw::m3View& v = w->v;
+    const int nb = v.maxBody;
+    const int np = v.pairCapacity;
+    const int K = rq->checkpointEvery > 0 ? rq->checkpointEvery : 10;
+    const int steps = rq->steps;
+    // Originals to restore afterwards.
+    Checkpoint origin;
+    Save(w, &origin);
+    std::vector<R> invMass0(v.invMass, v.invMass + nb);
+    std::vector<tw::m3Mat3> invI0(v.invI, v.invI + nb), inertia0(v.inertia, v.inertia + nb);
+    std::vector<R> friction0(v.friction, v.friction + v.maxShape);
+    std::vector<tw::m3Vec3> F(nb, tw::m3V3(0, 0, 0));
+    if (rq->force != nullptr)
+    {
+        for (int b = 0; b < nb; ++b)
+        {
+            F[b] = tw::m3V3((R)rq->force[3 * b], (R)rq->force[3 * b + 1], (R)rq->force[3 * b + 2]);
+        }
+    }
+    // Parameter offset.
+    if (rq->offsetKind != 0)
+    {
+        int b = rq->offsetBody, k = rq->offsetComp;
+        R d = (R)rq->offsetValue;
+        R* comp = nullptr;
+        if (rq->offsetKind == 1)
+            comp = k == 0 ? &v.lv[b].x : (k == 1 ? &v.lv[b].y : &v.lv[b].z);
+        if (rq->offsetKind == 2)
+            comp = k == 0 ? &v.av[b].x : (k == 1 ? &v.av[b].y : &v.av[b].z);
+        if (rq->offsetKind == 5)
+            comp = k == 0 ? &F[b].x : (k == 1 ? &F[b].y : &F[b].z);
+        if (comp != nullptr)
+        {
+            *comp += d;
+        }
+        if (rq->offsetKind == 3)
+        {
+            R m = (R)1.0 / v.invMass[b];
+            R m2 = m + d;
+            v.invMass[b] = (R)1.0 / m2;
+            R si = m / m2, sf = m2 / m;
+            tw::m3Mat3& a = v.invI[b];
+            a.cx = tw::m3MulSV3(si, a.cx);
+            a.cy = tw::m3MulSV3(si, a.cy);
+            a.cz = tw::m3MulSV3(si, a.cz);
+            tw::m3Mat3& c = v.inertia[b];
+            c.cx = tw::m3MulSV3(sf, c.cx);
+            c.cy = tw::m3MulSV3(sf, c.cy);
+            c.cz = tw::m3MulSV3(sf, c.cz);
+        }
+        if (rq->offsetKind == 4)
+        {
+            v.friction[FirstShape(v, b)] += d;
+        }
+    }
+    // Forward with checkpoints.
+    int segs = (steps + K - 1) / K;
+    std::vector<Checkpoint> cps(segs);
+    int rc = 0;
+    for (int s = 0; s < steps && rc == 0; ++s)
+    {
+        if (s % K == 0)
+        {
+            Save(w, &cps[s / K]);
+        }
+        ApplyForces(w, F);
+        rc = Step(w, p, nullptr);
+    }
+    double C[3], M;
+    Com(v, C, &M);
+    double L = 0.0;
+    for (int k = 0; k < 3; ++k)
+    {
+        L += (C[k] - (double)rq->target[k]) * (C[k] - (double)rq->target[k]);
+    }
+    if (rq->loss != nullptr)
+    {
+        *rq->loss = L;
+    }
+    if (rc == 0 && !rq->forwardOnly)
+    {
+        std::vector<tw::m3BodyBar> body(nb);
+        std::vector<tw::m3ParamBar> param(nb);
+        std::vector<tw::m3ConBar> cons(np);
+        std::vector<tw::m3ManBar> store(np), storePrev(np);
+        std::vector<tw::m3Constraint> prep(np);
+        std::vector<tw::m3Vec3> ccdP(nb);
+        std::vector<tw::m3Quat> ccdQ(nb);
+        memset(body.data(), 0, sizeof(tw::m3BodyBar) * nb);
+        memset(param.data(), 0, sizeof(tw::m3ParamBar) * nb);
+        memset(store.data(), 0, sizeof(tw::m3ManBar) * np);
+        R Cb[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            Cb[k] = (R)(2.0 * (C[k] - (double)rq->target[k]));
+        }
+        std::vector<R> comBar(nb, (R)0);
+        for (int b = 0; b < nb; ++b)
+        {
+            if (!v.bodyAlive[b] || v.type[b] != M3_DYNAMIC)
+                continue;
+            R m = (R)1.0 / v.invMass[b];
+            tw::m3Vec3 cbar = tw::m3V3(Cb[0] * m / (R)M, Cb[1] * m / (R)M, Cb[2] * m / (R)M);
+            body[b].p = cbar;
+            tw::m3AdjRotate(v.xf[b].q, v.lc[b], cbar, &body[b].q);
+            tw::m3Vec3 r = tw::m3RotateVec3(v.xf[b].q, v.lc[b]);
+            double cx = v.xf[b].p.x + (double)r.x - C[0];
+            double cy = v.xf[b].p.y + (double)r.y - C[1];
+            double cz = v.xf[b].p.z + (double)r.z - C[2];
+            comBar[b] = (R)((Cb[0] * cx + Cb[1] * cy + Cb[2] * cz) / M);
+        }
+        tw::m3AdjWorld aw;
+        aw.body = body.data();
+        aw.param = param.data();
+        aw.cons = cons.data();
+        aw.store = store.data();
+        aw.storePrev = storePrev.data();
+        aw.prep = prep.data();
+        aw.ccdP = ccdP.data();
+        aw.ccdQ = ccdQ.data();
+        aw.unsupported = 0;
+        aw.ccdPulled = 0;
+        tw::m3SerialParams sp = Params(p);
+        std::vector<TapeStore> tapes(K);
+        for (int k = 0; k < K; ++k)
+        {
+            tapes[k].Init(nb, np, p->substeps);
+        }
+        for (int seg = segs - 1; seg >= 0; --seg)
+        {
+            Load(w, &cps[seg]);
+            int s0 = seg * K;
+            int s1 = s0 + K < steps ? s0 + K : steps;
+            for (int s = s0; s < s1; ++s)
+            {
+                ApplyForces(w, F);
+                Step(w, p, &tapes[s - s0].t);
+            }
+            for (int s = s1 - 1; s >= s0; --s)
+            {
+                tw::m3AdjStep(&v, &sp, &tapes[s - s0].t, &aw);
+                std::swap(aw.store, aw.storePrev);
+            }
+        }
+        for (int b = 0; b < nb; ++b)
+        {
+            bool dyn = v.bodyAlive[b] && v.type[b] == M3_DYNAMIC;
+            R m = dyn ? (R)1.0 / v.invMass[b] : (R)0;
+            R gm = dyn ? param[b].invMass * (-v.invMass[b] * v.invMass[b]) + param[b].inertia / m -
+                             param[b].invI / m + comBar[b]
+                       : (R)0;
+            if (rq->gradLinearVelocity)
+            {
+                rq->gradLinearVelocity[3 * b] = (float)body[b].v.x;
+                rq->gradLinearVelocity[3 * b + 1] = (float)body[b].v.y;
+                rq->gradLinearVelocity[3 * b + 2] = (float)body[b].v.z;
+            }
+            if (rq->gradAngularVelocity)
+            {
+                rq->gradAngularVelocity[3 * b] = (float)body[b].w.x;
+                rq->gradAngularVelocity[3 * b + 1] = (float)body[b].w.y;
+                rq->gradAngularVelocity[3 * b + 2] = (float)body[b].w.z;
+            }
+            if (rq->gradMass)
+                rq->gradMass[b] = (float)gm;
+            if (rq->gradFriction)
+                rq->gradFriction[b] = (float)param[b].friction;
+            if (rq->gradForce)
+            {
+                rq->gradForce[3 * b] = (float)param[b].force.x;
+                rq->gradForce[3 * b + 1] = (float)param[b].force.y;
+                rq->gradForce[3 * b + 2] = (float)param[b].force.z;
+            }
+        }
+        if (rq->unsupported)
+            *rq->unsupported = aw.unsupported;
+        if (rq->ccdPulled)
+            *rq->ccdPulled = aw.ccdPulled;
+    }
+    // Restore.
+    Load(w, &origin);
+    std::copy(invMass0.begin(), invMass0.end(), v.invMass);
+    std::copy(invI0.begin(), invI0.end(), v.invI);
+    std::copy(inertia0.begin(), inertia0.end(), v.inertia);
+    std::copy(friction0.begin(), friction0.end(), v.friction);
+    return rc;
+}
+} // namespace
+
+extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* ps,
+                                  const m3gpuAdjointRequest* rqs, int32_t count)
+{
+    int rc = 0;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        rc |= Adjoint(ws[k], &ps[k], &rqs[k]);
+    }
+    return rc;
+}
+
+// The FP64 state itself, for checks that read it without rounding.
+extern "C" int m3twin_body_state(m3gpuWorld* w, int32_t b, double out[13])
+{
+    tw::m3View& v = w->v;
+    out[0] = v.xf[b].p.x;
+    out[1] = v.xf[b].p.y;
+    out[2] = v.xf[b].p.z;
+    out[3] = (double)v.xf[b].q.x;
+    out[4] = (double)v.xf[b].q.y;
+    out[5] = (double)v.xf[b].q.z;
+    out[6] = (double)v.xf[b].q.w;
+    out[7] = (double)v.lv[b].x;
+    out[8] = (double)v.lv[b].y;
+    out[9] = (double)v.lv[b].z;
+    out[10] = (double)v.av[b].x;
+    out[11] = (double)v.av[b].y;
+    out[12] = (double)v.av[b].z;
     return 0;
 }
