@@ -228,8 +228,17 @@ M3_HD void m3AdjSolveContact(const m3Constraint* c, const m3TapePass* rec, m3Vec
     wB[0] = rec->wB;
     m3real passNormal = 0.0f;
     m3real twistLimit = 0.0f;
-    for (int32_t k = 0; k < np; ++k)
+#pragma unroll
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
+        if (k >= np)
+        {
+            vA[k + 1] = vA[k];
+            wA[k + 1] = wA[k];
+            vB[k + 1] = vB[k];
+            wB[k + 1] = wB[k];
+            continue;
+        }
         const m3CPoint* cp = &c->points[k];
         m3Vec3 rsA = m3RotateVec3(dqA, cp->rA);
         m3Vec3 rsB = m3RotateVec3(dqB, cp->rB);
@@ -292,7 +301,8 @@ M3_HD void m3AdjSolveContact(const m3Constraint* c, const m3TapePass* rec, m3Vec
     if (!useBias)
     {
         // Twist and friction, forward values.
-        m3Vec3 vAt = vA[np], wAt = wA[np], vBt = vB[np], wBt = wB[np];
+        m3Vec3 vAt = vA[M3_MANIFOLD_MAX_POINTS], wAt = wA[M3_MANIFOLD_MAX_POINTS];
+        m3Vec3 vBt = vB[M3_MANIFOLD_MAX_POINTS], wBt = wB[M3_MANIFOLD_MAX_POINTS];
         m3Vec3 wRel = m3Sub3(wBt, wAt);
         m3real ts = m3Dot3(c->normal, wRel);
         m3real maxTwist = c->friction * twistLimit;
@@ -421,8 +431,13 @@ M3_HD void m3AdjSolveContact(const m3Constraint* c, const m3TapePass* rec, m3Vec
         cb->twist = oldTb;
     }
     // Normal points in reverse.
-    for (int32_t k = np - 1; k >= 0; --k)
+#pragma unroll
+    for (int32_t k = M3_MANIFOLD_MAX_POINTS - 1; k >= 0; --k)
     {
+        if (k >= np)
+        {
+            continue;
+        }
         const m3CPoint* cp = &c->points[k];
         m3real d = deltaK[k];
         m3Vec3 P = m3MulSV3(d, c->normal);
@@ -966,12 +981,40 @@ M3_HD void m3AdjSegmentClosest(m3Vec3 p1, m3Vec3 d1, m3Vec3 p2, m3Vec3 d2, m3Vec
 #define M3_ADJ_CLIP_PASSES 8
 #define M3_ADJ_CLIP_POINTS 16
 
-// Hull against hull in A's frame (m3CollideHullsQ) in reverse: given the
-// bars of the local manifold (normal, anchors, separations), adds the bars
-// of (q, p), B's pose in A's frame.  Returns 0 when the clip is larger than
-// this reverse keeps (the caller counts the pair as unsupported).
+// How a hull pair's local manifold was lifted to world (m3CollidePairHulls):
+// normal = R(qA) n, anchorA = R(qA) aA - R(qA) lcA,
+// anchorB = pA + R(qA) aB - pB - R(qB) lcB.  `mb` holds the world bars; the
+// lift's own bars add into the body bars.
+typedef struct m3HullLift
+{
+    m3Quat qA, qB;
+    m3Vec3 lcA, lcB;
+    const m3ManBar* mb;
+    m3Quat* qbA;
+    m3Quat* qbB;
+    m3Vec3* pbA;
+    m3Vec3* pbB;
+} m3HullLift;
+
+// Bars of one local point from the world bars through the lift.
+M3_HD void m3AdjLiftPoint(const m3HullLift* L, int32_t j, m3Vec3 aA, m3Vec3 aB, m3Vec3* aAb, m3Vec3* aBb)
+{
+    const m3ManBar* mb = L->mb;
+    *aAb = m3AdjRotate(L->qA, aA, mb->anchorA[j], L->qbA);
+    m3AdjRotate(L->qA, L->lcA, m3Neg3(mb->anchorA[j]), L->qbA);
+    *aBb = m3AdjRotate(L->qA, aB, mb->anchorB[j], L->qbA);
+    m3Acc3(L->pbA, mb->anchorB[j]);
+    m3Acc3(L->pbB, m3Neg3(mb->anchorB[j]));
+    m3AdjRotate(L->qB, L->lcB, m3Neg3(mb->anchorB[j]), L->qbB);
+}
+
+// Hull against hull in A's frame (m3CollideHullsQ), evaluated once with its
+// clip operations kept, then in reverse: the world bars come back through
+// the lift to local bars, and from those to the bars of (q, p), B's pose in
+// A's frame.  Returns 0 when the clip is larger than this reverse keeps or
+// the point count disagrees (the caller counts the pair as unsupported).
 M3_HD int m3AdjCollideHulls(const m3HullData* hullA, const m3HullData* hullB, m3Quat q, m3Vec3 p,
-                            const m3Manifold* local, const m3ManBar* lb, m3Quat* qb, m3Vec3* pb)
+                            int32_t pointCount, const m3HullLift* lift, m3Quat* qb, m3Vec3* pb)
 {
     m3FaceQuery faceA = m3QueryFaces(hullA, hullB, q, p, 1);
     m3FaceQuery faceB = m3QueryFaces(hullB, hullA, q, p, 0);
@@ -993,6 +1036,19 @@ M3_HD int m3AdjCollideHulls(const m3HullData* hullA, const m3HullData* hullB, m3
         m3Vec3 pBv = m3Add3(m3RotateVec3(q, vB0), p);
         m3Vec3 qBv = m3Add3(m3RotateVec3(q, vB1), p);
         m3Vec3 dB = m3Sub3(qBv, pBv);
+        if (pointCount != 1)
+        {
+            return 0;
+        }
+        m3Vec3 cA;
+        m3Vec3 cB;
+        m3SegmentClosest(pA, dA, pBv, dB, &cA, &cB);
+        m3ManBar lbv;
+        memset(&lbv, 0, sizeof(lbv));
+        m3ManBar* lb = &lbv;
+        lb->normal = m3AdjRotate(lift->qA, edge.axis, lift->mb->normal, lift->qbA);
+        lb->separation[0] = lift->mb->separation[0];
+        m3AdjLiftPoint(lift, 0, cA, cB, &lb->anchorA[0], &lb->anchorB[0]);
         m3Vec3 pBb = m3Z3();
         m3Vec3 dBb = m3Z3();
         m3AdjSegmentClosest(pA, dA, pBv, dB, lb->anchorA[0], lb->anchorB[0], &pBb, &dBb);
@@ -1147,12 +1203,15 @@ M3_HD int m3AdjCollideHulls(const m3HullData* hullA, const m3HullData* hullB, m3
         polyb[k] = m3Z3();
     }
     m3Vec3 outN = refIsA ? refN : m3Neg3(m3RotateVec3(q, refN));
+    m3ManBar lbv;
+    memset(&lbv, 0, sizeof(lbv));
+    m3ManBar* lb = &lbv;
+    lb->normal = m3AdjRotate(lift->qA, outN, lift->mb->normal, lift->qbA);
     if (!refIsA)
     {
         // normal = -R(q) refN
         m3AdjRotate(q, refN, m3Neg3(lb->normal), qb);
     }
-    (void)outN;
     // The forward's selection: points below the margin, the deepest four
     // (ties to the lower polygon id), emitted in ascending id order.
     int32_t candIdx[M3_ADJ_CLIP_POINTS];
@@ -1201,16 +1260,20 @@ M3_HD int m3AdjCollideHulls(const m3HullData* hullA, const m3HullData* hullB, m3
             }
         }
     }
-    if (want != local->pointCount)
+    if (want != pointCount)
     {
         return 0;
     }
-    for (int32_t j = 0; j < local->pointCount; ++j)
+    for (int32_t j = 0; j < pointCount; ++j)
     {
         int32_t kk = candIdx[kept[j]];
         m3Vec3 onInc = poly[passes][kk];
         m3real sep = m3Dot3(refN, onInc) - refOff;
         m3Vec3 onRef = m3Sub3(onInc, m3MulSV3(sep, refN));
+        m3Vec3 wInc = refIsA ? onInc : m3Add3(m3RotateVec3(q, onInc), p);
+        m3Vec3 wRef = refIsA ? onRef : m3Add3(m3RotateVec3(q, onRef), p);
+        lb->separation[j] = lift->mb->separation[j];
+        m3AdjLiftPoint(lift, j, refIsA ? wRef : wInc, refIsA ? wInc : wRef, &lb->anchorA[j], &lb->anchorB[j]);
         m3Vec3 wIncb = refIsA ? lb->anchorB[j] : lb->anchorA[j];
         m3Vec3 wRefb = refIsA ? lb->anchorA[j] : lb->anchorB[j];
         m3Vec3 onIncb, onRefb;
@@ -1298,27 +1361,19 @@ M3_HD int m3AdjPairHulls(const m3View* v, int32_t shapeA, int32_t shapeB, const 
     m3Transform xfB = m3ShapeXf(v, shapeB);
     const m3HullData* hA = &v->hulls[v->hullIndex[shapeA]];
     const m3HullData* hB = &v->hulls[v->hullIndex[shapeB]];
-    m3Manifold local = m3CollideHulls(hA, hB, qRel, pRel);
-    // World lift: normal = R(qA) nl; anchorA = R(qA) aAl - R(qA) lcA;
-    // anchorB = pA + R(qA) aBl - pB - R(qB) lcB.
-    m3ManBar lb;
-    memset(&lb, 0, sizeof(lb));
-    lb.normal = m3AdjRotate(xfA.q, local.normal, mb->normal, qbA);
-    m3Vec3 lcA = v->lc[bodyA];
-    m3Vec3 lcB = v->lc[bodyB];
-    for (int32_t k = 0; k < local.pointCount; ++k)
-    {
-        lb.separation[k] = mb->separation[k];
-        lb.anchorA[k] = m3AdjRotate(xfA.q, local.points[k].anchorA, mb->anchorA[k], qbA);
-        m3AdjRotate(xfA.q, lcA, m3Neg3(mb->anchorA[k]), qbA);
-        lb.anchorB[k] = m3AdjRotate(xfA.q, local.points[k].anchorB, mb->anchorB[k], qbA);
-        m3Acc3(pbA, mb->anchorB[k]);
-        m3Acc3(pbB, m3Neg3(mb->anchorB[k]));
-        m3AdjRotate(xfB.q, lcB, m3Neg3(mb->anchorB[k]), qbB);
-    }
+    m3HullLift lift;
+    lift.qA = xfA.q;
+    lift.qB = xfB.q;
+    lift.lcA = v->lc[bodyA];
+    lift.lcB = v->lc[bodyB];
+    lift.mb = mb;
+    lift.qbA = qbA;
+    lift.qbB = qbB;
+    lift.pbA = pbA;
+    lift.pbB = pbB;
     m3Quat qRelb = m3ZQ();
     m3Vec3 pRelb = m3Z3();
-    if (!m3AdjCollideHulls(hA, hB, qRel, pRel, &local, &lb, &qRelb, &pRelb))
+    if (!m3AdjCollideHulls(hA, hB, qRel, pRel, m->pointCount, &lift, &qRelb, &pRelb))
     {
         return 0;
     }
@@ -1540,38 +1595,83 @@ M3_HD int m3AdjContinuous(const m3View* v, int32_t i, m3Transform xf0, m3Transfo
     return 1;
 }
 
-// ---- one step in reverse -----------------------------------------------------
+// A constraint's contributions to its bodies' bars from prepare and the
+// narrowphase, gathered per body afterwards (no two lanes add into one body).
+typedef struct m3ConContrib
+{
+    m3Vec3 pA, pB;
+    m3Quat qA, qB;
+    m3ParamBar parA, parB;
+} m3ConContrib;
 
 // Reverse-pass context of one world.
 typedef struct m3AdjWorld
 {
-    m3BodyBar* body;     // [bodyCap]
-    m3ParamBar* param;   // [bodyCap]
-    m3ConBar* cons;      // [pairCap] scratch
-    m3ManBar* store;     // [pairCap] bars of this step's stored impulses (by pair)
-    m3ManBar* storePrev; // [pairCap] bars of the previous step's stored impulses
-    m3Constraint* prep;  // [pairCap] scratch: constraints re-prepared
-    m3Vec3* ccdP;        // [bodyCap] scratch: step-start pose bars of pull-backs
+    m3BodyBar* body;       // [bodyCap]
+    m3ParamBar* param;     // [bodyCap]
+    m3ConBar* cons;        // [pairCap] scratch
+    m3ManBar* store;       // [pairCap] bars of this step's stored impulses (by pair)
+    m3ManBar* storePrev;   // [pairCap] bars of the previous step's stored impulses
+    m3Constraint* prep;    // [pairCap] scratch: constraints re-prepared
+    m3ConContrib* contrib; // [pairCap] scratch
+    m3Vec3* ccdP;          // [bodyCap] scratch: step-start pose bars of pull-backs
     m3Quat* ccdQ;
     int32_t unsupported; // pairs or bodies outside the reverse's scope
     int32_t ccdPulled;   // pulled-back bodies passed through
 } m3AdjWorld;
 
-// Reverse of the step recorded in `tape`.  `v` is the world's view with its
-// static fields; its xf/lv/av/keys/manifolds are redirected to the tape's.
-// On entry aw->body holds the bars of the step's final state and aw->store
-// the bars of the impulses it stored; on return aw->body holds the bars of
-// its initial state and aw->storePrev those of the previous step's stored
-// impulses (cleared first).
-M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* tape, m3AdjWorld* aw)
+M3_HD void m3AccParam(m3ParamBar* a, const m3ParamBar* b)
 {
-    m3View vv = *vin;
-    m3View* v = &vv;
+    a->mass += b->mass;
+    a->inertia += b->inertia;
+    a->invI += b->invI;
+    a->invMass += b->invMass;
+    a->force = m3Add3(a->force, b->force);
+    a->friction += b->friction;
+}
+
+// Reverse of one constraint's solve in pass `mode` of substep `sub`, with
+// lane-private bars standing in for static sides.
+M3_HD void m3AdjSolveSlot(const m3View* v, const m3Tape* tape, m3AdjWorld* aw, int32_t sub, int32_t mode,
+                          int32_t k, m3real invH, m3real pushMax)
+{
+    const m3Constraint* c = &aw->prep[k];
+    const m3TapePass* rec = &tape->pass[((int64_t)sub * 2 + (mode - 1)) * tape->pairCap + k];
+    const m3Vec3* dpArr = mode == 2 ? tape->rdp : tape->sdp;
+    const m3Quat* dqArr = mode == 2 ? tape->rdq : tape->sdq;
+    const int32_t base = sub * tape->bodyCap;
+    m3BodyBar dummyA, dummyB;
+    memset(&dummyA, 0, sizeof(dummyA));
+    memset(&dummyB, 0, sizeof(dummyB));
+    m3BodyBar* A = c->dynA ? &aw->body[c->bodyA] : &dummyA;
+    m3BodyBar* B = c->dynB ? &aw->body[c->bodyB] : &dummyB;
+    m3AdjSolveContact(c, rec, dpArr[base + c->bodyA], dqArr[base + c->bodyA], dpArr[base + c->bodyB],
+                      dqArr[base + c->bodyB], invH, pushMax, mode == 1, A, B, &aw->cons[k]);
+    (void)v;
+}
+
+// Reverse of the step recorded in `tape`, phase by phase.  Lane `lane` of
+// `lanes` takes every lanes-th item of each phase and `sync` separates the
+// phases (a workgroup barrier on the device; nothing for one lane).  `v`
+// is the world's view with its pose, velocities and pairs pointed at the
+// tape's (m3AdjTapeView).  On entry aw->body holds the bars of the step's
+// final state and aw->store those of the impulses it stored; on return
+// aw->body holds the bars of its initial state and aw->storePrev those of
+// the previous step's stored impulses.
+// The world's view with the step's recorded pose, velocities and pairs.
+M3_HD void m3AdjTapeView(m3View* v, const m3Tape* tape)
+{
     v->xf = tape->xf0;
     v->lv = tape->lv0;
     v->av = tape->av0;
     v->keys = tape->keys;
     v->manifolds = tape->man;
+}
+
+template <typename Sync>
+M3_HD void m3AdjStepPar(const m3View* v, const m3SerialParams* p, const m3Tape* tape, m3AdjWorld* aw,
+                        int32_t lane, int32_t lanes, Sync sync)
+{
     const int32_t nb = v->maxBody;
     const int32_t pairCount = tape->counts[0];
     const int32_t nc = tape->counts[1];
@@ -1579,13 +1679,15 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
     m3real invH = h > 0.0f ? 1.0f / h : 0.0f;
     m3Softness soft = m3MakeSoft(p->contactHertz, p->contactDampingRatio, h);
     m3Softness staticSoft = m3MakeSoft(2.0f * p->contactHertz, p->contactDampingRatio, h);
-    for (int32_t i = 0; i < pairCount; ++i)
+    const int32_t* cs = tape->colorStarts;
+    // storePrev is indexed by the previous step's pairs: clear it whole.
+    for (int32_t i = lane; i < tape->pairCap; i += lanes)
     {
         memset(&aw->storePrev[i], 0, sizeof(m3ManBar));
     }
     // Sleep froze velocities to zero; the continuous pass pulled poses back
     // along the step's sweep.
-    for (int32_t b = 0; b < nb; ++b)
+    for (int32_t b = lane; b < nb; b += lanes)
     {
         if (tape->slept[b])
         {
@@ -1605,7 +1707,7 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
         aw->body[b].dq = m3ZQ();
     }
     // Re-prepare and seed the constraint bars from the stored impulses.
-    for (int32_t k = 0; k < nc; ++k)
+    for (int32_t k = lane; k < nc; k += lanes)
     {
         int32_t i = tape->consPair[k];
         m3PrepareContact(v, i, &aw->prep[k], soft, staticSoft);
@@ -1629,27 +1731,37 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
             aw->unsupported += 1;
         }
     }
+    sync();
     const int32_t B = tape->bodyCap;
     for (int32_t sub = p->substeps - 1; sub >= 0; --sub)
     {
         const int32_t base = sub * B;
         for (int32_t mode = 2; mode >= 1; --mode)
         {
-            for (int32_t k = nc - 1; k >= 0; --k)
+            // Overflow color first (serial in the forward), then 15..0.
+            if (lane == 0)
             {
-                const m3Constraint* c = &aw->prep[k];
-                const m3TapePass* rec = &tape->pass[((int64_t)sub * 2 + (mode - 1)) * tape->pairCap + k];
-                const m3Vec3* dpArr = mode == 2 ? tape->rdp : tape->sdp;
-                const m3Quat* dqArr = mode == 2 ? tape->rdq : tape->sdq;
-                m3BodyBar* A = &aw->body[c->bodyA];
-                m3BodyBar* Bb = &aw->body[c->bodyB];
-                m3AdjSolveContact(c, rec, dpArr[base + c->bodyA], dqArr[base + c->bodyA],
-                                  dpArr[base + c->bodyB], dqArr[base + c->bodyB], invH,
-                                  p->contactPushMaxSpeed, mode == 1, A, Bb, &aw->cons[k]);
+                for (int32_t k = cs[M3_GRAPH_COLORS + 1] - 1; k >= cs[M3_GRAPH_COLORS]; --k)
+                {
+                    m3AdjSolveSlot(v, tape, aw, sub, mode, k, invH, p->contactPushMaxSpeed);
+                }
+            }
+            sync();
+            for (int32_t color = M3_GRAPH_COLORS - 1; color >= 0; --color)
+            {
+                if (cs[color + 1] == cs[color])
+                {
+                    continue;
+                }
+                for (int32_t k = cs[color] + lane; k < cs[color + 1]; k += lanes)
+                {
+                    m3AdjSolveSlot(v, tape, aw, sub, mode, k, invH, p->contactPushMaxSpeed);
+                }
+                sync();
             }
             if (mode == 2)
             {
-                for (int32_t b = 0; b < nb; ++b)
+                for (int32_t b = lane; b < nb; b += lanes)
                 {
                     if (!tape->mover[b])
                     {
@@ -1658,15 +1770,17 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
                     m3AdjIntegratePosition(v, b, tape->sxf[base + b], tape->plv[base + b],
                                            tape->pav[base + b], tape->sdq[base + b], h, &aw->body[b]);
                 }
+                sync();
             }
         }
-        for (int32_t k = 0; k < nc; ++k)
+        for (int32_t k = lane; k < nc; k += lanes)
         {
             const m3Constraint* c = &aw->prep[k];
             const m3TapePass* rec = &tape->pass[((int64_t)sub * 2) * tape->pairCap + k];
             m3AdjWarmStart(c, rec, &aw->body[c->bodyA], &aw->body[c->bodyB], &aw->cons[k]);
         }
-        for (int32_t b = 0; b < nb; ++b)
+        sync();
+        for (int32_t b = lane; b < nb; b += lanes)
         {
             if (!tape->mover[b] || v->type[b] != M3_DYNAMIC)
             {
@@ -1675,9 +1789,10 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
             m3AdjIntegrateVelocity(v, b, tape->sxf[base + b], tape->slv[base + b], tape->sav[base + b],
                                    tape->force0[b], h, &aw->body[b], &aw->param[b]);
         }
+        sync();
     }
-    // Prepare and narrowphase in reverse.
-    for (int32_t k = 0; k < nc; ++k)
+    // Prepare and narrowphase in reverse, per constraint.
+    for (int32_t k = lane; k < nc; k += lanes)
     {
         int32_t i = tape->consPair[k];
         const m3Constraint* c = &aw->prep[k];
@@ -1685,25 +1800,23 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
         uint64_t key = tape->keys[i];
         int32_t sA = (int32_t)(key >> 32);
         int32_t sB = (int32_t)(key & 0xFFFFFFFFu);
+        m3ConContrib* o = &aw->contrib[k];
+        memset(o, 0, sizeof(*o));
         m3ManBar mb;
         memset(&mb, 0, sizeof(mb));
-        m3AdjPrepare(v, m, c, &aw->cons[k], &mb, &aw->body[c->bodyA].q, &aw->body[c->bodyB].q,
-                     &aw->param[c->bodyA], &aw->param[c->bodyB], sA, sB);
+        m3AdjPrepare(v, m, c, &aw->cons[k], &mb, &o->qA, &o->qB, &o->parA, &o->parB, sA, sB);
         uint8_t tA = v->shapeType[sA];
         uint8_t tB = v->shapeType[sB];
-        int32_t bA = v->shapeBody[sA];
-        int32_t bB = v->shapeBody[sB];
         if ((tA == M3_PLANE) != (tB == M3_PLANE) && (tA == M3_HULL || tB == M3_HULL))
         {
             int32_t plane = tA == M3_PLANE ? sA : sB;
             int32_t hull = tA == M3_PLANE ? sB : sA;
-            int32_t hb = v->shapeBody[hull];
-            m3AdjPlaneHull(v, sA, plane, hull, m, &mb, &aw->body[hb].p, &aw->body[hb].q);
+            int hullIsA = hull == sA;
+            m3AdjPlaneHull(v, sA, plane, hull, m, &mb, hullIsA ? &o->pA : &o->pB, hullIsA ? &o->qA : &o->qB);
         }
         else if (tA == M3_HULL && tB == M3_HULL)
         {
-            if (!m3AdjPairHulls(v, sA, sB, m, &mb, &aw->body[bA].p, &aw->body[bA].q, &aw->body[bB].p,
-                                &aw->body[bB].q))
+            if (!m3AdjPairHulls(v, sA, sB, m, &mb, &o->pA, &o->qA, &o->pB, &o->qB))
             {
                 aw->unsupported += 1;
             }
@@ -1712,7 +1825,8 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
         {
             aw->unsupported += 1;
         }
-        // Carried impulses flow to the previous step's stored values.
+        // Carried impulses flow to the previous step's stored values (pair
+        // keys are unique, so no two pairs share a previous pair).
         int32_t prev = tape->carryPair[i];
         if (prev >= 0)
         {
@@ -1721,22 +1835,17 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
             pbar->twistImpulse += mb.twistImpulse;
             for (int32_t q = 0; q < m->pointCount; ++q)
             {
-                int32_t o = tape->carryPoint[4 * i + q];
-                if (o >= 0)
+                int32_t o2 = tape->carryPoint[4 * i + q];
+                if (o2 >= 0)
                 {
-                    pbar->normalImpulse[o] += mb.normalImpulse[q];
+                    pbar->normalImpulse[o2] += mb.normalImpulse[q];
                 }
             }
         }
     }
-    for (int32_t b = 0; b < nb; ++b)
-    {
-        m3Acc3(&aw->body[b].p, aw->ccdP[b]);
-        m3AccQ(&aw->body[b].q, aw->ccdQ[b]);
-    }
     // Pairs that touched but were not solved (both sides asleep) keep their
     // stored impulses through the carry unchanged.
-    for (int32_t i = 0; i < pairCount; ++i)
+    for (int32_t i = lane; i < pairCount; i += lanes)
     {
         int32_t prev = tape->carryPair[i];
         const m3Manifold* m = &tape->man[i];
@@ -1763,13 +1872,58 @@ M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* t
         pbar->twistImpulse += sb->twistImpulse;
         for (int32_t q = 0; q < m->pointCount; ++q)
         {
-            int32_t o = tape->carryPoint[4 * i + q];
-            if (o >= 0)
+            int32_t o2 = tape->carryPoint[4 * i + q];
+            if (o2 >= 0)
             {
-                pbar->normalImpulse[o] += sb->normalImpulse[q];
+                pbar->normalImpulse[o2] += sb->normalImpulse[q];
             }
         }
     }
+    sync();
+    // Gather each body's contributions in constraint order.
+    for (int32_t b = lane; b < nb; b += lanes)
+    {
+        m3BodyBar* bar = &aw->body[b];
+        for (int32_t k = 0; k < nc; ++k)
+        {
+            const m3Constraint* c = &aw->prep[k];
+            const m3ConContrib* o = &aw->contrib[k];
+            if (c->bodyA == b)
+            {
+                m3Acc3(&bar->p, o->pA);
+                m3AccQ(&bar->q, o->qA);
+                m3AccParam(&aw->param[b], &o->parA);
+            }
+            if (c->bodyB == b)
+            {
+                m3Acc3(&bar->p, o->pB);
+                m3AccQ(&bar->q, o->qB);
+                m3AccParam(&aw->param[b], &o->parB);
+            }
+        }
+        m3Acc3(&bar->p, aw->ccdP[b]);
+        m3AccQ(&bar->q, aw->ccdQ[b]);
+    }
+    sync();
+}
+
+#if defined(__HIPCC__)
+#define M3_HD_MEMBER __host__ __device__
+#else
+#define M3_HD_MEMBER
+#endif
+
+typedef struct m3NoSync
+{
+    M3_HD_MEMBER void operator()() const {}
+} m3NoSync;
+
+// The step reverse on one thread.
+M3_HD void m3AdjStep(const m3View* vin, const m3SerialParams* p, const m3Tape* tape, m3AdjWorld* aw)
+{
+    m3View v = *vin;
+    m3AdjTapeView(&v, tape);
+    m3AdjStepPar(&v, p, tape, aw, 0, 1, m3NoSync());
 }
 
 #endif // M3_ADJOINT_H

@@ -61,6 +61,7 @@ struct m3Dev
     uint16_t* bodyMasks;
     int32_t* bodyOwner;
     uint32_t* barrier; // substep grid barrier: arrival count, generation
+    m3Tape* tape;      // intermediate-value record of this step (adjoint replays), or null
 };
 
 struct m3gpuWorld
@@ -1485,9 +1486,25 @@ __device__ static inline void GridBarrier(uint32_t* bar, int32_t* fault)
 
 // One graph color of one pass over the grid (mode 1 solve with bias,
 // 2 relax); the overflow color runs serially on one thread.
+// The velocities and accumulators a constraint solve starts from.
+__device__ static inline void TapeSolve(const m3View& v, const m3Constraint* c, m3TapePass* r)
+{
+    r->vA = v.lv[c->bodyA];
+    r->wA = v.av[c->bodyA];
+    r->vB = v.lv[c->bodyB];
+    r->wB = v.av[c->bodyB];
+    for (int32_t q = 0; q < M3_MANIFOLD_MAX_POINTS; ++q)
+    {
+        r->lambda[q] = q < c->pointCount ? c->points[q].normalImpulse : 0.0f;
+    }
+    r->f1 = c->frictionImpulse1;
+    r->f2 = c->frictionImpulse2;
+    r->twist = c->twistImpulse;
+}
+
 __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t end, int mode,
                                          const m3SubstepParams& sp, int32_t gid, int32_t gsz,
-                                         bool serial)
+                                         bool serial, m3TapePass* rec)
 {
     if (serial)
     {
@@ -1495,6 +1512,10 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
         {
             for (int32_t k = start; k < end; ++k)
             {
+                if (rec != nullptr)
+                {
+                    TapeSolve(v, &v.cons[k], &rec[k]);
+                }
                 m3SolveContact(&v, &v.cons[k], sp.invH, sp.push, mode == 1);
             }
         }
@@ -1502,6 +1523,10 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
     }
     for (int32_t k = start + gid; k < end; k += gsz)
     {
+        if (rec != nullptr)
+        {
+            TapeSolve(v, &v.cons[k], &rec[k]);
+        }
         m3SolveContact(&v, &v.cons[k], sp.invH, sp.push, mode == 1);
     }
 }
@@ -1530,8 +1555,9 @@ struct m3BlockSync
 
 template <typename Sync>
 __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams& sp, int32_t gid,
-                                          int32_t gsz, Sync sync)
+                                          int32_t gsz, Sync sync, m3Tape* tape = nullptr)
 {
+    const int32_t nb = v.maxBody;
     int32_t starts[M3_GRAPH_COLORS + 2];
     for (int32_t k = 0; k < M3_GRAPH_COLORS + 2; ++k)
     {
@@ -1545,6 +1571,19 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
     const int32_t comps = v.jointCompCount;
     for (int32_t sub = 0; sub < sp.substeps; ++sub)
     {
+        if (tape != nullptr)
+        {
+            const int32_t base = sub * tape->bodyCap;
+            for (int32_t b = gid; b < nb; b += gsz)
+            {
+                tape->sxf[base + b] = v.xf[b];
+                tape->slv[base + b] = v.lv[b];
+                tape->sav[base + b] = v.av[b];
+                tape->sdp[base + b] = v.deltaPos[b];
+                tape->sdq[base + b] = v.deltaRot[b];
+            }
+            sync();
+        }
         for (int32_t m = gid; m < movers; m += gsz)
         {
             int32_t b = v.movers[m];
@@ -1573,11 +1612,30 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
         {
             if (mode == 2)
             {
+                if (tape != nullptr)
+                {
+                    const int32_t base = sub * tape->bodyCap;
+                    for (int32_t b = gid; b < nb; b += gsz)
+                    {
+                        tape->plv[base + b] = v.lv[b];
+                        tape->pav[base + b] = v.av[b];
+                    }
+                    sync();
+                }
                 for (int32_t m = gid; m < movers; m += gsz)
                 {
                     m3IntegratePosition(&v, v.movers[m], sp.h);
                 }
                 sync();
+                if (tape != nullptr)
+                {
+                    const int32_t base = sub * tape->bodyCap;
+                    for (int32_t b = gid; b < nb; b += gsz)
+                    {
+                        tape->rdp[base + b] = v.deltaPos[b];
+                        tape->rdq[base + b] = v.deltaRot[b];
+                    }
+                }
             }
             if (joints)
             {
@@ -1595,9 +1653,23 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 {
                     continue;
                 }
-                ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS);
+                m3TapePass* rec =
+                    tape != nullptr ? tape->pass + ((int64_t)sub * 2 + (mode - 1)) * tape->pairCap : nullptr;
+                ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS, rec);
                 sync();
             }
+        }
+    }
+    if (tape != nullptr)
+    {
+        const int32_t nc = v.sc->constraintCount;
+        for (int32_t k = gid; k < nc; k += gsz)
+        {
+            TapeSolve(v, &v.cons[k], &tape->fin[k]);
+        }
+        for (int32_t b = gid; b < nb; b += gsz)
+        {
+            tape->xfEnd[b] = v.xf[b];
         }
     }
 }
@@ -1627,7 +1699,7 @@ __global__ void __launch_bounds__(M3_BLOCK_MAX_THREADS)
     {
         return;
     }
-    SubstepLoop(v, sp, (int32_t)threadIdx.x, (int32_t)blockDim.x, m3BlockSync{});
+    SubstepLoop(v, sp, (int32_t)threadIdx.x, (int32_t)blockDim.x, m3BlockSync{}, d.tape);
 }
 
 __global__ void k_clear_forces(const m3Dev* __restrict__ ds)
@@ -1680,7 +1752,11 @@ __global__ void k_ccd(const m3Dev* __restrict__ ds)
     {
         return;
     }
-    m3ContinuousBody(&v, i);
+    int pulled = m3ContinuousBody(&v, i);
+    if (d.tape != nullptr)
+    {
+        d.tape->ccd[i] = (uint8_t)pulled;
+    }
 }
 
 __global__ void k_sleep_timers(const m3Dev* __restrict__ ds, float dt, int sleepEnabled)
@@ -1740,6 +1816,10 @@ __global__ void k_sleep_freeze(const m3Dev* __restrict__ ds)
         v.av[i] = m3V3(0.0f, 0.0f, 0.0f);
         v.pending[i] = 1; // frozen pairs are recorded by the next sweep
         atomicAdd(&v.sc->discoverCount, 1);
+        if (d.tape != nullptr)
+        {
+            d.tape->slept[i] = 1;
+        }
     }
 }
 
@@ -2960,7 +3040,114 @@ static int BatchBuild(m3gpuWorld* const* ws, int32_t n)
 
 // The launch sequence of one batch step: the same stage order as
 // EnqueueStep, each stage one launch over all worlds (grid.y = world).
-static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* p)
+// ---- tape records of a batched step (adjoint replays) ------------------------
+
+// Step-start body state (before anything of the step runs).
+__global__ void k_tape_start(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= v.maxBody)
+    {
+        return;
+    }
+    m3Tape* t = d.tape;
+    t->xf0[b] = v.xf[b];
+    t->lv0[b] = v.lv[b];
+    t->av0[b] = v.av[b];
+    t->force0[b] = v.force[b];
+    t->slept[b] = 0;
+    t->ccd[b] = 0;
+    t->mover[b] = 0;
+}
+
+// The pair list after narrowphase and the carry's source of each point
+// (the same search as m3NarrowPair's).
+__global__ void k_tape_pairs(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.sc->pairCount)
+    {
+        return;
+    }
+    m3Tape* t = d.tape;
+    const m3Manifold* m = &v.manifolds[i];
+    uint64_t key = v.keys[i];
+    t->keys[i] = key;
+    t->man[i] = *m;
+    t->carryPair[i] = -1;
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
+    {
+        t->carryPoint[4 * i + k] = -1;
+    }
+    int32_t oldCount = v.sc->oldPairCount;
+    if (m->pointCount == 0 || oldCount == 0)
+    {
+        return;
+    }
+    int32_t lo = 0;
+    int32_t hi = oldCount - 1;
+    while (lo <= hi)
+    {
+        int32_t mid = (lo + hi) / 2;
+        uint64_t k2 = v.oldKeys[mid];
+        if (k2 == key)
+        {
+            t->carryPair[i] = mid;
+            const m3Manifold* prev = &v.oldManifolds[mid];
+            for (int32_t k = 0; k < m->pointCount; ++k)
+            {
+                for (int32_t o = 0; o < prev->pointCount; ++o)
+                {
+                    if (prev->points[o].id == m->points[k].id)
+                    {
+                        t->carryPoint[4 * i + k] = (int8_t)o;
+                        break;
+                    }
+                }
+            }
+            return;
+        }
+        if (k2 < key)
+        {
+            lo = mid + 1;
+        }
+        else
+        {
+            hi = mid - 1;
+        }
+    }
+}
+
+// Counts, the constraints' pairs in slot order, color starts and movers.
+__global__ void k_tape_solver(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    m3Tape* t = d.tape;
+    int32_t nc = v.sc->constraintCount;
+    int32_t nm = v.sc->moverCount;
+    for (int32_t k = threadIdx.x; k < nc; k += blockDim.x)
+    {
+        t->consPair[k] = v.cons[k].manifoldIndex;
+    }
+    for (int32_t m = threadIdx.x; m < nm; m += blockDim.x)
+    {
+        t->mover[v.movers[m]] = 1;
+    }
+    if (threadIdx.x == 0)
+    {
+        t->counts[0] = v.sc->pairCount;
+        t->counts[1] = nc;
+        t->counts[2] = nm;
+        for (int32_t k = 0; k < M3_GRAPH_COLORS + 2; ++k)
+        {
+            t->colorStarts[k] = v.sc->colorStarts[k];
+        }
+    }
+}
+
+static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* p, bool taped = false)
 {
     const uint32_t W = (uint32_t)s_batch.count;
     const int T = 256;
@@ -2970,6 +3157,10 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     const int32_t np = s_batch.maxPairs;
     const int32_t ne = s_batch.maxEmit;
     const m3SubstepParams sp = SubstepParams(p);
+    if (taped)
+    {
+        KL(k_tape_start, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    }
     KL(k_begin, dim3(1, W), dim3(1), 0, st, D);
     KL(k_fill_emit, dim3(Blocks(ne, T), W), dim3(T), 0, st, D);
     KL(k_fresh, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
@@ -2995,6 +3186,10 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
         KL(k_sat, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
     }
     KL(k_narrow, dim3(Blocks(np, 64), W), dim3(64), 0, st, D);
+    if (taped)
+    {
+        KL(k_tape_pairs, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    }
     KL(k_capture, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_union, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     if (s_batch.maxJoint > 0)
@@ -3016,6 +3211,10 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     }
     KL(k_mover_flags, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->dt);
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 2);
+    if (taped)
+    {
+        KL(k_tape_solver, dim3(1, W), dim3(256), 0, st, D);
+    }
     KL(k_substeps_block, dim3(1, W), dim3(s_batch.threads), 0, st, D, sp);
     KL(k_clear_forces, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_restitution, dim3(1, W), dim3(1), 0, st, D, p->restitutionThreshold);
@@ -3291,25 +3490,28 @@ static void ProfFree(m3gpuWorld* w)
 // Trajectory gradients (m3gpu_adjoint_many)
 // ---------------------------------------------------------------------------
 //
-// One thread per world runs the serial step (m3_serial.h) over the world's
-// own device buffers: a forward pass that checkpoints the state every
-// `checkpointEvery` steps, then, segment by segment from the last, a replay
-// from the segment's checkpoint that records the tape and the reverse pass
-// of m3_adjoint.h over it.  The world's state is restored at the end.
+// The worlds step through the batched launch sequence (one workgroup per
+// world in every block kernel): a forward pass that checkpoints each world's
+// mutable state every `checkpointEvery` steps, then, segment by segment from
+// the last, a replay from the segment's checkpoint with the tape kernels
+// recording each step's intermediate values, and the reverse pass of
+// m3_adjoint.h over the segment, one 64-lane workgroup per world (lanes over
+// the constraints of a color, the bodies, the pairs).  Each world's state and
+// parameters are restored at the end.
+
+#define M3_ADJ_LANES 64
 
 struct m3AdjCtx
 {
     m3View v;
     uint64_t* keysBuf[2];
     m3Manifold* mansBuf[2];
-    m3SerialScratch scratch;
     m3SerialParams sp;
     int32_t steps;
     int32_t every;
     int32_t segs;
     int32_t forwardOnly;
-    int32_t cur;
-    int32_t rc;
+    int32_t cur0;
     double target[3];
     const m3Vec3* force;
     int32_t offKind;
@@ -3326,7 +3528,7 @@ struct m3AdjCtx
     m3AdjWorld aw;
     m3real* comBar; // [maxBody]
     double loss;
-    float* grads;   // [maxBody * 11]
+    float* grads; // [maxBody * 11]
 };
 
 __host__ __device__ static inline int64_t AdjAlign(int64_t x) { return (x + 15) & ~(int64_t)15; }
@@ -3344,21 +3546,26 @@ __host__ static int64_t AdjCheckpointBytes(int32_t nb, int32_t ns, int32_t np)
     return b;
 }
 
-template <typename T> __device__ static inline void AdjCopy(T* dst, const T* src, int64_t n)
+template <typename T> __device__ static inline void AdjCopy(T* dst, const T* src, int64_t n, int lane)
 {
-    for (int64_t i = 0; i < n; ++i)
+    for (int64_t i = lane; i < n; i += M3_ADJ_LANES)
     {
         dst[i] = src[i];
     }
 }
 
-// Saves (save != 0) or loads the world's mutable state at checkpoint slot k.
-__device__ static void AdjCheckpoint(m3AdjCtx* c, int32_t k, int save)
+// Saves (save != 0) or loads world k's mutable state at checkpoint slot
+// `slot` (slot < 0: the state before the call).  One workgroup per world.
+__global__ void __launch_bounds__(M3_ADJ_LANES) k_adj_checkpoint(m3AdjCtx* ctxs, int32_t slot, int save)
 {
+    m3AdjCtx* c = &ctxs[blockIdx.x];
+    const int lane = threadIdx.x;
     m3View& v = c->v;
     int32_t nb = v.maxBody;
     int32_t ns = v.maxShape;
     int32_t np = v.pairCapacity;
+    int32_t k = slot < 0 ? c->segs : slot;
+    int32_t cur = slot < 0 ? c->cur0 : (c->cur0 ^ ((slot * c->every) & 1));
     uint8_t* p = c->ckpt + (int64_t)k * c->ckptStride;
     int32_t* hdr = (int32_t*)p;
     uint64_t* step = (uint64_t*)(p + 16);
@@ -3381,75 +3588,65 @@ __device__ static void AdjCheckpoint(m3AdjCtx* c, int32_t k, int save)
 #undef ADJ_FIELD
     if (save)
     {
-        hdr[0] = c->cur;
-        hdr[1] = v.sc->pairCount;
-        hdr[2] = v.sc->sleepCount;
-        hdr[3] = v.sc->fullQuery;
-        *step = v.sc->stepCount;
-        AdjCopy(xf, v.xf, nb);
-        AdjCopy(lv, v.lv, nb);
-        AdjCopy(av, v.av, nb);
-        AdjCopy(fo, v.force, nb);
-        AdjCopy(to, v.torque, nb);
-        AdjCopy(st, v.sleepTime, nb);
-        AdjCopy(aw, v.awake, nb);
-        AdjCopy(pe, v.pending, nb);
-        AdjCopy(keys, c->keysBuf[c->cur], hdr[1]);
-        AdjCopy(mans, c->mansBuf[c->cur], hdr[1]);
-        AdjCopy(sk, v.sleepKeys, hdr[2]);
-        AdjCopy(leaf, v.leaf, 6 * (int64_t)ns);
+        int32_t pc = v.sc->pairCount;
+        int32_t zc = v.sc->sleepCount;
+        if (lane == 0)
+        {
+            hdr[0] = cur;
+            hdr[1] = pc;
+            hdr[2] = zc;
+            hdr[3] = v.sc->fullQuery;
+            *step = v.sc->stepCount;
+        }
+        AdjCopy(xf, v.xf, nb, lane);
+        AdjCopy(lv, v.lv, nb, lane);
+        AdjCopy(av, v.av, nb, lane);
+        AdjCopy(fo, v.force, nb, lane);
+        AdjCopy(to, v.torque, nb, lane);
+        AdjCopy(st, v.sleepTime, nb, lane);
+        AdjCopy(aw, v.awake, nb, lane);
+        AdjCopy(pe, v.pending, nb, lane);
+        AdjCopy(keys, c->keysBuf[cur], pc, lane);
+        AdjCopy(mans, c->mansBuf[cur], pc, lane);
+        AdjCopy(sk, v.sleepKeys, zc, lane);
+        AdjCopy(leaf, v.leaf, 6 * (int64_t)ns, lane);
     }
     else
     {
-        c->cur = hdr[0];
-        v.sc->pairCount = hdr[1];
-        v.sc->sleepCount = hdr[2];
-        v.sc->fullQuery = hdr[3];
-        v.sc->stepCount = *step;
-        AdjCopy(v.xf, xf, nb);
-        AdjCopy(v.lv, lv, nb);
-        AdjCopy(v.av, av, nb);
-        AdjCopy(v.force, fo, nb);
-        AdjCopy(v.torque, to, nb);
-        AdjCopy(v.sleepTime, st, nb);
-        AdjCopy(v.awake, aw, nb);
-        AdjCopy(v.pending, pe, nb);
-        AdjCopy(c->keysBuf[c->cur], keys, hdr[1]);
-        AdjCopy(c->mansBuf[c->cur], mans, hdr[1]);
-        AdjCopy(v.sleepKeys, sk, hdr[2]);
-        AdjCopy(v.leaf, leaf, 6 * (int64_t)ns);
-    }
-}
-
-__device__ static int AdjStepOnce(m3AdjCtx* c, m3Tape* tape)
-{
-    m3View& v = c->v;
-    for (int32_t b = 0; b < v.maxBody; ++b)
-    {
-        if (v.bodyAlive[b] && v.type[b] == M3_DYNAMIC)
+        int32_t pc = hdr[1];
+        int32_t zc = hdr[2];
+        __syncthreads();
+        if (lane == 0)
         {
-            v.force[b] = c->force[b];
+            v.sc->pairCount = pc;
+            v.sc->sleepCount = zc;
+            v.sc->fullQuery = hdr[3];
+            v.sc->stepCount = *step;
+            v.sc->stall = 0;
         }
+        AdjCopy(v.xf, xf, nb, lane);
+        AdjCopy(v.lv, lv, nb, lane);
+        AdjCopy(v.av, av, nb, lane);
+        AdjCopy(v.force, fo, nb, lane);
+        AdjCopy(v.torque, to, nb, lane);
+        AdjCopy(v.sleepTime, st, nb, lane);
+        AdjCopy(v.awake, aw, nb, lane);
+        AdjCopy(v.pending, pe, nb, lane);
+        AdjCopy(c->keysBuf[cur], keys, pc, lane);
+        AdjCopy(c->mansBuf[cur], mans, pc, lane);
+        AdjCopy(v.sleepKeys, sk, zc, lane);
+        AdjCopy(v.leaf, leaf, 6 * (int64_t)ns, lane);
     }
-    c->cur ^= 1;
-    v.keys = c->keysBuf[c->cur];
-    v.manifolds = c->mansBuf[c->cur];
-    v.oldKeys = c->keysBuf[c->cur ^ 1];
-    v.oldManifolds = c->mansBuf[c->cur ^ 1];
-    v.sc->stall = 0;
-    if (m3SerialStep(&v, &c->scratch, &c->sp, tape) != 0)
-    {
-        c->cur ^= 1;
-        return 1;
-    }
-    return 0;
 }
 
-__device__ static void AdjOffset(m3AdjCtx* c)
+// The requested parameter offset (finite differences), after the original
+// values are saved for the restore.
+__global__ void k_adj_offset(m3AdjCtx* ctxs)
 {
+    m3AdjCtx* c = &ctxs[blockIdx.x];
     m3View& v = c->v;
     int32_t b = c->offBody;
-    if (c->offKind == 0 || b < 0 || b >= v.maxBody)
+    if (threadIdx.x != 0 || c->offKind == 0 || b < 0 || b >= v.maxBody)
     {
         return;
     }
@@ -3489,48 +3686,60 @@ __device__ static void AdjOffset(m3AdjCtx* c)
     }
 }
 
-__global__ void k_adj_forward(m3AdjCtx* ctxs, int32_t count)
+// The applied force of every dynamic body (cleared by each step).
+__global__ void k_adj_force(m3AdjCtx* ctxs)
 {
-    int32_t k = blockIdx.x;
-    if (k >= count || threadIdx.x != 0)
-    {
-        return;
-    }
-    m3AdjCtx* c = &ctxs[k];
+    m3AdjCtx* c = &ctxs[blockIdx.x];
     m3View& v = c->v;
-    AdjCheckpoint(c, c->segs, 1); // the original state
-    AdjOffset(c);
-    c->rc = 0;
-    for (int32_t s = 0; s < c->steps && c->rc == 0; ++s)
+    for (int32_t b = threadIdx.x; b < v.maxBody; b += blockDim.x)
     {
-        if (s % c->every == 0)
+        if (v.bodyAlive[b] && v.type[b] == M3_DYNAMIC)
         {
-            AdjCheckpoint(c, s / c->every, 1);
+            v.force[b] = c->force[b];
         }
-        c->rc = AdjStepOnce(c, nullptr);
     }
-    // Loss of the final mass-weighted centre of mass.
-    double M = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-    for (int32_t b = 0; b < v.maxBody; ++b)
+}
+
+// Loss of the final mass-weighted centre of mass and the reverse pass's
+// starting bars.
+__global__ void __launch_bounds__(M3_ADJ_LANES) k_adj_loss(m3AdjCtx* ctxs)
+{
+    m3AdjCtx* c = &ctxs[blockIdx.x];
+    m3View& v = c->v;
+    const int lane = threadIdx.x;
+    __shared__ double C[3];
+    __shared__ double M;
+    if (lane == 0)
     {
-        if (!v.bodyAlive[b] || v.type[b] != M3_DYNAMIC || v.invMass[b] == 0.0f)
+        double m0 = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int32_t b = 0; b < v.maxBody; ++b)
         {
-            continue;
+            if (!v.bodyAlive[b] || v.type[b] != M3_DYNAMIC || v.invMass[b] == 0.0f)
+            {
+                continue;
+            }
+            double m = 1.0 / (double)v.invMass[b];
+            m3Vec3 r = m3RotateVec3(v.xf[b].q, v.lc[b]);
+            cx += m * (v.xf[b].p.x + (double)r.x);
+            cy += m * (v.xf[b].p.y + (double)r.y);
+            cz += m * (v.xf[b].p.z + (double)r.z);
+            m0 += m;
         }
-        double m = 1.0 / (double)v.invMass[b];
-        m3Vec3 r = m3RotateVec3(v.xf[b].q, v.lc[b]);
-        cx += m * (v.xf[b].p.x + (double)r.x);
-        cy += m * (v.xf[b].p.y + (double)r.y);
-        cz += m * (v.xf[b].p.z + (double)r.z);
-        M += m;
+        C[0] = cx / m0;
+        C[1] = cy / m0;
+        C[2] = cz / m0;
+        M = m0;
+        double L = 0.0;
+        for (int i = 0; i < 3; ++i)
+        {
+            L += (C[i] - c->target[i]) * (C[i] - c->target[i]);
+        }
+        c->loss = L;
+        c->aw.unsupported = 0;
+        c->aw.ccdPulled = 0;
     }
-    double C[3] = {cx / M, cy / M, cz / M};
-    c->loss = 0.0;
-    for (int i = 0; i < 3; ++i)
-    {
-        c->loss += (C[i] - c->target[i]) * (C[i] - c->target[i]);
-    }
-    if (c->forwardOnly || c->rc != 0)
+    __syncthreads();
+    if (c->forwardOnly)
     {
         return;
     }
@@ -3540,7 +3749,7 @@ __global__ void k_adj_forward(m3AdjCtx* ctxs, int32_t count)
     {
         Cb[i] = (m3real)(2.0 * (C[i] - c->target[i]));
     }
-    for (int32_t b = 0; b < v.maxBody; ++b)
+    for (int32_t b = lane; b < v.maxBody; b += M3_ADJ_LANES)
     {
         memset(&a->body[b], 0, sizeof(m3BodyBar));
         memset(&a->param[b], 0, sizeof(m3ParamBar));
@@ -3559,55 +3768,114 @@ __global__ void k_adj_forward(m3AdjCtx* ctxs, int32_t count)
         double dz = v.xf[b].p.z + (double)r.z - C[2];
         c->comBar[b] = (m3real)((Cb[0] * dx + Cb[1] * dy + Cb[2] * dz) / M);
     }
-    for (int32_t i = 0; i < v.pairCapacity; ++i)
+    for (int32_t i = lane; i < v.pairCapacity; i += M3_ADJ_LANES)
     {
         memset(&a->store[i], 0, sizeof(m3ManBar));
     }
-    a->unsupported = 0;
-    a->ccdPulled = 0;
 }
 
-// Replay of segment `seg` with its tape, then its reverse pass.
-__global__ void k_adj_segment(m3AdjCtx* ctxs, int32_t count, int32_t seg)
+// LDS of the reverse pass: body and parameter bars for the segment, and
+// the step's re-prepared constraints, their bars and contributions when
+// they fit (M3_ADJ_LDS_CONS of them).
+#define M3_ADJ_LDS_CONS 64
+
+__host__ __device__ static inline int64_t AdjLdsBytes(int32_t nb)
 {
-    int32_t k = blockIdx.x;
-    if (k >= count || threadIdx.x != 0)
+    return AdjAlign((int64_t)nb * sizeof(m3BodyBar)) + AdjAlign((int64_t)nb * sizeof(m3ParamBar)) +
+           AdjAlign((int64_t)nb * sizeof(m3Vec3)) + AdjAlign((int64_t)nb * sizeof(m3Quat)) +
+           AdjAlign((int64_t)M3_ADJ_LDS_CONS * sizeof(m3Constraint)) +
+           AdjAlign((int64_t)M3_ADJ_LDS_CONS * sizeof(m3ConBar)) +
+           AdjAlign((int64_t)M3_ADJ_LDS_CONS * sizeof(m3ConContrib));
+}
+
+// Reverse pass over segment `seg` (its tape is recorded).
+__global__ void __launch_bounds__(M3_ADJ_LANES) k_adj_reverse(m3AdjCtx* ctxs, int32_t seg, int32_t region)
+{
+    extern __shared__ __align__(16) uint8_t s_adj[];
+    m3AdjCtx* c = &ctxs[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int32_t nb = c->v.maxBody;
+    uint8_t* q = s_adj;
+    m3BodyBar* body = (m3BodyBar*)q;
+    q += AdjAlign((int64_t)nb * sizeof(m3BodyBar));
+    m3ParamBar* param = (m3ParamBar*)q;
+    q += AdjAlign((int64_t)nb * sizeof(m3ParamBar));
+    m3Vec3* ccdP = (m3Vec3*)q;
+    q += AdjAlign((int64_t)nb * sizeof(m3Vec3));
+    m3Quat* ccdQ = (m3Quat*)q;
+    q += AdjAlign((int64_t)nb * sizeof(m3Quat));
+    m3Constraint* prep = (m3Constraint*)q;
+    q += AdjAlign((int64_t)M3_ADJ_LDS_CONS * sizeof(m3Constraint));
+    m3ConBar* cons = (m3ConBar*)q;
+    q += AdjAlign((int64_t)M3_ADJ_LDS_CONS * sizeof(m3ConBar));
+    m3ConContrib* contrib = (m3ConContrib*)q;
+    for (int32_t b = lane; b < nb; b += M3_ADJ_LANES)
     {
-        return;
+        body[b] = c->aw.body[b];
+        param[b] = c->aw.param[b];
     }
-    m3AdjCtx* c = &ctxs[k];
-    if (c->forwardOnly || c->rc != 0 || seg >= c->segs)
+    __shared__ m3View sv;
+    if (lane == 0)
     {
-        return;
+        sv = c->v;
     }
-    AdjCheckpoint(c, seg, 0);
+    m3AdjWorld a = c->aw;
+    a.body = body;
+    a.param = param;
+    a.ccdP = ccdP;
+    a.ccdQ = ccdQ;
+    a.unsupported = 0;
+    a.ccdPulled = 0;
+    __syncthreads();
     int32_t s0 = seg * c->every;
     int32_t s1 = s0 + c->every < c->steps ? s0 + c->every : c->steps;
-    for (int32_t s = s0; s < s1; ++s)
-    {
-        AdjStepOnce(c, &c->tapes[s - s0]);
-    }
     for (int32_t s = s1 - 1; s >= s0; --s)
     {
-        m3AdjStep(&c->v, &c->sp, &c->tapes[s - s0], &c->aw);
-        m3ManBar* t = c->aw.store;
-        c->aw.store = c->aw.storePrev;
-        c->aw.storePrev = t;
+        const m3Tape* t = &c->tapes[region * c->every + (s - s0)];
+        int inLds = t->counts[1] <= M3_ADJ_LDS_CONS;
+        a.prep = inLds ? prep : c->aw.prep;
+        a.cons = inLds ? cons : c->aw.cons;
+        a.contrib = inLds ? contrib : c->aw.contrib;
+        __syncthreads();
+        if (lane == 0)
+        {
+            m3AdjTapeView(&sv, t);
+        }
+        __syncthreads();
+        m3AdjStepPar(&sv, &c->sp, t, &a, lane, M3_ADJ_LANES, m3BlockSync{});
+        m3ManBar* sw = a.store;
+        a.store = a.storePrev;
+        a.storePrev = sw;
+    }
+    for (int32_t b = lane; b < nb; b += M3_ADJ_LANES)
+    {
+        c->aw.body[b] = body[b];
+        c->aw.param[b] = param[b];
+    }
+    if (a.unsupported != 0)
+    {
+        atomicAdd(&c->aw.unsupported, a.unsupported);
+    }
+    if (a.ccdPulled != 0)
+    {
+        atomicAdd(&c->aw.ccdPulled, a.ccdPulled);
+    }
+    if (lane == 0)
+    {
+        c->aw.store = a.store;
+        c->aw.storePrev = a.storePrev;
     }
 }
 
-__global__ void k_adj_finish(m3AdjCtx* ctxs, int32_t count)
+// Gradients, then the original parameters back (the state is restored by
+// the checkpoint kernel).
+__global__ void __launch_bounds__(M3_ADJ_LANES) k_adj_finish(m3AdjCtx* ctxs)
 {
-    int32_t k = blockIdx.x;
-    if (k >= count || threadIdx.x != 0)
-    {
-        return;
-    }
-    m3AdjCtx* c = &ctxs[k];
+    m3AdjCtx* c = &ctxs[blockIdx.x];
     m3View& v = c->v;
-    if (!c->forwardOnly && c->rc == 0)
+    if (!c->forwardOnly)
     {
-        for (int32_t b = 0; b < v.maxBody; ++b)
+        for (int32_t b = threadIdx.x; b < v.maxBody; b += M3_ADJ_LANES)
         {
             float* g = &c->grads[11 * b];
             const m3BodyBar* bb = &c->aw.body[b];
@@ -3628,9 +3896,8 @@ __global__ void k_adj_finish(m3AdjCtx* ctxs, int32_t count)
             g[10] = pb->force.z;
         }
     }
-    // Original state and parameters back.
-    AdjCheckpoint(c, c->segs, 0);
-    if (c->offKind != 0 && c->offBody >= 0 && c->offBody < v.maxBody)
+    __syncthreads();
+    if (threadIdx.x == 0 && c->offKind != 0 && c->offBody >= 0 && c->offBody < v.maxBody)
     {
         int32_t b = c->offBody;
         v.invMass[b] = c->saveInvMass;
@@ -3644,8 +3911,10 @@ __global__ void k_adj_finish(m3AdjCtx* ctxs, int32_t count)
     }
 }
 
-// Device arena of the adjoint, kept between calls and grown as needed.
+// Device arena of the adjoint, kept between calls and grown as needed, and
+// the stream its reverse passes run on.
 static void* s_adjArena = nullptr;
+static hipStream_t s_adjStream = nullptr;
 static size_t s_adjBytes = 0;
 
 static int AdjReserve(size_t bytes)
@@ -3665,15 +3934,6 @@ static int AdjReserve(size_t bytes)
     return 0;
 }
 
-// Per-world arena layout: tapes, bars and checkpoints.
-struct AdjLayout
-{
-    int64_t tapeStep;  // bytes of one step's tape
-    int64_t bars;      // bytes of the reverse scratch
-    int64_t ckptStride;
-    int64_t total;
-};
-
 static int64_t TapeStepBytes(int32_t nb, int32_t np, int32_t S)
 {
     int64_t b = 0;
@@ -3682,8 +3942,8 @@ static int64_t TapeStepBytes(int32_t nb, int32_t np, int32_t S)
     b += 3 * AdjAlign((int64_t)nb);                       // slept, ccd, mover
     b += AdjAlign((int64_t)np * sizeof(uint64_t));
     b += AdjAlign((int64_t)np * sizeof(m3Manifold));
-    b += 2 * AdjAlign((int64_t)np * sizeof(int32_t));     // carryPair, consPair
-    b += AdjAlign((int64_t)np * 4);                       // carryPoint
+    b += 2 * AdjAlign((int64_t)np * sizeof(int32_t)); // carryPair, consPair
+    b += AdjAlign((int64_t)np * 4);                   // carryPoint
     b += AdjAlign(4 * sizeof(int32_t)) + AdjAlign((M3_GRAPH_COLORS + 2) * sizeof(int32_t));
     int64_t sb = (int64_t)S * nb;
     b += AdjAlign(sb * sizeof(m3Transform));
@@ -3698,10 +3958,10 @@ static int64_t BarsBytes(int32_t nb, int32_t np)
 {
     return AdjAlign((int64_t)nb * sizeof(m3BodyBar)) + AdjAlign((int64_t)nb * sizeof(m3ParamBar)) +
            AdjAlign((int64_t)np * sizeof(m3ConBar)) + 2 * AdjAlign((int64_t)np * sizeof(m3ManBar)) +
-           AdjAlign((int64_t)np * sizeof(m3Constraint)) + AdjAlign((int64_t)nb * sizeof(m3Vec3)) +
-           AdjAlign((int64_t)nb * sizeof(m3Quat)) + AdjAlign((int64_t)nb * sizeof(m3real)) +
-           AdjAlign((int64_t)nb * 11 * sizeof(float)) + AdjAlign((int64_t)nb * sizeof(m3Vec3)) +
-           2 * AdjAlign((int64_t)nb * sizeof(int32_t));
+           AdjAlign((int64_t)np * sizeof(m3Constraint)) + AdjAlign((int64_t)np * sizeof(m3ConContrib)) +
+           AdjAlign((int64_t)nb * sizeof(m3Vec3)) + AdjAlign((int64_t)nb * sizeof(m3Quat)) +
+           AdjAlign((int64_t)nb * sizeof(m3real)) + AdjAlign((int64_t)nb * 11 * sizeof(float)) +
+           AdjAlign((int64_t)nb * sizeof(m3Vec3));
 }
 
 template <typename T> static T* Carve(uint8_t** p, int64_t n)
@@ -3747,49 +4007,86 @@ static void CarveTape(m3Tape* t, uint8_t** p, int32_t nb, int32_t np, int32_t S)
     t->fin = Carve<m3TapePass>(p, np);
 }
 
-extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* ps,
-                                  const m3gpuAdjointRequest* rqs, int32_t count)
+static int AdjointRun(m3gpuWorld* const* ws, const m3gpuStepParams* ps, const m3gpuAdjointRequest* rqs,
+                      int32_t count)
 {
-    if (ws == nullptr || ps == nullptr || rqs == nullptr || count <= 0)
-    {
-        return -1;
-    }
-    // Layouts and the arena.
-    std::vector<int64_t> offs(count + 1, 0);
-    int32_t maxSegs = 0;
+    const m3gpuAdjointRequest* r0 = &rqs[0];
+    const int32_t steps = r0->steps;
+    const int32_t every = r0->checkpointEvery > 0 ? r0->checkpointEvery : 10;
+    const int32_t segs = (steps + every - 1) / every;
+    const int forwardOnly = r0->forwardOnly;
+    // The last `resident` segments keep the tapes the forward pass records;
+    // earlier segments are replayed from their checkpoints into region 0.
+    const char* renv = getenv("MAUL3D_AMD_ADJOINT_RESIDENT");
+    int32_t resident = renv != nullptr ? atoi(renv) : 1;
+    resident = resident < 1 ? 1 : (resident > segs ? segs : resident);
+    // Replayed segments alternate between two regions so the replay of one
+    // segment overlaps the reverse pass of the next.
+    const int32_t spare = resident < segs ? (segs - resident > 1 ? 2 : 1) : 0;
+    const int32_t regions = resident + spare;
+    auto regionOf = [&](int32_t seg) -> int32_t {
+        if (seg >= segs - resident)
+        {
+            return spare + seg - (segs - resident);
+        }
+        return seg % spare;
+    };
     for (int32_t k = 0; k < count; ++k)
     {
         m3gpuWorld* w = ws[k];
-        if (w == nullptr || JoinBatch(w) != 0)
+        if (w == nullptr || rqs[k].steps != steps || rqs[k].checkpointEvery != r0->checkpointEvery ||
+            rqs[k].forwardOnly != forwardOnly || memcmp(&ps[k], &ps[0], sizeof(ps[0])) != 0 ||
+            !BatchEligible(w) || (ps[k].continuousEnabled && w->ccdUnsupported) || steps < 1)
         {
-            return -1;
+            snprintf(s_lastError, sizeof(s_lastError),
+                     "m3gpu_adjoint_many: worlds must share steps, checkpoints and parameters and fit the batched kernels");
+            return 1;
         }
-        HIP_TRY(hipStreamSynchronize(w->stream));
-        w->streamBusy = 0;
-        const m3View& v = w->view;
-        int32_t every = rqs[k].checkpointEvery > 0 ? rqs[k].checkpointEvery : 10;
-        int32_t segs = (rqs[k].steps + every - 1) / every;
-        maxSegs = segs > maxSegs ? segs : maxSegs;
+        for (int32_t j = 0; j < k; ++j)
+        {
+            if (ws[j] == w)
+            {
+                snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: a world appears twice");
+                return 1;
+            }
+        }
+    }
+    if (BatchBuild(ws, count) != 0)
+    {
+        return -1;
+    }
+    hipStream_t st = s_batch.stream;
+    // Arena: per world checkpoints, force, bars, tapes; then the contexts and
+    // the per-step kernel views.
+    std::vector<int64_t> offs(count + 1, 0);
+    for (int32_t k = 0; k < count; ++k)
+    {
+        const m3View& v = ws[k]->view;
         int64_t bytes = (int64_t)(segs + 1) * AdjCheckpointBytes(v.maxBody, v.maxShape, v.pairCapacity);
-        if (!rqs[k].forwardOnly)
+        bytes += AdjAlign((int64_t)v.maxBody * sizeof(m3Vec3));
+        bytes += BarsBytes(v.maxBody, v.pairCapacity);
+        if (!forwardOnly)
         {
-            bytes += (int64_t)every * TapeStepBytes(v.maxBody, v.pairCapacity, ps[k].substeps);
-            bytes += (int64_t)every * sizeof(m3Tape) + 64;
+            bytes += (int64_t)regions * every * TapeStepBytes(v.maxBody, v.pairCapacity, ps[k].substeps);
+            bytes += AdjAlign((int64_t)regions * every * sizeof(m3Tape));
         }
-        bytes += BarsBytes(v.maxBody, v.pairCapacity) + AdjAlign((int64_t)v.maxBody * sizeof(m3Vec3));
         offs[k + 1] = offs[k] + AdjAlign(bytes);
     }
-    size_t ctxBytes = AdjAlign((int64_t)count * sizeof(m3AdjCtx));
-    if (AdjReserve((size_t)offs[count] + ctxBytes) != 0)
+    int64_t ctxBytes = AdjAlign((int64_t)count * sizeof(m3AdjCtx));
+    int64_t viewBytes = AdjAlign((int64_t)steps * count * sizeof(m3Dev));
+    if (AdjReserve((size_t)(offs[count] + ctxBytes + 2 * viewBytes)) != 0)
     {
         return -1;
     }
     uint8_t* arena = (uint8_t*)s_adjArena;
     m3AdjCtx* dctx = (m3AdjCtx*)(arena + offs[count]);
+    m3Dev* dFwd = (m3Dev*)(arena + offs[count] + ctxBytes);
+    m3Dev* dRep = (m3Dev*)(arena + offs[count] + ctxBytes + viewBytes);
     std::vector<m3AdjCtx> ctx(count);
-    std::vector<std::vector<m3Tape>> tapes(count);
+    std::vector<m3Tape> tapes;
     std::vector<float> hforce;
-    hipStream_t st = ws[0]->stream;
+    std::vector<m3Dev> hv((size_t)steps * count);
+    std::vector<m3Dev> hr(forwardOnly ? 0 : (size_t)steps * count);
     for (int32_t k = 0; k < count; ++k)
     {
         m3gpuWorld* w = ws[k];
@@ -3803,7 +4100,7 @@ extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* 
         c.keysBuf[1] = w->keysBuf[1];
         c.mansBuf[0] = w->mansBuf[0];
         c.mansBuf[1] = w->mansBuf[1];
-        c.cur = w->cur;
+        c.cur0 = w->cur;
         const m3gpuStepParams* p = &ps[k];
         c.sp.dt = p->dt;
         c.sp.substeps = p->substeps;
@@ -3816,10 +4113,10 @@ extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* 
         c.sp.maximumAngularSpeed = p->maximumAngularSpeed;
         c.sp.sleepEnabled = p->sleepEnabled;
         c.sp.continuousEnabled = p->continuousEnabled;
-        c.steps = r->steps;
-        c.every = r->checkpointEvery > 0 ? r->checkpointEvery : 10;
-        c.segs = (r->steps + c.every - 1) / c.every;
-        c.forwardOnly = r->forwardOnly;
+        c.steps = steps;
+        c.every = every;
+        c.segs = segs;
+        c.forwardOnly = forwardOnly;
         for (int i = 0; i < 3; ++i)
         {
             c.target[i] = (double)r->target[i];
@@ -3830,7 +4127,7 @@ extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* 
         c.offValue = (m3real)r->offsetValue;
         uint8_t* q = arena + offs[k];
         c.ckptStride = AdjCheckpointBytes(nb, ns, np);
-        c.ckpt = Carve<uint8_t>(&q, (int64_t)(c.segs + 1) * c.ckptStride);
+        c.ckpt = Carve<uint8_t>(&q, (int64_t)(segs + 1) * c.ckptStride);
         m3Vec3* force = Carve<m3Vec3>(&q, nb);
         c.force = force;
         hforce.assign((size_t)nb * 3, 0.0f);
@@ -3838,56 +4135,159 @@ extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* 
         {
             memcpy(hforce.data(), r->force, (size_t)nb * 3 * sizeof(float));
         }
-        HIP_TRY(hipMemcpyAsync(force, hforce.data(), (size_t)nb * sizeof(m3Vec3), hipMemcpyHostToDevice, st));
-        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipMemcpy(force, hforce.data(), (size_t)nb * sizeof(m3Vec3), hipMemcpyHostToDevice));
         c.aw.body = Carve<m3BodyBar>(&q, nb);
         c.aw.param = Carve<m3ParamBar>(&q, nb);
         c.aw.cons = Carve<m3ConBar>(&q, np);
         c.aw.store = Carve<m3ManBar>(&q, np);
         c.aw.storePrev = Carve<m3ManBar>(&q, np);
         c.aw.prep = Carve<m3Constraint>(&q, np);
+        c.aw.contrib = Carve<m3ConContrib>(&q, np);
         c.aw.ccdP = Carve<m3Vec3>(&q, nb);
         c.aw.ccdQ = Carve<m3Quat>(&q, nb);
         c.comBar = Carve<m3real>(&q, nb);
         c.grads = Carve<float>(&q, (int64_t)nb * 11);
-        c.scratch.root = Carve<int32_t>(&q, nb);
-        c.scratch.masks = Carve<uint32_t>(&q, nb);
-        if (!r->forwardOnly)
+        if (!forwardOnly)
         {
-            m3Tape* dt = Carve<m3Tape>(&q, c.every);
-            tapes[k].resize(c.every);
-            for (int32_t s = 0; s < c.every; ++s)
+            m3Tape* dt = Carve<m3Tape>(&q, (int64_t)regions * every);
+            tapes.resize((size_t)regions * every);
+            for (int32_t s = 0; s < regions * every; ++s)
             {
-                CarveTape(&tapes[k][s], &q, nb, np, p->substeps);
+                CarveTape(&tapes[s], &q, nb, np, p->substeps);
             }
-            HIP_TRY(hipMemcpyAsync(dt, tapes[k].data(), c.every * sizeof(m3Tape), hipMemcpyHostToDevice, st));
-            HIP_TRY(hipStreamSynchronize(st));
+            HIP_TRY(hipMemcpy(dt, tapes.data(), tapes.size() * sizeof(m3Tape), hipMemcpyHostToDevice));
             c.tapes = dt;
         }
+        // Kernel views of every step: step s writes buffer cur0 ^ ((s + 1) & 1).
+        for (int32_t s = 0; s < steps; ++s)
+        {
+            int par = w->cur ^ ((s + 1) & 1);
+            m3Dev e = w->dev;
+            e.v = w->view;
+            e.v.keys = w->keysBuf[par];
+            e.v.manifolds = w->mansBuf[par];
+            e.v.oldKeys = w->keysBuf[par ^ 1];
+            e.v.oldManifolds = w->mansBuf[par ^ 1];
+            e.tape = nullptr;
+            hv[(size_t)s * count + k] = e;
+            if (!forwardOnly)
+            {
+                e.tape = c.tapes + (size_t)regionOf(s / every) * every + (s % every);
+                hr[(size_t)s * count + k] = e;
+            }
+        }
     }
-    HIP_TRY(hipMemcpyAsync(dctx, ctx.data(), count * sizeof(m3AdjCtx), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_adj_forward, dim3(count), dim3(1), 0, st, dctx, count);
-    HIP_TRY(hipGetLastError());
-    for (int32_t seg = maxSegs - 1; seg >= 0; --seg)
+    HIP_TRY(hipMemcpy(dctx, ctx.data(), count * sizeof(m3AdjCtx), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dFwd, hv.data(), hv.size() * sizeof(m3Dev), hipMemcpyHostToDevice));
+    if (!forwardOnly)
     {
-        hipLaunchKernelGGL(k_adj_segment, dim3(count), dim3(1), 0, st, dctx, count, seg);
-        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpy(dRep, hr.data(), hr.size() * sizeof(m3Dev), hipMemcpyHostToDevice));
     }
-    hipLaunchKernelGGL(k_adj_finish, dim3(count), dim3(1), 0, st, dctx, count);
+    const dim3 G(count);
+    const dim3 L(M3_ADJ_LANES);
+    int32_t nbMax = 0;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        nbMax = ws[k]->view.maxBody > nbMax ? ws[k]->view.maxBody : nbMax;
+    }
+    const size_t ldsBytes = (size_t)AdjLdsBytes(nbMax);
+    if (ldsBytes > 160 * 1024)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: worlds too large for the reverse pass");
+        return 1;
+    }
+    if (ldsBytes > 64 * 1024)
+    {
+        HIP_TRY(hipFuncSetAttribute((const void*)k_adj_reverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)ldsBytes));
+    }
+    hipLaunchKernelGGL(k_adj_checkpoint, G, L, 0, st, dctx, -1, 1);
+    hipLaunchKernelGGL(k_adj_offset, G, dim3(1), 0, st, dctx);
+    for (int32_t s = 0; s < steps; ++s)
+    {
+        if (s % every == 0)
+        {
+            hipLaunchKernelGGL(k_adj_checkpoint, G, L, 0, st, dctx, s / every, 1);
+        }
+        hipLaunchKernelGGL(k_adj_force, G, L, 0, st, dctx);
+        bool taped = !forwardOnly && s / every >= segs - resident;
+        if (BatchLaunches(st, (taped ? dRep : dFwd) + (size_t)s * count, &ps[0], taped) != 0)
+        {
+            return -1;
+        }
+    }
+    hipLaunchKernelGGL(k_adj_loss, G, L, 0, st, dctx);
+    if (!forwardOnly)
+    {
+        // Replays run on the batch stream, reverse passes on a second
+        // stream: reverse(seg) waits for replay(seg); a replay into a region
+        // waits for the reverse pass that last read it.
+        if (s_adjStream == nullptr)
+        {
+            HIP_TRY(hipStreamCreateWithFlags(&s_adjStream, hipStreamNonBlocking));
+        }
+        hipStream_t rs = s_adjStream;
+        std::vector<hipEvent_t> replayed(segs), reversed(segs);
+        for (int32_t k = 0; k < segs; ++k)
+        {
+            HIP_TRY(hipEventCreateWithFlags(&replayed[k], hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&reversed[k], hipEventDisableTiming));
+        }
+        HIP_TRY(hipEventRecord(replayed[segs - 1], st)); // the forward pass (resident tapes)
+        for (int32_t seg = segs - 1; seg >= 0; --seg)
+        {
+            if (seg < segs - resident)
+            {
+                int32_t user = seg + spare; // previous reader of this segment's region
+                if (spare == 2 && user < segs - resident)
+                {
+                    HIP_TRY(hipStreamWaitEvent(st, reversed[user], 0));
+                }
+                hipLaunchKernelGGL(k_adj_checkpoint, G, L, 0, st, dctx, seg, 0);
+                int32_t s0 = seg * every;
+                int32_t s1 = s0 + every < steps ? s0 + every : steps;
+                for (int32_t s = s0; s < s1; ++s)
+                {
+                    hipLaunchKernelGGL(k_adj_force, G, L, 0, st, dctx);
+                    if (BatchLaunches(st, dRep + (size_t)s * count, &ps[0], true) != 0)
+                    {
+                        return -1;
+                    }
+                }
+                HIP_TRY(hipEventRecord(replayed[seg], st));
+            }
+            else
+            {
+                HIP_TRY(hipEventRecord(replayed[seg], st));
+            }
+            HIP_TRY(hipStreamWaitEvent(rs, replayed[seg], 0));
+            hipLaunchKernelGGL(k_adj_reverse, G, L, ldsBytes, rs, dctx, seg, regionOf(seg));
+            HIP_TRY(hipEventRecord(reversed[seg], rs));
+            if (spare == 1 && seg < segs - resident)
+            {
+                // One replay region: the next replay waits for this reverse pass.
+                HIP_TRY(hipStreamWaitEvent(st, reversed[seg], 0));
+            }
+        }
+        HIP_TRY(hipStreamWaitEvent(st, reversed[0], 0));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int32_t k = 0; k < segs; ++k)
+        {
+            (void)hipEventDestroy(replayed[k]);
+            (void)hipEventDestroy(reversed[k]);
+        }
+    }
+    hipLaunchKernelGGL(k_adj_finish, G, L, 0, st, dctx);
+    hipLaunchKernelGGL(k_adj_checkpoint, G, L, 0, st, dctx, -1, 0);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(ctx.data(), dctx, count * sizeof(m3AdjCtx), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    int rc = 0;
     std::vector<float> g;
     for (int32_t k = 0; k < count; ++k)
     {
         const m3AdjCtx& c = ctx[k];
         const m3gpuAdjointRequest* r = &rqs[k];
         int32_t nb = ws[k]->view.maxBody;
-        if (c.rc != 0)
-        {
-            rc = 1;
-        }
         if (r->loss)
         {
             *r->loss = c.loss;
@@ -3900,7 +4300,7 @@ extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* 
         {
             *r->ccdPulled = c.aw.ccdPulled;
         }
-        if (r->forwardOnly)
+        if (forwardOnly)
         {
             continue;
         }
@@ -3931,5 +4331,28 @@ extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* 
             }
         }
     }
+    return 0;
+}
+
+extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* ps,
+                                  const m3gpuAdjointRequest* rqs, int32_t count)
+{
+    if (ws == nullptr || ps == nullptr || rqs == nullptr || count <= 0)
+    {
+        return -1;
+    }
+    for (int32_t k = 0; k < count; ++k)
+    {
+        if (ws[k] == nullptr || JoinBatch(ws[k]) != 0)
+        {
+            return -1;
+        }
+        HIP_TRY(hipStreamSynchronize(ws[k]->stream));
+        ws[k]->streamBusy = 0;
+    }
+    pthread_mutex_lock(&s_batchLock);
+    int rc = AdjointRun(ws, ps, rqs, count);
+    BatchDrop();
+    pthread_mutex_unlock(&s_batchLock);
     return rc;
 }

@@ -2001,11 +2001,12 @@ int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointD
         o->unsupported = 0;
         o->ccdPulled = 0;
     }
-    if (m3gpu_adjoint_many(devs, params, reqs, count) != 0)
+    int rc = m3gpu_adjoint_many(devs, params, reqs, count);
+    if (rc < 0)
     {
         DeviceFail("m3gpu_adjoint_many");
     }
-    return count;
+    return rc == 0 ? count : -1;
 }
 
 // Uploads pending host changes and waits for the world's queued device work

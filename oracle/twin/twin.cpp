@@ -790,6 +790,7 @@ int Adjoint(m3gpuWorld* w, const m3gpuStepParams* p, const m3gpuAdjointRequest* 
         std::vector<tw::m3ManBar> store(np), storePrev(np);
         std::vector<tw::m3Constraint> prep(np);
         std::vector<tw::m3Vec3> ccdP(nb);
+        std::vector<tw::m3ConContrib> contrib(np);
         std::vector<tw::m3Quat> ccdQ(nb);
         memset(body.data(), 0, sizeof(tw::m3BodyBar) * nb);
         memset(param.data(), 0, sizeof(tw::m3ParamBar) * nb);
@@ -822,6 +823,7 @@ int Adjoint(m3gpuWorld* w, const m3gpuStepParams* p, const m3gpuAdjointRequest* 
         aw.storePrev = storePrev.data();
         aw.prep = prep.data();
         aw.ccdP = ccdP.data();
+        aw.contrib = contrib.data();
         aw.ccdQ = ccdQ.data();
         aw.unsupported = 0;
         aw.ccdPulled = 0;
