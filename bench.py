@@ -10,7 +10,8 @@ The same line carries two more measurements: ``batched`` (256 independent
 worlds restored from a snapshot and resimulated 8 steps through
 m3World_StepMany, against the same worlds stepped one by one) and
 ``hull_chains`` (4000 QuickHull hulls plus 500 chains of 20 capsule links on
-spherical joints, 600 steps).
+spherical joints, 600 steps) and ``toss_adjoint`` (64 differentiable toss
+worlds: forward time, adjoint time and their ratio).
 
 Prints one JSON line.  ``roofline`` reports the dominant kernel's
 algorithmic HBM bytes over its event-timed average duration; ``cpu_baseline``
@@ -48,6 +49,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--batch-worlds", type=int, default=256)
     p.add_argument("--batch-only", action="store_true", help="run only the batched-rollback measurement")
     p.add_argument("--no-hull-chains", action="store_true", help="skip the hull-rain-with-chains measurement")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the toss adjoint measurement")
+    p.add_argument("--adjoint-only", action="store_true", help="run only the toss adjoint measurement")
     p.add_argument(
         "--hull-chains-only", action="store_true", help="run only the hull-rain-with-chains measurement"
     )
@@ -243,6 +246,96 @@ def run_hull_chains(steps: int) -> dict:
     }
 
 
+def run_toss_adjoint(worlds: int = 64, steps: int = 120) -> dict:
+    """Differentiable toss: ``worlds`` worlds of 32 boxes thrown onto a plane,
+    ``steps`` steps.  Forward: the worlds stepped together through
+    m3World_StepMany (hashes checked against the reference fixture).
+    Adjoint: m3World_Adjoint over the same worlds from the same initial
+    state, loss |COM_T - target|^2 with gradients for every body's initial
+    velocities, mass, friction and applied force, checkpoints every 10 steps
+    with every earlier segment recomputed from its checkpoint (and, as a
+    second figure, with all segment tapes kept resident).  Both timed from
+    the same resident initial state, after one untimed warm-up call each."""
+    import maul3d_amd
+    from maul3d_amd import api, scenes
+
+    gpu = maul3d_amd.get_api()
+    lib = maul3d_amd.library()
+    ws = [scenes.toss(gpu, seed=0x7055 + k) for k in range(worlds)]
+    ids = (api.WorldId * worlds)(*ws)
+    snaps = [gpu.snapshot(w) for w in ws]
+    nb = 40
+
+    def restore() -> None:
+        for w, snap in zip(ws, snaps):
+            if not gpu.restore(w, snap):
+                raise RuntimeError("restore refused")
+            lib.m3World_Sync(w)
+
+    def forward() -> float:
+        restore()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lib.m3World_StepMany(ids, worlds, C.c_float(DT), SUBSTEPS)
+        for w in ws:
+            lib.m3World_Sync(w)
+        return time.perf_counter() - t0
+
+    forward()
+    t_fwd = forward()
+    try:
+        with open(os.path.join(ROOT, "tests", "golden", "hashes.json")) as f:
+            fx = json.load(f)["toss_120"]
+        exact = all("%016x" % gpu.m3World_Hash(w) == fx[str(k)] for k, w in enumerate(ws)) if steps == 120 else None
+    except OSError:
+        exact = None
+    restore()
+    defs = (api.AdjointDef * worlds)()
+    res = (api.AdjointResult * worlds)()
+    keep = []
+    for k in range(worlds):
+        d = gpu.m3DefaultAdjointDef()
+        d.steps = steps
+        d.checkpointEvery = 10
+        d.target = api.Vec3(0.0, 0.5, 0.0)
+        defs[k] = d
+        arrs = [(api.Vec3 * nb)(), (api.Vec3 * nb)(), (api.Vec3 * nb)(), (C.c_float * nb)(), (C.c_float * nb)()]
+        keep.append(arrs)
+        res[k].gradLinearVelocity = C.cast(arrs[0], C.POINTER(api.Vec3))
+        res[k].gradAngularVelocity = C.cast(arrs[1], C.POINTER(api.Vec3))
+        res[k].gradForce = C.cast(arrs[2], C.POINTER(api.Vec3))
+        res[k].gradMass = C.cast(arrs[3], C.POINTER(C.c_float))
+        res[k].gradFriction = C.cast(arrs[4], C.POINTER(C.c_float))
+
+    def adjoint(resident: str) -> float:
+        os.environ["MAUL3D_AMD_ADJOINT_RESIDENT"] = resident
+        t0 = time.perf_counter()
+        if gpu.m3World_Adjoint(ids, worlds, defs, res) != worlds:
+            raise RuntimeError("m3World_Adjoint refused the worlds")
+        return time.perf_counter() - t0
+
+    adjoint("1")
+    t_adj = adjoint("1")
+    adjoint("1000")
+    t_res = adjoint("1000")
+    os.environ.pop("MAUL3D_AMD_ADJOINT_RESIDENT", None)
+    unsupported = sum(res[k].unsupported for k in range(worlds))
+    for w in ws:
+        gpu.m3DestroyWorld(w)
+    return {
+        "workload": f"{worlds} toss worlds x 32 boxes, {steps} steps, loss |COM_T - target|^2, gradients "
+        "for initial linear/angular velocity, mass, friction, applied force",
+        "forward_ms": 1000.0 * t_fwd,
+        "adjoint_ms": 1000.0 * t_adj,
+        "adjoint_forward_ratio": t_adj / t_fwd,
+        "checkpoint_every": 10,
+        "adjoint_resident_tapes_ms": 1000.0 * t_res,
+        "adjoint_forward_ratio_resident_tapes": t_res / t_fwd,
+        "forward_hash_bit_exact": exact,
+        "unsupported": unsupported,
+    }
+
+
 def copy_calibration() -> None:
     """A plain elementwise copy of known size, the scale for FETCH_SIZE and
     WRITE_SIZE counter readings of the step kernels."""
@@ -274,6 +367,9 @@ def main() -> None:
         return
     if args.hull_chains_only:
         print(json.dumps({"hull_chains": run_hull_chains(args.steps)}))
+        return
+    if args.adjoint_only:
+        print(json.dumps({"toss_adjoint": run_toss_adjoint()}))
         return
     res = run_gpu(args.steps, args.warmup)
     value = BODIES * SUBSTEPS * args.steps / res["wall"]
@@ -313,6 +409,8 @@ def main() -> None:
         line["batched"] = run_batched(args.batch_worlds)
     if not args.no_hull_chains:
         line["hull_chains"] = run_hull_chains(600)
+    if not args.no_adjoint:
+        line["toss_adjoint"] = run_toss_adjoint()
     if not args.no_cpu:
         line["cpu_baseline"] = run_cpu(args.cpu_steps)
     print(json.dumps(line))
