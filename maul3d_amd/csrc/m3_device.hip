@@ -413,79 +413,153 @@ __device__ static inline bool AllowedPacked(const m3View& v, int4 a, int4 b)
 // froze last step (m3DiscoverHit).  Runs before the proxy refresh: the
 // frozen discovery reads last step's stored boxes, as the reference's
 // end-of-step discovery does.
-__global__ void k_sap(const m3Dev* __restrict__ ds)
+// The workgroup's shapes and the sorted entries after them are staged in
+// LDS: a shape's candidates are the entries that follow it in the x order,
+// so neighbouring shapes scan largely the same window.
+#define M3_SAP_THREADS 64
+#define M3_SAP_WINDOW 1536
+
+// The sweep.  Each lane first collects, from the x-sorted entries in LDS,
+// the candidates whose float sweep boxes overlap its shape's (cheap, no
+// global loads), then the lanes test their candidates in step: the full
+// double-precision tests and filters load from global memory, and doing
+// them as they turn up would make every iteration of the wavefront's scan
+// wait on some lane's loads.
+#define M3_SAP_CAND 32
+
+__global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
+    __shared__ float4 sxy[M3_SAP_WINDOW];
+    __shared__ float2 szz[M3_SAP_WINDOW];
+    __shared__ int32_t cand[M3_SAP_CAND][M3_SAP_THREADS];
     const float4* xy = d.sapXY;
     const float2* zz = d.sapZ;
     const int4* attr = d.sapAttr;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxShape)
+    const int32_t base = blockIdx.x * blockDim.x;
+    const int32_t ns = v.maxShape;
+    const int32_t wend = base + M3_SAP_WINDOW < ns ? base + M3_SAP_WINDOW : ns;
+    for (int32_t j = base + threadIdx.x; j < wend; j += blockDim.x)
     {
-        return;
+        sxy[j - base] = xy[j];
+        szz[j - base] = zz[j];
     }
-    const int4 me = attr[i];
-    if (me.x < 0)
+    __syncthreads();
+    const int lane = threadIdx.x;
+    const int32_t i = base + lane;
+    int4 me = make_int4(-1, 0, 0, 0);
+    if (i < ns)
     {
-        return;
+        me = attr[i];
     }
-    const float4 mxy = xy[i];
-    const float2 mz = zz[i];
-    const double* a = &v.fresh[6 * me.x];
+    bool done = !(i < ns && me.x >= 0);
+    float4 mxy = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2 mz = make_float2(0.0f, 0.0f);
+    const double* a = nullptr;
+    if (!done)
+    {
+        mxy = sxy[i - base];
+        mz = szz[i - base];
+        a = &v.fresh[6 * me.x];
+    }
+    int32_t j = i + 1;
     m3EmitBuf buf;
     buf.n = 0;
-    for (int32_t j = i + 1; j < v.maxShape; ++j)
+    while (true)
     {
-        float4 oxy = xy[j];
-        if (oxy.x > mxy.y) // sentinels have +inf lower x: the sweep ends
+        int32_t nc = 0;
+        if (!done)
+        {
+            for (; j < wend && nc < M3_SAP_CAND; ++j)
+            {
+                float4 oxy = sxy[j - base];
+                if (oxy.x > mxy.y) // sentinels have +inf lower x: the sweep ends
+                {
+                    done = true;
+                    break;
+                }
+                if (oxy.z > mxy.w || mxy.z > oxy.w)
+                {
+                    continue;
+                }
+                float2 oz = szz[j - base];
+                if (oz.x > mz.y || mz.x > oz.y)
+                {
+                    continue;
+                }
+                cand[nc++][lane] = j;
+            }
+            if (!done && j >= wend)
+            {
+                for (; j < ns && nc < M3_SAP_CAND; ++j)
+                {
+                    float4 oxy = xy[j];
+                    if (oxy.x > mxy.y)
+                    {
+                        done = true;
+                        break;
+                    }
+                    if (oxy.z > mxy.w || mxy.z > oxy.w)
+                    {
+                        continue;
+                    }
+                    float2 oz = zz[j];
+                    if (oz.x > mz.y || mz.x > oz.y)
+                    {
+                        continue;
+                    }
+                    cand[nc++][lane] = j;
+                }
+            }
+            if (j >= ns)
+            {
+                done = true;
+            }
+        }
+        for (int32_t k = 0; k < nc; ++k)
+        {
+            const int32_t jj = cand[k][lane];
+            const int4 ot = attr[jj];
+            const double* b = &v.fresh[6 * ot.x];
+            if (!m3BoxOverlap(a, b))
+            {
+                continue;
+            }
+            bool pend = ((me.z | ot.z) & SA_PENDING) != 0;
+            bool hot = ((me.z | ot.z) & SA_HOT) != 0;
+            bool queries = ((me.z | ot.z) & SA_QUERIES) != 0;
+            if (!pend && !(hot && queries))
+            {
+                continue;
+            }
+            if (!AllowedPacked(v, me, ot))
+            {
+                continue;
+            }
+            if ((me.z & SA_PENDING) && m3BoxOverlap(&v.leaf[6 * ot.x], a))
+            {
+                AppendFrozen(v, me.x, ot.x);
+            }
+            if ((ot.z & SA_PENDING) && m3BoxOverlap(&v.leaf[6 * me.x], b))
+            {
+                AppendFrozen(v, ot.x, me.x);
+            }
+            if (!hot || !queries)
+            {
+                continue;
+            }
+            bool qa = (me.z & SA_QUERIES) != 0;
+            bool qb = (ot.z & SA_QUERIES) != 0;
+            bool emit = (qa && qb) || (qa ? m3BoxOverlap(&v.leaf[6 * ot.x], a)
+                                          : m3BoxOverlap(&v.leaf[6 * me.x], b));
+            if (emit)
+            {
+                EmitBuffered(v, buf, m3CompactKey(me.x, ot.x));
+            }
+        }
+        if (__all(done))
         {
             break;
-        }
-        if (oxy.z > mxy.w || mxy.z > oxy.w)
-        {
-            continue;
-        }
-        float2 oz = zz[j];
-        if (oz.x > mz.y || mz.x > oz.y)
-        {
-            continue;
-        }
-        const int4 ot = attr[j];
-        const double* b = &v.fresh[6 * ot.x];
-        if (!m3BoxOverlap(a, b))
-        {
-            continue;
-        }
-        bool pend = ((me.z | ot.z) & SA_PENDING) != 0;
-        bool hot = ((me.z | ot.z) & SA_HOT) != 0;
-        bool queries = ((me.z | ot.z) & SA_QUERIES) != 0;
-        if (!pend && !(hot && queries))
-        {
-            continue;
-        }
-        if (!AllowedPacked(v, me, ot))
-        {
-            continue;
-        }
-        if ((me.z & SA_PENDING) && m3BoxOverlap(&v.leaf[6 * ot.x], a))
-        {
-            AppendFrozen(v, me.x, ot.x);
-        }
-        if ((ot.z & SA_PENDING) && m3BoxOverlap(&v.leaf[6 * me.x], b))
-        {
-            AppendFrozen(v, ot.x, me.x);
-        }
-        if (!hot || !queries)
-        {
-            continue;
-        }
-        bool qa = (me.z & SA_QUERIES) != 0;
-        bool qb = (ot.z & SA_QUERIES) != 0;
-        bool emit = (qa && qb) || (qa ? m3BoxOverlap(&v.leaf[6 * ot.x], a)
-                                      : m3BoxOverlap(&v.leaf[6 * me.x], b));
-        if (emit)
-        {
-            EmitBuffered(v, buf, m3CompactKey(me.x, ot.x));
         }
     }
     EmitFlush(v, buf);
@@ -1484,8 +1558,6 @@ __device__ static inline void GridBarrier(uint32_t* bar, int32_t* fault)
     __syncthreads();
 }
 
-// One graph color of one pass over the grid (mode 1 solve with bias,
-// 2 relax); the overflow color runs serially on one thread.
 // The velocities and accumulators a constraint solve starts from.
 __device__ static inline void TapeSolve(const m3View& v, const m3Constraint* c, m3TapePass* r)
 {
@@ -1502,6 +1574,8 @@ __device__ static inline void TapeSolve(const m3View& v, const m3Constraint* c, 
     r->twist = c->twistImpulse;
 }
 
+// One graph color of one pass (mode 1 solve with bias, 2 relax) over the
+// participants; the overflow color runs serially on one thread.
 __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t end, int mode,
                                          const m3SubstepParams& sp, int32_t gid, int32_t gsz,
                                          bool serial, m3TapePass* rec)
@@ -1553,9 +1627,16 @@ struct m3BlockSync
     __device__ void operator()() const { __syncthreads(); }
 };
 
+// On a grid, the last colors of a pass are small (the greedy coloring's
+// sizes fall off quickly): the trailing colors holding at most
+// M3_TAIL_CONSTRAINTS constraints in all run in workgroup 0 alone, color
+// after color with workgroup barriers, and the grid meets once after them.
+#define M3_TAIL_CONSTRAINTS 1024
+
 template <typename Sync>
 __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams& sp, int32_t gid,
-                                          int32_t gsz, Sync sync, m3Tape* tape = nullptr)
+                                          int32_t gsz, Sync sync, m3Tape* tape = nullptr,
+                                          bool gridTail = false)
 {
     const int32_t nb = v.maxBody;
     int32_t starts[M3_GRAPH_COLORS + 2];
@@ -1564,6 +1645,30 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
         starts[k] = v.sc->colorStarts[k];
     }
     const int32_t movers = v.sc->moverCount;
+    // First color of the workgroup-0 tail (M3_GRAPH_COLORS + 1: none).
+    int32_t tailStart = M3_GRAPH_COLORS + 1;
+    if (gridTail)
+    {
+        int32_t rem = starts[M3_GRAPH_COLORS + 1] - starts[M3_GRAPH_COLORS];
+        if (rem <= M3_TAIL_CONSTRAINTS)
+        {
+            tailStart = M3_GRAPH_COLORS;
+            for (int32_t c = M3_GRAPH_COLORS - 1; c >= 0; --c)
+            {
+                int32_t size = starts[c + 1] - starts[c];
+                if (rem + size > M3_TAIL_CONSTRAINTS)
+                {
+                    break;
+                }
+                rem += size;
+                tailStart = c;
+            }
+            if (rem == 0)
+            {
+                tailStart = M3_GRAPH_COLORS + 1;
+            }
+        }
+    }
     // Joints run before the contacts in each pass, serially within a joint
     // component (src/solver.c:4149-4201); with joints the warm-start gather
     // waits for the joint warm start.
@@ -1645,7 +1750,9 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 }
                 sync();
             }
-            for (int32_t color = 0; color <= M3_GRAPH_COLORS; ++color)
+            m3TapePass* rec =
+                tape != nullptr ? tape->pass + ((int64_t)sub * 2 + (mode - 1)) * tape->pairCap : nullptr;
+            for (int32_t color = 0; color < tailStart && color <= M3_GRAPH_COLORS; ++color)
             {
                 int32_t start = starts[color];
                 int32_t end = starts[color + 1];
@@ -1653,9 +1760,26 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 {
                     continue;
                 }
-                m3TapePass* rec =
-                    tape != nullptr ? tape->pass + ((int64_t)sub * 2 + (mode - 1)) * tape->pairCap : nullptr;
                 ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS, rec);
+                sync();
+            }
+            if (tailStart <= M3_GRAPH_COLORS)
+            {
+                if (blockIdx.x == 0)
+                {
+                    for (int32_t color = tailStart; color <= M3_GRAPH_COLORS; ++color)
+                    {
+                        int32_t start = starts[color];
+                        int32_t end = starts[color + 1];
+                        if (end == start)
+                        {
+                            continue;
+                        }
+                        ColorPhase(v, start, end, mode, sp, (int32_t)threadIdx.x, (int32_t)blockDim.x,
+                                   color == M3_GRAPH_COLORS, rec);
+                        __syncthreads();
+                    }
+                }
                 sync();
             }
         }
@@ -1684,7 +1808,7 @@ __global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
         return;
     }
     SubstepLoop(v, sp, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x,
-                m3GridSync{d.barrier, &v.sc->fault});
+                m3GridSync{d.barrier, &v.sc->fault}, nullptr, true);
 }
 
 // A batch: worlds are independent, so each workgroup runs its own world's
@@ -2339,7 +2463,7 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         w->wideHulls = 0;
         for (int32_t k = 0; k < h->maxHull; ++k)
         {
-            if (((const m3HullData*)h->hullData)[k].edgeCount > 24)
+            if (((const m3HullData*)h->hullData)[k].edgeCount >= 24)
             {
                 w->wideHulls = 1;
             }
@@ -2666,7 +2790,7 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
                            ct, tb, hd.sapKeysIn, hd.sapKeysOut, hd.sapValsIn, hd.sapValsOut, ns, 0,
                            64, st)));
     STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
-    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, 64)), dim3(64), 0, st, D));
+    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, M3_SAP_THREADS)), dim3(M3_SAP_THREADS), 0, st, D));
     if (v.planeCount > 0)
     {
         STAGE(ST_PLANES,
@@ -3168,7 +3292,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_pending_all, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_sap_sort_block, dim3(1, W), dim3(BT), 0, st, D);
     KL(k_sap_gather, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
-    KL(k_sap, dim3(Blocks(ns, 64), W), dim3(64), 0, st, D);
+    KL(k_sap, dim3(Blocks(ns, M3_SAP_THREADS), W), dim3(M3_SAP_THREADS), 0, st, D);
     if (s_batch.maxPlaneWork > 0)
     {
         KL(k_planes, dim3(Blocks(s_batch.maxPlaneWork, T), W), dim3(T), 0, st, D);
