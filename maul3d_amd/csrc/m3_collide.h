@@ -204,12 +204,15 @@ typedef struct m3SatQuery
     m3EdgeQuery edge;
 } m3SatQuery;
 
-// Pairs whose edge query has at least box-against-box size (12 x 12 edge
-// pairs) take the wavefront query: its cost grows with the product of the
-// edge counts (up to 186 x 186 for two 64-vertex hulls).
-M3_HD int m3WideHullPair(const m3HullData* hullA, const m3HullData* hullB)
+// Pairs whose edge query is larger than box against box (12 x 12 edge
+// pairs), or at least that large when `boxes` is set, take the wavefront
+// query: its cost grows with the product of the edge counts (up to 186 x 186
+// for two 64-vertex hulls).  One large world sends box pairs there too; a
+// batch of small worlds keeps them one thread per pair.
+M3_HD int m3WideHullPair(const m3HullData* hullA, const m3HullData* hullB, int boxes)
 {
-    return (hullA->edgeCount >> 1) * (hullB->edgeCount >> 1) >= 144;
+    int32_t product = (hullA->edgeCount >> 1) * (hullB->edgeCount >> 1);
+    return boxes ? product >= 144 : product > 144;
 }
 
 // Hull versus hull in A's frame; B arrives through (q, p).  Face queries

@@ -88,6 +88,7 @@ struct m3gpuWorld
     int coopBlocks;  // co-resident blocks of the substep kernel
     int coopThreads; // threads per block of the substep kernel
     int wideHulls;   // some hull is larger than a box (wide hull pairs possible)
+    int boxHulls;    // some hull is at least box-sized
 };
 
 // The entry is read through the constant address space: its fields load as
@@ -706,7 +707,7 @@ __global__ void k_sat_list(const m3Dev* __restrict__ ds)
     {
         return;
     }
-    if (m3WideHullPair(&v.hulls[v.hullIndex[shapeA]], &v.hulls[v.hullIndex[shapeB]]))
+    if (m3WideHullPair(&v.hulls[v.hullIndex[shapeA]], &v.hulls[v.hullIndex[shapeB]], v.satBoxes))
     {
         v.satList[atomicAdd(&v.sc->satCount, 1)] = i;
     }
@@ -2461,12 +2462,12 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(v.hullIndex, h->shapeHullIndex, ns, int32_t);
         UP((void*)v.hulls, h->hullData, h->maxHull, m3HullData);
         w->wideHulls = 0;
+        w->boxHulls = 0;
         for (int32_t k = 0; k < h->maxHull; ++k)
         {
-            if (((const m3HullData*)h->hullData)[k].edgeCount >= 24)
-            {
-                w->wideHulls = 1;
-            }
+            int32_t e = ((const m3HullData*)h->hullData)[k].edgeCount;
+            w->wideHulls |= e > 24;
+            w->boxHulls |= e >= 24;
         }
         if (h->maxJoint > w->caps.jointCapacity)
         {
@@ -2715,6 +2716,7 @@ static int RefreshDevCopies(m3gpuWorld* w)
         copies[parity].v.manifolds = w->mansBuf[parity];
         copies[parity].v.oldKeys = w->keysBuf[parity ^ 1];
         copies[parity].v.oldManifolds = w->mansBuf[parity ^ 1];
+        copies[parity].v.satBoxes = 1;
     }
     HIP_TRY(hipStreamSynchronize(w->stream));
     HIP_TRY(hipMemcpy(w->devDev, copies, sizeof(copies), hipMemcpyHostToDevice));
@@ -2809,7 +2811,7 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
                                                                  v.sleepKeys, &v.sc->sleepCount,
                                                                  np, st)));
     // Narrowphase.
-    if (w->wideHulls)
+    if (w->wideHulls || w->boxHulls)
     {
         STAGE(ST_SAT_LIST, KL(k_sat_list, dim3(Blocks(np, T)), dim3(T), 0, st, D));
         STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
@@ -3141,6 +3143,7 @@ static int BatchBuild(m3gpuWorld* const* ws, int32_t n)
             e.v.manifolds = w->mansBuf[parity];
             e.v.oldKeys = w->keysBuf[parity ^ 1];
             e.v.oldManifolds = w->mansBuf[parity ^ 1];
+            e.v.satBoxes = 0;
         }
         s_batch.worlds[k] = w;
         s_batch.baseCur[k] = w->cur;

@@ -191,6 +191,7 @@ typedef struct m3View
     uint64_t* sleepKeys;
     uint32_t* emit;
     m3SatQuery* satQueries; // per pair: SAT queries of wide hull pairs, evaluated ahead
+    int32_t satBoxes;       // box pairs count as wide (one large world)
     int32_t* satList;       // pair indices of the wide hull pairs (any order)
     // solver scratch
     m3Constraint* cons;   // prepared constraints in color-slot order
@@ -916,7 +917,7 @@ M3_HD void m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
         const m3HullData* hullA = &v->hulls[v->hullIndex[shapeA]];
         const m3HullData* hullB = &v->hulls[v->hullIndex[shapeB]];
         const m3SatQuery* sat =
-            v->satQueries != NULL && m3WideHullPair(hullA, hullB) ? &v->satQueries[i] : NULL;
+            v->satQueries != NULL && m3WideHullPair(hullA, hullB, v->satBoxes) ? &v->satQueries[i] : NULL;
         fresh = m3CollidePairHulls(v, shapeA, shapeB, sat);
     }
     else if (planePair && capsulePair)
