@@ -23,6 +23,14 @@
 #define M3_HD static inline
 #endif
 
+// Full unrolling of a loop with a constant trip count (so that arrays it
+// indexes stay in registers).
+#if defined(__clang__)
+#define M3_UNROLL _Pragma("unroll")
+#else
+#define M3_UNROLL
+#endif
+
 #if defined(__cplusplus) && !defined(M3_REAL)
 extern "C" {
 #endif

@@ -1814,7 +1814,7 @@ __global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
 
 // A batch: worlds are independent, so each workgroup runs its own world's
 // whole substep loop with workgroup barriers.
-#define M3_BLOCK_MAX_THREADS 1024
+#define M3_BLOCK_MAX_THREADS 512
 
 __global__ void __launch_bounds__(M3_BLOCK_MAX_THREADS)
     k_substeps_block(const m3Dev* __restrict__ ds, m3SubstepParams sp)

@@ -1212,67 +1212,99 @@ M3_HD void m3WarmStartOne(const m3View* v, m3Constraint* c)
 }
 
 // The warm-start updates of constraint c applied to one of its bodies only
-// (side 0 = A, 1 = B).  WarmStartOne's impulses depend on stored
-// accumulators, never on velocities, so replaying each body's constraints
-// in color order with this function performs exactly the operations the
-// colored pass performs on that body, in the same order.
-M3_HD void m3WarmStartSide(const m3View* v, const m3Constraint* c, int side)
+// (side 0 = A, 1 = B), on that body's velocity held in registers.
+// WarmStartOne's impulses depend on stored accumulators, never on
+// velocities, so replaying each body's constraints in color order with this
+// function performs exactly the operations the colored pass performs on
+// that body, in the same order.  All fields are loaded before the first use.
+M3_HD void m3WarmStartSide(const m3Constraint* c, int side, m3Vec3* linIo, m3Vec3* angIo)
 {
-    int32_t b = side == 0 ? c->bodyA : c->bodyB;
-    m3Vec3 lin = v->lv[b];
-    m3Vec3 ang = v->av[b];
-    for (int32_t k = 0; k < c->pointCount; ++k)
+    const int32_t pointCount = c->pointCount;
+    const m3Vec3 normal = c->normal;
+    m3Vec3 arm[M3_MANIFOLD_MAX_POINTS];
+    m3real imp[M3_MANIFOLD_MAX_POINTS];
+    M3_UNROLL
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
-        const m3CPoint* cp = &c->points[k];
-        m3Vec3 impulse = m3MulSV3(cp->normalImpulse, c->normal);
+        arm[k] = side == 0 ? c->points[k].rA : c->points[k].rB;
+        imp[k] = c->points[k].normalImpulse;
+    }
+    const m3real invMass = side == 0 ? c->invMassA : c->invMassB;
+    const m3Mat3 invI = side == 0 ? c->invIA : c->invIB;
+    const m3Vec3 t1 = c->t1;
+    const m3Vec3 t2 = c->t2;
+    const m3real f1 = c->frictionImpulse1;
+    const m3real f2 = c->frictionImpulse2;
+    const m3Vec3 origin = side == 0 ? c->originA : c->originB;
+    const m3real twist = c->twistImpulse;
+    const m3Vec3 rolling = c->rollingImpulse;
+    m3Vec3 lin = *linIo;
+    m3Vec3 ang = *angIo;
+    M3_UNROLL
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
+    {
+        if (k >= pointCount)
+        {
+            break;
+        }
+        m3Vec3 impulse = m3MulSV3(imp[k], normal);
         if (side == 0)
         {
-            lin = m3Sub3(lin, m3MulSV3(c->invMassA, impulse));
-            ang = m3Sub3(ang, m3MulMV3(c->invIA, m3Cross3(cp->rA, impulse)));
+            lin = m3Sub3(lin, m3MulSV3(invMass, impulse));
+            ang = m3Sub3(ang, m3MulMV3(invI, m3Cross3(arm[k], impulse)));
         }
         else
         {
-            lin = m3Add3(lin, m3MulSV3(c->invMassB, impulse));
-            ang = m3Add3(ang, m3MulMV3(c->invIB, m3Cross3(cp->rB, impulse)));
+            lin = m3Add3(lin, m3MulSV3(invMass, impulse));
+            ang = m3Add3(ang, m3MulMV3(invI, m3Cross3(arm[k], impulse)));
         }
     }
-    m3Vec3 f = m3Add3(m3MulSV3(c->frictionImpulse1, c->t1), m3MulSV3(c->frictionImpulse2, c->t2));
+    m3Vec3 f = m3Add3(m3MulSV3(f1, t1), m3MulSV3(f2, t2));
     if (side == 0)
     {
-        lin = m3Sub3(lin, m3MulSV3(c->invMassA, f));
-        ang = m3Sub3(ang, m3MulMV3(c->invIA, m3Cross3(c->originA, f)));
+        lin = m3Sub3(lin, m3MulSV3(invMass, f));
+        ang = m3Sub3(ang, m3MulMV3(invI, m3Cross3(origin, f)));
     }
     else
     {
-        lin = m3Add3(lin, m3MulSV3(c->invMassB, f));
-        ang = m3Add3(ang, m3MulMV3(c->invIB, m3Cross3(c->originB, f)));
+        lin = m3Add3(lin, m3MulSV3(invMass, f));
+        ang = m3Add3(ang, m3MulMV3(invI, m3Cross3(origin, f)));
     }
-    m3Vec3 tw = m3Add3(m3MulSV3(c->twistImpulse, c->normal), c->rollingImpulse);
+    m3Vec3 tw = m3Add3(m3MulSV3(twist, normal), rolling);
     if (side == 0)
     {
-        ang = m3Sub3(ang, m3MulMV3(c->invIA, tw));
+        ang = m3Sub3(ang, m3MulMV3(invI, tw));
     }
     else
     {
-        ang = m3Add3(ang, m3MulMV3(c->invIB, tw));
+        ang = m3Add3(ang, m3MulMV3(invI, tw));
     }
-    v->lv[b] = lin;
-    v->av[b] = ang;
+    *linIo = lin;
+    *angIo = ang;
 }
 
 // Warm start of every constraint on dynamic body b in the order the
 // colored pass reaches them: one slot per color 0..15 from the body's slot
 // table, then the overflow color's slots [ovStart, ovEnd) in list order.
+// The body's velocity stays in registers throughout, and the next slot's
+// entry is fetched while the current constraint is applied.
 M3_HD void m3WarmStartBody(const m3View* v, int32_t b, int32_t ovStart, int32_t ovEnd)
 {
-    uint32_t mask = v->slotMask[b];
+    const uint32_t mask = v->slotMask[b];
+    const int32_t* row = v->slotTable + (int64_t)b * M3_GRAPH_COLORS;
+    m3Vec3 lin = v->lv[b];
+    m3Vec3 ang = v->av[b];
     uint32_t colors = mask & 0xFFFFu;
+    int32_t next = colors != 0u ? row[__builtin_ctz(colors)] : 0;
     while (colors != 0u)
     {
-        int32_t color = __builtin_ctz(colors);
+        const int32_t e = next;
         colors &= colors - 1u;
-        int32_t e = v->slotTable[(int64_t)b * M3_GRAPH_COLORS + color];
-        m3WarmStartSide(v, &v->cons[e >> 1], e & 1);
+        if (colors != 0u)
+        {
+            next = row[__builtin_ctz(colors)];
+        }
+        m3WarmStartSide(&v->cons[e >> 1], e & 1, &lin, &ang);
     }
     if ((mask & 0x10000u) != 0u)
     {
@@ -1281,14 +1313,16 @@ M3_HD void m3WarmStartBody(const m3View* v, int32_t b, int32_t ovStart, int32_t 
             const m3Constraint* c = &v->cons[k];
             if (c->dynA && c->bodyA == b)
             {
-                m3WarmStartSide(v, c, 0);
+                m3WarmStartSide(c, 0, &lin, &ang);
             }
             if (c->dynB && c->bodyB == b)
             {
-                m3WarmStartSide(v, c, 1);
+                m3WarmStartSide(c, 1, &lin, &ang);
             }
         }
     }
+    v->lv[b] = lin;
+    v->av[b] = ang;
 }
 
 // Cramer's rule; singular returns zero (src/solver.c:2909-2926).
@@ -1559,118 +1593,234 @@ M3_HD void m3JointComponentPass(const m3View* v, int32_t k, int mode, m3real inv
     }
 }
 
-// src/solver.c:430-581
+// The velocities of a constraint's two bodies, held in registers while the
+// constraint is solved.  The two bodies of a contact constraint are always
+// distinct, so these locals go through exactly the values the body arrays
+// would hold between the individual updates.
+typedef struct m3PairVelocity
+{
+    m3Vec3 vA;
+    m3Vec3 wA;
+    m3Vec3 vB;
+    m3Vec3 wB;
+} m3PairVelocity;
+
+// src/solver.c:430-581.  Every field the pass reads is loaded before the
+// first use, so that the loads are in flight together (a constraint solve
+// is a chain of short dependent steps; without this each manifold point
+// would wait for its own round trip to memory).  The body velocities are
+// read once, updated in registers in the reference's operation order, and
+// written back once: linear and angular velocity of each dynamic side, and
+// the angular velocity of a non-dynamic side that the rolling-resistance
+// row writes.
 M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real pushMax,
                           int useBias)
 {
+    const int32_t a = c->bodyA;
+    const int32_t b = c->bodyB;
+    const int32_t pointCount = c->pointCount;
+    const m3Vec3 normal = c->normal;
+    const m3CPoint* cps = c->points;
+    m3Vec3 prA[M3_MANIFOLD_MAX_POINTS];
+    m3Vec3 prB[M3_MANIFOLD_MAX_POINTS];
+    m3real pBase[M3_MANIFOLD_MAX_POINTS];
+    m3real pMass[M3_MANIFOLD_MAX_POINTS];
+    m3real pLever[M3_MANIFOLD_MAX_POINTS];
+    m3real pImpulse[M3_MANIFOLD_MAX_POINTS];
+    m3real pTotal[M3_MANIFOLD_MAX_POINTS];
+    M3_UNROLL
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
+    {
+        prA[k] = cps[k].rA;
+        prB[k] = cps[k].rB;
+        pBase[k] = cps[k].baseSeparation;
+        pMass[k] = cps[k].normalMass;
+        pLever[k] = cps[k].leverArm;
+        pImpulse[k] = cps[k].normalImpulse;
+        pTotal[k] = cps[k].totalNormalImpulse;
+    }
+    const uint8_t dynA = c->dynA;
+    const uint8_t dynB = c->dynB;
+    const m3real invMassA = c->invMassA;
+    const m3real invMassB = c->invMassB;
+    const m3Mat3 invIA = c->invIA;
+    const m3Mat3 invIB = c->invIB;
+    const m3Softness soft = c->softness;
+    const m3real friction = c->friction;
+    m3PairVelocity s;
+    s.vA = v->lv[a];
+    s.wA = v->av[a];
+    s.vB = v->lv[b];
+    s.wB = v->av[b];
+    const m3Quat dqA = v->deltaRot[a];
+    const m3Quat dqB = v->deltaRot[b];
+    const m3Vec3 dpA = v->deltaPos[a];
+    const m3Vec3 dpB = v->deltaPos[b];
     m3real passNormal = 0.0f;
     m3real twistLimit = 0.0f;
-    m3Quat dqA = v->deltaRot[c->bodyA];
-    m3Quat dqB = v->deltaRot[c->bodyB];
-    m3Vec3 dpA = v->deltaPos[c->bodyA];
-    m3Vec3 dpB = v->deltaPos[c->bodyB];
-    for (int32_t k = 0; k < c->pointCount; ++k)
+    M3_UNROLL
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
-        m3CPoint* cp = &c->points[k];
-        m3Vec3 rsA = m3RotateVec3(dqA, cp->rA);
-        m3Vec3 rsB = m3RotateVec3(dqB, cp->rB);
+        if (k >= pointCount)
+        {
+            break;
+        }
+        const m3Vec3 rA = prA[k];
+        const m3Vec3 rB = prB[k];
+        m3Vec3 rsA = m3RotateVec3(dqA, rA);
+        m3Vec3 rsB = m3RotateVec3(dqB, rB);
         m3Vec3 ds = m3Add3(m3Sub3(dpB, dpA), m3Sub3(rsB, rsA));
-        m3real s = cp->baseSeparation + m3Dot3(ds, c->normal);
+        m3real sep = pBase[k] + m3Dot3(ds, normal);
         m3real bias = 0.0f;
         m3real massScale = 1.0f;
         m3real impulseScale = 0.0f;
-        if (s > 0.0f)
+        if (sep > 0.0f)
         {
-            bias = s * invH;
+            bias = sep * invH;
         }
         else if (useBias)
         {
-            bias = m3MaxF(c->softness.biasRate * s, -pushMax);
-            massScale = c->softness.massScale;
-            impulseScale = c->softness.impulseScale;
+            bias = m3MaxF(soft.biasRate * sep, -pushMax);
+            massScale = soft.massScale;
+            impulseScale = soft.impulseScale;
         }
-        m3real vn = m3Dot3(
-            m3Sub3(m3VelocityAt(v, c->bodyB, cp->rB), m3VelocityAt(v, c->bodyA, cp->rA)), c->normal);
-        m3real impulse = -cp->normalMass * massScale * (vn + bias) - impulseScale * cp->normalImpulse;
-        m3real newImpulse = m3MaxF(cp->normalImpulse + impulse, 0.0f);
-        impulse = newImpulse - cp->normalImpulse;
+        m3Vec3 velB = m3Add3(s.vB, m3Cross3(s.wB, rB));
+        m3Vec3 velA = m3Add3(s.vA, m3Cross3(s.wA, rA));
+        m3real vn = m3Dot3(m3Sub3(velB, velA), normal);
+        const m3real oldImpulse = pImpulse[k];
+        m3real impulse = -pMass[k] * massScale * (vn + bias) - impulseScale * oldImpulse;
+        m3real newImpulse = m3MaxF(oldImpulse + impulse, 0.0f);
+        impulse = newImpulse - oldImpulse;
+        m3CPoint* cp = &c->points[k];
         cp->normalImpulse = newImpulse;
-        cp->totalNormalImpulse += newImpulse;
+        cp->totalNormalImpulse = pTotal[k] + newImpulse;
         passNormal += newImpulse;
-        twistLimit += cp->leverArm * newImpulse;
-        m3ApplyImpulse(v, c, m3MulSV3(impulse, c->normal), cp->rA, cp->rB);
-    }
-    if (useBias)
-    {
-        return;
-    }
-    {
-        m3Vec3 wRel = m3Sub3(v->av[c->bodyB], v->av[c->bodyA]);
-        m3real twistSpeed = m3Dot3(c->normal, wRel);
-        m3real maxTwist = c->friction * twistLimit;
-        m3real delta = -c->twistMass * twistSpeed;
-        m3real oldTwist = c->twistImpulse;
-        m3real newTwist = oldTwist + delta;
-        if (newTwist < -maxTwist)
+        twistLimit += pLever[k] * newImpulse;
+        const m3Vec3 P = m3MulSV3(impulse, normal);
+        if (dynA)
         {
-            newTwist = -maxTwist;
+            s.vA = m3Sub3(s.vA, m3MulSV3(invMassA, P));
+            s.wA = m3Sub3(s.wA, m3MulMV3(invIA, m3Cross3(rA, P)));
         }
-        else if (newTwist > maxTwist)
+        if (dynB)
         {
-            newTwist = maxTwist;
-        }
-        delta = newTwist - oldTwist;
-        c->twistImpulse = newTwist;
-        m3Vec3 tw = m3MulSV3(delta, c->normal);
-        if (c->dynA)
-        {
-            v->av[c->bodyA] = m3Sub3(v->av[c->bodyA], m3MulMV3(c->invIA, tw));
-        }
-        if (c->dynB)
-        {
-            v->av[c->bodyB] = m3Add3(v->av[c->bodyB], m3MulMV3(c->invIB, tw));
+            s.vB = m3Add3(s.vB, m3MulSV3(invMassB, P));
+            s.wB = m3Add3(s.wB, m3MulMV3(invIB, m3Cross3(rB, P)));
         }
     }
-    if (c->rollingResistance > 0.0f)
+    int rolled = 0;
+    if (!useBias)
     {
-        m3Vec3 wRel = m3Sub3(v->av[c->bodyB], v->av[c->bodyA]);
-        m3Vec3 delta = m3MulSV3(-1.0f, m3Solve3(&c->rollingK, wRel));
-        m3Vec3 accum = m3Add3(c->rollingImpulse, delta);
-        m3real maxRoll = c->rollingResistance * passNormal;
-        m3real mag2 = m3Dot3(accum, accum);
-        if (mag2 > maxRoll * maxRoll && mag2 > 0.0f)
+        const m3Vec3 t1 = c->t1;
+        const m3Vec3 t2 = c->t2;
+        const m3Vec3 originA = c->originA;
+        const m3Vec3 originB = c->originB;
+        const m3real k11 = c->frictionK11;
+        const m3real k12 = c->frictionK12;
+        const m3real k22 = c->frictionK22;
+        const m3real oldF1 = c->frictionImpulse1;
+        const m3real oldF2 = c->frictionImpulse2;
+        const m3real tv1 = c->tangentVelocity1;
+        const m3real tv2 = c->tangentVelocity2;
+        const m3real twistMass = c->twistMass;
+        const m3real oldTwist = c->twistImpulse;
+        const m3real rolling = c->rollingResistance;
         {
-            accum = m3MulSV3(maxRoll / m3Sqrt(mag2), accum);
+            m3Vec3 wRel = m3Sub3(s.wB, s.wA);
+            m3real twistSpeed = m3Dot3(normal, wRel);
+            m3real maxTwist = friction * twistLimit;
+            m3real delta = -twistMass * twistSpeed;
+            m3real newTwist = oldTwist + delta;
+            if (newTwist < -maxTwist)
+            {
+                newTwist = -maxTwist;
+            }
+            else if (newTwist > maxTwist)
+            {
+                newTwist = maxTwist;
+            }
+            delta = newTwist - oldTwist;
+            c->twistImpulse = newTwist;
+            m3Vec3 tw = m3MulSV3(delta, normal);
+            if (dynA)
+            {
+                s.wA = m3Sub3(s.wA, m3MulMV3(invIA, tw));
+            }
+            if (dynB)
+            {
+                s.wB = m3Add3(s.wB, m3MulMV3(invIB, tw));
+            }
         }
-        delta = m3Sub3(accum, c->rollingImpulse);
-        c->rollingImpulse = accum;
-        // The reference writes both sides unconditionally here; a body
-        // without inverse inertia receives a zero change.
-        v->av[c->bodyA] = m3Sub3(v->av[c->bodyA], m3MulMV3(c->invIA, delta));
-        v->av[c->bodyB] = m3Add3(v->av[c->bodyB], m3MulMV3(c->invIB, delta));
+        if (rolling > 0.0f)
+        {
+            m3Vec3 wRel = m3Sub3(s.wB, s.wA);
+            m3Vec3 delta = m3MulSV3(-1.0f, m3Solve3(&c->rollingK, wRel));
+            m3Vec3 accum = m3Add3(c->rollingImpulse, delta);
+            m3real maxRoll = rolling * passNormal;
+            m3real mag2 = m3Dot3(accum, accum);
+            if (mag2 > maxRoll * maxRoll && mag2 > 0.0f)
+            {
+                accum = m3MulSV3(maxRoll / m3Sqrt(mag2), accum);
+            }
+            delta = m3Sub3(accum, c->rollingImpulse);
+            c->rollingImpulse = accum;
+            // The reference writes both sides unconditionally here; a body
+            // without inverse inertia receives a zero change.
+            s.wA = m3Sub3(s.wA, m3MulMV3(invIA, delta));
+            s.wB = m3Add3(s.wB, m3MulMV3(invIB, delta));
+            rolled = 1;
+        }
+        {
+            m3Vec3 velB = m3Add3(s.vB, m3Cross3(s.wB, originB));
+            m3Vec3 velA = m3Add3(s.vA, m3Cross3(s.wA, originA));
+            m3Vec3 vrel = m3Sub3(velB, velA);
+            m3real vt1 = m3Dot3(vrel, t1) - tv1;
+            m3real vt2 = m3Dot3(vrel, t2) - tv2;
+            m3real f1 = oldF1 - (k11 * vt1 + k12 * vt2);
+            m3real f2 = oldF2 - (k12 * vt1 + k22 * vt2);
+            m3real maxFriction = friction * passNormal;
+            m3real mag2 = f1 * f1 + f2 * f2;
+            if (mag2 > maxFriction * maxFriction)
+            {
+                m3real mag = m3Sqrt(mag2);
+                m3real scale = mag > 0.0f ? maxFriction / mag : 0.0f;
+                f1 *= scale;
+                f2 *= scale;
+            }
+            m3real d1 = f1 - oldF1;
+            m3real d2 = f2 - oldF2;
+            c->frictionImpulse1 = f1;
+            c->frictionImpulse2 = f2;
+            const m3Vec3 P = m3Add3(m3MulSV3(d1, t1), m3MulSV3(d2, t2));
+            if (dynA)
+            {
+                s.vA = m3Sub3(s.vA, m3MulSV3(invMassA, P));
+                s.wA = m3Sub3(s.wA, m3MulMV3(invIA, m3Cross3(originA, P)));
+            }
+            if (dynB)
+            {
+                s.vB = m3Add3(s.vB, m3MulSV3(invMassB, P));
+                s.wB = m3Add3(s.wB, m3MulMV3(invIB, m3Cross3(originB, P)));
+            }
+        }
     }
+    if (dynA)
     {
-        m3Vec3 vrel = m3Sub3(m3VelocityAt(v, c->bodyB, c->originB),
-                             m3VelocityAt(v, c->bodyA, c->originA));
-        m3real vt1 = m3Dot3(vrel, c->t1) - c->tangentVelocity1;
-        m3real vt2 = m3Dot3(vrel, c->t2) - c->tangentVelocity2;
-        m3real f1 = c->frictionImpulse1 - (c->frictionK11 * vt1 + c->frictionK12 * vt2);
-        m3real f2 = c->frictionImpulse2 - (c->frictionK12 * vt1 + c->frictionK22 * vt2);
-        m3real maxFriction = c->friction * passNormal;
-        m3real mag2 = f1 * f1 + f2 * f2;
-        if (mag2 > maxFriction * maxFriction)
-        {
-            m3real mag = m3Sqrt(mag2);
-            m3real scale = mag > 0.0f ? maxFriction / mag : 0.0f;
-            f1 *= scale;
-            f2 *= scale;
-        }
-        m3real d1 = f1 - c->frictionImpulse1;
-        m3real d2 = f2 - c->frictionImpulse2;
-        c->frictionImpulse1 = f1;
-        c->frictionImpulse2 = f2;
-        m3ApplyImpulse(v, c, m3Add3(m3MulSV3(d1, c->t1), m3MulSV3(d2, c->t2)), c->originA,
-                       c->originB);
+        v->lv[a] = s.vA;
+        v->av[a] = s.wA;
+    }
+    else if (rolled)
+    {
+        v->av[a] = s.wA;
+    }
+    if (dynB)
+    {
+        v->lv[b] = s.vB;
+        v->av[b] = s.wB;
+    }
+    else if (rolled)
+    {
+        v->av[b] = s.wB;
     }
 }
 
