@@ -85,7 +85,8 @@ struct m3gpuWorld
     size_t cubBytes;
     m3Scalars* hostScalars; // pinned mirror for stats
     struct m3Profiler* prof;
-    int coopBlocks;  // co-resident blocks of the substep kernel
+    int coopBlocks;  // co-resident blocks of the substep kernel (maximum)
+    int coopFixed;   // coopBlocks set from the environment: launch exactly that many
     int coopThreads; // threads per block of the substep kernel
     int wideHulls;   // some hull is larger than a box (wide hull pairs possible)
     int boxHulls;    // some hull is at least box-sized
@@ -1575,11 +1576,60 @@ __device__ static inline void TapeSolve(const m3View& v, const m3Constraint* c, 
     r->twist = c->twistImpulse;
 }
 
+// The contact constraint a participant solves first in its next color
+// phase, gathered before the barrier that precedes that phase: its loads
+// then overlap the barrier instead of following it.  A constraint's slot
+// maps to the same participant in every pass, and the fields gathered are
+// read-only in the substep loop or written only by that participant's own
+// solve of the constraint, so the copy is current when it is used.
+struct m3Prefetch
+{
+    int32_t k; // constraint slot, or -1
+    m3ContactIn in;
+};
+
+template <int kMode>
+__device__ static inline void GatherFirst(const m3View& v, int32_t k, m3Prefetch* pf)
+{
+    pf->k = k;
+    if (k >= 0)
+    {
+        m3LoadContact(&v.cons[k], kMode == 1, &pf->in);
+    }
+}
+
+template <int kMode>
+__device__ static inline void ColorRun(const m3View& v, int32_t start, int32_t end,
+                                       const m3SubstepParams& sp, int32_t gid, int32_t gsz,
+                                       m3TapePass* rec, const m3Prefetch* pf)
+{
+    int32_t k = start + gid;
+    if (k < end && pf->k == k)
+    {
+        if (rec != nullptr)
+        {
+            TapeSolve(v, &v.cons[k], &rec[k]);
+        }
+        m3SolveContactIn(&v, &v.cons[k], &pf->in, sp.invH, sp.push, kMode == 1);
+        k += gsz;
+    }
+    for (; k < end; k += gsz)
+    {
+        if (rec != nullptr)
+        {
+            TapeSolve(v, &v.cons[k], &rec[k]);
+        }
+        m3SolveContact(&v, &v.cons[k], sp.invH, sp.push, kMode == 1);
+    }
+}
+
 // One graph color of one pass (mode 1 solve with bias, 2 relax) over the
-// participants; the overflow color runs serially on one thread.
+// participants; the overflow color runs serially on one thread.  Each
+// pass has a loop of its own so that every inlined solve sees a constant
+// mode.
 __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t end, int mode,
                                          const m3SubstepParams& sp, int32_t gid, int32_t gsz,
-                                         bool serial, m3TapePass* rec)
+                                         bool serial, m3TapePass* rec, const m3Prefetch* pf)
 {
     if (serial)
     {
@@ -1596,13 +1646,13 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
         }
         return;
     }
-    for (int32_t k = start + gid; k < end; k += gsz)
+    if (mode == 1)
     {
-        if (rec != nullptr)
-        {
-            TapeSolve(v, &v.cons[k], &rec[k]);
-        }
-        m3SolveContact(&v, &v.cons[k], sp.invH, sp.push, mode == 1);
+        ColorRun<1>(v, start, end, sp, gid, gsz, rec, pf);
+    }
+    else
+    {
+        ColorRun<2>(v, start, end, sp, gid, gsz, rec, pf);
     }
 }
 
@@ -1675,6 +1725,52 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
     // waits for the joint warm start.
     const bool joints = v.sc->jointActive > 0;
     const int32_t comps = v.jointCompCount;
+    // This participant's first slot in a color of a pass (-1: none, or the
+    // serial overflow color), and the next non-empty color after one.
+    auto firstSlot = [&](int32_t color) -> int32_t {
+        if (color < 0 || color >= M3_GRAPH_COLORS)
+        {
+            return -1;
+        }
+        int32_t k = starts[color];
+        if (color < tailStart)
+        {
+            k += gid;
+        }
+        else
+        {
+            if (blockIdx.x != 0)
+            {
+                return -1;
+            }
+            k += (int32_t)threadIdx.x;
+        }
+        return k < starts[color + 1] ? k : -1;
+    };
+    auto nextColor = [&](int32_t color) -> int32_t {
+        for (int32_t c = color + 1; c <= M3_GRAPH_COLORS; ++c)
+        {
+            if (starts[c + 1] > starts[c])
+            {
+                return c;
+            }
+        }
+        return -1;
+    };
+    const int32_t firstColor = nextColor(-1);
+    m3Prefetch pf;
+    pf.k = -1;
+    auto gather = [&](int mode, int32_t color) {
+        const int32_t k = firstSlot(color);
+        if (mode == 1)
+        {
+            GatherFirst<1>(v, k, &pf);
+        }
+        else
+        {
+            GatherFirst<2>(v, k, &pf);
+        }
+    };
     for (int32_t sub = 0; sub < sp.substeps; ++sub)
     {
         if (tape != nullptr)
@@ -1698,6 +1794,10 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
             {
                 m3WarmStartBody(&v, b, starts[M3_GRAPH_COLORS], starts[M3_GRAPH_COLORS + 1]);
             }
+        }
+        if (!joints)
+        {
+            gather(1, firstColor);
         }
         sync();
         if (joints)
@@ -1732,6 +1832,10 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 {
                     m3IntegratePosition(&v, v.movers[m], sp.h);
                 }
+                if (!joints)
+                {
+                    gather(2, firstColor);
+                }
                 sync();
                 if (tape != nullptr)
                 {
@@ -1749,6 +1853,7 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 {
                     m3JointComponentPass(&v, k, mode, sp.invH);
                 }
+                gather(mode, firstColor);
                 sync();
             }
             m3TapePass* rec =
@@ -1761,7 +1866,8 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 {
                     continue;
                 }
-                ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS, rec);
+                ColorPhase(v, start, end, mode, sp, gid, gsz, color == M3_GRAPH_COLORS, rec, &pf);
+                gather(mode, nextColor(color));
                 sync();
             }
             if (tailStart <= M3_GRAPH_COLORS)
@@ -1777,7 +1883,8 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                             continue;
                         }
                         ColorPhase(v, start, end, mode, sp, (int32_t)threadIdx.x, (int32_t)blockDim.x,
-                                   color == M3_GRAPH_COLORS, rec);
+                                   color == M3_GRAPH_COLORS, rec, &pf);
+                        gather(mode, nextColor(color));
                         __syncthreads();
                     }
                 }
@@ -2309,9 +2416,14 @@ static int CreateImpl(m3gpuWorld* w)
         w->coopThreads = M3_SUBSTEP_THREADS;
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_substeps, w->coopThreads, 0));
         int maxBlocks = perCu * prop.multiProcessorCount;
+        // Up to 40 workgroups of 256: one body per thread in the
+        // integrate-and-warm-start phase for worlds up to 10,240 bodies,
+        // while the grid barrier stays near its few-microsecond floor.  A
+        // world with fewer bodies launches fewer (EnqueueStep).
         const char* env = getenv("MAUL3D_AMD_SUBSTEP_BLOCKS");
-        int want = env != nullptr ? atoi(env) : 32;
+        int want = env != nullptr ? atoi(env) : 40;
         w->coopBlocks = want < 1 ? 1 : (want > maxBlocks ? maxBlocks : want);
+        w->coopFixed = env != nullptr;
     }
     HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
     HIP_TRY(hipMalloc(&w->devDev, 2 * sizeof(m3Dev)));
@@ -2858,8 +2970,13 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     {
         m3SubstepParams spArg = sp;
         void* args[] = {(void*)&D, (void*)&spArg};
-        STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps,
-                                                              dim3(w->coopBlocks),
+        int blocks = w->coopBlocks;
+        if (!w->coopFixed)
+        {
+            int perBody = (nb + w->coopThreads - 1) / w->coopThreads;
+            blocks = perBody < 1 ? 1 : (perBody < blocks ? perBody : blocks);
+        }
+        STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps, dim3(blocks),
                                                               dim3(w->coopThreads), args, 0, st)));
     }
     STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, D));

@@ -1240,23 +1240,28 @@ M3_HD void m3WarmStartSide(const m3Constraint* c, int side, m3Vec3* linIo, m3Vec
     const m3Vec3 rolling = c->rollingImpulse;
     m3Vec3 lin = *linIo;
     m3Vec3 ang = *angIo;
+    // Slots past pointCount are evaluated and discarded (no branch on the
+    // point count between the loads and their use).
     M3_UNROLL
     for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
-        if (k >= pointCount)
-        {
-            break;
-        }
         m3Vec3 impulse = m3MulSV3(imp[k], normal);
+        m3Vec3 nl;
+        m3Vec3 na;
         if (side == 0)
         {
-            lin = m3Sub3(lin, m3MulSV3(invMass, impulse));
-            ang = m3Sub3(ang, m3MulMV3(invI, m3Cross3(arm[k], impulse)));
+            nl = m3Sub3(lin, m3MulSV3(invMass, impulse));
+            na = m3Sub3(ang, m3MulMV3(invI, m3Cross3(arm[k], impulse)));
         }
         else
         {
-            lin = m3Add3(lin, m3MulSV3(invMass, impulse));
-            ang = m3Add3(ang, m3MulMV3(invI, m3Cross3(arm[k], impulse)));
+            nl = m3Add3(lin, m3MulSV3(invMass, impulse));
+            na = m3Add3(ang, m3MulMV3(invI, m3Cross3(arm[k], impulse)));
+        }
+        if (k < pointCount)
+        {
+            lin = nl;
+            ang = na;
         }
     }
     m3Vec3 f = m3Add3(m3MulSV3(f1, t1), m3MulSV3(f2, t2));
@@ -1605,22 +1610,20 @@ typedef struct m3PairVelocity
     m3Vec3 wB;
 } m3PairVelocity;
 
-// src/solver.c:430-581.  Every field the pass reads is loaded before the
-// first use, so that the loads are in flight together (a constraint solve
-// is a chain of short dependent steps; without this each manifold point
-// would wait for its own round trip to memory).  The body velocities are
-// read once, updated in registers in the reference's operation order, and
-// written back once: linear and angular velocity of each dynamic side, and
-// the angular velocity of a non-dynamic side that the rolling-resistance
-// row writes.
-M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real pushMax,
-                          int useBias)
+// The fields of one contact constraint that a solve or relax pass reads,
+// gathered into registers.  Loading them apart from the solve lets a
+// caller issue the loads early (before the barrier that precedes the
+// constraint's color); the fields a constraint's own solve writes belong to
+// that solve alone, so a copy taken after the constraint's previous solve
+// stays current.
+typedef struct m3ContactIn
 {
-    const int32_t a = c->bodyA;
-    const int32_t b = c->bodyB;
-    const int32_t pointCount = c->pointCount;
-    const m3Vec3 normal = c->normal;
-    const m3CPoint* cps = c->points;
+    int32_t a;
+    int32_t b;
+    int32_t pointCount;
+    uint8_t dynA;
+    uint8_t dynB;
+    m3Vec3 normal;
     m3Vec3 prA[M3_MANIFOLD_MAX_POINTS];
     m3Vec3 prB[M3_MANIFOLD_MAX_POINTS];
     m3real pBase[M3_MANIFOLD_MAX_POINTS];
@@ -1628,25 +1631,105 @@ M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real 
     m3real pLever[M3_MANIFOLD_MAX_POINTS];
     m3real pImpulse[M3_MANIFOLD_MAX_POINTS];
     m3real pTotal[M3_MANIFOLD_MAX_POINTS];
+    m3real invMassA;
+    m3real invMassB;
+    m3Mat3 invIA;
+    m3Mat3 invIB;
+    m3Softness soft;
+    m3real friction;
+    // Relax pass only: twist, rolling and friction rows.
+    m3Vec3 t1;
+    m3Vec3 t2;
+    m3Vec3 originA;
+    m3Vec3 originB;
+    m3real k11;
+    m3real k12;
+    m3real k22;
+    m3real oldF1;
+    m3real oldF2;
+    m3real tv1;
+    m3real tv2;
+    m3real twistMass;
+    m3real oldTwist;
+    m3real rolling;
+} m3ContactIn;
+
+// All loads are issued together; with a constant useBias the relax-only
+// fields of a solve pass are not loaded at all.
+M3_HD void m3LoadContact(const m3Constraint* c, int useBias, m3ContactIn* in)
+{
+    in->a = c->bodyA;
+    in->b = c->bodyB;
+    in->pointCount = c->pointCount;
+    in->normal = c->normal;
     M3_UNROLL
     for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
-        prA[k] = cps[k].rA;
-        prB[k] = cps[k].rB;
-        pBase[k] = cps[k].baseSeparation;
-        pMass[k] = cps[k].normalMass;
-        pLever[k] = cps[k].leverArm;
-        pImpulse[k] = cps[k].normalImpulse;
-        pTotal[k] = cps[k].totalNormalImpulse;
+        const m3CPoint* cp = &c->points[k];
+        in->prA[k] = cp->rA;
+        in->prB[k] = cp->rB;
+        in->pBase[k] = cp->baseSeparation;
+        in->pMass[k] = cp->normalMass;
+        in->pLever[k] = cp->leverArm;
+        in->pImpulse[k] = cp->normalImpulse;
+        in->pTotal[k] = cp->totalNormalImpulse;
     }
-    const uint8_t dynA = c->dynA;
-    const uint8_t dynB = c->dynB;
-    const m3real invMassA = c->invMassA;
-    const m3real invMassB = c->invMassB;
-    const m3Mat3 invIA = c->invIA;
-    const m3Mat3 invIB = c->invIB;
-    const m3Softness soft = c->softness;
-    const m3real friction = c->friction;
+    in->dynA = c->dynA;
+    in->dynB = c->dynB;
+    in->invMassA = c->invMassA;
+    in->invMassB = c->invMassB;
+    in->invIA = c->invIA;
+    in->invIB = c->invIB;
+    in->soft = c->softness;
+    in->friction = c->friction;
+    if (!useBias)
+    {
+        in->t1 = c->t1;
+        in->t2 = c->t2;
+        in->originA = c->originA;
+        in->originB = c->originB;
+        in->k11 = c->frictionK11;
+        in->k12 = c->frictionK12;
+        in->k22 = c->frictionK22;
+        in->oldF1 = c->frictionImpulse1;
+        in->oldF2 = c->frictionImpulse2;
+        in->tv1 = c->tangentVelocity1;
+        in->tv2 = c->tangentVelocity2;
+        in->twistMass = c->twistMass;
+        in->oldTwist = c->twistImpulse;
+        in->rolling = c->rollingResistance;
+    }
+}
+
+// src/solver.c:430-581 on a gathered constraint (m3LoadContact of c with
+// the same useBias).  A constraint solve is a chain of short dependent
+// steps, so memory round trips dominate it: the body velocities are read
+// once, updated in registers in the reference's operation order, and
+// written back once: linear and angular velocity of each dynamic side, and
+// the angular velocity of a non-dynamic side that the rolling-resistance
+// row writes.
+M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn* in, m3real invH,
+                            m3real pushMax, int useBias)
+{
+    const int32_t a = in->a;
+    const int32_t b = in->b;
+    const int32_t pointCount = in->pointCount;
+    const m3Vec3 normal = in->normal;
+    const uint8_t dynA = in->dynA;
+    const uint8_t dynB = in->dynB;
+    const m3real invMassA = in->invMassA;
+    const m3real invMassB = in->invMassB;
+    const m3Mat3 invIA = in->invIA;
+    const m3Mat3 invIB = in->invIB;
+    const m3Softness soft = in->soft;
+    const m3real friction = in->friction;
+    const m3Vec3* prA = in->prA;
+    const m3Vec3* prB = in->prB;
+    const m3real* pBase = in->pBase;
+    const m3real* pMass = in->pMass;
+    const m3real* pLever = in->pLever;
+    const m3real* pImpulse = in->pImpulse;
+    const m3real* pTotal = in->pTotal;
     m3PairVelocity s;
     s.vA = v->lv[a];
     s.wA = v->av[a];
@@ -1658,13 +1741,13 @@ M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real 
     const m3Vec3 dpB = v->deltaPos[b];
     m3real passNormal = 0.0f;
     m3real twistLimit = 0.0f;
+    // Every slot is evaluated and the results of slots past pointCount are
+    // discarded, so no load of a later point waits behind a branch on the
+    // point count.
     M3_UNROLL
     for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
-        if (k >= pointCount)
-        {
-            break;
-        }
+        const int live = k < pointCount;
         const m3Vec3 rA = prA[k];
         const m3Vec3 rB = prB[k];
         m3Vec3 rsA = m3RotateVec3(dqA, rA);
@@ -1691,40 +1774,47 @@ M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real 
         m3real impulse = -pMass[k] * massScale * (vn + bias) - impulseScale * oldImpulse;
         m3real newImpulse = m3MaxF(oldImpulse + impulse, 0.0f);
         impulse = newImpulse - oldImpulse;
-        m3CPoint* cp = &c->points[k];
-        cp->normalImpulse = newImpulse;
-        cp->totalNormalImpulse = pTotal[k] + newImpulse;
-        passNormal += newImpulse;
-        twistLimit += pLever[k] * newImpulse;
-        const m3Vec3 P = m3MulSV3(impulse, normal);
-        if (dynA)
+        if (live)
         {
-            s.vA = m3Sub3(s.vA, m3MulSV3(invMassA, P));
-            s.wA = m3Sub3(s.wA, m3MulMV3(invIA, m3Cross3(rA, P)));
+            m3CPoint* cp = &c->points[k];
+            cp->normalImpulse = newImpulse;
+            cp->totalNormalImpulse = pTotal[k] + newImpulse;
+            passNormal += newImpulse;
+            twistLimit += pLever[k] * newImpulse;
         }
-        if (dynB)
+        const m3Vec3 P = m3MulSV3(impulse, normal);
+        const m3Vec3 nvA = m3Sub3(s.vA, m3MulSV3(invMassA, P));
+        const m3Vec3 nwA = m3Sub3(s.wA, m3MulMV3(invIA, m3Cross3(rA, P)));
+        const m3Vec3 nvB = m3Add3(s.vB, m3MulSV3(invMassB, P));
+        const m3Vec3 nwB = m3Add3(s.wB, m3MulMV3(invIB, m3Cross3(rB, P)));
+        if (live && dynA)
         {
-            s.vB = m3Add3(s.vB, m3MulSV3(invMassB, P));
-            s.wB = m3Add3(s.wB, m3MulMV3(invIB, m3Cross3(rB, P)));
+            s.vA = nvA;
+            s.wA = nwA;
+        }
+        if (live && dynB)
+        {
+            s.vB = nvB;
+            s.wB = nwB;
         }
     }
     int rolled = 0;
     if (!useBias)
     {
-        const m3Vec3 t1 = c->t1;
-        const m3Vec3 t2 = c->t2;
-        const m3Vec3 originA = c->originA;
-        const m3Vec3 originB = c->originB;
-        const m3real k11 = c->frictionK11;
-        const m3real k12 = c->frictionK12;
-        const m3real k22 = c->frictionK22;
-        const m3real oldF1 = c->frictionImpulse1;
-        const m3real oldF2 = c->frictionImpulse2;
-        const m3real tv1 = c->tangentVelocity1;
-        const m3real tv2 = c->tangentVelocity2;
-        const m3real twistMass = c->twistMass;
-        const m3real oldTwist = c->twistImpulse;
-        const m3real rolling = c->rollingResistance;
+        const m3Vec3 t1 = in->t1;
+        const m3Vec3 t2 = in->t2;
+        const m3Vec3 originA = in->originA;
+        const m3Vec3 originB = in->originB;
+        const m3real k11 = in->k11;
+        const m3real k12 = in->k12;
+        const m3real k22 = in->k22;
+        const m3real oldF1 = in->oldF1;
+        const m3real oldF2 = in->oldF2;
+        const m3real tv1 = in->tv1;
+        const m3real tv2 = in->tv2;
+        const m3real twistMass = in->twistMass;
+        const m3real oldTwist = in->oldTwist;
+        const m3real rolling = in->rolling;
         {
             m3Vec3 wRel = m3Sub3(s.wB, s.wA);
             m3real twistSpeed = m3Dot3(normal, wRel);
@@ -1822,6 +1912,15 @@ M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real 
     {
         v->av[b] = s.wB;
     }
+}
+
+// src/solver.c:430-581 for one constraint.
+M3_HD void m3SolveContact(const m3View* v, m3Constraint* c, m3real invH, m3real pushMax,
+                          int useBias)
+{
+    m3ContactIn in;
+    m3LoadContact(c, useBias, &in);
+    m3SolveContactIn(v, c, &in, invH, pushMax, useBias);
 }
 
 // src/solver.c:2879-2907 for one constraint.
