@@ -1298,14 +1298,30 @@ __global__ void k_store_joints(const m3Dev* __restrict__ ds)
 }
 
 // Greedy first-fit coloring in constraint order (src/solver.c:2741-2822),
-// reproduced exactly by one workgroup.  Constraints are taken 1024 at a
-// time; inside a chunk, a constraint colors in the first round in which it
-// is the lowest pending constraint on each of its awake dynamic bodies
-// (an owner table claimed with atomic min), so every earlier constraint that
-// shares a body has already updated that body's mask.  Body masks and
+// reproduced exactly by one workgroup.  A constraint colors in the first
+// round in which it is the lowest pending constraint on each of its awake
+// dynamic bodies (an owner table claimed with atomic min), so every earlier
+// constraint that shares a body has already updated that body's mask.  Body masks and
 // owners live in LDS for worlds up to M3_COLOR_LDS_BODIES bodies.
 #define M3_COLOR_THREADS 1024
 #define M3_COLOR_LDS_BODIES 16384
+
+// Round barrier of the coloring loop: with the tables in LDS it orders LDS
+// only, so global loads and stores stay in flight across it.
+template <bool kLds>
+__device__ static inline void ColorBarrier()
+{
+    if (kLds)
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        __builtin_amdgcn_s_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    }
+    else
+    {
+        __syncthreads();
+    }
+}
 
 template <bool kLds>
 __global__ void __launch_bounds__(M3_COLOR_THREADS)
@@ -1336,78 +1352,122 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
     {
         counts[t] = 0;
     }
-    __syncthreads();
-    for (int32_t base = 0; base < n; base += M3_COLOR_THREADS)
+    __shared__ int32_t cursor;
+    __shared__ int32_t anyPending[2];
+    if (t == 0)
     {
-        int32_t c = base + t;
-        int32_t A = -1;
-        int32_t B = -1;
-        int pend = c < n;
-        if (pend)
-        {
-            A = v.consBodies[2 * c];
-            B = v.consBodies[2 * c + 1];
-        }
-        while (true)
-        {
-            if (pend)
-            {
-                if (A >= 0)
-                {
-                    atomicMin(&owner[A], t);
-                }
-                if (B >= 0)
-                {
-                    atomicMin(&owner[B], t);
-                }
-            }
-            if (!__syncthreads_or(pend))
-            {
-                break;
-            }
-            int ready = pend && (A < 0 || owner[A] == t) && (B < 0 || owner[B] == t);
-            if (ready)
-            {
-                uint32_t mask = (A >= 0 ? (uint32_t)masks[A] : 0u) | (B >= 0 ? (uint32_t)masks[B] : 0u);
-                int32_t color = M3_GRAPH_COLORS;
-                for (int32_t bit = 0; bit < M3_GRAPH_COLORS; ++bit)
-                {
-                    if ((mask & (1u << bit)) == 0u)
-                    {
-                        color = bit;
-                        break;
-                    }
-                }
-                if (color < M3_GRAPH_COLORS)
-                {
-                    if (A >= 0)
-                    {
-                        masks[A] = (uint16_t)(masks[A] | (1u << color));
-                    }
-                    if (B >= 0)
-                    {
-                        masks[B] = (uint16_t)(masks[B] | (1u << color));
-                    }
-                }
-                v.colors[c] = (uint8_t)color;
-                atomicAdd(&counts[color], 1);
-            }
-            __syncthreads();
-            if (ready)
-            {
-                pend = 0;
-                if (A >= 0)
-                {
-                    owner[A] = 0x7FFFFFFF;
-                }
-                if (B >= 0)
-                {
-                    owner[B] = 0x7FFFFFFF;
-                }
-            }
-            __syncthreads();
-        }
+        cursor = M3_COLOR_THREADS;
+        anyPending[0] = 0;
+        anyPending[1] = 0;
     }
+    __syncthreads();
+    // A sliding window of one pending constraint per participant: a
+    // participant whose constraint colors takes the next unclaimed index.
+    // Taken indices exceed every constraint already in the window, so the
+    // lowest-pending rule still sees every earlier constraint on a body.
+    // With the tables in LDS the round barriers order LDS only, and the
+    // load of a newly taken constraint's bodies overlaps the barrier.
+    const int32_t lane = t & 63;
+    int32_t cur = t < n ? t : -1;
+    int32_t A = -1;
+    int32_t B = -1;
+    if (cur >= 0)
+    {
+        A = v.consBodies[2 * cur];
+        B = v.consBodies[2 * cur + 1];
+    }
+    for (int32_t round = 0;; ++round)
+    {
+        if (cur >= 0)
+        {
+            if (A >= 0)
+            {
+                atomicMin(&owner[A], cur);
+            }
+            if (B >= 0)
+            {
+                atomicMin(&owner[B], cur);
+            }
+            anyPending[round & 1] = 1;
+        }
+        ColorBarrier<kLds>();
+        if (anyPending[round & 1] == 0)
+        {
+            break;
+        }
+        // Every round colors at least the lowest pending constraint, so
+        // more than n rounds would mean a broken table: stop and report.
+        if (round > n)
+        {
+            if (t == 0)
+            {
+                __hip_atomic_store(&sc->fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            break;
+        }
+        const int ready = cur >= 0 && (A < 0 || owner[A] == cur) && (B < 0 || owner[B] == cur);
+        if (ready)
+        {
+            uint32_t mask = (A >= 0 ? (uint32_t)masks[A] : 0u) | (B >= 0 ? (uint32_t)masks[B] : 0u);
+            // Lowest color free on both bodies (the serial scan's first
+            // clear bit), M3_GRAPH_COLORS when all are taken.
+            const uint32_t freeBits = ~mask & ((1u << M3_GRAPH_COLORS) - 1u);
+            const int32_t color = freeBits != 0u ? (int32_t)__builtin_ctz(freeBits) : M3_GRAPH_COLORS;
+            if (color < M3_GRAPH_COLORS)
+            {
+                if (A >= 0)
+                {
+                    masks[A] = (uint16_t)(masks[A] | (1u << color));
+                }
+                if (B >= 0)
+                {
+                    masks[B] = (uint16_t)(masks[B] | (1u << color));
+                }
+            }
+            v.colors[cur] = (uint8_t)color;
+            atomicAdd(&counts[color], 1);
+            // Only a body's owner matches its entry, so clearing it here
+            // cannot make another participant of this round ready.
+            if (A >= 0)
+            {
+                owner[A] = 0x7FFFFFFF;
+            }
+            if (B >= 0)
+            {
+                owner[B] = 0x7FFFFFFF;
+            }
+        }
+        // Participants that colored take the next indices (one LDS atomic
+        // per wave).
+        const uint64_t took = __ballot(ready);
+        int32_t base = 0;
+        if (lane == 0 && took != 0ull)
+        {
+            base = atomicAdd(&cursor, (int32_t)__popcll(took));
+        }
+        base = __shfl(base, 0);
+        if (ready)
+        {
+            const int32_t id = base + (int32_t)__popcll(took & ((1ull << lane) - 1ull));
+            cur = id < n ? id : -1;
+            A = -1;
+            B = -1;
+            if (cur >= 0)
+            {
+                A = v.consBodies[2 * cur];
+                B = v.consBodies[2 * cur + 1];
+            }
+        }
+        // The other round's flag was last read before this round's first
+        // barrier; it is next set after the barrier below.
+        if (t == 0)
+        {
+            anyPending[(round + 1) & 1] = 0;
+        }
+        ColorBarrier<kLds>();
+    }
+    // The list build reads every participant's colors from global memory.
+    __syncthreads();
     __shared__ int32_t starts[M3_GRAPH_COLORS + 2];
     if (t == 0)
     {
@@ -1432,7 +1492,6 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
         running[t] = 0;
     }
     __syncthreads();
-    const int32_t lane = t & 63;
     const int32_t wave = t >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
     for (int32_t base = 0; base < n; base += M3_COLOR_THREADS)
@@ -2702,7 +2761,7 @@ static int CheckFault(m3gpuWorld* w)
     HIP_TRY(hipStreamSynchronize(w->stream));
     if (w->hostScalars->fault != 0)
     {
-        snprintf(s_lastError, sizeof(s_lastError), "substep grid barrier timed out");
+        snprintf(s_lastError, sizeof(s_lastError), "device step gave up: a substep grid barrier timed out or the coloring stalled");
         return -1;
     }
     return 0;

@@ -120,7 +120,7 @@ typedef struct m3Scalars
     int32_t discoverCount;
     int32_t pointCount; // contact points over all constraints
     int32_t colorStarts[M3_GRAPH_COLORS + 2];
-    int32_t fault;     // sticky: a substep grid barrier timed out
+    int32_t fault;     // sticky: a grid barrier timed out or the coloring stalled
     int32_t fullQuery; // the next step rebuilds the frozen-pair buffer
     int32_t jointActive; // joints prepared this step
     int32_t satCount;    // wide hull pairs listed for the wavefront SAT queries
