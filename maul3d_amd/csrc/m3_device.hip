@@ -983,15 +983,74 @@ __global__ __launch_bounds__(64) void k_sat(const m3Dev* __restrict__ ds)
     }
 }
 
-__global__ void k_narrow(const m3Dev* __restrict__ ds)
+// Collision routine a pair dispatches to in m3NarrowPair (0 plane-hull,
+// 1 hull-hull, 2 plane-capsule, 3 capsule-hull, 4 GJK, 5 plane-sphere,
+// 6 sphere-sphere).
+__device__ static inline int NarrowClass(const m3View& v, int32_t i)
+{
+    const uint64_t key = v.keys[i];
+    const uint8_t typeA = v.shapeType[(int32_t)(key >> 32)];
+    const uint8_t typeB = v.shapeType[(int32_t)(key & 0xFFFFFFFFu)];
+    const int planePair = typeA == M3_PLANE || typeB == M3_PLANE;
+    const int hullPair = typeA == M3_HULL || typeB == M3_HULL;
+    const int capsulePair = typeA == M3_CAPSULE || typeB == M3_CAPSULE;
+    if (planePair && hullPair)
+    {
+        return 0;
+    }
+    if (typeA == M3_HULL && typeB == M3_HULL)
+    {
+        return 1;
+    }
+    if (planePair && capsulePair)
+    {
+        return 2;
+    }
+    if (hullPair && capsulePair)
+    {
+        return 3;
+    }
+    if (hullPair || capsulePair)
+    {
+        return 4;
+    }
+    return planePair ? 5 : 6;
+}
+
+// Each workgroup takes 256 consecutive pairs and hands them to its lanes
+// grouped by collision routine, so that a wavefront runs one or two of the
+// routines instead of all of them one after another.  Every pair still
+// writes its own manifold slot.
+#define M3_NARROW_THREADS 256
+
+__global__ void __launch_bounds__(M3_NARROW_THREADS) k_narrow(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || i >= v.sc->pairCount)
+    __shared__ int32_t counts[8];
+    __shared__ int32_t order[M3_NARROW_THREADS];
+    const int32_t t = threadIdx.x;
+    if (t < 8)
     {
-        return;
+        counts[t] = 0;
     }
-    m3NarrowPair(&v, i, v.sc->oldPairCount);
+    __syncthreads();
+    const int32_t n = v.sc->stall != 0 ? 0 : v.sc->pairCount;
+    const int32_t i = blockIdx.x * M3_NARROW_THREADS + t;
+    const int cls = i < n ? NarrowClass(v, i) : 7;
+    const int32_t rank = atomicAdd(&counts[cls], 1);
+    __syncthreads();
+    int32_t base = 0;
+    for (int c = 0; c < cls; ++c)
+    {
+        base += counts[c];
+    }
+    order[base + rank] = i;
+    __syncthreads();
+    const int32_t j = order[t];
+    if (j < n)
+    {
+        m3NarrowPair(&v, j, v.sc->oldPairCount);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2987,7 +3046,8 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
         STAGE(ST_SAT_LIST, KL(k_sat_list, dim3(Blocks(np, T)), dim3(T), 0, st, D));
         STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
     }
-    STAGE(ST_NARROW, KL(k_narrow, dim3(Blocks(np, 64)), dim3(64), 0, st, D));
+    STAGE(ST_NARROW, KL(k_narrow, dim3(Blocks(np, M3_NARROW_THREADS)), dim3(M3_NARROW_THREADS), 0,
+                          st, D));
     // Sweeps, islands, wake.
     STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, D));
@@ -3488,7 +3548,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
         KL(k_sat_list, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
         KL(k_sat, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
     }
-    KL(k_narrow, dim3(Blocks(np, 64), W), dim3(64), 0, st, D);
+    KL(k_narrow, dim3(Blocks(np, M3_NARROW_THREADS), W), dim3(M3_NARROW_THREADS), 0, st, D);
     if (taped)
     {
         KL(k_tape_pairs, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
