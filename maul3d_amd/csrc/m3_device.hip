@@ -428,6 +428,12 @@ __device__ static inline bool AllowedPacked(const m3View& v, int4 a, int4 b)
 // them as they turn up would make every iteration of the wavefront's scan
 // wait on some lane's loads.
 #define M3_SAP_CAND 32
+// Lanes per shape: the lanes of a shape scan interleaved entries of its
+// sweep range (entry i+1+k, i+1+k+M3_SAP_SPLIT, ...), which shortens the
+// serial scan and puts more wavefronts on the chip.  Each overlapping pair
+// is still found by exactly one lane.
+#define M3_SAP_SPLIT 4
+#define M3_SAP_SHAPES (M3_SAP_THREADS / M3_SAP_SPLIT)
 
 __global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict__ ds)
 {
@@ -438,7 +444,7 @@ __global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict_
     const float4* xy = d.sapXY;
     const float2* zz = d.sapZ;
     const int4* attr = d.sapAttr;
-    const int32_t base = blockIdx.x * blockDim.x;
+    const int32_t base = blockIdx.x * M3_SAP_SHAPES;
     const int32_t ns = v.maxShape;
     const int32_t wend = base + M3_SAP_WINDOW < ns ? base + M3_SAP_WINDOW : ns;
     for (int32_t j = base + threadIdx.x; j < wend; j += blockDim.x)
@@ -448,7 +454,7 @@ __global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict_
     }
     __syncthreads();
     const int lane = threadIdx.x;
-    const int32_t i = base + lane;
+    const int32_t i = base + lane / M3_SAP_SPLIT;
     int4 me = make_int4(-1, 0, 0, 0);
     if (i < ns)
     {
@@ -464,7 +470,7 @@ __global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict_
         mz = szz[i - base];
         a = &v.fresh[6 * me.x];
     }
-    int32_t j = i + 1;
+    int32_t j = i + 1 + lane % M3_SAP_SPLIT;
     m3EmitBuf buf;
     buf.n = 0;
     while (true)
@@ -472,7 +478,7 @@ __global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict_
         int32_t nc = 0;
         if (!done)
         {
-            for (; j < wend && nc < M3_SAP_CAND; ++j)
+            for (; j < wend && nc < M3_SAP_CAND; j += M3_SAP_SPLIT)
             {
                 float4 oxy = sxy[j - base];
                 if (oxy.x > mxy.y) // sentinels have +inf lower x: the sweep ends
@@ -493,7 +499,7 @@ __global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict_
             }
             if (!done && j >= wend)
             {
-                for (; j < ns && nc < M3_SAP_CAND; ++j)
+                for (; j < ns && nc < M3_SAP_CAND; j += M3_SAP_SPLIT)
                 {
                     float4 oxy = xy[j];
                     if (oxy.x > mxy.y)
@@ -3022,7 +3028,7 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
                            ct, tb, hd.sapKeysIn, hd.sapKeysOut, hd.sapValsIn, hd.sapValsOut, ns, 0,
                            64, st)));
     STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
-    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, M3_SAP_THREADS)), dim3(M3_SAP_THREADS), 0, st, D));
+    STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, M3_SAP_SHAPES)), dim3(M3_SAP_THREADS), 0, st, D));
     if (v.planeCount > 0)
     {
         STAGE(ST_PLANES,
@@ -3531,7 +3537,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_pending_all, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_sap_sort_block, dim3(1, W), dim3(BT), 0, st, D);
     KL(k_sap_gather, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
-    KL(k_sap, dim3(Blocks(ns, M3_SAP_THREADS), W), dim3(M3_SAP_THREADS), 0, st, D);
+    KL(k_sap, dim3(Blocks(ns, M3_SAP_SHAPES), W), dim3(M3_SAP_THREADS), 0, st, D);
     if (s_batch.maxPlaneWork > 0)
     {
         KL(k_planes, dim3(Blocks(s_batch.maxPlaneWork, T), W), dim3(T), 0, st, D);
