@@ -13,6 +13,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "m3gpu.h"
@@ -1023,13 +1024,25 @@ __device__ static inline int NarrowClass(const m3View& v, int32_t i)
     return planePair ? 5 : 6;
 }
 
-// Each workgroup takes 256 consecutive pairs and hands them to its lanes
-// grouped by collision routine, so that a wavefront runs one or two of the
-// routines instead of all of them one after another.  Every pair still
-// writes its own manifold slot.
+// Batched worlds: one pair per lane, in pair order.
+__global__ void k_narrow(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || i >= v.sc->pairCount)
+    {
+        return;
+    }
+    m3NarrowPair(&v, i, v.sc->oldPairCount);
+}
+
+// A single world: each workgroup takes 256 consecutive pairs and hands
+// them to its lanes grouped by collision routine, so that a wavefront runs
+// one or two of the routines instead of all of them one after another.
+// Every pair still writes its own manifold slot.
 #define M3_NARROW_THREADS 256
 
-__global__ void __launch_bounds__(M3_NARROW_THREADS) k_narrow(const m3Dev* __restrict__ ds)
+__global__ void __launch_bounds__(M3_NARROW_THREADS) k_narrow_grouped(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
     __shared__ int32_t counts[8];
@@ -1781,6 +1794,9 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
 }
 
 #define M3_SUBSTEP_THREADS 256
+#ifndef M3_BLOCK_GATHER
+#define M3_BLOCK_GATHER 0
+#endif
 
 // The whole substep loop of src/solver.c:4085-4203, with a barrier after
 // each phase:
@@ -1882,9 +1898,16 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
         return -1;
     };
     const int32_t firstColor = nextColor(-1);
+    // Gathering pays where a grid barrier separates the phases; under a
+    // workgroup barrier the loads would only lengthen register lifetimes.
+    constexpr bool kGather = M3_BLOCK_GATHER || !std::is_same<Sync, m3BlockSync>::value;
     m3Prefetch pf;
     pf.k = -1;
     auto gather = [&](int mode, int32_t color) {
+        if (!kGather)
+        {
+            return;
+        }
         const int32_t k = firstSlot(color);
         if (mode == 1)
         {
@@ -3052,8 +3075,8 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
         STAGE(ST_SAT_LIST, KL(k_sat_list, dim3(Blocks(np, T)), dim3(T), 0, st, D));
         STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
     }
-    STAGE(ST_NARROW, KL(k_narrow, dim3(Blocks(np, M3_NARROW_THREADS)), dim3(M3_NARROW_THREADS), 0,
-                          st, D));
+    STAGE(ST_NARROW, KL(k_narrow_grouped, dim3(Blocks(np, M3_NARROW_THREADS)),
+                          dim3(M3_NARROW_THREADS), 0, st, D));
     // Sweeps, islands, wake.
     STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, D));
@@ -3554,7 +3577,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
         KL(k_sat_list, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
         KL(k_sat, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
     }
-    KL(k_narrow, dim3(Blocks(np, M3_NARROW_THREADS), W), dim3(M3_NARROW_THREADS), 0, st, D);
+    KL(k_narrow, dim3(Blocks(np, 64), W), dim3(64), 0, st, D);
     if (taped)
     {
         KL(k_tape_pairs, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
