@@ -336,6 +336,31 @@ def run_toss_adjoint(worlds: int = 64, steps: int = 120) -> dict:
     }
 
 
+def pmc_traffic() -> dict:
+    """HBM bytes per substep launch from the committed counter runs
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by the same
+    counter's reading for a copy of known size), with the window they cover
+    and the modelled algorithmic bytes for that window."""
+    path = os.path.join(ROOT, "profiles", "k_substeps_rain10k_pmc.md")
+    fields = {}
+    try:
+        with open(path) as f:
+            for ln in f:
+                key, sep, val = ln.partition(":")
+                if sep and key.strip() in ("traffic_bytes_per_launch", "algorithmic_bytes_per_launch", "window"):
+                    fields[key.strip()] = val.strip()
+    except OSError:
+        return {"traffic": None}
+    if "traffic_bytes_per_launch" not in fields:
+        return {"traffic": None}
+    return {
+        "traffic": int(fields["traffic_bytes_per_launch"]),
+        "traffic_window": fields.get("window"),
+        "traffic_algorithmic_bytes": int(fields.get("algorithmic_bytes_per_launch", "0")) or None,
+        "traffic_source": "profiles/k_substeps_rain10k_pmc.md",
+    }
+
+
 def copy_calibration() -> None:
     """A plain elementwise copy of known size, the scale for FETCH_SIZE and
     WRITE_SIZE counter readings of the step kernels."""
@@ -404,6 +429,7 @@ def main() -> None:
             line["roofline_error"] = repr(exc)
         if prof is not None:
             line["roofline"] = prof["roofline"]
+            line["roofline"].update(pmc_traffic())
             line["kernels"] = prof["kernels"]
     if not args.no_batch:
         line["batched"] = run_batched(args.batch_worlds)
