@@ -1824,11 +1824,85 @@ struct m3BlockSync
 // after color with workgroup barriers, and the grid meets once after them.
 #define M3_TAIL_CONSTRAINTS 1024
 
-template <typename Sync>
-__device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams& sp, int32_t gid,
-                                          int32_t gsz, Sync sync, m3Tape* tape = nullptr,
-                                          bool gridTail = false)
+// The small trailing colors of a pass in workgroup 0, without a barrier
+// per color: a constraint waits only until every earlier tail constraint
+// on each of its dynamic bodies has been solved.  Each dynamic body keeps
+// a count of its solved tail constraints in LDS; a constraint's rank on a
+// body is the number of tail colors below its own in which the body has a
+// constraint (its slot mask), so it runs when both counts equal its ranks
+// and each body still sees its constraints in color order, the operation
+// sequence of the colored pass.  The lowest pending slot is always ready,
+// so the wait ends; it is bounded all the same, and a wait that gives up
+// marks the world faulted (reported by the host) instead of hanging.
+#define M3_TAIL_LDS_BODIES 32768
+#define M3_TAIL_SPINS (1u << 20)
+
+template <int kMode>
+__device__ static inline void TailFlow(const m3View& v, const m3SubstepParams& sp, int32_t tb,
+                                       int32_t te, uint32_t tailColors, uint8_t* done)
 {
+    volatile uint8_t* vd = done;
+    const int32_t t = threadIdx.x;
+    for (int32_t i = t; i < v.maxBody; i += blockDim.x)
+    {
+        done[i] = 0;
+    }
+    __syncthreads();
+    for (int32_t s0 = tb; s0 < te; s0 += blockDim.x)
+    {
+        const int32_t slot = s0 + t;
+        int pending = slot < te;
+        m3ContactIn in;
+        int32_t rA = -1;
+        int32_t rB = -1;
+        if (pending)
+        {
+            m3LoadContact(&v.cons[slot], kMode == 1, &in);
+            const uint32_t below = tailColors & ((1u << v.slotColors[slot]) - 1u);
+            if (in.dynA)
+            {
+                rA = __popc(v.slotMask[in.a] & below);
+            }
+            if (in.dynB)
+            {
+                rB = __popc(v.slotMask[in.b] & below);
+            }
+        }
+        uint32_t spins = 0;
+        while (__any(pending))
+        {
+            if (pending && (rA < 0 || vd[in.a] == rA) && (rB < 0 || vd[in.b] == rB))
+            {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                m3SolveContactIn(&v, &v.cons[slot], &in, sp.invH, sp.push, kMode == 1);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (rA >= 0)
+                {
+                    vd[in.a] = (uint8_t)(rA + 1);
+                }
+                if (rB >= 0)
+                {
+                    vd[in.b] = (uint8_t)(rB + 1);
+                }
+                pending = 0;
+            }
+            if (++spins == M3_TAIL_SPINS)
+            {
+                if (pending)
+                {
+                    __hip_atomic_store(&v.sc->fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                break;
+            }
+        }
+    }
+}
+
+template <typename Sync, bool kGridTail = false>
+__device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams& sp, int32_t gid,
+                                          int32_t gsz, Sync sync, m3Tape* tape = nullptr)
+{
+    const bool gridTail = kGridTail;
     const int32_t nb = v.maxBody;
     int32_t starts[M3_GRAPH_COLORS + 2];
     for (int32_t k = 0; k < M3_GRAPH_COLORS + 2; ++k)
@@ -1879,7 +1953,8 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
         }
         else
         {
-            if (blockIdx.x != 0)
+            // The barrier-free tail loads its constraints itself.
+            if (blockIdx.x != 0 || (kGridTail && nb <= M3_TAIL_LDS_BODIES))
             {
                 return -1;
             }
@@ -2017,7 +2092,30 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
                 gather(mode, nextColor(color));
                 sync();
             }
-            if (tailStart <= M3_GRAPH_COLORS)
+            if (kGridTail && tailStart <= M3_GRAPH_COLORS && nb <= M3_TAIL_LDS_BODIES)
+            {
+                if (blockIdx.x == 0)
+                {
+                    const int32_t tb = starts[tailStart];
+                    const int32_t te = starts[M3_GRAPH_COLORS];
+                    const uint32_t tailColors =
+                        ((1u << M3_GRAPH_COLORS) - 1u) & ~((1u << tailStart) - 1u);
+                    __shared__ uint8_t tailDone[M3_TAIL_LDS_BODIES];
+                    if (mode == 1)
+                    {
+                        TailFlow<1>(v, sp, tb, te, tailColors, tailDone);
+                    }
+                    else
+                    {
+                        TailFlow<2>(v, sp, tb, te, tailColors, tailDone);
+                    }
+                    __syncthreads();
+                    ColorPhase(v, starts[M3_GRAPH_COLORS], starts[M3_GRAPH_COLORS + 1], mode, sp,
+                               (int32_t)threadIdx.x, (int32_t)blockDim.x, true, rec, &pf);
+                }
+                sync();
+            }
+            else if (tailStart <= M3_GRAPH_COLORS)
             {
                 if (blockIdx.x == 0)
                 {
@@ -2062,8 +2160,8 @@ __global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
     {
         return;
     }
-    SubstepLoop(v, sp, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x,
-                m3GridSync{d.barrier, &v.sc->fault}, nullptr, true);
+    SubstepLoop<m3GridSync, true>(v, sp, blockIdx.x * blockDim.x + threadIdx.x,
+                                  gridDim.x * blockDim.x, m3GridSync{d.barrier, &v.sc->fault});
 }
 
 // A batch: worlds are independent, so each workgroup runs its own world's
