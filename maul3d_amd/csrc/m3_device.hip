@@ -63,6 +63,9 @@ struct m3Dev
     int32_t* bodyOwner;
     uint32_t* barrier; // substep grid barrier: arrival count, generation
     m3Tape* tape;      // intermediate-value record of this step (adjoint replays), or null
+    // Pairs binned by collision routine: 8 counts, then one bin of
+    // pairCapacity entries per routine.
+    int32_t* narrowBins;
 };
 
 struct m3gpuWorld
@@ -231,6 +234,13 @@ __global__ void k_begin(const m3Dev* __restrict__ ds)
     sc->pointCount = 0;
     sc->jointActive = 0;
     sc->satCount = 0;
+    if (d.narrowBins != nullptr)
+    {
+        for (int c = 0; c < 8; ++c)
+        {
+            d.narrowBins[c] = 0;
+        }
+    }
 }
 
 __global__ void k_fill_emit(const m3Dev* __restrict__ ds)
@@ -1036,40 +1046,75 @@ __global__ void k_narrow(const m3Dev* __restrict__ ds)
     m3NarrowPair(&v, i, v.sc->oldPairCount);
 }
 
-// A single world: each workgroup takes 256 consecutive pairs and hands
-// them to its lanes grouped by collision routine, so that a wavefront runs
-// one or two of the routines instead of all of them one after another.
-// Every pair still writes its own manifold slot.
-#define M3_NARROW_THREADS 256
+// A single world: the pairs are first binned by collision routine (the
+// order inside a bin is immaterial: every pair writes its own manifold
+// slot), then each wavefront of k_narrow_binned takes 64 pairs of one bin,
+// so no wavefront runs more than one routine (a wavefront mixing routines
+// runs them one after another).
+#define M3_NARROW_CLASSES 7
 
-__global__ void __launch_bounds__(M3_NARROW_THREADS) k_narrow_grouped(const m3Dev* __restrict__ ds)
+__global__ void k_narrow_bin(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    __shared__ int32_t counts[8];
-    __shared__ int32_t order[M3_NARROW_THREADS];
+    __shared__ int32_t cnt[8];
+    __shared__ int32_t base[8];
     const int32_t t = threadIdx.x;
     if (t < 8)
     {
-        counts[t] = 0;
+        cnt[t] = 0;
     }
     __syncthreads();
     const int32_t n = v.sc->stall != 0 ? 0 : v.sc->pairCount;
-    const int32_t i = blockIdx.x * M3_NARROW_THREADS + t;
-    const int cls = i < n ? NarrowClass(v, i) : 7;
-    const int32_t rank = atomicAdd(&counts[cls], 1);
-    __syncthreads();
-    int32_t base = 0;
-    for (int c = 0; c < cls; ++c)
+    const int32_t i = blockIdx.x * blockDim.x + t;
+    const int cls = i < n ? NarrowClass(v, i) : M3_NARROW_CLASSES;
+    int32_t rank = 0;
+    if (cls < M3_NARROW_CLASSES)
     {
-        base += counts[c];
+        rank = atomicAdd(&cnt[cls], 1);
     }
-    order[base + rank] = i;
     __syncthreads();
-    const int32_t j = order[t];
-    if (j < n)
+    if (t < M3_NARROW_CLASSES && cnt[t] > 0)
     {
-        m3NarrowPair(&v, j, v.sc->oldPairCount);
+        base[t] = atomicAdd(&d.narrowBins[t], cnt[t]);
     }
+    __syncthreads();
+    if (cls < M3_NARROW_CLASSES)
+    {
+        d.narrowBins[8 + (int64_t)cls * v.pairCapacity + base[cls] + rank] = i;
+    }
+}
+
+// Grid: ceil(pairCapacity / 64) + M3_NARROW_CLASSES workgroups of 64, enough
+// for every bin rounded up to whole wavefronts.  Workgroup b walks the bins'
+// wavefront counts to find its bin and its 64 entries.
+__global__ void __launch_bounds__(64) k_narrow_binned(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    int32_t b = blockIdx.x;
+    int cls = -1;
+    int32_t count = 0;
+    for (int c = 0; c < M3_NARROW_CLASSES; ++c)
+    {
+        count = d.narrowBins[c];
+        const int32_t waves = (count + 63) / 64;
+        if (b < waves)
+        {
+            cls = c;
+            break;
+        }
+        b -= waves;
+    }
+    const int32_t k = b * 64 + (int32_t)threadIdx.x;
+    if (cls < 0 || k >= count)
+    {
+        return;
+    }
+    const int32_t i = d.narrowBins[8 + (int64_t)cls * v.pairCapacity + k];
+    m3NarrowPair(&v, i, v.sc->oldPairCount);
 }
 
 // ---------------------------------------------------------------------------
@@ -2602,7 +2647,7 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.slotColors, np) ||
         DevAlloc(&v.consBodies, 2 * (int64_t)np) || DevAlloc(&v.slotOf, np) ||
         DevAlloc(&v.slotTable, (int64_t)M3_GRAPH_COLORS * nb) || DevAlloc(&v.slotMask, nb) ||
-        DevAlloc(&w->dev.barrier, 4))
+        DevAlloc(&w->dev.barrier, 4) || DevAlloc(&w->dev.narrowBins, 8 + 8 * (int64_t)np))
     {
         return -1;
     }
@@ -2727,7 +2772,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.sleepKeys, v.emit, v.satQueries, v.satList, w->dev.emitSorted, w->dev.emitUnique, v.cons, v.consPair,
                     v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
                     v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut, w->dev.sapValsIn,
-                    w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.root,
+                    w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.root, w->dev.narrowBins,
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
                     v.slotTable, v.slotMask, w->dev.barrier, w->cubTemp, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
@@ -3173,8 +3218,8 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
         STAGE(ST_SAT_LIST, KL(k_sat_list, dim3(Blocks(np, T)), dim3(T), 0, st, D));
         STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
     }
-    STAGE(ST_NARROW, KL(k_narrow_grouped, dim3(Blocks(np, M3_NARROW_THREADS)),
-                          dim3(M3_NARROW_THREADS), 0, st, D));
+    STAGE(ST_NARROW, KL(k_narrow_bin, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    STAGE(ST_NARROW, KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES), dim3(64), 0, st, D));
     // Sweeps, islands, wake.
     STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, D));
