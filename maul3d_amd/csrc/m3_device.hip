@@ -3621,36 +3621,22 @@ __global__ void k_tape_pairs(const m3Dev* __restrict__ ds)
     {
         return;
     }
-    int32_t lo = 0;
-    int32_t hi = oldCount - 1;
-    while (lo <= hi)
+    const int32_t mid = m3FindKey(v.oldKeys, oldCount, key);
+    if (mid < 0)
     {
-        int32_t mid = (lo + hi) / 2;
-        uint64_t k2 = v.oldKeys[mid];
-        if (k2 == key)
+        return;
+    }
+    t->carryPair[i] = mid;
+    const m3Manifold* prev = &v.oldManifolds[mid];
+    for (int32_t k = 0; k < m->pointCount; ++k)
+    {
+        for (int32_t o = 0; o < prev->pointCount; ++o)
         {
-            t->carryPair[i] = mid;
-            const m3Manifold* prev = &v.oldManifolds[mid];
-            for (int32_t k = 0; k < m->pointCount; ++k)
+            if (prev->points[o].id == m->points[k].id)
             {
-                for (int32_t o = 0; o < prev->pointCount; ++o)
-                {
-                    if (prev->points[o].id == m->points[k].id)
-                    {
-                        t->carryPoint[4 * i + k] = (int8_t)o;
-                        break;
-                    }
-                }
+                t->carryPoint[4 * i + k] = (int8_t)o;
+                break;
             }
-            return;
-        }
-        if (k2 < key)
-        {
-            lo = mid + 1;
-        }
-        else
-        {
-            hi = mid - 1;
         }
     }
 }

@@ -895,6 +895,77 @@ M3_HD m3Manifold m3CollideSpherePair(const m3View* v, int32_t shapeA, int32_t sh
 
 // One pair: fresh manifold by shape family, then the warm-start carry by
 // feature id from the previous step's sorted list.
+// Index of key in the ascending, duplicate-free keys[0, n), or -1.  A few
+// interpolation probes come first (pair keys spread evenly enough that they
+// land on or beside the key), then bisection of the range that remains.
+// Every probe keeps keys[lo] < key < keys[hi], so the answer is the same as
+// plain bisection's; only the number of dependent loads differs.
+M3_HD int32_t m3FindKey(const uint64_t* keys, int32_t n, uint64_t key)
+{
+    if (n <= 0)
+    {
+        return -1;
+    }
+    int32_t lo = 0;
+    int32_t hi = n - 1;
+    uint64_t klo = keys[lo];
+    uint64_t khi = keys[hi];
+    if (key == klo)
+    {
+        return lo;
+    }
+    if (key == khi)
+    {
+        return hi;
+    }
+    if (key < klo || key > khi)
+    {
+        return -1;
+    }
+    for (int probe = 0; probe < 3 && hi - lo > 1; ++probe)
+    {
+        const double f = (double)(key - klo) / (double)(khi - klo);
+        int32_t mid = lo + (int32_t)(f * (double)(hi - lo));
+        mid = mid <= lo ? lo + 1 : (mid >= hi ? hi - 1 : mid);
+        const uint64_t k = keys[mid];
+        if (k == key)
+        {
+            return mid;
+        }
+        if (k < key)
+        {
+            lo = mid;
+            klo = k;
+        }
+        else
+        {
+            hi = mid;
+            khi = k;
+        }
+    }
+    // keys[lo] < key < keys[hi]: bisect the open interval.
+    lo += 1;
+    hi -= 1;
+    while (lo <= hi)
+    {
+        const int32_t mid = (lo + hi) / 2;
+        const uint64_t k = keys[mid];
+        if (k == key)
+        {
+            return mid;
+        }
+        if (k < key)
+        {
+            lo = mid + 1;
+        }
+        else
+        {
+            hi = mid - 1;
+        }
+    }
+    return -1;
+}
+
 M3_HD void m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
 {
     uint64_t key = v->keys[i];
@@ -945,42 +1016,24 @@ M3_HD void m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
         fresh = m3CollideSpherePair(v, shapeA, shapeB);
     }
 
-    if (fresh.pointCount > 0 && oldCount > 0)
+    const int32_t mid = fresh.pointCount > 0 ? m3FindKey(v->oldKeys, oldCount, key) : -1;
+    if (mid >= 0)
     {
-        int32_t lo = 0;
-        int32_t hi = oldCount - 1;
-        while (lo <= hi)
+        const m3Manifold* prev = &v->oldManifolds[mid];
+        fresh.frictionImpulse = prev->frictionImpulse;
+        fresh.twistImpulse = prev->twistImpulse;
+        fresh.rollingImpulse = prev->rollingImpulse;
+        int32_t prevCount = prev->pointCount;
+        for (int32_t k2 = 0; k2 < fresh.pointCount; ++k2)
         {
-            int32_t mid = (lo + hi) / 2;
-            uint64_t k = v->oldKeys[mid];
-            if (k == key)
+            for (int32_t o = 0; o < prevCount; ++o)
             {
-                const m3Manifold* prev = &v->oldManifolds[mid];
-                fresh.frictionImpulse = prev->frictionImpulse;
-                fresh.twistImpulse = prev->twistImpulse;
-                fresh.rollingImpulse = prev->rollingImpulse;
-                int32_t prevCount = prev->pointCount;
-                for (int32_t k2 = 0; k2 < fresh.pointCount; ++k2)
+                if (prev->points[o].id == fresh.points[k2].id)
                 {
-                    for (int32_t o = 0; o < prevCount; ++o)
-                    {
-                        if (prev->points[o].id == fresh.points[k2].id)
-                        {
-                            fresh.points[k2].normalImpulse = prev->points[o].normalImpulse;
-                            fresh.points[k2].flags |= 1;
-                            break;
-                        }
-                    }
+                    fresh.points[k2].normalImpulse = prev->points[o].normalImpulse;
+                    fresh.points[k2].flags |= 1;
+                    break;
                 }
-                break;
-            }
-            if (k < key)
-            {
-                lo = mid + 1;
-            }
-            else
-            {
-                hi = mid - 1;
             }
         }
     }
