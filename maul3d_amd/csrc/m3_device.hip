@@ -1000,6 +1000,77 @@ __global__ __launch_bounds__(64) void k_sat(const m3Dev* __restrict__ ds)
     }
 }
 
+__device__ static int32_t Find(const int32_t* parent, int32_t i)
+{
+    while (true)
+    {
+        int32_t p = LoadRelaxed(&parent[i]);
+        if (p == i)
+        {
+            return i;
+        }
+        i = p;
+    }
+}
+
+// Lock-free union that always hooks the larger root under the smaller, so
+// each component's root is its lowest body index: the reference's
+// "lower root wins" labels.
+__device__ static void Unite(int32_t* parent, int32_t a, int32_t b)
+{
+    while (true)
+    {
+        a = Find(parent, a);
+        b = Find(parent, b);
+        if (a == b)
+        {
+            return;
+        }
+        int32_t lo = a < b ? a : b;
+        int32_t hi = a < b ? b : a;
+        if (atomicCAS(&parent[hi], hi, lo) == hi)
+        {
+            return;
+        }
+    }
+}
+
+// Island coupling of touching pair i (src/solver.c:3468-3580): two dynamic
+// bodies join; a dynamic body touching a moving kinematic one is forced
+// awake.
+__device__ static inline void UnionTouching(const m3View& v, int32_t i)
+{
+    uint64_t key = v.keys[i];
+    int32_t sA = (int32_t)(key >> 32);
+    int32_t sB = (int32_t)(key & 0xFFFFFFFFu);
+    if (v.sensor[sA] != 0 || v.sensor[sB] != 0)
+    {
+        return;
+    }
+    int32_t bA = v.shapeBody[sA];
+    int32_t bB = v.shapeBody[sB];
+    int dynA = v.type[bA] == M3_DYNAMIC;
+    int dynB = v.type[bB] == M3_DYNAMIC;
+    if (dynA && dynB)
+    {
+        Unite(v.parent, bA, bB);
+    }
+    else if (dynA || dynB)
+    {
+        int32_t kin = dynA ? bB : bA;
+        int32_t dyn = dynA ? bA : bB;
+        if (v.type[kin] == M3_KINEMATIC)
+        {
+            m3Vec3 lin = v.lv[kin];
+            m3Vec3 ang = v.av[kin];
+            if (m3Dot3(lin, lin) > 0.0f || m3Dot3(ang, ang) > 0.0f)
+            {
+                v.forced[dyn] = 1;
+            }
+        }
+    }
+}
+
 // Collision routine a pair dispatches to in m3NarrowPair (0 plane-hull,
 // 1 hull-hull, 2 plane-capsule, 3 capsule-hull, 4 GJK, 5 plane-sphere,
 // 6 sphere-sphere).
@@ -1114,7 +1185,12 @@ __global__ void __launch_bounds__(64) k_narrow_binned(const m3Dev* __restrict__ 
         return;
     }
     const int32_t i = d.narrowBins[8 + (int64_t)cls * v.pairCapacity + k];
-    m3NarrowPair(&v, i, v.sc->oldPairCount);
+    // The island union of a touching pair follows its manifold here (the
+    // bodies' union-find entries were reset by k_capture before this launch).
+    if (m3NarrowPair(&v, i, v.sc->oldPairCount) > 0)
+    {
+        UnionTouching(v, i);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1148,41 +1224,6 @@ __global__ void k_capture(const m3Dev* __restrict__ ds)
     v.deltaRot[i] = m3IdentityQuat();
 }
 
-__device__ static int32_t Find(const int32_t* parent, int32_t i)
-{
-    while (true)
-    {
-        int32_t p = LoadRelaxed(&parent[i]);
-        if (p == i)
-        {
-            return i;
-        }
-        i = p;
-    }
-}
-
-// Lock-free union that always hooks the larger root under the smaller, so
-// each component's root is its lowest body index: the reference's
-// "lower root wins" labels.
-__device__ static void Unite(int32_t* parent, int32_t a, int32_t b)
-{
-    while (true)
-    {
-        a = Find(parent, a);
-        b = Find(parent, b);
-        if (a == b)
-        {
-            return;
-        }
-        int32_t lo = a < b ? a : b;
-        int32_t hi = a < b ? b : a;
-        if (atomicCAS(&parent[hi], hi, lo) == hi)
-        {
-            return;
-        }
-    }
-}
-
 __global__ void k_union(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
@@ -1191,35 +1232,7 @@ __global__ void k_union(const m3Dev* __restrict__ ds)
     {
         return;
     }
-    uint64_t key = v.keys[i];
-    int32_t sA = (int32_t)(key >> 32);
-    int32_t sB = (int32_t)(key & 0xFFFFFFFFu);
-    if (v.sensor[sA] != 0 || v.sensor[sB] != 0)
-    {
-        return;
-    }
-    int32_t bA = v.shapeBody[sA];
-    int32_t bB = v.shapeBody[sB];
-    int dynA = v.type[bA] == M3_DYNAMIC;
-    int dynB = v.type[bB] == M3_DYNAMIC;
-    if (dynA && dynB)
-    {
-        Unite(v.parent, bA, bB);
-    }
-    else if (dynA || dynB)
-    {
-        int32_t kin = dynA ? bB : bA;
-        int32_t dyn = dynA ? bA : bB;
-        if (v.type[kin] == M3_KINEMATIC)
-        {
-            m3Vec3 lin = v.lv[kin];
-            m3Vec3 ang = v.av[kin];
-            if (m3Dot3(lin, lin) > 0.0f || m3Dot3(ang, ang) > 0.0f)
-            {
-                v.forced[dyn] = 1;
-            }
-        }
-    }
+    UnionTouching(v, i);
 }
 
 // Joints couple islands like touching contacts; a moving kinematic
@@ -3218,11 +3231,12 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
         STAGE(ST_SAT_LIST, KL(k_sat_list, dim3(Blocks(np, T)), dim3(T), 0, st, D));
         STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
     }
+    // Step-start body state and union-find reset, then the narrowphase,
+    // which also unites the bodies of each touching pair.
+    STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_NARROW, KL(k_narrow_bin, dim3(Blocks(np, T)), dim3(T), 0, st, D));
     STAGE(ST_NARROW, KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES), dim3(64), 0, st, D));
-    // Sweeps, islands, wake.
-    STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
-    STAGE(ST_UNION, KL(k_union, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    // Islands and wake.
     if (v.maxJoint > 0)
     {
         STAGE(ST_UNION, KL(k_union_joints, dim3(Blocks(v.maxJoint, T)), dim3(T), 0, st, D));

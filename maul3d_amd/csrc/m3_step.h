@@ -966,7 +966,8 @@ M3_HD int32_t m3FindKey(const uint64_t* keys, int32_t n, uint64_t key)
     return -1;
 }
 
-M3_HD void m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
+// Returns the new manifold's point count.
+M3_HD int32_t m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
 {
     uint64_t key = v->keys[i];
     int32_t shapeA = (int32_t)(key >> 32);
@@ -1038,6 +1039,7 @@ M3_HD void m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
         }
     }
     v->manifolds[i] = fresh;
+    return fresh.pointCount;
 }
 
 // ---------------------------------------------------------------------------
