@@ -483,6 +483,37 @@ def hull_pile(api: Api, hulls: int = 200, seed: int = 0x4C4) -> WorldId:
     return world
 
 
+def crowded_plank(api: Api, n: int = 7, seed: int = 0xC0FFEE) -> WorldId:
+    """A dynamic 8 x 0.5 x 8 plank on a plane under two layers of n x n unit
+    spheres dropped from jittered heights: from about step 30 the plank
+    carries more constraints than there are graph colors, so the coloring's
+    overflow color (solved serially, after every colored constraint) is in
+    use for the rest of the first 150 steps.
+    """
+    world = _world(api, 256, 256)
+    _ground(api, world)
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    bd.position = Pos3(0.0, 0.25, 0.0)
+    plank = api.m3CreateBody(world, C.byref(bd))
+    _check(api.m3CreateBoxShape(plank, C.byref(sd), Vec3(4.0, 0.25, 4.0)), "plank")
+    rng = SplitMix64(seed)
+    for i in range(n):
+        for j in range(n):
+            for layer in range(2):
+                x = -3.0 + i * 1.0 + rng.range(-0.2, 0.2)
+                y = 1.0 + layer * 1.1 + rng.range(0.0, 2.0)
+                z = -3.0 + j * 1.0 + rng.range(-0.2, 0.2)
+                bd.position = Pos3(x, y, z)
+                body = api.m3CreateBody(world, C.byref(bd))
+                ball = Sphere(Vec3(0.0, 0.0, 0.0), 0.5)
+                _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+    return world
+
+
 SCENES = {
     "bench_pyramid": bench_pyramid,
     "smoke10k": smoke10k,
@@ -491,4 +522,5 @@ SCENES = {
     "chains": chains,
     "hull_chains": hull_chains,
     "hull_pile": hull_pile,
+    "crowded_plank": crowded_plank,
 }

@@ -36,6 +36,7 @@ CASES = [
     ("box_pyramid", scenes.box_pyramid, 150),
     ("batch7", lambda a: scenes.batch_world(a, 7), 200),
     ("hull_pile", lambda a: scenes.hull_pile(a, hulls=60), 200),
+    ("crowded_plank", scenes.crowded_plank, 150),
 ]
 
 
