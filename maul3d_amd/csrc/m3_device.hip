@@ -1105,23 +1105,11 @@ __device__ static inline int NarrowClass(const m3View& v, int32_t i)
     return planePair ? 5 : 6;
 }
 
-// Batched worlds: one pair per lane, in pair order.
-__global__ void k_narrow(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || i >= v.sc->pairCount)
-    {
-        return;
-    }
-    m3NarrowPair(&v, i, v.sc->oldPairCount);
-}
-
-// A single world: the pairs are first binned by collision routine (the
-// order inside a bin is immaterial: every pair writes its own manifold
-// slot), then each wavefront of k_narrow_binned takes 64 pairs of one bin,
-// so no wavefront runs more than one routine (a wavefront mixing routines
-// runs them one after another).
+// The pairs are first binned by collision routine (the order inside a bin
+// is immaterial: every pair writes its own manifold slot), then each
+// wavefront of k_narrow_binned takes 64 pairs of one bin, so no wavefront
+// runs more than one routine (a wavefront mixing routines runs them one
+// after another).  Batched worlds bin per world (grid.y).
 #define M3_NARROW_CLASSES 7
 
 __global__ void k_narrow_bin(const m3Dev* __restrict__ ds)
@@ -1222,17 +1210,6 @@ __global__ void k_capture(const m3Dev* __restrict__ ds)
     v.pending[i] = 0; // the sweep has recorded last step's freezes
     v.deltaPos[i] = m3V3(0.0f, 0.0f, 0.0f);
     v.deltaRot[i] = m3IdentityQuat();
-}
-
-__global__ void k_union(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || i >= v.sc->pairCount || v.manifolds[i].pointCount == 0)
-    {
-        return;
-    }
-    UnionTouching(v, i);
 }
 
 // Joints couple islands like touching contacts; a moving kinematic
@@ -3720,13 +3697,15 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
         KL(k_sat_list, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
         KL(k_sat, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
     }
-    KL(k_narrow, dim3(Blocks(np, 64), W), dim3(64), 0, st, D);
+    // Step-start reset, then the narrowphase binned by collision routine
+    // per world, which also unites the bodies of each touching pair.
+    KL(k_capture, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
+    KL(k_narrow_bin, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES, W), dim3(64), 0, st, D);
     if (taped)
     {
         KL(k_tape_pairs, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     }
-    KL(k_capture, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
-    KL(k_union, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     if (s_batch.maxJoint > 0)
     {
         KL(k_union_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);
