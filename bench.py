@@ -147,7 +147,9 @@ def run_batched(worlds: int) -> dict:
     through m3World_StepMany (every launch shared by all worlds) and once
     world by world through m3World_Step, waiting for each world before the
     next (the sequential single-world path).  Restores happen before each
-    timed region; hashes are checked against the reference fixture."""
+    timed region; hashes are checked against the reference fixture.  Before
+    timing, m3World_Autotune picks the batched substep kernel's workgroup
+    size on the restored state (8-step trials per candidate)."""
     import maul3d_amd
     from maul3d_amd import api, scenes
 
@@ -176,6 +178,23 @@ def run_batched(worlds: int) -> dict:
             if not gpu.restore(w, snap):
                 raise RuntimeError("restore refused")
             lib.m3World_Sync(w)
+
+    # Launch autotuning on the restored state (every candidate starts from
+    # the snapshot; the worlds come back restored), outside the timed region.
+    tune = api.AutotuneResult()
+    t_tune = time.perf_counter()
+    ntune = gpu.m3World_Autotune(ids, worlds, DT, SUBSTEPS, resim, C.byref(tune))
+    t_tune = time.perf_counter() - t_tune
+    autotune = None
+    if ntune > 0:
+        autotune = {
+            "threads_per_world_workgroup": tune.batchThreads,
+            "candidates_ms_per_step": {
+                str(tune.candidateA[k]): round(tune.candidateMs[k], 3) for k in range(ntune)
+            },
+            "trial_steps": resim,
+            "seconds": round(t_tune, 2),
+        }
 
     restore_all()
     t0 = time.perf_counter()
@@ -207,6 +226,7 @@ def run_batched(worlds: int) -> dict:
         "sequential_ms_per_step": 1000.0 * t_seq / resim,
         "speedup_vs_sequential": t_seq / t_batch,
         "hash_bit_exact": (ok_batch and ok_seq) if ok_batch is not None else None,
+        "autotune": autotune,
     }
 
 

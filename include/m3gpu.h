@@ -188,6 +188,25 @@ M3GPU_API int m3gpu_step(m3gpuWorld* world, const m3gpuStepParams* params,
 M3GPU_API int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,
                               int32_t count);
 
+// Launch tuning (no counterpart in the reference, whose solver has no
+// launch shapes; used by m3World_Autotune).  None of these settings changes
+// what a step computes.
+//
+// Workgroups of one world's cooperative substep grid (0: one per 256
+// bodies, at most 40) and the constraint budget of the trailing colors its
+// workgroup 0 solves without grid barriers (0: 1024).  Fails when blocks
+// exceeds the co-resident limit of the cooperative launch.
+M3GPU_API int m3gpu_set_substep_tuning(m3gpuWorld* world, int32_t blocks, int32_t tailLimit);
+// The values the world's next step launches with.
+M3GPU_API int m3gpu_get_substep_tuning(const m3gpuWorld* world, int32_t* blocks,
+                                       int32_t* tailLimit);
+// Threads per world workgroup of the batched substep kernel, process-wide
+// (0: MAUL3D_AMD_BATCH_THREADS, else 256; otherwise a multiple of 64 in
+// [64, 512]).  Waits for queued batch work and drops the captured batch
+// launch graphs, which hold the old size.
+M3GPU_API int m3gpu_set_batch_threads(int32_t threads);
+M3GPU_API int32_t m3gpu_get_batch_threads(void);
+
 // Waits for all queued device work of this world.
 M3GPU_API int m3gpu_sync(m3gpuWorld* world);
 

@@ -299,6 +299,33 @@ M3_API m3AdjointDef m3DefaultAdjointDef(void);
 M3_API int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointDef* defs,
                                m3AdjointResult* results);
 
+// Launch autotuning (no reference counterpart; the reference solver has no
+// launch shapes).  Times candidate launch shapes on the worlds' current
+// state for `trialSteps` steps each and keeps the fastest: for several
+// worlds (a batch stepped with m3World_StepMany) the threads per world
+// workgroup of the batched substep kernel (64, 128, 256, 512; process-wide);
+// for one world its cooperative substep workgroups (default, 24, 32, 48)
+// times the constraint budget of the trailing colors solved in workgroup 0
+// (512, 1024, 2048).  Every world is snapshotted first and restored before
+// each candidate and at the end, so the call leaves the worlds as they were;
+// the shapes change a step's speed, never its result.  Returns the number of
+// candidates timed, or -1 on bad arguments or a failed snapshot/restore.
+#define M3_AUTOTUNE_MAX_CANDIDATES 16
+typedef struct m3AutotuneResult
+{
+    int32_t batchThreads;  // chosen (batch), else 0
+    int32_t substepBlocks; // chosen (one world): workgroups its steps launch, else 0
+    int32_t tailLimit;     // chosen (one world), else 0
+    int32_t candidates;
+    int32_t candidateA[M3_AUTOTUNE_MAX_CANDIDATES]; // threads (batch) or workgroups (0: default)
+    int32_t candidateB[M3_AUTOTUNE_MAX_CANDIDATES]; // 0 (batch) or tail budget
+    float candidateMs[M3_AUTOTUNE_MAX_CANDIDATES];  // mean ms per step over the trial
+} m3AutotuneResult;
+
+M3_API int32_t m3World_Autotune(const m3WorldId* worlds, int32_t count, float timeStep,
+                                int32_t subStepCount, int32_t trialSteps,
+                                m3AutotuneResult* result);
+
 // Stage profile of the steps between Begin and End (struct in m3gpu.h).
 struct m3gpuProfile;
 M3_API int32_t m3World_ProfileBegin(m3WorldId worldId);

@@ -288,6 +288,27 @@ class AdjointResult(C.Structure):
     ]
 
 
+AUTOTUNE_MAX_CANDIDATES = 16
+
+
+class AutotuneResult(C.Structure):
+    """m3AutotuneResult (this build's launch autotuner)."""
+
+    _fields_ = [
+        ("batchThreads", c_i32),
+        ("substepBlocks", c_i32),
+        ("tailLimit", c_i32),
+        ("candidates", c_i32),
+        ("candidateA", c_i32 * AUTOTUNE_MAX_CANDIDATES),
+        ("candidateB", c_i32 * AUTOTUNE_MAX_CANDIDATES),
+        ("candidateMs", C.c_float * AUTOTUNE_MAX_CANDIDATES),
+    ]
+
+
+_SIGNATURES["m3World_Autotune"] = (
+    c_i32,
+    [C.POINTER(WorldId), c_i32, C.c_float, c_i32, c_i32, C.POINTER(AutotuneResult)],
+)
 _SIGNATURES["m3DefaultAdjointDef"] = (AdjointDef, [])
 _SIGNATURES["m3World_Adjoint"] = (
     c_i32,
@@ -308,6 +329,7 @@ class Api:
             "m3World_GetProfile",
             "m3DefaultAdjointDef",
             "m3World_Adjoint",
+            "m3World_Autotune",
         ),
     ):
         self.lib = lib

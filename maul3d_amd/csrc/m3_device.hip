@@ -90,7 +90,11 @@ struct m3gpuWorld
     m3Scalars* hostScalars; // pinned mirror for stats
     struct m3Profiler* prof;
     int coopBlocks;  // co-resident blocks of the substep kernel (maximum)
-    int coopFixed;   // coopBlocks set from the environment: launch exactly that many
+    int coopFixed;   // coopBlocks set from the environment or tuning: launch exactly that many
+    int coopDefault; // coopBlocks as created (restored when tuning is cleared)
+    int coopMax;     // co-resident limit of the cooperative launch
+    int coopEnvFixed; // MAUL3D_AMD_SUBSTEP_BLOCKS was set at creation
+    int tailLimit;   // tuned budget of the workgroup-0 tail (0: M3_TAIL_CONSTRAINTS)
     int coopThreads; // threads per block of the substep kernel
     int wideHulls;   // some hull is larger than a box (wide hull pairs possible)
     int boxHulls;    // some hull is at least box-sized
@@ -1680,6 +1684,7 @@ __global__ void k_mover_flags(const m3Dev* __restrict__ ds, float dt)
 struct m3SubstepParams
 {
     int32_t substeps;
+    int32_t tailLimit; // constraint budget of the trailing colors solved in workgroup 0
     float h;
     float invH;
     float push;
@@ -1950,13 +1955,13 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
     if (gridTail)
     {
         int32_t rem = starts[M3_GRAPH_COLORS + 1] - starts[M3_GRAPH_COLORS];
-        if (rem <= M3_TAIL_CONSTRAINTS)
+        if (rem <= sp.tailLimit)
         {
             tailStart = M3_GRAPH_COLORS;
             for (int32_t c = M3_GRAPH_COLORS - 1; c >= 0; --c)
             {
                 int32_t size = starts[c + 1] - starts[c];
-                if (rem + size > M3_TAIL_CONSTRAINTS)
+                if (rem + size > sp.tailLimit)
                 {
                     break;
                 }
@@ -2704,6 +2709,9 @@ static int CreateImpl(m3gpuWorld* w)
         int want = env != nullptr ? atoi(env) : 40;
         w->coopBlocks = want < 1 ? 1 : (want > maxBlocks ? maxBlocks : want);
         w->coopFixed = env != nullptr;
+        w->coopEnvFixed = w->coopFixed;
+        w->coopDefault = w->coopBlocks;
+        w->coopMax = maxBlocks;
     }
     HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
     HIP_TRY(hipMalloc(&w->devDev, 2 * sizeof(m3Dev)));
@@ -3133,6 +3141,7 @@ static m3SubstepParams SubstepParams(const m3gpuStepParams* p)
     float h = p->dt / (float)p->substeps;
     m3SubstepParams sp;
     sp.substeps = p->substeps;
+    sp.tailLimit = M3_TAIL_CONSTRAINTS;
     sp.h = h;
     sp.invH = h > 0.0f ? 1.0f / h : 0.0f;
     sp.push = p->contactPushMaxSpeed;
@@ -3251,6 +3260,10 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     // Substeps: one cooperative launch.
     {
         m3SubstepParams spArg = sp;
+        if (w->tailLimit > 0)
+        {
+            spArg.tailLimit = w->tailLimit;
+        }
         void* args[] = {(void*)&D, (void*)&spArg};
         int blocks = w->coopBlocks;
         if (!w->coopFixed)
@@ -3383,6 +3396,17 @@ struct m3gpuBatch
 
 static m3gpuBatch s_batch;
 static pthread_mutex_t s_batchLock = PTHREAD_MUTEX_INITIALIZER;
+static int s_batchThreadsWanted = 0; // set by m3gpu_set_batch_threads (0: default)
+
+// Threads per world workgroup of the batched substep kernel: the tuned
+// value, else MAUL3D_AMD_BATCH_THREADS, else 256.
+static int BatchThreadsDefault(void)
+{
+    const char* env = getenv("MAUL3D_AMD_BATCH_THREADS");
+    int t = s_batchThreadsWanted > 0 ? s_batchThreadsWanted : (env != nullptr ? atoi(env) : 256);
+    t = t < 64 ? 64 : (t > M3_BLOCK_MAX_THREADS ? M3_BLOCK_MAX_THREADS : t);
+    return t & ~63;
+}
 
 // A world whose latest work ran in the batch stream waits for it before
 // anything else touches the world.
@@ -3463,10 +3487,7 @@ static int BatchReserve(int32_t n)
         HIP_TRY(hipEventCreateWithFlags(&s_batch.done, hipEventDisableTiming));
         const char* genv = getenv("MAUL3D_AMD_BATCH_GRAPHS");
         s_batch.useGraphs = genv == nullptr || atoi(genv) != 0;
-        const char* env = getenv("MAUL3D_AMD_BATCH_THREADS");
-        int t = env != nullptr ? atoi(env) : 256;
-        t = t < 64 ? 64 : (t > M3_BLOCK_MAX_THREADS ? M3_BLOCK_MAX_THREADS : t);
-        s_batch.threads = t & ~63;
+        s_batch.threads = BatchThreadsDefault();
     }
     int32_t cap = n < 64 ? 64 : n;
     free(s_batch.worlds);
@@ -3803,6 +3824,89 @@ static int BatchEnqueue(const m3gpuStepParams* p)
     HIP_TRY(hipEventRecord(s_batch.done, st));
     s_batch.steps += 1;
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Launch tuning.  None of these change what a step computes (every
+// participant count and workgroup size gives the same bits); they only
+// choose launch shapes, so an autotuner can time candidates on a world's
+// real state and keep the fastest.
+// ---------------------------------------------------------------------------
+
+extern "C" int m3gpu_set_substep_tuning(m3gpuWorld* w, int32_t blocks, int32_t tailLimit)
+{
+    if (w == nullptr || blocks < 0 || tailLimit < 0)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_set_substep_tuning: bad arguments");
+        return -1;
+    }
+    if (blocks > w->coopMax)
+    {
+        snprintf(s_lastError, sizeof(s_lastError),
+                 "m3gpu_set_substep_tuning: %d workgroups exceed the %d that can be co-resident",
+                 blocks, w->coopMax);
+        return -1;
+    }
+    w->coopBlocks = blocks > 0 ? blocks : w->coopDefault;
+    w->coopFixed = blocks > 0 || w->coopEnvFixed;
+    w->tailLimit = tailLimit;
+    return 0;
+}
+
+extern "C" int m3gpu_get_substep_tuning(const m3gpuWorld* w, int32_t* blocks, int32_t* tailLimit)
+{
+    if (w == nullptr)
+    {
+        return -1;
+    }
+    if (blocks != nullptr)
+    {
+        int b = w->coopBlocks;
+        if (!w->coopFixed)
+        {
+            const int perBody = (w->view.maxBody + w->coopThreads - 1) / w->coopThreads;
+            b = perBody < 1 ? 1 : (perBody < b ? perBody : b);
+        }
+        *blocks = b;
+    }
+    if (tailLimit != nullptr)
+    {
+        *tailLimit = w->tailLimit > 0 ? w->tailLimit : M3_TAIL_CONSTRAINTS;
+    }
+    return 0;
+}
+
+extern "C" int m3gpu_set_batch_threads(int32_t threads)
+{
+    if (threads != 0 && (threads < 64 || threads > M3_BLOCK_MAX_THREADS || threads % 64 != 0))
+    {
+        snprintf(s_lastError, sizeof(s_lastError),
+                 "m3gpu_set_batch_threads: threads must be 0 or a multiple of 64 in [64, %d]",
+                 M3_BLOCK_MAX_THREADS);
+        return -1;
+    }
+    pthread_mutex_lock(&s_batchLock);
+    s_batchThreadsWanted = threads;
+    if (s_batch.capacity > 0)
+    {
+        if (s_batch.count > 0)
+        {
+            (void)hipStreamSynchronize(s_batch.stream);
+        }
+        // The captured launch sequences hold the old workgroup size.
+        BatchDropGraphs();
+        s_batch.threads = BatchThreadsDefault();
+    }
+    pthread_mutex_unlock(&s_batchLock);
+    return 0;
+}
+
+extern "C" int32_t m3gpu_get_batch_threads(void)
+{
+    pthread_mutex_lock(&s_batchLock);
+    const int32_t t = s_batch.capacity > 0 ? s_batch.threads : BatchThreadsDefault();
+    pthread_mutex_unlock(&s_batchLock);
+    return t;
 }
 
 extern "C" int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,

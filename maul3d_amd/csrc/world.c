@@ -10,6 +10,9 @@
 // writes (transforms, velocities, sleep state, pairs, manifolds); the host
 // copy is refreshed lazily on the first read after a step.
 
+// clock_gettime (the autotuner's trial timing) under -std=c17.
+#define _POSIX_C_SOURCE 199309L
+
 #include "m3_core.h"
 
 #include "maul3d_amd.h"
@@ -19,6 +22,7 @@
 #include <stddef.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <time.h>
 
 // Larger than the reference's 64-slot table (src/world_internal.h:37) so one
 // process can hold a 256-world rollback batch.
@@ -2023,6 +2027,161 @@ void m3World_Sync(m3WorldId worldId)
     {
         DeviceFail("m3gpu_sync");
     }
+}
+
+// Launch autotuning: times each candidate launch shape on the worlds' own
+// current state (snapshotted first and restored before every candidate and
+// at the end), keeps the fastest, and leaves the worlds as they were.  The
+// shapes never change a step's bits, only its speed.
+static double MonotonicSeconds(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1.0e-9 * (double)ts.tv_nsec;
+}
+
+static bool RestoreAll(const m3WorldId* worlds, int32_t count, void* const* snaps,
+                       const int32_t* sizes)
+{
+    for (int32_t k = 0; k < count; ++k)
+    {
+        if (!m3World_Restore(worlds[k], snaps[k], sizes[k]))
+        {
+            return false;
+        }
+    }
+    return true;
+}
+
+static void SyncAll(const m3WorldId* worlds, int32_t count)
+{
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3World_Sync(worlds[k]);
+    }
+}
+
+int32_t m3World_Autotune(const m3WorldId* worlds, int32_t count, float timeStep,
+                         int32_t subStepCount, int32_t trialSteps, m3AutotuneResult* result)
+{
+    if (worlds == NULL || result == NULL || count <= 0 || count > 1024 || !(timeStep > 0.0f) ||
+        subStepCount < 1 || subStepCount > M3_MAX_SUBSTEPS || trialSteps < 1 || trialSteps > 64)
+    {
+        return -1;
+    }
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3World* w = WorldFromId(worlds[k]);
+        if (w == NULL)
+        {
+            return -1;
+        }
+        EnsureDevice(w);
+    }
+    memset(result, 0, sizeof(*result));
+    void** snaps = (void**)calloc((size_t)count, sizeof(void*));
+    int32_t* sizes = (int32_t*)calloc((size_t)count, sizeof(int32_t));
+    int32_t rc = -1;
+    if (snaps == NULL || sizes == NULL)
+    {
+        goto done;
+    }
+    for (int32_t k = 0; k < count; ++k)
+    {
+        sizes[k] = m3World_SnapshotSize(worlds[k]);
+        snaps[k] = sizes[k] > 0 ? malloc((size_t)sizes[k]) : NULL;
+        if (snaps[k] == NULL || m3World_Snapshot(worlds[k], snaps[k], sizes[k]) != sizes[k])
+        {
+            goto done;
+        }
+    }
+    // Candidates: threads per world workgroup for a batch; cooperative
+    // workgroups x tail budget for one world.
+    const bool batch = count > 1;
+    static const int32_t kThreads[] = {64, 128, 256, 512};
+    static const int32_t kBlocks[] = {0, 24, 32, 48};
+    static const int32_t kTails[] = {512, 1024, 2048};
+    int32_t n = 0;
+    int32_t best = -1;
+    const int32_t total = batch ? 4 : 12;
+    for (int32_t c = 0; c < total && n < M3_AUTOTUNE_MAX_CANDIDATES; ++c)
+    {
+        const int32_t a = batch ? kThreads[c] : kBlocks[c / 3];
+        const int32_t b = batch ? 0 : kTails[c % 3];
+        if (batch ? m3gpu_set_batch_threads(a) != 0
+                  : m3gpu_set_substep_tuning(WorldFromId(worlds[0])->dev, a, b) != 0)
+        {
+            continue; // beyond this device's limits
+        }
+        if (!RestoreAll(worlds, count, snaps, sizes))
+        {
+            goto done;
+        }
+        // One untimed step (graph capture, first touches), then the trial.
+        if (batch)
+        {
+            m3World_StepMany(worlds, count, timeStep, subStepCount);
+        }
+        else
+        {
+            m3World_Step(worlds[0], timeStep, subStepCount);
+        }
+        SyncAll(worlds, count);
+        const double t0 = MonotonicSeconds();
+        for (int32_t s = 0; s < trialSteps; ++s)
+        {
+            if (batch)
+            {
+                m3World_StepMany(worlds, count, timeStep, subStepCount);
+            }
+            else
+            {
+                m3World_Step(worlds[0], timeStep, subStepCount);
+            }
+        }
+        SyncAll(worlds, count);
+        const double ms = 1.0e3 * (MonotonicSeconds() - t0) / (double)trialSteps;
+        result->candidateA[n] = a;
+        result->candidateB[n] = b;
+        result->candidateMs[n] = (float)ms;
+        if (best < 0 || ms < (double)result->candidateMs[best])
+        {
+            best = n;
+        }
+        n += 1;
+    }
+    if (best < 0)
+    {
+        goto done;
+    }
+    if (batch)
+    {
+        m3gpu_set_batch_threads(result->candidateA[best]);
+        result->batchThreads = result->candidateA[best];
+    }
+    else
+    {
+        m3gpuWorld* dev = WorldFromId(worlds[0])->dev;
+        m3gpu_set_substep_tuning(dev, result->candidateA[best], result->candidateB[best]);
+        m3gpu_get_substep_tuning(dev, &result->substepBlocks, &result->tailLimit);
+    }
+    result->candidates = n;
+    if (!RestoreAll(worlds, count, snaps, sizes))
+    {
+        goto done;
+    }
+    rc = n;
+done:
+    if (snaps != NULL)
+    {
+        for (int32_t k = 0; k < count; ++k)
+        {
+            free(snaps[k]);
+        }
+    }
+    free(snaps);
+    free(sizes);
+    return rc;
 }
 
 // Stage profile of the following steps (device event times + counts).

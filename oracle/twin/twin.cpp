@@ -924,3 +924,34 @@ extern "C" int m3twin_body_state(m3gpuWorld* w, int32_t b, double out[13])
     out[12] = (double)v.av[b].z;
     return 0;
 }
+
+// Launch tuning has no meaning for the serial CPU twin: accepted, ignored.
+extern "C" int m3gpu_set_substep_tuning(m3gpuWorld* w, int32_t blocks, int32_t tailLimit)
+{
+    (void)w;
+    return blocks >= 0 && tailLimit >= 0 ? 0 : -1;
+}
+
+extern "C" int m3gpu_get_substep_tuning(const m3gpuWorld* w, int32_t* blocks, int32_t* tailLimit)
+{
+    (void)w;
+    if (blocks != nullptr)
+    {
+        *blocks = 1;
+    }
+    if (tailLimit != nullptr)
+    {
+        *tailLimit = 0;
+    }
+    return 0;
+}
+
+extern "C" int m3gpu_set_batch_threads(int32_t threads)
+{
+    return threads >= 0 ? 0 : -1;
+}
+
+extern "C" int32_t m3gpu_get_batch_threads(void)
+{
+    return 1;
+}
