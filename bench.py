@@ -16,7 +16,8 @@ worlds: forward time, adjoint time and their ratio).
 Prints one JSON line.  ``roofline`` reports the dominant kernel's
 algorithmic HBM bytes over its event-timed average duration; ``cpu_baseline``
 times the reference engine (built from its C sources into oracle/_ref) on
-the host cores over a bounded prefix of the same trajectory.
+the host cores over a bounded prefix of the same trajectory, serial and
+through its task hooks on a host thread pool, and reports the faster.
 """
 
 from __future__ import annotations
@@ -109,27 +110,77 @@ def run_roofline(steps: int, warmup: int) -> dict | None:
     return stats
 
 
-def run_cpu(steps: int) -> dict | None:
-    ref_path = os.path.join(ROOT, "oracle", "_ref", "libmaul3d_ref.so")
-    if not os.path.exists(ref_path):
-        return None
-    from maul3d_amd import api, scenes
+def host_threads() -> int:
+    """Threads for the threaded reference: the host cores this process may
+    use, at most 16 (one GPU's share of the host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
-    ref = api.Api(C.CDLL(ref_path))
+
+def _time_reference(ref, steps: int) -> tuple[float, str]:
+    from maul3d_amd import scenes
+
     world = scenes.rain(ref, count=BODIES)
     t0 = time.perf_counter()
     for _ in range(steps):
         ref.m3World_Step(world, DT, SUBSTEPS)
     wall = time.perf_counter() - t0
+    h = "%016x" % ref.m3World_Hash(world)
     ref.m3DestroyWorld(world)
-    return {
-        "value": BODIES * SUBSTEPS * steps / wall,
+    return wall, h
+
+
+def run_cpu(steps: int) -> dict | None:
+    """The reference engine on the host, serial (its default, no task hooks)
+    and threaded (its task hooks served by oracle/task_pool.c: parallel
+    narrowphase and colored solve, include/maul3d/world.h:14-26), on steps
+    1..steps of the same rain.  ``value`` is the faster of the two."""
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libmaul3d_ref.so")
+    if not os.path.exists(ref_path):
+        return None
+    from maul3d_amd import api
+
+    ref = api.Api(C.CDLL(ref_path))
+    serial_wall, serial_hash = _time_reference(ref, steps)
+    rate = lambda wall: BODIES * SUBSTEPS * steps / wall  # noqa: E731
+    out = {
+        "value": rate(serial_wall),
         "unit": "body-substeps/s",
         "cores": 1,
         "kind": "reference",
-        "sample": f"reference engine (oracle/_ref, serial, no task hooks) on steps 1-{steps} "
-        f"of the same 10k rain; {wall:.1f} s",
+        "serial_value": rate(serial_wall),
+        "sample": f"reference engine (oracle/_ref) on steps 1-{steps} of the same 10k rain; "
+        f"serial (no task hooks) {serial_wall:.1f} s",
     }
+    pool_path = os.path.join(ROOT, "oracle", "libm3pool.so")
+    if not os.path.exists(pool_path):
+        return out
+    threads = host_threads()
+    pool_lib = C.CDLL(pool_path)
+    for name in ("m3pool_create", "m3pool_enqueue_fn", "m3pool_finish_fn"):
+        getattr(pool_lib, name).restype = C.c_void_p
+    pool_lib.m3pool_create.argtypes = [C.c_int32]
+    pool_lib.m3pool_destroy.argtypes = [C.c_void_p]
+    pool = pool_lib.m3pool_create(threads)
+    if not pool:
+        return out
+    ref.task_hooks = (pool_lib.m3pool_enqueue_fn(), pool_lib.m3pool_finish_fn(), pool)
+    try:
+        thr_wall, thr_hash = _time_reference(ref, steps)
+    finally:
+        ref.task_hooks = None
+        pool_lib.m3pool_destroy(pool)
+    out["threaded_value"] = rate(thr_wall)
+    out["threaded_cores"] = threads
+    out["threaded_hash_equals_serial"] = thr_hash == serial_hash
+    out["sample"] += f", threaded on {threads} host threads {thr_wall:.1f} s"
+    if thr_wall < serial_wall:
+        out["value"] = rate(thr_wall)
+        out["cores"] = threads
+    return out
 
 
 def batch_fixture() -> dict | None:

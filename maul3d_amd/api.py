@@ -333,6 +333,10 @@ class Api:
         ),
     ):
         self.lib = lib
+        # (enqueueTask, finishTask, userTaskContext) pointers written into
+        # every m3WorldDef the scene builders make through this Api, or None
+        # for serial worlds (the reference default, include/maul3d/world.h:60).
+        self.task_hooks: tuple[int, int, int] | None = None
         for name, (res, args) in _SIGNATURES.items():
             try:
                 fn = getattr(lib, name)

@@ -65,6 +65,9 @@ def _world(api: Api, bodies: int, shapes: int, joints: int | None = None) -> Wor
     d.shapeCapacity = shapes
     if joints is not None:
         d.jointCapacity = joints
+    hooks = getattr(api, "task_hooks", None)
+    if hooks is not None:
+        d.enqueueTask, d.finishTask, d.userTaskContext = hooks
     w = api.m3CreateWorld(C.byref(d))
     if w.index1 == 0:
         raise RuntimeError("m3CreateWorld refused the def")

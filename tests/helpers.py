@@ -29,6 +29,28 @@ def ref_api():
     return api.Api(C.CDLL(REF_LIB))
 
 
+POOL_LIB = os.path.join(ROOT, "oracle", "libm3pool.so")
+
+
+def threaded_ref_api(threads: int):
+    """The reference engine with its task hooks (include/maul3d/world.h:14-26)
+    served by a pool of ``threads`` host threads; returns (api, destroy)."""
+    import pytest
+
+    if not os.path.exists(POOL_LIB):
+        pytest.skip("oracle/libm3pool.so not built (make -C oracle restated)")
+    ref = ref_api()
+    pool_lib = C.CDLL(POOL_LIB)
+    for name in ("m3pool_create", "m3pool_enqueue_fn", "m3pool_finish_fn"):
+        getattr(pool_lib, name).restype = C.c_void_p
+    pool_lib.m3pool_create.argtypes = [C.c_int32]
+    pool_lib.m3pool_destroy.argtypes = [C.c_void_p]
+    pool = pool_lib.m3pool_create(threads)
+    assert pool
+    ref.task_hooks = (pool_lib.m3pool_enqueue_fn(), pool_lib.m3pool_finish_fn(), pool)
+    return ref, lambda: pool_lib.m3pool_destroy(pool)
+
+
 def product_api():
     import maul3d_amd
 
