@@ -85,6 +85,7 @@ typedef struct m3gpuHostState
     float* shapeRestitution;
     float* shapeRollingResistance;
     uint8_t* shapeSensor;
+    uint8_t* shapeHitEvents;
     uint8_t* shapeHasOffset;
     void* shapeLocalPos;   // m3Vec3
     void* shapeLocalRot;   // m3Quat
@@ -207,6 +208,39 @@ M3GPU_API int m3gpu_get_substep_tuning(const m3gpuWorld* world, int32_t* blocks,
 M3GPU_API int m3gpu_set_batch_threads(int32_t threads);
 M3GPU_API int32_t m3gpu_get_batch_threads(void);
 
+// Event streams of the world's last step (src/solver.c:3732-3815,
+// 2991-3026, 4295-4306), computed on the device when called: contact and
+// sensor begin/end events as pair keys in ascending key order, hit events
+// (hit threshold = the world's at that step) in pair order and body move
+// events in ascending body order.  The caller's arrays hold pairCapacity
+// keys or hits and bodyCapacity moves.  After a capacity stall every count
+// is zero and `stalled` is set.
+typedef struct m3gpuHit
+{
+    uint64_t key; // shape A << 32 | shape B
+    double point[3];
+    float normal[3];
+    float approachSpeed;
+} m3gpuHit;
+typedef struct m3gpuMove
+{
+    int32_t body;
+    uint8_t fellAsleep;
+    uint8_t pad[3];
+    double p[3]; // post-step transform (m3Transform: f64 position, f32 quaternion)
+    float q[4];
+} m3gpuMove;
+typedef struct m3gpuEvents
+{
+    uint64_t* beginKeys;
+    uint64_t* endKeys;
+    uint64_t* sensorBeginKeys;
+    uint64_t* sensorEndKeys;
+    m3gpuHit* hits;
+    m3gpuMove* moves;
+    int32_t counts[7]; // begin, end, sensor begin, sensor end, hits, moves, stalled
+} m3gpuEvents;
+M3GPU_API int m3gpu_events(m3gpuWorld* world, float hitThreshold, m3gpuEvents* out);
 // Waits for all queued device work of this world.
 M3GPU_API int m3gpu_sync(m3gpuWorld* world);
 

@@ -246,6 +246,48 @@ M3_API uint64_t m3World_Hash(m3WorldId worldId);
 M3_API m3Counters m3World_GetCounters(m3WorldId worldId);       // world.h:554
 M3_API m3Profile m3World_GetProfile(m3WorldId worldId);         // world.h:593
 
+// ---- event streams of the last step (include/maul3d/world.h:206-313) -------
+// Same records, order and lifetime as the reference: pointers stay valid until
+// the next step, restore or world destruction; restore clears the streams.
+// The device builds them when first read after a step (or before a host
+// mutation of the world); joints never break in this build, so the joint
+// break stream is always empty, and hits never overflow (at most one per
+// constraint).
+typedef struct m3ContactEvent // world.h:213-217
+{
+    m3ShapeId shapeA; // the lower shape index of the pair
+    m3ShapeId shapeB;
+} m3ContactEvent;
+typedef struct m3HitEvent // world.h:273-280
+{
+    m3ShapeId shapeA;
+    m3ShapeId shapeB;
+    m3Pos3 point;
+    m3Vec3 normal;
+    m3real approachSpeed;
+} m3HitEvent;
+typedef struct m3BodyMoveEvent // world.h:295-300
+{
+    m3BodyId body;
+    m3Transform transform;
+    bool fellAsleep;
+} m3BodyMoveEvent;
+typedef struct m3JointBreakEvent // world.h:308-311
+{
+    m3JointId joint;
+} m3JointBreakEvent;
+M3_API const m3ContactEvent* m3World_ContactBeginEvents(m3WorldId worldId, int32_t* count); // world.h:223
+M3_API const m3ContactEvent* m3World_ContactEndEvents(m3WorldId worldId, int32_t* count);   // world.h:224
+M3_API const m3ContactEvent* m3World_SensorBeginEvents(m3WorldId worldId, int32_t* count);  // world.h:265
+M3_API const m3ContactEvent* m3World_SensorEndEvents(m3WorldId worldId, int32_t* count);    // world.h:266
+M3_API const m3HitEvent* m3World_HitEvents(m3WorldId worldId, int32_t* count);              // world.h:282
+M3_API int32_t m3World_HitEventsDropped(m3WorldId worldId);                                 // world.h:285
+M3_API void m3World_SetHitEventThreshold(m3WorldId worldId, float value);                   // world.h:288
+M3_API const m3BodyMoveEvent* m3World_BodyMoveEvents(m3WorldId worldId, int32_t* count);    // world.h:302
+M3_API const m3JointBreakEvent* m3World_JointBreakEvents(m3WorldId worldId, int32_t* count); // world.h:313
+M3_API void m3Shape_EnableHitEvents(m3ShapeId shapeId, bool flag);                          // shape.h:300
+M3_API bool m3Shape_AreHitEventsEnabled(m3ShapeId shapeId);                                 // shape.h:301
+
 // Extensions of this build (no reference counterpart): step many worlds with
 // one call (the batched rollback and toss workloads; returns the number of
 // valid worlds stepped or -1 on bad arguments), and wait for a world's

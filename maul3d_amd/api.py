@@ -316,6 +316,51 @@ _SIGNATURES["m3World_Adjoint"] = (
 )
 
 
+class ContactEvent(C.Structure):  # include/maul3d/world.h:213-217
+    _fields_ = [("shapeA", ShapeId), ("shapeB", ShapeId)]
+
+
+class HitEvent(C.Structure):  # include/maul3d/world.h:273-280
+    _fields_ = [
+        ("shapeA", ShapeId),
+        ("shapeB", ShapeId),
+        ("point", Pos3),
+        ("normal", Vec3),
+        ("approachSpeed", c_f),
+    ]
+
+
+class BodyMoveEvent(C.Structure):  # include/maul3d/world.h:295-300
+    _fields_ = [("body", BodyId), ("transform", Transform), ("fellAsleep", C.c_bool)]
+
+
+class JointBreakEvent(C.Structure):  # include/maul3d/world.h:308-311
+    _fields_ = [("joint", c_i32), ("world0", c_u16), ("generation", c_u16)]
+
+
+EVENT_STREAMS = {
+    "m3World_ContactBeginEvents": ContactEvent,
+    "m3World_ContactEndEvents": ContactEvent,
+    "m3World_SensorBeginEvents": ContactEvent,
+    "m3World_SensorEndEvents": ContactEvent,
+    "m3World_HitEvents": HitEvent,
+    "m3World_BodyMoveEvents": BodyMoveEvent,
+    "m3World_JointBreakEvents": JointBreakEvent,
+}
+for _name, _rec in EVENT_STREAMS.items():
+    _SIGNATURES[_name] = (C.POINTER(_rec), [WorldId, C.POINTER(c_i32)])
+_SIGNATURES["m3World_HitEventsDropped"] = (c_i32, [WorldId])
+_SIGNATURES["m3World_SetHitEventThreshold"] = (None, [WorldId, c_f])
+_SIGNATURES["m3Shape_EnableHitEvents"] = (None, [ShapeId, C.c_bool])
+_SIGNATURES["m3Shape_AreHitEventsEnabled"] = (C.c_bool, [ShapeId])
+
+
+def _id_tuple(sid) -> tuple[int, int]:
+    """(index1, generation): the world slot is left out, so ids compare
+    across libraries and processes."""
+    return (sid.index1, sid.generation)
+
+
 class Api:
     """Typed view of one library's ``m3*`` exports."""
 
@@ -349,6 +394,52 @@ class Api:
             setattr(self, name, fn)
 
     # convenience wrappers -------------------------------------------------
+    def events(self, world: WorldId) -> dict:
+        """The last step's event streams as plain tuples (bit patterns of the
+        floating-point fields, so comparisons are exact)."""
+        import struct
+
+        def bits_d(x: float) -> int:
+            return struct.unpack("<Q", struct.pack("<d", x))[0]
+
+        def bits_f(x: float) -> int:
+            return struct.unpack("<I", struct.pack("<f", x))[0]
+
+        out = {}
+        for name, rec in EVENT_STREAMS.items():
+            n = c_i32(0)
+            ptr = getattr(self, name)(world, C.byref(n))
+            items = []
+            for i in range(n.value):
+                e = ptr[i]
+                if rec is ContactEvent:
+                    items.append((_id_tuple(e.shapeA), _id_tuple(e.shapeB)))
+                elif rec is HitEvent:
+                    items.append(
+                        (
+                            _id_tuple(e.shapeA),
+                            _id_tuple(e.shapeB),
+                            tuple(bits_d(c) for c in (e.point.x, e.point.y, e.point.z)),
+                            tuple(bits_f(c) for c in (e.normal.x, e.normal.y, e.normal.z)),
+                            bits_f(e.approachSpeed),
+                        )
+                    )
+                elif rec is BodyMoveEvent:
+                    t = e.transform
+                    items.append(
+                        (
+                            _id_tuple(e.body),
+                            tuple(bits_d(c) for c in (t.p.x, t.p.y, t.p.z)),
+                            tuple(bits_f(c) for c in (t.q.x, t.q.y, t.q.z, t.q.w)),
+                            bool(e.fellAsleep),
+                        )
+                    )
+                else:
+                    items.append((e.joint, e.generation))
+            out[name.replace("m3World_", "")] = items
+        out["HitEventsDropped"] = self.m3World_HitEventsDropped(world)
+        return out
+
     def snapshot(self, world: WorldId) -> bytes:
         size = self.m3World_SnapshotSize(world)
         if size < 0:

@@ -97,6 +97,8 @@ struct m3gpuWorld
     int tailLimit;   // tuned budget of the workgroup-0 tail (0: M3_TAIL_CONSTRAINTS)
     int coopThreads; // threads per block of the substep kernel
     int wideHulls;   // some hull is larger than a box (wide hull pairs possible)
+    m3HitRec* hitRecBuf; // per pair; in the view only while some shape has hit events
+    struct m3EventBufs* events; // event classification and output (allocated at first read)
     int boxHulls;    // some hull is at least box-sized
 };
 
@@ -1177,6 +1179,10 @@ __global__ void __launch_bounds__(64) k_narrow_binned(const m3Dev* __restrict__ 
         return;
     }
     const int32_t i = d.narrowBins[8 + (int64_t)cls * v.pairCapacity + k];
+    if (v.hitRecs != nullptr)
+    {
+        v.hitRecs[i].flag = 0;
+    }
     // The island union of a touching pair follows its manifold here (the
     // bodies' union-find entries were reset by k_capture before this launch).
     if (m3NarrowPair(&v, i, v.sc->oldPairCount) > 0)
@@ -2356,6 +2362,222 @@ __global__ void k_end(const m3Dev* __restrict__ ds)
 }
 
 // ---------------------------------------------------------------------------
+// Event streams of the last step (src/solver.c:3732-3815 contact and sensor
+// begin/end, :2991-3026 hits, :4295-4306 body moves), built when a host reads
+// them.  The reference walks the old and new pair lists (both ascending)
+// together; every begin event is a new pair (touching now, not touching or
+// absent before) and every end event an old pair (touching before, not
+// touching or absent now), so each stream is one list filtered in its own
+// order.  Hits keep constraint order, which is pair order.
+// ---------------------------------------------------------------------------
+
+struct m3EventBufs
+{
+    uint8_t* codeNew;   // per new pair: 1 begin, 2 sensor begin, | 4 hit
+    uint8_t* codeOld;   // per old pair: 1 end, 2 sensor end
+    int32_t* counts;    // begin, end, sensor begin, sensor end, hits, moves, stalled
+    uint64_t* lists;    // 4 x pairCapacity keys: begin, end, sensor begin, sensor end
+    m3gpuHit* hits;     // pairCapacity
+    m3gpuMove* moves;   // bodyCapacity
+    int32_t* hostCounts; // pinned
+};
+
+// The index of key in keys[0..n) (ascending), or -1.
+__device__ static inline int32_t FindKey(const uint64_t* keys, int32_t n, uint64_t key)
+{
+    int32_t lo = 0;
+    int32_t hi = n;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi) >> 1;
+        const uint64_t k = keys[mid];
+        if (k < key)
+        {
+            lo = mid + 1;
+        }
+        else if (k > key)
+        {
+            hi = mid;
+        }
+        else
+        {
+            return mid;
+        }
+    }
+    return -1;
+}
+
+__device__ static inline uint8_t PairEventCode(const m3View& v, uint64_t key)
+{
+    const int32_t sA = (int32_t)(key >> 32);
+    const int32_t sB = (int32_t)(key & 0xFFFFFFFFu);
+    // A pair whose shape died emits nothing (its id would be stale).
+    if (v.shapeAlive[sA] == 0 || v.shapeAlive[sB] == 0)
+    {
+        return 0;
+    }
+    return (v.sensor[sA] != 0 || v.sensor[sB] != 0) ? 2 : 1;
+}
+
+__global__ void k_event_codes(const m3Dev* __restrict__ ds, m3EventBufs e, float hitThreshold)
+{
+    M3_WORLD;
+    const m3Scalars* sc = v.sc;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sc->stall != 0)
+    {
+        return;
+    }
+    const int32_t np = sc->pairCount;
+    const int32_t no = sc->oldPairCount;
+    if (i < np)
+    {
+        const uint64_t key = v.keys[i];
+        const int touchNew = v.manifolds[i].pointCount > 0;
+        const int32_t j = FindKey(v.oldKeys, no, key);
+        const int touchOld = j >= 0 && v.oldManifolds[j].pointCount > 0;
+        uint8_t code = (touchNew && !touchOld) ? PairEventCode(v, key) : 0;
+        if (v.hitRecs != nullptr && v.hitRecs[i].flag != 0 &&
+            v.hitRecs[i].relativeVelocity < -hitThreshold)
+        {
+            code |= 4;
+        }
+        e.codeNew[i] = code;
+    }
+    if (i < no)
+    {
+        const uint64_t key = v.oldKeys[i];
+        const int touchOld = v.oldManifolds[i].pointCount > 0;
+        const int32_t j = FindKey(v.keys, np, key);
+        const int touchNew = j >= 0 && v.manifolds[j].pointCount > 0;
+        e.codeOld[i] = (touchOld && !touchNew) ? PairEventCode(v, key) : 0;
+    }
+    if (i < sc->moverCount)
+    {
+        const int32_t b = v.movers[i];
+        m3gpuMove mv;
+        mv.body = b;
+        mv.fellAsleep = v.type[b] == M3_DYNAMIC && v.awake[b] == 0;
+        mv.pad[0] = 0;
+        mv.pad[1] = 0;
+        mv.pad[2] = 0;
+        const m3Transform xf = v.xf[b];
+        mv.p[0] = xf.p.x;
+        mv.p[1] = xf.p.y;
+        mv.p[2] = xf.p.z;
+        mv.q[0] = xf.q.x;
+        mv.q[1] = xf.q.y;
+        mv.q[2] = xf.q.z;
+        mv.q[3] = xf.q.w;
+        e.moves[i] = mv;
+    }
+}
+
+#define M3_EVENT_THREADS 1024
+#define M3_EVENT_LISTS 5
+
+// One workgroup: the five filtered lists in order, by ballot ranks within a
+// wavefront and per-wavefront offsets in LDS.
+__global__ void __launch_bounds__(M3_EVENT_THREADS) k_event_compact(const m3Dev* __restrict__ ds, m3EventBufs e)
+{
+    M3_WORLD;
+    const m3Scalars* sc = v.sc;
+    constexpr int W = M3_EVENT_THREADS / 64;
+    __shared__ int32_t waveCount[M3_EVENT_LISTS][W];
+    __shared__ int32_t total[M3_EVENT_LISTS];
+    const int t = (int)threadIdx.x;
+    const int wave = t >> 6;
+    const int lane = t & 63;
+    const int stalled = sc->stall != 0;
+    if (t < M3_EVENT_LISTS)
+    {
+        total[t] = 0;
+    }
+    __syncthreads();
+    const int32_t np = stalled ? 0 : sc->pairCount;
+    const int32_t no = stalled ? 0 : sc->oldPairCount;
+    const int32_t n = np > no ? np : no;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int32_t base = 0; base < n; base += M3_EVENT_THREADS)
+    {
+        const int32_t i = base + t;
+        const uint8_t cn = i < np ? e.codeNew[i] : 0;
+        const uint8_t co = i < no ? e.codeOld[i] : 0;
+        bool pred[M3_EVENT_LISTS];
+        pred[0] = (cn & 3) == 1;
+        pred[1] = (co & 3) == 1;
+        pred[2] = (cn & 3) == 2;
+        pred[3] = (co & 3) == 2;
+        pred[4] = (cn & 4) != 0;
+        uint64_t mask[M3_EVENT_LISTS];
+        for (int k = 0; k < M3_EVENT_LISTS; ++k)
+        {
+            mask[k] = __ballot(pred[k]);
+            if (lane == 0)
+            {
+                waveCount[k][wave] = __popcll(mask[k]);
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < M3_EVENT_LISTS; ++k)
+        {
+            if (!pred[k])
+            {
+                continue;
+            }
+            int32_t pos = total[k];
+            for (int w2 = 0; w2 < wave; ++w2)
+            {
+                pos += waveCount[k][w2];
+            }
+            pos += __popcll(mask[k] & below);
+            if (k == 0 || k == 2)
+            {
+                e.lists[(int64_t)k * v.pairCapacity + pos] = v.keys[i];
+            }
+            else if (k == 1 || k == 3)
+            {
+                e.lists[(int64_t)k * v.pairCapacity + pos] = v.oldKeys[i];
+            }
+            else
+            {
+                const m3HitRec r = v.hitRecs[i];
+                m3gpuHit hit;
+                hit.key = v.keys[i];
+                hit.point[0] = r.point[0];
+                hit.point[1] = r.point[1];
+                hit.point[2] = r.point[2];
+                hit.normal[0] = r.normal.x;
+                hit.normal[1] = r.normal.y;
+                hit.normal[2] = r.normal.z;
+                hit.approachSpeed = -r.relativeVelocity;
+                e.hits[pos] = hit;
+            }
+        }
+        __syncthreads();
+        if (t < M3_EVENT_LISTS)
+        {
+            int32_t sum = 0;
+            for (int w2 = 0; w2 < W; ++w2)
+            {
+                sum += waveCount[t][w2];
+            }
+            total[t] += sum;
+        }
+        __syncthreads();
+    }
+    if (t < M3_EVENT_LISTS)
+    {
+        e.counts[t] = total[t];
+    }
+    if (t == 0)
+    {
+        e.counts[5] = stalled ? 0 : sc->moverCount;
+        e.counts[6] = stalled;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Batches of small worlds: one workgroup per world replaces each
 // device-wide sort and compaction, with the world's data staged in LDS.
 // ---------------------------------------------------------------------------
@@ -2609,7 +2831,7 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.localPos, ns) || DevAlloc(&v.localRot, ns) || DevAlloc(&v.surfaceVel, ns) ||
         DevAlloc(&v.category, ns) || DevAlloc(&v.mask, ns) || DevAlloc(&v.group, ns) ||
         DevAlloc(&v.shapeNext, ns) || DevAlloc(&v.hullIndex, ns) || DevAlloc(&v.leaf, 6 * (int64_t)ns) ||
-        DevAlloc(&v.fresh, 6 * (int64_t)ns) || DevAlloc(&v.planes, ns))
+        DevAlloc(&v.fresh, 6 * (int64_t)ns) || DevAlloc(&v.planes, ns) || DevAlloc(&v.hitEvents, ns))
     {
         return -1;
     }
@@ -2642,7 +2864,8 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.slotColors, np) ||
         DevAlloc(&v.consBodies, 2 * (int64_t)np) || DevAlloc(&v.slotOf, np) ||
         DevAlloc(&v.slotTable, (int64_t)M3_GRAPH_COLORS * nb) || DevAlloc(&v.slotMask, nb) ||
-        DevAlloc(&w->dev.barrier, 4) || DevAlloc(&w->dev.narrowBins, 8 + 8 * (int64_t)np))
+        DevAlloc(&w->dev.barrier, 4) || DevAlloc(&w->dev.narrowBins, 8 + 8 * (int64_t)np) ||
+        DevAlloc(&w->hitRecBuf, np))
     {
         return -1;
     }
@@ -2721,6 +2944,9 @@ static int CreateImpl(m3gpuWorld* w)
     v.manifolds = w->mansBuf[0];
     v.oldKeys = w->keysBuf[1];
     v.oldManifolds = w->mansBuf[1];
+    // The clears of DevAlloc ran on the null stream, which does not order
+    // against the world's non-blocking stream.
+    HIP_TRY(hipStreamSynchronize(nullptr));
     HIP_TRY(hipStreamSynchronize(w->stream));
     return 0;
 }
@@ -2749,6 +2975,7 @@ extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* caps)
 }
 
 static void ProfFree(m3gpuWorld* w);
+static void EventBufsFree(m3gpuWorld* w);
 
 extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
 {
@@ -2763,7 +2990,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.linDamp, v.angDamp, v.bodyAlive, v.type, v.awake, v.sleepTime, v.bullet,
                     v.minExt, v.maxExt, v.force, v.torque, v.enabled, v.locks, v.sleepThreshold,
                     v.canSleep, v.hasTarget, v.target, v.shapeHead, v.shapeAlive, v.shapeBody,
-                    v.shapeType, v.geom, v.friction, v.restitution, v.rolling, v.sensor,
+                    v.shapeType, v.geom, v.friction, v.restitution, v.rolling, v.sensor, v.hitEvents, w->hitRecBuf,
                     v.hasOffset, v.localPos, v.localRot, v.surfaceVel, v.category, v.mask,
                     v.group, v.shapeNext, v.hullIndex, v.leaf, v.fresh, v.planes,
                     (void*)v.hulls, w->keysBuf[0], w->keysBuf[1], w->mansBuf[0], w->mansBuf[1],
@@ -2786,6 +3013,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
         }
     }
     (void)hipHostFree(w->hostScalars);
+    EventBufsFree(w);
     ProfFree(w);
     (void)hipStreamDestroy(w->stream);
     free(w);
@@ -2851,6 +3079,17 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(v.restitution, h->shapeRestitution, ns, float);
         UP(v.rolling, h->shapeRollingResistance, ns, float);
         UP(v.sensor, h->shapeSensor, ns, uint8_t);
+        UP(v.hitEvents, h->shapeHitEvents, ns, uint8_t);
+        {
+            // Hit records are kept (cleared by the narrowphase, written by the
+            // impulse store) only in worlds where some shape opted in.
+            int anyHit = 0;
+            for (int32_t s = 0; s < ns && !anyHit; ++s)
+            {
+                anyHit = h->shapeAlive[s] != 0 && h->shapeHitEvents[s] != 0;
+            }
+            v.hitRecs = anyHit ? w->hitRecBuf : nullptr;
+        }
         UP(v.hasOffset, h->shapeHasOffset, ns, uint8_t);
         UP(v.localPos, h->shapeLocalPos, ns, m3Vec3);
         UP(v.localRot, h->shapeLocalRot, ns, m3Quat);
@@ -2916,9 +3155,11 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
             {
                 unsupported = 1;
             }
-            else if (h->types[h->shapeBody[s]] == M3_STATIC)
+            else if (h->types[h->shapeBody[s]] == M3_STATIC && h->shapeSensor[s] == 0)
             {
-                unsupported = 1; // static tree targets for continuous sweeps
+                // A static tree target of continuous sweeps (sensors never
+                // stop a sweep, src/solver.c:3102-3105).
+                unsupported = 1;
             }
         }
         for (int32_t b = 0; b < nb; ++b)
@@ -3149,6 +3390,101 @@ static m3SubstepParams SubstepParams(const m3gpuStepParams* p)
     sp.maxAngular = p->maximumAngularSpeed;
     sp.gravity = m3V3(p->gravity[0], p->gravity[1], p->gravity[2]);
     return sp;
+}
+
+static int EventBufsCreate(m3gpuWorld* w)
+{
+    if (w->events != nullptr)
+    {
+        return 0;
+    }
+    m3EventBufs* e = (m3EventBufs*)calloc(1, sizeof(m3EventBufs));
+    if (e == nullptr)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_events: out of host memory");
+        return -1;
+    }
+    const int32_t np = w->caps.pairCapacity;
+    const int32_t nb = w->caps.bodyCapacity;
+    if (DevAlloc(&e->codeNew, np) || DevAlloc(&e->codeOld, np) || DevAlloc(&e->counts, 8) ||
+        DevAlloc(&e->lists, 4 * (int64_t)np) || DevAlloc(&e->hits, np) || DevAlloc(&e->moves, nb) ||
+        hipHostMalloc((void**)&e->hostCounts, 8 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_events: device allocation failed");
+        w->events = e;
+        EventBufsFree(w); // nothing half-built stays behind
+        return -1;
+    }
+    // DevAlloc clears with hipMemset on the null stream, which does not order
+    // against the world's non-blocking stream: finish the clears before the
+    // event kernels write these buffers.
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    w->events = e;
+    return 0;
+}
+
+static void EventBufsFree(m3gpuWorld* w)
+{
+    m3EventBufs* e = w->events;
+    if (e == nullptr)
+    {
+        return;
+    }
+    void* ptrs[] = {e->codeNew, e->codeOld, e->counts, e->lists, e->hits, e->moves};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
+    {
+        if (ptrs[i] != nullptr)
+        {
+            (void)hipFree(ptrs[i]);
+        }
+    }
+    if (e->hostCounts != nullptr)
+    {
+        (void)hipHostFree(e->hostCounts);
+    }
+    free(e);
+    w->events = nullptr;
+}
+
+extern "C" int m3gpu_events(m3gpuWorld* w, float hitThreshold, m3gpuEvents* out)
+{
+    if (w == nullptr || out == nullptr || JoinBatch(w) != 0)
+    {
+        return -1;
+    }
+    if (EventBufsCreate(w) != 0)
+    {
+        return -1;
+    }
+    if (w->devDirty && RefreshDevCopies(w) != 0)
+    {
+        return -1;
+    }
+    m3EventBufs& e = *w->events;
+    const m3Dev* D = w->devDev + w->cur;
+    const int32_t np = w->caps.pairCapacity;
+    const int32_t nb = w->caps.bodyCapacity;
+    const int32_t n = np > nb ? np : nb;
+    hipStream_t st = w->stream;
+    KL(k_event_codes, dim3(Blocks(n, 256)), dim3(256), 0, st, D, e, hitThreshold);
+    HIP_TRY(hipGetLastError());
+    KL(k_event_compact, dim3(1), dim3(M3_EVENT_THREADS), 0, st, D, e);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(e.hostCounts, e.counts, 7 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    for (int k = 0; k < 7; ++k)
+    {
+        out->counts[k] = e.hostCounts[k];
+    }
+    uint64_t* dst[4] = {out->beginKeys, out->endKeys, out->sensorBeginKeys, out->sensorEndKeys};
+    for (int k = 0; k < 4; ++k)
+    {
+        DOWN(dst[k], e.lists + (int64_t)k * np, out->counts[k], uint64_t);
+    }
+    DOWN(out->hits, e.hits, out->counts[4], m3gpuHit);
+    DOWN(out->moves, e.moves, out->counts[5], m3gpuMove);
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
 }
 
 // Enqueue one step.  Launch geometry only depends on host-known capacities

@@ -264,6 +264,10 @@ M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams
     }
     for (int32_t i = 0; i < count; ++i)
     {
+        if (v->hitRecs != nullptr)
+        {
+            v->hitRecs[i].flag = 0;
+        }
         m3NarrowPair(v, i, oldPairCount);
     }
     if (tape != NULL)

@@ -127,6 +127,20 @@ typedef struct m3Scalars
     uint64_t stepCount;
 } m3Scalars;
 
+// A step's hit candidate for one pair (src/solver.c:2991-3026): the
+// fastest-approaching point among those whose accumulated normal impulse is
+// positive, kept whatever the world's hit threshold; the threshold is applied
+// when the events are read.  flag is cleared by the narrowphase and set by
+// the impulse store.
+typedef struct m3HitRec
+{
+    double point[3]; // world point: body A's position + rotated local center + rA
+    m3Vec3 normal;
+    m3real relativeVelocity;
+    int32_t flag;
+    int32_t pad;
+} m3HitRec;
+
 typedef struct m3View
 {
     int32_t maxBody;
@@ -170,6 +184,7 @@ typedef struct m3View
     m3real* restitution;
     m3real* rolling;
     uint8_t* sensor;
+    uint8_t* hitEvents; // per shape: hit events opted in
     uint8_t* hasOffset;
     m3Vec3* localPos;
     m3Quat* localRot;
@@ -213,6 +228,7 @@ typedef struct m3View
     uint8_t* flag;
     uint8_t* pending; // bodies whose frozen pairs the next sweep records
     m3Scalars* sc;
+    m3HitRec* hitRecs; // per pair of the current list, or null (no hit records)
     // joints: slot arrays (the reference's world arrays), the joint
     // components (joints linked through non-static bodies, each a run of
     // ascending slots), and this step's prepared records
@@ -2002,13 +2018,51 @@ M3_HD void m3RestitutionOne(const m3View* v, m3Constraint* c, m3real threshold)
     }
 }
 
-// src/solver.c:2979-3048 without the observer hit events.
+// src/solver.c:2979-3048.  The hit event's point selection (:2991-3026)
+// is recorded per pair when a hit record array is present: among the points
+// with a positive accumulated impulse, the first with the lowest relative
+// normal velocity.  When that velocity is below minus the threshold it is
+// exactly the point the reference's loop selects (the reference's candidate
+// set is this set cut at the threshold, and its minimum is this minimum);
+// otherwise the reference selects none.
 M3_HD void m3StoreOne(const m3View* v, const m3Constraint* c)
 {
     m3Manifold* m = &v->manifolds[c->manifoldIndex];
     for (int32_t k = 0; k < c->pointCount; ++k)
     {
         m->points[k].normalImpulse = c->points[k].normalImpulse;
+    }
+    if (v->hitRecs != nullptr)
+    {
+        const uint64_t key = v->keys[c->manifoldIndex];
+        const int32_t shapeA = (int32_t)(key >> 32);
+        const int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+        if (v->hitEvents[shapeA] != 0 || v->hitEvents[shapeB] != 0)
+        {
+            int32_t best = -1;
+            for (int32_t k = 0; k < c->pointCount; ++k)
+            {
+                const m3real rv = c->points[k].relativeVelocity;
+                if (rv == rv && c->points[k].totalNormalImpulse > 0.0f &&
+                    (best < 0 || rv < c->points[best].relativeVelocity))
+                {
+                    best = k;
+                }
+            }
+            if (best >= 0)
+            {
+                m3HitRec* r = &v->hitRecs[c->manifoldIndex];
+                const m3Transform xa = v->xf[c->bodyA];
+                const m3Vec3 lcA = m3RotateVec3(xa.q, v->lc[c->bodyA]);
+                const m3Vec3 rA = c->points[best].rA;
+                r->point[0] = xa.p.x + (double)lcA.x + (double)rA.x;
+                r->point[1] = xa.p.y + (double)lcA.y + (double)rA.y;
+                r->point[2] = xa.p.z + (double)lcA.z + (double)rA.z;
+                r->normal = c->normal;
+                r->relativeVelocity = c->points[best].relativeVelocity;
+                r->flag = 1;
+            }
+        }
     }
     m->frictionImpulse =
         m3Add3(m3MulSV3(c->frictionImpulse1, c->t1), m3MulSV3(c->frictionImpulse2, c->t2));

@@ -260,6 +260,21 @@ typedef struct m3World
     uint32_t devDirty;  // field groups the host changed since the last upload
     uint32_t hostStale; // field groups the device changed since the last download
 
+    // event streams of the last step: 0 none (creation, restore), 1 to be
+    // built on the device, 2 built into the arrays below
+    int32_t eventsState;
+    float eventThreshold; // the hit threshold the last step ran with
+    m3ContactEvent* beginEvents;
+    m3ContactEvent* endEvents;
+    m3ContactEvent* sensorBeginEvents;
+    m3ContactEvent* sensorEndEvents;
+    m3HitEvent* hitEvents;
+    m3BodyMoveEvent* moveEvents;
+    int32_t eventCounts[6];
+    uint64_t* eventKeys; // 4 x pairCapacity
+    m3gpuHit* eventHits;
+    m3gpuMove* eventMoves;
+
     m3Profile profile;
     int32_t lastIslandCount;
     int32_t lastColorCount;
@@ -332,6 +347,7 @@ static void HostView(m3World* w, m3gpuHostState* h)
     h->shapeRestitution = w->shapeRestitution;
     h->shapeRollingResistance = w->shapeRollingResistance;
     h->shapeSensor = w->shapeSensor;
+    h->shapeHitEvents = w->shapeHitEvents;
     h->shapeHasOffset = w->shapeHasOffset;
     h->shapeLocalPos = w->shapeLocalPos;
     h->shapeLocalRot = w->shapeLocalRot;
@@ -496,10 +512,118 @@ static void SyncHost(m3World* w)
     w->hostStale = 0;
 }
 
-// Every mutation goes through here: read the device state back first, then
-// mark the groups the mutation touches for upload before the next step.
+// Event streams (src/solver.c:3732-3815, 2991-3026, 4295-4306): the device
+// builds the last step's streams into host arrays that stay valid until the
+// next step or restore.  Contact ids carry the shapes' generations.
+static m3ShapeId ShapeIdOf(const m3World* w, int32_t s)
+{
+    m3ShapeId id = {s + 1, w->worldIndex0, w->shapePool.generations[s]};
+    return id;
+}
+
+static void BuildEvents(m3World* w)
+{
+    if (w->eventsState != 1)
+    {
+        return;
+    }
+    w->eventsState = 2;
+    memset(w->eventCounts, 0, sizeof(w->eventCounts));
+    if (w->dev == NULL)
+    {
+        return;
+    }
+    const int32_t np = w->pairCapacity;
+    const int32_t nb = w->bodyCapacity;
+    if (w->eventKeys == NULL)
+    {
+        w->eventKeys = (uint64_t*)Zalloc(4 * (size_t)np * sizeof(uint64_t));
+        w->eventHits = (m3gpuHit*)Zalloc((size_t)np * sizeof(m3gpuHit));
+        w->eventMoves = (m3gpuMove*)Zalloc((size_t)nb * sizeof(m3gpuMove));
+        w->beginEvents = (m3ContactEvent*)Zalloc((size_t)np * sizeof(m3ContactEvent));
+        w->endEvents = (m3ContactEvent*)Zalloc((size_t)np * sizeof(m3ContactEvent));
+        w->sensorBeginEvents = (m3ContactEvent*)Zalloc((size_t)np * sizeof(m3ContactEvent));
+        w->sensorEndEvents = (m3ContactEvent*)Zalloc((size_t)np * sizeof(m3ContactEvent));
+        w->hitEvents = (m3HitEvent*)Zalloc((size_t)np * sizeof(m3HitEvent));
+        w->moveEvents = (m3BodyMoveEvent*)Zalloc((size_t)nb * sizeof(m3BodyMoveEvent));
+    }
+    m3gpuEvents out;
+    memset(&out, 0, sizeof(out));
+    out.beginKeys = w->eventKeys;
+    out.endKeys = w->eventKeys + np;
+    out.sensorBeginKeys = w->eventKeys + 2 * (size_t)np;
+    out.sensorEndKeys = w->eventKeys + 3 * (size_t)np;
+    out.hits = w->eventHits;
+    out.moves = w->eventMoves;
+    if (m3gpu_events(w->dev, w->eventThreshold, &out) != 0)
+    {
+        DeviceFail("m3gpu_events");
+    }
+    const uint64_t* keys[4] = {out.beginKeys, out.endKeys, out.sensorBeginKeys, out.sensorEndKeys};
+    m3ContactEvent* lists[4] = {w->beginEvents, w->endEvents, w->sensorBeginEvents,
+                                w->sensorEndEvents};
+    for (int k = 0; k < 4; ++k)
+    {
+        for (int32_t i = 0; i < out.counts[k]; ++i)
+        {
+            lists[k][i].shapeA = ShapeIdOf(w, (int32_t)(keys[k][i] >> 32));
+            lists[k][i].shapeB = ShapeIdOf(w, (int32_t)(keys[k][i] & 0xFFFFFFFFu));
+        }
+        w->eventCounts[k] = out.counts[k];
+    }
+    for (int32_t i = 0; i < out.counts[4]; ++i)
+    {
+        const m3gpuHit* g = &out.hits[i];
+        m3HitEvent* e = &w->hitEvents[i];
+        memset(e, 0, sizeof(*e));
+        e->shapeA = ShapeIdOf(w, (int32_t)(g->key >> 32));
+        e->shapeB = ShapeIdOf(w, (int32_t)(g->key & 0xFFFFFFFFu));
+        e->point.x = g->point[0];
+        e->point.y = g->point[1];
+        e->point.z = g->point[2];
+        e->normal = m3V3(g->normal[0], g->normal[1], g->normal[2]);
+        e->approachSpeed = g->approachSpeed;
+    }
+    w->eventCounts[4] = out.counts[4];
+    for (int32_t i = 0; i < out.counts[5]; ++i)
+    {
+        const m3gpuMove* g = &out.moves[i];
+        m3BodyMoveEvent* e = &w->moveEvents[i];
+        memset(e, 0, sizeof(*e));
+        e->body.index1 = g->body + 1;
+        e->body.world0 = w->worldIndex0;
+        e->body.generation = w->bodyPool.generations[g->body];
+        e->transform.p.x = g->p[0];
+        e->transform.p.y = g->p[1];
+        e->transform.p.z = g->p[2];
+        e->transform.q.x = g->q[0];
+        e->transform.q.y = g->q[1];
+        e->transform.q.z = g->q[2];
+        e->transform.q.w = g->q[3];
+        e->fellAsleep = g->fellAsleep != 0;
+    }
+    w->eventCounts[5] = out.counts[5];
+}
+
+static void FreeEvents(m3World* w)
+{
+    free(w->eventKeys);
+    free(w->eventHits);
+    free(w->eventMoves);
+    free(w->beginEvents);
+    free(w->endEvents);
+    free(w->sensorBeginEvents);
+    free(w->sensorEndEvents);
+    free(w->hitEvents);
+    free(w->moveEvents);
+}
+
+// Every mutation goes through here: build the last step's event streams
+// while the device still holds that step's state, read the device state back,
+// then mark the groups the mutation touches for upload before the next step.
 static void Touch(m3World* w, uint32_t groups)
 {
+    BuildEvents(w);
     SyncHost(w);
     w->devDirty |= groups;
 }
@@ -744,6 +868,7 @@ void m3DestroyWorld(m3WorldId worldId)
     {
         free(arrays[i]);
     }
+    FreeEvents(w);
     PoolDestroy(&w->bodyPool);
     PoolDestroy(&w->shapePool);
     PoolDestroy(&w->hullPool);
@@ -1899,6 +2024,8 @@ void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
         DeviceFail("m3gpu_step");
     }
     w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
+    w->eventsState = 1;
+    w->eventThreshold = w->hitEventThreshold;
 }
 
 // Batched stepping of many worlds with one call: the rollback and toss
@@ -1926,6 +2053,8 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
         FillParams(w, dt, substeps, &params[n]);
         n += 1;
         w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
+        w->eventsState = 1;
+        w->eventThreshold = w->hitEventThreshold;
     }
     int rc = m3gpu_step_many(devs, params, n);
     if (rc < 0)
@@ -1970,6 +2099,9 @@ int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointD
         {
             return -1;
         }
+        // The run reuses the world's device pair buffers: build the last
+        // step's event streams first.
+        BuildEvents(w);
         for (int32_t j = 0; j < w->jointPool.maxIndex; ++j)
         {
             if (w->jointPool.alive[j])
@@ -2209,6 +2341,127 @@ int32_t m3World_ProfileEnd(m3WorldId worldId, struct m3gpuProfile* out)
 // ---------------------------------------------------------------------------
 // Hash (src/snapshot.c:932-1390, the branches live for this build's state)
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Event streams (src/world.c:960-1075, src/shape.c:1583-1593)
+// ---------------------------------------------------------------------------
+
+static const void* EventList(m3WorldId worldId, int32_t* count, int k)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        if (count != NULL)
+        {
+            *count = 0;
+        }
+        return NULL;
+    }
+    BuildEvents(w);
+    if (count != NULL)
+    {
+        *count = w->eventCounts[k];
+    }
+    switch (k)
+    {
+    case 0: return w->beginEvents;
+    case 1: return w->endEvents;
+    case 2: return w->sensorBeginEvents;
+    case 3: return w->sensorEndEvents;
+    case 4: return w->hitEvents;
+    default: return w->moveEvents;
+    }
+}
+
+const m3ContactEvent* m3World_ContactBeginEvents(m3WorldId worldId, int32_t* count)
+{
+    return (const m3ContactEvent*)EventList(worldId, count, 0);
+}
+
+const m3ContactEvent* m3World_ContactEndEvents(m3WorldId worldId, int32_t* count)
+{
+    return (const m3ContactEvent*)EventList(worldId, count, 1);
+}
+
+const m3ContactEvent* m3World_SensorBeginEvents(m3WorldId worldId, int32_t* count)
+{
+    return (const m3ContactEvent*)EventList(worldId, count, 2);
+}
+
+const m3ContactEvent* m3World_SensorEndEvents(m3WorldId worldId, int32_t* count)
+{
+    return (const m3ContactEvent*)EventList(worldId, count, 3);
+}
+
+const m3HitEvent* m3World_HitEvents(m3WorldId worldId, int32_t* count)
+{
+    return (const m3HitEvent*)EventList(worldId, count, 4);
+}
+
+// At most one hit per constraint and pairCapacity slots: never drops.
+int32_t m3World_HitEventsDropped(m3WorldId worldId)
+{
+    (void)worldId;
+    return 0;
+}
+
+const m3BodyMoveEvent* m3World_BodyMoveEvents(m3WorldId worldId, int32_t* count)
+{
+    return (const m3BodyMoveEvent*)EventList(worldId, count, 5);
+}
+
+// Joints never break in this build (no break limits): always empty.
+const m3JointBreakEvent* m3World_JointBreakEvents(m3WorldId worldId, int32_t* count)
+{
+    static m3JointBreakEvent none;
+    if (count != NULL)
+    {
+        *count = 0;
+    }
+    return WorldFromId(worldId) != NULL ? &none : NULL;
+}
+
+void m3World_SetHitEventThreshold(m3WorldId worldId, float value)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || !FiniteF(value) || value < 0.0f)
+    {
+        return;
+    }
+    w->hitEventThreshold = value; // the last step's streams keep the threshold they ran with
+}
+
+static m3World* ResolveShape(m3ShapeId id, int32_t* out)
+{
+    m3World* w = WorldFromIndex0(id.world0);
+    int32_t index = id.index1 - 1;
+    if (w == NULL || !PoolValid(&w->shapePool, index, id.generation))
+    {
+        *out = -1;
+        return NULL;
+    }
+    *out = index;
+    return w;
+}
+
+void m3Shape_EnableHitEvents(m3ShapeId shapeId, bool flag)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    if (w == NULL)
+    {
+        return;
+    }
+    Touch(w, M3GPU_FIELDS_STATIC);
+    w->shapeHitEvents[s] = flag ? 1 : 0;
+}
+
+bool m3Shape_AreHitEventsEnabled(m3ShapeId shapeId)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    return w != NULL && w->shapeHitEvents[s] != 0;
+}
 
 uint64_t m3World_Hash(m3WorldId worldId)
 {
@@ -2705,8 +2958,11 @@ bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size)
     {
         return false;
     }
-    // Validation passed: now write.  Pending device results are dropped.
+    // Validation passed: now write.  Pending device results are dropped, and
+    // so are the event streams (events are observations, not state).
     w->hostStale = 0;
+    w->eventsState = 0;
+    memset(w->eventCounts, 0, sizeof(w->eventCounts));
     memset(&k, 0, sizeof(k));
     k.in = (const uint8_t*)data + sizeof(hdr);
     k.limit = size - (int64_t)sizeof(hdr);

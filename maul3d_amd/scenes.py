@@ -517,6 +517,52 @@ def crowded_plank(api: Api, n: int = 7, seed: int = 0xC0FFEE) -> WorldId:
     return world
 
 
+def event_rain(api: Api, count: int = 400, seed: int = 0xE7E47, extent: float = 8.0) -> WorldId:
+    """A rain for the event streams: ``count`` bodies as in ``rain`` (boxes,
+    spheres and capsules by index, positions in [-extent,extent] x [2,30] x
+    [-extent,extent]) falling through four static sensor boxes (half-extents
+    2.5 x 0.75 x 2.5 centred at y = 9) onto a plane.  Every third body's
+    shape opts into hit events."""
+    cap = count + 64
+    world = _world(api, cap, cap)
+    _ground(api, world)
+    sens = api.m3DefaultShapeDef()
+    sens.isSensor = True
+    gd = api.m3DefaultBodyDef()
+    for cx, cz in ((-4.0, -4.0), (4.0, -4.0), (-4.0, 4.0), (4.0, 4.0)):
+        gd.position = Pos3(cx, 9.0, cz)
+        holder = api.m3CreateBody(world, C.byref(gd))
+        _check(api.m3CreateBoxShape(holder, C.byref(sens), Vec3(2.5, 0.75, 2.5)), "sensor box")
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    rng = SplitMix64(seed)
+    for k in range(count):
+        x = rng.range(-extent, extent)
+        y = rng.range(2.0, 30.0)
+        z = rng.range(-extent, extent)
+        qx, qy, qz, qw = rng.unit_quat()
+        bd.position = Pos3(x, y, z)
+        bd.rotation = Quat(qx, qy, qz, qw)
+        body = api.m3CreateBody(world, C.byref(bd))
+        sd.enableHitEvents = k % 3 == 0
+        kind = k % 5
+        if kind < 2:
+            he = Vec3(rng.range(0.25, 1.0), rng.range(0.25, 1.0), rng.range(0.25, 1.0))
+            _check(api.m3CreateBoxShape(body, C.byref(sd), he), "box")
+        elif kind < 4:
+            ball = Sphere(Vec3(0.0, 0.0, 0.0), rng.range(0.25, 0.75))
+            _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+        else:
+            r = rng.range(0.2, 0.5)
+            hh = rng.range(0.25, 1.0)
+            cap_ = Capsule(Vec3(-hh, 0.0, 0.0), Vec3(hh, 0.0, 0.0), r)
+            _check(api.m3CreateCapsuleShape(body, C.byref(sd), C.byref(cap_)), "capsule")
+    return world
+
+
 SCENES = {
     "bench_pyramid": bench_pyramid,
     "smoke10k": smoke10k,
@@ -526,4 +572,5 @@ SCENES = {
     "hull_chains": hull_chains,
     "hull_pile": hull_pile,
     "crowded_plank": crowded_plank,
+    "event_rain": event_rain,
 }

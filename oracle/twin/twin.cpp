@@ -173,6 +173,7 @@ struct m3gpuWorld
     std::vector<uint32_t> masks;
     int jointsPresent;
     int targetsPresent;
+    tw::m3HitRec* hitRecBuf;
 };
 
 static char s_err[256];
@@ -222,6 +223,8 @@ extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* c)
     v.restitution = A<R>(ns);
     v.rolling = A<R>(ns);
     v.sensor = A<uint8_t>(ns);
+    v.hitEvents = A<uint8_t>(ns);
+    w->hitRecBuf = A<tw::m3HitRec>(np);
     v.hasOffset = A<uint8_t>(ns);
     v.localPos = A<tw::m3Vec3>(ns);
     v.localRot = A<tw::m3Quat>(ns);
@@ -307,6 +310,7 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
         const ::m3Quat* lr = (const ::m3Quat*)h->shapeLocalRot;
         const ::m3Vec3* sv = (const ::m3Vec3*)h->shapeSurfaceVel;
         int pc = 0;
+        bool anyHit = false;
         for (int s = 0; s < ns; ++s)
         {
             v.shapeAlive[s] = h->shapeAlive[s];
@@ -317,6 +321,8 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
             v.restitution[s] = (R)h->shapeRestitution[s];
             v.rolling[s] = (R)h->shapeRollingResistance[s];
             v.sensor[s] = h->shapeSensor[s];
+            v.hitEvents[s] = h->shapeHitEvents[s];
+            anyHit = anyHit || (h->shapeAlive[s] != 0 && h->shapeHitEvents[s] != 0);
             v.hasOffset[s] = h->shapeHasOffset[s];
             v.localPos[s] = V(lp[s]);
             v.localRot[s] = Q(lr[s]);
@@ -332,6 +338,7 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
             }
         }
         v.planeCount = pc;
+        v.hitRecs = anyHit ? w->hitRecBuf : nullptr;
         const ::m3HullData* hulls = (const ::m3HullData*)h->hullData;
         for (int k = 0; k < h->maxHull; ++k)
         {
@@ -497,6 +504,103 @@ extern "C" int m3gpu_step_many(m3gpuWorld* const* ws, const m3gpuStepParams* ps,
         stalled += m3gpu_step(ws[k], &ps[k], nullptr) != 0;
     }
     return stalled;
+}
+
+// The event streams of the last step, by the device's rule (every begin
+// event a new pair touching now and not before, every end event an old pair
+// touching before and not now; hits in pair order; moves in mover order).
+static int32_t TwinFindKey(const uint64_t* keys, int32_t n, uint64_t key)
+{
+    const uint64_t* it = std::lower_bound(keys, keys + n, key);
+    return (it != keys + n && *it == key) ? (int32_t)(it - keys) : -1;
+}
+
+static int TwinPairCode(const tw::m3View& v, uint64_t key)
+{
+    const int32_t a = (int32_t)(key >> 32);
+    const int32_t b = (int32_t)(key & 0xFFFFFFFFu);
+    if (v.shapeAlive[a] == 0 || v.shapeAlive[b] == 0)
+    {
+        return 0;
+    }
+    return (v.sensor[a] != 0 || v.sensor[b] != 0) ? 2 : 1;
+}
+
+extern "C" int m3gpu_events(m3gpuWorld* w, float hitThreshold, m3gpuEvents* out)
+{
+    const tw::m3View& v = w->v;
+    const tw::m3Scalars* sc = v.sc;
+    memset(out->counts, 0, sizeof(out->counts));
+    if (sc->stall != 0)
+    {
+        out->counts[6] = 1;
+        return 0;
+    }
+    const int32_t np = sc->pairCount;
+    const int32_t no = sc->oldPairCount;
+    for (int32_t i = 0; i < np; ++i)
+    {
+        const uint64_t key = v.keys[i];
+        const int32_t j = TwinFindKey(v.oldKeys, no, key);
+        const bool touchNew = v.manifolds[i].pointCount > 0;
+        const bool touchOld = j >= 0 && v.oldManifolds[j].pointCount > 0;
+        const int code = (touchNew && !touchOld) ? TwinPairCode(v, key) : 0;
+        if (code == 1)
+        {
+            out->beginKeys[out->counts[0]++] = key;
+        }
+        else if (code == 2)
+        {
+            out->sensorBeginKeys[out->counts[2]++] = key;
+        }
+        if (v.hitRecs != nullptr && v.hitRecs[i].flag != 0 &&
+            v.hitRecs[i].relativeVelocity < -(R)hitThreshold)
+        {
+            const tw::m3HitRec& r = v.hitRecs[i];
+            m3gpuHit& hit = out->hits[out->counts[4]++];
+            hit.key = key;
+            hit.point[0] = r.point[0];
+            hit.point[1] = r.point[1];
+            hit.point[2] = r.point[2];
+            hit.normal[0] = (float)r.normal.x;
+            hit.normal[1] = (float)r.normal.y;
+            hit.normal[2] = (float)r.normal.z;
+            hit.approachSpeed = (float)-r.relativeVelocity;
+        }
+    }
+    for (int32_t i = 0; i < no; ++i)
+    {
+        const uint64_t key = v.oldKeys[i];
+        const int32_t j = TwinFindKey(v.keys, np, key);
+        const bool touchOld = v.oldManifolds[i].pointCount > 0;
+        const bool touchNew = j >= 0 && v.manifolds[j].pointCount > 0;
+        const int code = (touchOld && !touchNew) ? TwinPairCode(v, key) : 0;
+        if (code == 1)
+        {
+            out->endKeys[out->counts[1]++] = key;
+        }
+        else if (code == 2)
+        {
+            out->sensorEndKeys[out->counts[3]++] = key;
+        }
+    }
+    for (int32_t m = 0; m < sc->moverCount; ++m)
+    {
+        const int32_t b = v.movers[m];
+        m3gpuMove& mv = out->moves[m];
+        memset(&mv, 0, sizeof(mv));
+        mv.body = b;
+        mv.fellAsleep = v.type[b] == M3_DYNAMIC && v.awake[b] == 0;
+        mv.p[0] = v.xf[b].p.x;
+        mv.p[1] = v.xf[b].p.y;
+        mv.p[2] = v.xf[b].p.z;
+        mv.q[0] = (float)v.xf[b].q.x;
+        mv.q[1] = (float)v.xf[b].q.y;
+        mv.q[2] = (float)v.xf[b].q.z;
+        mv.q[3] = (float)v.xf[b].q.w;
+    }
+    out->counts[5] = sc->moverCount;
+    return 0;
 }
 
 extern "C" int m3gpu_sync(m3gpuWorld*) { return 0; }

@@ -5,7 +5,9 @@ reference's own C sources into oracle/_ref/libmaul3d_ref.so).  For each
 scene of maul3d_amd.scenes it records the world hash before stepping and
 after selected steps of m3World_Step(1/60, 4), plus the counters the tests
 use to size their checks.  The output, hashes.json, is data: inputs are the
-seeded scene definitions, outputs the reference's hashes.
+seeded scene definitions, outputs the reference's hashes.  ``events``
+writes events.json: per step of the event rain, the hash and a digest of
+each event stream.
 """
 
 from __future__ import annotations
@@ -91,6 +93,26 @@ def batch_rollback_fixture(ref: api.Api, worlds: int, warm: int, resim: int) -> 
     return out
 
 
+def events_fixture(ref: api.Api) -> dict:
+    """Per step of the event rain: the world hash and the digest (count and
+    SHA-256 prefix) of every event stream, with the scene's mid-run changes
+    (tests/helpers.py EventActions)."""
+    from tests.helpers import EVENT_STEPS, EventActions, event_digest
+
+    world = scenes.event_rain(ref)
+    acts = EventActions(ref)
+    out = {"steps": EVENT_STEPS, "init": "%016x" % ref.m3World_Hash(world), "per_step": {}}
+    for i in range(1, EVENT_STEPS + 1):
+        ref.m3World_Step(world, 1.0 / 60.0, 4)
+        out["per_step"][str(i)] = {
+            "hash": "%016x" % ref.m3World_Hash(world),
+            "events": event_digest(ref.events(world)),
+        }
+        acts.after_step(world, i)
+    ref.m3DestroyWorld(world)
+    return out
+
+
 def main() -> None:
     ref = api.Api(C.CDLL(REF))
     if len(sys.argv) > 2 and sys.argv[1] == "scene":
@@ -104,6 +126,12 @@ def main() -> None:
         with open(path, "w") as f:
             json.dump(data, f, indent=1, sort_keys=True)
         print("updated", name)
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == "events":
+        data = events_fixture(ref)
+        with open(os.path.join(HERE, "events.json"), "w") as f:
+            json.dump(data, f, indent=0, sort_keys=True)
+        print("wrote events.json")
         return
     if len(sys.argv) > 1 and sys.argv[1] == "batch":
         data = batch_rollback_fixture(ref, 256, 100, 8)

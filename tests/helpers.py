@@ -59,3 +59,44 @@ def product_api():
 
 def hexh(h: int) -> str:
     return "%016x" % h
+
+
+EVENT_STEPS = 360
+
+
+def event_digest(ev: dict) -> dict:
+    """Count and a SHA-256 prefix of each event stream (Api.events output)."""
+    import hashlib
+
+    out = {}
+    for name, items in sorted(ev.items()):
+        if not isinstance(items, list):
+            out[name] = items
+            continue
+        out[name] = [len(items), hashlib.sha256(repr(items).encode()).hexdigest()[:16]]
+    return out
+
+
+class EventActions:
+    """Mid-run changes of the event scene, applied after the given steps: the
+    world's hit threshold raised to 4 m/s after step 120, and the hit opt-in
+    of the shapes named by step 1's first five contact begin events flipped
+    after step 200."""
+
+    def __init__(self, api):
+        self.api = api
+        self.flip = []
+
+    def after_step(self, world, step: int) -> None:
+        api = self.api
+        if step == 1:
+            n = C.c_int32(0)
+            ptr = api.m3World_ContactBeginEvents(world, C.byref(n))
+            self.flip = [ptr[i].shapeB for i in range(min(5, n.value))]
+            for sid in self.flip:
+                assert api.m3Shape_AreHitEventsEnabled(sid) in (True, False)
+        if step == 120:
+            api.m3World_SetHitEventThreshold(world, 4.0)
+        if step == 200:
+            for sid in self.flip:
+                api.m3Shape_EnableHitEvents(sid, not api.m3Shape_AreHitEventsEnabled(sid))
