@@ -286,6 +286,17 @@ M3_API void m3World_SetHitEventThreshold(m3WorldId worldId, float value);       
 M3_API const m3BodyMoveEvent* m3World_BodyMoveEvents(m3WorldId worldId, int32_t* count);    // world.h:302
 M3_API const m3JointBreakEvent* m3World_JointBreakEvents(m3WorldId worldId, int32_t* count); // world.h:313
 M3_API void m3Shape_EnableHitEvents(m3ShapeId shapeId, bool flag);                          // shape.h:300
+// ---- journal (include/maul3d/world.h:600-606) ------------------------------
+// The reference's record stream (op, byte count, payload; op codes and record
+// layouts of src/world_internal.h:108-192, 543-581), for the calls this build
+// supports: step, body/shape/hull/joint creation, joint destruction, velocity
+// setters, forces and torques, gravity, the sleep and continuous toggles, the
+// hit threshold and shape hit opt-in.  Replay is atomic: a journal holding
+// any other op, or whose ids do not come out as recorded, is refused and the
+// world is left as it was.
+M3_API bool m3World_JournalBegin(m3WorldId worldId, void* buffer, int32_t capacity);       // world.h:600
+M3_API int32_t m3World_JournalEnd(m3WorldId worldId);                                      // world.h:601
+M3_API bool m3World_JournalReplay(m3WorldId worldId, const void* data, int32_t size);      // world.h:606
 M3_API bool m3Shape_AreHitEventsEnabled(m3ShapeId shapeId);                                 // shape.h:301
 
 // Extensions of this build (no reference counterpart): step many worlds with

@@ -350,6 +350,9 @@ EVENT_STREAMS = {
 for _name, _rec in EVENT_STREAMS.items():
     _SIGNATURES[_name] = (C.POINTER(_rec), [WorldId, C.POINTER(c_i32)])
 _SIGNATURES["m3World_HitEventsDropped"] = (c_i32, [WorldId])
+_SIGNATURES["m3World_JournalBegin"] = (C.c_bool, [WorldId, C.c_void_p, c_i32])
+_SIGNATURES["m3World_JournalEnd"] = (c_i32, [WorldId])
+_SIGNATURES["m3World_JournalReplay"] = (C.c_bool, [WorldId, C.c_void_p, c_i32])
 _SIGNATURES["m3World_SetHitEventThreshold"] = (None, [WorldId, c_f])
 _SIGNATURES["m3Shape_EnableHitEvents"] = (None, [ShapeId, C.c_bool])
 _SIGNATURES["m3Shape_AreHitEventsEnabled"] = (C.c_bool, [ShapeId])

@@ -260,6 +260,14 @@ typedef struct m3World
     uint32_t devDirty;  // field groups the host changed since the last upload
     uint32_t hostStale; // field groups the device changed since the last download
 
+    // journal (src/world.c:1348-1400): op, byte count and payload records
+    // into the caller's buffer
+    uint8_t* journalBuffer;
+    int32_t journalCapacity;
+    int32_t journalCursor;
+    uint8_t journalActive;
+    uint8_t journalOverflow;
+
     // event streams of the last step: 0 none (creation, restore), 1 to be
     // built on the device, 2 built into the arrays below
     int32_t eventsState;
@@ -510,6 +518,78 @@ static void SyncHost(m3World* w)
         DeviceFail("m3gpu_download");
     }
     w->hostStale = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Journal (src/world.c:1348-1400): every mutating call of the supported
+// surface appends (op, byte count, payload) to the caller's buffer, with the
+// reference's op codes (src/world_internal.h:108-192) and record layouts, so
+// a journal written by either library replays on the other.
+// ---------------------------------------------------------------------------
+
+enum
+{
+    M3_OP_STEP = 1,
+    M3_OP_CREATE_BODY = 2,
+    M3_OP_SET_LINEAR_VELOCITY = 4,
+    M3_OP_SET_ANGULAR_VELOCITY = 5,
+    M3_OP_CREATE_SHAPE = 6,
+    M3_OP_CREATE_HULL_SHAPE = 7,
+    M3_OP_CREATE_JOINT = 9,
+    M3_OP_DESTROY_JOINT = 10,
+    M3_OP_APPLY_FORCE = 26,
+    M3_OP_APPLY_TORQUE = 27,
+    M3_OP_SET_GRAVITY = 39,
+    M3_OP_ENABLE_SLEEPING = 47,
+    M3_OP_ENABLE_CONTINUOUS = 48,
+    M3_OP_SET_HIT_EVENT_THRESHOLD = 49,
+    M3_OP_ENABLE_SHAPE_HIT_EVENTS = 50,
+};
+
+#define M3_HULL_MAX_INPUT 256
+
+typedef struct m3OpStep { float dt; int32_t substeps; } m3OpStep;
+typedef struct m3OpCreateBody { m3BodyDef def; m3BodyId expected; } m3OpCreateBody;
+typedef struct m3OpBodyVec { m3BodyId id; m3Vec3 v; } m3OpBodyVec;
+typedef struct m3OpCreateShape // src/world_internal.h:543-551
+{
+    m3ShapeDef def;
+    m3ShapeGeom geom;
+    m3BodyId body;
+    m3ShapeId expected;
+    uint8_t type;
+    uint8_t pad[7];
+} m3OpCreateShape;
+typedef struct m3OpCreateHullShape // src/world_internal.h:574-581
+{
+    m3ShapeDef def;
+    m3BodyId body;
+    m3ShapeId expected;
+    int32_t count;
+    m3Vec3 points[M3_HULL_MAX_INPUT];
+} m3OpCreateHullShape;
+typedef struct m3OpCreateJoint { m3JointDef def; m3JointId expected; } m3OpCreateJoint;
+typedef struct m3OpShapeFlag { m3ShapeId id; int32_t on; } m3OpShapeFlag;
+
+static void JournalRecord(m3World* w, int32_t op, const void* payload, int32_t bytes)
+{
+    if (w->journalActive == 0)
+    {
+        return;
+    }
+    const int32_t need = 8 + bytes;
+    if (w->journalCursor + need > w->journalCapacity)
+    {
+        // Overflow latches and stops recording; JournalEnd reports -1.
+        w->journalOverflow = 1;
+        w->journalActive = 0;
+        return;
+    }
+    uint8_t* out = w->journalBuffer + w->journalCursor;
+    memcpy(out, &op, 4);
+    memcpy(out + 4, &bytes, 4);
+    memcpy(out + 8, payload, (size_t)bytes);
+    w->journalCursor += need;
 }
 
 // Event streams (src/solver.c:3732-3815, 2991-3026, 4295-4306): the device
@@ -887,6 +967,7 @@ void m3World_SetGravity(m3WorldId worldId, m3Vec3 gravity)
     {
         return;
     }
+    JournalRecord(w, M3_OP_SET_GRAVITY, &gravity, (int32_t)sizeof(gravity));
     w->gravity = gravity;
 }
 
@@ -901,6 +982,8 @@ void m3World_EnableContinuous(m3WorldId worldId, bool flag)
     m3World* w = WorldFromId(worldId);
     if (w != NULL)
     {
+        int32_t on = flag ? 1 : 0;
+        JournalRecord(w, M3_OP_ENABLE_CONTINUOUS, &on, (int32_t)sizeof(on));
         w->continuousEnabled = flag ? 1 : 0;
     }
 }
@@ -912,6 +995,8 @@ void m3World_EnableSleeping(m3WorldId worldId, bool flag)
     {
         return;
     }
+    int32_t on = flag ? 1 : 0;
+    JournalRecord(w, M3_OP_ENABLE_SLEEPING, &on, (int32_t)sizeof(on));
     w->sleepEnabled = flag ? 1 : 0;
     if (!flag)
     {
@@ -1016,6 +1101,14 @@ m3BodyId m3CreateBody(m3WorldId worldId, const m3BodyDef* def)
     w->bodyTarget[i].q.w = 1.0f;
     w->bodyShapeHead[i] = -1;
     m3BodyId id = {i + 1, w->worldIndex0, w->bodyPool.generations[i]};
+    if (w->journalActive != 0)
+    {
+        m3OpCreateBody rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.def = *def;
+        rec.expected = id;
+        JournalRecord(w, M3_OP_CREATE_BODY, &rec, (int32_t)sizeof(rec));
+    }
     return id;
 }
 
@@ -1083,6 +1176,14 @@ void m3Body_SetLinearVelocity(m3BodyId bodyId, m3Vec3 velocity)
     {
         return;
     }
+    if (w->journalActive != 0)
+    {
+        m3OpBodyVec rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.id = bodyId;
+        rec.v = velocity;
+        JournalRecord(w, M3_OP_SET_LINEAR_VELOCITY, &rec, (int32_t)sizeof(rec));
+    }
     Touch(w, M3GPU_FIELDS_DYNAMIC);
     w->awake[i] = 1;
     w->sleepTimes[i] = 0.0f;
@@ -1096,6 +1197,14 @@ void m3Body_SetAngularVelocity(m3BodyId bodyId, m3Vec3 velocity)
     if (w == NULL || !FiniteV3(velocity))
     {
         return;
+    }
+    if (w->journalActive != 0)
+    {
+        m3OpBodyVec rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.id = bodyId;
+        rec.v = velocity;
+        JournalRecord(w, M3_OP_SET_ANGULAR_VELOCITY, &rec, (int32_t)sizeof(rec));
     }
     Touch(w, M3GPU_FIELDS_DYNAMIC);
     w->awake[i] = 1;
@@ -1128,6 +1237,14 @@ static void ApplyForceLike(m3BodyId bodyId, m3Vec3 value, int torque)
     if (!(w->types[i] == M3_DYNAMIC && w->invMass[i] > 0.0f))
     {
         return;
+    }
+    if (w->journalActive != 0)
+    {
+        m3OpBodyVec rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.id = bodyId;
+        rec.v = value;
+        JournalRecord(w, torque ? M3_OP_APPLY_TORQUE : M3_OP_APPLY_FORCE, &rec, (int32_t)sizeof(rec));
     }
     Touch(w, M3GPU_FIELDS_DYNAMIC);
     if (torque)
@@ -1616,6 +1733,18 @@ static m3ShapeId CreateShape(m3BodyId bodyId, const m3ShapeDef* def, uint8_t typ
     }
     RecomputeMass(w, b);
     m3ShapeId id = {s + 1, w->worldIndex0, w->shapePool.generations[s]};
+    if (hull == NULL && w->journalActive != 0)
+    {
+        // Point-cloud hulls journal their input points instead.
+        m3OpCreateShape rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.def = *def;
+        rec.geom = *geom;
+        rec.body = bodyId;
+        rec.expected = id;
+        rec.type = type;
+        JournalRecord(w, M3_OP_CREATE_SHAPE, &rec, (int32_t)sizeof(rec));
+    }
     return id;
 }
 
@@ -1722,7 +1851,25 @@ m3ShapeId m3CreateHullShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Vec3
     }
     m3ShapeGeom geom;
     memset(&geom, 0, sizeof(geom));
-    return CreateShape(bodyId, def, M3_HULL, &geom, &data);
+    m3ShapeId id = CreateShape(bodyId, def, M3_HULL, &geom, &data);
+    if (id.index1 != 0 && w->journalActive != 0 && count <= M3_HULL_MAX_INPUT)
+    {
+        m3OpCreateHullShape* rec = (m3OpCreateHullShape*)calloc(1, sizeof(m3OpCreateHullShape));
+        if (rec == NULL)
+        {
+            w->journalOverflow = 1; // cannot record: the journal is invalid
+            w->journalActive = 0;
+            return id;
+        }
+        rec->def = *def;
+        rec->body = bodyId;
+        rec->expected = id;
+        rec->count = count;
+        memcpy(rec->points, points, (size_t)count * sizeof(m3Vec3));
+        JournalRecord(w, M3_OP_CREATE_HULL_SHAPE, rec, (int32_t)sizeof(*rec));
+        free(rec);
+    }
+    return id;
 }
 
 // ---------------------------------------------------------------------------
@@ -1881,6 +2028,14 @@ m3JointId m3CreateJoint(const m3JointDef* def)
         w->sleepTimes[b] = 0.0f;
     }
     m3JointId id = {index + 1, w->worldIndex0, w->jointPool.generations[index]};
+    if (w->journalActive != 0)
+    {
+        m3OpCreateJoint rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.def = *def;
+        rec.expected = id;
+        JournalRecord(w, M3_OP_CREATE_JOINT, &rec, (int32_t)sizeof(rec));
+    }
     return id;
 }
 
@@ -1907,6 +2062,7 @@ void m3DestroyJoint(m3JointId jointId)
     {
         return;
     }
+    JournalRecord(w, M3_OP_DESTROY_JOINT, &jointId, (int32_t)sizeof(jointId));
     Touch(w, M3GPU_FIELDS_STATIC | M3GPU_FIELDS_DYNAMIC);
     int32_t a = w->jointBodyA[index];
     int32_t b = w->jointBodyB[index];
@@ -2026,6 +2182,15 @@ void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
     w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
     w->eventsState = 1;
     w->eventThreshold = w->hitEventThreshold;
+    // Recorded after the step, as the reference (src/solver.c:4332-4354).
+    if (w->journalActive != 0)
+    {
+        m3OpStep rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.dt = dt;
+        rec.substeps = substeps;
+        JournalRecord(w, M3_OP_STEP, &rec, (int32_t)sizeof(rec));
+    }
 }
 
 // Batched stepping of many worlds with one call: the rollback and toss
@@ -2060,6 +2225,18 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
     if (rc < 0)
     {
         DeviceFail("m3gpu_step_many");
+    }
+    m3OpStep rec;
+    memset(&rec, 0, sizeof(rec));
+    rec.dt = dt;
+    rec.substeps = substeps;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3World* w = WorldFromId(worlds[k]);
+        if (w != NULL)
+        {
+            JournalRecord(w, M3_OP_STEP, &rec, (int32_t)sizeof(rec));
+        }
     }
     return n;
 }
@@ -2428,6 +2605,7 @@ void m3World_SetHitEventThreshold(m3WorldId worldId, float value)
     {
         return;
     }
+    JournalRecord(w, M3_OP_SET_HIT_EVENT_THRESHOLD, &value, (int32_t)sizeof(value));
     w->hitEventThreshold = value; // the last step's streams keep the threshold they ran with
 }
 
@@ -2452,6 +2630,14 @@ void m3Shape_EnableHitEvents(m3ShapeId shapeId, bool flag)
     {
         return;
     }
+    if (w->journalActive != 0)
+    {
+        m3OpShapeFlag rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.id = shapeId;
+        rec.on = flag ? 1 : 0;
+        JournalRecord(w, M3_OP_ENABLE_SHAPE_HIT_EVENTS, &rec, (int32_t)sizeof(rec));
+    }
     Touch(w, M3GPU_FIELDS_STATIC);
     w->shapeHitEvents[s] = flag ? 1 : 0;
 }
@@ -2461,6 +2647,319 @@ bool m3Shape_AreHitEventsEnabled(m3ShapeId shapeId)
     int32_t s;
     m3World* w = ResolveShape(shapeId, &s);
     return w != NULL && w->shapeHitEvents[s] != 0;
+}
+
+bool m3World_JournalBegin(m3WorldId worldId, void* buffer, int32_t capacity)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || buffer == NULL || capacity < 8 || w->journalActive != 0)
+    {
+        return false;
+    }
+    w->journalBuffer = (uint8_t*)buffer;
+    w->journalCapacity = capacity;
+    w->journalCursor = 0;
+    w->journalActive = 1;
+    w->journalOverflow = 0;
+    return true;
+}
+
+int32_t m3World_JournalEnd(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return -1;
+    }
+    int32_t bytes = w->journalOverflow != 0 ? -1 : w->journalCursor;
+    w->journalBuffer = NULL;
+    w->journalCapacity = 0;
+    w->journalCursor = 0;
+    w->journalActive = 0;
+    w->journalOverflow = 0;
+    return bytes;
+}
+
+// Booleans read from a journal are normalised to 0/1 (src/world.c:1490-1503).
+static void NormalizeBool(void* record, size_t offset)
+{
+    uint8_t* b = (uint8_t*)record + offset;
+    *b = *b != 0 ? 1 : 0;
+}
+
+// Applies one journal's records through the public entry points with ids
+// retargeted to this world; any record this build cannot apply (an unknown
+// or unsupported op, a wrong size, an id that does not come out as
+// recorded) fails the whole replay.
+static bool JournalApply(m3World* w, m3WorldId worldId, const uint8_t* stream, int32_t size)
+{
+    int32_t cursor = 0;
+    while (cursor < size)
+    {
+        if (cursor + 8 > size)
+        {
+            return false;
+        }
+        int32_t op;
+        int32_t bytes;
+        memcpy(&op, stream + cursor, 4);
+        memcpy(&bytes, stream + cursor + 4, 4);
+        cursor += 8;
+        if (bytes < 0 || bytes > size - cursor)
+        {
+            return false;
+        }
+        const uint8_t* payload = stream + cursor;
+        cursor += bytes;
+        switch (op)
+        {
+        case M3_OP_STEP:
+        {
+            m3OpStep rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            m3World_Step(worldId, rec.dt, rec.substeps);
+            break;
+        }
+        case M3_OP_CREATE_BODY:
+        {
+            m3OpCreateBody rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            NormalizeBool(&rec.def, offsetof(m3BodyDef, isBullet));
+            m3BodyId id = m3CreateBody(worldId, &rec.def);
+            if (id.index1 != rec.expected.index1 || id.generation != rec.expected.generation)
+            {
+                return false;
+            }
+            break;
+        }
+        case M3_OP_SET_LINEAR_VELOCITY:
+        case M3_OP_SET_ANGULAR_VELOCITY:
+        case M3_OP_APPLY_FORCE:
+        case M3_OP_APPLY_TORQUE:
+        {
+            m3OpBodyVec rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            rec.id.world0 = w->worldIndex0;
+            if (BodySlot(w, rec.id) < 0)
+            {
+                return false;
+            }
+            if (op == M3_OP_SET_LINEAR_VELOCITY)
+            {
+                m3Body_SetLinearVelocity(rec.id, rec.v);
+            }
+            else if (op == M3_OP_SET_ANGULAR_VELOCITY)
+            {
+                m3Body_SetAngularVelocity(rec.id, rec.v);
+            }
+            else
+            {
+                ApplyForceLike(rec.id, rec.v, op == M3_OP_APPLY_TORQUE);
+            }
+            break;
+        }
+        case M3_OP_CREATE_SHAPE:
+        {
+            m3OpCreateShape rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            rec.body.world0 = w->worldIndex0;
+            NormalizeBool(&rec.def, offsetof(m3ShapeDef, isSensor));
+            NormalizeBool(&rec.def, offsetof(m3ShapeDef, enableHitEvents));
+            NormalizeBool(&rec.def, offsetof(m3ShapeDef, enablePreSolveEvents));
+            if (rec.type != M3_SPHERE && rec.type != M3_PLANE && rec.type != M3_CAPSULE &&
+                rec.type != M3_HULL)
+            {
+                return false;
+            }
+            m3ShapeId id = CreateShape(rec.body, &rec.def, rec.type, &rec.geom, NULL);
+            if (id.index1 != rec.expected.index1 || id.generation != rec.expected.generation)
+            {
+                return false;
+            }
+            break;
+        }
+        case M3_OP_CREATE_HULL_SHAPE:
+        {
+            m3OpCreateHullShape* rec = (m3OpCreateHullShape*)malloc(sizeof(m3OpCreateHullShape));
+            if (rec == NULL || bytes != (int32_t)sizeof(*rec))
+            {
+                free(rec);
+                return false;
+            }
+            memcpy(rec, payload, sizeof(*rec));
+            rec->body.world0 = w->worldIndex0;
+            NormalizeBool(&rec->def, offsetof(m3ShapeDef, isSensor));
+            NormalizeBool(&rec->def, offsetof(m3ShapeDef, enableHitEvents));
+            NormalizeBool(&rec->def, offsetof(m3ShapeDef, enablePreSolveEvents));
+            m3ShapeId id = {0, 0, 0};
+            if (rec->count >= 1 && rec->count <= M3_HULL_MAX_INPUT)
+            {
+                id = m3CreateHullShape(rec->body, &rec->def, rec->points, rec->count);
+            }
+            bool ok = id.index1 == rec->expected.index1 && id.index1 != 0 &&
+                      id.generation == rec->expected.generation;
+            free(rec);
+            if (!ok)
+            {
+                return false;
+            }
+            break;
+        }
+        case M3_OP_CREATE_JOINT:
+        {
+            m3OpCreateJoint rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            NormalizeBool(&rec.def, offsetof(m3JointDef, enableLimit));
+            NormalizeBool(&rec.def, offsetof(m3JointDef, enableMotor));
+            NormalizeBool(&rec.def, offsetof(m3JointDef, enableCone));
+            NormalizeBool(&rec.def, offsetof(m3JointDef, collideConnected));
+            rec.def.bodyA.world0 = w->worldIndex0;
+            rec.def.bodyB.world0 = w->worldIndex0;
+            m3JointId id = m3CreateJoint(&rec.def);
+            if (id.index1 != rec.expected.index1 || id.generation != rec.expected.generation)
+            {
+                return false;
+            }
+            break;
+        }
+        case M3_OP_DESTROY_JOINT:
+        {
+            m3JointId id;
+            if (bytes != (int32_t)sizeof(id))
+            {
+                return false;
+            }
+            memcpy(&id, payload, sizeof(id));
+            id.world0 = w->worldIndex0;
+            int32_t index;
+            if (ResolveJoint(id, &index) != w)
+            {
+                return false;
+            }
+            m3DestroyJoint(id);
+            break;
+        }
+        case M3_OP_SET_GRAVITY:
+        {
+            m3Vec3 g;
+            if (bytes != (int32_t)sizeof(g))
+            {
+                return false;
+            }
+            memcpy(&g, payload, sizeof(g));
+            if (!FiniteV3(g))
+            {
+                return false;
+            }
+            m3World_SetGravity(worldId, g);
+            break;
+        }
+        case M3_OP_ENABLE_SLEEPING:
+        case M3_OP_ENABLE_CONTINUOUS:
+        {
+            int32_t on;
+            if (bytes != (int32_t)sizeof(on))
+            {
+                return false;
+            }
+            memcpy(&on, payload, sizeof(on));
+            if (op == M3_OP_ENABLE_SLEEPING)
+            {
+                m3World_EnableSleeping(worldId, on != 0);
+            }
+            else
+            {
+                m3World_EnableContinuous(worldId, on != 0);
+            }
+            break;
+        }
+        case M3_OP_SET_HIT_EVENT_THRESHOLD:
+        {
+            float value;
+            if (bytes != (int32_t)sizeof(value))
+            {
+                return false;
+            }
+            memcpy(&value, payload, sizeof(value));
+            if (!FiniteF(value) || value < 0.0f)
+            {
+                return false;
+            }
+            m3World_SetHitEventThreshold(worldId, value);
+            break;
+        }
+        case M3_OP_ENABLE_SHAPE_HIT_EVENTS:
+        {
+            m3OpShapeFlag rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            rec.id.world0 = w->worldIndex0;
+            int32_t sIndex;
+            if (ResolveShape(rec.id, &sIndex) != w)
+            {
+                return false;
+            }
+            m3Shape_EnableHitEvents(rec.id, rec.on != 0);
+            break;
+        }
+        default:
+            return false; // an op outside this build's surface
+        }
+    }
+    return true;
+}
+
+// src/world.c:3187-3220: atomic; a refused journal leaves the world as it was
+// (a snapshot taken first is restored).
+bool m3World_JournalReplay(m3WorldId worldId, const void* data, int32_t size)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || data == NULL || size < 0)
+    {
+        return false;
+    }
+    int32_t snapBytes = m3World_SnapshotSize(worldId);
+    void* snap = snapBytes > 0 ? malloc((size_t)snapBytes) : NULL;
+    if (snap == NULL || m3World_Snapshot(worldId, snap, snapBytes) != snapBytes)
+    {
+        free(snap);
+        return false;
+    }
+    // Replay does not record into a journal that is being written.
+    const uint8_t active = w->journalActive;
+    w->journalActive = 0;
+    bool ok = JournalApply(w, worldId, (const uint8_t*)data, size);
+    w->journalActive = active;
+    if (!ok)
+    {
+        bool restored = m3World_Restore(worldId, snap, snapBytes);
+        (void)restored;
+    }
+    free(snap);
+    return ok;
 }
 
 uint64_t m3World_Hash(m3WorldId worldId)

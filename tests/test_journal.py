@@ -1,0 +1,156 @@
+"""The journal (include/maul3d/world.h:600-606; records src/world.c:1348-1400,
+op codes and layouts src/world_internal.h:108-192, 543-581; replay
+src/world.c:1505-3220): a session recorded by one library replays on the
+other and lands on the same world hash, and a refused journal leaves the
+world untouched.
+
+Cross replay is the layout check: every record's size is verified on
+replay, every created id must come out as recorded, and the final hash
+covers every body, shape, joint and pair.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import pytest
+
+from tests.helpers import DT, ROOT, SUBSTEPS, hexh, product_api, ref_api
+
+from maul3d_amd import api, scenes
+
+TWIN32 = os.path.join(ROOT, "oracle", "_ref", "libm3twin32.so")
+
+
+def _session(m3, world, joints: bool) -> None:
+    """A session over the supported surface, on a box-pyramid world (the
+    joint records only where the stepping library has joints: the CPU twin
+    does not)."""
+    sd = m3.m3DefaultShapeDef()
+    sd.density = 800.0
+    sd.friction = 0.5
+    bd = m3.m3DefaultBodyDef()
+    bd.type = scenes.DYNAMIC
+    bodies = []
+    for k in range(4):
+        bd.position = api.Pos3(-6.0 + 3.0 * k, 14.0, 2.0)
+        b = m3.m3CreateBody(world, C.byref(bd))
+        bodies.append(b)
+    assert m3.m3CreateSphereShape(bodies[0], C.byref(sd), C.byref(scenes.Sphere(api.Vec3(0, 0, 0), 0.5))).index1
+    assert m3.m3CreateBoxShape(bodies[1], C.byref(sd), api.Vec3(0.4, 0.3, 0.6)).index1
+    cap = scenes.Capsule(api.Vec3(-0.5, 0, 0), api.Vec3(0.5, 0, 0), 0.3)
+    assert m3.m3CreateCapsuleShape(bodies[2], C.byref(sd), C.byref(cap)).index1
+    rng = scenes.SplitMix64(0x10C4)
+    pts = (api.Vec3 * 24)(*[api.Vec3(rng.range(-0.6, 0.6), rng.range(-0.6, 0.6), rng.range(-0.6, 0.6)) for _ in range(24)])
+    hull = m3.m3CreateHullShape(bodies[3], C.byref(sd), pts, 24)
+    assert hull.index1
+    joint = None
+    if joints:
+        jd = m3.m3DefaultJointDef()
+        jd.bodyA = bodies[0]
+        jd.bodyB = bodies[1]
+        jd.localAnchorA = api.Vec3(1.5, 0.0, 0.0)
+        jd.localAnchorB = api.Vec3(-1.5, 0.0, 0.0)
+        joint = m3.m3CreateJoint(C.byref(jd))
+        assert joint.index1
+    m3.m3Body_SetLinearVelocity(bodies[2], api.Vec3(1.0, 2.0, -0.5))
+    m3.m3Body_SetAngularVelocity(bodies[3], api.Vec3(0.0, 3.0, 1.0))
+    m3.m3World_SetHitEventThreshold(world, 2.0)
+    m3.m3Shape_EnableHitEvents(hull, True)
+    for step in range(40):
+        m3.m3Body_ApplyForce(bodies[1], api.Vec3(0.0, 0.0, 40.0))
+        m3.m3Body_ApplyTorque(bodies[3], api.Vec3(0.0, 5.0, 0.0))
+        m3.m3World_Step(world, DT, SUBSTEPS)
+        if step == 10:
+            m3.m3World_SetGravity(world, api.Vec3(0.5, -9.0, 0.0))
+        if step == 20:
+            m3.m3World_EnableSleeping(world, False)
+            m3.m3World_EnableContinuous(world, True)
+        if step == 30 and joint is not None:
+            m3.m3DestroyJoint(joint)
+
+
+def _record(m3, joints: bool) -> tuple[bytes, str]:
+    world = scenes.box_pyramid(m3, rows=6)
+    for _ in range(5):
+        m3.m3World_Step(world, DT, SUBSTEPS)
+    buf = C.create_string_buffer(1 << 20)
+    assert m3.m3World_JournalBegin(world, buf, len(buf))
+    _session(m3, world, joints)
+    n = m3.m3World_JournalEnd(world)
+    assert n > 0
+    h = hexh(m3.m3World_Hash(world))
+    m3.m3DestroyWorld(world)
+    return buf.raw[:n], h
+
+
+def _replay(m3, journal: bytes) -> str:
+    world = scenes.box_pyramid(m3, rows=6)
+    for _ in range(5):
+        m3.m3World_Step(world, DT, SUBSTEPS)
+    data = C.create_string_buffer(journal, len(journal))
+    assert m3.m3World_JournalReplay(world, data, len(journal))
+    h = hexh(m3.m3World_Hash(world))
+    m3.m3DestroyWorld(world)
+    return h
+
+
+def _ops(journal: bytes) -> list[tuple[int, int]]:
+    out = []
+    i = 0
+    while i < len(journal):
+        op = int.from_bytes(journal[i : i + 4], "little")
+        n = int.from_bytes(journal[i + 4 : i + 8], "little")
+        out.append((op, n))
+        i += 8 + n
+    return out
+
+
+def _cross(other, joints: bool) -> None:
+    ref = ref_api()
+    jr, hr = _record(ref, joints)
+    jo, ho = _record(other, joints)
+    assert _ops(jo) == _ops(jr)  # same op codes and record sizes, in order
+    assert len(jo) == len(jr)
+    assert ho == hr
+    assert _replay(other, jr) == hr  # the reference's journal on this build
+    assert _replay(ref, jo) == hr  # this build's journal on the reference
+
+
+def _refused_replay_leaves_world(m3) -> None:
+    jr, _ = _record(ref_api(), False)
+    world = scenes.box_pyramid(m3, rows=6)
+    m3.m3World_Step(world, DT, SUBSTEPS)
+    before = m3.m3World_Hash(world)
+    for bad in (jr[: len(jr) - 3], jr[:4], (999).to_bytes(4, "little") + (0).to_bytes(4, "little")):
+        data = C.create_string_buffer(bad, len(bad))
+        assert not m3.m3World_JournalReplay(world, data, len(bad))
+        assert m3.m3World_Hash(world) == before
+    m3.m3DestroyWorld(world)
+
+
+def _twin():
+    if not os.path.exists(TWIN32):
+        pytest.skip("oracle/_ref/libm3twin32.so not built")
+    return api.Api(C.CDLL(TWIN32))
+
+
+def test_cpu_twin_journal_cross_replays_with_reference():
+    """The host world's journal code (the same world.c the product links)
+    over the CPU twin of the step."""
+    _cross(_twin(), joints=False)
+
+
+def test_cpu_twin_refused_journal_leaves_world():
+    _refused_replay_leaves_world(_twin())
+
+
+@pytest.mark.gpu
+def test_journal_cross_replays_with_reference():
+    _cross(product_api(), joints=True)
+
+
+@pytest.mark.gpu
+def test_refused_journal_leaves_world():
+    _refused_replay_leaves_world(product_api())
