@@ -1852,9 +1852,11 @@ m3ShapeId m3CreateHullShape(m3BodyId bodyId, const m3ShapeDef* def, const m3Vec3
     m3ShapeGeom geom;
     memset(&geom, 0, sizeof(geom));
     m3ShapeId id = CreateShape(bodyId, def, M3_HULL, &geom, &data);
-    if (id.index1 != 0 && w->journalActive != 0 && count <= M3_HULL_MAX_INPUT)
+    if (id.index1 != 0 && w->journalActive != 0)
     {
-        m3OpCreateHullShape* rec = (m3OpCreateHullShape*)calloc(1, sizeof(m3OpCreateHullShape));
+        m3OpCreateHullShape* rec = count <= M3_HULL_MAX_INPUT
+                                       ? (m3OpCreateHullShape*)calloc(1, sizeof(m3OpCreateHullShape))
+                                       : NULL;
         if (rec == NULL)
         {
             w->journalOverflow = 1; // cannot record: the journal is invalid
