@@ -51,6 +51,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--batch-only", action="store_true", help="run only the batched-rollback measurement")
     p.add_argument("--no-hull-chains", action="store_true", help="skip the hull-rain-with-chains measurement")
     p.add_argument("--no-adjoint", action="store_true", help="skip the toss adjoint measurement")
+    p.add_argument("--no-events", action="store_true", help="skip the event-stream read measurement")
+    p.add_argument("--events-only", action="store_true", help="run only the event-stream read measurement")
     p.add_argument("--adjoint-only", action="store_true", help="run only the toss adjoint measurement")
     p.add_argument(
         "--hull-chains-only", action="store_true", help="run only the hull-rain-with-chains measurement"
@@ -181,6 +183,39 @@ def run_cpu(steps: int) -> dict | None:
         out["value"] = rate(thr_wall)
         out["cores"] = threads
     return out
+
+
+def run_events_read(steps: int = 20, warm: int = 200) -> dict:
+    """Cost of reading the event streams on the 10k rain: after ``warm``
+    steps (bodies landing), each of ``steps`` steps is synced, then every
+    stream is read (the device builds them at the first read).  Reports the
+    read time per step and the stream sizes of the last step."""
+    import maul3d_amd
+    from maul3d_amd import scenes
+
+    gpu = maul3d_amd.get_api()
+    lib = maul3d_amd.library()
+    world = scenes.rain(gpu, count=BODIES)
+    for _ in range(warm):
+        gpu.m3World_Step(world, DT, SUBSTEPS)
+    total = 0.0
+    sizes = {}
+    for _ in range(steps):
+        gpu.m3World_Step(world, DT, SUBSTEPS)
+        lib.m3World_Sync(world)
+        t0 = time.perf_counter()
+        for name in ("ContactBeginEvents", "ContactEndEvents", "SensorBeginEvents", "SensorEndEvents",
+                     "HitEvents", "BodyMoveEvents"):
+            n = C.c_int32(0)
+            getattr(gpu, "m3World_" + name)(world, C.byref(n))
+            sizes[name] = n.value
+        total += time.perf_counter() - t0
+    gpu.m3DestroyWorld(world)
+    return {
+        "workload": f"10k rain, steps {warm + 1}-{warm + steps}: every event stream read after each step",
+        "ms_per_read": 1000.0 * total / steps,
+        "last_step_sizes": sizes,
+    }
 
 
 def batch_fixture() -> dict | None:
@@ -464,6 +499,9 @@ def main() -> None:
     if args.hull_chains_only:
         print(json.dumps({"hull_chains": run_hull_chains(args.steps)}))
         return
+    if args.events_only:
+        print(json.dumps({"events_read": run_events_read()}))
+        return
     if args.adjoint_only:
         print(json.dumps({"toss_adjoint": run_toss_adjoint()}))
         return
@@ -508,6 +546,8 @@ def main() -> None:
         line["hull_chains"] = run_hull_chains(600)
     if not args.no_adjoint:
         line["toss_adjoint"] = run_toss_adjoint()
+    if not args.no_events:
+        line["events_read"] = run_events_read()
     if not args.no_cpu:
         line["cpu_baseline"] = run_cpu(args.cpu_steps)
     print(json.dumps(line))
