@@ -11,7 +11,9 @@
 //
 // Every entry point takes plain pointers and sizes.  Return codes: 0 = ok,
 // M3GPU_STALL = the step refused for capacity (the world did not advance),
-// negative = a HIP error (the message is available from m3gpu_last_error).
+// M3GPU_REFUSED = the world holds something the device step does not
+// simulate (nothing was enqueued), negative = a HIP error (the message is
+// available from m3gpu_last_error).
 #ifndef M3GPU_H
 #define M3GPU_H
 
@@ -24,6 +26,11 @@ extern "C" {
 #define M3GPU_API __attribute__((visibility("default"))) extern
 
 #define M3GPU_STALL 1
+#define M3GPU_REFUSED 2
+
+// Largest shapeCapacity of a device world: pair keys are packed as two
+// 16-bit shape slots on the device.
+#define M3_DEVICE_MAX_SHAPES 65535
 
 // Field groups for upload/download.
 #define M3GPU_FIELDS_STATIC  0x1u // mass, shapes, hulls, filters, materials
@@ -184,8 +191,12 @@ M3GPU_API int m3gpu_download(m3gpuWorld* world, m3gpuHostState* host, uint32_t f
 M3GPU_API int m3gpu_step(m3gpuWorld* world, const m3gpuStepParams* params,
                          m3gpuStepStats* stats);
 
+// 1 when m3gpu_step would run this world with these params, 0 when it would
+// return M3GPU_REFUSED.
+M3GPU_API int m3gpu_step_supported(const m3gpuWorld* world, const m3gpuStepParams* params);
+
 // Steps several worlds back to back on the device, each with its own
-// params; returns the number of worlds that stalled.
+// params (every world must be supported); returns 0, or -1 on a HIP error.
 M3GPU_API int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,
                               int32_t count);
 

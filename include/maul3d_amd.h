@@ -306,6 +306,12 @@ M3_API bool m3Shape_AreHitEventsEnabled(m3ShapeId shapeId);                     
 M3_API int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt,
                                 int32_t substeps);
 M3_API void m3World_Sync(m3WorldId worldId);
+// Device fault report (no reference counterpart: the reference step cannot
+// fail).  A world whose device call failed (a HIP error, a substep barrier
+// that timed out) stops advancing: m3World_Step is a no-op, m3World_Hash
+// returns 0, m3World_Snapshot returns -1 and m3World_Restore false.  Returns
+// the failure text, or null for a healthy world or a bad id.
+M3_API const char* m3World_GetDeviceError(m3WorldId worldId);
 // Trajectory gradients (no reference counterpart; the reference has no
 // differentiable path).  For each world, from its current state: `steps`
 // steps of (timeStep, subStepCount), `force[b]` applied to dynamic body slot

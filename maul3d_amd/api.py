@@ -356,6 +356,8 @@ _SIGNATURES["m3World_JournalReplay"] = (C.c_bool, [WorldId, C.c_void_p, c_i32])
 _SIGNATURES["m3World_SetHitEventThreshold"] = (None, [WorldId, c_f])
 _SIGNATURES["m3Shape_EnableHitEvents"] = (None, [ShapeId, C.c_bool])
 _SIGNATURES["m3Shape_AreHitEventsEnabled"] = (C.c_bool, [ShapeId])
+_SIGNATURES["m3World_StepMany"] = (c_i32, [C.POINTER(WorldId), c_i32, c_f, c_i32])
+_SIGNATURES["m3World_GetDeviceError"] = (C.c_char_p, [WorldId])
 
 
 def _id_tuple(sid) -> tuple[int, int]:
@@ -378,6 +380,8 @@ class Api:
             "m3DefaultAdjointDef",
             "m3World_Adjoint",
             "m3World_Autotune",
+            "m3World_StepMany",
+            "m3World_GetDeviceError",
         ),
     ):
         self.lib = lib

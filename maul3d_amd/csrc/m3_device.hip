@@ -2954,7 +2954,7 @@ static int CreateImpl(m3gpuWorld* w)
 extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* caps)
 {
     if (caps == nullptr || caps->bodyCapacity <= 0 || caps->shapeCapacity <= 0 ||
-        caps->pairCapacity <= 0 || caps->shapeCapacity > 65535)
+        caps->pairCapacity <= 0 || caps->shapeCapacity > M3_DEVICE_MAX_SHAPES)
     {
         snprintf(s_lastError, sizeof(s_lastError),
                  "m3gpu_world_create: capacities must be positive and shapeCapacity <= 65535");
@@ -3665,6 +3665,11 @@ static int ProfileCollect(m3gpuWorld* w, int32_t substeps)
     return 0;
 }
 
+extern "C" int m3gpu_step_supported(const m3gpuWorld* w, const m3gpuStepParams* p)
+{
+    return w != nullptr && p != nullptr && !(p->continuousEnabled && w->ccdUnsupported);
+}
+
 extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStats* stats)
 {
     if (w == nullptr || p == nullptr || !(p->dt > 0.0f) || p->substeps < 1 ||
@@ -3673,12 +3678,12 @@ extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStat
         snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step: bad arguments");
         return -1;
     }
-    if (p->continuousEnabled && w->ccdUnsupported)
+    if (!m3gpu_step_supported(w, p))
     {
         snprintf(s_lastError, sizeof(s_lastError),
                  "m3gpu_step: continuous collision against static non-plane shapes or bullets "
                  "is not implemented on the device path");
-        return -1;
+        return M3GPU_REFUSED;
     }
     if (JoinBatch(w) != 0 || EnqueueStep(w, p) != 0)
     {

@@ -257,6 +257,11 @@ typedef struct m3World
 
     // device mirror
     m3gpuWorld* dev;
+    // A device call failed (HIP error, a substep barrier that timed out):
+    // the world keeps its last host state, steps become no-ops and
+    // device-backed reads refuse; m3World_GetDeviceError returns the text.
+    uint8_t faulted;
+    char faultText[256];
     uint32_t devDirty;  // field groups the host changed since the last upload
     uint32_t hostStale; // field groups the device changed since the last download
 
@@ -498,26 +503,41 @@ static void BuildJointComponents(m3World* w)
     free(sizes);
 }
 
-static void DeviceFail(const char* what)
+// A failed device call faults the world instead of ending the process: the
+// reference's step is void and never fails (src/solver.c:4322-4330), so the
+// nearest contract is a world that stops advancing.  The message is kept for
+// m3World_GetDeviceError and printed once.
+static void DeviceFail(m3World* w, const char* what)
 {
-    fprintf(stderr, "maul3d_amd: %s failed: %s\n", what, m3gpu_last_error());
-    abort();
+    if (w->faulted == 0)
+    {
+        snprintf(w->faultText, sizeof(w->faultText), "%s failed: %s", what, m3gpu_last_error());
+        fprintf(stderr, "maul3d_amd: %s\n", w->faultText);
+    }
+    w->faulted = 1;
 }
 
-// Bring the host copy up to date with the device (after a step).
-static void SyncHost(m3World* w)
+// Bring the host copy up to date with the device (after a step).  Returns 0
+// when the world is faulted (the host copy is then the last good one).
+static int SyncHost(m3World* w)
 {
+    if (w->faulted != 0)
+    {
+        return 0;
+    }
     if (w->hostStale == 0 || w->dev == NULL)
     {
-        return;
+        return 1;
     }
     m3gpuHostState h;
     HostView(w, &h);
     if (m3gpu_download(w->dev, &h, w->hostStale) != 0)
     {
-        DeviceFail("m3gpu_download");
+        DeviceFail(w, "m3gpu_download");
+        return 0;
     }
     w->hostStale = 0;
+    return 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -635,9 +655,13 @@ static void BuildEvents(m3World* w)
     out.sensorEndKeys = w->eventKeys + 3 * (size_t)np;
     out.hits = w->eventHits;
     out.moves = w->eventMoves;
-    if (m3gpu_events(w->dev, w->eventThreshold, &out) != 0)
+    if (w->faulted != 0 || m3gpu_events(w->dev, w->eventThreshold, &out) != 0)
     {
-        DeviceFail("m3gpu_events");
+        if (w->faulted == 0)
+        {
+            DeviceFail(w, "m3gpu_events");
+        }
+        return;
     }
     const uint64_t* keys[4] = {out.beginKeys, out.endKeys, out.sensorBeginKeys, out.sensorEndKeys};
     m3ContactEvent* lists[4] = {w->beginEvents, w->endEvents, w->sensorBeginEvents,
@@ -762,6 +786,13 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
         !FiniteF(def->maximumLinearSpeed) || def->maximumLinearSpeed <= 0.0f ||
         !FiniteF(def->hitEventThreshold) || def->hitEventThreshold < 0.0f)
     {
+        return nullId;
+    }
+    if (def->shapeCapacity > M3_DEVICE_MAX_SHAPES)
+    {
+        // The device pair list packs two shape slots into 32-bit keys:
+        // refuse at creation (the reference's null id for a def it cannot
+        // serve) rather than at the first step.
         return nullId;
     }
     int32_t slot = -1;
@@ -1670,6 +1701,32 @@ static m3ShapeId CreateShape(m3BodyId bodyId, const m3ShapeDef* def, uint8_t typ
     {
         return nullId;
     }
+    if (hull == NULL)
+    {
+        // Per-type geometry wall of the plain-geometry door
+        // (src/shape.c:433-470): journal replay hands raw record bytes here.
+        bool ok = false;
+        if (type == M3_SPHERE)
+        {
+            ok = FiniteV3(geom->v) && FiniteF(geom->s) && geom->s > 0.0f;
+        }
+        else if (type == M3_PLANE)
+        {
+            ok = FiniteV3(geom->v) && FiniteF(geom->s) && m3Dot3(geom->v, geom->v) > 1.0e-12f;
+        }
+        else if (type == M3_CAPSULE)
+        {
+            ok = FiniteV3(geom->v) && FiniteV3(geom->v2) && FiniteF(geom->s) && geom->s > 0.0f;
+        }
+        else if (type == M3_HULL)
+        {
+            ok = FiniteV3(geom->v) && geom->v.x > 0.0f && geom->v.y > 0.0f && geom->v.z > 0.0f;
+        }
+        if (!ok)
+        {
+            return nullId;
+        }
+    }
     Touch(w, M3GPU_FIELDS_ALL);
     int32_t s = PoolAlloc(&w->shapePool);
     if (s < 0)
@@ -2115,6 +2172,10 @@ bool m3Joint_IsValid(m3JointId jointId)
 
 static int EnsureDevice(m3World* w)
 {
+    if (w->faulted != 0)
+    {
+        return 0;
+    }
     if (w->dev == NULL)
     {
         m3gpuCaps caps;
@@ -2126,7 +2187,8 @@ static int EnsureDevice(m3World* w)
         w->dev = m3gpu_world_create(&caps);
         if (w->dev == NULL)
         {
-            DeviceFail("m3gpu_world_create");
+            DeviceFail(w, "m3gpu_world_create");
+            return 0;
         }
         w->devDirty = M3GPU_FIELDS_ALL;
     }
@@ -2140,7 +2202,8 @@ static int EnsureDevice(m3World* w)
         HostView(w, &h);
         if (m3gpu_upload(w->dev, &h, w->devDirty) != 0)
         {
-            DeviceFail("m3gpu_upload");
+            DeviceFail(w, "m3gpu_upload");
+            return 0;
         }
         w->devDirty = 0;
         w->pairsFullQuery = 0; // consumed by the device's next step
@@ -2166,20 +2229,29 @@ static void FillParams(const m3World* w, float dt, int32_t substeps, m3gpuStepPa
     p->continuousEnabled = w->continuousEnabled;
 }
 
-void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
+// One step; false when the step did not run (bad arguments, a faulted world,
+// or a world the device step refuses), in which case nothing changed.
+static bool StepWorld(m3World* w, float dt, int32_t substeps)
 {
-    m3World* w = WorldFromId(worldId);
     if (w == NULL || !(dt > 0.0f) || substeps < 1 || substeps > M3_MAX_SUBSTEPS)
     {
-        return;
+        return false;
     }
-    EnsureDevice(w);
+    if (!EnsureDevice(w))
+    {
+        return false;
+    }
     m3gpuStepParams p;
     FillParams(w, dt, substeps, &p);
     int rc = m3gpu_step(w->dev, &p, NULL);
+    if (rc == M3GPU_REFUSED)
+    {
+        return false; // a world the device step does not support: no-op
+    }
     if (rc < 0)
     {
-        DeviceFail("m3gpu_step");
+        DeviceFail(w, "m3gpu_step");
+        return false;
     }
     w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
     w->eventsState = 1;
@@ -2193,6 +2265,18 @@ void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
         rec.substeps = substeps;
         JournalRecord(w, M3_OP_STEP, &rec, (int32_t)sizeof(rec));
     }
+    return true;
+}
+
+void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
+{
+    (void)StepWorld(WorldFromId(worldId), dt, substeps);
+}
+
+const char* m3World_GetDeviceError(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    return (w != NULL && w->faulted != 0) ? w->faultText : NULL;
 }
 
 // Batched stepping of many worlds with one call: the rollback and toss
@@ -2206,39 +2290,51 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
     }
     enum { CHUNK = 1024 };
     m3gpuWorld* devs[CHUNK];
+    m3World* hosts[CHUNK];
     m3gpuStepParams params[CHUNK];
     int32_t n = 0;
     for (int32_t k = 0; k < count && n < CHUNK; ++k)
     {
         m3World* w = WorldFromId(worlds[k]);
-        if (w == NULL)
+        if (w == NULL || !EnsureDevice(w))
         {
             continue;
         }
-        EnsureDevice(w);
-        devs[n] = w->dev;
         FillParams(w, dt, substeps, &params[n]);
+        if (!m3gpu_step_supported(w->dev, &params[n]))
+        {
+            continue; // refused like m3World_Step: this world does not step
+        }
+        devs[n] = w->dev;
+        hosts[n] = w;
         n += 1;
-        w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
-        w->eventsState = 1;
-        w->eventThreshold = w->hitEventThreshold;
+    }
+    if (n == 0)
+    {
+        return 0;
     }
     int rc = m3gpu_step_many(devs, params, n);
     if (rc < 0)
     {
-        DeviceFail("m3gpu_step_many");
+        for (int32_t k = 0; k < n; ++k)
+        {
+            DeviceFail(hosts[k], "m3gpu_step_many");
+        }
+        return -1;
     }
     m3OpStep rec;
     memset(&rec, 0, sizeof(rec));
     rec.dt = dt;
     rec.substeps = substeps;
-    for (int32_t k = 0; k < count; ++k)
+    // Only the worlds handed to the device stepped; each records its step
+    // (a world listed twice stepped twice and records twice).
+    for (int32_t k = 0; k < n; ++k)
     {
-        m3World* w = WorldFromId(worlds[k]);
-        if (w != NULL)
-        {
-            JournalRecord(w, M3_OP_STEP, &rec, (int32_t)sizeof(rec));
-        }
+        m3World* w = hosts[k];
+        w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
+        w->eventsState = 1;
+        w->eventThreshold = w->hitEventThreshold;
+        JournalRecord(w, M3_OP_STEP, &rec, (int32_t)sizeof(rec));
     }
     return n;
 }
@@ -2288,7 +2384,10 @@ int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointD
                 return -1; // joints are outside the adjoint's scope
             }
         }
-        EnsureDevice(w);
+        if (!EnsureDevice(w))
+        {
+            return -1;
+        }
         devs[k] = w->dev;
         FillParams(w, d->timeStep, d->subStepCount, &params[k]);
         m3gpuAdjointRequest* r = &reqs[k];
@@ -2319,7 +2418,10 @@ int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointD
     int rc = m3gpu_adjoint_many(devs, params, reqs, count);
     if (rc < 0)
     {
-        DeviceFail("m3gpu_adjoint_many");
+        for (int32_t k = 0; k < count; ++k)
+        {
+            DeviceFail(WorldFromId(worlds[k]), "m3gpu_adjoint_many");
+        }
     }
     return rc == 0 ? count : -1;
 }
@@ -2333,10 +2435,9 @@ void m3World_Sync(m3WorldId worldId)
     {
         return;
     }
-    EnsureDevice(w);
-    if (m3gpu_sync(w->dev) != 0)
+    if (EnsureDevice(w) && m3gpu_sync(w->dev) != 0)
     {
-        DeviceFail("m3gpu_sync");
+        DeviceFail(w, "m3gpu_sync");
     }
 }
 
@@ -2383,19 +2484,27 @@ int32_t m3World_Autotune(const m3WorldId* worlds, int32_t count, float timeStep,
     for (int32_t k = 0; k < count; ++k)
     {
         m3World* w = WorldFromId(worlds[k]);
-        if (w == NULL)
+        if (w == NULL || !EnsureDevice(w))
         {
             return -1;
         }
-        EnsureDevice(w);
     }
     memset(result, 0, sizeof(*result));
     void** snaps = (void**)calloc((size_t)count, sizeof(void*));
     int32_t* sizes = (int32_t*)calloc((size_t)count, sizeof(int32_t));
+    // The trials are not part of any session: journals stay silent through
+    // them (the worlds are restored afterwards), as during a replay.
+    uint8_t* journaling = (uint8_t*)calloc((size_t)count, 1);
     int32_t rc = -1;
-    if (snaps == NULL || sizes == NULL)
+    if (snaps == NULL || sizes == NULL || journaling == NULL)
     {
         goto done;
+    }
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3World* w = WorldFromId(worlds[k]);
+        journaling[k] = w->journalActive;
+        w->journalActive = 0;
     }
     for (int32_t k = 0; k < count; ++k)
     {
@@ -2483,6 +2592,14 @@ int32_t m3World_Autotune(const m3WorldId* worlds, int32_t count, float timeStep,
     }
     rc = n;
 done:
+    if (journaling != NULL)
+    {
+        for (int32_t k = 0; k < count; ++k)
+        {
+            m3World* w = WorldFromId(worlds[k]);
+            w->journalActive = w->journalActive | journaling[k];
+        }
+    }
     if (snaps != NULL)
     {
         for (int32_t k = 0; k < count; ++k)
@@ -2492,6 +2609,7 @@ done:
     }
     free(snaps);
     free(sizes);
+    free(journaling);
     return rc;
 }
 
@@ -2499,11 +2617,10 @@ done:
 int32_t m3World_ProfileBegin(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);
-    if (w == NULL)
+    if (w == NULL || !EnsureDevice(w))
     {
         return -1;
     }
-    EnsureDevice(w);
     return m3gpu_profile_begin(w->dev);
 }
 
@@ -2723,7 +2840,12 @@ static bool JournalApply(m3World* w, m3WorldId worldId, const uint8_t* stream, i
                 return false;
             }
             memcpy(&rec, payload, sizeof(rec));
-            m3World_Step(worldId, rec.dt, rec.substeps);
+            // A hostile record refuses (src/world.c:1635-1642), and so does
+            // a step this build's device does not run.
+            if (!StepWorld(w, rec.dt, rec.substeps))
+            {
+                return false;
+            }
             break;
         }
         case M3_OP_CREATE_BODY:
@@ -2967,11 +3089,10 @@ bool m3World_JournalReplay(m3WorldId worldId, const void* data, int32_t size)
 uint64_t m3World_Hash(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);
-    if (w == NULL)
+    if (w == NULL || !SyncHost(w))
     {
-        return 0;
+        return 0; // a faulted world has no current state to hash
     }
-    SyncHost(w);
     uint64_t h = M3_HASH_INIT;
     h = m3HashBytes(h, &w->stepCount, 8);
     h = m3HashBytes(h, &w->gravity, (int32_t)sizeof(m3Vec3));
@@ -3377,11 +3498,10 @@ static void WalkWorld(m3Walker* k, m3World* w, int32_t nb, int32_t ns, int32_t n
 int32_t m3World_SnapshotSize(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);
-    if (w == NULL)
+    if (w == NULL || !SyncHost(w))
     {
         return -1;
     }
-    SyncHost(w);
     m3Walker k;
     memset(&k, 0, sizeof(k));
     k.limit = -1;
@@ -3428,7 +3548,7 @@ int32_t m3World_Snapshot(m3WorldId worldId, void* out, int32_t capacity)
 bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size)
 {
     m3World* w = WorldFromId(worldId);
-    if (w == NULL || data == NULL || size < (int32_t)sizeof(m3SnapHeader))
+    if (w == NULL || w->faulted != 0 || data == NULL || size < (int32_t)sizeof(m3SnapHeader))
     {
         return false;
     }

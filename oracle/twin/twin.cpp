@@ -173,6 +173,7 @@ struct m3gpuWorld
     std::vector<uint32_t> masks;
     int jointsPresent;
     int targetsPresent;
+    int ccdUnsupported; // the device step's refusal rule (continuous vs tree shapes)
     tw::m3HitRec* hitRecBuf;
 };
 
@@ -361,6 +362,22 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
         const ::m3Vec3* fo = (const ::m3Vec3*)h->bodyForce;
         const ::m3Vec3* to = (const ::m3Vec3*)h->bodyTorque;
         w->targetsPresent = 0;
+        w->ccdUnsupported = 0;
+        for (int s2 = 0; s2 < ns; ++s2)
+        {
+            if (h->shapeAlive[s2] != 0 && h->shapeType[s2] != M3_PLANE &&
+                h->types[h->shapeBody[s2]] == M3_STATIC && h->shapeSensor[s2] == 0)
+            {
+                w->ccdUnsupported = 1;
+            }
+        }
+        for (int b = 0; b < nb; ++b)
+        {
+            if (h->bodyAlive[b] != 0 && h->bulletFlags[b] != 0)
+            {
+                w->ccdUnsupported = 1;
+            }
+        }
         for (int b = 0; b < nb; ++b)
         {
             v.xf[b] = X(xf[b]);
@@ -477,13 +494,22 @@ static int Step(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
     return 0;
 }
 
+extern "C" int m3gpu_step_supported(const m3gpuWorld* w, const m3gpuStepParams* p)
+{
+    return w != nullptr && p != nullptr && !(p->continuousEnabled && w->ccdUnsupported);
+}
+
 extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStats* st)
 {
+    if (!m3gpu_step_supported(w, p))
+    {
+        snprintf(s_err, sizeof(s_err), "twin: continuous collision against tree shapes");
+        return M3GPU_REFUSED;
+    }
     int rc = Step(w, p, nullptr);
     if (rc < 0)
     {
-        fprintf(stderr, "%s\n", s_err);
-        abort();
+        return -1;
     }
     if (st != nullptr)
     {
@@ -498,12 +524,14 @@ extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStat
 
 extern "C" int m3gpu_step_many(m3gpuWorld* const* ws, const m3gpuStepParams* ps, int32_t n)
 {
-    int stalled = 0;
     for (int32_t k = 0; k < n; ++k)
     {
-        stalled += m3gpu_step(ws[k], &ps[k], nullptr) != 0;
+        if (m3gpu_step(ws[k], &ps[k], nullptr) < 0)
+        {
+            return -1;
+        }
     }
-    return stalled;
+    return 0;
 }
 
 // The event streams of the last step, by the device's rule (every begin
