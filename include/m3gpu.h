@@ -179,6 +179,10 @@ M3GPU_API const char* m3gpu_last_error(void);
 
 M3GPU_API m3gpuWorld* m3gpu_world_create(const m3gpuCaps* caps);
 M3GPU_API void m3gpu_world_destroy(m3gpuWorld* world);
+// Releases the process-wide device state (batch launch graphs and views,
+// the adjoint arena, their streams); later calls recreate it on demand.
+// Every world must be destroyed first.
+M3GPU_API void m3gpu_shutdown(void);
 
 // Copies the selected field groups host -> device (upload) or device -> host
 // (download).  Both block until the copy is complete.

@@ -3737,6 +3737,11 @@ struct m3gpuBatch
 
 static m3gpuBatch s_batch;
 static pthread_mutex_t s_batchLock = PTHREAD_MUTEX_INITIALIZER;
+// The adjoint's process-lifetime arena and replay stream (defined with the
+// adjoint below).
+static void* s_adjArena;
+static hipStream_t s_adjStream;
+static size_t s_adjBytes;
 static int s_batchThreadsWanted = 0; // set by m3gpu_set_batch_threads (0: default)
 
 // Threads per world workgroup of the batched substep kernel: the tuned
@@ -4332,6 +4337,41 @@ extern "C" int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams*
     return rc;
 }
 
+// Releases what the library holds for the whole process: the batch's
+// captured launch graphs, views, stream and event, and the adjoint arena and
+// stream.  The host world calls it from an atexit handler registered after
+// the HIP runtime initialised, so it runs before the runtime's own teardown.
+extern "C" void m3gpu_shutdown(void)
+{
+    pthread_mutex_lock(&s_batchLock);
+    BatchDrop();
+    if (s_batch.capacity > 0)
+    {
+        (void)hipStreamSynchronize(s_batch.stream);
+        (void)hipFree(s_batch.devViews);
+        (void)hipEventDestroy(s_batch.done);
+        (void)hipStreamDestroy(s_batch.stream);
+        free(s_batch.worlds);
+        free(s_batch.baseCur);
+        free(s_batch.layouts);
+        free(s_batch.hostViews);
+    }
+    memset(&s_batch, 0, sizeof(s_batch));
+    pthread_mutex_unlock(&s_batchLock);
+    if (s_adjStream != nullptr)
+    {
+        (void)hipStreamSynchronize(s_adjStream);
+        (void)hipStreamDestroy(s_adjStream);
+        s_adjStream = nullptr;
+    }
+    if (s_adjArena != nullptr)
+    {
+        (void)hipFree(s_adjArena);
+        s_adjArena = nullptr;
+        s_adjBytes = 0;
+    }
+}
+
 extern "C" int m3gpu_sync(m3gpuWorld* w)
 {
     if (w == nullptr || JoinBatch(w) != 0)
@@ -4872,9 +4912,6 @@ __global__ void __launch_bounds__(M3_ADJ_LANES) k_adj_finish(m3AdjCtx* ctxs)
 
 // Device arena of the adjoint, kept between calls and grown as needed, and
 // the stream its reverse passes run on.
-static void* s_adjArena = nullptr;
-static hipStream_t s_adjStream = nullptr;
-static size_t s_adjBytes = 0;
 
 static int AdjReserve(size_t bytes)
 {

@@ -2170,6 +2170,24 @@ bool m3Joint_IsValid(m3JointId jointId)
     return ResolveJoint(jointId, &index) != NULL;
 }
 
+// At process exit the device mirrors of live worlds and the library's
+// process-wide device state are released before the HIP runtime tears itself
+// down: the handler is registered after the first device world exists, and
+// exit handlers run in reverse order of registration.
+static void ReleaseDeviceAtExit(void)
+{
+    for (int32_t i = 0; i < M3_MAX_WORLDS; ++i)
+    {
+        m3World* w = s_worlds[i];
+        if (w != NULL && w->dev != NULL)
+        {
+            m3gpu_world_destroy(w->dev);
+            w->dev = NULL;
+        }
+    }
+    m3gpu_shutdown();
+}
+
 static int EnsureDevice(m3World* w)
 {
     if (w->faulted != 0)
@@ -2189,6 +2207,12 @@ static int EnsureDevice(m3World* w)
         {
             DeviceFail(w, "m3gpu_world_create");
             return 0;
+        }
+        static int s_exitHandler = 0;
+        if (s_exitHandler == 0)
+        {
+            s_exitHandler = 1;
+            atexit(ReleaseDeviceAtExit);
         }
         w->devDirty = M3GPU_FIELDS_ALL;
     }

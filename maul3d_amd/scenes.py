@@ -116,6 +116,81 @@ def bench_pyramid(api: Api) -> WorldId:
     return world
 
 
+def hulljam(api: Api) -> WorldId:
+    """The reference bench's hull jam (bench/bench_main.c:97-135): 80 dynamic
+    bodies dropped in a column over a plane, every third a 12-point QuickHull
+    rock (a 0.25 m box where the hull is refused), the rest 0.3 x 0.25 x 0.35
+    boxes, friction 0.5.
+
+    Its hash after 300 steps of (1/60, 4) is pinned at 7bb4425d3adc3a51 in
+    bench/pins.txt:2.
+    """
+    world = _world(api, 256, 256)
+    _ground(api, world)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    sd = api.m3DefaultShapeDef()
+    sd.friction = 0.5
+    rng = SplitMix64(0xBEEF)
+    for k in range(80):
+        x = rng.range(-2.0, 2.0)
+        z = rng.range(-2.0, 2.0)
+        bd.position = Pos3(x, 0.6 + 0.5 * float(k), z)
+        body = api.m3CreateBody(world, C.byref(bd))
+        if k % 3 == 0:
+            cloud = (Vec3 * 12)(
+                *[Vec3(rng.range(-0.3, 0.3), rng.range(-0.3, 0.3), rng.range(-0.3, 0.3)) for _ in range(12)]
+            )
+            if api.m3CreateHullShape(body, C.byref(sd), cloud, 12).index1 == 0:
+                _check(api.m3CreateBoxShape(body, C.byref(sd), Vec3(0.25, 0.25, 0.25)), "box")
+        else:
+            _check(api.m3CreateBoxShape(body, C.byref(sd), Vec3(0.3, 0.25, 0.35)), "box")
+    return world
+
+
+def bounce(api: Api, count: int = 160, seed: int = 0xB0A7CE, restitution: bool = True) -> WorldId:
+    """Restitution exercise: ``count`` spheres, boxes and capsules with
+    restitution U[0, 0.95] (friction U[0.2, 0.8]) dropped from 2-14 m onto a
+    plane of restitution 0.5 inside a 6 m square, so they strike the plane
+    and each other above the 1 m/s restitution threshold.  Every contact's
+    restitution is max(a, b) (src/solver.c:156-404) and the serial
+    restitution pass (src/solver.c:2879-2907) runs every step.  With
+    ``restitution`` false every coefficient is 0 (same draws), the control
+    that shows the pass changes the trajectory."""
+    world = _world(api, count + 1, count + 1)
+    gd = api.m3DefaultBodyDef()
+    ground = api.m3CreateBody(world, C.byref(gd))
+    sd = api.m3DefaultShapeDef()
+    sd.friction = 0.6
+    sd.restitution = 0.5 if restitution else 0.0
+    _check(api.m3CreatePlaneShape(ground, C.byref(sd), C.byref(Plane(Vec3(0.0, 1.0, 0.0), 0.0))), "plane")
+    rng = SplitMix64(seed)
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    for k in range(count):
+        bd.position = Pos3(rng.range(-3.0, 3.0), rng.range(2.0, 14.0), rng.range(-3.0, 3.0))
+        q = rng.unit_quat()
+        bd.rotation = Quat(*q)
+        bd.linearVelocity = Vec3(rng.range(-2.0, 2.0), rng.range(-4.0, 0.0), rng.range(-2.0, 2.0))
+        body = api.m3CreateBody(world, C.byref(bd))
+        e = rng.range(0.0, 0.95)
+        sd.restitution = e if restitution else 0.0
+        sd.friction = rng.range(0.2, 0.8)
+        kind = k % 3
+        if kind == 0:
+            ball = Sphere(Vec3(0.0, 0.0, 0.0), rng.range(0.2, 0.5))
+            _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+        elif kind == 1:
+            he = Vec3(rng.range(0.2, 0.5), rng.range(0.2, 0.5), rng.range(0.2, 0.5))
+            _check(api.m3CreateBoxShape(body, C.byref(sd), he), "box")
+        else:
+            r = rng.range(0.15, 0.35)
+            hh = rng.range(0.2, 0.5)
+            cap = Capsule(Vec3(-hh, 0.0, 0.0), Vec3(hh, 0.0, 0.0), r)
+            _check(api.m3CreateCapsuleShape(body, C.byref(sd), C.byref(cap)), "capsule")
+    return world
+
+
 def smoke10k(api: Api) -> WorldId:
     """The reference bench's 10k mixed rain (bench/bench_main.c:612-650).
 

@@ -272,6 +272,7 @@ extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* c)
 }
 
 extern "C" void m3gpu_world_destroy(m3gpuWorld* w) { delete w; }
+extern "C" void m3gpu_shutdown(void) {}
 
 extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
 {
