@@ -195,9 +195,23 @@ M3GPU_API int m3gpu_download(m3gpuWorld* world, m3gpuHostState* host, uint32_t f
 M3GPU_API int m3gpu_step(m3gpuWorld* world, const m3gpuStepParams* params,
                          m3gpuStepStats* stats);
 
-// 1 when m3gpu_step would run this world with these params, 0 when it would
-// return M3GPU_REFUSED.
+// 1 when m3gpu_step would run this world with these params in one call, 0
+// when it would return M3GPU_REFUSED.
 M3GPU_API int m3gpu_step_supported(const m3gpuWorld* world, const m3gpuStepParams* params);
+
+// 1 when this step's continuous pass meets shapes of the broadphase tree
+// (continuous collision on, and a static non-plane shape or a bullet body
+// present).  The reference sweeps those in its tree's depth-first order
+// (src/solver.c:3402), so the host, which keeps the tree, steps such a world
+// in two calls: m3gpu_step_begin enqueues everything before the continuous
+// pass, m3gpu_step_leaves waits for the step's leaf boxes (maxShape x 6
+// doubles: lo xyz, hi xyz; valid until the next step), the host replays the
+// tree's leaf moves from them and passes every shape's depth-first rank
+// (-1: not in the tree) to m3gpu_step_end, which enqueues the rest.
+M3GPU_API int m3gpu_step_needs_tree(const m3gpuWorld* world, const m3gpuStepParams* params);
+M3GPU_API int m3gpu_step_begin(m3gpuWorld* world, const m3gpuStepParams* params, int32_t wantLeaves);
+M3GPU_API const double* m3gpu_step_leaves(m3gpuWorld* world);
+M3GPU_API int m3gpu_step_end(m3gpuWorld* world, const m3gpuStepParams* params, const int32_t* ranks);
 
 // Steps several worlds back to back on the device, each with its own
 // params (every world must be supported); returns 0, or -1 on a HIP error.

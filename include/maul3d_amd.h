@@ -312,6 +312,20 @@ M3_API void m3World_Sync(m3WorldId worldId);
 // returns 0, m3World_Snapshot returns -1 and m3World_Restore false.  Returns
 // the failure text, or null for a healthy world or a bad id.
 M3_API const char* m3World_GetDeviceError(m3WorldId worldId);
+// The broadphase tree (no reference counterpart: the reference always keeps
+// its tree; the device step does not need one).  With the mirror on, the
+// host replays every step's leaf moves into the reference's tree
+// (src/broad_phase.c:505-539), so snapshots carry the reference's exact tree
+// bytes and the world continues on the tree the reference would hold; it
+// costs a host synchronisation and the tree work per step.  Off (the
+// default), a snapshot carries the canonical rebuild of the current leaves
+// (what m3World_RebuildBroadphase leaves).  A world whose continuous pass
+// meets tree shapes (a static non-plane shape or a bullet) keeps the mirror
+// current whatever the setting.  Returns false for a bad or faulted world.
+M3_API bool m3World_EnableTreeMirror(m3WorldId worldId, bool flag);
+// True when the host tree equals the reference's for this world's history.
+M3_API bool m3World_IsTreeMirrorCurrent(m3WorldId worldId);
+M3_API void m3World_RebuildBroadphase(m3WorldId worldId); // world.h:100
 // Trajectory gradients (no reference counterpart; the reference has no
 // differentiable path).  For each world, from its current state: `steps`
 // steps of (timeStep, subStepCount), `force[b]` applied to dynamic body slot

@@ -928,9 +928,40 @@ M3_HD void m3WitnessPoints(const m3Simplex* s, m3Vec3* pointA, m3Vec3* pointB)
     }
 }
 
-// Cold-start GJK on the cores (radii not applied): the only form the
-// narrowphase uses (empty cache, useRadii false; src/distance.c:505-706).
-M3_HD m3GjkOut m3ShapeDistance(const m3Proxy* proxyA, const m3Proxy* proxyB, m3Quat q, m3Vec3 p)
+// The simplex cache a time-of-impact search carries between its distance
+// queries (src/world_internal.h:1310-1316, written by src/distance.c:123-132).
+typedef struct m3SimplexCache
+{
+    m3real metric;
+    uint16_t count;
+    uint8_t indexA[4];
+    uint8_t indexB[4];
+} m3SimplexCache;
+
+M3_HD m3real m3SimplexMetric(const m3Simplex* s) // src/distance.c:103-121
+{
+    switch (s->count)
+    {
+    case 1:
+        return 0.0f;
+    case 2:
+        return m3Length3(m3Sub3(s->v[1].w, s->v[0].w));
+    case 3:
+        return 0.5f * m3Length3(m3Cross3(m3Sub3(s->v[1].w, s->v[0].w), m3Sub3(s->v[2].w, s->v[0].w)));
+    case 4:
+        return m3ScalarTriple(m3Sub3(s->v[1].w, s->v[0].w), m3Sub3(s->v[2].w, s->v[0].w),
+                              m3Sub3(s->v[3].w, s->v[0].w)) /
+               6.0f;
+    default:
+        return 0.0f;
+    }
+}
+
+// GJK on the cores (radii not applied), seeded from and writing back a
+// simplex cache (src/distance.c:505-706).  The overlap exits return before
+// the cache is written, as there.
+M3_HD m3GjkOut m3ShapeDistanceCached(const m3Proxy* proxyA, const m3Proxy* proxyB, m3Quat q, m3Vec3 p,
+                                     m3SimplexCache* cache)
 {
     m3GjkOut out;
     out.pointA = m3V3(0.0f, 0.0f, 0.0f);
@@ -940,13 +971,43 @@ M3_HD m3GjkOut m3ShapeDistance(const m3Proxy* proxyA, const m3Proxy* proxyB, m3Q
 
     m3Simplex simplex;
     memset(&simplex, 0, sizeof(simplex));
-    simplex.count = 1;
-    simplex.v[0].indexA = 0;
-    simplex.v[0].indexB = 0;
-    simplex.v[0].wA = proxyA->points[0];
-    simplex.v[0].wB = m3Add3(m3RotateVec3(q, proxyB->points[0]), p);
-    simplex.v[0].w = m3Sub3(simplex.v[0].wB, simplex.v[0].wA);
-    simplex.v[0].a = 0.0f;
+    simplex.count = cache->count <= 4 ? (int32_t)cache->count : 0;
+    for (int32_t i = 0; i < simplex.count; ++i)
+    {
+        const int32_t indexA = cache->indexA[i];
+        const int32_t indexB = cache->indexB[i];
+        if (indexA >= proxyA->count || indexB >= proxyB->count)
+        {
+            simplex.count = 0;
+            break;
+        }
+        simplex.v[i].indexA = indexA;
+        simplex.v[i].indexB = indexB;
+        simplex.v[i].wA = proxyA->points[indexA];
+        simplex.v[i].wB = m3Add3(m3RotateVec3(q, proxyB->points[indexB]), p);
+        simplex.v[i].w = m3Sub3(simplex.v[i].wB, simplex.v[i].wA);
+        simplex.v[i].a = 0.0f;
+    }
+    if (simplex.count > 0)
+    {
+        // Flush a cache whose simplex changed size substantially.
+        const m3real metric1 = cache->metric;
+        const m3real metric2 = m3SimplexMetric(&simplex);
+        if (2.0f * metric1 < metric2 || metric2 < 0.5f * metric1 || metric2 < FLT_EPSILON)
+        {
+            simplex.count = 0;
+        }
+    }
+    if (simplex.count == 0)
+    {
+        simplex.count = 1;
+        simplex.v[0].indexA = 0;
+        simplex.v[0].indexB = 0;
+        simplex.v[0].wA = proxyA->points[0];
+        simplex.v[0].wB = m3Add3(m3RotateVec3(q, proxyB->points[0]), p);
+        simplex.v[0].w = m3Sub3(simplex.v[0].wB, simplex.v[0].wA);
+        simplex.v[0].a = 0.0f;
+    }
 
     m3Simplex backup;
     memset(&backup, 0, sizeof(backup));
@@ -1069,11 +1130,28 @@ M3_HD m3GjkOut m3ShapeDistance(const m3Proxy* proxyA, const m3Proxy* proxyB, m3Q
     m3Vec3 pointA;
     m3Vec3 pointB;
     m3WitnessPoints(&simplex, &pointA, &pointB);
+    cache->metric = m3SimplexMetric(&simplex);
+    cache->count = (uint16_t)simplex.count;
+    for (int32_t i = 0; i < simplex.count; ++i)
+    {
+        cache->indexA[i] = (uint8_t)simplex.v[i].indexA;
+        cache->indexB[i] = (uint8_t)simplex.v[i].indexB;
+    }
     out.pointA = pointA;
     out.pointB = pointB;
     out.distance = m3Length3(m3Sub3(pointA, pointB));
     out.normal = normal;
     return out;
+}
+
+// Cold-start GJK on the cores (radii not applied): the form the narrowphase
+// uses (empty cache, useRadii false; src/distance.c:505-706).
+M3_HD m3GjkOut m3ShapeDistance(const m3Proxy* proxyA, const m3Proxy* proxyB, m3Quat q, m3Vec3 p)
+{
+    m3SimplexCache cache;
+    cache.metric = 0.0f;
+    cache.count = 0;
+    return m3ShapeDistanceCached(proxyA, proxyB, q, p, &cache);
 }
 
 #endif // M3_COLLIDE_H

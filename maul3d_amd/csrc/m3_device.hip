@@ -66,6 +66,11 @@ struct m3Dev
     // Pairs binned by collision routine: 8 counts, then one bin of
     // pairCapacity entries per routine.
     int32_t* narrowBins;
+    // Continuous pass with bullets: the pulled-back pose of each body and
+    // whether it was pulled back, before they are applied.
+    m3Transform* ccdXf;
+    uint8_t* ccdHit;
+    int32_t ccdBullets; // some bullet body exists (the three-pass form)
 };
 
 struct m3gpuWorld
@@ -78,6 +83,10 @@ struct m3gpuWorld
     int cur;
     int fullQuery; // rebuild the frozen-pair buffer before the next step
     int ccdUnsupported;
+    int ccdNeedsTree; // continuous sweeps meet tree shapes (static non-plane targets or bullets)
+    int32_t* rankStage; // pinned staging of the tree ranks a step carries
+    double* leafStage;  // pinned copy of the step's leaf boxes (m3gpu_step_leaves)
+    hipEvent_t leavesCopied;
     m3Dev dev;      // host copy of what kernels see (scratch pointers, view)
     m3Dev* devDev;  // device copies: [parity] = pair buffers for a step with cur = parity
     int devDirty;   // the device copies need refreshing
@@ -2267,6 +2276,13 @@ __global__ void k_store(const m3Dev* __restrict__ ds)
 // Continuous collision and sleep (src/solver.c:3349-3430, 3582-3660)
 // ---------------------------------------------------------------------------
 
+// Continuous collision (src/solver.c:3349-3430).  Without bullets a fast
+// body sweeps only against static shapes and planes, which the pass never
+// moves, so bodies run in parallel and apply their own pull-back.  With
+// bullets, k_ccd computes the other bodies into ccdXf / ccdHit,
+// k_ccd_bullets the bullets (each sees the targets of lower slots pulled
+// back, as the reference's slot-order loop does; bullets never sweep against
+// bullets), and k_ccd_apply applies both.
 __global__ void k_ccd(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
@@ -2275,10 +2291,52 @@ __global__ void k_ccd(const m3Dev* __restrict__ ds)
     {
         return;
     }
-    int pulled = m3ContinuousBody(&v, i);
-    if (d.tape != nullptr)
+    if (d.ccdBullets == 0)
     {
-        d.tape->ccd[i] = (uint8_t)pulled;
+        int pulled = m3ContinuousBody(&v, i);
+        if (d.tape != nullptr)
+        {
+            d.tape->ccd[i] = (uint8_t)pulled;
+        }
+        return;
+    }
+    d.ccdHit[i] = 0;
+    m3Transform xf;
+    if (v.bullet[i] == 0 && m3ContinuousSweep(&v, nullptr, i, &xf))
+    {
+        d.ccdXf[i] = xf;
+        d.ccdHit[i] = 1;
+    }
+}
+
+__global__ void k_ccd_bullets(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0 || v.bullet[i] == 0)
+    {
+        return;
+    }
+    const m3CcdPoses poses = {d.ccdXf, d.ccdHit};
+    m3Transform xf;
+    if (m3ContinuousSweep(&v, &poses, i, &xf))
+    {
+        d.ccdXf[i] = xf;
+        d.ccdHit[i] = 1;
+    }
+}
+
+__global__ void k_ccd_apply(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.maxBody || v.sc->stall != 0)
+    {
+        return;
+    }
+    if (d.ccdHit[i] != 0)
+    {
+        v.xf[i] = d.ccdXf[i];
     }
 }
 
@@ -2865,10 +2923,16 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.consBodies, 2 * (int64_t)np) || DevAlloc(&v.slotOf, np) ||
         DevAlloc(&v.slotTable, (int64_t)M3_GRAPH_COLORS * nb) || DevAlloc(&v.slotMask, nb) ||
         DevAlloc(&w->dev.barrier, 4) || DevAlloc(&w->dev.narrowBins, 8 + 8 * (int64_t)np) ||
-        DevAlloc(&w->hitRecBuf, np))
+        DevAlloc(&w->hitRecBuf, np) || DevAlloc(&v.shapeRank, ns) || DevAlloc(&v.ccdStatics, ns) ||
+        DevAlloc(&w->dev.ccdXf, nb) || DevAlloc(&w->dev.ccdHit, nb))
     {
         return -1;
     }
+    HIP_TRY(hipHostMalloc((void**)&w->rankStage, (size_t)(ns > 0 ? ns : 1) * sizeof(int32_t),
+                          hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&w->leafStage, (size_t)(ns > 0 ? ns : 1) * 6 * sizeof(double),
+                          hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&w->leavesCopied, hipEventDisableTiming));
     int32_t nj = c.jointCapacity > 0 ? c.jointCapacity : 1;
     if (DevAlloc(&v.jointAlive, nj) || DevAlloc(&v.jointType, nj) || DevAlloc(&v.jointBodyA, nj) ||
         DevAlloc(&v.jointBodyB, nj) || DevAlloc(&v.jointLocalA, nj) ||
@@ -3004,7 +3068,8 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
                     v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
                     v.jointAngularImpulse, v.jointSpringImpulse, v.jointNextA, v.jointNextB,
-                    v.bodyJointHead, v.jointCompStart, v.jointCompList, v.jcons, v.jointActiveFlag};
+                    v.bodyJointHead, v.jointCompStart, v.jointCompList, v.jcons, v.jointActiveFlag,
+                    v.shapeRank, v.ccdStatics, w->dev.ccdXf, w->dev.ccdHit};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     {
         if (ptrs[i] != nullptr)
@@ -3013,6 +3078,18 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
         }
     }
     (void)hipHostFree(w->hostScalars);
+    if (w->rankStage != nullptr)
+    {
+        (void)hipHostFree(w->rankStage);
+    }
+    if (w->leafStage != nullptr)
+    {
+        (void)hipHostFree(w->leafStage);
+    }
+    if (w->leavesCopied != nullptr)
+    {
+        (void)hipEventDestroy(w->leavesCopied);
+    }
     EventBufsFree(w);
     ProfFree(w);
     (void)hipStreamDestroy(w->stream);
@@ -3135,12 +3212,24 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
             UP(v.jointCompStart, h->jointCompStart, h->jointCompCount + 1, int32_t);
             UP(v.jointCompList, h->jointCompList, mj, int32_t);
         }
-        // The plane list and the unsupported-target census are host facts.
+        // The plane list, the static tree shapes (continuous targets) and
+        // the census of what the continuous pass meets are host facts.
         const uint8_t* alive = h->shapeAlive;
         const uint8_t* stype = h->shapeType;
         int32_t* planes = (int32_t*)malloc((size_t)(ns > 0 ? ns : 1) * sizeof(int32_t));
+        int32_t* statics = (int32_t*)malloc((size_t)(ns > 0 ? ns : 1) * sizeof(int32_t));
+        if (planes == nullptr || statics == nullptr)
+        {
+            free(planes);
+            free(statics);
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_upload: out of host memory");
+            return -1;
+        }
         int32_t pc = 0;
+        int32_t sc = 0;
         int unsupported = 0;
+        int needsTree = 0;
+        int bullets = 0;
         for (int32_t s = 0; s < ns; ++s)
         {
             if (alive[s] == 0)
@@ -3155,20 +3244,32 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
             {
                 unsupported = 1;
             }
-            else if (h->types[h->shapeBody[s]] == M3_STATIC && h->shapeSensor[s] == 0)
+            else if (h->types[h->shapeBody[s]] == M3_STATIC)
             {
-                // A static tree target of continuous sweeps (sensors never
-                // stop a sweep, src/solver.c:3102-3105).
-                unsupported = 1;
+                statics[sc++] = s;
+                // Sensors never stop a sweep (src/solver.c:3102-3105).
+                needsTree |= h->shapeSensor[s] == 0;
             }
         }
         for (int32_t b = 0; b < nb; ++b)
         {
             if (h->bodyAlive[b] != 0 && h->bulletFlags[b] != 0)
             {
-                unsupported = 1;
+                needsTree = 1;
+                bullets = 1;
             }
         }
+        w->ccdNeedsTree = needsTree;
+        w->dev.ccdBullets = bullets;
+        v.ccdStaticCount = sc;
+        if (Copy(w, v.ccdStatics, statics, (int64_t)sc * 4, hipMemcpyHostToDevice) != 0)
+        {
+            free(planes);
+            free(statics);
+            return -1;
+        }
+        HIP_TRY(hipStreamSynchronize(w->stream));
+        free(statics);
         v.planeCount = pc;
         w->ccdUnsupported = unsupported;
         int rc = Copy(w, v.planes, planes, (int64_t)pc * 4, hipMemcpyHostToDevice);
@@ -3489,7 +3590,10 @@ extern "C" int m3gpu_events(m3gpuWorld* w, float hitThreshold, m3gpuEvents* out)
 
 // Enqueue one step.  Launch geometry only depends on host-known capacities
 // and high-water marks; every data-dependent count is read on the device.
-static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
+// The step up to the continuous pass; EnqueueStepEnd enqueues the rest.
+// With `leaves` set, a copy of this step's leaf boxes into the pinned stage
+// follows the broadphase (m3gpu_step_leaves waits for it).
+static int EnqueueStepBegin(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
 {
     hipStream_t st = w->stream;
     if (w->devDirty && RefreshDevCopies(w) != 0)
@@ -3536,6 +3640,12 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
               KL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, D));
     }
     STAGE(ST_REFRESH, KL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
+    if (leaves)
+    {
+        HIP_TRY(hipMemcpyAsync(w->leafStage, v.leaf, (size_t)ns * 6 * sizeof(double),
+                               hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipEventRecord(w->leavesCopied, st));
+    }
     STAGE(ST_FROZEN, KL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, D));
     STAGE(ST_EMIT_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ct, tb, v.emit, hd.emitSorted,
                                                                    ne, 0, 32, st)));
@@ -3617,9 +3727,47 @@ static int EnqueueStep(m3gpuWorld* w, const m3gpuStepParams* p)
     {
         STAGE(ST_STORE, KL(k_store_joints, dim3(Blocks(v.maxJoint, T)), dim3(T), 0, st, D));
     }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// The continuous pass and the rest of the step.  `ranks` (maxShape entries,
+// or null) is the depth-first rank of every shape in the broadphase tree
+// after this step's leaf refresh; with it the continuous pass also sweeps
+// against tree shapes.
+static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t* ranks)
+{
+    hipStream_t st = w->stream;
+    const m3Dev* D = w->devDev + w->cur;
+    const m3View& v = w->view;
+    const int T = 256;
+    const int32_t nb = v.maxBody;
+    const int32_t ns = v.maxShape;
+    const int tree = ranks != nullptr ? 1 : 0;
+    if (tree != w->dev.v.treeCcd)
+    {
+        // The flag lives in the device copies of the world's entry.
+        w->dev.v.treeCcd = tree;
+        w->view.treeCcd = tree;
+        if (RefreshDevCopies(w) != 0)
+        {
+            return -1;
+        }
+    }
+    if (tree)
+    {
+        memcpy(w->rankStage, ranks, (size_t)ns * sizeof(int32_t));
+        HIP_TRY(hipMemcpyAsync(v.shapeRank, w->rankStage, (size_t)ns * sizeof(int32_t),
+                               hipMemcpyHostToDevice, st));
+    }
     if (p->continuousEnabled)
     {
         STAGE(ST_CCD, KL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, D));
+        if (w->dev.ccdBullets)
+        {
+            STAGE(ST_CCD, KL(k_ccd_bullets, dim3(Blocks(nb, 64)), dim3(64), 0, st, D));
+            STAGE(ST_CCD, KL(k_ccd_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+        }
     }
     STAGE(ST_SLEEP_TIMERS,
           KL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt, p->sleepEnabled));
@@ -3665,27 +3813,89 @@ static int ProfileCollect(m3gpuWorld* w, int32_t substeps)
     return 0;
 }
 
-extern "C" int m3gpu_step_supported(const m3gpuWorld* w, const m3gpuStepParams* p)
+extern "C" int m3gpu_step_needs_tree(const m3gpuWorld* w, const m3gpuStepParams* p)
 {
-    return w != nullptr && p != nullptr && !(p->continuousEnabled && w->ccdUnsupported);
+    return w != nullptr && p != nullptr && p->continuousEnabled && w->ccdNeedsTree;
 }
 
-extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStats* stats)
+extern "C" int m3gpu_step_supported(const m3gpuWorld* w, const m3gpuStepParams* p)
+{
+    return w != nullptr && p != nullptr && !(p->continuousEnabled && w->ccdUnsupported) &&
+           !m3gpu_step_needs_tree(w, p);
+}
+
+static int StepArgs(const m3gpuWorld* w, const m3gpuStepParams* p, const char* who)
 {
     if (w == nullptr || p == nullptr || !(p->dt > 0.0f) || p->substeps < 1 ||
         p->substeps > M3_MAX_SUBSTEPS)
     {
-        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step: bad arguments");
+        snprintf(s_lastError, sizeof(s_lastError), "%s: bad arguments", who);
         return -1;
     }
-    if (!m3gpu_step_supported(w, p))
+    if (p->continuousEnabled && w->ccdUnsupported)
     {
         snprintf(s_lastError, sizeof(s_lastError),
-                 "m3gpu_step: continuous collision against static non-plane shapes or bullets "
-                 "is not implemented on the device path");
+                 "%s: continuous collision against mesh, voxel or heightfield shapes is not "
+                 "simulated on the device path", who);
         return M3GPU_REFUSED;
     }
-    if (JoinBatch(w) != 0 || EnqueueStep(w, p) != 0)
+    return 0;
+}
+
+extern "C" int m3gpu_step_begin(m3gpuWorld* w, const m3gpuStepParams* p, int32_t wantLeaves)
+{
+    const int rc = StepArgs(w, p, "m3gpu_step_begin");
+    if (rc != 0)
+    {
+        return rc;
+    }
+    if (JoinBatch(w) != 0 || EnqueueStepBegin(w, p, wantLeaves != 0) != 0)
+    {
+        return -1;
+    }
+    w->streamBusy = 1;
+    return 0;
+}
+
+extern "C" const double* m3gpu_step_leaves(m3gpuWorld* w)
+{
+    if (w == nullptr || hipEventSynchronize(w->leavesCopied) != hipSuccess)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step_leaves: the copy failed");
+        return nullptr;
+    }
+    return w->leafStage;
+}
+
+extern "C" int m3gpu_step_end(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t* ranks)
+{
+    const int rc = StepArgs(w, p, "m3gpu_step_end");
+    if (rc != 0)
+    {
+        return rc;
+    }
+    if (EnqueueStepEnd(w, p, ranks) != 0)
+    {
+        return -1;
+    }
+    return ProfileCollect(w, p->substeps);
+}
+
+extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStats* stats)
+{
+    const int rc = StepArgs(w, p, "m3gpu_step");
+    if (rc != 0)
+    {
+        return rc;
+    }
+    if (m3gpu_step_needs_tree(w, p))
+    {
+        snprintf(s_lastError, sizeof(s_lastError),
+                 "m3gpu_step: the continuous pass meets tree shapes; step with m3gpu_step_begin, "
+                 "the tree ranks and m3gpu_step_end");
+        return M3GPU_REFUSED;
+    }
+    if (JoinBatch(w) != 0 || EnqueueStepBegin(w, p, false) != 0 || EnqueueStepEnd(w, p, nullptr) != 0)
     {
         return -1;
     }
@@ -4275,7 +4485,7 @@ extern "C" int m3gpu_step_many(m3gpuWorld* const* worlds, const m3gpuStepParams*
         m3gpuWorld* w = worlds[k];
         const m3gpuStepParams* p = &params[k];
         if (w == nullptr || !(p->dt > 0.0f) || p->substeps < 1 || p->substeps > M3_MAX_SUBSTEPS ||
-            (p->continuousEnabled && w->ccdUnsupported) || !BatchEligible(w) ||
+            !m3gpu_step_supported(w, p) || !BatchEligible(w) ||
             memcmp(p, &params[0], sizeof(*p)) != 0)
         {
             batch = false;
@@ -5032,7 +5242,7 @@ static int AdjointRun(m3gpuWorld* const* ws, const m3gpuStepParams* ps, const m3
         m3gpuWorld* w = ws[k];
         if (w == nullptr || rqs[k].steps != steps || rqs[k].checkpointEvery != r0->checkpointEvery ||
             rqs[k].forwardOnly != forwardOnly || memcmp(&ps[k], &ps[0], sizeof(ps[0])) != 0 ||
-            !BatchEligible(w) || (ps[k].continuousEnabled && w->ccdUnsupported) || steps < 1)
+            !BatchEligible(w) || !m3gpu_step_supported(w, &ps[k]) || steps < 1)
         {
             snprintf(s_lastError, sizeof(s_lastError),
                      "m3gpu_adjoint_many: worlds must share steps, checkpoints and parameters and fit the batched kernels");

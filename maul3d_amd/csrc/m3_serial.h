@@ -140,7 +140,9 @@ typedef struct m3SerialParams
 // The frozen buffer v->sleepKeys holds sorted keys; discovery appends in the
 // reference are merged the same way here: a pair discovered this step joins
 // the frozen set before the merge.
-M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
+// The step up to the continuous pass (m3SerialStepEnd runs the rest);
+// nonzero on a capacity stall (the world did not advance).
+M3_HD int m3SerialStepBegin(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
 {
     m3Scalars* sc = v->sc;
     int32_t nb = v->maxBody;
@@ -629,6 +631,15 @@ M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams
             tape->xfEnd[b] = v->xf[b];
         }
     }
+    return 0;
+}
+
+// The continuous pass (tree candidates when v->treeCcd is set), islands
+// and sleep, and the step count.
+M3_HD void m3SerialStepEnd(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
+{
+    m3Scalars* sc = v->sc;
+    const int32_t nb = v->maxBody;
     if (p->continuousEnabled)
     {
         for (int32_t i = 0; i < nb; ++i)
@@ -689,6 +700,15 @@ M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams
         }
     }
     sc->stepCount += 1;
+}
+
+M3_HD int m3SerialStep(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
+{
+    if (m3SerialStepBegin(v, s, p, tape) != 0)
+    {
+        return 1;
+    }
+    m3SerialStepEnd(v, s, p, tape);
     return 0;
 }
 

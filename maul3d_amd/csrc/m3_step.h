@@ -257,6 +257,13 @@ typedef struct m3View
     int32_t* jointCompList;
     m3JointCon* jcons;
     uint8_t* jointActiveFlag;
+    // continuous collision against tree shapes: each shape's depth-first
+    // rank in the broadphase tree (-1 when not in the tree), valid for the
+    // step when treeCcd is set; the static tree shapes, ascending
+    int32_t* shapeRank;
+    int32_t treeCcd;
+    int32_t ccdStaticCount;
+    int32_t* ccdStatics;
 } m3View;
 
 // ---------------------------------------------------------------------------
@@ -2250,6 +2257,448 @@ M3_HD m3Sweep m3RelativeSweep(const m3View* v, int32_t body, m3Pos3 base)
     return sweep;
 }
 
+// ---------------------------------------------------------------------------
+// Time of impact between two swept proxies: conservative advancement with
+// separation functions, the reference's kernel (src/distance.c:734-1215).
+// Both sweeps are re-centred so A starts at the origin; the state reached
+// (hit, separated, overlapped, failed) and the fraction follow the same
+// branches, so the fraction's bits equal the reference's.
+// ---------------------------------------------------------------------------
+
+enum { M3_TOI_UNKNOWN = 0, M3_TOI_FAILED = 1, M3_TOI_OVERLAPPED = 2, M3_TOI_HIT = 3, M3_TOI_SEPARATED = 4 };
+enum { M3_SEP_VERTICES = 0, M3_SEP_EDGES = 1, M3_SEP_FACE_A = 2, M3_SEP_FACE_B = 3 };
+
+typedef struct m3SepFn
+{
+    const m3Proxy* proxyA;
+    const m3Proxy* proxyB;
+    const m3Sweep* sweepA;
+    const m3Sweep* sweepB;
+    m3Vec3 witness1;
+    m3Vec3 witness2;
+    int32_t type;
+} m3SepFn;
+
+M3_HD m3Vec3 m3ToiPoint(const m3Transform* xf, m3Vec3 p) // src/distance.c:736-740
+{
+    m3Vec3 r = m3RotateVec3(xf->q, p);
+    return m3V3((m3real)xf->p.x + r.x, (m3real)xf->p.y + r.y, (m3real)xf->p.z + r.z);
+}
+
+M3_HD int32_t m3ToiUnique(int32_t count, const int32_t idx[3]) // src/distance.c:742-761
+{
+    if (count == 1)
+    {
+        return 1;
+    }
+    if (count == 2)
+    {
+        return idx[0] != idx[1] ? 2 : 1;
+    }
+    if (idx[0] != idx[1] && idx[0] != idx[2] && idx[1] != idx[2])
+    {
+        return 3;
+    }
+    if (idx[0] == idx[1] && idx[0] == idx[2])
+    {
+        return 1;
+    }
+    return 2;
+}
+
+M3_HD int32_t m3ToiSupport(const m3Proxy* proxy, m3Vec3 d) // src/distance.c:763-776
+{
+    int32_t best = 0;
+    m3real bestDot = m3Dot3(proxy->points[0], d);
+    for (int32_t i = 1; i < proxy->count; ++i)
+    {
+        const m3real dot = m3Dot3(proxy->points[i], d);
+        if (dot > bestDot)
+        {
+            bestDot = dot;
+            best = i;
+        }
+    }
+    return best;
+}
+
+// Does the edge cross product flip across the sweep (src/distance.c:780-789)?
+M3_HD bool m3ToiFastEdges(const m3Sweep* sweepA, const m3Sweep* sweepB, m3Vec3 edgeLocalA,
+                          m3Vec3 edgeLocalB, m3Vec3 axis0)
+{
+    m3Transform xfA = m3SweepTransform(sweepA, 1.0f);
+    m3Transform xfB = m3SweepTransform(sweepB, 1.0f);
+    m3Vec3 axis = m3Cross3(m3RotateVec3(xfA.q, edgeLocalA), m3RotateVec3(xfB.q, edgeLocalB));
+    return m3Dot3(axis, axis0) < 0.0f;
+}
+
+// The separation function of the cached simplex at t1 (src/distance.c:809-929).
+M3_HD m3SepFn m3ToiMakeSep(const m3SimplexCache* cache, const m3Proxy* proxyA, const m3Sweep* sweepA,
+                           const m3Proxy* proxyB, const m3Sweep* sweepB, m3Vec3 worldNormal, m3real t1)
+{
+    m3SepFn f;
+    f.proxyA = proxyA;
+    f.proxyB = proxyB;
+    f.sweepA = sweepA;
+    f.sweepB = sweepB;
+    f.witness1 = worldNormal;
+    f.witness2 = m3V3(0.0f, 0.0f, 0.0f);
+    f.type = M3_SEP_VERTICES;
+    const int32_t ia3[3] = {cache->indexA[0], cache->indexA[1], cache->indexA[2]};
+    const int32_t ib3[3] = {cache->indexB[0], cache->indexB[1], cache->indexB[2]};
+    const int32_t count = cache->count < 3 ? (int32_t)cache->count : 3;
+    const int32_t uniqueA = m3ToiUnique(count, ia3);
+    const int32_t uniqueB = m3ToiUnique(count, ib3);
+    m3Transform xfA1 = m3SweepTransform(sweepA, t1);
+    m3Transform xfB1 = m3SweepTransform(sweepB, t1);
+    const m3Vec3 deltaP = m3V3((m3real)(xfB1.p.x - xfA1.p.x), (m3real)(xfB1.p.y - xfA1.p.y),
+                               (m3real)(xfB1.p.z - xfA1.p.z));
+    if (count == 2 || (count == 3 && uniqueA <= 2 && uniqueB <= 2))
+    {
+        int32_t ea[2];
+        int32_t eb[2];
+        if (count == 2 && uniqueA == 2 && uniqueB == 2)
+        {
+            ea[0] = ia3[0];
+            ea[1] = ia3[1];
+            eb[0] = ib3[0];
+            eb[1] = ib3[1];
+        }
+        else if (count == 3 && uniqueA == 2 && uniqueB == 2)
+        {
+            ea[0] = ia3[0];
+            ea[1] = ia3[0] != ia3[1] ? ia3[1] : ia3[2];
+            eb[0] = ib3[0];
+            eb[1] = ib3[0] != ib3[1] ? ib3[1] : ib3[2];
+        }
+        else
+        {
+            return f; // vertex against edge: the world axis
+        }
+        const m3Vec3 vA1 = proxyA->points[ea[0]];
+        m3Vec3 edgeLocalA = m3Normalize3(m3Sub3(proxyA->points[ea[1]], vA1));
+        const m3Vec3 edgeA = m3RotateVec3(xfA1.q, edgeLocalA);
+        const m3Vec3 vB1 = proxyB->points[eb[0]];
+        m3Vec3 edgeLocalB = m3Normalize3(m3Sub3(proxyB->points[eb[1]], vB1));
+        const m3Vec3 edgeB = m3RotateVec3(xfB1.q, edgeLocalB);
+        m3Vec3 axis = m3Cross3(edgeA, edgeB);
+        const m3real len2 = m3Dot3(axis, axis);
+        const m3real tolerance2 = count == 2 ? 0.05f * 0.05f : 0.005f * 0.005f;
+        if (len2 < tolerance2)
+        {
+            return f; // near parallel: the world axis
+        }
+        const m3Vec3 delta =
+            m3Add3(m3Sub3(m3RotateVec3(xfB1.q, vB1), m3RotateVec3(xfA1.q, vA1)), deltaP);
+        if (m3Dot3(delta, axis) < 0.0f)
+        {
+            axis = m3Neg3(axis);
+            edgeLocalB = m3Neg3(edgeLocalB);
+        }
+        if (m3ToiFastEdges(sweepA, sweepB, edgeLocalA, edgeLocalB, axis))
+        {
+            f.witness1 = m3Normalize3(axis); // unsafe: a fixed world axis
+            return f;
+        }
+        f.type = M3_SEP_EDGES;
+        f.witness1 = edgeLocalA;
+        f.witness2 = edgeLocalB;
+        return f;
+    }
+    if (count == 3 && uniqueA == 3)
+    {
+        const m3Vec3 vA1 = proxyA->points[ia3[0]];
+        const m3Vec3 vA2 = proxyA->points[ia3[1]];
+        const m3Vec3 vA3 = proxyA->points[ia3[2]];
+        m3Vec3 axisLocal = m3Normalize3(m3Cross3(m3Sub3(vA2, vA1), m3Sub3(vA3, vA1)));
+        const m3Vec3 axisA = m3RotateVec3(xfA1.q, axisLocal);
+        const m3Vec3 pointLocalA = m3MulSV3(1.0f / 3.0f, m3Add3(m3Add3(vA1, vA2), vA3));
+        const m3Vec3 pointLocalB = proxyB->points[ib3[0]];
+        const m3Vec3 delta = m3Add3(
+            m3Sub3(m3RotateVec3(xfB1.q, pointLocalB), m3RotateVec3(xfA1.q, pointLocalA)), deltaP);
+        if (m3Dot3(delta, axisA) < 0.0f)
+        {
+            axisLocal = m3Neg3(axisLocal);
+        }
+        f.type = M3_SEP_FACE_A;
+        f.witness1 = axisLocal;
+        f.witness2 = pointLocalA;
+        return f;
+    }
+    if (count == 3 && uniqueB == 3)
+    {
+        const m3Vec3 vB1 = proxyB->points[ib3[0]];
+        const m3Vec3 vB2 = proxyB->points[ib3[1]];
+        const m3Vec3 vB3 = proxyB->points[ib3[2]];
+        m3Vec3 axisLocal = m3Normalize3(m3Cross3(m3Sub3(vB2, vB1), m3Sub3(vB3, vB1)));
+        const m3Vec3 axisB = m3RotateVec3(xfB1.q, axisLocal);
+        const m3Vec3 pointLocalA = proxyA->points[ia3[0]];
+        const m3Vec3 pointLocalB = m3MulSV3(1.0f / 3.0f, m3Add3(m3Add3(vB1, vB2), vB3));
+        const m3Vec3 delta = m3Sub3(
+            m3Sub3(m3RotateVec3(xfA1.q, pointLocalA), m3RotateVec3(xfB1.q, pointLocalB)), deltaP);
+        if (m3Dot3(delta, axisB) < 0.0f)
+        {
+            axisLocal = m3Neg3(axisLocal);
+        }
+        f.type = M3_SEP_FACE_B;
+        f.witness1 = axisLocal;
+        f.witness2 = pointLocalB;
+        return f;
+    }
+    return f; // one vertex or a mixed simplex: the world axis
+}
+
+// The deepest points along the function's axis at time t
+// (src/distance.c:931-984).
+M3_HD m3real m3ToiMinSep(const m3SepFn* f, int32_t* indexA, int32_t* indexB, m3real t)
+{
+    m3Transform xfA = m3SweepTransform(f->sweepA, t);
+    m3Transform xfB = m3SweepTransform(f->sweepB, t);
+    const m3Vec3 deltaP =
+        m3V3((m3real)(xfB.p.x - xfA.p.x), (m3real)(xfB.p.y - xfA.p.y), (m3real)(xfB.p.z - xfA.p.z));
+    switch (f->type)
+    {
+    case M3_SEP_VERTICES:
+    {
+        const m3Vec3 axis = f->witness1;
+        *indexA = m3ToiSupport(f->proxyA, m3InvRotateVec3(xfA.q, axis));
+        *indexB = m3ToiSupport(f->proxyB, m3InvRotateVec3(xfB.q, m3Neg3(axis)));
+        const m3Vec3 delta = m3Add3(m3Sub3(m3RotateVec3(xfB.q, f->proxyB->points[*indexB]),
+                                           m3RotateVec3(xfA.q, f->proxyA->points[*indexA])),
+                                    deltaP);
+        return m3Dot3(delta, axis);
+    }
+    case M3_SEP_EDGES:
+    {
+        const m3Vec3 edgeA = m3RotateVec3(xfA.q, f->witness1);
+        const m3Vec3 edgeB = m3RotateVec3(xfB.q, f->witness2);
+        const m3Vec3 axis = m3Normalize3(m3Cross3(edgeA, edgeB));
+        *indexA = m3ToiSupport(f->proxyA, m3InvRotateVec3(xfA.q, axis));
+        *indexB = m3ToiSupport(f->proxyB, m3InvRotateVec3(xfB.q, m3Neg3(axis)));
+        const m3Vec3 delta = m3Add3(m3Sub3(m3RotateVec3(xfB.q, f->proxyB->points[*indexB]),
+                                           m3RotateVec3(xfA.q, f->proxyA->points[*indexA])),
+                                    deltaP);
+        return m3Dot3(delta, axis);
+    }
+    case M3_SEP_FACE_A:
+    {
+        const m3Vec3 normal = m3RotateVec3(xfA.q, f->witness1);
+        *indexA = -1;
+        const m3Vec3 pointA = m3ToiPoint(&xfA, f->witness2);
+        *indexB = m3ToiSupport(f->proxyB, m3InvRotateVec3(xfB.q, m3Neg3(normal)));
+        const m3Vec3 pointB = m3ToiPoint(&xfB, f->proxyB->points[*indexB]);
+        return m3Dot3(m3Sub3(pointB, pointA), normal);
+    }
+    default:
+    {
+        const m3Vec3 normal = m3RotateVec3(xfB.q, f->witness1);
+        *indexA = m3ToiSupport(f->proxyA, m3InvRotateVec3(xfA.q, m3Neg3(normal)));
+        const m3Vec3 pointA = m3ToiPoint(&xfA, f->proxyA->points[*indexA]);
+        *indexB = -1;
+        const m3Vec3 pointB = m3ToiPoint(&xfB, f->witness2);
+        return m3Dot3(m3Sub3(pointA, pointB), normal);
+    }
+    }
+}
+
+// The separation of a fixed point pair at time t (src/distance.c:986-1027).
+M3_HD m3real m3ToiEvalSep(const m3SepFn* f, int32_t indexA, int32_t indexB, m3real t)
+{
+    m3Transform xfA = m3SweepTransform(f->sweepA, t);
+    m3Transform xfB = m3SweepTransform(f->sweepB, t);
+    switch (f->type)
+    {
+    case M3_SEP_VERTICES:
+    {
+        const m3Vec3 pointA = m3ToiPoint(&xfA, f->proxyA->points[indexA]);
+        const m3Vec3 pointB = m3ToiPoint(&xfB, f->proxyB->points[indexB]);
+        return m3Dot3(m3Sub3(pointB, pointA), f->witness1);
+    }
+    case M3_SEP_EDGES:
+    {
+        const m3Vec3 edgeA = m3RotateVec3(xfA.q, f->witness1);
+        const m3Vec3 edgeB = m3RotateVec3(xfB.q, f->witness2);
+        const m3Vec3 axis = m3Normalize3(m3Cross3(edgeA, edgeB));
+        const m3Vec3 pointA = m3ToiPoint(&xfA, f->proxyA->points[indexA]);
+        const m3Vec3 pointB = m3ToiPoint(&xfB, f->proxyB->points[indexB]);
+        return m3Dot3(m3Sub3(pointB, pointA), axis);
+    }
+    case M3_SEP_FACE_A:
+    {
+        const m3Vec3 axis = m3RotateVec3(xfA.q, f->witness1);
+        const m3Vec3 pointA = m3ToiPoint(&xfA, f->witness2);
+        const m3Vec3 pointB = m3ToiPoint(&xfB, f->proxyB->points[indexB]);
+        return m3Dot3(m3Sub3(pointB, pointA), axis);
+    }
+    default:
+    {
+        const m3Vec3 axis = m3RotateVec3(xfB.q, f->witness1);
+        const m3Vec3 pointA = m3ToiPoint(&xfA, f->proxyA->points[indexA]);
+        const m3Vec3 pointB = m3ToiPoint(&xfB, f->witness2);
+        return m3Dot3(m3Sub3(pointA, pointB), axis);
+    }
+    }
+}
+
+// An edge function whose root search failed becomes a fixed world axis
+// (src/distance.c:1029-1037).
+M3_HD void m3ToiFixAxis(m3SepFn* f, m3real t)
+{
+    m3Transform xfA = m3SweepTransform(f->sweepA, t);
+    m3Transform xfB = m3SweepTransform(f->sweepB, t);
+    const m3Vec3 edgeA = m3RotateVec3(xfA.q, f->witness1);
+    const m3Vec3 edgeB = m3RotateVec3(xfB.q, f->witness2);
+    f->witness1 = m3Normalize3(m3Cross3(edgeA, edgeB));
+    f->witness2 = m3V3(0.0f, 0.0f, 0.0f);
+    f->type = M3_SEP_VERTICES;
+}
+
+// m3TimeOfImpact (src/distance.c:1039-1215); returns the state and writes the
+// fraction.
+M3_HD int32_t m3TimeOfImpact(const m3Proxy* proxyA, m3Sweep sweepA, const m3Proxy* proxyB, m3Sweep sweepB,
+                             m3real tMax, m3real* fractionOut)
+{
+    int32_t state = M3_TOI_UNKNOWN;
+    m3real fraction = tMax;
+    const m3Vec3 origin = sweepA.c1;
+    sweepA.c1 = m3Sub3(sweepA.c1, origin);
+    sweepA.c2 = m3Sub3(sweepA.c2, origin);
+    sweepB.c1 = m3Sub3(sweepB.c1, origin);
+    sweepB.c2 = m3Sub3(sweepB.c2, origin);
+    const m3real linearSlop = M3_LINEAR_SLOP;
+    const m3real totalRadius = proxyA->radius + proxyB->radius;
+    const m3real target = m3MaxF(linearSlop, totalRadius - linearSlop);
+    const m3real tolerance = 0.25f * linearSlop;
+    const int32_t maxPushBack = proxyA->count + proxyB->count;
+    m3real t1 = 0.0f;
+    int32_t distanceIterations = 0;
+    m3SimplexCache cache;
+    cache.count = 0;
+    cache.metric = 0.0f;
+    for (;;)
+    {
+        m3Transform xfA = m3SweepTransform(&sweepA, t1);
+        m3Transform xfB = m3SweepTransform(&sweepB, t1);
+        const m3Quat conjA = m3Q(-xfA.q.x, -xfA.q.y, -xfA.q.z, xfA.q.w);
+        const m3Quat q = m3MulQuat(conjA, xfB.q);
+        const m3Vec3 dp = m3V3((m3real)(xfB.p.x - xfA.p.x), (m3real)(xfB.p.y - xfA.p.y),
+                               (m3real)(xfB.p.z - xfA.p.z));
+        const m3Vec3 p = m3InvRotateVec3(xfA.q, dp);
+        const m3GjkOut d = m3ShapeDistanceCached(proxyA, proxyB, q, p, &cache);
+        distanceIterations += 1;
+        if (d.distance <= 0.0f)
+        {
+            state = M3_TOI_OVERLAPPED;
+            fraction = 0.0f;
+            break;
+        }
+        if (d.distance <= target + tolerance)
+        {
+            state = M3_TOI_HIT;
+            fraction = t1;
+            break;
+        }
+        if (distanceIterations == 25)
+        {
+            state = M3_TOI_FAILED;
+            fraction = t1;
+            break;
+        }
+        const m3Vec3 worldNormal = m3RotateVec3(xfA.q, d.normal);
+        m3SepFn f = m3ToiMakeSep(&cache, proxyA, &sweepA, proxyB, &sweepB, worldNormal, t1);
+        bool done = false;
+        m3real t2 = tMax;
+        int32_t pushBack = 0;
+        for (;;)
+        {
+            int32_t indexA;
+            int32_t indexB;
+            m3real s2 = m3ToiMinSep(&f, &indexA, &indexB, t2);
+            if (s2 - target > tolerance)
+            {
+                state = M3_TOI_SEPARATED;
+                fraction = tMax;
+                done = true;
+                break;
+            }
+            if (s2 >= target - tolerance)
+            {
+                t1 = t2;
+                break;
+            }
+            m3real s1 = m3ToiEvalSep(&f, indexA, indexB, t1);
+            if (s1 < target - tolerance)
+            {
+                state = M3_TOI_FAILED;
+                fraction = t1;
+                done = true;
+                break;
+            }
+            if (s1 <= target + tolerance)
+            {
+                state = M3_TOI_HIT;
+                fraction = t1;
+                done = true;
+                break;
+            }
+            // False position alternating with bisection.
+            int32_t rootIterations = 0;
+            m3real a1 = t1;
+            m3real a2 = t2;
+            for (;;)
+            {
+                m3real t;
+                if (rootIterations & 1)
+                {
+                    t = a1 + (target - s1) * (a2 - a1) / (s2 - s1);
+                }
+                else
+                {
+                    t = 0.5f * (a1 + a2);
+                }
+                rootIterations += 1;
+                const m3real sep = m3ToiEvalSep(&f, indexA, indexB, t);
+                const m3real err = sep - target;
+                if ((err < 0.0f ? -err : err) <= tolerance)
+                {
+                    t2 = t;
+                    break;
+                }
+                if (sep > target)
+                {
+                    a1 = t;
+                    s1 = sep;
+                }
+                else
+                {
+                    a2 = t;
+                    s2 = sep;
+                }
+                if (rootIterations == 50)
+                {
+                    break;
+                }
+            }
+            if (rootIterations == 49 && f.type == M3_SEP_EDGES)
+            {
+                t2 = tMax;
+                m3ToiFixAxis(&f, t1);
+            }
+            pushBack += 1;
+            if (pushBack == maxPushBack)
+            {
+                break;
+            }
+        }
+        if (done)
+        {
+            break;
+        }
+    }
+    *fractionOut = fraction;
+    return state;
+}
+
 M3_HD m3real m3ContinuousVersusPlane(const m3View* v, int32_t fastBody, int32_t fastShape,
                                      const m3Sweep* sweep, m3Pos3 base, int32_t planeShape,
                                      m3real fraction)
@@ -2304,10 +2753,125 @@ M3_HD m3real m3ContinuousVersusPlane(const m3View* v, int32_t fastBody, int32_t 
     return fraction;
 }
 
-// The fast test and the plane sweeps for one body; returns 1 when the body
-// was pulled back.  Static non-plane targets are not supported on the
-// device path (none exist in this build's scenes): see the orchestrator.
-M3_HD int m3ContinuousBody(const m3View* v, int32_t i)
+// Poses the continuous pass sees for its targets.  The reference sweeps the
+// bodies one by one in slot order and pulls each back at once
+// (src/solver.c:3349-3430), so a bullet sweeping against a dynamic target
+// of a lower slot sees that target's pulled-back pose.  The serial form
+// applies pull-backs as it goes (poses = null: the current transforms);
+// the parallel form computes the non-bullet bodies first into `pulled` /
+// `hit` and gives bullets the poses of lower slots from there.
+typedef struct m3CcdPoses
+{
+    const m3Transform* pulled;
+    const uint8_t* hit;
+} m3CcdPoses;
+
+M3_HD m3Transform m3CcdPose(const m3View* v, const m3CcdPoses* poses, int32_t body, int32_t fastBody)
+{
+    if (poses != NULL && body < fastBody && poses->hit[body] != 0)
+    {
+        return poses->pulled[body];
+    }
+    return v->xf[body];
+}
+
+// MakeRelativeSweep for a body at pose `xf` (src/solver.c:3065-3079).
+M3_HD m3Sweep m3RelativeSweepAt(const m3View* v, int32_t body, m3Transform xf, m3Pos3 base)
+{
+    m3Sweep sweep;
+    sweep.localCenter = v->lc[body];
+    sweep.c1 = m3V3((m3real)(v->com0[body].x - base.x), (m3real)(v->com0[body].y - base.y),
+                    (m3real)(v->com0[body].z - base.z));
+    m3Vec3 rlc = m3RotateVec3(xf.q, v->lc[body]);
+    sweep.c2 = m3V3((m3real)(xf.p.x + (double)rlc.x - base.x), (m3real)(xf.p.y + (double)rlc.y - base.y),
+                    (m3real)(xf.p.z + (double)rlc.z - base.z));
+    sweep.q1 = v->rot0[body];
+    sweep.q2 = xf.q;
+    return sweep;
+}
+
+// The next tree candidate of a continuous query: among the shapes whose tree
+// leaf overlaps [lo, hi], the one of smallest depth-first rank above
+// `after` (-1 when none).  The reference meets its candidates in exactly
+// this order (a depth-first walk visits the leaves it reaches in the full
+// walk's order).  Non-bullet fast bodies only ever hit static targets, so
+// their scan runs over the static tree shapes only.
+M3_HD int32_t m3CcdNextCandidate(const m3View* v, const double box[6], int32_t after, int allShapes,
+                                 int32_t* rankOut)
+{
+    int32_t best = -1;
+    int32_t bestRank = 0x7FFFFFFF;
+    const int32_t n = allShapes ? v->maxShape : v->ccdStaticCount;
+    for (int32_t k = 0; k < n; ++k)
+    {
+        const int32_t c = allShapes ? k : v->ccdStatics[k];
+        const int32_t r = v->shapeRank[c];
+        if (r <= after || r >= bestRank)
+        {
+            continue;
+        }
+        if (!m3BoxOverlap(&v->leaf[6 * c], box))
+        {
+            continue;
+        }
+        best = c;
+        bestRank = r;
+    }
+    *rankOut = bestRank;
+    return best;
+}
+
+// One tree candidate of ContinuousQueryCallback (src/solver.c:3081-3278):
+// the filters in the reference's order, then the time of impact with the
+// fraction threaded through.
+M3_HD m3real m3CcdVersusShape(const m3View* v, const m3CcdPoses* poses, int32_t fastBody, int32_t fastShape,
+                              const m3Sweep* fastSweep, m3Pos3 base, int32_t shape, m3real fraction)
+{
+    if (shape == fastShape)
+    {
+        return fraction;
+    }
+    const int32_t body = v->shapeBody[shape];
+    if (body == fastBody || v->enabled[body] == 0 || v->bullet[body] != 0 || v->sensor[shape] != 0)
+    {
+        return fraction;
+    }
+    if (v->type[body] != M3_STATIC && v->bullet[fastBody] == 0)
+    {
+        return fraction; // only bullets sweep against dynamic and kinematic bodies
+    }
+    const int32_t gi = v->group[shape];
+    if (gi != 0 && gi == v->group[fastShape])
+    {
+        if (gi < 0)
+        {
+            return fraction;
+        }
+    }
+    else if (!((v->category[shape] & v->mask[fastShape]) != 0 &&
+               (v->category[fastShape] & v->mask[shape]) != 0))
+    {
+        return fraction;
+    }
+    m3Vec3 scratchA[2];
+    m3Vec3 scratchB[2];
+    const m3Proxy proxyA = m3MakeProxy(v, shape, scratchA);
+    const m3Proxy proxyB = m3MakeProxy(v, fastShape, scratchB);
+    const m3Sweep sweepA = m3RelativeSweepAt(v, body, m3CcdPose(v, poses, body, fastBody), base);
+    m3real t = fraction;
+    const int32_t state = m3TimeOfImpact(&proxyA, sweepA, &proxyB, *fastSweep, fraction, &t);
+    if (state == M3_TOI_HIT && 0.0f < t && t < fraction)
+    {
+        return t;
+    }
+    return fraction;
+}
+
+// The fast test, the tree and plane sweeps of one dynamic body
+// (src/solver.c:3349-3430).  Returns 1 and the pulled-back pose when the
+// body is pulled back; nothing is written to the view.  Tree candidates are
+// swept only when the step carries the tree's leaf ranks (v->treeCcd).
+M3_HD int m3ContinuousSweep(const m3View* v, const m3CcdPoses* poses, int32_t i, m3Transform* pulled)
 {
     if (v->bodyAlive[i] == 0 || v->type[i] != M3_DYNAMIC)
     {
@@ -2330,12 +2894,36 @@ M3_HD int m3ContinuousBody(const m3View* v, int32_t i)
     }
     m3Pos3 base = c0;
     m3real fraction = 1.0f;
-    m3Sweep sweep = m3RelativeSweep(v, i, base);
+    m3Sweep sweep = m3RelativeSweepAt(v, i, xfi, base);
+    // Swept candidate box: both centre-of-mass endpoints padded by the
+    // body's largest extent.
+    const double pad = (double)(v->maxExt[i] + M3_AABB_MARGIN);
+    double box[6];
+    box[0] = (c0.x < cx ? c0.x : cx) - pad;
+    box[1] = (c0.y < cy ? c0.y : cy) - pad;
+    box[2] = (c0.z < cz ? c0.z : cz) - pad;
+    box[3] = (c0.x > cx ? c0.x : cx) + pad;
+    box[4] = (c0.y > cy ? c0.y : cy) + pad;
+    box[5] = (c0.z > cz ? c0.z : cz) + pad;
+    const int allShapes = v->bullet[i] != 0;
     for (int32_t s = v->shapeHead[i]; s != -1; s = v->shapeNext[s])
     {
         if (v->sensor[s] != 0)
         {
             continue;
+        }
+        if (v->treeCcd != 0)
+        {
+            int32_t rank = -1;
+            for (;;)
+            {
+                const int32_t c = m3CcdNextCandidate(v, box, rank, allShapes, &rank);
+                if (c < 0)
+                {
+                    break;
+                }
+                fraction = m3CcdVersusShape(v, poses, i, s, &sweep, base, c, fraction);
+            }
         }
         for (int32_t pi = 0; pi < v->planeCount; ++pi)
         {
@@ -2345,10 +2933,23 @@ M3_HD int m3ContinuousBody(const m3View* v, int32_t i)
     if (fraction < 1.0f)
     {
         m3Transform xf = m3SweepTransform(&sweep, fraction);
-        v->xf[i].q = xf.q;
-        v->xf[i].p.x = base.x + xf.p.x;
-        v->xf[i].p.y = base.y + xf.p.y;
-        v->xf[i].p.z = base.z + xf.p.z;
+        pulled->q = xf.q;
+        pulled->p.x = base.x + xf.p.x;
+        pulled->p.y = base.y + xf.p.y;
+        pulled->p.z = base.z + xf.p.z;
+        return 1;
+    }
+    return 0;
+}
+
+// The serial form: sweep and apply in place (bodies in slot order give the
+// reference's sequence).
+M3_HD int m3ContinuousBody(const m3View* v, int32_t i)
+{
+    m3Transform xf;
+    if (m3ContinuousSweep(v, NULL, i, &xf))
+    {
+        v->xf[i] = xf;
         return 1;
     }
     return 0;

@@ -18,6 +18,7 @@
 #include "maul3d_amd.h"
 #include "m3gpu.h"
 #include "m3_hull.h"
+#include "m3_tree.h"
 
 #include <stddef.h>
 #include <stdio.h>
@@ -155,6 +156,13 @@ typedef struct m3World
     int32_t bodyCapacity;
     int32_t shapeCapacity;
     int32_t jointCapacity;
+    // capacities of the def's subsystems this build does not simulate; they
+    // size the empty blocks of the reference's snapshot layout
+    int32_t meshCapacity;
+    int32_t voxelCapacity;
+    int32_t characterCapacity;
+    int32_t vehicleCapacity;
+    int32_t softBodyCapacity;
     uint16_t generation;
     uint16_t worldIndex0;
 
@@ -209,6 +217,19 @@ typedef struct m3World
     int32_t* shapeMeshIndex;
     uint8_t* shapeSensor;
     double* leafBoxes; // the stored proxy box of every tree shape
+
+    // The reference's broadphase tree, mirrored on the host (m3_tree.h);
+    // proxyIds[s] is shape s's leaf node (-1: planes).  Shape creation
+    // inserts; while the mirror is on (m3World_EnableTreeMirror, or a
+    // continuous pass that sweeps against tree shapes) every step replays
+    // the device's leaf moves in shape order.  A step taken without it
+    // leaves the tree stale: the next use rebuilds it canonically from the
+    // current leaves (the reference's m3World_RebuildBroadphase).
+    m3Tree tree;
+    int32_t* proxyIds;
+    int32_t* treeRanks;
+    uint8_t treeMirror;
+    uint8_t treeStale;
 
     m3IdPool hullPool;
     m3HullData* hullData;
@@ -564,6 +585,7 @@ enum
     M3_OP_ENABLE_CONTINUOUS = 48,
     M3_OP_SET_HIT_EVENT_THRESHOLD = 49,
     M3_OP_ENABLE_SHAPE_HIT_EVENTS = 50,
+    M3_OP_REBUILD_BROADPHASE = 73,
 };
 
 #define M3_HULL_MAX_INPUT 256
@@ -822,6 +844,11 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
     w->bodyCapacity = def->bodyCapacity;
     w->shapeCapacity = def->shapeCapacity;
     w->jointCapacity = def->jointCapacity;
+    w->meshCapacity = def->meshCapacity;
+    w->voxelCapacity = def->voxelCapacity;
+    w->characterCapacity = def->characterCapacity;
+    w->vehicleCapacity = def->vehicleCapacity;
+    w->softBodyCapacity = def->softBodyCapacity;
     w->generation = s_worldGenerations[slot];
     w->worldIndex0 = (uint16_t)slot;
 
@@ -891,6 +918,17 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
         w->shapeHullIndex[i] = -1;
         w->shapeMeshIndex[i] = -1;
     }
+    if (!m3TreeCreate(&w->tree, 2 * shapeCap))
+    {
+        fprintf(stderr, "maul3d_amd: out of host memory\n");
+        abort();
+    }
+    ALLOC(w->proxyIds, shapeCap, int32_t);
+    ALLOC(w->treeRanks, shapeCap, int32_t);
+    for (int32_t i = 0; i < shapeCap; ++i)
+    {
+        w->proxyIds[i] = M3_TREE_NULL;
+    }
     w->hullPool = PoolCreate(shapeCap);
     ALLOC(w->hullData, shapeCap, m3HullData);
     ALLOC(w->hullRefCounts, shapeCap, int32_t);
@@ -927,9 +965,6 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
         w->jointBodyB[j] = -1;
         w->jointNextA[j] = -1;
         w->jointNextB[j] = -1;
-        w->jointFrameQA[j].w = 1.0f;
-        w->jointFrameQB[j].w = 1.0f;
-        w->jointTargetQ[j].w = 1.0f;
     }
 
     w->pairCapacity = 8 * shapeCap;
@@ -973,13 +1008,14 @@ void m3DestroyWorld(m3WorldId worldId)
         w->jointAngularImpulse, w->jointFrameQA, w->jointFrameQB, w->jointFlags, w->jointMotor,
         w->jointBreak, w->jointSpring, w->jointTargetScalar, w->jointTargetQ,
         w->jointSpringImpulse, w->jointLimits, w->jointNextA, w->jointNextB, w->bodyJointHead,
-        w->jointCompStart, w->jointCompList, w->jointScratch,
+        w->jointCompStart, w->jointCompList, w->jointScratch, w->proxyIds, w->treeRanks,
     };
     for (size_t i = 0; i < sizeof(arrays) / sizeof(arrays[0]); ++i)
     {
         free(arrays[i]);
     }
     FreeEvents(w);
+    m3TreeDestroy(&w->tree);
     PoolDestroy(&w->bodyPool);
     PoolDestroy(&w->shapePool);
     PoolDestroy(&w->hullPool);
@@ -1787,6 +1823,9 @@ static m3ShapeId CreateShape(m3BodyId bodyId, const m3ShapeDef* def, uint8_t typ
     if (type != M3_PLANE)
     {
         ShapeBounds(w, s, &w->leafBoxes[6 * s]);
+        // The tree leaf (src/shape.c:592-599); the pool holds two nodes per
+        // shape slot, so the insert cannot run out.
+        w->proxyIds[s] = m3TreeInsert(&w->tree, &w->leafBoxes[6 * s], &w->leafBoxes[6 * s + 3], s);
     }
     RecomputeMass(w, b);
     m3ShapeId id = {s + 1, w->worldIndex0, w->shapePool.generations[s]};
@@ -2253,6 +2292,97 @@ static void FillParams(const m3World* w, float dt, int32_t substeps, m3gpuStepPa
     p->continuousEnabled = w->continuousEnabled;
 }
 
+// ---------------------------------------------------------------------------
+// Broadphase tree mirror (m3_tree.h)
+// ---------------------------------------------------------------------------
+
+// The reference's canonical rebuild (src/world.c:1140-1198): live proxies in
+// shape order with their current leaf boxes.  The host's leaf boxes must be
+// current (SyncHost).
+static void TreeRebuild(m3World* w)
+{
+    const int32_t ns = w->shapePool.maxIndex;
+    int32_t count = 0;
+    for (int32_t s = 0; s < ns; ++s)
+    {
+        count += w->shapePool.alive[s] != 0 && w->proxyIds[s] >= 0;
+    }
+    w->treeStale = 0;
+    if (count == 0)
+    {
+        return;
+    }
+    double (*los)[3] = (double (*)[3])Zalloc((size_t)count * 3 * sizeof(double));
+    double (*his)[3] = (double (*)[3])Zalloc((size_t)count * 3 * sizeof(double));
+    int32_t* uds = (int32_t*)Zalloc((size_t)count * sizeof(int32_t));
+    int32_t* nodes = (int32_t*)Zalloc((size_t)count * sizeof(int32_t));
+    int32_t n = 0;
+    for (int32_t s = 0; s < ns; ++s)
+    {
+        if (w->shapePool.alive[s] == 0 || w->proxyIds[s] < 0)
+        {
+            continue;
+        }
+        memcpy(los[n], &w->leafBoxes[6 * s], 3 * sizeof(double));
+        memcpy(his[n], &w->leafBoxes[6 * s + 3], 3 * sizeof(double));
+        uds[n] = s;
+        n += 1;
+    }
+    if (m3TreeRebuild(&w->tree, (const double (*)[3])los, (const double (*)[3])his, uds, n, nodes))
+    {
+        for (int32_t i = 0; i < n; ++i)
+        {
+            w->proxyIds[uds[i]] = nodes[i];
+        }
+    }
+    free(los);
+    free(his);
+    free(uds);
+    free(nodes);
+}
+
+// Replays one step's leaf moves (src/broad_phase.c:505-539): every shape
+// whose leaf box changed escaped its old leaf, and is removed and reinserted
+// with the new box, in shape order.  A leaf box changes exactly when the
+// shape escaped (the new box contains the tight box the old one did not).
+static void TreeReplay(m3World* w, const double* leaves)
+{
+    const int32_t ns = w->shapePool.maxIndex;
+    for (int32_t s = 0; s < ns; ++s)
+    {
+        const int32_t leaf = w->proxyIds[s];
+        if (w->shapePool.alive[s] == 0 || leaf < 0)
+        {
+            continue;
+        }
+        const m3TreeNode* node = &w->tree.nodes[leaf];
+        const double* box = &leaves[6 * s];
+        if (memcmp(node->lo, box, 3 * sizeof(double)) == 0 &&
+            memcmp(node->hi, box + 3, 3 * sizeof(double)) == 0)
+        {
+            continue;
+        }
+        m3TreeRemove(&w->tree, leaf);
+        w->proxyIds[s] = m3TreeInsert(&w->tree, box, box + 3, s);
+    }
+}
+
+static bool TreeHasLeaves(const m3World* w) { return w->tree.root != M3_TREE_NULL; }
+
+// The mirror is current from here on (rebuilding a stale one).
+static bool TreeFresh(m3World* w)
+{
+    if (w->treeStale != 0)
+    {
+        if (!SyncHost(w))
+        {
+            return false;
+        }
+        TreeRebuild(w);
+    }
+    return true;
+}
+
 // One step; false when the step did not run (bad arguments, a faulted world,
 // or a world the device step refuses), in which case nothing changed.
 static bool StepWorld(m3World* w, float dt, int32_t substeps)
@@ -2267,7 +2397,50 @@ static bool StepWorld(m3World* w, float dt, int32_t substeps)
     }
     m3gpuStepParams p;
     FillParams(w, dt, substeps, &p);
-    int rc = m3gpu_step(w->dev, &p, NULL);
+    const bool treeCcd = m3gpu_step_needs_tree(w->dev, &p) != 0;
+    int rc;
+    if (treeCcd || w->treeMirror != 0)
+    {
+        // The device stops before the continuous pass; the host replays
+        // the step's leaf moves and hands the tree's depth-first leaf
+        // ranks to the continuous pass.
+        if (!TreeFresh(w))
+        {
+            return false;
+        }
+        rc = m3gpu_step_begin(w->dev, &p, 1);
+        if (rc == 0)
+        {
+            const double* leaves = m3gpu_step_leaves(w->dev);
+            if (leaves == NULL)
+            {
+                DeviceFail(w, "m3gpu_step_leaves");
+                return false;
+            }
+            TreeReplay(w, leaves);
+            const int32_t* ranks = NULL;
+            if (treeCcd)
+            {
+                // The reference's query stack holds 64 entries; deeper
+                // trees would drop leaves from its walks.
+                if (m3TreeLeafRanks(&w->tree, w->treeRanks, w->shapePool.maxIndex) > 64)
+                {
+                    fprintf(stderr, "maul3d_amd: broadphase tree deeper than the reference's "
+                                    "query stack; continuous candidates may differ\n");
+                }
+                ranks = w->treeRanks;
+            }
+            rc = m3gpu_step_end(w->dev, &p, ranks);
+        }
+    }
+    else
+    {
+        rc = m3gpu_step(w->dev, &p, NULL);
+        if (rc == 0 && TreeHasLeaves(w))
+        {
+            w->treeStale = 1;
+        }
+    }
     if (rc == M3GPU_REFUSED)
     {
         return false; // a world the device step does not support: no-op
@@ -2297,6 +2470,44 @@ void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
     (void)StepWorld(WorldFromId(worldId), dt, substeps);
 }
 
+bool m3World_EnableTreeMirror(m3WorldId worldId, bool flag)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || w->faulted != 0)
+    {
+        return false;
+    }
+    if (flag && !TreeFresh(w))
+    {
+        return false;
+    }
+    w->treeMirror = flag ? 1 : 0;
+    return true;
+}
+
+bool m3World_IsTreeMirrorCurrent(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    return w != NULL && w->treeStale == 0;
+}
+
+void m3World_RebuildBroadphase(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL || !SyncHost(w))
+    {
+        return;
+    }
+    if (w->journalActive != 0)
+    {
+        int32_t zero = 0;
+        JournalRecord(w, M3_OP_REBUILD_BROADPHASE, &zero, 4);
+    }
+    // The leaves keep their boxes (the reference preserves fatness), so the
+    // device's leaf boxes stay as they are.
+    TreeRebuild(w);
+}
+
 const char* m3World_GetDeviceError(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);
@@ -2317,6 +2528,7 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
     m3World* hosts[CHUNK];
     m3gpuStepParams params[CHUNK];
     int32_t n = 0;
+    int32_t solo = 0;
     for (int32_t k = 0; k < count && n < CHUNK; ++k)
     {
         m3World* w = WorldFromId(worlds[k]);
@@ -2325,9 +2537,12 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
             continue;
         }
         FillParams(w, dt, substeps, &params[n]);
-        if (!m3gpu_step_supported(w->dev, &params[n]))
+        if (!m3gpu_step_supported(w->dev, &params[n]) || w->treeMirror != 0)
         {
-            continue; // refused like m3World_Step: this world does not step
+            // Worlds whose step involves the host tree step on their own
+            // (refused worlds do not step at all).
+            solo += StepWorld(w, dt, substeps) ? 1 : 0;
+            continue;
         }
         devs[n] = w->dev;
         hosts[n] = w;
@@ -2335,7 +2550,7 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
     }
     if (n == 0)
     {
-        return 0;
+        return solo;
     }
     int rc = m3gpu_step_many(devs, params, n);
     if (rc < 0)
@@ -2358,9 +2573,13 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
         w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
         w->eventsState = 1;
         w->eventThreshold = w->hitEventThreshold;
+        if (TreeHasLeaves(w))
+        {
+            w->treeStale = 1;
+        }
         JournalRecord(w, M3_OP_STEP, &rec, (int32_t)sizeof(rec));
     }
-    return n;
+    return n + solo;
 }
 
 m3AdjointDef m3DefaultAdjointDef(void)
@@ -2918,6 +3137,15 @@ static bool JournalApply(m3World* w, m3WorldId worldId, const uint8_t* stream, i
             }
             break;
         }
+        case M3_OP_REBUILD_BROADPHASE:
+        {
+            if (bytes != 4)
+            {
+                return false;
+            }
+            m3World_RebuildBroadphase(worldId);
+            break;
+        }
         case M3_OP_CREATE_SHAPE:
         {
             m3OpCreateShape rec;
@@ -3329,8 +3557,15 @@ m3Counters m3World_GetCounters(m3WorldId worldId)
     {
         c.contactCount += w->manifolds[i].pointCount > 0 ? 1 : 0;
     }
+    for (int32_t i = 0; i < w->jointPool.maxIndex; ++i)
+    {
+        c.jointCount += w->jointPool.alive[i];
+    }
     c.islandCount = w->lastIslandCount;
     c.colorCount = w->lastColorCount;
+    // The reference's tree height (src/world.c:949-950) while the host tree
+    // is current; -1 while it is stale (steps taken without the mirror).
+    c.treeHeight = w->treeStale != 0 ? -1 : (w->tree.root != M3_TREE_NULL ? m3TreeHeight(&w->tree) : 0);
     c.snapshotBytes = m3World_SnapshotSize(worldId);
     return c;
 }

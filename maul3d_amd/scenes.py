@@ -191,6 +191,66 @@ def bounce(api: Api, count: int = 160, seed: int = 0xB0A7CE, restitution: bool =
     return world
 
 
+def ccd_arena(api: Api, seed: int = 0xCCD5, movers: int = 60, bullets: int = 3) -> WorldId:
+    """Continuous collision against tree shapes (src/solver.c:3081-3430): a
+    static box floor and four static box walls (no plane), static 12-point
+    hull rocks and static capsules inside, ``movers`` spheres, boxes and
+    capsules thrown at 15-45 m/s, and ``bullets`` small bullet spheres
+    (include/maul3d/body.h:32-37) at 60-90 m/s, which also sweep against the
+    dynamic bodies.  Fast bodies are pulled back to their first time of
+    impact with the static geometry each step."""
+    world = _world(api, movers + bullets + 32, movers + bullets + 32)
+    rng = SplitMix64(seed)
+    sd = api.m3DefaultShapeDef()
+    sd.friction = 0.6
+    gd = api.m3DefaultBodyDef()
+    gd.position = Pos3(0.0, -0.5, 0.0)
+    floor = api.m3CreateBody(world, C.byref(gd))
+    _check(api.m3CreateBoxShape(floor, C.byref(sd), Vec3(16.0, 0.5, 16.0)), "floor")
+    for x, z, hx, hz in ((15.5, 0.0, 0.5, 16.0), (-15.5, 0.0, 0.5, 16.0), (0.0, 15.5, 16.0, 0.5), (0.0, -15.5, 16.0, 0.5)):
+        gd.position = Pos3(x, 3.0, z)
+        wall = api.m3CreateBody(world, C.byref(gd))
+        _check(api.m3CreateBoxShape(wall, C.byref(sd), Vec3(hx, 3.0, hz)), "wall")
+    for k in range(8):
+        gd.position = Pos3(rng.range(-10.0, 10.0), rng.range(0.5, 2.0), rng.range(-10.0, 10.0))
+        rock = api.m3CreateBody(world, C.byref(gd))
+        if k % 2 == 0:
+            cloud = (Vec3 * 12)(
+                *[Vec3(rng.range(-1.0, 1.0), rng.range(-1.0, 1.0), rng.range(-1.0, 1.0)) for _ in range(12)]
+            )
+            if api.m3CreateHullShape(rock, C.byref(sd), cloud, 12).index1 == 0:
+                _check(api.m3CreateBoxShape(rock, C.byref(sd), Vec3(0.8, 0.8, 0.8)), "rock")
+        else:
+            cap = Capsule(Vec3(-1.2, 0.0, 0.0), Vec3(1.2, 0.0, 0.0), rng.range(0.3, 0.6))
+            _check(api.m3CreateCapsuleShape(rock, C.byref(sd), C.byref(cap)), "post")
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    for k in range(movers + bullets):
+        bullet = k >= movers
+        bd.position = Pos3(rng.range(-12.0, 12.0), rng.range(2.0, 10.0), rng.range(-12.0, 12.0))
+        bd.rotation = Quat(*rng.unit_quat())
+        speed = rng.range(60.0, 90.0) if bullet else rng.range(15.0, 45.0)
+        d = (rng.range(-1.0, 1.0), rng.range(-1.0, -0.2), rng.range(-1.0, 1.0))
+        n = math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+        bd.linearVelocity = Vec3(speed * d[0] / n, speed * d[1] / n, speed * d[2] / n)
+        bd.angularVelocity = Vec3(rng.range(-5.0, 5.0), rng.range(-5.0, 5.0), rng.range(-5.0, 5.0))
+        bd.isBullet = bullet
+        body = api.m3CreateBody(world, C.byref(bd))
+        kind = 0 if bullet else k % 3
+        if kind == 0:
+            ball = Sphere(Vec3(0.0, 0.0, 0.0), 0.1 if bullet else rng.range(0.2, 0.4))
+            _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+        elif kind == 1:
+            he = Vec3(rng.range(0.15, 0.4), rng.range(0.15, 0.4), rng.range(0.15, 0.4))
+            _check(api.m3CreateBoxShape(body, C.byref(sd), he), "box")
+        else:
+            hh = rng.range(0.2, 0.4)
+            cap = Capsule(Vec3(-hh, 0.0, 0.0), Vec3(hh, 0.0, 0.0), rng.range(0.12, 0.25))
+            _check(api.m3CreateCapsuleShape(body, C.byref(sd), C.byref(cap)), "capsule")
+    bd.isBullet = False
+    return world
+
+
 def smoke10k(api: Api) -> WorldId:
     """The reference bench's 10k mixed rain (bench/bench_main.c:612-650).
 

@@ -173,7 +173,9 @@ struct m3gpuWorld
     std::vector<uint32_t> masks;
     int jointsPresent;
     int targetsPresent;
-    int ccdUnsupported; // the device step's refusal rule (continuous vs tree shapes)
+    int ccdNeedsTree; // continuous sweeps meet tree shapes (static non-plane shapes or bullets)
+    int begun;        // m3gpu_step_begin ran; m3gpu_step_end finishes the step
+    int beginStalled;
     tw::m3HitRec* hitRecBuf;
 };
 
@@ -238,6 +240,8 @@ extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* c)
     v.leaf = A<double>(6 * (int64_t)ns);
     v.fresh = A<double>(6 * (int64_t)ns);
     v.planes = A<int32_t>(ns);
+    v.shapeRank = A<int32_t>(ns);
+    v.ccdStatics = A<int32_t>(ns);
     v.hulls = A<tw::m3HullData>(c->hullCapacity);
     for (int k = 0; k < 2; ++k)
     {
@@ -363,20 +367,22 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
         const ::m3Vec3* fo = (const ::m3Vec3*)h->bodyForce;
         const ::m3Vec3* to = (const ::m3Vec3*)h->bodyTorque;
         w->targetsPresent = 0;
-        w->ccdUnsupported = 0;
+        w->ccdNeedsTree = 0;
+        v.ccdStaticCount = 0;
         for (int s2 = 0; s2 < ns; ++s2)
         {
             if (h->shapeAlive[s2] != 0 && h->shapeType[s2] != M3_PLANE &&
-                h->types[h->shapeBody[s2]] == M3_STATIC && h->shapeSensor[s2] == 0)
+                h->types[h->shapeBody[s2]] == M3_STATIC)
             {
-                w->ccdUnsupported = 1;
+                v.ccdStatics[v.ccdStaticCount++] = s2;
+                w->ccdNeedsTree |= h->shapeSensor[s2] == 0;
             }
         }
         for (int b = 0; b < nb; ++b)
         {
             if (h->bodyAlive[b] != 0 && h->bulletFlags[b] != 0)
             {
-                w->ccdUnsupported = 1;
+                w->ccdNeedsTree = 1;
             }
         }
         for (int b = 0; b < nb; ++b)
@@ -469,7 +475,7 @@ static tw::m3SerialParams Params(const m3gpuStepParams* p)
     return s;
 }
 
-static int Step(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
+static int StepBegin(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
 {
     if (w->jointsPresent || w->targetsPresent)
     {
@@ -487,7 +493,7 @@ static int Step(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
     s.masks = w->masks.data();
     tw::m3SerialParams sp = Params(p);
     v.sc->stall = 0;
-    if (tw::m3SerialStep(&v, &s, &sp, tape) != 0)
+    if (tw::m3SerialStepBegin(&v, &s, &sp, tape) != 0)
     {
         w->cur ^= 1;
         return 1;
@@ -495,16 +501,76 @@ static int Step(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
     return 0;
 }
 
+static void StepEnd(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape, const int32_t* ranks)
+{
+    tw::m3View& v = w->v;
+    v.treeCcd = ranks != nullptr ? 1 : 0;
+    if (ranks != nullptr)
+    {
+        memcpy(v.shapeRank, ranks, (size_t)v.maxShape * sizeof(int32_t));
+    }
+    tw::m3SerialScratch s;
+    s.root = w->root.data();
+    s.masks = w->masks.data();
+    tw::m3SerialParams sp = Params(p);
+    tw::m3SerialStepEnd(&v, &s, &sp, tape);
+}
+
+static int Step(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
+{
+    const int rc = StepBegin(w, p, tape);
+    if (rc != 0)
+    {
+        return rc;
+    }
+    StepEnd(w, p, tape, nullptr);
+    return 0;
+}
+
+extern "C" int m3gpu_step_needs_tree(const m3gpuWorld* w, const m3gpuStepParams* p)
+{
+    return w != nullptr && p != nullptr && p->continuousEnabled && w->ccdNeedsTree;
+}
+
 extern "C" int m3gpu_step_supported(const m3gpuWorld* w, const m3gpuStepParams* p)
 {
-    return w != nullptr && p != nullptr && !(p->continuousEnabled && w->ccdUnsupported);
+    return w != nullptr && p != nullptr && !m3gpu_step_needs_tree(w, p);
+}
+
+extern "C" int m3gpu_step_begin(m3gpuWorld* w, const m3gpuStepParams* p, int32_t)
+{
+    const int rc = StepBegin(w, p, nullptr);
+    if (rc < 0)
+    {
+        return -1;
+    }
+    w->begun = 1;
+    w->beginStalled = rc != 0;
+    return 0;
+}
+
+extern "C" const double* m3gpu_step_leaves(m3gpuWorld* w) { return w->v.leaf; }
+
+extern "C" int m3gpu_step_end(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t* ranks)
+{
+    if (!w->begun)
+    {
+        snprintf(s_err, sizeof(s_err), "twin: m3gpu_step_end without m3gpu_step_begin");
+        return -1;
+    }
+    w->begun = 0;
+    if (!w->beginStalled)
+    {
+        StepEnd(w, p, nullptr, ranks);
+    }
+    return 0;
 }
 
 extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStats* st)
 {
     if (!m3gpu_step_supported(w, p))
     {
-        snprintf(s_err, sizeof(s_err), "twin: continuous collision against tree shapes");
+        snprintf(s_err, sizeof(s_err), "twin: the continuous pass needs the tree ranks");
         return M3GPU_REFUSED;
     }
     int rc = Step(w, p, nullptr);

@@ -117,3 +117,23 @@ def test_pair_capacity_stall_does_not_advance_and_resumes():
         gpu.m3DestroyWorld(w)
     for w in (rw, rfit):
         ref.m3DestroyWorld(w)
+
+
+@pytest.mark.parametrize("bullets", [0, 3])
+def test_continuous_against_tree_shapes_matches_live_reference_every_step(bullets):
+    """A static box floor and walls, static hull rocks and capsules, fast
+    bodies and (second case) bullets: the device's continuous pass sweeps
+    against the tree shapes in the host tree's depth-first order, with the
+    fraction threaded through the candidates (src/solver.c:3081-3430)."""
+    ref = ref_api()
+    gpu = product_api()
+    a = scenes.ccd_arena(ref, bullets=bullets)
+    b = scenes.ccd_arena(gpu, bullets=bullets)
+    for step in range(1, 241):
+        ref.m3World_Step(a, DT, SUBSTEPS)
+        gpu.m3World_Step(b, DT, SUBSTEPS)
+        assert hexh(gpu.m3World_Hash(b)) == hexh(ref.m3World_Hash(a)), f"step {step}"
+    assert gpu.m3World_GetCounters(b).treeHeight == ref.m3World_GetCounters(a).treeHeight
+    assert gpu.m3World_GetDeviceError(b) is None
+    ref.m3DestroyWorld(a)
+    gpu.m3DestroyWorld(b)
