@@ -37,6 +37,10 @@ extern "C" {
 #define M3GPU_FIELDS_DYNAMIC 0x2u // transforms, velocities, awake, sleep timers, forces
 #define M3GPU_FIELDS_PAIRS   0x4u // pair keys, manifolds, frozen keys, leaf boxes
 #define M3GPU_FIELDS_ALL     0x7u
+// The pair table in the reference's layout (pairCapacity keys and
+// manifolds, the tail past the current count holding what earlier steps
+// wrote there); only while m3gpu_set_pair_table has it on.
+#define M3GPU_FIELDS_PAIR_TABLE 0x8u
 
 typedef struct m3gpuWorld m3gpuWorld;
 
@@ -194,6 +198,13 @@ M3GPU_API int m3gpu_download(m3gpuWorld* world, m3gpuHostState* host, uint32_t f
 // stats the step is only enqueued and the caller may return at once.
 M3GPU_API int m3gpu_step(m3gpuWorld* world, const m3gpuStepParams* params,
                          m3gpuStepStats* stats);
+
+// Keep (or stop keeping) the pair table in the reference's layout: each
+// step writes its pair list over the table's first pairCount entries and
+// leaves the rest, as the reference's single pair array does.  The table is
+// allocated on first use; its content comes from an upload of
+// M3GPU_FIELDS_PAIR_TABLE.
+M3GPU_API int m3gpu_set_pair_table(m3gpuWorld* world, int32_t on);
 
 // 1 when m3gpu_step would run this world with these params in one call, 0
 // when it would return M3GPU_REFUSED.

@@ -358,6 +358,9 @@ _SIGNATURES["m3Shape_EnableHitEvents"] = (None, [ShapeId, C.c_bool])
 _SIGNATURES["m3Shape_AreHitEventsEnabled"] = (C.c_bool, [ShapeId])
 _SIGNATURES["m3World_StepMany"] = (c_i32, [C.POINTER(WorldId), c_i32, c_f, c_i32])
 _SIGNATURES["m3World_GetDeviceError"] = (C.c_char_p, [WorldId])
+_SIGNATURES["m3World_EnableTreeMirror"] = (C.c_bool, [WorldId, C.c_bool])
+_SIGNATURES["m3World_IsTreeMirrorCurrent"] = (C.c_bool, [WorldId])
+_SIGNATURES["m3World_RebuildBroadphase"] = (None, [WorldId])
 
 
 def _id_tuple(sid) -> tuple[int, int]:
@@ -382,6 +385,8 @@ class Api:
             "m3World_Autotune",
             "m3World_StepMany",
             "m3World_GetDeviceError",
+            "m3World_EnableTreeMirror",
+            "m3World_IsTreeMirrorCurrent",
         ),
     ):
         self.lib = lib

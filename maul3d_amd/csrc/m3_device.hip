@@ -84,6 +84,10 @@ struct m3gpuWorld
     int fullQuery; // rebuild the frozen-pair buffer before the next step
     int ccdUnsupported;
     int ccdNeedsTree; // continuous sweeps meet tree shapes (static non-plane targets or bullets)
+    // the pair table in the reference's layout (m3gpu_set_pair_table)
+    uint64_t* tableKeys;
+    m3Manifold* tableMans;
+    int pairTable;
     int32_t* rankStage; // pinned staging of the tree ranks a step carries
     double* leafStage;  // pinned copy of the step's leaf boxes (m3gpu_step_leaves)
     hipEvent_t leavesCopied;
@@ -2340,6 +2344,23 @@ __global__ void k_ccd_apply(const m3Dev* __restrict__ ds)
     }
 }
 
+// The step's pair list over the first pairCount entries of the table; the
+// rest keeps what earlier steps left (the reference's single pair array).
+__global__ void k_pair_table(const m3Dev* __restrict__ ds, uint64_t* tableKeys, m3Manifold* tableMans)
+{
+    M3_WORLD;
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    const int32_t n = v.sc->pairCount;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        tableKeys[i] = v.keys[i];
+        tableMans[i] = v.manifolds[i];
+    }
+}
+
 __global__ void k_sleep_timers(const m3Dev* __restrict__ ds, float dt, int sleepEnabled)
 {
     M3_WORLD;
@@ -3069,7 +3090,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
                     v.jointAngularImpulse, v.jointSpringImpulse, v.jointNextA, v.jointNextB,
                     v.bodyJointHead, v.jointCompStart, v.jointCompList, v.jcons, v.jointActiveFlag,
-                    v.shapeRank, v.ccdStatics, w->dev.ccdXf, w->dev.ccdHit};
+                    v.shapeRank, v.ccdStatics, w->dev.ccdXf, w->dev.ccdHit, w->tableKeys, w->tableMans};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); ++i)
     {
         if (ptrs[i] != nullptr)
@@ -3316,6 +3337,16 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(&v.sc->fullQuery, &fullQuery, 1, int32_t);
         HIP_TRY(hipStreamSynchronize(w->stream));
     }
+    if (fields & M3GPU_FIELDS_PAIR_TABLE)
+    {
+        if (w->tableKeys == nullptr)
+        {
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_upload: no pair table");
+            return -1;
+        }
+        UP(w->tableKeys, h->pairKeys, v.pairCapacity, uint64_t);
+        UP(w->tableMans, h->manifolds, v.pairCapacity, m3Manifold);
+    }
     HIP_TRY(hipStreamSynchronize(w->stream));
     return RefreshDevCopies(w);
 }
@@ -3380,6 +3411,16 @@ extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t fields)
         DOWN(h->sleepingPairKeys, v.sleepKeys, zc, uint64_t);
         DOWN(h->leafBoxes, v.leaf, 6 * (int64_t)ns, double);
         *h->frozenDirty = 0;
+    }
+    if (fields & M3GPU_FIELDS_PAIR_TABLE)
+    {
+        if (w->tableKeys == nullptr)
+        {
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_download: no pair table");
+            return -1;
+        }
+        DOWN(h->pairKeys, w->tableKeys, v.pairCapacity, uint64_t);
+        DOWN(h->manifolds, w->tableMans, v.pairCapacity, m3Manifold);
     }
     HIP_TRY(hipStreamSynchronize(w->stream));
     return CheckFault(w);
@@ -3777,6 +3818,10 @@ static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t
         STAGE(ST_SLEEP_FREEZE, KL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     }
     STAGE(ST_END, KL(k_end, dim3(1), dim3(1), 0, st, D));
+    if (w->pairTable)
+    {
+        KL(k_pair_table, dim3(Blocks(v.pairCapacity, T)), dim3(T), 0, st, D, w->tableKeys, w->tableMans);
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -3810,6 +3855,23 @@ static int ProfileCollect(m3gpuWorld* w, int32_t substeps)
     pr->sumShapes += w->view.maxShape;
     pr->sumBodies += w->view.maxBody;
     pr->sumOverflow += s->colorStarts[M3_GRAPH_COLORS + 1] - s->colorStarts[M3_GRAPH_COLORS];
+    return 0;
+}
+
+extern "C" int m3gpu_set_pair_table(m3gpuWorld* w, int32_t on)
+{
+    if (w == nullptr)
+    {
+        return -1;
+    }
+    if (on && w->tableKeys == nullptr)
+    {
+        if (DevAlloc(&w->tableKeys, w->view.pairCapacity) || DevAlloc(&w->tableMans, w->view.pairCapacity))
+        {
+            return -1;
+        }
+    }
+    w->pairTable = on ? 1 : 0;
     return 0;
 }
 

@@ -472,3 +472,66 @@ int32_t m3TreeHeight(const m3Tree* t)
 {
     return t->root == M3_TREE_NULL ? -1 : t->nodes[t->root].height;
 }
+
+bool m3TreeValid(const m3Tree* t)
+{
+    const int32_t n = t->capacity;
+    if (t->root < M3_TREE_NULL || t->root >= n || t->freeList < M3_TREE_NULL || t->freeList >= n)
+    {
+        return false;
+    }
+    uint8_t* seen = (uint8_t*)calloc((size_t)(n > 0 ? n : 1), 1);
+    int32_t* stack = (int32_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    bool ok = seen != NULL && stack != NULL;
+    int32_t top = 0;
+    int32_t visited = 0;
+    if (ok && t->root != M3_TREE_NULL)
+    {
+        if (t->nodes[t->root].parent != M3_TREE_NULL)
+        {
+            ok = false;
+        }
+        stack[top++] = t->root;
+    }
+    while (ok && top > 0)
+    {
+        const int32_t id = stack[--top];
+        if (seen[id] != 0)
+        {
+            ok = false;
+            break;
+        }
+        seen[id] = 1;
+        visited += 1;
+        const m3TreeNode* node = &t->nodes[id];
+        if (node->height == 0)
+        {
+            continue;
+        }
+        const int32_t c1 = node->child1;
+        const int32_t c2 = node->child2;
+        if (node->height < 0 || c1 < 0 || c1 >= n || c2 < 0 || c2 >= n || c1 == c2 ||
+            t->nodes[c1].parent != id || t->nodes[c2].parent != id ||
+            node->height != 1 + MaxI(t->nodes[c1].height, t->nodes[c2].height) || top + 2 > n)
+        {
+            ok = false;
+            break;
+        }
+        stack[top++] = c1;
+        stack[top++] = c2;
+    }
+    for (int32_t id = t->freeList; ok && id != M3_TREE_NULL; id = t->nodes[id].parent)
+    {
+        if (id < 0 || id >= n || seen[id] != 0 || t->nodes[id].height != -1)
+        {
+            ok = false;
+            break;
+        }
+        seen[id] = 1;
+        visited += 1;
+    }
+    ok = ok && visited == n;
+    free(seen);
+    free(stack);
+    return ok;
+}

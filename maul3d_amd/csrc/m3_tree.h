@@ -74,4 +74,11 @@ int32_t m3TreeLeafRanks(const m3Tree* tree, int32_t* rank, int32_t rankCount);
 // Height of the root (0 for a single leaf, -1 for an empty tree).
 int32_t m3TreeHeight(const m3Tree* tree);
 
+// True when the node array (as a restore brings it in) is a well-formed
+// tree: root and free chain in range, every reachable node reached once,
+// internal nodes with two in-range children pointing back and heights one
+// above the taller child, leaves of height 0, free nodes of height -1, and
+// the reachable nodes plus the free chain covering the pool exactly.
+bool m3TreeValid(const m3Tree* tree);
+
 #endif // M3_TREE_H

@@ -230,6 +230,12 @@ typedef struct m3World
     int32_t* treeRanks;
     uint8_t treeMirror;
     uint8_t treeStale;
+    // The pair table in the reference's layout (the whole capacity, the
+    // tail past pairCount holding what earlier steps wrote there) is known
+    // while every step since creation or the last restore ran with the
+    // mirror; the device keeps it current from the first such step on.
+    uint8_t pairTableLive;
+    uint8_t pairTableOnDevice;
 
     m3IdPool hullPool;
     m3HullData* hullData;
@@ -925,6 +931,7 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
     }
     ALLOC(w->proxyIds, shapeCap, int32_t);
     ALLOC(w->treeRanks, shapeCap, int32_t);
+    w->pairTableLive = 1; // all zero, as the reference's freshly allocated table
     for (int32_t i = 0; i < shapeCap; ++i)
     {
         w->proxyIds[i] = M3_TREE_NULL;
@@ -2408,6 +2415,19 @@ static bool StepWorld(m3World* w, float dt, int32_t substeps)
         {
             return false;
         }
+        if (w->pairTableLive != 0 && w->pairTableOnDevice == 0)
+        {
+            // The device takes over the table from the host arrays.
+            m3gpuHostState h;
+            HostView(w, &h);
+            if (m3gpu_set_pair_table(w->dev, 1) != 0 ||
+                m3gpu_upload(w->dev, &h, M3GPU_FIELDS_PAIR_TABLE) != 0)
+            {
+                DeviceFail(w, "m3gpu_upload");
+                return false;
+            }
+            w->pairTableOnDevice = 1;
+        }
         rc = m3gpu_step_begin(w->dev, &p, 1);
         if (rc == 0)
         {
@@ -2435,10 +2455,17 @@ static bool StepWorld(m3World* w, float dt, int32_t substeps)
     }
     else
     {
-        rc = m3gpu_step(w->dev, &p, NULL);
-        if (rc == 0 && TreeHasLeaves(w))
+        if (w->pairTableOnDevice != 0 && m3gpu_set_pair_table(w->dev, 0) != 0)
         {
-            w->treeStale = 1;
+            DeviceFail(w, "m3gpu_set_pair_table");
+            return false;
+        }
+        rc = m3gpu_step(w->dev, &p, NULL);
+        if (rc == 0)
+        {
+            w->treeStale = TreeHasLeaves(w) ? 1 : 0;
+            w->pairTableLive = 0;
+            w->pairTableOnDevice = 0;
         }
     }
     if (rc == M3GPU_REFUSED)
@@ -2577,6 +2604,8 @@ int32_t m3World_StepMany(const m3WorldId* worlds, int32_t count, float dt, int32
         {
             w->treeStale = 1;
         }
+        w->pairTableLive = 0;
+        w->pairTableOnDevice = 0;
         JournalRecord(w, M3_OP_STEP, &rec, (int32_t)sizeof(rec));
     }
     return n + solo;
@@ -3584,83 +3613,248 @@ m3Profile m3World_GetProfile(m3WorldId worldId)
 }
 
 // ---------------------------------------------------------------------------
-// Snapshot / restore.  This build's own block format (the reference's
-// includes its dynamic tree, which this build replaces by per-shape proxy
-// boxes): a header, then fixed-order blocks sized by the pool high-water
-// marks.  Restore validates every size before it writes anything.
+// Snapshot / restore in the reference's layout (src/snapshot.c:25-929, format
+// version 54): a 192-byte header, the persistent arrays as fixed blocks in
+// the reference's order, then the hull, mesh and heightfield content sized by
+// counts.  A snapshot written here restores in the reference and the other
+// way round.  The blocks of subsystems this build does not simulate (voxels,
+// characters, vehicles, soft bodies, meshes, heightfields, water, wind, body
+// names) are written in the state a world that never used them holds, and a
+// restore refuses a snapshot whose such blocks say otherwise.  The tree
+// block is the host tree (exact while the mirror runs, else the canonical
+// rebuild of the current leaves).  The pair table block holds the whole
+// capacity: past pairCount the reference keeps what earlier steps wrote
+// there; this build keeps that tail only while the mirror runs from world
+// creation (on the device, m3gpu pair table), and writes zeros otherwise.
 // ---------------------------------------------------------------------------
 
-#define M3_SNAP_MAGIC 0x31414D33u // "3MA1"
-#define M3_SNAP_VERSION 2u
+#define M3_SNAP_MAGIC 0x4D33534Eu // 'M3SN'
+#define M3_SNAP_VERSION 54u
+#define M3_SOLVER_REV 21 // src/world_internal.h:47
+#define M3_BODY_NAME_CAPACITY 32
+#define M3_MAX_WATER_VOLUMES 8
+#define M3_VEHICLE_MAX_WHEELS 8
+#define M3_DRIVETRAIN_MAX_CURVE 8
+#define M3_DRIVETRAIN_MAX_GEARS 6
+#define M3_SOFTBODY_MAX_PARTICLES 512
+#define M3_SOFTBODY_MAX_EDGES 5120
+#define M3_SOFTBODY_MAX_ANCHORS 32
+#define M3_SOFTBODY_MAX_TETS 1024
+#define M3_VOXEL_CHUNK_BYTES (8 + 512 + 8192 + 4096)
 
-typedef struct m3SnapHeader
+typedef struct m3SnapHeader // src/snapshot.c:84-130
 {
     uint32_t magic;
-    uint32_t version;
-    uint32_t recordSizes; // sizeof(m3Manifold) << 16 | sizeof(m3Transform)
-    uint32_t hullSize;
-    int32_t bodyCapacity;
-    int32_t shapeCapacity;
-    int32_t pairCapacity;
-    int32_t bodyMax;
-    int32_t shapeMax;
-    int32_t hullMax;
-    int32_t pairCount;
-    int32_t sleepingPairCount;
-    int32_t jointCapacity;
-    int32_t reserved;
+    uint32_t formatVersion;
+    uint64_t configHash;
     uint64_t stepCount;
-    uint64_t payloadHash;
+    int32_t bodyCapacity;
+    int32_t maxIndex;
+    int32_t freeHead;
+    int32_t freeCount;
+    int32_t retiredCount;
+    int32_t shapeCapacity;
+    int32_t shapeMaxIndex;
+    int32_t shapeFreeHead;
+    int32_t shapeFreeCount;
+    int32_t shapeRetiredCount;
+    int32_t pairCount;
+    int32_t treeRoot;
+    int32_t treeFreeList;
+    int32_t hullMaxIndex;
+    int32_t hullFreeHead;
+    int32_t hullFreeCount;
+    int32_t hullRetiredCount;
+    int32_t meshCapacity;
+    int32_t meshMaxIndex;
+    int32_t meshFreeHead;
+    int32_t meshFreeCount;
+    int32_t meshRetiredCount;
+    int32_t jointCapacity;
+    int32_t jointMaxIndex;
+    int32_t jointFreeHead;
+    int32_t jointFreeCount;
+    int32_t jointRetiredCount;
+    int32_t voxelCapacity;
+    int32_t voxelMaxIndex;
+    int32_t voxelFreeHead;
+    int32_t voxelFreeCount;
+    int32_t voxelRetiredCount;
+    int32_t charCapacity;
+    int32_t charMaxIndex;
+    int32_t charFreeHead;
+    int32_t charFreeCount;
+    int32_t charRetiredCount;
+    int32_t reserved[2];
+    m3Vec3 gravity;
 } m3SnapHeader;
 
+_Static_assert(sizeof(m3SnapHeader) == 192, "snapshot header layout");
+
+// What the bytes mean: engine version, solver revision, precision and FP
+// policy (src/snapshot.c:132-148).
+static uint64_t SnapConfigHash(void)
+{
+    uint64_t h = M3_HASH_INIT;
+    const int32_t version = m3GetVersion();
+    const int32_t solverRev = M3_SOLVER_REV;
+    const int32_t realSize = (int32_t)sizeof(float);
+    const int32_t posSize = (int32_t)sizeof(double);
+    static const char policy[] = "contract-off;no-fast-math;no-fuse-4wide";
+    h = m3HashBytes(h, &version, 4);
+    h = m3HashBytes(h, &solverRev, 4);
+    h = m3HashBytes(h, &realSize, 4);
+    h = m3HashBytes(h, &posSize, 4);
+    h = m3HashBytes(h, policy, (int32_t)(sizeof(policy) - 1));
+    return h;
+}
+
+// One walk serves four passes: measure (no buffers), write (out), verify
+// (in, dry: bounds and the fixed contents of unsimulated blocks are
+// checked, nothing is written) and read (in).
 typedef struct m3Walker
 {
-    uint8_t* out;       // write pass when non-null
-    const uint8_t* in;  // read pass when non-null
+    uint8_t* out;
+    const uint8_t* in;
+    int dry;
     int64_t cursor;
     int64_t limit;
     int failed;
+    // payload offsets of the tree nodes and proxy ids (found by walking)
+    const void* treeField;
+    const void* proxyField;
+    int64_t treeAt;
+    int64_t proxyAt;
 } m3Walker;
 
-static void Walk(m3Walker* k, void* field, int64_t bytes)
+static bool WalkRoom(m3Walker* k, int64_t bytes)
 {
-    if (k->failed || bytes <= 0)
+    if (k->failed)
     {
-        return;
+        return false;
     }
     if (k->limit >= 0 && k->cursor + bytes > k->limit)
     {
         k->failed = 1;
+        return false;
+    }
+    return true;
+}
+
+static void Walk(m3Walker* k, void* field, int64_t bytes)
+{
+    if (bytes <= 0 || !WalkRoom(k, bytes))
+    {
         return;
+    }
+    if (field == k->treeField)
+    {
+        k->treeAt = k->cursor;
+    }
+    if (field == k->proxyField)
+    {
+        k->proxyAt = k->cursor;
     }
     if (k->out != NULL)
     {
         memcpy(k->out + k->cursor, field, (size_t)bytes);
     }
-    else if (k->in != NULL && field != NULL)
+    else if (k->in != NULL && !k->dry)
     {
         memcpy(field, k->in + k->cursor, (size_t)bytes);
     }
     k->cursor += bytes;
 }
 
-static void WalkPool(m3Walker* k, m3IdPool* p)
+// A block of `count` copies of a `unit`-byte value (unit at most 16; zero
+// when value is null): written as such, and on restore required to be
+// exactly that.
+static void WalkFixed(m3Walker* k, int64_t count, int32_t unit, const void* value)
 {
-    Walk(k, &p->maxIndex, 4);
-    Walk(k, &p->freeHead, 4);
-    Walk(k, &p->freeCount, 4);
-    Walk(k, &p->retiredCount, 4);
+    const int64_t bytes = count * unit;
+    if (bytes <= 0 || !WalkRoom(k, bytes))
+    {
+        return;
+    }
+    static const uint8_t zeros[16] = {0};
+    const uint8_t* v = value != NULL ? (const uint8_t*)value : zeros;
+    if (k->out != NULL)
+    {
+        for (int64_t i = 0; i < count; ++i)
+        {
+            memcpy(k->out + k->cursor + i * unit, v, (size_t)unit);
+        }
+    }
+    else if (k->in != NULL)
+    {
+        for (int64_t i = 0; i < count; ++i)
+        {
+            if (memcmp(k->in + k->cursor + i * unit, v, (size_t)unit) != 0)
+            {
+                k->failed = 2; // state this build does not simulate
+                return;
+            }
+        }
+    }
+    k->cursor += bytes;
+}
+
+static const int32_t kMinusOne = -1;
+
+// An id pool's arrays (src/snapshot.c:216-219); the cursors ride the header.
+static void WalkPoolArrays(m3Walker* k, m3IdPool* p)
+{
     Walk(k, p->generations, (int64_t)p->capacity * 2);
     Walk(k, p->alive, p->capacity);
     Walk(k, p->freeQueue, (int64_t)p->capacity * 4);
 }
 
-// One walk for measure, write and read.  Counts come from the world (write)
-// or the validated header (read); either way the same order.
-static void WalkWorld(m3Walker* k, m3World* w, int32_t nb, int32_t ns, int32_t nh, int32_t np,
-                      int32_t nz)
+// An empty pool of a subsystem this build does not simulate.
+static void WalkEmptyPool(m3Walker* k, int32_t capacity)
 {
-    Walk(k, &w->gravity, sizeof(m3Vec3));
+    WalkFixed(k, capacity, 2, NULL);
+    WalkFixed(k, capacity, 1, NULL);
+    WalkFixed(k, capacity, 4, NULL);
+}
+
+// The fixed blocks (src/snapshot.c:181-432), in the reference's order.
+static void WalkBlocks(m3Walker* k, m3World* w)
+{
+    const int64_t cap = w->bodyCapacity;
+    const int64_t sc = w->shapeCapacity;
+    const int64_t jc = w->jointCapacity;
+    const int64_t pc = w->pairCapacity;
+    const int64_t cc = w->characterCapacity;
+    const int64_t vc = w->vehicleCapacity;
+    const int64_t sbc = w->softBodyCapacity;
+    const int64_t W = M3_VEHICLE_MAX_WHEELS;
+
+    Walk(k, w->transforms, cap * (int64_t)sizeof(m3Transform));
+    Walk(k, w->linearVelocities, cap * 12);
+    Walk(k, w->angularVelocities, cap * 12);
+    Walk(k, w->invMass, cap * 4);
+    Walk(k, w->invInertiaLocal, cap * 36);
+    Walk(k, w->inertiaLocal, cap * 36);
+    Walk(k, w->localCenters, cap * 12);
+    Walk(k, w->gravityScales, cap * 4);
+    Walk(k, w->linearDamping, cap * 4);
+    Walk(k, w->angularDamping, cap * 4);
+    Walk(k, w->types, cap);
+    Walk(k, w->awake, cap);
+    Walk(k, w->sleepTimes, cap * 4);
+    Walk(k, w->bulletFlags, cap);
+    Walk(k, w->minExtents, cap * 4);
+    Walk(k, w->maxExtents, cap * 4);
+    Walk(k, w->userData, cap * 8);
+    WalkFixed(k, cap * M3_BODY_NAME_CAPACITY, 1, NULL); // debug names
+    Walk(k, w->bodyForce, cap * 12);
+    Walk(k, w->bodyTorque, cap * 12);
+    Walk(k, w->bodyEnabled, cap);
+    Walk(k, w->bodyLocks, cap);
+    Walk(k, w->bodySleepThreshold, cap * 4);
+    Walk(k, w->bodyCanSleep, cap);
+    Walk(k, w->bodyHasTarget, cap);
+    Walk(k, w->bodyTarget, cap * (int64_t)sizeof(m3Transform));
     Walk(k, &w->contactHertz, 4);
     Walk(k, &w->contactDampingRatio, 4);
     Walk(k, &w->contactPushMaxSpeed, 4);
@@ -3670,138 +3864,284 @@ static void WalkWorld(m3Walker* k, m3World* w, int32_t nb, int32_t ns, int32_t n
     Walk(k, &w->sleepEnabled, 1);
     Walk(k, &w->continuousEnabled, 1);
     Walk(k, &w->hitEventThreshold, 4);
-    WalkPool(k, &w->bodyPool);
-    WalkPool(k, &w->shapePool);
-    WalkPool(k, &w->hullPool);
-    Walk(k, w->transforms, (int64_t)nb * sizeof(m3Transform));
-    Walk(k, w->linearVelocities, (int64_t)nb * sizeof(m3Vec3));
-    Walk(k, w->angularVelocities, (int64_t)nb * sizeof(m3Vec3));
-    Walk(k, w->invMass, (int64_t)nb * 4);
-    Walk(k, w->invInertiaLocal, (int64_t)nb * sizeof(m3Mat3));
-    Walk(k, w->inertiaLocal, (int64_t)nb * sizeof(m3Mat3));
-    Walk(k, w->localCenters, (int64_t)nb * sizeof(m3Vec3));
-    Walk(k, w->gravityScales, (int64_t)nb * 4);
-    Walk(k, w->linearDamping, (int64_t)nb * 4);
-    Walk(k, w->angularDamping, (int64_t)nb * 4);
-    Walk(k, w->types, nb);
-    Walk(k, w->awake, nb);
-    Walk(k, w->sleepTimes, (int64_t)nb * 4);
-    Walk(k, w->bulletFlags, nb);
-    Walk(k, w->minExtents, (int64_t)nb * 4);
-    Walk(k, w->maxExtents, (int64_t)nb * 4);
-    Walk(k, w->userData, (int64_t)nb * 8);
-    Walk(k, w->bodyForce, (int64_t)nb * sizeof(m3Vec3));
-    Walk(k, w->bodyTorque, (int64_t)nb * sizeof(m3Vec3));
-    Walk(k, w->bodyEnabled, nb);
-    Walk(k, w->bodyLocks, nb);
-    Walk(k, w->bodySleepThreshold, (int64_t)nb * 4);
-    Walk(k, w->bodyCanSleep, nb);
-    Walk(k, w->bodyHasTarget, nb);
-    Walk(k, w->bodyTarget, (int64_t)nb * sizeof(m3Transform));
-    Walk(k, w->bodyShapeHead, (int64_t)nb * 4);
-    Walk(k, w->shapeBody, (int64_t)ns * 4);
-    Walk(k, w->shapeType, ns);
-    Walk(k, w->shapeGeom, (int64_t)ns * sizeof(m3ShapeGeom));
-    Walk(k, w->shapeDensity, (int64_t)ns * 4);
-    Walk(k, w->shapeFriction, (int64_t)ns * 4);
-    Walk(k, w->shapeRestitution, (int64_t)ns * 4);
-    Walk(k, w->shapeHitEvents, ns);
-    Walk(k, w->shapeLocalPos, (int64_t)ns * sizeof(m3Vec3));
-    Walk(k, w->shapeLocalRot, (int64_t)ns * sizeof(m3Quat));
-    Walk(k, w->shapeHasOffset, ns);
-    Walk(k, w->shapeSurfaceVel, (int64_t)ns * sizeof(m3Vec3));
-    Walk(k, w->shapePreSolve, ns);
-    Walk(k, w->shapeRollingResistance, (int64_t)ns * 4);
-    Walk(k, w->shapeCategory, (int64_t)ns * 8);
-    Walk(k, w->shapeMask, (int64_t)ns * 8);
-    Walk(k, w->shapeGroup, (int64_t)ns * 4);
-    Walk(k, w->shapeUserData, (int64_t)ns * 8);
-    Walk(k, w->shapeNext, (int64_t)ns * 4);
-    Walk(k, w->shapeHullIndex, (int64_t)ns * 4);
-    Walk(k, w->shapeMeshIndex, (int64_t)ns * 4);
-    Walk(k, w->shapeSensor, ns);
-    Walk(k, w->leafBoxes, (int64_t)ns * 6 * 8);
-    Walk(k, w->hullData, (int64_t)nh * sizeof(m3HullData));
-    Walk(k, w->hullRefCounts, (int64_t)nh * 4);
-    Walk(k, w->pairKeys, (int64_t)np * 8);
-    Walk(k, w->manifolds, (int64_t)np * sizeof(m3Manifold));
-    Walk(k, w->sleepingPairKeys, (int64_t)nz * 8);
-    Walk(k, &w->frozenDirty, 1);
-    int64_t nj = w->jointCapacity;
-    WalkPool(k, &w->jointPool);
-    Walk(k, w->jointType, nj);
-    Walk(k, w->jointBodyA, nj * 4);
-    Walk(k, w->jointBodyB, nj * 4);
-    Walk(k, w->jointLocalA, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointLocalB, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointCollide, nj);
-    Walk(k, w->jointImpulse, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointPerpImpulse, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointLimitImpulse, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointAngularImpulse, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointFrameQA, nj * (int64_t)sizeof(m3Quat));
-    Walk(k, w->jointFrameQB, nj * (int64_t)sizeof(m3Quat));
-    Walk(k, w->jointFlags, nj);
-    Walk(k, w->jointMotor, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointBreak, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointSpring, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointTargetScalar, nj * 4);
-    Walk(k, w->jointTargetQ, nj * (int64_t)sizeof(m3Quat));
-    Walk(k, w->jointSpringImpulse, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointLimits, nj * (int64_t)sizeof(m3Vec3));
-    Walk(k, w->jointNextA, nj * 4);
-    Walk(k, w->jointNextB, nj * 4);
-    Walk(k, w->bodyJointHead, (int64_t)nb * 4);
+    WalkFixed(k, 1, 12, NULL); // wind direction
+    WalkFixed(k, 4, 4, NULL);  // wind speed, gust hertz, gust scale, phase
+    WalkPoolArrays(k, &w->bodyPool);
+    Walk(k, w->bodyShapeHead, cap * 4);
+
+    Walk(k, w->shapeBody, sc * 4);
+    Walk(k, w->shapeType, sc);
+    Walk(k, w->shapeGeom, sc * (int64_t)sizeof(m3ShapeGeom));
+    Walk(k, w->shapeDensity, sc * 4);
+    Walk(k, w->shapeFriction, sc * 4);
+    Walk(k, w->shapeRestitution, sc * 4);
+    Walk(k, w->shapeUserData, sc * 8);
+    Walk(k, w->shapeNext, sc * 4);
+    WalkPoolArrays(k, &w->shapePool);
+
+    Walk(k, w->pairKeys, pc * 8);
+    Walk(k, w->proxyIds, sc * 4);
+    Walk(k, w->tree.nodes, (int64_t)w->tree.capacity * (int64_t)sizeof(m3TreeNode));
+    Walk(k, w->shapeHullIndex, sc * 4);
+    Walk(k, w->hullRefCounts, sc * 4);
+    WalkPoolArrays(k, &w->hullPool);
+    Walk(k, w->shapeMeshIndex, sc * 4);
+    Walk(k, w->shapeSensor, sc);
+    Walk(k, w->shapeRollingResistance, sc * 4);
+    Walk(k, w->shapeHitEvents, sc);
+    Walk(k, w->shapePreSolve, sc);
+    Walk(k, w->shapeSurfaceVel, sc * 12);
+    Walk(k, w->shapeLocalPos, sc * 12);
+    Walk(k, w->shapeLocalRot, sc * 16);
+    Walk(k, w->shapeHasOffset, sc);
+    Walk(k, w->shapeCategory, sc * 8);
+    Walk(k, w->shapeMask, sc * 8);
+    Walk(k, w->shapeGroup, sc * 4);
+
+    // Voxels and heightfields (none on this build).
+    WalkFixed(k, (int64_t)w->voxelCapacity * M3_VOXEL_CHUNK_BYTES, 1, NULL);
+    WalkFixed(k, w->voxelCapacity, 4, NULL);
+    WalkEmptyPool(k, w->voxelCapacity);
+    WalkFixed(k, sc, 4, &kMinusOne); // shapeVoxelIndex
+    WalkFixed(k, sc, 4, &kMinusOne); // shapeHfIndex
+    WalkFixed(k, sc, 4, NULL);       // heightfield ref counts
+    WalkEmptyPool(k, (int32_t)sc);
+
+    // Characters (none): charBody is -1, every other block zero.
+    WalkFixed(k, cc, 4, &kMinusOne);
+    WalkFixed(k, cc * 6, 4, NULL);   // radius, half height, cos slope, snap, skin, step
+    WalkFixed(k, cc, 1, NULL);       // grounded
+    WalkFixed(k, cc, 12, NULL);      // ground normal
+    WalkFixed(k, cc * 3, 4, NULL);   // mass, push max, ground body
+    WalkFixed(k, cc, 2, NULL);       // ground generation
+    WalkEmptyPool(k, (int32_t)cc);
+
+    // Vehicles (none): the chassis slot is -1, every other block zero.
+    WalkFixed(k, vc, 4, &kMinusOne);
+    const int64_t vehicleZeroBytes =
+        vc * (2 + 4 + 4 + 4 + 4 + 8) + vc * W * (12 + 12 + 4 + 4 + 4 + 4 + 4 + 1 + 4) +
+        vc * (1 + 4 + 4 + 4) + vc * W * (4 + 1) + vc * (4 + 4 + 4 + 4) + vc * W * 4 +
+        vc * (1 + 4) + vc * M3_DRIVETRAIN_MAX_CURVE * 8 + vc * 4 + vc * M3_DRIVETRAIN_MAX_GEARS * 4 +
+        vc * (4 + 4 + 4 + 4) + vc * W * 4 + vc * (4 + 4 + 4 + 1 + 1 + 4 + 4);
+    WalkFixed(k, vehicleZeroBytes, 1, NULL);
+    WalkEmptyPool(k, (int32_t)vc);
+
+    // Soft bodies (none).
+    const int64_t P = M3_SOFTBODY_MAX_PARTICLES;
+    const int64_t E = M3_SOFTBODY_MAX_EDGES;
+    const int64_t A = M3_SOFTBODY_MAX_ANCHORS;
+    const int64_t T = M3_SOFTBODY_MAX_TETS;
+    const int64_t softBytes = sbc * (4 + 4 + 4 + 4 + 4 + 2 + 2 + 2 + 4 + 4 + 4) + sbc * T * (2 * 4 + 4) +
+                              sbc * P * 24 + sbc * (4 + 4 + 4 + 8) + sbc * P * (24 + 24 + 4 + 12) +
+                              sbc * E * (2 + 2 + 4) + sbc * 4 + sbc * A * (4 + 4 + 2 + 12) + sbc * 4 +
+                              sbc * A * (4 + 4 + 2 + 4);
+    WalkFixed(k, softBytes, 1, NULL);
+    WalkEmptyPool(k, (int32_t)sbc);
+
+    // Meshes (none).
+    WalkFixed(k, w->meshCapacity, 4, NULL);
+    WalkEmptyPool(k, w->meshCapacity);
+
+    // Joints.
+    Walk(k, w->jointType, jc);
+    Walk(k, w->jointBodyA, jc * 4);
+    Walk(k, w->jointBodyB, jc * 4);
+    Walk(k, w->jointLocalA, jc * 12);
+    Walk(k, w->jointLocalB, jc * 12);
+    Walk(k, w->jointCollide, jc);
+    Walk(k, w->jointImpulse, jc * 12);
+    Walk(k, w->jointPerpImpulse, jc * 12);
+    Walk(k, w->jointLimitImpulse, jc * 12);
+    Walk(k, w->jointAngularImpulse, jc * 12);
+    Walk(k, w->jointFrameQA, jc * 16);
+    Walk(k, w->jointFrameQB, jc * 16);
+    Walk(k, w->jointFlags, jc);
+    Walk(k, w->jointMotor, jc * 12);
+    Walk(k, w->jointBreak, jc * 12);
+    Walk(k, w->jointSpring, jc * 12);
+    Walk(k, w->jointTargetScalar, jc * 4);
+    Walk(k, w->jointTargetQ, jc * 16);
+    Walk(k, w->jointSpringImpulse, jc * 12);
+    Walk(k, w->jointLimits, jc * 12);
+    // Generic-joint modes and limits and pulley ground anchors: zero for
+    // the spherical joints this build simulates (src/joint.c:240-280).
+    WalkFixed(k, jc * (2 + 4 * 12 + 2 * 24), 1, NULL);
+
+    // Water volumes (none).
+    WalkFixed(k, (int64_t)M3_MAX_WATER_VOLUMES * (24 + 24 + 4 + 4 + 4 + 12), 1, NULL);
+    WalkEmptyPool(k, M3_MAX_WATER_VOLUMES);
+
+    Walk(k, w->jointNextA, jc * 4);
+    Walk(k, w->jointNextB, jc * 4);
+    Walk(k, w->bodyJointHead, cap * 4);
+    WalkPoolArrays(k, &w->jointPool);
+    Walk(k, w->manifolds, pc * (int64_t)sizeof(m3Manifold));
+}
+
+// Content tails (src/snapshot.c:434-578): every mesh slot's two counts (all
+// zero here), every hull slot's four counts and used prefixes, every
+// heightfield slot's three words (all zero here).  In the verify pass the
+// hull counts come from the buffer and are checked against the caps.
+static void WalkTails(m3Walker* k, m3World* w)
+{
+    WalkFixed(k, (int64_t)w->meshCapacity * 2, 4, NULL);
+    for (int32_t h = 0; h < w->shapeCapacity && !k->failed; ++h)
+    {
+        m3HullData* hull = &w->hullData[h];
+        int32_t counts[4] = {hull->vertexCount, hull->faceCount, hull->indexCount, hull->edgeCount};
+        if (k->in != NULL)
+        {
+            if (!WalkRoom(k, 16))
+            {
+                return;
+            }
+            memcpy(counts, k->in + k->cursor, 16);
+            if (counts[0] < 0 || counts[0] > M3_HULL_MAX_VERTS || counts[1] < 0 ||
+                counts[1] > M3_HULL_MAX_FACES || counts[2] < 0 || counts[2] > M3_HULL_MAX_FACE_INDICES ||
+                counts[3] < 0 || counts[3] > M3_HULL_MAX_HALF_EDGES ||
+                (counts[0] == 0 && (counts[1] | counts[2] | counts[3]) != 0))
+            {
+                k->failed = 1;
+                return;
+            }
+            if (!k->dry)
+            {
+                memset(hull, 0, sizeof(*hull));
+                hull->vertexCount = counts[0];
+                hull->faceCount = counts[1];
+                hull->indexCount = counts[2];
+                hull->edgeCount = counts[3];
+            }
+            k->cursor += 16;
+        }
+        else
+        {
+            Walk(k, &hull->vertexCount, 4);
+            Walk(k, &hull->faceCount, 4);
+            Walk(k, &hull->indexCount, 4);
+            Walk(k, &hull->edgeCount, 4);
+        }
+        if (counts[0] > 0)
+        {
+            Walk(k, &hull->unitMass, 4);
+            Walk(k, &hull->unitCom, 12);
+            Walk(k, &hull->unitInertiaCom, 36);
+            Walk(k, &hull->center, 12);
+            Walk(k, hull->vertices, (int64_t)counts[0] * 12);
+            Walk(k, hull->faceNormals, (int64_t)counts[1] * 12);
+            Walk(k, hull->faceOffsets, (int64_t)counts[1] * 4);
+            Walk(k, hull->faceVertCounts, counts[1]);
+            Walk(k, hull->faceVertStart, (int64_t)counts[1] * 2);
+            Walk(k, hull->faceIndices, counts[2]);
+            Walk(k, hull->edges, (int64_t)counts[3] * (int64_t)sizeof(m3HullHalfEdge));
+        }
+    }
+    WalkFixed(k, (int64_t)w->shapeCapacity * 3, 4, NULL);
+}
+
+static void PoolToHeader(const m3IdPool* p, int32_t* maxIndex, int32_t* freeHead, int32_t* freeCount,
+                         int32_t* retired)
+{
+    *maxIndex = p->maxIndex;
+    *freeHead = p->freeHead;
+    *freeCount = p->freeCount;
+    *retired = p->retiredCount;
+}
+
+static bool PoolHeaderSane(int32_t maxIndex, int32_t freeHead, int32_t freeCount, int32_t retired,
+                           int32_t cap) // src/snapshot.c:704-707
+{
+    return maxIndex >= 0 && maxIndex <= cap && freeHead >= -1 && freeHead < cap && freeCount >= 0 &&
+           freeCount <= cap && retired >= 0 && retired <= cap;
+}
+
+// The host arrays the snapshot reads, brought up to date: device state,
+// the tree (rebuilt when stale), and the pair table (the device's table
+// while the mirror keeps one, else the live list and a zero tail).
+static bool SnapshotPrepare(m3World* w)
+{
+    if (w->faulted != 0 || !SyncHost(w) || !TreeFresh(w))
+    {
+        return false;
+    }
+    if (w->pairTableLive != 0 && w->pairTableOnDevice != 0)
+    {
+        m3gpuHostState h;
+        HostView(w, &h);
+        if (m3gpu_download(w->dev, &h, M3GPU_FIELDS_PAIR_TABLE) != 0)
+        {
+            DeviceFail(w, "m3gpu_download");
+            return false;
+        }
+    }
+    else if (w->pairTableLive == 0)
+    {
+        const int32_t tail = w->pairCapacity - w->pairCount;
+        memset(w->pairKeys + w->pairCount, 0, (size_t)tail * sizeof(uint64_t));
+        memset(w->manifolds + w->pairCount, 0, (size_t)tail * sizeof(m3Manifold));
+    }
+    return true;
 }
 
 int32_t m3World_SnapshotSize(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);
-    if (w == NULL || !SyncHost(w))
+    if (w == NULL || w->faulted != 0)
     {
         return -1;
     }
     m3Walker k;
     memset(&k, 0, sizeof(k));
     k.limit = -1;
-    WalkWorld(&k, w, w->bodyPool.maxIndex, w->shapePool.maxIndex, w->hullPool.maxIndex,
-              w->pairCount, w->sleepingPairCount);
-    int64_t total = (int64_t)sizeof(m3SnapHeader) + k.cursor;
+    WalkBlocks(&k, w);
+    WalkTails(&k, w);
+    const int64_t total = (int64_t)sizeof(m3SnapHeader) + k.cursor;
     return total > 0x7FFFFFFF ? -1 : (int32_t)total;
 }
 
 int32_t m3World_Snapshot(m3WorldId worldId, void* out, int32_t capacity)
 {
     m3World* w = WorldFromId(worldId);
-    int32_t size = m3World_SnapshotSize(worldId);
-    if (w == NULL || out == NULL || size < 0 || capacity < size)
+    if (w == NULL || out == NULL || !SnapshotPrepare(w))
+    {
+        return -1;
+    }
+    const int32_t size = m3World_SnapshotSize(worldId);
+    if (size < 0 || capacity < size)
     {
         return -1;
     }
     m3SnapHeader hdr;
     memset(&hdr, 0, sizeof(hdr));
     hdr.magic = M3_SNAP_MAGIC;
-    hdr.version = M3_SNAP_VERSION;
-    hdr.recordSizes = ((uint32_t)sizeof(m3Manifold) << 16) | (uint32_t)sizeof(m3Transform);
-    hdr.hullSize = (uint32_t)sizeof(m3HullData);
-    hdr.bodyCapacity = w->bodyCapacity;
-    hdr.shapeCapacity = w->shapeCapacity;
-    hdr.pairCapacity = w->pairCapacity;
-    hdr.bodyMax = w->bodyPool.maxIndex;
-    hdr.shapeMax = w->shapePool.maxIndex;
-    hdr.hullMax = w->hullPool.maxIndex;
-    hdr.pairCount = w->pairCount;
-    hdr.sleepingPairCount = w->sleepingPairCount;
-    hdr.jointCapacity = w->jointCapacity;
+    hdr.formatVersion = M3_SNAP_VERSION;
+    hdr.configHash = SnapConfigHash();
     hdr.stepCount = w->stepCount;
+    hdr.bodyCapacity = w->bodyCapacity;
+    PoolToHeader(&w->bodyPool, &hdr.maxIndex, &hdr.freeHead, &hdr.freeCount, &hdr.retiredCount);
+    hdr.shapeCapacity = w->shapeCapacity;
+    PoolToHeader(&w->shapePool, &hdr.shapeMaxIndex, &hdr.shapeFreeHead, &hdr.shapeFreeCount,
+                 &hdr.shapeRetiredCount);
+    hdr.pairCount = w->pairCount;
+    hdr.treeRoot = w->tree.root;
+    hdr.treeFreeList = w->tree.freeList;
+    PoolToHeader(&w->hullPool, &hdr.hullMaxIndex, &hdr.hullFreeHead, &hdr.hullFreeCount,
+                 &hdr.hullRetiredCount);
+    hdr.meshCapacity = w->meshCapacity;
+    hdr.jointCapacity = w->jointCapacity;
+    PoolToHeader(&w->jointPool, &hdr.jointMaxIndex, &hdr.jointFreeHead, &hdr.jointFreeCount,
+                 &hdr.jointRetiredCount);
+    hdr.charCapacity = w->characterCapacity;
+    hdr.voxelCapacity = w->voxelCapacity;
+    hdr.gravity = w->gravity;
     m3Walker k;
     memset(&k, 0, sizeof(k));
     k.out = (uint8_t*)out + sizeof(hdr);
     k.limit = size - (int64_t)sizeof(hdr);
-    WalkWorld(&k, w, hdr.bodyMax, hdr.shapeMax, hdr.hullMax, hdr.pairCount, hdr.sleepingPairCount);
-    hdr.payloadHash = m3HashBytes(M3_HASH_INIT, (uint8_t*)out + sizeof(hdr), size - (int32_t)sizeof(hdr));
+    WalkBlocks(&k, w);
+    WalkTails(&k, w);
     memcpy(out, &hdr, sizeof(hdr));
-    return size;
+    return k.failed ? -1 : size;
 }
 
 bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size)
@@ -3813,46 +4153,134 @@ bool m3World_Restore(m3WorldId worldId, const void* data, int32_t size)
     }
     m3SnapHeader hdr;
     memcpy(&hdr, data, sizeof(hdr));
-    if (hdr.magic != M3_SNAP_MAGIC || hdr.version != M3_SNAP_VERSION ||
-        hdr.recordSizes != (((uint32_t)sizeof(m3Manifold) << 16) | (uint32_t)sizeof(m3Transform)) ||
-        hdr.hullSize != (uint32_t)sizeof(m3HullData) || hdr.bodyCapacity != w->bodyCapacity ||
-        hdr.shapeCapacity != w->shapeCapacity || hdr.pairCapacity != w->pairCapacity ||
-        hdr.bodyMax < 0 || hdr.bodyMax > w->bodyCapacity || hdr.shapeMax < 0 ||
-        hdr.shapeMax > w->shapeCapacity || hdr.hullMax < 0 || hdr.hullMax > w->shapeCapacity ||
-        hdr.pairCount < 0 || hdr.pairCount > w->pairCapacity || hdr.sleepingPairCount < 0 ||
-        hdr.sleepingPairCount > w->pairCapacity || hdr.jointCapacity != w->jointCapacity)
+    if (hdr.magic != M3_SNAP_MAGIC || hdr.formatVersion != M3_SNAP_VERSION ||
+        hdr.configHash != SnapConfigHash() || hdr.bodyCapacity != w->bodyCapacity ||
+        hdr.shapeCapacity != w->shapeCapacity || hdr.meshCapacity != w->meshCapacity ||
+        hdr.jointCapacity != w->jointCapacity || hdr.voxelCapacity != w->voxelCapacity ||
+        hdr.charCapacity != w->characterCapacity)
     {
         return false;
     }
-    if (m3HashBytes(M3_HASH_INIT, (const uint8_t*)data + sizeof(hdr),
-                    size - (int32_t)sizeof(hdr)) != hdr.payloadHash)
+    if (!PoolHeaderSane(hdr.maxIndex, hdr.freeHead, hdr.freeCount, hdr.retiredCount, w->bodyCapacity) ||
+        !PoolHeaderSane(hdr.shapeMaxIndex, hdr.shapeFreeHead, hdr.shapeFreeCount, hdr.shapeRetiredCount,
+                        w->shapeCapacity) ||
+        !PoolHeaderSane(hdr.hullMaxIndex, hdr.hullFreeHead, hdr.hullFreeCount, hdr.hullRetiredCount,
+                        w->shapeCapacity) ||
+        !PoolHeaderSane(hdr.jointMaxIndex, hdr.jointFreeHead, hdr.jointFreeCount,
+                        hdr.jointRetiredCount, w->jointCapacity) ||
+        hdr.pairCount < 0 || hdr.pairCount > w->pairCapacity || hdr.treeRoot < -1 ||
+        hdr.treeRoot >= w->tree.capacity || hdr.treeFreeList < -1 || hdr.treeFreeList >= w->tree.capacity)
     {
         return false;
     }
-    // Measure with the header's counts: the payload must be exactly that.
+    // Pools this build does not simulate must be empty.
+    if ((hdr.meshMaxIndex | hdr.meshFreeHead | hdr.meshFreeCount | hdr.meshRetiredCount |
+         hdr.voxelMaxIndex | hdr.voxelFreeHead | hdr.voxelFreeCount | hdr.voxelRetiredCount |
+         hdr.charMaxIndex | hdr.charFreeHead | hdr.charFreeCount | hdr.charRetiredCount) != 0)
+    {
+        return false;
+    }
+    // Verify the whole payload before anything lands (restore is atomic).
+    const uint8_t* payload = (const uint8_t*)data + sizeof(hdr);
     m3Walker k;
     memset(&k, 0, sizeof(k));
-    k.limit = -1;
-    WalkWorld(&k, w, hdr.bodyMax, hdr.shapeMax, hdr.hullMax, hdr.pairCount, hdr.sleepingPairCount);
-    if (k.cursor != (int64_t)size - (int64_t)sizeof(hdr))
+    k.in = payload;
+    k.dry = 1;
+    k.limit = size - (int64_t)sizeof(hdr);
+    k.treeField = w->tree.nodes;
+    k.proxyField = w->proxyIds;
+    WalkBlocks(&k, w);
+    WalkTails(&k, w);
+    if (k.failed || k.cursor != k.limit)
     {
         return false;
     }
-    // Validation passed: now write.  Pending device results are dropped, and
-    // so are the event streams (events are observations, not state).
+    // The tree must be well formed and agree with the proxy ids: every id
+    // is -1 or a leaf that names its shape, and every leaf is some shape's.
+    {
+        m3Tree probe;
+        if (!m3TreeCreate(&probe, w->tree.capacity))
+        {
+            return false;
+        }
+        memcpy(probe.nodes, payload + k.treeAt, (size_t)probe.capacity * sizeof(m3TreeNode));
+        probe.root = hdr.treeRoot;
+        probe.freeList = hdr.treeFreeList;
+        bool ok = m3TreeValid(&probe);
+        int32_t leaves = 0;
+        for (int32_t i = 0; ok && i < probe.capacity; ++i)
+        {
+            leaves += probe.nodes[i].height == 0;
+        }
+        int32_t named = 0;
+        for (int32_t sh = 0; ok && sh < w->shapeCapacity; ++sh)
+        {
+            int32_t id;
+            memcpy(&id, payload + k.proxyAt + 4 * (int64_t)sh, 4);
+            if (id == M3_TREE_NULL)
+            {
+                continue;
+            }
+            ok = id >= 0 && id < probe.capacity && probe.nodes[id].height == 0 &&
+                 probe.nodes[id].userData == sh;
+            named += 1;
+        }
+        ok = ok && named == leaves;
+        m3TreeDestroy(&probe);
+        if (!ok)
+        {
+            return false;
+        }
+    }
+    // Validation passed: now write.  Pending device results are dropped,
+    // and so are the event streams (src/snapshot.c:878-890).
     w->hostStale = 0;
     w->eventsState = 0;
     memset(w->eventCounts, 0, sizeof(w->eventCounts));
     memset(&k, 0, sizeof(k));
-    k.in = (const uint8_t*)data + sizeof(hdr);
+    k.in = payload;
     k.limit = size - (int64_t)sizeof(hdr);
-    WalkWorld(&k, w, hdr.bodyMax, hdr.shapeMax, hdr.hullMax, hdr.pairCount, hdr.sleepingPairCount);
-    w->pairCount = hdr.pairCount;
-    w->sleepingPairCount = hdr.sleepingPairCount;
+    WalkBlocks(&k, w);
+    WalkTails(&k, w);
     w->stepCount = hdr.stepCount;
-    // The frozen buffer is rebuilt from the sleeping state on the next step
-    // (src/snapshot.c:893-894).
+    w->gravity = hdr.gravity;
+    w->bodyPool.maxIndex = hdr.maxIndex;
+    w->bodyPool.freeHead = hdr.freeHead;
+    w->bodyPool.freeCount = hdr.freeCount;
+    w->bodyPool.retiredCount = hdr.retiredCount;
+    w->shapePool.maxIndex = hdr.shapeMaxIndex;
+    w->shapePool.freeHead = hdr.shapeFreeHead;
+    w->shapePool.freeCount = hdr.shapeFreeCount;
+    w->shapePool.retiredCount = hdr.shapeRetiredCount;
+    w->hullPool.maxIndex = hdr.hullMaxIndex;
+    w->hullPool.freeHead = hdr.hullFreeHead;
+    w->hullPool.freeCount = hdr.hullFreeCount;
+    w->hullPool.retiredCount = hdr.hullRetiredCount;
+    w->jointPool.maxIndex = hdr.jointMaxIndex;
+    w->jointPool.freeHead = hdr.jointFreeHead;
+    w->jointPool.freeCount = hdr.jointFreeCount;
+    w->jointPool.retiredCount = hdr.jointRetiredCount;
+    w->pairCount = hdr.pairCount;
+    w->tree.root = hdr.treeRoot;
+    w->tree.freeList = hdr.treeFreeList;
+    w->treeStale = 0;
+    // The device's leaf boxes are the tree's leaves.
+    for (int32_t sh = 0; sh < w->shapeCapacity; ++sh)
+    {
+        const int32_t id = w->proxyIds[sh];
+        if (id >= 0)
+        {
+            memcpy(&w->leafBoxes[6 * sh], w->tree.nodes[id].lo, 3 * sizeof(double));
+            memcpy(&w->leafBoxes[6 * sh + 3], w->tree.nodes[id].hi, 3 * sizeof(double));
+        }
+    }
+    // The frozen-pair buffer is derived from a pair list the restore just
+    // replaced: rebuilt by the next step (src/snapshot.c:893-894).
+    w->sleepingPairCount = 0;
     w->pairsFullQuery = 1;
+    // The pair table holds the snapshot's whole capacity again.
+    w->pairTableLive = 1;
+    w->pairTableOnDevice = 0;
     w->devDirty = M3GPU_FIELDS_ALL;
     return true;
 }

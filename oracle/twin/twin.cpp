@@ -176,6 +176,9 @@ struct m3gpuWorld
     int ccdNeedsTree; // continuous sweeps meet tree shapes (static non-plane shapes or bullets)
     int begun;        // m3gpu_step_begin ran; m3gpu_step_end finishes the step
     int beginStalled;
+    int pairTable;    // keep the pair table in the reference's layout
+    std::vector<uint64_t> tableKeys;
+    std::vector<::m3Manifold> tableMans;
     tw::m3HitRec* hitRecBuf;
 };
 
@@ -414,6 +417,23 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
         memcpy(v.leaf, h->leafBoxes, (size_t)6 * ns * sizeof(double));
         v.sc->fullQuery = *h->pairsFullQuery != 0;
     }
+    if (f & M3GPU_FIELDS_PAIR_TABLE)
+    {
+        const ::m3Manifold* mans = (const ::m3Manifold*)h->manifolds;
+        w->tableKeys.assign(h->pairKeys, h->pairKeys + v.pairCapacity);
+        w->tableMans.assign(mans, mans + v.pairCapacity);
+    }
+    return 0;
+}
+
+extern "C" int m3gpu_set_pair_table(m3gpuWorld* w, int32_t on)
+{
+    w->pairTable = on != 0;
+    if (on && w->tableKeys.empty())
+    {
+        w->tableKeys.assign(w->v.pairCapacity, 0);
+        w->tableMans.assign(w->v.pairCapacity, ::m3Manifold());
+    }
     return 0;
 }
 
@@ -454,6 +474,16 @@ extern "C" int m3gpu_download(m3gpuWorld* w, m3gpuHostState* h, uint32_t f)
         memcpy(h->sleepingPairKeys, v.sleepKeys, (size_t)v.sc->sleepCount * sizeof(uint64_t));
         memcpy(h->leafBoxes, v.leaf, (size_t)6 * ns * sizeof(double));
         *h->frozenDirty = 0;
+    }
+    if (f & M3GPU_FIELDS_PAIR_TABLE)
+    {
+        if (w->tableKeys.empty())
+        {
+            snprintf(s_err, sizeof(s_err), "twin: no pair table");
+            return -1;
+        }
+        memcpy(h->pairKeys, w->tableKeys.data(), w->tableKeys.size() * sizeof(uint64_t));
+        memcpy(h->manifolds, w->tableMans.data(), w->tableMans.size() * sizeof(::m3Manifold));
     }
     return 0;
 }
@@ -514,6 +544,14 @@ static void StepEnd(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape, c
     s.masks = w->masks.data();
     tw::m3SerialParams sp = Params(p);
     tw::m3SerialStepEnd(&v, &s, &sp, tape);
+    if (w->pairTable)
+    {
+        for (int32_t i = 0; i < v.sc->pairCount; ++i)
+        {
+            w->tableKeys[i] = v.keys[i];
+            w->tableMans[i] = Mb(v.manifolds[i]);
+        }
+    }
 }
 
 static int Step(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
