@@ -326,6 +326,43 @@ M3_API bool m3World_EnableTreeMirror(m3WorldId worldId, bool flag);
 // True when the host tree equals the reference's for this world's history.
 M3_API bool m3World_IsTreeMirrorCurrent(m3WorldId worldId);
 M3_API void m3World_RebuildBroadphase(m3WorldId worldId); // world.h:100
+
+// The M3J1 replay container and the divergence finder
+// (include/maul3d/replay.h:15-87): the same structs, framing and ordering.
+typedef struct m3JournalInfo // replay.h:25-29
+{
+    int32_t opCount;
+    int32_t stepCount;
+} m3JournalInfo;
+
+typedef struct m3ReplayView // replay.h:35-43
+{
+    const void* snapshot;
+    int32_t snapshotBytes;
+    const void* journal;
+    int32_t journalBytes;
+    uint64_t finalHash;
+    m3JournalInfo info;
+} m3ReplayView;
+
+typedef struct m3BodyDiff // replay.h:63-71
+{
+    m3BodyId body;
+    float positionError;
+    float rotationError;
+    float velocityError;
+    bool onlyInA;
+    bool onlyInB;
+} m3BodyDiff;
+
+M3_API bool m3JournalDescribe(const void* journal, int32_t bytes, m3JournalInfo* out); // replay.h:31
+M3_API int32_t m3ReplayEncodeSize(int32_t snapshotBytes, int32_t journalBytes);        // replay.h:46
+M3_API int32_t m3ReplayEncode(const void* snapshot, int32_t snapshotBytes, const void* journal,
+                              int32_t journalBytes, uint64_t finalHash, void* out,
+                              int32_t capacity);                                       // replay.h:52
+M3_API bool m3ReplayDecode(const void* data, int32_t bytes, m3ReplayView* out);        // replay.h:58
+M3_API int32_t m3World_DiffReport(m3WorldId worldA, m3WorldId worldB, m3BodyDiff* out,
+                                  int32_t capacity, int32_t* outCount);                // replay.h:80
 // Trajectory gradients (no reference counterpart; the reference has no
 // differentiable path).  For each world, from its current state: `steps`
 // steps of (timeStep, subStepCount), `force[b]` applied to dynamic body slot

@@ -72,14 +72,16 @@ def build(verbose: bool = False) -> str:
     world_o = os.path.join(BUILD, "world.o")
     hull_o = os.path.join(BUILD, "hull_build.o")
     tree_o = os.path.join(BUILD, "m3_tree.o")
+    replay_o = os.path.join(BUILD, "replay.o")
     dev_o = os.path.join(BUILD, "m3_device.o")
     _run(["gcc", *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, "world.c"), "-o", world_o])
     _run(["gcc", *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, "hull_build.c"), "-o", hull_o])
     _run(["gcc", *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, "m3_tree.c"), "-o", tree_o])
+    _run(["gcc", *HOST_FLAGS, *inc, "-c", os.path.join(CSRC, "replay.c"), "-o", replay_o])
     _run([_hipcc(), *HIP_FLAGS, *inc, "-c", os.path.join(CSRC, "m3_device.hip"), "-o", dev_o])
     tmp = LIB + ".tmp"
     _run([_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp,
-          world_o, hull_o, tree_o, dev_o])
+          world_o, hull_o, tree_o, replay_o, dev_o])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -2535,6 +2535,92 @@ void m3World_RebuildBroadphase(m3WorldId worldId)
     TreeRebuild(w);
 }
 
+// The divergence finder's body comparison (src/diff.c): every body slot of
+// two worlds of equal capacity, rows of the slots that differ, worst first
+// (presence, then position + rotation + velocity error), slot ascending on
+// ties.  A read: both worlds are brought up to date from their devices.
+static float DiffTotal(const m3BodyDiff* d)
+{
+    const float presence = (d->onlyInA || d->onlyInB) ? 1.0e30f : 0.0f;
+    return presence + d->positionError + d->rotationError + d->velocityError;
+}
+
+static int DiffOrder(const void* pa, const void* pb)
+{
+    const m3BodyDiff* a = (const m3BodyDiff*)pa;
+    const m3BodyDiff* b = (const m3BodyDiff*)pb;
+    const float ta = DiffTotal(a);
+    const float tb = DiffTotal(b);
+    if (ta != tb)
+    {
+        return ta > tb ? -1 : 1;
+    }
+    return (a->body.index1 > b->body.index1) - (a->body.index1 < b->body.index1);
+}
+
+int32_t m3World_DiffReport(m3WorldId worldA, m3WorldId worldB, m3BodyDiff* out, int32_t capacity,
+                           int32_t* outCount)
+{
+    m3World* a = WorldFromId(worldA);
+    m3World* b = WorldFromId(worldB);
+    if (outCount != NULL)
+    {
+        *outCount = 0;
+    }
+    if (a == NULL || b == NULL || out == NULL || capacity <= 0 || outCount == NULL ||
+        a->bodyCapacity != b->bodyCapacity || !SyncHost(a) || !SyncHost(b))
+    {
+        return -1;
+    }
+    const int32_t maxA = a->bodyPool.maxIndex;
+    const int32_t maxB = b->bodyPool.maxIndex;
+    const int32_t n = maxA > maxB ? maxA : maxB;
+    m3BodyDiff* rows = (m3BodyDiff*)Zalloc((size_t)(n > 0 ? n : 1) * sizeof(m3BodyDiff));
+    int32_t found = 0;
+    for (int32_t i = 0; i < n; ++i)
+    {
+        const int inA = i < maxA && a->bodyPool.alive[i] != 0;
+        const int inB = i < maxB && b->bodyPool.alive[i] != 0;
+        if (!inA && !inB)
+        {
+            continue;
+        }
+        m3BodyDiff d;
+        memset(&d, 0, sizeof(d));
+        d.body.index1 = i + 1;
+        d.body.world0 = a->worldIndex0;
+        d.body.generation = inA ? a->bodyPool.generations[i] : b->bodyPool.generations[i];
+        if (inA != inB)
+        {
+            d.onlyInA = inA != 0;
+            d.onlyInB = inB != 0;
+            rows[found++] = d;
+            continue;
+        }
+        const m3Transform* xa = &a->transforms[i];
+        const m3Transform* xb = &b->transforms[i];
+        const m3Vec3 dp = m3V3((float)(xa->p.x - xb->p.x), (float)(xa->p.y - xb->p.y),
+                               (float)(xa->p.z - xb->p.z));
+        const float qdot = xa->q.x * xb->q.x + xa->q.y * xb->q.y + xa->q.z * xb->q.z + xa->q.w * xb->q.w;
+        const m3Vec3 dv = m3Sub3(a->linearVelocities[i], b->linearVelocities[i]);
+        const m3Vec3 dw = m3Sub3(a->angularVelocities[i], b->angularVelocities[i]);
+        d.positionError = m3Length3(dp);
+        d.rotationError = 1.0f - (qdot < 0.0f ? -qdot : qdot);
+        d.velocityError = m3Length3(dv) + m3Length3(dw);
+        if (d.positionError != 0.0f || d.rotationError != 0.0f || d.velocityError != 0.0f ||
+            a->awake[i] != b->awake[i])
+        {
+            rows[found++] = d;
+        }
+    }
+    qsort(rows, (size_t)found, sizeof(m3BodyDiff), DiffOrder);
+    const int32_t written = found < capacity ? found : capacity;
+    memcpy(out, rows, (size_t)written * sizeof(m3BodyDiff));
+    *outCount = written;
+    free(rows);
+    return found;
+}
+
 const char* m3World_GetDeviceError(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);

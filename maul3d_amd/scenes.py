@@ -708,4 +708,7 @@ SCENES = {
     "hull_pile": hull_pile,
     "crowded_plank": crowded_plank,
     "event_rain": event_rain,
+    "hulljam": hulljam,
+    "bounce": bounce,
+    "ccd_arena": ccd_arena,
 }

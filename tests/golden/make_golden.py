@@ -7,7 +7,8 @@ after selected steps of m3World_Step(1/60, 4), plus the counters the tests
 use to size their checks.  The output, hashes.json, is data: inputs are the
 seeded scene definitions, outputs the reference's hashes.  ``events``
 writes events.json: per step of the event rain, the hash and a digest of
-each event stream.
+each event stream.  ``snapshots`` writes snapshots.json: the size and
+SHA-256 of the reference's snapshot bytes at selected steps of two scenes.
 """
 
 from __future__ import annotations
@@ -113,8 +114,38 @@ def events_fixture(ref: api.Api) -> dict:
     return out
 
 
+# scene -> steps after which the snapshot's digest is recorded
+SNAPSHOT_PLAN = {"rain": (1, 100, 600), "hull_chains": (1, 60)}
+
+
+def snapshots_fixture(ref: api.Api) -> dict:
+    """The reference's snapshot bytes (size and SHA-256) of the 10k rain and
+    the hull-rain-with-chains scene at selected steps."""
+    import hashlib
+
+    out = {}
+    for name, marks in SNAPSHOT_PLAN.items():
+        world = scenes.SCENES[name](ref)
+        rec = {}
+        for i in range(1, max(marks) + 1):
+            ref.m3World_Step(world, 1.0 / 60.0, 4)
+            if i in marks:
+                snap = ref.snapshot(world)
+                rec[str(i)] = {"bytes": len(snap), "sha256": hashlib.sha256(snap).hexdigest()}
+        out[name] = rec
+        ref.m3DestroyWorld(world)
+        print("snapshots", name, flush=True)
+    return out
+
+
 def main() -> None:
     ref = api.Api(C.CDLL(REF))
+    if len(sys.argv) > 1 and sys.argv[1] == "snapshots":
+        data = snapshots_fixture(ref)
+        with open(os.path.join(HERE, "snapshots.json"), "w") as f:
+            json.dump(data, f, indent=1, sort_keys=True)
+        print("wrote snapshots.json")
+        return
     if len(sys.argv) > 2 and sys.argv[1] == "scene":
         # Refresh one scene's entry in hashes.json.
         path = os.path.join(HERE, "hashes.json")
