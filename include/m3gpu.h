@@ -97,6 +97,7 @@ typedef struct m3gpuHostState
     float* shapeRollingResistance;
     uint8_t* shapeSensor;
     uint8_t* shapeHitEvents;
+    uint8_t* shapePreSolve;
     uint8_t* shapeHasOffset;
     void* shapeLocalPos;   // m3Vec3
     void* shapeLocalRot;   // m3Quat
@@ -220,7 +221,37 @@ M3GPU_API int m3gpu_step_supported(const m3gpuWorld* world, const m3gpuStepParam
 // tree's leaf moves from them and passes every shape's depth-first rank
 // (-1: not in the tree) to m3gpu_step_end, which enqueues the rest.
 M3GPU_API int m3gpu_step_needs_tree(const m3gpuWorld* world, const m3gpuStepParams* params);
-M3GPU_API int m3gpu_step_begin(m3gpuWorld* world, const m3gpuStepParams* params, int32_t wantLeaves);
+
+// Flags of m3gpu_step_begin.
+#define M3GPU_STEP_LEAVES 0x1u   // copy the step's leaf boxes for m3gpu_step_leaves
+#define M3GPU_STEP_PRESOLVE 0x2u // stop after the island wake pass for the pre-solve callback
+
+// The pre-solve callback's inputs for one candidate pair
+// (src/solver.c:214-236): the pair's key and index in this step's list, the
+// deepest contact point and the normal (A to B).
+typedef struct m3gpuPreSolve
+{
+    uint64_t key;
+    int32_t pair;
+    int32_t pad;
+    double point[3];
+    float normal[3];
+    float pad2;
+} m3gpuPreSolve;
+
+// m3gpu_step_begin enqueues the step up to the continuous pass.  With
+// M3GPU_STEP_PRESOLVE it stops after the island wake pass instead:
+// m3gpu_step_presolve waits and returns the candidates in pair order (host
+// memory owned by the device layer, valid until the next step), and
+// m3gpu_step_vetoes takes the keys the host vetoed (ascending) and enqueues
+// the rest up to the continuous pass.  Without it, `vetoKeys` (ascending;
+// null or count 0 for none) are vetoed directly, as a replayed journal's
+// recorded vetoes are (src/solver.c:184-212).
+M3GPU_API int m3gpu_step_begin(m3gpuWorld* world, const m3gpuStepParams* params, uint32_t flags,
+                               const uint64_t* vetoKeys, int32_t vetoCount);
+M3GPU_API int m3gpu_step_presolve(m3gpuWorld* world, const m3gpuPreSolve** candidates, int32_t* count);
+M3GPU_API int m3gpu_step_vetoes(m3gpuWorld* world, const m3gpuStepParams* params, const uint64_t* vetoKeys,
+                                int32_t vetoCount);
 M3GPU_API const double* m3gpu_step_leaves(m3gpuWorld* world);
 M3GPU_API int m3gpu_step_end(m3gpuWorld* world, const m3gpuStepParams* params, const int32_t* ranks);
 

@@ -312,6 +312,17 @@ M3_API void m3World_Sync(m3WorldId worldId);
 // returns 0, m3World_Snapshot returns -1 and m3World_Restore false.  Returns
 // the failure text, or null for a healthy world or a bad id.
 M3_API const char* m3World_GetDeviceError(m3WorldId worldId);
+// Pre-solve veto (include/maul3d/world.h:315-340, shape.h:302-303): the
+// callback runs on the host in pair order for every touching pair with an
+// awake dynamic side where either shape opted in; returning false drops the
+// contact for that step.  The device step pauses after its island wake pass
+// while the host calls it.  Vetoes are journaled before their step and a
+// replayed journal applies them with the callback silent.
+typedef bool m3PreSolveFn(m3ShapeId shapeA, m3ShapeId shapeB, m3Pos3 point, m3Vec3 normal,
+                          void* context);
+M3_API void m3World_SetPreSolveCallback(m3WorldId worldId, m3PreSolveFn* fn, void* context); // world.h:340
+M3_API void m3Shape_EnablePreSolve(m3ShapeId shapeId, bool flag);                            // shape.h:302
+M3_API bool m3Shape_IsPreSolveEnabled(m3ShapeId shapeId);                                   // shape.h:303
 // The broadphase tree (no reference counterpart: the reference always keeps
 // its tree; the device step does not need one).  With the mirror on, the
 // host replays every step's leaf moves into the reference's tree

@@ -84,6 +84,15 @@ struct m3gpuWorld
     int fullQuery; // rebuild the frozen-pair buffer before the next step
     int ccdUnsupported;
     int ccdNeedsTree; // continuous sweeps meet tree shapes (static non-plane targets or bullets)
+    // pre-solve: candidate records and their count, the vetoed keys and
+    // per-pair flags of the step, pinned host staging
+    m3gpuPreSolve* preCands;
+    int32_t* preCount;
+    uint64_t* vetoKeys;
+    uint8_t* vetoFlags;
+    m3gpuPreSolve* preStage;
+    uint64_t* vetoStage;
+    hipEvent_t preGathered;
     // the pair table in the reference's layout (m3gpu_set_pair_table)
     uint64_t* tableKeys;
     m3Manifold* tableMans;
@@ -1316,6 +1325,74 @@ __global__ void k_wake_apply(const m3Dev* __restrict__ ds)
 // ---------------------------------------------------------------------------
 // Prepare and coloring (src/solver.c:156-404, 2741-2822)
 // ---------------------------------------------------------------------------
+
+// Pre-solve candidates of this step (src/solver.c:214-236), appended in any
+// order with their pair index (the host puts them in pair order).
+__global__ void k_presolve_gather(const m3Dev* __restrict__ ds, m3gpuPreSolve* out, int32_t* count)
+{
+    M3_WORLD;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || i >= v.sc->pairCount)
+    {
+        return;
+    }
+    double point[3];
+    m3Vec3 normal;
+    if (!m3PreSolveCandidate(&v, i, point, &normal))
+    {
+        return;
+    }
+    const int32_t k = atomicAdd(count, 1);
+    m3gpuPreSolve r;
+    r.key = v.keys[i];
+    r.pair = i;
+    r.pad = 0;
+    r.point[0] = point[0];
+    r.point[1] = point[1];
+    r.point[2] = point[2];
+    r.normal[0] = normal.x;
+    r.normal[1] = normal.y;
+    r.normal[2] = normal.z;
+    r.pad2 = 0.0f;
+    out[k] = r;
+}
+
+// One flag per pair of the current list: its key is among the step's
+// vetoed keys (ascending).
+__global__ void k_veto_mark(const m3Dev* __restrict__ ds, const uint64_t* keys, int32_t n, uint8_t* flags)
+{
+    M3_WORLD;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.pairCapacity)
+    {
+        return;
+    }
+    uint8_t hit = 0;
+    if (i < v.sc->pairCount)
+    {
+        const uint64_t key = v.keys[i];
+        int32_t lo = 0;
+        int32_t hi = n - 1;
+        while (lo <= hi)
+        {
+            const int32_t mid = (lo + hi) >> 1;
+            if (keys[mid] == key)
+            {
+                hit = 1;
+                break;
+            }
+            if (keys[mid] < key)
+            {
+                lo = mid + 1;
+            }
+            else
+            {
+                hi = mid - 1;
+            }
+        }
+    }
+    flags[i] = hit;
+}
 
 __global__ void k_constraint_flags(const m3Dev* __restrict__ ds)
 {
@@ -2910,7 +2987,8 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.localPos, ns) || DevAlloc(&v.localRot, ns) || DevAlloc(&v.surfaceVel, ns) ||
         DevAlloc(&v.category, ns) || DevAlloc(&v.mask, ns) || DevAlloc(&v.group, ns) ||
         DevAlloc(&v.shapeNext, ns) || DevAlloc(&v.hullIndex, ns) || DevAlloc(&v.leaf, 6 * (int64_t)ns) ||
-        DevAlloc(&v.fresh, 6 * (int64_t)ns) || DevAlloc(&v.planes, ns) || DevAlloc(&v.hitEvents, ns))
+        DevAlloc(&v.fresh, 6 * (int64_t)ns) || DevAlloc(&v.planes, ns) || DevAlloc(&v.hitEvents, ns) ||
+        DevAlloc(&v.preSolve, ns))
     {
         return -1;
     }
@@ -3076,6 +3154,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.minExt, v.maxExt, v.force, v.torque, v.enabled, v.locks, v.sleepThreshold,
                     v.canSleep, v.hasTarget, v.target, v.shapeHead, v.shapeAlive, v.shapeBody,
                     v.shapeType, v.geom, v.friction, v.restitution, v.rolling, v.sensor, v.hitEvents, w->hitRecBuf,
+                    v.preSolve, w->preCands, w->preCount, w->vetoKeys, w->vetoFlags,
                     v.hasOffset, v.localPos, v.localRot, v.surfaceVel, v.category, v.mask,
                     v.group, v.shapeNext, v.hullIndex, v.leaf, v.fresh, v.planes,
                     (void*)v.hulls, w->keysBuf[0], w->keysBuf[1], w->mansBuf[0], w->mansBuf[1],
@@ -3110,6 +3189,12 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
     if (w->leavesCopied != nullptr)
     {
         (void)hipEventDestroy(w->leavesCopied);
+    }
+    if (w->preStage != nullptr)
+    {
+        (void)hipHostFree(w->preStage);
+        (void)hipHostFree(w->vetoStage);
+        (void)hipEventDestroy(w->preGathered);
     }
     EventBufsFree(w);
     ProfFree(w);
@@ -3178,6 +3263,7 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP(v.rolling, h->shapeRollingResistance, ns, float);
         UP(v.sensor, h->shapeSensor, ns, uint8_t);
         UP(v.hitEvents, h->shapeHitEvents, ns, uint8_t);
+        UP(v.preSolve, h->shapePreSolve, ns, uint8_t);
         {
             // Hit records are kept (cleared by the narrowphase, written by the
             // impulse store) only in worlds where some shape opted in.
@@ -3631,10 +3717,11 @@ extern "C" int m3gpu_events(m3gpuWorld* w, float hitThreshold, m3gpuEvents* out)
 
 // Enqueue one step.  Launch geometry only depends on host-known capacities
 // and high-water marks; every data-dependent count is read on the device.
-// The step up to the continuous pass; EnqueueStepEnd enqueues the rest.
-// With `leaves` set, a copy of this step's leaf boxes into the pinned stage
-// follows the broadphase (m3gpu_step_leaves waits for it).
-static int EnqueueStepBegin(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
+// The step through the island wake pass (pair list, narrowphase, islands);
+// EnqueueStepB and EnqueueStepEnd enqueue the rest.  With `leaves` set, a
+// copy of this step's leaf boxes into the pinned stage follows the
+// broadphase (m3gpu_step_leaves waits for it).
+static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
 {
     hipStream_t st = w->stream;
     if (w->devDirty && RefreshDevCopies(w) != 0)
@@ -3717,6 +3804,41 @@ static int EnqueueStepBegin(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves
     STAGE(ST_ROOT, KL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_WAKE_MARK, KL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
     STAGE(ST_WAKE_APPLY, KL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Constraints, coloring, prepare, the substeps, restitution and the impulse
+// store: the step from after the island wake pass up to the continuous
+// pass.  With `vetoKeys` (ascending) those pairs make no constraint.
+static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t* vetoKeys, int32_t vetoCount)
+{
+    hipStream_t st = w->stream;
+    const m3Dev* D = w->devDev + w->cur;
+    const m3Dev& hd = w->dev;
+    const m3View& v = w->view;
+    const int T = 256;
+    const int32_t nb = v.maxBody;
+    const int32_t np = v.pairCapacity;
+    size_t tb = w->cubBytes;
+    void* ct = w->cubTemp;
+    const bool vetoes = vetoCount > 0;
+    if (vetoes != (w->view.veto != nullptr))
+    {
+        // The flags are read through the device copies of the world's entry.
+        w->view.veto = vetoes ? w->vetoFlags : nullptr;
+        if (RefreshDevCopies(w) != 0)
+        {
+            return -1;
+        }
+    }
+    if (vetoes)
+    {
+        memcpy(w->vetoStage, vetoKeys, (size_t)vetoCount * sizeof(uint64_t));
+        HIP_TRY(hipMemcpyAsync(w->vetoKeys, w->vetoStage, (size_t)vetoCount * sizeof(uint64_t),
+                               hipMemcpyHostToDevice, st));
+        KL(k_veto_mark, dim3(Blocks(np, T)), dim3(T), 0, st, D, w->vetoKeys, vetoCount, w->vetoFlags);
+    }
     // Prepare and color.
     const m3SubstepParams sp = SubstepParams(p);
     STAGE(ST_CONS_FLAGS, KL(k_constraint_flags, dim3(Blocks(np, T)), dim3(T), 0, st, D));
@@ -3904,19 +4026,105 @@ static int StepArgs(const m3gpuWorld* w, const m3gpuStepParams* p, const char* w
     return 0;
 }
 
-extern "C" int m3gpu_step_begin(m3gpuWorld* w, const m3gpuStepParams* p, int32_t wantLeaves)
+// Pinned staging and buffers of the pre-solve path, allocated at first use.
+static int PreSolveBuffers(m3gpuWorld* w)
+{
+    if (w->preStage != nullptr)
+    {
+        return 0;
+    }
+    const int32_t np = w->view.pairCapacity;
+    if (DevAlloc(&w->preCands, np) || DevAlloc(&w->preCount, 1) || DevAlloc(&w->vetoKeys, np) ||
+        DevAlloc(&w->vetoFlags, np))
+    {
+        return -1;
+    }
+    HIP_TRY(hipHostMalloc((void**)&w->vetoStage, (size_t)np * sizeof(uint64_t), hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&w->preGathered, hipEventDisableTiming));
+    HIP_TRY(hipHostMalloc((void**)&w->preStage, (size_t)np * sizeof(m3gpuPreSolve), hipHostMallocDefault));
+    return 0;
+}
+
+extern "C" int m3gpu_step_begin(m3gpuWorld* w, const m3gpuStepParams* p, uint32_t flags,
+                                const uint64_t* vetoKeys, int32_t vetoCount)
 {
     const int rc = StepArgs(w, p, "m3gpu_step_begin");
     if (rc != 0)
     {
         return rc;
     }
-    if (JoinBatch(w) != 0 || EnqueueStepBegin(w, p, wantLeaves != 0) != 0)
+    const bool presolve = (flags & M3GPU_STEP_PRESOLVE) != 0;
+    if ((presolve || vetoCount > 0) && PreSolveBuffers(w) != 0)
+    {
+        return -1;
+    }
+    if (vetoCount < 0 || vetoCount > w->view.pairCapacity || (vetoCount > 0 && vetoKeys == nullptr))
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step_begin: bad veto list");
+        return -1;
+    }
+    if (JoinBatch(w) != 0 || EnqueueStepA(w, p, (flags & M3GPU_STEP_LEAVES) != 0) != 0)
     {
         return -1;
     }
     w->streamBusy = 1;
+    if (presolve)
+    {
+        const int T = 256;
+        const m3Dev* D = w->devDev + w->cur;
+        HIP_TRY(hipMemsetAsync(w->preCount, 0, sizeof(int32_t), w->stream));
+        KL(k_presolve_gather, dim3(Blocks(w->view.pairCapacity, T)), dim3(T), 0, w->stream, D, w->preCands,
+           w->preCount);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(w->preGathered, w->stream));
+        return 0;
+    }
+    return EnqueueStepB(w, p, vetoKeys, vetoCount);
+}
+
+static int PreOrder(const void* a, const void* b)
+{
+    const int32_t x = ((const m3gpuPreSolve*)a)->pair;
+    const int32_t y = ((const m3gpuPreSolve*)b)->pair;
+    return (x > y) - (x < y);
+}
+
+extern "C" int m3gpu_step_presolve(m3gpuWorld* w, const m3gpuPreSolve** candidates, int32_t* count)
+{
+    if (w == nullptr || candidates == nullptr || count == nullptr || w->preStage == nullptr)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step_presolve: no pre-solve step begun");
+        return -1;
+    }
+    int32_t n = 0;
+    HIP_TRY(hipEventSynchronize(w->preGathered));
+    HIP_TRY(hipMemcpy(&n, w->preCount, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (n < 0 || n > w->view.pairCapacity)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step_presolve: bad candidate count");
+        return -1;
+    }
+    HIP_TRY(hipMemcpy(w->preStage, w->preCands, (size_t)n * sizeof(m3gpuPreSolve), hipMemcpyDeviceToHost));
+    qsort(w->preStage, (size_t)n, sizeof(m3gpuPreSolve), PreOrder);
+    *candidates = w->preStage;
+    *count = n;
     return 0;
+}
+
+extern "C" int m3gpu_step_vetoes(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t* vetoKeys,
+                                 int32_t vetoCount)
+{
+    const int rc = StepArgs(w, p, "m3gpu_step_vetoes");
+    if (rc != 0)
+    {
+        return rc;
+    }
+    if (vetoCount < 0 || vetoCount > w->view.pairCapacity || (vetoCount > 0 && vetoKeys == nullptr))
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_step_vetoes: bad veto list");
+        return -1;
+    }
+    return EnqueueStepB(w, p, vetoKeys, vetoCount);
 }
 
 extern "C" const double* m3gpu_step_leaves(m3gpuWorld* w)
@@ -3957,7 +4165,8 @@ extern "C" int m3gpu_step(m3gpuWorld* w, const m3gpuStepParams* p, m3gpuStepStat
                  "the tree ranks and m3gpu_step_end");
         return M3GPU_REFUSED;
     }
-    if (JoinBatch(w) != 0 || EnqueueStepBegin(w, p, false) != 0 || EnqueueStepEnd(w, p, nullptr) != 0)
+    if (JoinBatch(w) != 0 || EnqueueStepA(w, p, false) != 0 || EnqueueStepB(w, p, nullptr, 0) != 0 ||
+        EnqueueStepEnd(w, p, nullptr) != 0)
     {
         return -1;
     }

@@ -140,9 +140,10 @@ typedef struct m3SerialParams
 // The frozen buffer v->sleepKeys holds sorted keys; discovery appends in the
 // reference are merged the same way here: a pair discovered this step joins
 // the frozen set before the merge.
-// The step up to the continuous pass (m3SerialStepEnd runs the rest);
-// nonzero on a capacity stall (the world did not advance).
-M3_HD int m3SerialStepBegin(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
+// The step through the island wake pass (pair list, narrowphase, islands):
+// where the pre-solve callback's candidates are read.  Nonzero on a capacity
+// stall (the world did not advance).
+M3_HD int m3SerialStepA(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
 {
     m3Scalars* sc = v->sc;
     int32_t nb = v->maxBody;
@@ -398,6 +399,16 @@ M3_HD int m3SerialStepBegin(const m3View* v, m3SerialScratch* s, const m3SerialP
             v->sleepTime[i] = 0.0f;
         }
     }
+    return 0;
+}
+
+// From the constraint list (pairs vetoed through v->veto left out) to the
+// impulse store: everything before the continuous pass.
+M3_HD void m3SerialStepB(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
+{
+    m3Scalars* sc = v->sc;
+    const int32_t nb = v->maxBody;
+    const int32_t count = sc->pairCount;
     // Constraints, coloring, prepare in color-slot order.
     m3real h = p->dt / (m3real)p->substeps;
     m3real invH = h > 0.0f ? 1.0f / h : 0.0f;
@@ -631,6 +642,17 @@ M3_HD int m3SerialStepBegin(const m3View* v, m3SerialScratch* s, const m3SerialP
             tape->xfEnd[b] = v->xf[b];
         }
     }
+}
+
+// The step up to the continuous pass (m3SerialStepEnd runs the rest);
+// nonzero on a capacity stall (the world did not advance).
+M3_HD int m3SerialStepBegin(const m3View* v, m3SerialScratch* s, const m3SerialParams* p, m3Tape* tape)
+{
+    if (m3SerialStepA(v, s, p, tape) != 0)
+    {
+        return 1;
+    }
+    m3SerialStepB(v, s, p, tape);
     return 0;
 }
 

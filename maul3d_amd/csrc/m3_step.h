@@ -264,6 +264,10 @@ typedef struct m3View
     int32_t treeCcd;
     int32_t ccdStaticCount;
     int32_t* ccdStatics;
+    // pre-solve: per shape opt-in (m3Shape_EnablePreSolve) and, for the
+    // step that carries vetoes, one flag per pair of the current list
+    uint8_t* preSolve;
+    const uint8_t* veto;
 } m3View;
 
 // ---------------------------------------------------------------------------
@@ -1142,6 +1146,51 @@ M3_HD int m3PairIsConstraint(const m3View* v, int32_t i)
     {
         return 0;
     }
+    if (v->veto != NULL && v->veto[i] != 0)
+    {
+        return 0; // vetoed for this step (src/solver.c:184-248)
+    }
+    return 1;
+}
+
+// The pre-solve callback's inputs for pair i (src/solver.c:214-236): pairs
+// that would become constraints and where either shape opted in.  The point
+// is the deepest manifold point (ties to the lower index) in world space from
+// body B's centre of mass; the normal points from A to B.  Returns 0 when
+// the pair is no candidate.
+M3_HD int m3PreSolveCandidate(const m3View* v, int32_t i, double point[3], m3Vec3* normal)
+{
+    const m3Manifold* m = &v->manifolds[i];
+    if (m->pointCount == 0)
+    {
+        return 0;
+    }
+    const uint64_t key = v->keys[i];
+    const int32_t shapeA = (int32_t)(key >> 32);
+    const int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+    const int32_t bodyA = v->shapeBody[shapeA];
+    const int32_t bodyB = v->shapeBody[shapeB];
+    const int awakeDynA = v->type[bodyA] == M3_DYNAMIC && v->awake[bodyA] != 0;
+    const int awakeDynB = v->type[bodyB] == M3_DYNAMIC && v->awake[bodyB] != 0;
+    if ((!awakeDynA && !awakeDynB) || v->sensor[shapeA] != 0 || v->sensor[shapeB] != 0 ||
+        (v->preSolve[shapeA] == 0 && v->preSolve[shapeB] == 0))
+    {
+        return 0;
+    }
+    int32_t deep = 0;
+    for (int32_t k = 1; k < m->pointCount; ++k)
+    {
+        if (m->points[k].separation < m->points[deep].separation)
+        {
+            deep = k;
+        }
+    }
+    const m3Transform xf = v->xf[bodyB];
+    const m3Vec3 lc = m3RotateVec3(xf.q, v->lc[bodyB]);
+    point[0] = xf.p.x + (double)lc.x + (double)m->points[deep].anchorB.x;
+    point[1] = xf.p.y + (double)lc.y + (double)m->points[deep].anchorB.y;
+    point[2] = xf.p.z + (double)lc.z + (double)m->points[deep].anchorB.z;
+    *normal = m->normal;
     return 1;
 }
 

@@ -237,6 +237,17 @@ typedef struct m3World
     uint8_t pairTableLive;
     uint8_t pairTableOnDevice;
 
+    // Pre-solve veto (src/solver.c:184-248, include/maul3d/world.h:315-340):
+    // the host's callback (never state), this step's vetoed keys (the
+    // journal annex), and a replayed journal's recorded vetoes for the next
+    // step.
+    m3PreSolveFn* preSolveFn;
+    void* preSolveContext;
+    uint64_t* stepVetoKeys;
+    int32_t stepVetoCount;
+    uint64_t* replayVetoKeys;
+    int32_t replayVetoCount;
+
     m3IdPool hullPool;
     m3HullData* hullData;
     int32_t* hullRefCounts;
@@ -388,6 +399,7 @@ static void HostView(m3World* w, m3gpuHostState* h)
     h->shapeRollingResistance = w->shapeRollingResistance;
     h->shapeSensor = w->shapeSensor;
     h->shapeHitEvents = w->shapeHitEvents;
+    h->shapePreSolve = w->shapePreSolve;
     h->shapeHasOffset = w->shapeHasOffset;
     h->shapeLocalPos = w->shapeLocalPos;
     h->shapeLocalRot = w->shapeLocalRot;
@@ -591,7 +603,9 @@ enum
     M3_OP_ENABLE_CONTINUOUS = 48,
     M3_OP_SET_HIT_EVENT_THRESHOLD = 49,
     M3_OP_ENABLE_SHAPE_HIT_EVENTS = 50,
+    M3_OP_ENABLE_SHAPE_PRE_SOLVE = 51,
     M3_OP_REBUILD_BROADPHASE = 73,
+    M3_OP_STEP_VETOES = 79,
 };
 
 #define M3_HULL_MAX_INPUT 256
@@ -975,6 +989,8 @@ m3WorldId m3CreateWorld(const m3WorldDef* def)
     }
 
     w->pairCapacity = 8 * shapeCap;
+    ALLOC(w->stepVetoKeys, w->pairCapacity, uint64_t);
+    ALLOC(w->replayVetoKeys, w->pairCapacity, uint64_t);
     ALLOC(w->pairKeys, w->pairCapacity, uint64_t);
     ALLOC(w->manifolds, w->pairCapacity, m3Manifold);
     ALLOC(w->sleepingPairKeys, w->pairCapacity, uint64_t);
@@ -1016,6 +1032,7 @@ void m3DestroyWorld(m3WorldId worldId)
         w->jointBreak, w->jointSpring, w->jointTargetScalar, w->jointTargetQ,
         w->jointSpringImpulse, w->jointLimits, w->jointNextA, w->jointNextB, w->bodyJointHead,
         w->jointCompStart, w->jointCompList, w->jointScratch, w->proxyIds, w->treeRanks,
+        w->stepVetoKeys, w->replayVetoKeys,
     };
     for (size_t i = 0; i < sizeof(arrays) / sizeof(arrays[0]); ++i)
     {
@@ -2390,6 +2407,44 @@ static bool TreeFresh(m3World* w)
     return true;
 }
 
+static bool AnyPreSolveShape(const m3World* w)
+{
+    for (int32_t s = 0; s < w->shapePool.maxIndex; ++s)
+    {
+        if (w->shapePool.alive[s] != 0 && w->shapePreSolve[s] != 0)
+        {
+            return true;
+        }
+    }
+    return false;
+}
+
+// The serial pre-solve pass (src/solver.c:214-248): the callback in pair
+// order for every candidate the device gathered; the keys it vetoes (in pair
+// order, so ascending) go back to the device and into the journal annex.
+static int PreSolvePass(m3World* w, const m3gpuStepParams* p)
+{
+    const m3gpuPreSolve* cands = NULL;
+    int32_t n = 0;
+    if (m3gpu_step_presolve(w->dev, &cands, &n) != 0)
+    {
+        return -1;
+    }
+    for (int32_t k = 0; k < n; ++k)
+    {
+        const m3gpuPreSolve* c = &cands[k];
+        const int32_t a = (int32_t)(c->key >> 32);
+        const int32_t b = (int32_t)(c->key & 0xFFFFFFFFu);
+        m3Pos3 point = {c->point[0], c->point[1], c->point[2]};
+        m3Vec3 normal = m3V3(c->normal[0], c->normal[1], c->normal[2]);
+        if (!w->preSolveFn(ShapeIdOf(w, a), ShapeIdOf(w, b), point, normal, w->preSolveContext))
+        {
+            w->stepVetoKeys[w->stepVetoCount++] = c->key;
+        }
+    }
+    return m3gpu_step_vetoes(w->dev, p, w->stepVetoKeys, w->stepVetoCount);
+}
+
 // One step; false when the step did not run (bad arguments, a faulted world,
 // or a world the device step refuses), in which case nothing changed.
 static bool StepWorld(m3World* w, float dt, int32_t substeps)
@@ -2405,17 +2460,24 @@ static bool StepWorld(m3World* w, float dt, int32_t substeps)
     m3gpuStepParams p;
     FillParams(w, dt, substeps, &p);
     const bool treeCcd = m3gpu_step_needs_tree(w->dev, &p) != 0;
+    const bool tree = treeCcd || w->treeMirror != 0;
+    const bool replayVetoes = w->replayVetoCount > 0;
+    // A replayed journal's recorded vetoes own the step; the callback stays
+    // silent then (src/solver.c:184-212).
+    const bool presolve = !replayVetoes && w->preSolveFn != NULL && AnyPreSolveShape(w);
+    w->stepVetoCount = 0;
     int rc;
-    if (treeCcd || w->treeMirror != 0)
+    if (tree || presolve || replayVetoes)
     {
-        // The device stops before the continuous pass; the host replays
-        // the step's leaf moves and hands the tree's depth-first leaf
-        // ranks to the continuous pass.
-        if (!TreeFresh(w))
+        // The device stops where the host takes part: after the island
+        // wake pass for the pre-solve callback, before the continuous pass
+        // for the broadphase tree's leaf moves and the continuous pass's
+        // tree order.
+        if (tree && !TreeFresh(w))
         {
             return false;
         }
-        if (w->pairTableLive != 0 && w->pairTableOnDevice == 0)
+        if (tree && w->pairTableLive != 0 && w->pairTableOnDevice == 0)
         {
             // The device takes over the table from the host arrays.
             m3gpuHostState h;
@@ -2428,29 +2490,45 @@ static bool StepWorld(m3World* w, float dt, int32_t substeps)
             }
             w->pairTableOnDevice = 1;
         }
-        rc = m3gpu_step_begin(w->dev, &p, 1);
+        const uint32_t flags = (tree ? M3GPU_STEP_LEAVES : 0u) | (presolve ? M3GPU_STEP_PRESOLVE : 0u);
+        rc = m3gpu_step_begin(w->dev, &p, flags, replayVetoes ? w->replayVetoKeys : NULL,
+                              replayVetoes ? w->replayVetoCount : 0);
+        w->replayVetoCount = 0;
+        if (rc == 0 && presolve)
+        {
+            rc = PreSolvePass(w, &p);
+        }
         if (rc == 0)
         {
-            const double* leaves = m3gpu_step_leaves(w->dev);
-            if (leaves == NULL)
-            {
-                DeviceFail(w, "m3gpu_step_leaves");
-                return false;
-            }
-            TreeReplay(w, leaves);
             const int32_t* ranks = NULL;
-            if (treeCcd)
+            if (tree)
             {
-                // The reference's query stack holds 64 entries; deeper
-                // trees would drop leaves from its walks.
-                if (m3TreeLeafRanks(&w->tree, w->treeRanks, w->shapePool.maxIndex) > 64)
+                const double* leaves = m3gpu_step_leaves(w->dev);
+                if (leaves == NULL)
                 {
-                    fprintf(stderr, "maul3d_amd: broadphase tree deeper than the reference's "
-                                    "query stack; continuous candidates may differ\n");
+                    DeviceFail(w, "m3gpu_step_leaves");
+                    return false;
                 }
-                ranks = w->treeRanks;
+                TreeReplay(w, leaves);
+                if (treeCcd)
+                {
+                    // The reference's query stack holds 64 entries; deeper
+                    // trees would drop leaves from its walks.
+                    if (m3TreeLeafRanks(&w->tree, w->treeRanks, w->shapePool.maxIndex) > 64)
+                    {
+                        fprintf(stderr, "maul3d_amd: broadphase tree deeper than the reference's "
+                                        "query stack; continuous candidates may differ\n");
+                    }
+                    ranks = w->treeRanks;
+                }
             }
             rc = m3gpu_step_end(w->dev, &p, ranks);
+        }
+        if (rc == 0 && !tree)
+        {
+            w->treeStale = TreeHasLeaves(w) ? 1 : 0;
+            w->pairTableLive = 0;
+            w->pairTableOnDevice = 0;
         }
     }
     else
@@ -2480,9 +2558,15 @@ static bool StepWorld(m3World* w, float dt, int32_t substeps)
     w->hostStale = M3GPU_FIELDS_DYNAMIC | M3GPU_FIELDS_PAIRS;
     w->eventsState = 1;
     w->eventThreshold = w->hitEventThreshold;
-    // Recorded after the step, as the reference (src/solver.c:4332-4354).
+    // Recorded after the step, as the reference (src/solver.c:4332-4354):
+    // the step's vetoes first, so a bare replay repeats them.
     if (w->journalActive != 0)
     {
+        if (w->stepVetoCount > 0)
+        {
+            JournalRecord(w, M3_OP_STEP_VETOES, w->stepVetoKeys,
+                          w->stepVetoCount * (int32_t)sizeof(uint64_t));
+        }
         m3OpStep rec;
         memset(&rec, 0, sizeof(rec));
         rec.dt = dt;
@@ -2495,6 +2579,17 @@ static bool StepWorld(m3World* w, float dt, int32_t substeps)
 void m3World_Step(m3WorldId worldId, float dt, int32_t substeps)
 {
     (void)StepWorld(WorldFromId(worldId), dt, substeps);
+}
+
+void m3World_SetPreSolveCallback(m3WorldId worldId, m3PreSolveFn* fn, void* context)
+{
+    m3World* w = WorldFromId(worldId);
+    if (w == NULL)
+    {
+        return;
+    }
+    w->preSolveFn = fn;
+    w->preSolveContext = context;
 }
 
 bool m3World_EnableTreeMirror(m3WorldId worldId, bool flag)
@@ -3119,6 +3214,33 @@ void m3Shape_EnableHitEvents(m3ShapeId shapeId, bool flag)
     w->shapeHitEvents[s] = flag ? 1 : 0;
 }
 
+void m3Shape_EnablePreSolve(m3ShapeId shapeId, bool flag)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    if (w == NULL)
+    {
+        return;
+    }
+    if (w->journalActive != 0)
+    {
+        m3OpShapeFlag rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.id = shapeId;
+        rec.on = flag ? 1 : 0;
+        JournalRecord(w, M3_OP_ENABLE_SHAPE_PRE_SOLVE, &rec, (int32_t)sizeof(rec));
+    }
+    Touch(w, M3GPU_FIELDS_STATIC);
+    w->shapePreSolve[s] = flag ? 1 : 0;
+}
+
+bool m3Shape_IsPreSolveEnabled(m3ShapeId shapeId)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    return w != NULL && w->shapePreSolve[s] != 0;
+}
+
 bool m3Shape_AreHitEventsEnabled(m3ShapeId shapeId)
 {
     int32_t s;
@@ -3414,6 +3536,34 @@ static bool JournalApply(m3World* w, m3WorldId worldId, const uint8_t* stream, i
                 return false;
             }
             m3Shape_EnableHitEvents(rec.id, rec.on != 0);
+            break;
+        }
+        case M3_OP_ENABLE_SHAPE_PRE_SOLVE:
+        {
+            m3OpShapeFlag rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            rec.id.world0 = w->worldIndex0;
+            int32_t sIndex;
+            if (ResolveShape(rec.id, &sIndex) != w)
+            {
+                return false;
+            }
+            m3Shape_EnablePreSolve(rec.id, rec.on != 0);
+            break;
+        }
+        case M3_OP_STEP_VETOES:
+        {
+            // The next step's recorded vetoes (src/world.c:1613-1621).
+            if (bytes <= 0 || (bytes % 8) != 0 || bytes / 8 > w->pairCapacity)
+            {
+                return false;
+            }
+            memcpy(w->replayVetoKeys, payload, (size_t)bytes);
+            w->replayVetoCount = bytes / 8;
             break;
         }
         default:

@@ -251,6 +251,60 @@ def ccd_arena(api: Api, seed: int = 0xCCD5, movers: int = 60, bullets: int = 3) 
     return world
 
 
+def platforms(api: Api, seed: int = 0x0DE7A, movers: int = 80) -> tuple[WorldId, list[int]]:
+    """One-way platforms for the pre-solve veto (src/solver.c:214-248): a
+    plane, five static box platforms (opted in with enablePreSolveEvents) at
+    2-7 m, and ``movers`` spheres and boxes launched upward from under them at
+    8-14 m/s, which pass up through a platform when its contact is vetoed and
+    land on it on the way down.  Returns the world and the platforms' shape
+    slots (the callback's only state)."""
+    world = _world(api, movers + 8, movers + 8)
+    _ground(api, world)
+    rng = SplitMix64(seed)
+    sd = api.m3DefaultShapeDef()
+    sd.friction = 0.6
+    gd = api.m3DefaultBodyDef()
+    slots = []
+    for k in range(5):
+        gd.position = Pos3(-8.0 + 4.0 * k, 2.0 + 1.2 * k, 0.0)
+        plat = api.m3CreateBody(world, C.byref(gd))
+        sd.enablePreSolveEvents = True
+        sid = api.m3CreateBoxShape(plat, C.byref(sd), Vec3(1.8, 0.15, 3.0))
+        _check(sid, "platform")
+        slots.append(sid.index1 - 1)
+    sd.enablePreSolveEvents = False
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    for k in range(movers):
+        lane = k % 5
+        bd.position = Pos3(-8.0 + 4.0 * lane + rng.range(-1.5, 1.5), rng.range(0.4, 1.2), rng.range(-2.5, 2.5))
+        bd.linearVelocity = Vec3(rng.range(-0.5, 0.5), rng.range(8.0, 14.0), rng.range(-0.5, 0.5))
+        body = api.m3CreateBody(world, C.byref(bd))
+        if k % 2 == 0:
+            _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(Sphere(Vec3(0, 0, 0), rng.range(0.15, 0.3)))), "sphere")
+        else:
+            he = Vec3(rng.range(0.12, 0.25), rng.range(0.12, 0.25), rng.range(0.12, 0.25))
+            _check(api.m3CreateBoxShape(body, C.byref(sd), he), "box")
+    bd.linearVelocity = Vec3(0.0, 0.0, 0.0)
+    return world, slots
+
+
+def one_way_rule(platform_slots: list[int]):
+    """The pure pre-solve callback of ``platforms``: a contact with a
+    platform holds only when the other shape is above it (the normal, from A
+    to B, points away from the platform's top)."""
+    plats = set(platform_slots)
+
+    def rule(a: int, b: int, normal_y: float) -> bool:
+        if a in plats:
+            return normal_y > 0.5
+        if b in plats:
+            return normal_y < -0.5
+        return True
+
+    return rule
+
+
 def smoke10k(api: Api) -> WorldId:
     """The reference bench's 10k mixed rain (bench/bench_main.c:612-650).
 

@@ -177,6 +177,8 @@ struct m3gpuWorld
     int begun;        // m3gpu_step_begin ran; m3gpu_step_end finishes the step
     int beginStalled;
     int pairTable;    // keep the pair table in the reference's layout
+    std::vector<m3gpuPreSolve> preCands;
+    std::vector<uint8_t> vetoFlags;
     std::vector<uint64_t> tableKeys;
     std::vector<::m3Manifold> tableMans;
     tw::m3HitRec* hitRecBuf;
@@ -245,6 +247,7 @@ extern "C" m3gpuWorld* m3gpu_world_create(const m3gpuCaps* c)
     v.planes = A<int32_t>(ns);
     v.shapeRank = A<int32_t>(ns);
     v.ccdStatics = A<int32_t>(ns);
+    v.preSolve = A<uint8_t>(ns);
     v.hulls = A<tw::m3HullData>(c->hullCapacity);
     for (int k = 0; k < 2; ++k)
     {
@@ -331,6 +334,7 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t f)
             v.rolling[s] = (R)h->shapeRollingResistance[s];
             v.sensor[s] = h->shapeSensor[s];
             v.hitEvents[s] = h->shapeHitEvents[s];
+            v.preSolve[s] = h->shapePreSolve[s];
             anyHit = anyHit || (h->shapeAlive[s] != 0 && h->shapeHitEvents[s] != 0);
             v.hasOffset[s] = h->shapeHasOffset[s];
             v.localPos[s] = V(lp[s]);
@@ -505,7 +509,8 @@ static tw::m3SerialParams Params(const m3gpuStepParams* p)
     return s;
 }
 
-static int StepBegin(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
+// Through the island wake pass; nonzero on a stall (1) or a failure (-1).
+static int StepA(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
 {
     if (w->jointsPresent || w->targetsPresent)
     {
@@ -518,16 +523,50 @@ static int StepBegin(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
     v.manifolds = w->mansBuf[w->cur].data();
     v.oldKeys = w->keysBuf[w->cur ^ 1].data();
     v.oldManifolds = w->mansBuf[w->cur ^ 1].data();
+    v.veto = nullptr;
     tw::m3SerialScratch s;
     s.root = w->root.data();
     s.masks = w->masks.data();
     tw::m3SerialParams sp = Params(p);
     v.sc->stall = 0;
-    if (tw::m3SerialStepBegin(&v, &s, &sp, tape) != 0)
+    if (tw::m3SerialStepA(&v, &s, &sp, tape) != 0)
     {
         w->cur ^= 1;
         return 1;
     }
+    return 0;
+}
+
+// From the constraint list to the impulse store, `keys` (ascending) vetoed.
+static void StepB(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape, const uint64_t* keys, int32_t n)
+{
+    tw::m3View& v = w->v;
+    v.veto = nullptr;
+    if (n > 0)
+    {
+        w->vetoFlags.assign(v.pairCapacity, 0);
+        for (int32_t i = 0; i < v.sc->pairCount; ++i)
+        {
+            w->vetoFlags[i] = std::binary_search(keys, keys + n, v.keys[i]) ? 1 : 0;
+        }
+        v.veto = w->vetoFlags.data();
+    }
+    tw::m3SerialScratch s;
+    s.root = w->root.data();
+    s.masks = w->masks.data();
+    tw::m3SerialParams sp = Params(p);
+    tw::m3SerialStepB(&v, &s, &sp, tape);
+    v.veto = nullptr;
+}
+
+static int StepBegin(m3gpuWorld* w, const m3gpuStepParams* p, tw::m3Tape* tape)
+{
+    const int rc = StepA(w, p, tape);
+    if (rc != 0)
+    {
+        return rc;
+    }
+    StepB(w, p, tape, nullptr, 0);
     return 0;
 }
 
@@ -575,15 +614,63 @@ extern "C" int m3gpu_step_supported(const m3gpuWorld* w, const m3gpuStepParams* 
     return w != nullptr && p != nullptr && !m3gpu_step_needs_tree(w, p);
 }
 
-extern "C" int m3gpu_step_begin(m3gpuWorld* w, const m3gpuStepParams* p, int32_t)
+extern "C" int m3gpu_step_begin(m3gpuWorld* w, const m3gpuStepParams* p, uint32_t flags,
+                                const uint64_t* vetoKeys, int32_t vetoCount)
 {
-    const int rc = StepBegin(w, p, nullptr);
+    const int rc = StepA(w, p, nullptr);
     if (rc < 0)
     {
         return -1;
     }
     w->begun = 1;
     w->beginStalled = rc != 0;
+    w->preCands.clear();
+    if (w->beginStalled)
+    {
+        return 0;
+    }
+    if (flags & M3GPU_STEP_PRESOLVE)
+    {
+        tw::m3View& v = w->v;
+        for (int32_t i = 0; i < v.sc->pairCount; ++i)
+        {
+            double point[3];
+            tw::m3Vec3 normal;
+            if (tw::m3PreSolveCandidate(&v, i, point, &normal))
+            {
+                m3gpuPreSolve c;
+                memset(&c, 0, sizeof(c));
+                c.key = v.keys[i];
+                c.pair = i;
+                c.point[0] = point[0];
+                c.point[1] = point[1];
+                c.point[2] = point[2];
+                c.normal[0] = (float)normal.x;
+                c.normal[1] = (float)normal.y;
+                c.normal[2] = (float)normal.z;
+                w->preCands.push_back(c);
+            }
+        }
+        return 0;
+    }
+    StepB(w, p, nullptr, vetoKeys, vetoCount);
+    return 0;
+}
+
+extern "C" int m3gpu_step_presolve(m3gpuWorld* w, const m3gpuPreSolve** candidates, int32_t* count)
+{
+    *candidates = w->preCands.data();
+    *count = (int32_t)w->preCands.size();
+    return 0;
+}
+
+extern "C" int m3gpu_step_vetoes(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t* vetoKeys,
+                                 int32_t vetoCount)
+{
+    if (!w->beginStalled)
+    {
+        StepB(w, p, nullptr, vetoKeys, vetoCount);
+    }
     return 0;
 }
 
