@@ -312,6 +312,11 @@ M3_API void m3World_Sync(m3WorldId worldId);
 // returns 0, m3World_Snapshot returns -1 and m3World_Restore false.  Returns
 // the failure text, or null for a healthy world or a bad id.
 M3_API const char* m3World_GetDeviceError(m3WorldId worldId);
+// The world's device mirror, for callers that drive the m3gpu layer
+// (include/m3gpu.h) directly, e.g. m3gpu_events; null before the
+// world's first step or after a device fault.  Nothing is allocated.
+struct m3gpuWorld;
+M3_API struct m3gpuWorld* m3World_DeviceWorld(m3WorldId worldId);
 // Pre-solve veto (include/maul3d/world.h:315-340, shape.h:302-303): the
 // callback runs on the host in pair order for every touching pair with an
 // awake dynamic side where either shape opted in; returning false drops the

@@ -361,6 +361,7 @@ _SIGNATURES["m3World_GetDeviceError"] = (C.c_char_p, [WorldId])
 _SIGNATURES["m3World_EnableTreeMirror"] = (C.c_bool, [WorldId, C.c_bool])
 _SIGNATURES["m3World_IsTreeMirrorCurrent"] = (C.c_bool, [WorldId])
 _SIGNATURES["m3World_RebuildBroadphase"] = (None, [WorldId])
+_SIGNATURES["m3World_DeviceWorld"] = (C.c_void_p, [WorldId])
 
 
 def _id_tuple(sid) -> tuple[int, int]:
@@ -387,6 +388,7 @@ class Api:
             "m3World_GetDeviceError",
             "m3World_EnableTreeMirror",
             "m3World_IsTreeMirrorCurrent",
+            "m3World_DeviceWorld",
         ),
     ):
         self.lib = lib

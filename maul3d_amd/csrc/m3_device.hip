@@ -93,6 +93,10 @@ struct m3gpuWorld
     m3gpuPreSolve* preStage;
     uint64_t* vetoStage;
     hipEvent_t preGathered;
+    // A pinned stage is rewritten only after the copy that reads it has run:
+    // the host may enqueue the next step before the last one started.
+    hipEvent_t vetoCopied;
+    int vetoCopyPending;
     // the pair table in the reference's layout (m3gpu_set_pair_table)
     uint64_t* tableKeys;
     m3Manifold* tableMans;
@@ -100,6 +104,8 @@ struct m3gpuWorld
     int32_t* rankStage; // pinned staging of the tree ranks a step carries
     double* leafStage;  // pinned copy of the step's leaf boxes (m3gpu_step_leaves)
     hipEvent_t leavesCopied;
+    hipEvent_t ranksCopied;
+    int rankCopyPending;
     m3Dev dev;      // host copy of what kernels see (scratch pointers, view)
     m3Dev* devDev;  // device copies: [parity] = pair buffers for a step with cur = parity
     int devDirty;   // the device copies need refreshing
@@ -3032,6 +3038,7 @@ static int CreateImpl(m3gpuWorld* w)
     HIP_TRY(hipHostMalloc((void**)&w->leafStage, (size_t)(ns > 0 ? ns : 1) * 6 * sizeof(double),
                           hipHostMallocDefault));
     HIP_TRY(hipEventCreateWithFlags(&w->leavesCopied, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&w->ranksCopied, hipEventDisableTiming));
     int32_t nj = c.jointCapacity > 0 ? c.jointCapacity : 1;
     if (DevAlloc(&v.jointAlive, nj) || DevAlloc(&v.jointType, nj) || DevAlloc(&v.jointBodyA, nj) ||
         DevAlloc(&v.jointBodyB, nj) || DevAlloc(&v.jointLocalA, nj) ||
@@ -3190,11 +3197,16 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
     {
         (void)hipEventDestroy(w->leavesCopied);
     }
+    if (w->ranksCopied != nullptr)
+    {
+        (void)hipEventDestroy(w->ranksCopied);
+    }
     if (w->preStage != nullptr)
     {
         (void)hipHostFree(w->preStage);
         (void)hipHostFree(w->vetoStage);
         (void)hipEventDestroy(w->preGathered);
+        (void)hipEventDestroy(w->vetoCopied);
     }
     EventBufsFree(w);
     ProfFree(w);
@@ -3834,9 +3846,16 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
     }
     if (vetoes)
     {
+        if (w->vetoCopyPending)
+        {
+            HIP_TRY(hipEventSynchronize(w->vetoCopied));
+            w->vetoCopyPending = 0;
+        }
         memcpy(w->vetoStage, vetoKeys, (size_t)vetoCount * sizeof(uint64_t));
         HIP_TRY(hipMemcpyAsync(w->vetoKeys, w->vetoStage, (size_t)vetoCount * sizeof(uint64_t),
                                hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(w->vetoCopied, st));
+        w->vetoCopyPending = 1;
         KL(k_veto_mark, dim3(Blocks(np, T)), dim3(T), 0, st, D, w->vetoKeys, vetoCount, w->vetoFlags);
     }
     // Prepare and color.
@@ -3919,9 +3938,16 @@ static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t
     }
     if (tree)
     {
+        if (w->rankCopyPending)
+        {
+            HIP_TRY(hipEventSynchronize(w->ranksCopied));
+            w->rankCopyPending = 0;
+        }
         memcpy(w->rankStage, ranks, (size_t)ns * sizeof(int32_t));
         HIP_TRY(hipMemcpyAsync(v.shapeRank, w->rankStage, (size_t)ns * sizeof(int32_t),
                                hipMemcpyHostToDevice, st));
+        HIP_TRY(hipEventRecord(w->ranksCopied, st));
+        w->rankCopyPending = 1;
     }
     if (p->continuousEnabled)
     {
@@ -3992,6 +4018,9 @@ extern "C" int m3gpu_set_pair_table(m3gpuWorld* w, int32_t on)
         {
             return -1;
         }
+        // The clears run on the null stream; finish them before an upload
+        // or a step writes the table.
+        HIP_TRY(hipStreamSynchronize(nullptr));
     }
     w->pairTable = on ? 1 : 0;
     return 0;
@@ -4041,7 +4070,12 @@ static int PreSolveBuffers(m3gpuWorld* w)
     }
     HIP_TRY(hipHostMalloc((void**)&w->vetoStage, (size_t)np * sizeof(uint64_t), hipHostMallocDefault));
     HIP_TRY(hipEventCreateWithFlags(&w->preGathered, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&w->vetoCopied, hipEventDisableTiming));
     HIP_TRY(hipHostMalloc((void**)&w->preStage, (size_t)np * sizeof(m3gpuPreSolve), hipHostMallocDefault));
+    // DevAlloc clears on the null stream, which does not order against the
+    // world's non-blocking stream: finish the clears before the step's
+    // copies and kernels write these buffers.
+    HIP_TRY(hipStreamSynchronize(nullptr));
     return 0;
 }
 

@@ -2716,6 +2716,12 @@ int32_t m3World_DiffReport(m3WorldId worldA, m3WorldId worldB, m3BodyDiff* out, 
     return found;
 }
 
+struct m3gpuWorld* m3World_DeviceWorld(m3WorldId worldId)
+{
+    m3World* w = WorldFromId(worldId);
+    return (w != NULL && w->faulted == 0) ? w->dev : NULL;
+}
+
 const char* m3World_GetDeviceError(m3WorldId worldId)
 {
     m3World* w = WorldFromId(worldId);
