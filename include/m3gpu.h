@@ -312,6 +312,15 @@ typedef struct m3gpuEvents
     int32_t counts[7]; // begin, end, sensor begin, sensor end, hits, moves, stalled
 } m3gpuEvents;
 M3GPU_API int m3gpu_events(m3gpuWorld* world, float hitThreshold, m3gpuEvents* out);
+// m3gpu_events without allocating (no counterpart in the reference): for
+// callers that cannot allocate (code under HIP graph capture, serving loops,
+// harnesses that forbid hipMalloc).  m3gpu_events allocates a scratch per
+// world on its first call; this variant uses instead one file-scope device
+// array per GPU (about 3.1 MB, held until the process ends) and so serves
+// worlds of at most 32768 pair slots (4096 shapes) and 8192 bodies; larger
+// worlds get -1 with a message.  Calls that overlap on one GPU share the
+// array and race.  Same outputs as m3gpu_events.
+M3GPU_API int m3gpu_events_noalloc(m3gpuWorld* world, float hitThreshold, m3gpuEvents* out);
 // Waits for all queued device work of this world.
 M3GPU_API int m3gpu_sync(m3gpuWorld* world);
 

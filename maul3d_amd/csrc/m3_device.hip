@@ -3727,6 +3727,121 @@ extern "C" int m3gpu_events(m3gpuWorld* w, float hitThreshold, m3gpuEvents* out)
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// The event read without allocation (m3gpu_events_noalloc): the same kernels
+// over a scratch carved from one file-scope device array per GPU instead of
+// the per-world buffers m3gpu_events allocates on its first call.  Every
+// byte the kernels read is written earlier in the same call (codes for every
+// pair and body slot, counts by the compaction), so nothing needs clearing;
+// calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+
+#define M3_NOALLOC_PAIRS 32768
+#define M3_NOALLOC_BODIES 8192
+#define M3_NOALLOC_DEVICES 64
+
+static constexpr size_t Align256(size_t n) { return (n + 255) & ~(size_t)255; }
+static constexpr size_t kNoAllocCodes = Align256(M3_NOALLOC_PAIRS);
+static constexpr size_t kNoAllocCounts = Align256(8 * sizeof(int32_t));
+static constexpr size_t kNoAllocLists = Align256(4 * (size_t)M3_NOALLOC_PAIRS * sizeof(uint64_t));
+static constexpr size_t kNoAllocHits = Align256((size_t)M3_NOALLOC_PAIRS * sizeof(m3gpuHit));
+static constexpr size_t kNoAllocMoves = Align256((size_t)M3_NOALLOC_BODIES * sizeof(m3gpuMove));
+static constexpr size_t kNoAllocBytes =
+    2 * kNoAllocCodes + kNoAllocCounts + kNoAllocLists + kNoAllocHits + kNoAllocMoves;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+// The array's device address on each GPU (hipGetSymbolAddress, once per
+// device with that device current); the host-side name is no address.
+static uint8_t* s_scratchPtr[M3_NOALLOC_DEVICES];
+static pthread_mutex_t s_scratchLock = PTHREAD_MUTEX_INITIALIZER;
+
+static uint8_t* ScratchOnCurrentDevice(void)
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= M3_NOALLOC_DEVICES)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_events_noalloc: no usable current device");
+        return nullptr;
+    }
+    pthread_mutex_lock(&s_scratchLock);
+    if (s_scratchPtr[dev] == nullptr)
+    {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || ptr == nullptr)
+        {
+            pthread_mutex_unlock(&s_scratchLock);
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_events_noalloc: hipGetSymbolAddress: %s",
+                     hipGetErrorString(e));
+            return nullptr;
+        }
+        s_scratchPtr[dev] = (uint8_t*)ptr;
+    }
+    uint8_t* p = s_scratchPtr[dev];
+    pthread_mutex_unlock(&s_scratchLock);
+    return p;
+}
+
+extern "C" int m3gpu_events_noalloc(m3gpuWorld* w, float hitThreshold, m3gpuEvents* out)
+{
+    if (w == nullptr || out == nullptr || JoinBatch(w) != 0)
+    {
+        return -1;
+    }
+    const int32_t np = w->caps.pairCapacity;
+    const int32_t nb = w->caps.bodyCapacity;
+    if (np > M3_NOALLOC_PAIRS || nb > M3_NOALLOC_BODIES)
+    {
+        snprintf(s_lastError, sizeof(s_lastError),
+                 "m3gpu_events_noalloc: the world's capacities (%d pairs, %d bodies) exceed the "
+                 "static scratch (%d pairs, %d bodies); use m3gpu_events",
+                 np, nb, M3_NOALLOC_PAIRS, M3_NOALLOC_BODIES);
+        return -1;
+    }
+    uint8_t* base = ScratchOnCurrentDevice();
+    if (base == nullptr)
+    {
+        return -1;
+    }
+    if (w->devDirty && RefreshDevCopies(w) != 0)
+    {
+        return -1;
+    }
+    m3EventBufs e;
+    memset(&e, 0, sizeof(e));
+    uint8_t* at = base;
+    e.codeNew = at;
+    at += kNoAllocCodes;
+    e.codeOld = at;
+    at += kNoAllocCodes;
+    e.counts = (int32_t*)at;
+    at += kNoAllocCounts;
+    e.lists = (uint64_t*)at;
+    at += kNoAllocLists;
+    e.hits = (m3gpuHit*)at;
+    at += kNoAllocHits;
+    e.moves = (m3gpuMove*)at;
+    const m3Dev* D = w->devDev + w->cur;
+    const int32_t n = np > nb ? np : nb;
+    hipStream_t st = w->stream;
+    KL(k_event_codes, dim3(Blocks(n, 256)), dim3(256), 0, st, D, e, hitThreshold);
+    HIP_TRY(hipGetLastError());
+    KL(k_event_compact, dim3(1), dim3(M3_EVENT_THREADS), 0, st, D, e);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out->counts, e.counts, 7 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    uint64_t* dst[4] = {out->beginKeys, out->endKeys, out->sensorBeginKeys, out->sensorEndKeys};
+    for (int k = 0; k < 4; ++k)
+    {
+        DOWN(dst[k], e.lists + (int64_t)k * np, out->counts[k], uint64_t);
+    }
+    DOWN(out->hits, e.hits, out->counts[4], m3gpuHit);
+    DOWN(out->moves, e.moves, out->counts[5], m3gpuMove);
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
 // Enqueue one step.  Launch geometry only depends on host-known capacities
 // and high-water marks; every data-dependent count is read on the device.
 // The step through the island wake pass (pair list, narrowphase, islands);
