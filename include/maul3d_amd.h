@@ -298,6 +298,22 @@ M3_API bool m3World_JournalBegin(m3WorldId worldId, void* buffer, int32_t capaci
 M3_API int32_t m3World_JournalEnd(m3WorldId worldId);                                      // world.h:601
 M3_API bool m3World_JournalReplay(m3WorldId worldId, const void* data, int32_t size);      // world.h:606
 M3_API bool m3Shape_AreHitEventsEnabled(m3ShapeId shapeId);                                 // shape.h:301
+// Shape materials (include/maul3d/shape.h:286-296): non-finite or negative
+// values (density: not positive) are ignored; journaled as ops 40-43.
+M3_API void m3Shape_SetFriction(m3ShapeId shapeId, float friction);                         // shape.h:286
+M3_API float m3Shape_GetFriction(m3ShapeId shapeId);                                        // shape.h:287
+M3_API void m3Shape_SetRestitution(m3ShapeId shapeId, float restitution);                   // shape.h:288
+M3_API float m3Shape_GetRestitution(m3ShapeId shapeId);                                     // shape.h:289
+M3_API void m3Shape_SetRollingResistance(m3ShapeId shapeId, float value);                   // shape.h:290
+M3_API float m3Shape_GetRollingResistance(m3ShapeId shapeId);                               // shape.h:291
+M3_API void m3Shape_SetDensity(m3ShapeId shapeId, float density, bool updateBodyMass);      // shape.h:295
+M3_API float m3Shape_GetDensity(m3ShapeId shapeId);                                         // shape.h:296
+M3_API bool m3Shape_IsValid(m3ShapeId shapeId);                                             // shape.h:267
+M3_API m3BodyId m3Shape_GetBody(m3ShapeId shapeId);                                         // shape.h:271
+// A dynamic body's mass (1 / its inverse mass), 0 for other bodies and bad
+// ids.  No counterpart in the reference (which exposes only densities); the
+// gradient harness reads it to write a mass through the shapes' densities.
+M3_API float m3Body_GetMass(m3BodyId bodyId);
 
 // Extensions of this build (no reference counterpart): step many worlds with
 // one call (the batched rollback and toss workloads; returns the number of

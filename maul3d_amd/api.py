@@ -362,6 +362,17 @@ _SIGNATURES["m3World_EnableTreeMirror"] = (C.c_bool, [WorldId, C.c_bool])
 _SIGNATURES["m3World_IsTreeMirrorCurrent"] = (C.c_bool, [WorldId])
 _SIGNATURES["m3World_RebuildBroadphase"] = (None, [WorldId])
 _SIGNATURES["m3World_DeviceWorld"] = (C.c_void_p, [WorldId])
+_SIGNATURES["m3Shape_SetFriction"] = (None, [ShapeId, c_f])
+_SIGNATURES["m3Shape_GetFriction"] = (c_f, [ShapeId])
+_SIGNATURES["m3Shape_SetRestitution"] = (None, [ShapeId, c_f])
+_SIGNATURES["m3Shape_GetRestitution"] = (c_f, [ShapeId])
+_SIGNATURES["m3Shape_SetRollingResistance"] = (None, [ShapeId, c_f])
+_SIGNATURES["m3Shape_GetRollingResistance"] = (c_f, [ShapeId])
+_SIGNATURES["m3Shape_SetDensity"] = (None, [ShapeId, c_f, C.c_bool])
+_SIGNATURES["m3Shape_GetDensity"] = (c_f, [ShapeId])
+_SIGNATURES["m3Shape_IsValid"] = (C.c_bool, [ShapeId])
+_SIGNATURES["m3Shape_GetBody"] = (BodyId, [ShapeId])
+_SIGNATURES["m3Body_GetMass"] = (c_f, [BodyId])
 
 
 def _id_tuple(sid) -> tuple[int, int]:
@@ -389,6 +400,7 @@ class Api:
             "m3World_EnableTreeMirror",
             "m3World_IsTreeMirrorCurrent",
             "m3World_DeviceWorld",
+            "m3Body_GetMass",
         ),
     ):
         self.lib = lib

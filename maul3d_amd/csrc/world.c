@@ -599,6 +599,10 @@ enum
     M3_OP_APPLY_FORCE = 26,
     M3_OP_APPLY_TORQUE = 27,
     M3_OP_SET_GRAVITY = 39,
+    M3_OP_SET_SHAPE_FRICTION = 40,
+    M3_OP_SET_SHAPE_RESTITUTION = 41,
+    M3_OP_SET_SHAPE_ROLLING = 42,
+    M3_OP_SET_SHAPE_DENSITY = 43,
     M3_OP_ENABLE_SLEEPING = 47,
     M3_OP_ENABLE_CONTINUOUS = 48,
     M3_OP_SET_HIT_EVENT_THRESHOLD = 49,
@@ -632,6 +636,8 @@ typedef struct m3OpCreateHullShape // src/world_internal.h:574-581
 } m3OpCreateHullShape;
 typedef struct m3OpCreateJoint { m3JointDef def; m3JointId expected; } m3OpCreateJoint;
 typedef struct m3OpShapeFlag { m3ShapeId id; int32_t on; } m3OpShapeFlag;
+typedef struct m3OpShapeScalar { m3ShapeId id; float value; } m3OpShapeScalar; // src/shape.c:1439-1443
+typedef struct m3OpShapeDensity { m3ShapeId id; float value; int32_t updateMass; } m3OpShapeDensity;
 
 static void JournalRecord(m3World* w, int32_t op, const void* payload, int32_t bytes)
 {
@@ -3240,6 +3246,128 @@ void m3Shape_EnablePreSolve(m3ShapeId shapeId, bool flag)
     w->shapePreSolve[s] = flag ? 1 : 0;
 }
 
+// Shape materials (src/shape.c:1388-1541).  Friction, restitution and
+// rolling resistance are read by the next step's prepare; a density change
+// rebuilds the body's mass properties when asked.  Each is journaled after
+// validation, as the reference records it.
+static float* ShapeScalarField(m3World* w, int32_t op)
+{
+    return op == M3_OP_SET_SHAPE_FRICTION      ? w->shapeFriction
+           : op == M3_OP_SET_SHAPE_RESTITUTION ? w->shapeRestitution
+                                               : w->shapeRollingResistance;
+}
+
+static void SetShapeScalar(m3ShapeId shapeId, int32_t op, float value)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    if (w == NULL || !FiniteF(value) || value < 0.0f)
+    {
+        return;
+    }
+    if (w->journalActive != 0)
+    {
+        m3OpShapeScalar rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.id = shapeId;
+        rec.value = value;
+        JournalRecord(w, op, &rec, (int32_t)sizeof(rec));
+    }
+    Touch(w, M3GPU_FIELDS_STATIC);
+    ShapeScalarField(w, op)[s] = value;
+}
+
+static float GetShapeScalar(m3ShapeId shapeId, int32_t op)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    return w != NULL ? ShapeScalarField(w, op)[s] : 0.0f;
+}
+
+void m3Shape_SetFriction(m3ShapeId shapeId, float friction)
+{
+    SetShapeScalar(shapeId, M3_OP_SET_SHAPE_FRICTION, friction);
+}
+
+float m3Shape_GetFriction(m3ShapeId shapeId) { return GetShapeScalar(shapeId, M3_OP_SET_SHAPE_FRICTION); }
+
+void m3Shape_SetRestitution(m3ShapeId shapeId, float restitution)
+{
+    SetShapeScalar(shapeId, M3_OP_SET_SHAPE_RESTITUTION, restitution);
+}
+
+float m3Shape_GetRestitution(m3ShapeId shapeId) { return GetShapeScalar(shapeId, M3_OP_SET_SHAPE_RESTITUTION); }
+
+void m3Shape_SetRollingResistance(m3ShapeId shapeId, float value)
+{
+    SetShapeScalar(shapeId, M3_OP_SET_SHAPE_ROLLING, value);
+}
+
+float m3Shape_GetRollingResistance(m3ShapeId shapeId) { return GetShapeScalar(shapeId, M3_OP_SET_SHAPE_ROLLING); }
+
+void m3Shape_SetDensity(m3ShapeId shapeId, float density, bool updateBodyMass)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    if (w == NULL || !FiniteF(density) || !(density > 0.0f))
+    {
+        return;
+    }
+    if (w->journalActive != 0)
+    {
+        m3OpShapeDensity rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.id = shapeId;
+        rec.value = density;
+        rec.updateMass = updateBodyMass ? 1 : 0;
+        JournalRecord(w, M3_OP_SET_SHAPE_DENSITY, &rec, (int32_t)sizeof(rec));
+    }
+    Touch(w, M3GPU_FIELDS_STATIC);
+    w->shapeDensity[s] = density;
+    if (updateBodyMass)
+    {
+        RecomputeMass(w, w->shapeBody[s]);
+    }
+}
+
+float m3Shape_GetDensity(m3ShapeId shapeId)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    return w != NULL ? w->shapeDensity[s] : 0.0f;
+}
+
+bool m3Shape_IsValid(m3ShapeId shapeId)
+{
+    int32_t s;
+    return ResolveShape(shapeId, &s) != NULL;
+}
+
+m3BodyId m3Shape_GetBody(m3ShapeId shapeId)
+{
+    int32_t s;
+    m3World* w = ResolveShape(shapeId, &s);
+    m3BodyId none = {0, 0, 0};
+    if (w == NULL)
+    {
+        return none;
+    }
+    const int32_t b = w->shapeBody[s];
+    m3BodyId id = {b + 1, w->worldIndex0, w->bodyPool.generations[b]};
+    return id;
+}
+
+float m3Body_GetMass(m3BodyId bodyId)
+{
+    int32_t i;
+    m3World* w = ResolveBody(bodyId, &i);
+    if (w == NULL || w->types[i] != M3_DYNAMIC || !(w->invMass[i] > 0.0f))
+    {
+        return 0.0f;
+    }
+    return 1.0f / w->invMass[i];
+}
+
 bool m3Shape_IsPreSolveEnabled(m3ShapeId shapeId)
 {
     int32_t s;
@@ -3542,6 +3670,42 @@ static bool JournalApply(m3World* w, m3WorldId worldId, const uint8_t* stream, i
                 return false;
             }
             m3Shape_EnableHitEvents(rec.id, rec.on != 0);
+            break;
+        }
+        case M3_OP_SET_SHAPE_FRICTION:
+        case M3_OP_SET_SHAPE_RESTITUTION:
+        case M3_OP_SET_SHAPE_ROLLING:
+        {
+            m3OpShapeScalar rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            rec.id.world0 = w->worldIndex0;
+            int32_t sIndex;
+            if (ResolveShape(rec.id, &sIndex) != w || !FiniteF(rec.value) || rec.value < 0.0f)
+            {
+                return false;
+            }
+            SetShapeScalar(rec.id, op, rec.value);
+            break;
+        }
+        case M3_OP_SET_SHAPE_DENSITY:
+        {
+            m3OpShapeDensity rec;
+            if (bytes != (int32_t)sizeof(rec))
+            {
+                return false;
+            }
+            memcpy(&rec, payload, sizeof(rec));
+            rec.id.world0 = w->worldIndex0;
+            int32_t sIndex;
+            if (ResolveShape(rec.id, &sIndex) != w || !FiniteF(rec.value) || !(rec.value > 0.0f))
+            {
+                return false;
+            }
+            m3Shape_SetDensity(rec.id, rec.value, rec.updateMass != 0);
             break;
         }
         case M3_OP_ENABLE_SHAPE_PRE_SOLVE:

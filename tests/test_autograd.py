@@ -1,6 +1,10 @@
 """The torch.autograd wrapper of the adjoint (maul3d_amd.autograd), on the
-FP64 CPU twin: backward hands out exactly what m3World_Adjoint returns, and a
-descent step on the initial velocities lowers the loss."""
+FP64 CPU twin: the parameters it is given are the ones the world runs with
+(masses through the shapes' densities, frictions on every shape), backward
+hands out exactly what m3World_Adjoint returns for them, central
+differences of the wrapper's own loss in a written mass or friction match
+the gradient on contact-stable samples, and a descent step lowers the
+loss."""
 
 from __future__ import annotations
 
@@ -17,6 +21,8 @@ torch = pytest.importorskip("torch")
 
 TWIN64 = os.path.join(ROOT, "oracle", "_ref", "libm3twin64.so")
 NB = 33
+STEPS = 30
+TARGET = (0.5, 0.3, -1.0)
 
 
 def _twin():
@@ -36,30 +42,77 @@ def _state(T, w):
     return torch.tensor(lin), torch.tensor(ang)
 
 
+def _loss(T, w, lin, ang, mass, fric):
+    from maul3d_amd.autograd import trajectory_loss
+
+    return float(trajectory_loss(T, w, TARGET, lin, ang, torch.zeros(NB, 3), mass, fric, steps=STEPS))
+
+
 def test_backward_returns_adjoint_gradients_and_descends():
-    from maul3d_amd.autograd import _adjoint, trajectory_loss
+    from maul3d_amd.autograd import _adjoint, trajectory_loss, world_parameters
 
     T = _twin()
     w = scenes.toss(T, 2)
     lin, ang = _state(T, w)
+    mass, fric = world_parameters(T, w, NB)
+    assert mass[0] == 0.0 and all(m > 0.0 for m in mass[1:].tolist())  # slot 0 is the ground
     force = torch.zeros(NB, 3)
-    mass = torch.ones(NB)
-    fric = torch.ones(NB)
-    target = (0.5, 0.3, -1.0)
-    steps = 30
     for t in (lin, ang, force, mass, fric):
         t.requires_grad_(True)
-    loss = trajectory_loss(T, w, target, lin, ang, force, mass, fric, steps=steps, checkpoint_every=10)
+    loss = trajectory_loss(T, w, TARGET, lin, ang, force, mass, fric, steps=STEPS, checkpoint_every=10)
     loss.backward()
-    ref = _adjoint(T, w, NB, target, [[0.0, 0.0, 0.0]] * NB, steps, 10, 1.0 / 60.0, 4)
+    ref = _adjoint(T, w, NB, TARGET, [[0.0, 0.0, 0.0]] * NB, STEPS, 10, 1.0 / 60.0, 4)
     assert float(loss) == ref[0]
     assert torch.equal(lin.grad, torch.tensor(ref[1]))
     assert torch.equal(ang.grad, torch.tensor(ref[2]))
     assert torch.equal(mass.grad, torch.tensor(ref[4]))
+    assert torch.equal(fric.grad, torch.tensor(ref[5]))
     # one small descent step on the initial linear velocities lowers the loss
     with torch.no_grad():
         lin2 = lin - 0.05 * lin.grad
-    loss2 = trajectory_loss(T, w, target, lin2, ang.detach(), force.detach(), mass.detach(),
-                            fric.detach(), steps=steps)
+    loss2 = trajectory_loss(T, w, TARGET, lin2, ang.detach(), force.detach(), mass.detach(),
+                            fric.detach(), steps=STEPS)
     assert float(loss2) < float(loss)
+    T.m3DestroyWorld(w)
+
+
+def test_written_mass_and_friction_drive_the_trajectory():
+    """The wrapper writes each mass and friction into the world: they read
+    back, and central differences of its loss in them (abscissae read back
+    from the world, so the float densities' rounding does not enter) match
+    its gradients to 1e-3 on contact-stable samples."""
+    from maul3d_amd.autograd import trajectory_loss, world_parameters
+
+    T = _twin()
+    w = scenes.toss(T, 4)
+    lin, ang = _state(T, w)
+    mass0, fric0 = world_parameters(T, w, NB)
+    m = mass0.clone().requires_grad_(True)
+    f = fric0.clone().requires_grad_(True)
+    loss = trajectory_loss(T, w, TARGET, lin, ang, torch.zeros(NB, 3), m, f, steps=STEPS)
+    loss.backward()
+    stable = 0
+    for body in (3, 7, 12, 18, 25, 30):
+        for kind in ("mass", "friction"):
+            base = (mass0 if kind == "mass" else fric0).clone()
+            h = 1e-3 * float(base[body])
+            vals = {}
+            for e in (h, -h, 4 * h, -4 * h):
+                p = base.clone()
+                p[body] += e
+                mm, ff = (p, fric0) if kind == "mass" else (mass0, p)
+                L = _loss(T, w, lin, ang, mm, ff)
+                now, nowf = world_parameters(T, w, NB)
+                x = float(now[body] if kind == "mass" else nowf[body])
+                assert x == pytest.approx(float(p[body]), rel=1e-6)  # written into the world
+                vals[e] = (x, L)
+            fd = (vals[h][1] - vals[-h][1]) / (vals[h][0] - vals[-h][0])
+            fd4 = (vals[4 * h][1] - vals[-4 * h][1]) / (vals[4 * h][0] - vals[-4 * h][0])
+            scale = max(abs(fd), 1e-6)
+            if abs(fd - fd4) / scale > 1e-3:
+                continue
+            stable += 1
+            g = float((m.grad if kind == "mass" else f.grad)[body])
+            assert abs(g - fd) / scale < 1e-3, (kind, body, g, fd)
+    assert stable >= 6
     T.m3DestroyWorld(w)

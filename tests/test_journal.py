@@ -37,10 +37,13 @@ def _session(m3, world, joints: bool) -> None:
         bd.position = api.Pos3(-6.0 + 3.0 * k, 14.0, 2.0)
         b = m3.m3CreateBody(world, C.byref(bd))
         bodies.append(b)
-    assert m3.m3CreateSphereShape(bodies[0], C.byref(sd), C.byref(scenes.Sphere(api.Vec3(0, 0, 0), 0.5))).index1
-    assert m3.m3CreateBoxShape(bodies[1], C.byref(sd), api.Vec3(0.4, 0.3, 0.6)).index1
+    sphere = m3.m3CreateSphereShape(bodies[0], C.byref(sd), C.byref(scenes.Sphere(api.Vec3(0, 0, 0), 0.5)))
+    assert sphere.index1
+    box = m3.m3CreateBoxShape(bodies[1], C.byref(sd), api.Vec3(0.4, 0.3, 0.6))
+    assert box.index1
     cap = scenes.Capsule(api.Vec3(-0.5, 0, 0), api.Vec3(0.5, 0, 0), 0.3)
-    assert m3.m3CreateCapsuleShape(bodies[2], C.byref(sd), C.byref(cap)).index1
+    capsule = m3.m3CreateCapsuleShape(bodies[2], C.byref(sd), C.byref(cap))
+    assert capsule.index1
     rng = scenes.SplitMix64(0x10C4)
     pts = (api.Vec3 * 24)(*[api.Vec3(rng.range(-0.6, 0.6), rng.range(-0.6, 0.6), rng.range(-0.6, 0.6)) for _ in range(24)])
     hull = m3.m3CreateHullShape(bodies[3], C.byref(sd), pts, 24)
@@ -67,6 +70,13 @@ def _session(m3, world, joints: bool) -> None:
         if step == 20:
             m3.m3World_EnableSleeping(world, False)
             m3.m3World_EnableContinuous(world, True)
+        if step == 25:
+            m3.m3Shape_SetFriction(hull, 0.9)
+            m3.m3Shape_SetRestitution(box, 0.5)
+            m3.m3Shape_SetRollingResistance(sphere, 0.2)
+            m3.m3Shape_SetDensity(box, 1500.0, True)
+            m3.m3Shape_SetDensity(capsule, 200.0, False)
+            m3.m3Shape_SetFriction(hull, -1.0)  # ignored, not recorded
         if step == 30 and joint is not None:
             m3.m3DestroyJoint(joint)
 
