@@ -1868,12 +1868,25 @@ M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn*
     const m3Vec3 dpB = v->deltaPos[b];
     m3real passNormal = 0.0f;
     m3real twistLimit = 0.0f;
-    // Every slot is evaluated and the results of slots past pointCount are
-    // discarded, so no load of a later point waits behind a branch on the
-    // point count.
+    // Every slot up to the largest point count among the lanes running this
+    // solve together is evaluated, and the results of slots past a lane's
+    // own pointCount are discarded, so no load waits behind a branch on the
+    // point count (the loads were issued by m3LoadContact) and the bound is
+    // uniform across the wavefront.  Slots past every lane's count would be
+    // discarded anyway, so skipping them changes no result.
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int32_t slots = __ballot(pointCount > 2) != 0 ? (__ballot(pointCount > 3) != 0 ? 4 : 3)
+                                                        : (__ballot(pointCount > 1) != 0 ? 2 : 1);
+#else
+    const int32_t slots = M3_MANIFOLD_MAX_POINTS;
+#endif
     M3_UNROLL
     for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
+        if (k >= slots)
+        {
+            break;
+        }
         const int live = k < pointCount;
         const m3Vec3 rA = prA[k];
         const m3Vec3 rB = prB[k];
