@@ -58,6 +58,7 @@ struct m3Dev
     uint32_t* emitUnique;
     uint8_t* pairFlags;
     uint8_t* bodyFlags;
+    int32_t* blockCounts; // set flags per 256-thread workgroup of the last flag pass
     int32_t* root;
     uint16_t* bodyMasks;
     int32_t* bodyOwner;
@@ -84,6 +85,7 @@ struct m3gpuWorld
     int fullQuery; // rebuild the frozen-pair buffer before the next step
     int ccdUnsupported;
     int ccdNeedsTree; // continuous sweeps meet tree shapes (static non-plane targets or bullets)
+    int anyRestitution; // some live shape has restitution (else the restitution pass is not launched)
     // pre-solve: candidate records and their count, the vetoed keys and
     // per-pair flags of the step, pinned host staging
     m3gpuPreSolve* preCands;
@@ -254,9 +256,9 @@ __device__ static inline void EmitWave(const m3View& v, bool valid, uint32_t key
 // Broadphase (src/broad_phase.c:391-637)
 // ---------------------------------------------------------------------------
 
-__global__ void k_begin(const m3Dev* __restrict__ ds)
+__device__ static inline void StepBegin(const m3Dev& d)
 {
-    M3_WORLD;
+    const m3View& v = d.v;
     m3Scalars* sc = v.sc;
     sc->oldPairCount = sc->pairCount;
     sc->emitCount = 0;
@@ -277,14 +279,97 @@ __global__ void k_begin(const m3Dev* __restrict__ ds)
     }
 }
 
-__global__ void k_fill_emit(const m3Dev* __restrict__ ds)
+// The step's opening: thread 0 resets the step's scalars, every thread
+// fills its slot of the emission buffer with the sentinel.
+__global__ void k_open(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
+    {
+        StepBegin(d);
+    }
     if (i < v.emitCapacity)
     {
         v.emit[i] = M3_SENTINEL32;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Ordered compaction.  A flag pass leaves the number of set flags of each
+// 256-thread workgroup in blockCounts (FlagCount); the matching write pass,
+// launched with the same grid, gives each workgroup its output offset by
+// summing the counts of the workgroups before it and writes its flagged
+// items in index order (FlagRank), the last workgroup the total.
+// ---------------------------------------------------------------------------
+
+#define M3_COMPACT_THREADS 256
+
+// Every thread of the workgroup calls it.
+__device__ static inline void FlagCount(int flag, int32_t* blockCounts)
+{
+    __shared__ int32_t waveCount[M3_COMPACT_THREADS / 64];
+    const uint64_t mask = __ballot(flag);
+    const int32_t lane = threadIdx.x & 63;
+    const int32_t wave = threadIdx.x >> 6;
+    if (lane == 0)
+    {
+        waveCount[wave] = __popcll(mask);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int32_t n = 0;
+        for (int32_t k = 0; k < M3_COMPACT_THREADS / 64; ++k)
+        {
+            n += waveCount[k];
+        }
+        blockCounts[blockIdx.x] = n;
+    }
+}
+
+// This thread's output position if its flag is set (every thread calls it);
+// *total receives the count over all workgroups.
+__device__ static inline int32_t FlagRank(int flag, const int32_t* blockCounts, int32_t* total)
+{
+    __shared__ int32_t partial[M3_COMPACT_THREADS];
+    __shared__ int32_t waveCount[M3_COMPACT_THREADS / 64];
+    const int32_t t = threadIdx.x;
+    int32_t sum = 0;
+    for (int32_t k = t; k < (int32_t)blockIdx.x; k += M3_COMPACT_THREADS)
+    {
+        sum += blockCounts[k];
+    }
+    partial[t] = sum;
+    const uint64_t mask = __ballot(flag);
+    const int32_t lane = t & 63;
+    const int32_t wave = t >> 6;
+    if (lane == 0)
+    {
+        waveCount[wave] = __popcll(mask);
+    }
+    __syncthreads();
+    for (int32_t half = M3_COMPACT_THREADS / 2; half > 0; half >>= 1)
+    {
+        if (t < half)
+        {
+            partial[t] += partial[t + half];
+        }
+        __syncthreads();
+    }
+    int32_t pos = partial[0];
+    for (int32_t k = 0; k < wave; ++k)
+    {
+        pos += waveCount[k];
+    }
+    pos += __popcll(mask & ((1ull << lane) - 1ull));
+    int32_t mine = 0;
+    for (int32_t k = 0; k < M3_COMPACT_THREADS / 64; ++k)
+    {
+        mine += waveCount[k];
+    }
+    *total = partial[0] + mine;
+    return pos;
 }
 
 // Fresh bounds of every alive non-plane shape and its sweep key.
@@ -662,8 +747,12 @@ __global__ void k_frozen(const m3Dev* __restrict__ ds)
 }
 
 // The unique sorted keys become this step's pair list; the cold ones are
-// the frozen buffer carried to the next step.
-__global__ void k_pair_count(const m3Dev* __restrict__ ds)
+// the frozen buffer carried to the next step.  On a capacity stall the
+// previous pair list stays the world's list.  Every thread derives the
+// count and the stall from scalars that no thread of this pass changes
+// (thread 0 only publishes them), then expands its key or restores its old
+// pair, flags the cold keys and counts them per workgroup.
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_pairs_settle(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
     const uint32_t* unique = d.emitUnique;
@@ -673,51 +762,54 @@ __global__ void k_pair_count(const m3Dev* __restrict__ ds)
     {
         n -= 1;
     }
-    if (sc->emitCount > v.emitCapacity || n > v.pairCapacity)
+    const int stall = sc->stall != 0 || sc->emitCount > v.emitCapacity || n > v.pairCapacity;
+    const int32_t oldCount = sc->oldPairCount;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
     {
-        sc->stall = 1;
-    }
-    if (sc->stall)
-    {
-        sc->pairCount = sc->oldPairCount;
-        return;
-    }
-    sc->pairCount = n;
-}
-
-__global__ void k_expand(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    const uint32_t* unique = d.emitUnique;
-    uint8_t* coldFlags = d.pairFlags;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.pairCapacity)
-    {
-        return;
+        if (stall)
+        {
+            sc->stall = 1;
+        }
+        sc->pairCount = stall ? oldCount : n;
     }
     uint8_t cold = 0;
-    if (v.sc->stall == 0 && i < v.sc->pairCount)
+    if (i < v.pairCapacity)
     {
-        uint64_t key = m3ExpandKey(unique[i]);
-        v.keys[i] = key;
-        cold = m3PairHot(&v, (int32_t)(key >> 32), (int32_t)(key & 0xFFFFFFFFu)) ? 0 : 1;
+        if (stall)
+        {
+            if (i < oldCount)
+            {
+                v.keys[i] = v.oldKeys[i];
+                v.manifolds[i] = v.oldManifolds[i];
+            }
+        }
+        else if (i < n)
+        {
+            const uint64_t key = m3ExpandKey(unique[i]);
+            v.keys[i] = key;
+            cold = m3PairHot(&v, (int32_t)(key >> 32), (int32_t)(key & 0xFFFFFFFFu)) ? 0 : 1;
+        }
+        d.pairFlags[i] = cold;
     }
-    coldFlags[i] = cold;
+    FlagCount(cold, d.blockCounts);
 }
 
-// On a capacity stall the previous pair list stays the world's list.
-__global__ void k_stall_restore(const m3Dev* __restrict__ ds)
+// The cold keys of the pair list, in order, become the frozen buffer.
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_frozen(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    if (v.sc->stall == 0)
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int flag = i < v.pairCapacity && d.pairFlags[i] != 0;
+    int32_t total;
+    const int32_t pos = FlagRank(flag, d.blockCounts, &total);
+    if (flag)
     {
-        return;
+        v.sleepKeys[pos] = v.keys[i];
     }
-    int32_t n = v.sc->oldPairCount;
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
     {
-        v.keys[i] = v.oldKeys[i];
-        v.manifolds[i] = v.oldManifolds[i];
+        v.sc->sleepCount = total;
     }
 }
 
@@ -1139,11 +1231,36 @@ __device__ static inline int NarrowClass(const m3View& v, int32_t i)
     return planePair ? 5 : 6;
 }
 
+// Step-start body state and union-find reset of body i.
+__device__ static inline void CaptureBody(const m3View& v, int32_t i)
+{
+    if (v.bodyAlive[i] != 0)
+    {
+        m3Transform xf = v.xf[i];
+        m3Vec3 rlc = m3RotateVec3(xf.q, v.lc[i]);
+        m3Pos3 c;
+        c.x = xf.p.x + (double)rlc.x;
+        c.y = xf.p.y + (double)rlc.y;
+        c.z = xf.p.z + (double)rlc.z;
+        v.com0[i] = c;
+        v.rot0[i] = xf.q;
+    }
+    v.parent[i] = i;
+    v.forced[i] = 0;
+    v.flag[i] = 0;
+    v.pending[i] = 0; // the sweep has recorded last step's freezes
+    v.deltaPos[i] = m3V3(0.0f, 0.0f, 0.0f);
+    v.deltaRot[i] = m3IdentityQuat();
+}
+
 // The pairs are first binned by collision routine (the order inside a bin
 // is immaterial: every pair writes its own manifold slot), then each
 // wavefront of k_narrow_binned takes 64 pairs of one bin, so no wavefront
 // runs more than one routine (a wavefront mixing routines runs them one
-// after another).  Batched worlds bin per world (grid.y).
+// after another).  Batched worlds bin per world (grid.y).  The same pass
+// captures every body's step-start state first (the grid covers the larger
+// of the pair and body capacities); the narrowphase that follows unites
+// bodies over the reset union-find.
 #define M3_NARROW_CLASSES 7
 
 __global__ void k_narrow_bin(const m3Dev* __restrict__ ds)
@@ -1152,6 +1269,13 @@ __global__ void k_narrow_bin(const m3Dev* __restrict__ ds)
     __shared__ int32_t cnt[8];
     __shared__ int32_t base[8];
     const int32_t t = threadIdx.x;
+    {
+        const int32_t bi = blockIdx.x * blockDim.x + t;
+        if (bi < v.maxBody && v.sc->stall == 0)
+        {
+            CaptureBody(v, bi);
+        }
+    }
     if (t < 8)
     {
         cnt[t] = 0;
@@ -1212,7 +1336,7 @@ __global__ void __launch_bounds__(64) k_narrow_binned(const m3Dev* __restrict__ 
         v.hitRecs[i].flag = 0;
     }
     // The island union of a touching pair follows its manifold here (the
-    // bodies' union-find entries were reset by k_capture before this launch).
+    // bodies' union-find entries were reset by k_narrow_bin before this launch).
     if (m3NarrowPair(&v, i, v.sc->oldPairCount) > 0)
     {
         UnionTouching(v, i);
@@ -1222,33 +1346,6 @@ __global__ void __launch_bounds__(64) k_narrow_binned(const m3Dev* __restrict__ 
 // ---------------------------------------------------------------------------
 // Sweep capture and islands (src/solver.c:3437-3580, 3851-3881)
 // ---------------------------------------------------------------------------
-
-__global__ void k_capture(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0)
-    {
-        return;
-    }
-    if (v.bodyAlive[i] != 0)
-    {
-        m3Transform xf = v.xf[i];
-        m3Vec3 rlc = m3RotateVec3(xf.q, v.lc[i]);
-        m3Pos3 c;
-        c.x = xf.p.x + (double)rlc.x;
-        c.y = xf.p.y + (double)rlc.y;
-        c.z = xf.p.z + (double)rlc.z;
-        v.com0[i] = c;
-        v.rot0[i] = xf.q;
-    }
-    v.parent[i] = i;
-    v.forced[i] = 0;
-    v.flag[i] = 0;
-    v.pending[i] = 0; // the sweep has recorded last step's freezes
-    v.deltaPos[i] = m3V3(0.0f, 0.0f, 0.0f);
-    v.deltaRot[i] = m3IdentityQuat();
-}
 
 // Joints couple islands like touching contacts; a moving kinematic
 // partner is a wake source through a joint too (src/solver.c:3520-3547).
@@ -1400,16 +1497,42 @@ __global__ void k_veto_mark(const m3Dev* __restrict__ ds, const uint64_t* keys, 
     flags[i] = hit;
 }
 
-__global__ void k_constraint_flags(const m3Dev* __restrict__ ds)
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_constraint_flags(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
     uint8_t* flags = d.pairFlags;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.pairCapacity)
+    uint8_t f = 0;
+    if (i < v.pairCapacity)
     {
-        return;
+        f = (v.sc->stall == 0 && i < v.sc->pairCount && m3PairIsConstraint(&v, i)) ? 1 : 0;
+        flags[i] = f;
     }
-    flags[i] = (v.sc->stall == 0 && i < v.sc->pairCount && m3PairIsConstraint(&v, i)) ? 1 : 0;
+    FlagCount(f, d.blockCounts);
+}
+
+// The constraint list in pair order, with the awake dynamic bodies of each
+// constraint, the only bodies coloring tracks (src/solver.c:2741-2822).
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_constraints(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int flag = i < v.pairCapacity && d.pairFlags[i] != 0;
+    int32_t total;
+    const int32_t c = FlagRank(flag, d.blockCounts, &total);
+    if (flag)
+    {
+        v.consPair[c] = i;
+        const uint64_t key = v.keys[i];
+        const int32_t a = v.shapeBody[(int32_t)(key >> 32)];
+        const int32_t b = v.shapeBody[(int32_t)(key & 0xFFFFFFFFu)];
+        v.consBodies[2 * c] = (v.type[a] == M3_DYNAMIC && v.awake[a] != 0) ? a : -1;
+        v.consBodies[2 * c + 1] = (v.type[b] == M3_DYNAMIC && v.awake[b] != 0) ? b : -1;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    {
+        v.sc->constraintCount = total;
+    }
 }
 
 // The awake dynamic bodies of each constraint, the only bodies coloring
@@ -1758,17 +1881,13 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
 // Movers and the substep loop (src/solver.c:3938-4203)
 // ---------------------------------------------------------------------------
 
-__global__ void k_mover_flags(const m3Dev* __restrict__ ds, float dt)
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_mover_flags(const m3Dev* __restrict__ ds, float dt)
 {
     M3_WORLD;
     uint8_t* flags = d.bodyFlags;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody)
-    {
-        return;
-    }
     uint8_t f = 0;
-    if (v.sc->stall == 0 && v.bodyAlive[i] != 0 && v.enabled[i] != 0)
+    if (i < v.maxBody && v.sc->stall == 0 && v.bodyAlive[i] != 0 && v.enabled[i] != 0)
     {
         uint8_t type = v.type[i];
         if (type == M3_KINEMATIC || (type == M3_DYNAMIC && v.awake[i] != 0))
@@ -1780,7 +1899,29 @@ __global__ void k_mover_flags(const m3Dev* __restrict__ ds, float dt)
             m3KinematicServo(&v, i, dt);
         }
     }
-    flags[i] = f;
+    if (i < v.maxBody)
+    {
+        flags[i] = f;
+    }
+    FlagCount(f, d.blockCounts);
+}
+
+// The movers (solved bodies) in slot order.
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_movers(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int flag = i < v.maxBody && d.bodyFlags[i] != 0;
+    int32_t total;
+    const int32_t pos = FlagRank(flag, d.blockCounts, &total);
+    if (flag)
+    {
+        v.movers[pos] = i;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    {
+        v.sc->moverCount = total;
+    }
 }
 
 struct m3SubstepParams
@@ -2321,19 +2462,6 @@ __global__ void __launch_bounds__(M3_BLOCK_MAX_THREADS)
     SubstepLoop(v, sp, (int32_t)threadIdx.x, (int32_t)blockDim.x, m3BlockSync{}, d.tape);
 }
 
-__global__ void k_clear_forces(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || m >= v.sc->moverCount)
-    {
-        return;
-    }
-    int32_t i = v.movers[m];
-    v.force[i] = m3V3(0.0f, 0.0f, 0.0f);
-    v.torque[i] = m3V3(0.0f, 0.0f, 0.0f);
-}
-
 __global__ void k_restitution(const m3Dev* __restrict__ ds, float threshold)
 {
     M3_WORLD;
@@ -2348,15 +2476,26 @@ __global__ void k_restitution(const m3Dev* __restrict__ ds, float threshold)
     }
 }
 
+// The impulse store, and the applied forces of every mover cleared (the
+// grid covers the larger of the pair and body capacities).
 __global__ void k_store(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
     int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || c >= v.sc->constraintCount)
+    if (v.sc->stall != 0)
     {
         return;
     }
-    m3StoreOne(&v, &v.cons[c]);
+    if (c < v.sc->moverCount)
+    {
+        const int32_t i = v.movers[c];
+        v.force[i] = m3V3(0.0f, 0.0f, 0.0f);
+        v.torque[i] = m3V3(0.0f, 0.0f, 0.0f);
+    }
+    if (c < v.sc->constraintCount)
+    {
+        m3StoreOne(&v, &v.cons[c]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -3022,7 +3161,8 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.sc, 1) || DevAlloc(&w->dev.sapKeysIn, ns) || DevAlloc(&w->dev.sapKeysOut, ns) ||
         DevAlloc(&w->dev.sapValsIn, ns) || DevAlloc(&w->dev.sapValsOut, ns) ||
         DevAlloc(&w->dev.sapXY, ns) || DevAlloc(&w->dev.sapZ, ns) || DevAlloc(&w->dev.sapAttr, ns) ||
-        DevAlloc(&w->dev.pairFlags, np) || DevAlloc(&w->dev.bodyFlags, nb) || DevAlloc(&w->dev.root, nb) ||
+        DevAlloc(&w->dev.pairFlags, np) || DevAlloc(&w->dev.bodyFlags, nb) ||
+        DevAlloc(&w->dev.blockCounts, Blocks(np > nb ? np : nb, M3_COMPACT_THREADS)) || DevAlloc(&w->dev.root, nb) ||
         DevAlloc(&w->dev.bodyMasks, nb) || DevAlloc(&w->dev.bodyOwner, nb) ||
         DevAlloc(&v.slotColors, np) ||
         DevAlloc(&v.consBodies, 2 * (int64_t)np) || DevAlloc(&v.slotOf, np) ||
@@ -3068,16 +3208,6 @@ static int CreateImpl(m3gpuWorld* w)
     need = bytes > need ? bytes : need;
     HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, bytes, w->dev.emitSorted, w->dev.emitUnique,
                                          &v.sc->uniqueCount, v.emitCapacity, w->stream));
-    need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, v.keys, w->dev.pairFlags, v.sleepKeys,
-                                          &v.sc->sleepCount, np, w->stream));
-    need = bytes > need ? bytes : need;
-    hipcub::CountingInputIterator<int32_t> counting(0);
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->dev.pairFlags, v.consPair,
-                                          &v.sc->constraintCount, np, w->stream));
-    need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceSelect::Flagged(nullptr, bytes, counting, w->dev.bodyFlags, v.movers,
-                                          &v.sc->moverCount, nb, w->stream));
     need = bytes > need ? bytes : need;
     w->cubBytes = need > 0 ? need : 1;
     {
@@ -3168,7 +3298,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.sleepKeys, v.emit, v.satQueries, v.satList, w->dev.emitSorted, w->dev.emitUnique, v.cons, v.consPair,
                     v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
                     v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut, w->dev.sapValsIn,
-                    w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.root, w->dev.narrowBins,
+                    w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.blockCounts, w->dev.root, w->dev.narrowBins,
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
                     v.slotTable, v.slotMask, w->dev.barrier, w->cubTemp, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
@@ -3349,12 +3479,14 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         int unsupported = 0;
         int needsTree = 0;
         int bullets = 0;
+        int restitution = 0;
         for (int32_t s = 0; s < ns; ++s)
         {
             if (alive[s] == 0)
             {
                 continue;
             }
+            restitution |= h->shapeRestitution[s] > 0.0f;
             if (stype[s] == M3_PLANE)
             {
                 planes[pc++] = s;
@@ -3379,6 +3511,7 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
             }
         }
         w->ccdNeedsTree = needsTree;
+        w->anyRestitution = restitution;
         w->dev.ccdBullets = bullets;
         v.ccdStaticCount = sc;
         if (Copy(w, v.ccdStatics, statics, (int64_t)sc * 4, hipMemcpyHostToDevice) != 0)
@@ -3872,8 +4005,7 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     size_t tb = w->cubBytes;
     void* ct = w->cubTemp;
 
-    STAGE(ST_BEGIN, KL(k_begin, dim3(1), dim3(1), 0, st, D));
-    STAGE(ST_FILL, KL(k_fill_emit, dim3(Blocks(ne, T)), dim3(T), 0, st, D));
+    STAGE(ST_FILL, KL(k_open, dim3(Blocks(ne, T)), dim3(T), 0, st, D));
     // Broadphase.
     STAGE(ST_FRESH, KL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
     if (w->fullQuery)
@@ -3906,12 +4038,9 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
                                                                    ne, 0, 32, st)));
     STAGE(ST_UNIQUE, HIP_TRY(hipcub::DeviceSelect::Unique(ct, tb, hd.emitSorted, hd.emitUnique,
                                                           &v.sc->uniqueCount, ne, st)));
-    STAGE(ST_PAIR_COUNT, KL(k_pair_count, dim3(1), dim3(1), 0, st, D));
-    STAGE(ST_EXPAND, KL(k_expand, dim3(Blocks(np, T)), dim3(T), 0, st, D));
-    STAGE(ST_STALL_RESTORE, KL(k_stall_restore, dim3(Blocks(np, T)), dim3(T), 0, st, D));
-    STAGE(ST_SLEEP_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, v.keys, hd.pairFlags,
-                                                                 v.sleepKeys, &v.sc->sleepCount,
-                                                                 np, st)));
+    const int CT = M3_COMPACT_THREADS;
+    STAGE(ST_EXPAND, KL(k_pairs_settle, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
+    STAGE(ST_SLEEP_SELECT, KL(k_compact_frozen, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
     // Narrowphase.
     if (w->wideHulls || w->boxHulls)
     {
@@ -3920,8 +4049,7 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     }
     // Step-start body state and union-find reset, then the narrowphase,
     // which also unites the bodies of each touching pair.
-    STAGE(ST_CAPTURE, KL(k_capture, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
-    STAGE(ST_NARROW, KL(k_narrow_bin, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    STAGE(ST_NARROW, KL(k_narrow_bin, dim3(Blocks(np > nb ? np : nb, T)), dim3(T), 0, st, D));
     STAGE(ST_NARROW, KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES), dim3(64), 0, st, D));
     // Islands and wake.
     if (v.maxJoint > 0)
@@ -3975,12 +4103,9 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
     }
     // Prepare and color.
     const m3SubstepParams sp = SubstepParams(p);
-    STAGE(ST_CONS_FLAGS, KL(k_constraint_flags, dim3(Blocks(np, T)), dim3(T), 0, st, D));
-    hipcub::CountingInputIterator<int32_t> counting(0);
-    STAGE(ST_CONS_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, hd.pairFlags,
-                                                                v.consPair,
-                                                                &v.sc->constraintCount, np, st)));
-    STAGE(ST_CONS_BODIES, KL(k_cons_bodies, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    const int CT = M3_COMPACT_THREADS;
+    STAGE(ST_CONS_FLAGS, KL(k_constraint_flags, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
+    STAGE(ST_CONS_SELECT, KL(k_compact_constraints, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
     if (nb <= M3_COLOR_LDS_BODIES)
     {
         STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D));
@@ -3996,10 +4121,8 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
         STAGE(ST_PREPARE,
               KL(k_prepare_joints, dim3(Blocks(v.maxJoint, 64)), dim3(64), 0, st, D, sp.h));
     }
-    STAGE(ST_MOVER_FLAGS, KL(k_mover_flags, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt));
-    STAGE(ST_MOVER_SELECT, HIP_TRY(hipcub::DeviceSelect::Flagged(ct, tb, counting, hd.bodyFlags,
-                                                                 v.movers, &v.sc->moverCount, nb,
-                                                                 st)));
+    STAGE(ST_MOVER_FLAGS, KL(k_mover_flags, dim3(Blocks(nb, CT)), dim3(CT), 0, st, D, p->dt));
+    STAGE(ST_MOVER_SELECT, KL(k_compact_movers, dim3(Blocks(nb, CT)), dim3(CT), 0, st, D));
     // Substeps: one cooperative launch.
     {
         m3SubstepParams spArg = sp;
@@ -4017,9 +4140,11 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
         STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps, dim3(blocks),
                                                               dim3(w->coopThreads), args, 0, st)));
     }
-    STAGE(ST_CLEAR_FORCES, KL(k_clear_forces, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
-    STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, D, p->restitutionThreshold));
-    STAGE(ST_STORE, KL(k_store, dim3(Blocks(np, T)), dim3(T), 0, st, D));
+    if (w->anyRestitution)
+    {
+        STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, D, p->restitutionThreshold));
+    }
+    STAGE(ST_STORE, KL(k_store, dim3(Blocks(np > nb ? np : nb, T)), dim3(T), 0, st, D));
     if (v.maxJoint > 0)
     {
         STAGE(ST_STORE, KL(k_store_joints, dim3(Blocks(v.maxJoint, T)), dim3(T), 0, st, D));
@@ -4670,8 +4795,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     {
         KL(k_tape_start, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     }
-    KL(k_begin, dim3(1, W), dim3(1), 0, st, D);
-    KL(k_fill_emit, dim3(Blocks(ne, T), W), dim3(T), 0, st, D);
+    KL(k_open, dim3(Blocks(ne, T), W), dim3(T), 0, st, D);
     KL(k_fresh, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
     KL(k_reset_sleep, dim3(1, W), dim3(1), 0, st, D);
     KL(k_pending_all, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
@@ -4685,9 +4809,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_refresh, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
     KL(k_frozen, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     KL(k_emit_sort_block, dim3(1, W), dim3(BT), 0, st, D);
-    KL(k_pair_count, dim3(1, W), dim3(1), 0, st, D);
-    KL(k_expand, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
-    KL(k_stall_restore, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_pairs_settle, dim3(Blocks(np, M3_COMPACT_THREADS), W), dim3(M3_COMPACT_THREADS), 0, st, D);
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 0);
     if (s_batch.wideHulls)
     {
@@ -4696,8 +4818,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     }
     // Step-start reset, then the narrowphase binned by collision routine
     // per world, which also unites the bodies of each touching pair.
-    KL(k_capture, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
-    KL(k_narrow_bin, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_narrow_bin, dim3(Blocks(np > nb ? np : nb, T), W), dim3(T), 0, st, D);
     KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES, W), dim3(64), 0, st, D);
     if (taped)
     {
@@ -4727,9 +4848,8 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
         KL(k_tape_solver, dim3(1, W), dim3(256), 0, st, D);
     }
     KL(k_substeps_block, dim3(1, W), dim3(s_batch.threads), 0, st, D, sp);
-    KL(k_clear_forces, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_restitution, dim3(1, W), dim3(1), 0, st, D, p->restitutionThreshold);
-    KL(k_store, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
+    KL(k_store, dim3(Blocks(np > nb ? np : nb, T), W), dim3(T), 0, st, D);
     if (s_batch.maxJoint > 0)
     {
         KL(k_store_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);

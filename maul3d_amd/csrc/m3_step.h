@@ -1367,11 +1367,22 @@ M3_HD void m3WarmStartSide(const m3Constraint* c, int side, m3Vec3* linIo, m3Vec
     const m3Vec3 rolling = c->rollingImpulse;
     m3Vec3 lin = *linIo;
     m3Vec3 ang = *angIo;
-    // Slots past pointCount are evaluated and discarded (no branch on the
-    // point count between the loads and their use).
+    // Slots up to the largest point count among the lanes running together
+    // are evaluated and those past a lane's own count discarded (the bound
+    // is uniform across the wavefront; the loads were issued above).
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int32_t slots = __ballot(pointCount > 2) != 0 ? (__ballot(pointCount > 3) != 0 ? 4 : 3)
+                                                        : (__ballot(pointCount > 1) != 0 ? 2 : 1);
+#else
+    const int32_t slots = M3_MANIFOLD_MAX_POINTS;
+#endif
     M3_UNROLL
     for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
+        if (k >= slots)
+        {
+            break;
+        }
         m3Vec3 impulse = m3MulSV3(imp[k], normal);
         m3Vec3 nl;
         m3Vec3 na;
