@@ -389,6 +389,11 @@ typedef struct m3gpuAdjointRequest
     float* gradForce;           // maxBody * 3
     int32_t* unsupported; // touching pairs or constraints outside the scope
     int32_t* ccdPulled;   // continuous pull-backs passed through as identity
+    // 32: the float trajectory on the device's step kernels.  64: the step
+    // and its reverse in double precision, from `host` (the world's current
+    // host arrays, required then); every request of a call shares it.
+    int32_t precision;
+    const m3gpuHostState* host;
 } m3gpuAdjointRequest;
 
 M3GPU_API int m3gpu_adjoint_many(m3gpuWorld* const* worlds, const m3gpuStepParams* params,

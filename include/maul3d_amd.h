@@ -423,6 +423,12 @@ typedef struct m3AdjointDef
     int32_t offsetBody; // body slot
     int32_t offsetComp;
     double offsetValue;
+    // 32 (the default): the gradients of the float trajectory the step
+    // runs, exact to the world's own steps.  64: the trajectory and its
+    // reverse recomputed in double precision from the world's state, the
+    // gradients FP64 finite differences converge to (one workgroup per
+    // world; all worlds of a call share the precision).
+    int32_t precision;
 } m3AdjointDef;
 
 typedef struct m3AdjointResult

@@ -270,6 +270,7 @@ class AdjointDef(C.Structure):
         ("offsetBody", c_i32),
         ("offsetComp", c_i32),
         ("offsetValue", C.c_double),
+        ("precision", c_i32),
     ]
 
 

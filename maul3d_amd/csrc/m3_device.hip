@@ -18,6 +18,7 @@
 
 #include "m3gpu.h"
 #include "m3_adjoint.h"
+#include "m3_adj64.h"
 
 #define M3_SENTINEL32 0xFFFFFFFFu
 #define M3_SENTINEL64 0xFFFFFFFFFFFFFFFFull
@@ -6080,6 +6081,173 @@ static int AdjointRun(m3gpuWorld* const* ws, const m3gpuStepParams* ps, const m3
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Trajectory gradients in double precision (m3_adj64.h): one 64-lane
+// workgroup per world over its image in the adjoint arena.
+//
+// Switched off (M3_ADJ64_DEVICE 0): its first run on the MI355X ended in a
+// GPU memory fault whose cause is not found.  The same image and driver run
+// clean on the CPU (one lane and 64 threaded lanes, exact-size allocations
+// under AddressSanitizer) and equal the FP64 twin bit for bit; the kernel
+// needs 20.9 KB of scratch per lane, four times any kernel run here so far.
+// Until the cause is shown, precision 64 is refused.
+// ---------------------------------------------------------------------------
+
+#ifndef M3_ADJ64_DEVICE
+#define M3_ADJ64_DEVICE 0
+#endif
+
+#if M3_ADJ64_DEVICE
+__global__ void __launch_bounds__(M3_ADJ_LANES) k_adj64(m3Adj64World* worlds)
+{
+    __shared__ m3Adj64Shared sh;
+    Adj64Body(&worlds[blockIdx.x], (int)threadIdx.x, M3_ADJ_LANES, m3BlockSync{}, &sh);
+}
+
+static int Adjoint64Run(m3gpuWorld* const* ws, const m3gpuStepParams* ps, const m3gpuAdjointRequest* rqs,
+                        int32_t count)
+{
+    for (int32_t k = 0; k < count; ++k)
+    {
+        const m3gpuHostState* h = rqs[k].host;
+        if (h == nullptr || rqs[k].steps < 1)
+        {
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: precision 64 needs the host state");
+            return -1;
+        }
+        for (int32_t b = 0; b < h->maxBody; ++b)
+        {
+            if (h->bodyAlive[b] != 0 && h->bodyHasTarget[b] != 0)
+            {
+                snprintf(s_lastError, sizeof(s_lastError),
+                         "m3gpu_adjoint_many: kinematic targets are outside the double-precision pass");
+                return -1;
+            }
+        }
+    }
+    // Sizing pass, then one image: world records first, then each world's
+    // arrays.
+    std::vector<int64_t> base(count + 1);
+    const int64_t recBytes = Adj64Align((int64_t)count * sizeof(m3Adj64World));
+    base[0] = recBytes;
+    std::vector<m3Adj64World> recs(count);
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3Adj64Carver B{nullptr, nullptr, 0};
+        Adj64Carve(B, ws[k]->caps.pairCapacity, &ps[k], &rqs[k], &recs[k]);
+        base[k + 1] = base[k] + B.off;
+    }
+    const int64_t total = base[count];
+    if (AdjReserve((size_t)total) != 0)
+    {
+        return -1;
+    }
+    uint8_t* image = (uint8_t*)calloc((size_t)total, 1);
+    if (image == nullptr)
+    {
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: out of host memory");
+        return -1;
+    }
+    uint8_t* dev = (uint8_t*)s_adjArena;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        m3Adj64Carver B{image + base[k], dev + base[k], 0};
+        Adj64Carve(B, ws[k]->caps.pairCapacity, &ps[k], &rqs[k], &recs[k]);
+    }
+    memcpy(image, recs.data(), (size_t)count * sizeof(m3Adj64World));
+    if (s_adjStream == nullptr)
+    {
+        const hipError_t e = hipStreamCreateWithFlags(&s_adjStream, hipStreamNonBlocking);
+        if (e != hipSuccess)
+        {
+            free(image);
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: %s", hipGetErrorString(e));
+            return -1;
+        }
+    }
+    hipStream_t st = s_adjStream;
+    hipError_t e = hipMemcpyAsync(dev, image, (size_t)total, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+    {
+        hipLaunchKernelGGL(k_adj64, dim3(count), dim3(M3_ADJ_LANES), 0, st, (m3Adj64World*)dev);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+    {
+        e = hipMemcpyAsync(image, dev, (size_t)total, hipMemcpyDeviceToHost, st);
+    }
+    if (e == hipSuccess)
+    {
+        e = hipStreamSynchronize(st);
+    }
+    if (e != hipSuccess)
+    {
+        free(image);
+        snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: %s", hipGetErrorString(e));
+        return -1;
+    }
+    int rc = 0;
+    for (int32_t k = 0; k < count; ++k)
+    {
+        const m3Adj64World* c = (const m3Adj64World*)image + k;
+        const m3gpuAdjointRequest* r = &rqs[k];
+        if (c->failed != 0)
+        {
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: a step stalled");
+            rc = -1;
+            continue;
+        }
+        if (r->loss != nullptr)
+        {
+            *r->loss = c->loss;
+        }
+        if (r->forwardOnly)
+        {
+            continue;
+        }
+        const float* g = (const float*)(image + ((uint8_t*)c->grads - dev));
+        const int32_t nb = r->host->maxBody;
+        for (int32_t b = 0; b < nb; ++b)
+        {
+            const float* gb = g + 11 * b;
+            for (int i = 0; i < 3; ++i)
+            {
+                if (r->gradLinearVelocity != nullptr)
+                {
+                    r->gradLinearVelocity[3 * b + i] = gb[i];
+                }
+                if (r->gradAngularVelocity != nullptr)
+                {
+                    r->gradAngularVelocity[3 * b + i] = gb[3 + i];
+                }
+                if (r->gradForce != nullptr)
+                {
+                    r->gradForce[3 * b + i] = gb[8 + i];
+                }
+            }
+            if (r->gradMass != nullptr)
+            {
+                r->gradMass[b] = gb[6];
+            }
+            if (r->gradFriction != nullptr)
+            {
+                r->gradFriction[b] = gb[7];
+            }
+        }
+        if (r->unsupported != nullptr)
+        {
+            *r->unsupported = c->aw.unsupported;
+        }
+        if (r->ccdPulled != nullptr)
+        {
+            *r->ccdPulled = c->aw.ccdPulled;
+        }
+    }
+    free(image);
+    return rc;
+}
+#endif // M3_ADJ64_DEVICE
+
 extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* ps,
                                   const m3gpuAdjointRequest* rqs, int32_t count)
 {
@@ -6096,8 +6264,28 @@ extern "C" int m3gpu_adjoint_many(m3gpuWorld* const* ws, const m3gpuStepParams* 
         HIP_TRY(hipStreamSynchronize(ws[k]->stream));
         ws[k]->streamBusy = 0;
     }
+    for (int32_t k = 0; k < count; ++k)
+    {
+        if ((rqs[k].precision == 64) != (rqs[0].precision == 64))
+        {
+            snprintf(s_lastError, sizeof(s_lastError), "m3gpu_adjoint_many: requests mix precisions");
+            return -1;
+        }
+    }
+#if !M3_ADJ64_DEVICE
+    if (rqs[0].precision == 64)
+    {
+        snprintf(s_lastError, sizeof(s_lastError),
+                 "m3gpu_adjoint_many: precision 64 is switched off on the device");
+        return M3GPU_REFUSED;
+    }
+#endif
     pthread_mutex_lock(&s_batchLock);
+#if M3_ADJ64_DEVICE
+    int rc = rqs[0].precision == 64 ? Adjoint64Run(ws, ps, rqs, count) : AdjointRun(ws, ps, rqs, count);
+#else
     int rc = AdjointRun(ws, ps, rqs, count);
+#endif
     BatchDrop();
     pthread_mutex_unlock(&s_batchLock);
     return rc;

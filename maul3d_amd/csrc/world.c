@@ -2812,6 +2812,7 @@ m3AdjointDef m3DefaultAdjointDef(void)
     d.subStepCount = 4;
     d.steps = 120;
     d.checkpointEvery = 10;
+    d.precision = 32;
     return d;
 }
 
@@ -2830,13 +2831,25 @@ int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointD
     m3gpuWorld* devs[CHUNK];
     m3gpuStepParams params[CHUNK];
     m3gpuAdjointRequest reqs[CHUNK];
+    const int32_t precision = defs[0].precision == 64 ? 64 : 32;
+    m3gpuHostState* hosts = NULL;
+    if (precision == 64)
+    {
+        hosts = (m3gpuHostState*)calloc((size_t)count, sizeof(m3gpuHostState));
+        if (hosts == NULL)
+        {
+            return -1;
+        }
+    }
     for (int32_t k = 0; k < count; ++k)
     {
         m3World* w = WorldFromId(worlds[k]);
         const m3AdjointDef* d = &defs[k];
         if (w == NULL || !(d->timeStep > 0.0f) || d->subStepCount < 1 ||
-            d->subStepCount > M3_MAX_SUBSTEPS || d->steps < 1 || d->checkpointEvery < 1)
+            d->subStepCount > M3_MAX_SUBSTEPS || d->steps < 1 || d->checkpointEvery < 1 ||
+            (d->precision == 64) != (precision == 64))
         {
+            free(hosts);
             return -1;
         }
         // The run reuses the world's device pair buffers: build the last
@@ -2846,11 +2859,13 @@ int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointD
         {
             if (w->jointPool.alive[j])
             {
+                free(hosts);
                 return -1; // joints are outside the adjoint's scope
             }
         }
-        if (!EnsureDevice(w))
+        if (!EnsureDevice(w) || (hosts != NULL && !SyncHost(w)))
         {
+            free(hosts);
             return -1;
         }
         devs[k] = w->dev;
@@ -2877,10 +2892,17 @@ int32_t m3World_Adjoint(const m3WorldId* worlds, int32_t count, const m3AdjointD
         r->gradForce = (float*)o->gradForce;
         r->unsupported = &o->unsupported;
         r->ccdPulled = &o->ccdPulled;
+        r->precision = precision;
+        if (hosts != NULL)
+        {
+            HostView(w, &hosts[k]);
+            r->host = &hosts[k];
+        }
         o->unsupported = 0;
         o->ccdPulled = 0;
     }
     int rc = m3gpu_adjoint_many(devs, params, reqs, count);
+    free(hosts);
     if (rc < 0)
     {
         for (int32_t k = 0; k < count; ++k)

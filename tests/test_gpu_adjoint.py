@@ -1,4 +1,4 @@
-"""Trajectory gradients on the device against the CPU twin.
+"""Trajectory gradients on the device against the CPU twin and FP64 differences.
 
 The device runs the serial step and the hand-written reverse pass one
 thread per world at float precision; the float twin (oracle/twin) runs the
@@ -154,3 +154,95 @@ def test_trajectory_loss_on_device_equals_float_twin():
     assert lg == lt
     for a, b in zip(gg, gt):
         assert torch.allclose(a, b, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.skip(reason="precision 64 is switched off on the device: its first MI355X run ended in a "
+                  "GPU memory fault whose cause is not yet shown (its driver runs clean on the CPU)")
+def test_fp64_device_gradients_match_fp64_differences_on_all_64_worlds():
+    """Precision 64 (the trajectory and its reverse in double precision on
+    the device) on the full toss configuration: all 64 worlds, 120 steps.
+    Every gradient of every world equals the FP64 twin's adjoint to float
+    output rounding, the losses agree, and on two random parameters per
+    world the device gradient matches FP64 central differences (taken on
+    the FP64 twin, steps eps and 4 eps agreeing to 1e-5: contact-stable) to
+    1e-3 relative."""
+    import random
+
+    from tests.test_adjoint import EPS, KINDS, TWIN64
+    from tests.test_adjoint import adjoint as twin_adjoint
+
+    if not os.path.exists(TWIN64):
+        pytest.skip("oracle twin not built (make -C oracle/twin)")
+    T64 = api.Api(C.CDLL(TWIN64))
+    gpu = product_api()
+    seeds = list(range(64))
+    wg = [scenes.toss(gpu, s) for s in seeds]
+    n = len(seeds)
+    defs = (api.AdjointDef * n)()
+    res = (api.AdjointResult * n)()
+    keep = []
+    for k in range(n):
+        d = gpu.m3DefaultAdjointDef()
+        d.steps = 120
+        d.checkpointEvery = 10
+        d.target = api.Vec3(1.0, 0.5, -2.0)
+        d.precision = 64
+        defs[k] = d
+        arrs = {}
+        for name in NAMES[:3]:
+            a = (api.Vec3 * NB)()
+            arrs[name] = a
+            setattr(res[k], name, C.cast(a, C.POINTER(api.Vec3)))
+        for name in NAMES[3:]:
+            a = (C.c_float * NB)()
+            arrs[name] = a
+            setattr(res[k], name, C.cast(a, C.POINTER(C.c_float)))
+        keep.append(arrs)
+    ids = (api.WorldId * n)(*wg)
+    assert gpu.m3World_Adjoint(ids, n, defs, res) == n
+    rng = random.Random(0xF64)
+    stable = 0
+    for k, s in enumerate(seeds):
+        w64 = scenes.toss(T64, s)
+        r64, g64 = twin_adjoint(T64, w64, 120, every=10)
+        assert res[k].unsupported == r64.unsupported == 0
+        assert res[k].loss == pytest.approx(r64.loss, rel=1e-12, abs=1e-12), s
+        for a, b in zip(flat(keep[k]), flat(g64)):
+            assert abs(a - b) <= 2e-6 * max(1.0, abs(b)), (s, a, b)
+        for _ in range(2):
+            kind = rng.choice([1, 2, 3, 4, 5])
+            body = rng.randrange(1, 33)
+            comp = rng.randrange(3) if kind in (1, 2, 5) else 0
+            eps = EPS[kind]
+            L = lambda e: twin_adjoint(T64, w64, 120, (kind, body, comp, e), True, every=10)[0].loss  # noqa: E731
+            fd = (L(eps) - L(-eps)) / (2 * eps)
+            fd4 = (L(4 * eps) - L(-4 * eps)) / (8 * eps)
+            scale = max(abs(fd), 1e-6)
+            if abs(fd - fd4) / scale > 1e-5:
+                continue
+            stable += 1
+            name = KINDS[kind]
+            a = keep[k][name][body]
+            dev = (a.x, a.y, a.z)[comp] if kind in (1, 2, 5) else a
+            assert abs(dev - fd) / scale < 1e-3, (s, name, body, comp, dev, fd)
+        T64.m3DestroyWorld(w64)
+    assert stable >= 6, f"only {stable} contact-stable samples"
+    for w in wg:
+        gpu.m3DestroyWorld(w)
+
+
+def test_precision_64_is_refused_without_faulting_the_world():
+    """While the double-precision device pass is switched off, a request
+    for it is refused (-1) and the world keeps stepping."""
+    gpu = product_api()
+    w = scenes.toss(gpu, 3)
+    d = gpu.m3DefaultAdjointDef()
+    d.steps = 10
+    d.precision = 64
+    r = api.AdjointResult()
+    assert gpu.m3World_Adjoint((api.WorldId * 1)(w), 1, C.byref(d), C.byref(r)) == -1
+    assert gpu.m3World_GetDeviceError(w) is None
+    h = gpu.m3World_Hash(w)
+    gpu.m3World_Step(w, DT, SUBSTEPS)
+    assert gpu.m3World_Hash(w) != h
+    gpu.m3DestroyWorld(w)
