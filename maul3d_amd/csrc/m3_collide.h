@@ -435,59 +435,102 @@ M3_HD void m3DeepPointInHull(const m3HullData* hull, m3Vec3 q, m3Vec3* normalOut
 }
 
 // ---- capsule segment versus hull (src/manifold.c:534-714) ------------------
-M3_HD m3Manifold m3CollideSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 p2, m3real radius)
+
+// Separation of the segment against hull face f: the deeper endpoint.
+M3_HD m3real m3SegmentFaceSep(const m3HullData* hull, int32_t f, m3Vec3 p1, m3Vec3 p2)
+{
+    m3real d1 = m3Dot3(hull->faceNormals[f], p1) - hull->faceOffsets[f];
+    m3real d2 = m3Dot3(hull->faceNormals[f], p2) - hull->faceOffsets[f];
+    return m3MinF(d1, d2);
+}
+
+// The axis of hull edge e (even half-edge index) crossed with the segment,
+// oriented away from the hull's centre; 0 when the two are parallel.
+M3_HD int m3SegmentEdgeAxis(const m3HullData* hull, int32_t e, m3Vec3 segDir, m3Vec3* axisOut)
+{
+    m3Vec3 a = hull->vertices[hull->edges[e].origin];
+    m3Vec3 b = hull->vertices[hull->edges[e + 1].origin];
+    m3Vec3 axis = m3Cross3(m3Sub3(b, a), segDir);
+    m3real len2 = m3Dot3(axis, axis);
+    if (len2 < 1.0e-10f)
+    {
+        return 0;
+    }
+    axis = m3MulSV3(1.0f / m3Sqrt(len2), axis);
+    if (m3Dot3(axis, m3Sub3(a, hull->center)) < 0.0f)
+    {
+        axis = m3Neg3(axis);
+    }
+    *axisOut = axis;
+    return 1;
+}
+
+// Hulls whose edge query against a segment (edges x vertices projections)
+// is larger than a box's (12 x 8) take the wavefront query on the device.
+M3_HD int m3WideSegmentHull(const m3HullData* hull)
+{
+    return (hull->edgeCount >> 1) * hull->vertexCount > 96;
+}
+
+// `sat`, when given, holds the face query (faceA: the deepest face and its
+// separation) and the edge query (edge: indexA the half-edge, -1 when no
+// edge axis, its separation and axis) already evaluated; they are pure
+// functions of the hull and the segment, so taking them from there changes
+// nothing but the work done.
+M3_HD m3Manifold m3CollideSegmentHullQ(const m3HullData* hull, m3Vec3 p1, m3Vec3 p2, m3real radius,
+                                       const m3SatQuery* sat)
 {
     const m3real linearSlop = M3_LINEAR_SLOP;
     m3Manifold manifold = m3ZeroManifold();
 
     int32_t bestFace = 0;
     m3real bestFaceSep = -3.4e38f;
-    for (int32_t f = 0; f < hull->faceCount; ++f)
-    {
-        m3real d1 = m3Dot3(hull->faceNormals[f], p1) - hull->faceOffsets[f];
-        m3real d2 = m3Dot3(hull->faceNormals[f], p2) - hull->faceOffsets[f];
-        m3real sep = m3MinF(d1, d2);
-        if (sep > bestFaceSep)
-        {
-            bestFaceSep = sep;
-            bestFace = f;
-        }
-    }
-
     m3Vec3 segDir = m3Sub3(p2, p1);
     m3Vec3 bestEdgeAxis = m3V3(0.0f, 1.0f, 0.0f);
     m3real bestEdgeSep = -3.4e38f;
     int32_t bestEdge = -1;
-    for (int32_t e = 0; e < hull->edgeCount; e += 2)
+    if (sat != NULL)
     {
-        m3Vec3 a = hull->vertices[hull->edges[e].origin];
-        m3Vec3 b = hull->vertices[hull->edges[e + 1].origin];
-        m3Vec3 axis = m3Cross3(m3Sub3(b, a), segDir);
-        m3real len2 = m3Dot3(axis, axis);
-        if (len2 < 1.0e-10f)
+        bestFace = sat->faceA.faceIndex;
+        bestFaceSep = sat->faceA.separation;
+        bestEdge = sat->edge.indexA;
+        bestEdgeSep = sat->edge.separation;
+        bestEdgeAxis = sat->edge.axis;
+    }
+    else
+    {
+        for (int32_t f = 0; f < hull->faceCount; ++f)
         {
-            continue;
-        }
-        axis = m3MulSV3(1.0f / m3Sqrt(len2), axis);
-        if (m3Dot3(axis, m3Sub3(a, hull->center)) < 0.0f)
-        {
-            axis = m3Neg3(axis);
-        }
-        m3real hullMax = -3.4e38f;
-        for (int32_t v = 0; v < hull->vertexCount; ++v)
-        {
-            m3real proj = m3Dot3(axis, hull->vertices[v]);
-            if (proj > hullMax)
+            m3real sep = m3SegmentFaceSep(hull, f, p1, p2);
+            if (sep > bestFaceSep)
             {
-                hullMax = proj;
+                bestFaceSep = sep;
+                bestFace = f;
             }
         }
-        m3real sep = m3MinF(m3Dot3(axis, p1), m3Dot3(axis, p2)) - hullMax;
-        if (sep > bestEdgeSep)
+        for (int32_t e = 0; e < hull->edgeCount; e += 2)
         {
-            bestEdgeSep = sep;
-            bestEdgeAxis = axis;
-            bestEdge = e;
+            m3Vec3 axis;
+            if (!m3SegmentEdgeAxis(hull, e, segDir, &axis))
+            {
+                continue;
+            }
+            m3real hullMax = -3.4e38f;
+            for (int32_t v = 0; v < hull->vertexCount; ++v)
+            {
+                m3real proj = m3Dot3(axis, hull->vertices[v]);
+                if (proj > hullMax)
+                {
+                    hullMax = proj;
+                }
+            }
+            m3real sep = m3MinF(m3Dot3(axis, p1), m3Dot3(axis, p2)) - hullMax;
+            if (sep > bestEdgeSep)
+            {
+                bestEdgeSep = sep;
+                bestEdgeAxis = axis;
+                bestEdge = e;
+            }
         }
     }
 
@@ -590,6 +633,11 @@ M3_HD m3Manifold m3CollideSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 
     return manifold;
 }
 
+M3_HD m3Manifold m3CollideSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 p2, m3real radius)
+{
+    return m3CollideSegmentHullQ(hull, p1, p2, radius, NULL);
+}
+
 // ---- GJK distance (src/distance.c:1-710) -----------------------------------
 
 #define M3_MAX_GJK_ITERATIONS 32
@@ -616,6 +664,7 @@ typedef struct m3Simplex
     m3SimplexVertex v[4];
     int32_t count;
 } m3Simplex;
+
 
 typedef struct m3GjkOut
 {

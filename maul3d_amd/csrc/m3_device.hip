@@ -848,11 +848,18 @@ __global__ void k_sat_list(const m3Dev* __restrict__ ds)
     uint64_t key = v.keys[i];
     int32_t shapeA = (int32_t)(key >> 32);
     int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
-    if (v.shapeType[shapeA] != M3_HULL || v.shapeType[shapeB] != M3_HULL)
+    const uint8_t typeA = v.shapeType[shapeA];
+    const uint8_t typeB = v.shapeType[shapeB];
+    int wide = 0;
+    if (typeA == M3_HULL && typeB == M3_HULL)
     {
-        return;
+        wide = m3WideHullPair(&v.hulls[v.hullIndex[shapeA]], &v.hulls[v.hullIndex[shapeB]], v.satBoxes);
     }
-    if (m3WideHullPair(&v.hulls[v.hullIndex[shapeA]], &v.hulls[v.hullIndex[shapeB]], v.satBoxes))
+    else if ((typeA == M3_HULL && typeB == M3_CAPSULE) || (typeA == M3_CAPSULE && typeB == M3_HULL))
+    {
+        wide = m3WideSegmentHull(&v.hulls[v.hullIndex[typeA == M3_HULL ? shapeA : shapeB]]);
+    }
+    if (wide)
     {
         v.satList[atomicAdd(&v.sc->satCount, 1)] = i;
     }
@@ -1079,6 +1086,94 @@ __device__ m3EdgeQuery WaveQueryEdges(const m3HullData* hullA, const m3HullData*
     return out;
 }
 
+// The face and edge queries of a capsule segment (p1, p2 in the hull's
+// frame) against the hull over the lanes: each lane evaluates its share of
+// the faces and edges with the serial scan's expressions (m3SegmentFaceSep,
+// m3SegmentEdgeAxis and the vertex maximum), and the reduction keeps the
+// serial scan's choice, the first index of the largest separation (a lane
+// offers its candidates in ascending index, with the same strict test), so
+// the queries equal m3CollideSegmentHullQ's own loops bit for bit.
+__device__ m3SatQuery WaveSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 p2, SatLds& L, int lane)
+{
+    const int32_t nv = hull->vertexCount;
+    for (int32_t k = lane; k < nv; k += 64)
+    {
+        L.w[k] = hull->vertices[k];
+    }
+    __syncthreads();
+    int32_t face = 0;
+    m3real faceSep = -3.4e38f;
+    for (int32_t f = lane; f < hull->faceCount; f += 64)
+    {
+        const m3real sep = m3SegmentFaceSep(hull, f, p1, p2);
+        if (sep > faceSep)
+        {
+            faceSep = sep;
+            face = f;
+        }
+    }
+    const m3Vec3 segDir = m3Sub3(p2, p1);
+    int32_t edge = -1;
+    m3real edgeSep = -3.4e38f;
+    m3Vec3 edgeAxis = m3V3(0.0f, 1.0f, 0.0f);
+    for (int32_t e = 2 * lane; e < hull->edgeCount; e += 128)
+    {
+        m3Vec3 axis;
+        if (!m3SegmentEdgeAxis(hull, e, segDir, &axis))
+        {
+            continue;
+        }
+        m3real hullMax = -3.4e38f;
+        for (int32_t k = 0; k < nv; ++k)
+        {
+            const m3real proj = m3Dot3(axis, L.w[k]);
+            if (proj > hullMax)
+            {
+                hullMax = proj;
+            }
+        }
+        const m3real sep = m3MinF(m3Dot3(axis, p1), m3Dot3(axis, p2)) - hullMax;
+        if (sep > edgeSep)
+        {
+            edgeSep = sep;
+            edgeAxis = axis;
+            edge = e;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        const int32_t oFace = __shfl_xor(face, off, 64);
+        const m3real oFaceSep = __shfl_xor(faceSep, off, 64);
+        if (oFaceSep > faceSep || (oFaceSep == faceSep && oFace < face))
+        {
+            faceSep = oFaceSep;
+            face = oFace;
+        }
+        const int32_t oEdge = __shfl_xor(edge, off, 64);
+        const m3real oEdgeSep = __shfl_xor(edgeSep, off, 64);
+        const m3real ox = __shfl_xor(edgeAxis.x, off, 64);
+        const m3real oy = __shfl_xor(edgeAxis.y, off, 64);
+        const m3real oz = __shfl_xor(edgeAxis.z, off, 64);
+        if (oEdgeSep > edgeSep || (oEdgeSep == edgeSep && oEdge < edge))
+        {
+            edgeSep = oEdgeSep;
+            edge = oEdge;
+            edgeAxis = m3V3(ox, oy, oz);
+        }
+    }
+    __syncthreads();
+    m3SatQuery out;
+    out.faceA.faceIndex = face;
+    out.faceA.separation = faceSep;
+    out.faceB.faceIndex = 0;
+    out.faceB.separation = 0.0f;
+    out.edge.indexA = edge;
+    out.edge.indexB = -1;
+    out.edge.separation = edgeSep;
+    out.edge.axis = edgeAxis;
+    return out;
+}
+
 // One 64-lane workgroup per listed pair, grid-stride over the list.
 __global__ __launch_bounds__(64) void k_sat(const m3Dev* __restrict__ ds)
 {
@@ -1096,6 +1191,20 @@ __global__ __launch_bounds__(64) void k_sat(const m3Dev* __restrict__ ds)
         uint64_t key = v.keys[i];
         int32_t shapeA = (int32_t)(key >> 32);
         int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+        if (v.shapeType[shapeA] == M3_CAPSULE || v.shapeType[shapeB] == M3_CAPSULE)
+        {
+            const int32_t hullShape = v.shapeType[shapeA] == M3_HULL ? shapeA : shapeB;
+            const int32_t capShape = hullShape == shapeA ? shapeB : shapeA;
+            m3Vec3 s1;
+            m3Vec3 s2;
+            m3CapsuleInHull(&v, hullShape, capShape, &s1, &s2);
+            const m3SatQuery out = WaveSegmentHull(&v.hulls[v.hullIndex[hullShape]], s1, s2, L, lane);
+            if (lane == 0)
+            {
+                v.satQueries[i] = out;
+            }
+            continue;
+        }
         const m3HullData* hullA = &v.hulls[v.hullIndex[shapeA]];
         const m3HullData* hullB = &v.hulls[v.hullIndex[shapeB]];
         m3Transform xfA;

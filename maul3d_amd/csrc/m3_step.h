@@ -709,22 +709,35 @@ M3_HD m3Manifold m3CollidePlaneCapsule(const m3View* v, int32_t shapeA, int32_t 
     return fresh;
 }
 
-M3_HD m3Manifold m3CollideCapsuleHull(const m3View* v, int32_t shapeA, int32_t hullShape,
-                                      int32_t capShape)
+// The capsule's segment in the hull shape's frame; returns the hull shape's
+// world transform.
+M3_HD m3Transform m3CapsuleInHull(const m3View* v, int32_t hullShape, int32_t capShape, m3Vec3* s1,
+                                  m3Vec3* s2)
 {
-    m3Manifold fresh = m3ZeroManifold();
-    int32_t hullBody = v->shapeBody[hullShape];
-    int32_t capBody = v->shapeBody[capShape];
     m3Transform xfH = m3ShapeXf(v, hullShape);
     m3Transform xfC = m3ShapeXf(v, capShape);
     m3Quat qRel = m3MulQuat(m3Conj(xfH.q), xfC.q);
     m3Vec3 dp = m3V3((m3real)(xfC.p.x - xfH.p.x), (m3real)(xfC.p.y - xfH.p.y),
                      (m3real)(xfC.p.z - xfH.p.z));
     m3Vec3 pRel = m3InvRotateVec3(xfH.q, dp);
-    m3Vec3 s1 = m3Add3(m3RotateVec3(qRel, v->geom[capShape].v), pRel);
-    m3Vec3 s2 = m3Add3(m3RotateVec3(qRel, v->geom[capShape].v2), pRel);
+    *s1 = m3Add3(m3RotateVec3(qRel, v->geom[capShape].v), pRel);
+    *s2 = m3Add3(m3RotateVec3(qRel, v->geom[capShape].v2), pRel);
+    return xfH;
+}
+
+// `sat` as for m3CollideSegmentHullQ (the queries of the segment against
+// the hull, evaluated ahead), or null.
+M3_HD m3Manifold m3CollideCapsuleHull(const m3View* v, int32_t shapeA, int32_t hullShape,
+                                      int32_t capShape, const m3SatQuery* sat)
+{
+    m3Manifold fresh = m3ZeroManifold();
+    int32_t hullBody = v->shapeBody[hullShape];
+    int32_t capBody = v->shapeBody[capShape];
+    m3Vec3 s1;
+    m3Vec3 s2;
+    m3Transform xfH = m3CapsuleInHull(v, hullShape, capShape, &s1, &s2);
     m3Manifold local =
-        m3CollideSegmentHull(&v->hulls[v->hullIndex[hullShape]], s1, s2, v->geom[capShape].s);
+        m3CollideSegmentHullQ(&v->hulls[v->hullIndex[hullShape]], s1, s2, v->geom[capShape].s, sat);
     if (local.pointCount > 0)
     {
         m3Vec3 nWorld = m3RotateVec3(xfH.q, local.normal);
@@ -1029,7 +1042,10 @@ M3_HD int32_t m3NarrowPair(const m3View* v, int32_t i, int32_t oldCount)
     {
         int32_t hullShape = typeA == M3_HULL ? shapeA : shapeB;
         int32_t capShape = hullShape == shapeA ? shapeB : shapeA;
-        fresh = m3CollideCapsuleHull(v, shapeA, hullShape, capShape);
+        const m3SatQuery* sat =
+            v->satQueries != NULL && m3WideSegmentHull(&v->hulls[v->hullIndex[hullShape]]) ? &v->satQueries[i]
+                                                                                           : NULL;
+        fresh = m3CollideCapsuleHull(v, shapeA, hullShape, capShape, sat);
     }
     else if (hullPair || capsulePair)
     {
