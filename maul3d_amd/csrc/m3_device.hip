@@ -64,6 +64,20 @@ struct m3Dev
     uint16_t* bodyMasks;
     int32_t* bodyOwner;
     uint32_t* barrier; // substep grid barrier: arrival count, generation
+    // Barrier-free passes of the single-world substep kernel: per body six
+    // versioned velocity words, per slot the constraint's ranks on its two
+    // bodies (see SubstepCells).
+    uint64_t* flowRec;
+    uint32_t* flowRank;
+    m3WarmAdd* warmAdd; // per slot and side: the next warm start's addends (barrier-free form)
+    // Cells of the barrier-free form: per mover its cell and index there,
+    // per cell its movers and counts, per slot the solving cell.
+    uint8_t* cellOf;
+    uint16_t* cellLocal;
+    int32_t* cellBodies;
+    int32_t* cellCount;
+    int32_t* cellConsCount;
+    uint8_t* slotOwner;
     m3Tape* tape;      // intermediate-value record of this step (adjoint replays), or null
     // Pairs binned by collision routine: 8 counts, then one bin of
     // pairCapacity entries per routine.
@@ -126,6 +140,9 @@ struct m3gpuWorld
     int coopMax;     // co-resident limit of the cooperative launch
     int coopEnvFixed; // MAUL3D_AMD_SUBSTEP_BLOCKS was set at creation
     int tailLimit;   // tuned budget of the workgroup-0 tail (0: M3_TAIL_CONSTRAINTS)
+    int flow;        // barrier-free solve and relax passes (MAUL3D_AMD_FLOW=0 turns them off)
+    int cellsX;      // cells of the barrier-free passes: slabs along x
+    int cellsZ;      // and cells per slab along z
     int coopThreads; // threads per block of the substep kernel
     int wideHulls;   // some hull is larger than a box (wide hull pairs possible)
     m3HitRec* hitRecBuf; // per pair; in the view only while some shape has hit events
@@ -153,6 +170,13 @@ typedef const __attribute__((address_space(4))) m3Dev m3ConstDev;
 // ---------------------------------------------------------------------------
 // Small helpers
 // ---------------------------------------------------------------------------
+
+// Rank of a constraint side that is not an awake dynamic body (barrier-free
+// substep passes).
+#define M3_FLOW_NONE 0xFFu
+// Solve and relax passes of one step the versions cover (two per substep;
+// more substeps take the colored path).
+#define M3_FLOW_PASSES 16
 
 static inline unsigned Blocks(int64_t n, int threads)
 {
@@ -271,6 +295,7 @@ __device__ static inline void StepBegin(const m3Dev& d)
     sc->pointCount = 0;
     sc->jointActive = 0;
     sc->satCount = 0;
+    sc->flowVeto = 0;
     if (d.narrowBins != nullptr)
     {
         for (int c = 0; c < 8; ++c)
@@ -1682,7 +1707,7 @@ __device__ static inline void RegisterSlot(const m3View& v, int32_t body, int32_
 // dynamic body or a static/kinematic one, and touching dynamic bodies share
 // an island, which the wake pass woke whole.  So coloring gave it at most
 // one slot per color.
-__global__ void k_prepare(const m3Dev* __restrict__ ds, float hertz, float zeta, float h)
+__global__ void k_prepare(const m3Dev* __restrict__ ds, float hertz, float zeta, float h, int flowPrep)
 {
     M3_WORLD;
     int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1709,6 +1734,31 @@ __global__ void k_prepare(const m3Dev* __restrict__ ds, float hertz, float zeta,
     if (con->restitution > 0.0f)
     {
         v.sc->anyRestitution = 1;
+    }
+    if (flowPrep)
+    {
+        // The barrier-free passes track awake dynamic bodies only, and the
+        // rolling row writes a non-dynamic side's angular velocity.
+        const bool awakeA = v.consBodies[2 * c] >= 0;
+        const bool awakeB = v.consBodies[2 * c + 1] >= 0;
+        if ((con->dynA != 0) != awakeA || (con->dynB != 0) != awakeB ||
+            (con->rollingResistance > 0.0f && (!con->dynA || !con->dynB)))
+        {
+            v.sc->flowVeto = 1;
+        }
+        m3Vec3 rA[M3_MANIFOLD_MAX_POINTS];
+        m3Vec3 rB[M3_MANIFOLD_MAX_POINTS];
+        m3real imp[M3_MANIFOLD_MAX_POINTS];
+        for (int32_t q = 0; q < M3_MANIFOLD_MAX_POINTS; ++q)
+        {
+            rA[q] = con->points[q].rA;
+            rB[q] = con->points[q].rB;
+            imp[q] = con->points[q].normalImpulse;
+        }
+        m3WarmAddends(con->pointCount, con->normal, rA, rB, imp, con->invMassA, con->invMassB, &con->invIA,
+                      &con->invIB, con->t1, con->t2, con->frictionImpulse1, con->frictionImpulse2,
+                      con->originA, con->originB, con->twistImpulse, con->rollingImpulse, con->dynA,
+                      con->dynB, &d.warmAdd[2 * k], &d.warmAdd[2 * k + 1]);
     }
 }
 
@@ -1777,7 +1827,7 @@ __device__ static inline void ColorBarrier()
 
 template <bool kLds>
 __global__ void __launch_bounds__(M3_COLOR_THREADS)
-    k_color(const m3Dev* __restrict__ ds)
+    k_color(const m3Dev* __restrict__ ds, int flowPrep)
 {
     M3_WORLD;
     uint16_t* gMasks = d.bodyMasks;
@@ -1981,6 +2031,18 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
             int32_t slot = starts[color] + waveBase[color][wave] + rank;
             v.colorLists[slot] = c;
             v.slotColors[slot] = (uint8_t)color;
+            if (flowPrep)
+            {
+                // The constraint's rank on each awake dynamic body: the
+                // colors below its own in which the body has a constraint
+                // (the masks are final here).
+                const uint32_t lower = (1u << color) - 1u;
+                const int32_t A = v.consBodies[2 * c];
+                const int32_t B = v.consBodies[2 * c + 1];
+                const uint32_t rA = A >= 0 ? (uint32_t)__popc((uint32_t)masks[A] & lower) : M3_FLOW_NONE;
+                const uint32_t rB = B >= 0 ? (uint32_t)__popc((uint32_t)masks[B] & lower) : M3_FLOW_NONE;
+                d.flowRank[slot] = rA | (rB << 8);
+            }
         }
         __syncthreads();
     }
@@ -2038,6 +2100,7 @@ struct m3SubstepParams
 {
     int32_t substeps;
     int32_t tailLimit; // constraint budget of the trailing colors solved in workgroup 0
+    int32_t flow;      // barrier-free passes allowed (single-world kernel)
     float h;
     float invH;
     float push;
@@ -2186,7 +2249,7 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
     }
 }
 
-#define M3_SUBSTEP_THREADS 256
+#define M3_SUBSTEP_THREADS 320 // M3_CELL_THREADS
 #ifndef M3_BLOCK_GATHER
 #define M3_BLOCK_GATHER 0
 #endif
@@ -2544,8 +2607,818 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
     }
 }
 
-// One world over a cooperative grid.
-__global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
+// ---------------------------------------------------------------------------
+// Barrier-free solve and relax passes over cells (single world)
+//
+// Inside a pass, a contact constraint only needs each of its dynamic bodies
+// to have received exactly the constraints of lower colors on that body:
+// the colored pass gives each body its constraints in color order, and
+// solves on different bodies commute.  So instead of a grid barrier per
+// color, a constraint waits for its two bodies alone: a body's version is
+// pass << 8 | the number of its constraints solved in the pass, and a
+// constraint of rank r on a body (the number of colors below its own in
+// which the body has a constraint, from the coloring) waits for version
+// pass << 8 | r and leaves r + 1.
+//
+// Where a body's velocity travels decides the cost of that wait, so the
+// island is cut into cells, one workgroup each (k_cells: a quantile grid
+// over the movers' x and z).  A body lives in its cell's LDS; a constraint
+// belongs to the cell of one of its dynamic bodies (k_cell_owner).  When
+// the previous constraint on a body was solved in the same cell, the wait
+// is on an LDS version.  Otherwise the body's velocity comes from its six
+// global words: each holds one float and the version, written and read
+// with agent-scope atomics, so a reader on any XCD sees each word whole,
+// and the version in the word says which write it holds (no fence between
+// data and flag).  One wavefront of each workgroup (the mail wavefront)
+// polls those words for the cell's pending imports and copies each into an
+// LDS mailbox, so solver wavefronts only ever wait on LDS.  Every solve
+// writes the global words of its dynamic bodies, so after a pass they hold
+// the final velocities.
+//
+// Every participant takes its cell's constraints in slot order, every
+// mail lane polls all its imports each round, and all workgroups are
+// co-resident (cooperative launch), so the lowest unsolved slot is always
+// ready and the passes end.  Waits are bounded all the same: one that
+// gives up marks the world faulted, and every wait ends once it sees the
+// fault.  The grid still meets between phases (integrate velocities and
+// warm start, the solve pass, integrate positions, the relax pass), where
+// positions change.
+// ---------------------------------------------------------------------------
+// A wait gives up after this much wall time (s_memrealtime runs at
+// 100 MHz): far beyond any step, so only a broken wait reaches it.
+#define M3_FLOW_TICKS 200000000ull
+#define M3_CELL_SOLVERS 256 // solver threads of a cell workgroup
+#define M3_CELL_MAIL 64     // the mail wavefront
+#define M3_CELL_THREADS (M3_CELL_SOLVERS + M3_CELL_MAIL)
+
+__device__ static inline uint64_t FlowWord(float x, uint32_t ver)
+{
+    return ((uint64_t)ver << 32) | (uint64_t)__float_as_uint(x);
+}
+
+__device__ static inline void FlowStore(uint64_t* rec, int32_t b, m3Vec3 lin, m3Vec3 ang, uint32_t ver)
+{
+    uint64_t* r = rec + 6 * (int64_t)b;
+    __hip_atomic_store(&r[0], FlowWord(lin.x, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r[1], FlowWord(lin.y, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r[2], FlowWord(lin.z, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r[3], FlowWord(ang.x, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r[4], FlowWord(ang.y, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&r[5], FlowWord(ang.z, ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Reads a body's six words.  Returns the version all six carry, or
+// 0xFFFFFFFF when they disagree (a write in progress).
+__device__ static inline uint32_t FlowLoad(const uint64_t* rec, int32_t b, float* x)
+{
+    const uint64_t* r = rec + 6 * (int64_t)b;
+    uint64_t w[6];
+    M3_UNROLL
+    for (int j = 0; j < 6; ++j)
+    {
+        w[j] = __hip_atomic_load(&r[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint32_t ver = (uint32_t)(w[5] >> 32);
+    bool same = true;
+    M3_UNROLL
+    for (int j = 0; j < 6; ++j)
+    {
+        same = same && (uint32_t)(w[j] >> 32) == ver;
+        x[j] = __uint_as_float((uint32_t)w[j]);
+    }
+    return same ? ver : 0xFFFFFFFFu;
+}
+
+// Quantile cells over the movers: gx slabs of equal mover count along x by
+// gz of equal count along z.  Any partition gives the same results (the
+// per-body operation order does not depend on it); a compact one keeps
+// most consecutive constraints on a body in one cell.
+#define M3_CELL_MAX 256
+#define M3_CELL_BODIES 1024 // movers homed in one cell
+#define M3_CELL_CONS 1024   // constraints owned by one cell
+#define M3_CELL_BINS 1024   // histogram bins per axis (one per thread of k_cells)
+#define M3_CELL_GX_MAX 16
+#define M3_CELL_CACHE 12288 // movers whose x, z k_cells keeps in LDS
+
+__device__ static inline int32_t CellBin(float x, float lo, float scale)
+{
+    float f = (x - lo) * scale;
+    if (!(f >= 0.0f))
+    {
+        f = 0.0f;
+    }
+    if (f > (float)(M3_CELL_BINS - 1))
+    {
+        f = (float)(M3_CELL_BINS - 1);
+    }
+    return (int32_t)f;
+}
+
+__device__ static inline float WaveMin(float x)
+{
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        x = fminf(x, __shfl_xor(x, o));
+    }
+    return x;
+}
+
+__device__ static inline float WaveMax(float x)
+{
+    for (int o = 32; o > 0; o >>= 1)
+    {
+        x = fmaxf(x, __shfl_xor(x, o));
+    }
+    return x;
+}
+
+// Exclusive prefix sum over one value per thread of a 1024-thread block.
+__device__ static inline int32_t BlockExclusiveSum(int32_t x, int32_t* waveTotals)
+{
+    const int32_t lane = threadIdx.x & 63;
+    const int32_t wave = threadIdx.x >> 6;
+    int32_t inc = x;
+    for (int o = 1; o < 64; o <<= 1)
+    {
+        const int32_t y = __shfl_up(inc, o);
+        if (lane >= o)
+        {
+            inc += y;
+        }
+    }
+    if (lane == 63)
+    {
+        waveTotals[wave] = inc;
+    }
+    __syncthreads();
+    int32_t before = 0;
+    for (int32_t w = 0; w < wave; ++w)
+    {
+        before += waveTotals[w];
+    }
+    __syncthreads();
+    return before + inc - x;
+}
+
+__global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict__ ds, int32_t gx, int32_t gz)
+{
+    M3_WORLD;
+    __shared__ float red[4][M3_CELL_BINS / 64];
+    __shared__ float box[4];
+    __shared__ int32_t hx[M3_CELL_BINS];
+    __shared__ int32_t hz[M3_CELL_BINS];
+    __shared__ uint8_t xOf[M3_CELL_BINS];
+    __shared__ uint8_t zOf[M3_CELL_BINS];
+    __shared__ int32_t totals[M3_CELL_BINS / 64];
+    __shared__ int32_t cnt[M3_CELL_MAX];
+    __shared__ float2 pos[M3_CELL_CACHE];
+    const int32_t t = threadIdx.x;
+    const int32_t cells = gx * gz;
+    const int32_t n = v.sc->stall != 0 ? 0 : v.sc->moverCount;
+    if (t < M3_CELL_MAX)
+    {
+        cnt[t] = 0;
+        d.cellConsCount[t] = 0;
+    }
+    hx[t] = 0;
+    hz[t] = 0;
+    // Bodies that are not movers this step belong to no cell.
+    for (int32_t b = t; b < v.maxBody; b += blockDim.x)
+    {
+        d.cellOf[b] = 0xFF;
+    }
+    auto at = [&](int32_t m) -> float2 {
+        if (m < M3_CELL_CACHE)
+        {
+            return pos[m];
+        }
+        const m3Pos3 p = v.xf[v.movers[m]].p;
+        return make_float2((float)p.x, (float)p.z);
+    };
+    float x0 = INFINITY, x1 = -INFINITY, z0 = INFINITY, z1 = -INFINITY;
+    for (int32_t m = t; m < n; m += blockDim.x)
+    {
+        const m3Pos3 p = v.xf[v.movers[m]].p;
+        const float2 q = make_float2((float)p.x, (float)p.z);
+        if (m < M3_CELL_CACHE)
+        {
+            pos[m] = q;
+        }
+        x0 = fminf(x0, q.x);
+        x1 = fmaxf(x1, q.x);
+        z0 = fminf(z0, q.y);
+        z1 = fmaxf(z1, q.y);
+    }
+    x0 = WaveMin(x0);
+    x1 = WaveMax(x1);
+    z0 = WaveMin(z0);
+    z1 = WaveMax(z1);
+    if ((t & 63) == 0)
+    {
+        red[0][t >> 6] = x0;
+        red[1][t >> 6] = x1;
+        red[2][t >> 6] = z0;
+        red[3][t >> 6] = z1;
+    }
+    __syncthreads();
+    if (t == 0)
+    {
+        float a = red[0][0], b = red[1][0], c = red[2][0], e = red[3][0];
+        for (int32_t w = 1; w < M3_CELL_BINS / 64; ++w)
+        {
+            a = fminf(a, red[0][w]);
+            b = fmaxf(b, red[1][w]);
+            c = fminf(c, red[2][w]);
+            e = fmaxf(e, red[3][w]);
+        }
+        box[0] = a;
+        box[1] = b > a ? (float)M3_CELL_BINS / (b - a) : 0.0f;
+        box[2] = c;
+        box[3] = e > c ? (float)M3_CELL_BINS / (e - c) : 0.0f;
+    }
+    __syncthreads();
+    for (int32_t m = t; m < n; m += blockDim.x)
+    {
+        const float2 q = at(m);
+        atomicAdd(&hx[CellBin(q.x, box[0], box[1])], 1);
+        atomicAdd(&hz[CellBin(q.y, box[2], box[3])], 1);
+    }
+    __syncthreads();
+    // Bin t's slab (row) is the quantile of its middle mover.
+    const int32_t ex = BlockExclusiveSum(hx[t], totals);
+    const int32_t ez = BlockExclusiveSum(hz[t], totals);
+    const int64_t nn = n > 0 ? n : 1;
+    const int32_t sx = (int32_t)(((int64_t)ex + hx[t] / 2) * gx / nn);
+    const int32_t sz = (int32_t)(((int64_t)ez + hz[t] / 2) * gz / nn);
+    xOf[t] = (uint8_t)(sx < gx ? sx : gx - 1);
+    zOf[t] = (uint8_t)(sz < gz ? sz : gz - 1);
+    __syncthreads();
+    for (int32_t m = t; m < n; m += blockDim.x)
+    {
+        const float2 q = at(m);
+        const int32_t b = v.movers[m];
+        const int32_t cell = xOf[CellBin(q.x, box[0], box[1])] * gz + zOf[CellBin(q.y, box[2], box[3])];
+        const int32_t local = atomicAdd(&cnt[cell], 1);
+        d.cellOf[b] = (uint8_t)cell;
+        d.cellLocal[b] = (uint16_t)(local < M3_CELL_BODIES ? local : 0);
+        if (local < M3_CELL_BODIES)
+        {
+            d.cellBodies[cell * M3_CELL_BODIES + local] = b;
+        }
+    }
+    __syncthreads();
+    if (t < cells)
+    {
+        d.cellCount[t] = cnt[t] < M3_CELL_BODIES ? cnt[t] : M3_CELL_BODIES;
+        if (cnt[t] > M3_CELL_BODIES)
+        {
+            v.sc->flowVeto = 1;
+        }
+    }
+}
+
+// The cell that solves each slot: the cell of its body A when A is dynamic,
+// else of body B.  A dynamic side outside every cell, or a cell over its
+// capacity, sends the step to the colored form.
+__global__ void k_cell_owner(const m3Dev* __restrict__ ds, int32_t cells)
+{
+    M3_WORLD;
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0 || k >= v.sc->colorStarts[M3_GRAPH_COLORS])
+    {
+        return;
+    }
+    const m3Constraint* c = &v.cons[k];
+    const int32_t body = c->dynA ? c->bodyA : c->bodyB;
+    const uint8_t owner = (c->dynA || c->dynB) ? d.cellOf[body] : 0;
+    d.slotOwner[k] = owner;
+    const bool outside = (c->dynA && d.cellOf[c->bodyA] >= cells) || (c->dynB && d.cellOf[c->bodyB] >= cells);
+    if (outside || owner >= cells || atomicAdd(&d.cellConsCount[owner], 1) >= M3_CELL_CONS)
+    {
+        v.sc->flowVeto = 1;
+    }
+}
+
+// Per-cell LDS of the substep kernel.
+struct m3CellLds
+{
+    float state[M3_CELL_BODIES][6]; // velocities of the cell's bodies
+    uint32_t ver[M3_CELL_BODIES];
+    float mail[2 * M3_CELL_CONS][6]; // imported velocities
+    uint32_t mailVer[2 * M3_CELL_CONS];
+    int32_t mailBody[2 * M3_CELL_CONS];
+    int32_t slot[M3_CELL_CONS]; // the cell's constraints, ascending
+    uint16_t loc[M3_CELL_CONS][2]; // per side: where it reads (LDS body or mailbox index)
+    uint16_t own[M3_CELL_CONS][2]; // per side: the LDS body it writes (home sides)
+    uint8_t flags[M3_CELL_CONS];   // per side: bit 0 / 2 read LDS body, bit 1 / 3 write LDS body
+    int32_t count;
+    int32_t imports;
+};
+
+// Sorts the cell's slots ascending (bitonic, in LDS; count <= M3_CELL_CONS).
+__device__ static inline void CellSort(m3CellLds& L, int32_t count)
+{
+    int32_t n = 1;
+    while (n < count)
+    {
+        n <<= 1;
+    }
+    for (int32_t i = count + (int32_t)threadIdx.x; i < n; i += blockDim.x)
+    {
+        L.slot[i] = 0x7FFFFFFF;
+    }
+    __syncthreads();
+    for (int32_t size = 2; size <= n; size <<= 1)
+    {
+        for (int32_t stride = size >> 1; stride > 0; stride >>= 1)
+        {
+            for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
+            {
+                const int32_t j = i ^ stride;
+                if (j > i)
+                {
+                    const int32_t a = L.slot[i];
+                    const int32_t b = L.slot[j];
+                    const bool up = (i & size) == 0;
+                    if ((a > b) == up)
+                    {
+                        L.slot[i] = b;
+                        L.slot[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Warm start of awake dynamic body b from its constraints' addends, in
+// color order, four constraints' loads at a time.
+__device__ static inline void FlowWarmStart(const m3View& v, const m3WarmAdd* warm, int32_t b,
+                                            m3Vec3* lin, m3Vec3* ang)
+{
+    uint32_t colors = v.slotMask[b] & 0xFFFFu;
+    const int32_t* row = v.slotTable + (int64_t)b * M3_GRAPH_COLORS;
+    while (colors != 0u)
+    {
+        int32_t e[4];
+        int32_t n = 0;
+        M3_UNROLL
+        for (int q = 0; q < 4; ++q)
+        {
+            if (colors != 0u)
+            {
+                e[q] = row[__builtin_ctz(colors)];
+                colors &= colors - 1u;
+                n = q + 1;
+            }
+            else
+            {
+                e[q] = -1;
+            }
+        }
+        m3WarmAdd a[4];
+        M3_UNROLL
+        for (int q = 0; q < 4; ++q)
+        {
+            if (q < n)
+            {
+                a[q] = warm[e[q]]; // slot << 1 | side indexes the pair of records
+            }
+        }
+        M3_UNROLL
+        for (int q = 0; q < 4; ++q)
+        {
+            if (q < n)
+            {
+                m3WarmApply(&a[q], e[q] & 1, lin, ang);
+            }
+        }
+    }
+}
+
+__device__ static inline void CellPut(m3CellLds& L, int32_t l, m3Vec3 lin, m3Vec3 ang, uint32_t ver)
+{
+    L.state[l][0] = lin.x;
+    L.state[l][1] = lin.y;
+    L.state[l][2] = lin.z;
+    L.state[l][3] = ang.x;
+    L.state[l][4] = ang.y;
+    L.state[l][5] = ang.z;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __hip_atomic_store(&L.ver[l], ver, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Waits are bounded in wall time: every 256 rounds a waiting participant
+// checks the clock (from the first check of this wait) and the world's
+// fault flag; one that gives up marks the world faulted.  `since` is 0 at
+// the start of each wait.
+__device__ static inline bool FlowGiveUp(uint32_t* spins, uint64_t* since, int32_t* fault)
+{
+    if ((++*spins & 255u) != 0u)
+    {
+        return false;
+    }
+    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+    if (*since == 0ull)
+    {
+        *since = now;
+    }
+    if (now - *since > M3_FLOW_TICKS ||
+        __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    {
+        __hip_atomic_store(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+    return false;
+}
+
+// One pass (kMode 1 solve, 2 relax) of one cell.  Solver threads take the
+// cell's constraints in slot order; the mail wavefront fills the
+// mailboxes.  A relax pass that precedes another substep (`addends`) also
+// leaves each constraint's warm-start addends for that substep.
+template <int kMode>
+__device__ static inline void CellPass(const m3View& v, const m3SubstepParams& sp, m3CellLds& L,
+                                       uint64_t* rec, const uint32_t* rank, m3WarmAdd* warm,
+                                       bool addends, uint32_t pass)
+{
+    const uint32_t base = pass << 8;
+    int32_t* fault = &v.sc->fault;
+    const int32_t count = L.count;
+    const int32_t t = threadIdx.x;
+    if (t >= M3_CELL_SOLVERS)
+    {
+        // Mail: each lane polls its imports (import q of item i waits for
+        // the version the item's rank names) until all are in.
+        const int32_t imports = L.imports;
+        uint32_t spins = 0;
+        uint64_t since = 0;
+        uint64_t left = 0; // bit j: import t - M3_CELL_SOLVERS + 64 j still pending
+        for (int32_t q = t - M3_CELL_SOLVERS, j = 0; q < imports; q += M3_CELL_MAIL, ++j)
+        {
+            left |= 1ull << j;
+        }
+        while (__any(left != 0ull))
+        {
+            bool got = false;
+            for (uint64_t m = left; m != 0ull;)
+            {
+                int32_t js[4];
+                M3_UNROLL
+                for (int g = 0; g < 4; ++g)
+                {
+                    js[g] = -1;
+                    if (m != 0ull)
+                    {
+                        js[g] = (int32_t)__builtin_ctzll(m);
+                        m &= m - 1ull;
+                    }
+                }
+                float x[4][6];
+                uint32_t got4[4];
+                M3_UNROLL
+                for (int g = 0; g < 4; ++g)
+                {
+                    got4[g] = 0xFFFFFFFFu;
+                    if (js[g] >= 0)
+                    {
+                        got4[g] = FlowLoad(rec, L.mailBody[t - M3_CELL_SOLVERS + M3_CELL_MAIL * js[g]], x[g]);
+                    }
+                }
+                M3_UNROLL
+                for (int g = 0; g < 4; ++g)
+                {
+                    if (js[g] < 0)
+                    {
+                        continue;
+                    }
+                    const int32_t q = t - M3_CELL_SOLVERS + M3_CELL_MAIL * js[g];
+                    const uint32_t want = L.mailVer[q] & 0x7FFFFFFFu;
+                    if (got4[g] == want)
+                    {
+                        for (int c = 0; c < 6; ++c)
+                        {
+                            L.mail[q][c] = x[g][c];
+                        }
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                        __hip_atomic_store(&L.mailVer[q], want, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        left &= ~(1ull << js[g]);
+                        got = true;
+                    }
+                }
+            }
+            if (!got)
+            {
+                if (left != 0ull && FlowGiveUp(&spins, &since, fault))
+                {
+                    left = 0ull;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            else
+            {
+                spins = 0;
+                since = 0;
+            }
+        }
+        return;
+    }
+    int32_t i = t;
+    bool have = i < count;
+    int32_t k = 0;
+    m3ContactIn in;
+    m3ContactBias cb;
+    m3Vec3 sLinA, sAngA, sLinB, sAngB;
+    uint32_t wantA = 0, wantB = 0;
+    uint32_t fl = 0;
+    int32_t locA = 0, locB = 0, ownA = 0, ownB = 0;
+    auto take = [&]() {
+        k = L.slot[i];
+        m3LoadContact(&v.cons[k], kMode == 1, &in);
+        const uint32_t rk = rank[k];
+        wantA = base | (rk & 0xFFu);
+        wantB = base | ((rk >> 8) & 0xFFu);
+        fl = L.flags[i];
+        locA = L.loc[i][0];
+        locB = L.loc[i][1];
+        ownA = L.own[i][0];
+        ownB = L.own[i][1];
+        // Non-dynamic sides keep their velocities for the whole pass.
+        sLinA = v.lv[in.a];
+        sAngA = v.av[in.a];
+        sLinB = v.lv[in.b];
+        sAngB = v.av[in.b];
+        M3_UNROLL
+        for (int32_t q = 0; q < M3_MANIFOLD_MAX_POINTS; ++q)
+        {
+            cb.bias[q] = 0.0f;
+            cb.massScale[q] = 1.0f;
+            cb.impulseScale[q] = 0.0f;
+        }
+        const int32_t slots =
+            in.pointCount < M3_MANIFOLD_MAX_POINTS ? in.pointCount : M3_MANIFOLD_MAX_POINTS;
+        m3ContactBiasTerms(&in, v.deltaPos[in.a], v.deltaRot[in.a], v.deltaPos[in.b], v.deltaRot[in.b],
+                           sp.invH, sp.push, kMode == 1, slots, &cb);
+    };
+    if (have)
+    {
+        take();
+    }
+    uint32_t spins = 0;
+    uint64_t since = 0;
+    while (__any(have))
+    {
+        bool progressed = false;
+        if (have)
+        {
+            // A side reads its cell body (bit 0 / 2) or its mailbox.
+            bool ready = true;
+            m3PairVelocity s;
+            s.vA = sLinA;
+            s.wA = sAngA;
+            s.vB = sLinB;
+            s.wB = sAngB;
+            for (int side = 0; side < 2; ++side)
+            {
+                const bool dyn = side == 0 ? in.dynA != 0 : in.dynB != 0;
+                if (!dyn || !ready)
+                {
+                    continue;
+                }
+                const uint32_t want = side == 0 ? wantA : wantB;
+                const int32_t l = side == 0 ? locA : locB;
+                const bool body = (fl >> (2 * side)) & 1u;
+                const uint32_t seen =
+                    __hip_atomic_load(body ? &L.ver[l] : &L.mailVer[l], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (seen != want)
+                {
+                    ready = false;
+                    continue;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                const float* x = body ? L.state[l] : L.mail[l];
+                const m3Vec3 lin = m3V3(x[0], x[1], x[2]);
+                const m3Vec3 ang = m3V3(x[3], x[4], x[5]);
+                if (side == 0)
+                {
+                    s.vA = lin;
+                    s.wA = ang;
+                }
+                else
+                {
+                    s.vB = lin;
+                    s.wB = ang;
+                }
+            }
+            if (ready)
+            {
+                const int32_t slots = m3ContactSlots(in.pointCount);
+                m3ContactAccum acc;
+                m3ContactImpulses(&v.cons[k], &in, &cb, kMode == 1, slots, &s, &acc);
+                if (in.dynA)
+                {
+                    if (fl & 2u)
+                    {
+                        CellPut(L, ownA, s.vA, s.wA, wantA + 1u);
+                    }
+                    FlowStore(rec, in.a, s.vA, s.wA, wantA + 1u);
+                }
+                if (in.dynB)
+                {
+                    if (fl & 8u)
+                    {
+                        CellPut(L, ownB, s.vB, s.wB, wantB + 1u);
+                    }
+                    FlowStore(rec, in.b, s.vB, s.wB, wantB + 1u);
+                }
+                if (kMode == 2 && addends)
+                {
+                    m3WarmAddends(in.pointCount, in.normal, in.prA, in.prB, acc.normal, in.invMassA,
+                                  in.invMassB, &in.invIA, &in.invIB, in.t1, in.t2, acc.f1, acc.f2,
+                                  in.originA, in.originB, acc.twist, v.cons[k].rollingImpulse, in.dynA,
+                                  in.dynB, &warm[2 * k], &warm[2 * k + 1]);
+                }
+                progressed = true;
+                spins = 0;
+                since = 0;
+                i += M3_CELL_SOLVERS;
+                have = i < count;
+                if (have)
+                {
+                    take();
+                }
+            }
+            else if (FlowGiveUp(&spins, &since, fault))
+            {
+                have = false;
+            }
+        }
+        if (!__any(progressed))
+        {
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+}
+
+// Cell prologue: the cell's constraints in slot order, each side's place
+// (cell body or mailbox), and the imports with the versions they wait for
+// (bit 31 set: not yet in).  A side reads its cell body when the body is
+// homed here and the body's previous operation in the pass (the previous
+// constraint in color order, or the phase operation for rank 0) also ran
+// here; every other dynamic side is an import.
+__device__ static inline void CellSetup(const m3View& v, const m3Dev& d, m3CellLds& L, int32_t cell)
+{
+    const int32_t t = threadIdx.x;
+    const int32_t begin = v.sc->colorStarts[0];
+    const int32_t end = v.sc->colorStarts[M3_GRAPH_COLORS];
+    if (t == 0)
+    {
+        L.count = 0;
+        L.imports = 0;
+    }
+    __syncthreads();
+    for (int32_t k = begin + t; k < end; k += blockDim.x)
+    {
+        if (d.slotOwner[k] == (uint8_t)cell)
+        {
+            const int32_t at = atomicAdd(&L.count, 1);
+            if (at < M3_CELL_CONS)
+            {
+                L.slot[at] = k;
+            }
+        }
+    }
+    __syncthreads();
+    const int32_t count = L.count < M3_CELL_CONS ? L.count : M3_CELL_CONS;
+    CellSort(L, count);
+    for (int32_t i = t; i < count; i += blockDim.x)
+    {
+        const int32_t k = L.slot[i];
+        const m3Constraint* c = &v.cons[k];
+        const uint32_t lower = (1u << v.slotColors[k]) - 1u;
+        uint32_t fl = 0;
+        for (int side = 0; side < 2; ++side)
+        {
+            const bool dyn = side == 0 ? c->dynA != 0 : c->dynB != 0;
+            if (!dyn)
+            {
+                L.loc[i][side] = 0;
+                L.own[i][side] = 0;
+                continue;
+            }
+            const int32_t b = side == 0 ? c->bodyA : c->bodyB;
+            const bool home = d.cellOf[b] == (uint8_t)cell;
+            const uint32_t below = v.slotMask[b] & lower;
+            bool prevHere = below == 0u;
+            if (!prevHere)
+            {
+                const int32_t prevColor = 31 - __clz(below);
+                const int32_t prev = v.slotTable[(int64_t)b * M3_GRAPH_COLORS + prevColor] >> 1;
+                prevHere = d.slotOwner[prev] == (uint8_t)cell;
+            }
+            L.own[i][side] = home ? d.cellLocal[b] : 0;
+            if (home)
+            {
+                fl |= 2u << (2 * side);
+            }
+            if (home && prevHere)
+            {
+                fl |= 1u << (2 * side);
+                L.loc[i][side] = d.cellLocal[b];
+            }
+            else
+            {
+                const int32_t q = atomicAdd(&L.imports, 1);
+                L.loc[i][side] = (uint16_t)q;
+                L.mailBody[q] = b;
+                // The wanted version's pass is filled in by the pass.
+                L.mailVer[q] = 0x80000000u | (d.flowRank[k] >> (8 * side) & 0xFFu);
+            }
+        }
+        L.flags[i] = (uint8_t)fl;
+    }
+    __syncthreads();
+}
+
+// Arms the mailboxes for a pass: each waits for version pass << 8 | rank.
+__device__ static inline void CellArm(m3CellLds& L, uint32_t pass)
+{
+    for (int32_t q = threadIdx.x; q < L.imports; q += blockDim.x)
+    {
+        L.mailVer[q] = 0x80000000u | (pass << 8) | (L.mailVer[q] & 0xFFu);
+    }
+}
+
+// The substep loop of src/solver.c:4085-4203 over cells.  Body phases run
+// in each body's home cell.
+__device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, const m3SubstepParams& sp,
+                                           m3GridSync sync, m3CellLds& L)
+{
+    const int32_t cell = blockIdx.x;
+    const int32_t t = threadIdx.x;
+    const int32_t nHome = d.cellCount[cell];
+    const int32_t* home = d.cellBodies + cell * M3_CELL_BODIES;
+    uint64_t* rec = d.flowRec;
+    CellSetup(v, d, L, cell);
+    for (int32_t sub = 0; sub < sp.substeps; ++sub)
+    {
+        const uint32_t solvePass = 2u * (uint32_t)sub;
+        for (int32_t j = t; j < nHome; j += blockDim.x)
+        {
+            const int32_t b = home[j];
+            if (sub > 0)
+            {
+                float x[6];
+                FlowLoad(rec, b, x);
+                v.lv[b] = m3V3(x[0], x[1], x[2]);
+                v.av[b] = m3V3(x[3], x[4], x[5]);
+            }
+            m3IntegrateVelocity(&v, b, sp.h, sp.gravity, sp.maxLinear, sp.maxAngular);
+            m3Vec3 lin = v.lv[b];
+            m3Vec3 ang = v.av[b];
+            if (v.type[b] == M3_DYNAMIC)
+            {
+                FlowWarmStart(v, d.warmAdd, b, &lin, &ang);
+            }
+            v.lv[b] = lin;
+            v.av[b] = ang;
+            CellPut(L, j, lin, ang, solvePass << 8);
+            FlowStore(rec, b, lin, ang, solvePass << 8);
+        }
+        CellArm(L, solvePass);
+        sync();
+        CellPass<1>(v, sp, L, rec, d.flowRank, d.warmAdd, false, solvePass);
+        sync();
+        for (int32_t j = t; j < nHome; j += blockDim.x)
+        {
+            const int32_t b = home[j];
+            float x[6];
+            FlowLoad(rec, b, x);
+            v.lv[b] = m3V3(x[0], x[1], x[2]);
+            v.av[b] = m3V3(x[3], x[4], x[5]);
+            m3IntegratePosition(&v, b, sp.h);
+            CellPut(L, j, v.lv[b], v.av[b], (solvePass + 1u) << 8);
+            FlowStore(rec, b, v.lv[b], v.av[b], (solvePass + 1u) << 8);
+        }
+        CellArm(L, solvePass + 1u);
+        sync();
+        CellPass<2>(v, sp, L, rec, d.flowRank, d.warmAdd, sub + 1 < sp.substeps, solvePass + 1u);
+        sync();
+    }
+    for (int32_t j = t; j < nHome; j += blockDim.x)
+    {
+        const int32_t b = home[j];
+        float x[6];
+        FlowLoad(rec, b, x);
+        v.lv[b] = m3V3(x[0], x[1], x[2]);
+        v.av[b] = m3V3(x[3], x[4], x[5]);
+    }
+}
+
+// One world over a cooperative grid: over cells when every constraint of
+// the step fits them (see SubstepCells), else color by color.
+__global__ void __launch_bounds__(M3_CELL_THREADS)
     k_substeps(const m3Dev* __restrict__ ds, m3SubstepParams sp)
 {
     M3_WORLD;
@@ -2553,8 +3426,23 @@ __global__ void __launch_bounds__(M3_SUBSTEP_THREADS)
     {
         return;
     }
-    SubstepLoop<m3GridSync, true>(v, sp, blockIdx.x * blockDim.x + threadIdx.x,
-                                  gridDim.x * blockDim.x, m3GridSync{d.barrier, &v.sc->fault});
+    __shared__ m3CellLds cellLds;
+    const int32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t gsz = gridDim.x * blockDim.x;
+    const m3GridSync sync{d.barrier, &v.sc->fault};
+    // Every input of the choice was written by earlier launches, so all
+    // participants take the same branch.
+    const bool cells = sp.flow != 0 && d.flowRec != nullptr && 2 * sp.substeps <= M3_FLOW_PASSES &&
+                       v.sc->jointActive == 0 && v.sc->flowVeto == 0 &&
+                       v.sc->colorStarts[M3_GRAPH_COLORS] == v.sc->colorStarts[M3_GRAPH_COLORS + 1];
+    if (cells)
+    {
+        SubstepCells(v, d, sp, sync, cellLds);
+    }
+    else
+    {
+        SubstepLoop<m3GridSync, true>(v, sp, gid, gsz, sync);
+    }
 }
 
 // A batch: worlds are independent, so each workgroup runs its own world's
@@ -3278,6 +4166,12 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.consBodies, 2 * (int64_t)np) || DevAlloc(&v.slotOf, np) ||
         DevAlloc(&v.slotTable, (int64_t)M3_GRAPH_COLORS * nb) || DevAlloc(&v.slotMask, nb) ||
         DevAlloc(&w->dev.barrier, 4) || DevAlloc(&w->dev.narrowBins, 8 + 8 * (int64_t)np) ||
+        DevAlloc(&w->dev.flowRec, 6 * (int64_t)nb) || DevAlloc(&w->dev.flowRank, np) ||
+        DevAlloc(&w->dev.warmAdd, 2 * (int64_t)np) ||
+        DevAlloc(&w->dev.cellOf, nb) || DevAlloc(&w->dev.cellLocal, nb) ||
+        DevAlloc(&w->dev.cellBodies, (int64_t)M3_CELL_MAX * M3_CELL_BODIES) ||
+        DevAlloc(&w->dev.cellCount, M3_CELL_MAX) || DevAlloc(&w->dev.cellConsCount, M3_CELL_MAX) ||
+        DevAlloc(&w->dev.slotOwner, np) ||
         DevAlloc(&w->hitRecBuf, np) || DevAlloc(&v.shapeRank, ns) || DevAlloc(&v.ccdStatics, ns) ||
         DevAlloc(&w->dev.ccdXf, nb) || DevAlloc(&w->dev.ccdHit, nb))
     {
@@ -3345,6 +4239,21 @@ static int CreateImpl(m3gpuWorld* w)
         w->coopEnvFixed = w->coopFixed;
         w->coopDefault = w->coopBlocks;
         w->coopMax = maxBlocks;
+        const char* flowEnv = getenv("MAUL3D_AMD_FLOW");
+        w->flow = flowEnv != nullptr && atoi(flowEnv) == 0 ? 0 : 1;
+        // About 160 movers per cell (MAUL3D_AMD_CELLS overrides), as a grid
+        // of x slabs by z cells, within the co-resident limit.
+        const char* cellEnv = getenv("MAUL3D_AMD_CELLS");
+        int cells = cellEnv != nullptr ? atoi(cellEnv) : (nb + 159) / 160;
+        const int cellMax = maxBlocks < M3_CELL_MAX ? maxBlocks : M3_CELL_MAX;
+        cells = cells < 1 ? 1 : (cells > cellMax ? cellMax : cells);
+        int gx = 1;
+        while ((gx * 2) * (gx * 2) <= cells && gx * 2 <= M3_CELL_GX_MAX)
+        {
+            gx *= 2;
+        }
+        w->cellsX = gx;
+        w->cellsZ = cells / gx;
     }
     HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
     HIP_TRY(hipMalloc(&w->devDev, 2 * sizeof(m3Dev)));
@@ -3410,7 +4319,9 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut, w->dev.sapValsIn,
                     w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.blockCounts, w->dev.root, w->dev.narrowBins,
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
-                    v.slotTable, v.slotMask, w->dev.barrier, w->cubTemp, w->devDev,
+                    v.slotTable, v.slotMask, w->dev.barrier, w->dev.flowRec, w->dev.flowRank,
+                    w->dev.warmAdd, w->dev.cellOf, w->dev.cellLocal, w->dev.cellBodies, w->dev.cellCount,
+                    w->dev.cellConsCount, w->dev.slotOwner, w->cubTemp, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
                     v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
                     v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
@@ -3774,7 +4685,7 @@ enum m3Stage
     ST_FROZEN, ST_EMIT_SORT, ST_UNIQUE, ST_PAIR_COUNT, ST_EXPAND, ST_STALL_RESTORE,
     ST_SLEEP_SELECT, ST_SAT_LIST, ST_SAT, ST_NARROW, ST_CAPTURE, ST_UNION, ST_ROOT, ST_WAKE_MARK, ST_WAKE_APPLY,
     ST_CONS_FLAGS, ST_CONS_SELECT, ST_CONS_BODIES, ST_COLOR, ST_PREPARE,
-    ST_MOVER_FLAGS, ST_MOVER_SELECT, ST_SUBSTEPS, ST_CLEAR_FORCES, ST_RESTITUTION, ST_STORE, ST_CCD,
+    ST_MOVER_FLAGS, ST_MOVER_SELECT, ST_CELLS, ST_SUBSTEPS, ST_CLEAR_FORCES, ST_RESTITUTION, ST_STORE, ST_CCD,
     ST_SLEEP_TIMERS, ST_SLEEP_READY, ST_SLEEP_FREEZE, ST_END,
     ST_COUNT
 };
@@ -3784,7 +4695,7 @@ static const char* const kStageNames[] = {
     "plane_pairs", "frozen_merge", "pair_sort", "pair_unique", "pair_count", "pair_expand",
     "stall_restore", "frozen_select", "hull_sat_list", "hull_sat", "narrowphase", "capture", "island_union", "island_root",
     "wake_mark", "wake_apply", "constraint_flags", "constraint_select", "constraint_bodies",
-    "coloring", "prepare", "mover_flags", "mover_select", "substeps", "clear_forces",
+    "coloring", "prepare", "mover_flags", "mover_select", "cells", "substeps", "clear_forces",
     "restitution", "store_impulses", "continuous", "sleep_timers", "sleep_ready", "sleep_freeze",
     "end"};
 static_assert(sizeof(kStageNames) / sizeof(kStageNames[0]) == ST_COUNT, "stage names out of sync");
@@ -3866,6 +4777,7 @@ static m3SubstepParams SubstepParams(const m3gpuStepParams* p)
     m3SubstepParams sp;
     sp.substeps = p->substeps;
     sp.tailLimit = M3_TAIL_CONSTRAINTS;
+    sp.flow = 0;
     sp.h = h;
     sp.invH = h > 0.0f ? 1.0f / h : 0.0f;
     sp.push = p->contactPushMaxSpeed;
@@ -4218,14 +5130,14 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
     STAGE(ST_CONS_SELECT, KL(k_compact_constraints, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
     if (nb <= M3_COLOR_LDS_BODIES)
     {
-        STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D));
+        STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D, 1));
     }
     else
     {
-        STAGE(ST_COLOR, KL(k_color<false>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D));
+        STAGE(ST_COLOR, KL(k_color<false>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D, 1));
     }
     STAGE(ST_PREPARE, KL(k_prepare, dim3(Blocks(np, 64)), dim3(64), 0, st, D, p->contactHertz,
-                         p->contactDampingRatio, sp.h));
+                         p->contactDampingRatio, sp.h, w->flow));
     if (v.maxJoint > 0)
     {
         STAGE(ST_PREPARE,
@@ -4240,12 +5152,21 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
         {
             spArg.tailLimit = w->tailLimit;
         }
+        spArg.flow = w->flow;
         void* args[] = {(void*)&D, (void*)&spArg};
         int blocks = w->coopBlocks;
         if (!w->coopFixed)
         {
             int perBody = (nb + w->coopThreads - 1) / w->coopThreads;
             blocks = perBody < 1 ? 1 : (perBody < blocks ? perBody : blocks);
+        }
+        if (w->flow)
+        {
+            // One workgroup per cell; the colored form, taken when a step
+            // does not fit the cells, runs on the same grid.
+            blocks = w->cellsX * w->cellsZ;
+            STAGE(ST_CELLS, KL(k_cells, dim3(1), dim3(M3_CELL_BINS), 0, st, D, w->cellsX, w->cellsZ));
+            STAGE(ST_CELLS, KL(k_cell_owner, dim3(Blocks(np, T)), dim3(T), 0, st, D, blocks));
         }
         STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps, dim3(blocks),
                                                               dim3(w->coopThreads), args, 0, st)));
@@ -4944,9 +5865,9 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_constraint_flags, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 1);
     KL(k_cons_bodies, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
-    KL(k_color<true>, dim3(1, W), dim3(M3_COLOR_THREADS), 0, st, D);
+    KL(k_color<true>, dim3(1, W), dim3(M3_COLOR_THREADS), 0, st, D, 0);
     KL(k_prepare, dim3(Blocks(np, 64), W), dim3(64), 0, st, D, p->contactHertz,
-       p->contactDampingRatio, sp.h);
+       p->contactDampingRatio, sp.h, 0);
     if (s_batch.maxJoint > 0)
     {
         KL(k_prepare_joints, dim3(Blocks(s_batch.maxJoint, 64), W), dim3(64), 0, st, D, sp.h);

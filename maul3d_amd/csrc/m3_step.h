@@ -124,6 +124,7 @@ typedef struct m3Scalars
     int32_t fullQuery; // the next step rebuilds the frozen-pair buffer
     int32_t jointActive; // joints prepared this step
     int32_t satCount;    // wide hull pairs listed for the wavefront SAT queries
+    int32_t flowVeto;    // a constraint does not fit the barrier-free substep passes
     uint64_t stepCount;
 } m3Scalars;
 
@@ -1442,6 +1443,83 @@ M3_HD void m3WarmStartSide(const m3Constraint* c, int side, m3Vec3* linIo, m3Vec
     *angIo = ang;
 }
 
+// The warm-start updates that constraint c applies to one body (side 0 =
+// A, subtracted; 1 = B, added), each already multiplied out: per live
+// point the linear and angular change of its normal impulse, then the
+// friction pair's, then the twist and rolling row's angular change.  They
+// depend only on the constraint's stored accumulators and geometry, so any
+// thread may evaluate them ahead; a body applying them one by one in color
+// order performs the operations of m3WarmStartSide in the same order.
+typedef struct m3WarmAdd
+{
+    m3Vec3 lin[M3_MANIFOLD_MAX_POINTS];
+    m3Vec3 ang[M3_MANIFOLD_MAX_POINTS];
+    m3Vec3 linF;
+    m3Vec3 angF;
+    m3Vec3 angT;
+    int32_t pointCount;
+    int32_t pad[3];
+} m3WarmAdd;
+
+// m3WarmStartSide's addends for both sides from the constraint's geometry
+// and accumulators (normal impulses imp[], friction f1, f2, twist, rolling).
+M3_HD void m3WarmAddends(int32_t pointCount, m3Vec3 normal, const m3Vec3* rA, const m3Vec3* rB,
+                         const m3real* imp, m3real invMassA, m3real invMassB, const m3Mat3* invIA,
+                         const m3Mat3* invIB, m3Vec3 t1, m3Vec3 t2, m3real f1, m3real f2,
+                         m3Vec3 originA, m3Vec3 originB, m3real twist, m3Vec3 rolling, int dynA,
+                         int dynB, m3WarmAdd* outA, m3WarmAdd* outB)
+{
+    const m3Vec3 f = m3Add3(m3MulSV3(f1, t1), m3MulSV3(f2, t2));
+    const m3Vec3 tw = m3Add3(m3MulSV3(twist, normal), rolling);
+    for (int side = 0; side < 2; ++side)
+    {
+        if (side == 0 ? !dynA : !dynB)
+        {
+            continue;
+        }
+        m3WarmAdd* o = side == 0 ? outA : outB;
+        const m3real invMass = side == 0 ? invMassA : invMassB;
+        const m3Mat3* invI = side == 0 ? invIA : invIB;
+        const m3Vec3* arm = side == 0 ? rA : rB;
+        M3_UNROLL
+        for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
+        {
+            if (k < pointCount)
+            {
+                const m3Vec3 impulse = m3MulSV3(imp[k], normal);
+                o->lin[k] = m3MulSV3(invMass, impulse);
+                o->ang[k] = m3MulMV3(*invI, m3Cross3(arm[k], impulse));
+            }
+        }
+        o->linF = m3MulSV3(invMass, f);
+        o->angF = m3MulMV3(*invI, m3Cross3(side == 0 ? originA : originB, f));
+        o->angT = m3MulMV3(*invI, tw);
+        o->pointCount = pointCount;
+    }
+}
+
+// Applies one side's addends (m3WarmAddends) in m3WarmStartSide's order.
+M3_HD void m3WarmApply(const m3WarmAdd* a, int side, m3Vec3* linIo, m3Vec3* angIo)
+{
+    m3Vec3 lin = *linIo;
+    m3Vec3 ang = *angIo;
+    const int32_t pointCount = a->pointCount;
+    M3_UNROLL
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
+    {
+        if (k < pointCount)
+        {
+            lin = side == 0 ? m3Sub3(lin, a->lin[k]) : m3Add3(lin, a->lin[k]);
+            ang = side == 0 ? m3Sub3(ang, a->ang[k]) : m3Add3(ang, a->ang[k]);
+        }
+    }
+    lin = side == 0 ? m3Sub3(lin, a->linF) : m3Add3(lin, a->linF);
+    ang = side == 0 ? m3Sub3(ang, a->angF) : m3Add3(ang, a->angF);
+    ang = side == 0 ? m3Sub3(ang, a->angT) : m3Add3(ang, a->angT);
+    *linIo = lin;
+    *angIo = ang;
+}
+
 // Warm start of every constraint on dynamic body b in the order the
 // colored pass reaches them: one slot per color 0..15 from the body's slot
 // table, then the overflow color's slots [ovStart, ovEnd) in list order.
@@ -1855,58 +1933,22 @@ M3_HD void m3LoadContact(const m3Constraint* c, int useBias, m3ContactIn* in)
     }
 }
 
-// src/solver.c:430-581 on a gathered constraint (m3LoadContact of c with
-// the same useBias).  A constraint solve is a chain of short dependent
-// steps, so memory round trips dominate it: the body velocities are read
-// once, updated in registers in the reference's operation order, and
-// written back once: linear and angular velocity of each dynamic side, and
-// the angular velocity of a non-dynamic side that the rolling-resistance
-// row writes.
-M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn* in, m3real invH,
-                            m3real pushMax, int useBias)
+// The per-point terms of a contact solve that depend only on the bodies'
+// position deltas (src/solver.c:446-470: current separation, then the
+// speculative bias or the soft bias, mass scale and impulse scale).  The
+// deltas do not change inside a solve or relax pass, so a caller may
+// evaluate these before the velocities it waits for are ready.
+typedef struct m3ContactBias
 {
-    const int32_t a = in->a;
-    const int32_t b = in->b;
-    const int32_t pointCount = in->pointCount;
-    const m3Vec3 normal = in->normal;
-    const uint8_t dynA = in->dynA;
-    const uint8_t dynB = in->dynB;
-    const m3real invMassA = in->invMassA;
-    const m3real invMassB = in->invMassB;
-    const m3Mat3 invIA = in->invIA;
-    const m3Mat3 invIB = in->invIB;
-    const m3Softness soft = in->soft;
-    const m3real friction = in->friction;
-    const m3Vec3* prA = in->prA;
-    const m3Vec3* prB = in->prB;
-    const m3real* pBase = in->pBase;
-    const m3real* pMass = in->pMass;
-    const m3real* pLever = in->pLever;
-    const m3real* pImpulse = in->pImpulse;
-    const m3real* pTotal = in->pTotal;
-    m3PairVelocity s;
-    s.vA = v->lv[a];
-    s.wA = v->av[a];
-    s.vB = v->lv[b];
-    s.wB = v->av[b];
-    const m3Quat dqA = v->deltaRot[a];
-    const m3Quat dqB = v->deltaRot[b];
-    const m3Vec3 dpA = v->deltaPos[a];
-    const m3Vec3 dpB = v->deltaPos[b];
-    m3real passNormal = 0.0f;
-    m3real twistLimit = 0.0f;
-    // Every slot up to the largest point count among the lanes running this
-    // solve together is evaluated, and the results of slots past a lane's
-    // own pointCount are discarded, so no load waits behind a branch on the
-    // point count (the loads were issued by m3LoadContact) and the bound is
-    // uniform across the wavefront.  Slots past every lane's count would be
-    // discarded anyway, so skipping them changes no result.
-#if defined(__HIP_DEVICE_COMPILE__)
-    const int32_t slots = __ballot(pointCount > 2) != 0 ? (__ballot(pointCount > 3) != 0 ? 4 : 3)
-                                                        : (__ballot(pointCount > 1) != 0 ? 2 : 1);
-#else
-    const int32_t slots = M3_MANIFOLD_MAX_POINTS;
-#endif
+    m3real bias[M3_MANIFOLD_MAX_POINTS];
+    m3real massScale[M3_MANIFOLD_MAX_POINTS];
+    m3real impulseScale[M3_MANIFOLD_MAX_POINTS];
+} m3ContactBias;
+
+M3_HD void m3ContactBiasTerms(const m3ContactIn* in, m3Vec3 dpA, m3Quat dqA, m3Vec3 dpB, m3Quat dqB,
+                              m3real invH, m3real pushMax, int useBias, int32_t slots,
+                              m3ContactBias* out)
+{
     M3_UNROLL
     for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
     {
@@ -1914,13 +1956,10 @@ M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn*
         {
             break;
         }
-        const int live = k < pointCount;
-        const m3Vec3 rA = prA[k];
-        const m3Vec3 rB = prB[k];
-        m3Vec3 rsA = m3RotateVec3(dqA, rA);
-        m3Vec3 rsB = m3RotateVec3(dqB, rB);
+        m3Vec3 rsA = m3RotateVec3(dqA, in->prA[k]);
+        m3Vec3 rsB = m3RotateVec3(dqB, in->prB[k]);
         m3Vec3 ds = m3Add3(m3Sub3(dpB, dpA), m3Sub3(rsB, rsA));
-        m3real sep = pBase[k] + m3Dot3(ds, normal);
+        m3real sep = in->pBase[k] + m3Dot3(ds, in->normal);
         m3real bias = 0.0f;
         m3real massScale = 1.0f;
         m3real impulseScale = 0.0f;
@@ -1930,24 +1969,73 @@ M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn*
         }
         else if (useBias)
         {
-            bias = m3MaxF(soft.biasRate * sep, -pushMax);
-            massScale = soft.massScale;
-            impulseScale = soft.impulseScale;
+            bias = m3MaxF(in->soft.biasRate * sep, -pushMax);
+            massScale = in->soft.massScale;
+            impulseScale = in->soft.impulseScale;
         }
+        out->bias[k] = bias;
+        out->massScale[k] = massScale;
+        out->impulseScale[k] = impulseScale;
+    }
+}
+
+// The velocity part of a contact solve (src/solver.c:471-581): the normal
+// points in order, then (relax pass) twist, rolling resistance and
+// friction, on the pair's velocities held in registers.  Writes the
+// constraint's accumulators; returns whether the rolling row ran (it
+// writes the angular velocity of both sides).
+typedef struct m3ContactAccum
+{
+    m3real normal[M3_MANIFOLD_MAX_POINTS];
+    m3real f1;
+    m3real f2;
+    m3real twist;
+} m3ContactAccum;
+
+M3_HD int m3ContactImpulses(m3Constraint* c, const m3ContactIn* in, const m3ContactBias* cb,
+                            int useBias, int32_t slots, m3PairVelocity* sv, m3ContactAccum* acc = nullptr)
+{
+    const int32_t pointCount = in->pointCount;
+    const m3Vec3 normal = in->normal;
+    const uint8_t dynA = in->dynA;
+    const uint8_t dynB = in->dynB;
+    const m3real invMassA = in->invMassA;
+    const m3real invMassB = in->invMassB;
+    const m3Mat3 invIA = in->invIA;
+    const m3Mat3 invIB = in->invIB;
+    const m3real friction = in->friction;
+    m3PairVelocity s = *sv;
+    m3real passNormal = 0.0f;
+    m3real twistLimit = 0.0f;
+    M3_UNROLL
+    for (int32_t k = 0; k < M3_MANIFOLD_MAX_POINTS; ++k)
+    {
+        if (k >= slots)
+        {
+            break;
+        }
+        const int live = k < pointCount;
+        const m3Vec3 rA = in->prA[k];
+        const m3Vec3 rB = in->prB[k];
         m3Vec3 velB = m3Add3(s.vB, m3Cross3(s.wB, rB));
         m3Vec3 velA = m3Add3(s.vA, m3Cross3(s.wA, rA));
         m3real vn = m3Dot3(m3Sub3(velB, velA), normal);
-        const m3real oldImpulse = pImpulse[k];
-        m3real impulse = -pMass[k] * massScale * (vn + bias) - impulseScale * oldImpulse;
+        const m3real oldImpulse = in->pImpulse[k];
+        m3real impulse = -in->pMass[k] * cb->massScale[k] * (vn + cb->bias[k]) -
+                         cb->impulseScale[k] * oldImpulse;
         m3real newImpulse = m3MaxF(oldImpulse + impulse, 0.0f);
         impulse = newImpulse - oldImpulse;
         if (live)
         {
             m3CPoint* cp = &c->points[k];
             cp->normalImpulse = newImpulse;
-            cp->totalNormalImpulse = pTotal[k] + newImpulse;
+            cp->totalNormalImpulse = in->pTotal[k] + newImpulse;
             passNormal += newImpulse;
-            twistLimit += pLever[k] * newImpulse;
+            twistLimit += in->pLever[k] * newImpulse;
+        }
+        if (acc != nullptr)
+        {
+            acc->normal[k] = newImpulse;
         }
         const m3Vec3 P = m3MulSV3(impulse, normal);
         const m3Vec3 nvA = m3Sub3(s.vA, m3MulSV3(invMassA, P));
@@ -1998,6 +2086,10 @@ M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn*
             }
             delta = newTwist - oldTwist;
             c->twistImpulse = newTwist;
+            if (acc != nullptr)
+            {
+                acc->twist = newTwist;
+            }
             m3Vec3 tw = m3MulSV3(delta, normal);
             if (dynA)
             {
@@ -2048,6 +2140,11 @@ M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn*
             m3real d2 = f2 - oldF2;
             c->frictionImpulse1 = f1;
             c->frictionImpulse2 = f2;
+            if (acc != nullptr)
+            {
+                acc->f1 = f1;
+                acc->f2 = f2;
+            }
             const m3Vec3 P = m3Add3(m3MulSV3(d1, t1), m3MulSV3(d2, t2));
             if (dynA)
             {
@@ -2061,7 +2158,50 @@ M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn*
             }
         }
     }
-    if (dynA)
+    *sv = s;
+    return rolled;
+}
+
+// The number of manifold slots a solve evaluates: every slot up to the
+// largest point count among the lanes running this solve together.  The
+// results of slots past a lane's own pointCount are discarded, so no load
+// waits behind a branch on the point count and the bound is uniform across
+// the wavefront; slots past every lane's count would be discarded anyway,
+// so skipping them changes no result.
+M3_HD int32_t m3ContactSlots(int32_t pointCount)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(pointCount > 2) != 0 ? (__ballot(pointCount > 3) != 0 ? 4 : 3)
+                                         : (__ballot(pointCount > 1) != 0 ? 2 : 1);
+#else
+    (void)pointCount;
+    return M3_MANIFOLD_MAX_POINTS;
+#endif
+}
+
+// src/solver.c:430-581 on a gathered constraint (m3LoadContact of c with
+// the same useBias).  A constraint solve is a chain of short dependent
+// steps, so memory round trips dominate it: the body velocities are read
+// once, updated in registers in the reference's operation order, and
+// written back once: linear and angular velocity of each dynamic side, and
+// the angular velocity of a non-dynamic side that the rolling-resistance
+// row writes.
+M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn* in, m3real invH,
+                            m3real pushMax, int useBias)
+{
+    const int32_t a = in->a;
+    const int32_t b = in->b;
+    m3PairVelocity s;
+    s.vA = v->lv[a];
+    s.wA = v->av[a];
+    s.vB = v->lv[b];
+    s.wB = v->av[b];
+    const int32_t slots = m3ContactSlots(in->pointCount);
+    m3ContactBias cb;
+    m3ContactBiasTerms(in, v->deltaPos[a], v->deltaRot[a], v->deltaPos[b], v->deltaRot[b], invH,
+                       pushMax, useBias, slots, &cb);
+    const int rolled = m3ContactImpulses(c, in, &cb, useBias, slots, &s);
+    if (in->dynA)
     {
         v->lv[a] = s.vA;
         v->av[a] = s.wA;
@@ -2070,7 +2210,7 @@ M3_HD void m3SolveContactIn(const m3View* v, m3Constraint* c, const m3ContactIn*
     {
         v->av[a] = s.wA;
     }
-    if (dynB)
+    if (in->dynB)
     {
         v->lv[b] = s.vB;
         v->av[b] = s.wB;
