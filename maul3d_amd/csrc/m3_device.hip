@@ -77,6 +77,7 @@ struct m3Dev
     int32_t* cellBodies;
     int32_t* cellCount;
     int32_t* cellConsCount;
+    int32_t* cellCons; // per cell: its slots (any order)
     uint8_t* slotOwner;
     m3Tape* tape;      // intermediate-value record of this step (adjoint replays), or null
     // Pairs binned by collision routine: 8 counts, then one bin of
@@ -2120,6 +2121,12 @@ struct m3SubstepParams
 
 __device__ static inline void GridBarrier(uint32_t* bar, int32_t* fault)
 {
+    // Every wavefront waits for its own stores first: the workgroup barrier
+    // alone does not (on gfx950 it is a bare s_barrier), and the arriving
+    // thread's release waits only for its own wavefront's.  Once they are
+    // acknowledged, device-scope stores are at the coherence point and
+    // plain stores are in this XCD's L2, which that release writes back.
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (threadIdx.x == 0 &&
         __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
@@ -2249,7 +2256,7 @@ __device__ static inline void ColorPhase(const m3View& v, int32_t start, int32_t
     }
 }
 
-#define M3_SUBSTEP_THREADS 320 // M3_CELL_THREADS
+#define M3_SUBSTEP_THREADS 256 // M3_CELL_THREADS
 #ifndef M3_BLOCK_GATHER
 #define M3_BLOCK_GATHER 0
 #endif
@@ -2647,7 +2654,7 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
 // A wait gives up after this much wall time (s_memrealtime runs at
 // 100 MHz): far beyond any step, so only a broken wait reaches it.
 #define M3_FLOW_TICKS 200000000ull
-#define M3_CELL_SOLVERS 256 // solver threads of a cell workgroup
+#define M3_CELL_SOLVERS 192 // solver threads of a cell workgroup
 #define M3_CELL_MAIL 64     // the mail wavefront
 #define M3_CELL_THREADS (M3_CELL_SOLVERS + M3_CELL_MAIL)
 
@@ -2796,18 +2803,36 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
         return make_float2((float)p.x, (float)p.z);
     };
     float x0 = INFINITY, x1 = -INFINITY, z0 = INFINITY, z1 = -INFINITY;
-    for (int32_t m = t; m < n; m += blockDim.x)
+    // The cached movers' loads are issued together (index, then position).
+    constexpr int32_t kPer = M3_CELL_CACHE / M3_CELL_BINS;
+    int32_t mb[kPer];
+    M3_UNROLL
+    for (int32_t j = 0; j < kPer; ++j)
+    {
+        const int32_t m = t + j * M3_CELL_BINS;
+        mb[j] = m < n ? v.movers[m] : -1;
+    }
+    M3_UNROLL
+    for (int32_t j = 0; j < kPer; ++j)
+    {
+        if (mb[j] >= 0)
+        {
+            const m3Pos3 p = v.xf[mb[j]].p;
+            const float2 q = make_float2((float)p.x, (float)p.z);
+            pos[t + j * M3_CELL_BINS] = q;
+            x0 = fminf(x0, q.x);
+            x1 = fmaxf(x1, q.x);
+            z0 = fminf(z0, q.y);
+            z1 = fmaxf(z1, q.y);
+        }
+    }
+    for (int32_t m = M3_CELL_CACHE + t; m < n; m += blockDim.x)
     {
         const m3Pos3 p = v.xf[v.movers[m]].p;
-        const float2 q = make_float2((float)p.x, (float)p.z);
-        if (m < M3_CELL_CACHE)
-        {
-            pos[m] = q;
-        }
-        x0 = fminf(x0, q.x);
-        x1 = fmaxf(x1, q.x);
-        z0 = fminf(z0, q.y);
-        z1 = fmaxf(z1, q.y);
+        x0 = fminf(x0, (float)p.x);
+        x1 = fmaxf(x1, (float)p.x);
+        z0 = fminf(z0, (float)p.z);
+        z1 = fmaxf(z1, (float)p.z);
     }
     x0 = WaveMin(x0);
     x1 = WaveMax(x1);
@@ -2893,10 +2918,18 @@ __global__ void k_cell_owner(const m3Dev* __restrict__ ds, int32_t cells)
     const uint8_t owner = (c->dynA || c->dynB) ? d.cellOf[body] : 0;
     d.slotOwner[k] = owner;
     const bool outside = (c->dynA && d.cellOf[c->bodyA] >= cells) || (c->dynB && d.cellOf[c->bodyB] >= cells);
-    if (outside || owner >= cells || atomicAdd(&d.cellConsCount[owner], 1) >= M3_CELL_CONS)
+    if (outside || owner >= cells)
     {
         v.sc->flowVeto = 1;
+        return;
     }
+    const int32_t at = atomicAdd(&d.cellConsCount[owner], 1);
+    if (at >= M3_CELL_CONS)
+    {
+        v.sc->flowVeto = 1;
+        return;
+    }
+    d.cellCons[owner * M3_CELL_CONS + at] = k;
 }
 
 // Per-cell LDS of the substep kernel.
@@ -2904,6 +2937,7 @@ struct m3CellLds
 {
     float state[M3_CELL_BODIES][6]; // velocities of the cell's bodies
     uint32_t ver[M3_CELL_BODIES];
+    uint8_t deg[M3_CELL_BODIES]; // constraints on each of the cell's bodies
     float mail[2 * M3_CELL_CONS][6]; // imported velocities
     uint32_t mailVer[2 * M3_CELL_CONS];
     int32_t mailBody[2 * M3_CELL_CONS];
@@ -3009,6 +3043,25 @@ __device__ static inline void CellPut(m3CellLds& L, int32_t l, m3Vec3 lin, m3Vec
     __hip_atomic_store(&L.ver[l], ver, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The warm-start addends of a prepared or solved constraint (both sides).
+__device__ static inline void CellAddends(const m3Constraint* con, m3WarmAdd* out)
+{
+    m3Vec3 rA[M3_MANIFOLD_MAX_POINTS];
+    m3Vec3 rB[M3_MANIFOLD_MAX_POINTS];
+    m3real imp[M3_MANIFOLD_MAX_POINTS];
+    M3_UNROLL
+    for (int32_t q = 0; q < M3_MANIFOLD_MAX_POINTS; ++q)
+    {
+        rA[q] = con->points[q].rA;
+        rB[q] = con->points[q].rB;
+        imp[q] = con->points[q].normalImpulse;
+    }
+    m3WarmAddends(con->pointCount, con->normal, rA, rB, imp, con->invMassA, con->invMassB, &con->invIA,
+                  &con->invIB, con->t1, con->t2, con->frictionImpulse1, con->frictionImpulse2, con->originA,
+                  con->originB, con->twistImpulse, con->rollingImpulse, con->dynA, con->dynB, &out[0],
+                  &out[1]);
+}
+
 // Waits are bounded in wall time: every 256 rounds a waiting participant
 // checks the clock (from the first check of this wait) and the world's
 // fault flag; one that gives up marks the world faulted.  `since` is 0 at
@@ -3040,7 +3093,7 @@ __device__ static inline bool FlowGiveUp(uint32_t* spins, uint64_t* since, int32
 template <int kMode>
 __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& sp, m3CellLds& L,
                                        uint64_t* rec, const uint32_t* rank, m3WarmAdd* warm,
-                                       bool addends, uint32_t pass)
+                                       bool addends, uint32_t pass, const m3ContactIn* first)
 {
     const uint32_t base = pass << 8;
     int32_t* fault = &v.sc->fault;
@@ -3113,7 +3166,6 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
                 {
                     left = 0ull;
                 }
-                __builtin_amdgcn_s_sleep(1);
             }
             else
             {
@@ -3132,9 +3184,12 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
     uint32_t wantA = 0, wantB = 0;
     uint32_t fl = 0;
     int32_t locA = 0, locB = 0, ownA = 0, ownB = 0;
-    auto take = [&]() {
+    auto take = [&](bool loaded) {
         k = L.slot[i];
-        m3LoadContact(&v.cons[k], kMode == 1, &in);
+        if (!loaded)
+        {
+            m3LoadContact(&v.cons[k], kMode == 1, &in);
+        }
         const uint32_t rk = rank[k];
         wantA = base | (rk & 0xFFu);
         wantB = base | ((rk >> 8) & 0xFFu);
@@ -3162,7 +3217,8 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
     };
     if (have)
     {
-        take();
+        in = *first;
+        take(true);
     }
     uint32_t spins = 0;
     uint64_t since = 0;
@@ -3214,8 +3270,7 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
             if (ready)
             {
                 const int32_t slots = m3ContactSlots(in.pointCount);
-                m3ContactAccum acc;
-                m3ContactImpulses(&v.cons[k], &in, &cb, kMode == 1, slots, &s, &acc);
+                m3ContactImpulses(&v.cons[k], &in, &cb, kMode == 1, slots, &s);
                 if (in.dynA)
                 {
                     if (fl & 2u)
@@ -3232,13 +3287,6 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
                     }
                     FlowStore(rec, in.b, s.vB, s.wB, wantB + 1u);
                 }
-                if (kMode == 2 && addends)
-                {
-                    m3WarmAddends(in.pointCount, in.normal, in.prA, in.prB, acc.normal, in.invMassA,
-                                  in.invMassB, &in.invIA, &in.invIB, in.t1, in.t2, acc.f1, acc.f2,
-                                  in.originA, in.originB, acc.twist, v.cons[k].rollingImpulse, in.dynA,
-                                  in.dynB, &warm[2 * k], &warm[2 * k + 1]);
-                }
                 progressed = true;
                 spins = 0;
                 since = 0;
@@ -3246,7 +3294,7 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
                 have = i < count;
                 if (have)
                 {
-                    take();
+                    take(false);
                 }
             }
             else if (FlowGiveUp(&spins, &since, fault))
@@ -3257,6 +3305,16 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
         if (!__any(progressed))
         {
             __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    // The next substep's warm-start addends, from the accumulators this
+    // pass left (off the pass's dependency chains).
+    if (kMode == 2 && addends)
+    {
+        for (int32_t j = t; j < count; j += M3_CELL_SOLVERS)
+        {
+            const int32_t kk = L.slot[j];
+            CellAddends(&v.cons[kk], &warm[2 * kk]);
         }
     }
 }
@@ -3270,27 +3328,23 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
 __device__ static inline void CellSetup(const m3View& v, const m3Dev& d, m3CellLds& L, int32_t cell)
 {
     const int32_t t = threadIdx.x;
-    const int32_t begin = v.sc->colorStarts[0];
-    const int32_t end = v.sc->colorStarts[M3_GRAPH_COLORS];
+    const int32_t nHome = d.cellCount[cell];
+    for (int32_t j = t; j < nHome; j += blockDim.x)
+    {
+        L.deg[j] = (uint8_t)__popc(v.slotMask[d.cellBodies[cell * M3_CELL_BODIES + j]] & 0xFFFFu);
+    }
+    const int32_t total = d.cellConsCount[cell];
+    const int32_t count = total < M3_CELL_CONS ? total : M3_CELL_CONS;
     if (t == 0)
     {
-        L.count = 0;
+        L.count = count;
         L.imports = 0;
     }
-    __syncthreads();
-    for (int32_t k = begin + t; k < end; k += blockDim.x)
+    for (int32_t i = t; i < count; i += blockDim.x)
     {
-        if (d.slotOwner[k] == (uint8_t)cell)
-        {
-            const int32_t at = atomicAdd(&L.count, 1);
-            if (at < M3_CELL_CONS)
-            {
-                L.slot[at] = k;
-            }
-        }
+        L.slot[i] = d.cellCons[cell * M3_CELL_CONS + i];
     }
     __syncthreads();
-    const int32_t count = L.count < M3_CELL_CONS ? L.count : M3_CELL_CONS;
     CellSort(L, count);
     for (int32_t i = t; i < count; i += blockDim.x)
     {
@@ -3341,6 +3395,50 @@ __device__ static inline void CellSetup(const m3View& v, const m3Dev& d, m3CellL
     __syncthreads();
 }
 
+// A home body's velocities after pass `pass`: from LDS when the body's last
+// constraint of the pass ran in this cell (its LDS version then counts
+// all of them), else from its global words once all six carry that last
+// constraint's version.  An acknowledged device-scope store from another
+// XCD can still be on its way after the grid barrier; waiting for the
+// final version also means no older store to these words can land after
+// the body phase rewrites them (every solve saw its predecessor's version
+// in each word before writing).
+__device__ static inline void CellFinal(const m3CellLds& L, const uint64_t* rec, int32_t j, int32_t b,
+                                        uint32_t pass, m3Vec3* lin, m3Vec3* ang, int32_t* fault)
+{
+    float x[6];
+    const uint32_t want = (pass << 8) | L.deg[j];
+    if (L.ver[j] == want)
+    {
+        for (int c = 0; c < 6; ++c)
+        {
+            x[c] = L.state[j][c];
+        }
+    }
+    else
+    {
+        uint32_t spins = 0;
+        uint64_t since = 0;
+        while (FlowLoad(rec, b, x) != want && !FlowGiveUp(&spins, &since, fault))
+        {
+        }
+    }
+    *lin = m3V3(x[0], x[1], x[2]);
+    *ang = m3V3(x[3], x[4], x[5]);
+}
+
+// The first constraint of a solver thread, loaded before the barrier that
+// opens a pass (its fields are fixed or written only by this thread).
+template <int kMode>
+__device__ static inline void CellFirst(const m3View& v, const m3CellLds& L, m3ContactIn* in)
+{
+    const int32_t t = threadIdx.x;
+    if (t < M3_CELL_SOLVERS && t < L.count)
+    {
+        m3LoadContact(&v.cons[L.slot[t]], kMode == 1, in);
+    }
+}
+
 // Arms the mailboxes for a pass: each waits for version pass << 8 | rank.
 __device__ static inline void CellArm(m3CellLds& L, uint32_t pass)
 {
@@ -3369,10 +3467,10 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             const int32_t b = home[j];
             if (sub > 0)
             {
-                float x[6];
-                FlowLoad(rec, b, x);
-                v.lv[b] = m3V3(x[0], x[1], x[2]);
-                v.av[b] = m3V3(x[3], x[4], x[5]);
+                m3Vec3 lin, ang;
+                CellFinal(L, rec, j, b, solvePass - 1u, &lin, &ang, &v.sc->fault);
+                v.lv[b] = lin;
+                v.av[b] = ang;
             }
             m3IntegrateVelocity(&v, b, sp.h, sp.gravity, sp.maxLinear, sp.maxAngular);
             m3Vec3 lin = v.lv[b];
@@ -3387,32 +3485,41 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             FlowStore(rec, b, lin, ang, solvePass << 8);
         }
         CellArm(L, solvePass);
-        sync();
-        CellPass<1>(v, sp, L, rec, d.flowRank, d.warmAdd, false, solvePass);
+        {
+            m3ContactIn first;
+            CellFirst<1>(v, L, &first);
+            sync();
+            CellPass<1>(v, sp, L, rec, d.flowRank, d.warmAdd, false, solvePass, &first);
+        }
         sync();
         for (int32_t j = t; j < nHome; j += blockDim.x)
         {
             const int32_t b = home[j];
-            float x[6];
-            FlowLoad(rec, b, x);
-            v.lv[b] = m3V3(x[0], x[1], x[2]);
-            v.av[b] = m3V3(x[3], x[4], x[5]);
+            m3Vec3 lin, ang;
+            CellFinal(L, rec, j, b, solvePass, &lin, &ang, &v.sc->fault);
+            v.lv[b] = lin;
+            v.av[b] = ang;
             m3IntegratePosition(&v, b, sp.h);
             CellPut(L, j, v.lv[b], v.av[b], (solvePass + 1u) << 8);
             FlowStore(rec, b, v.lv[b], v.av[b], (solvePass + 1u) << 8);
         }
         CellArm(L, solvePass + 1u);
-        sync();
-        CellPass<2>(v, sp, L, rec, d.flowRank, d.warmAdd, sub + 1 < sp.substeps, solvePass + 1u);
+        {
+            m3ContactIn first;
+            CellFirst<2>(v, L, &first);
+            sync();
+            CellPass<2>(v, sp, L, rec, d.flowRank, d.warmAdd, sub + 1 < sp.substeps, solvePass + 1u,
+                        &first);
+        }
         sync();
     }
     for (int32_t j = t; j < nHome; j += blockDim.x)
     {
         const int32_t b = home[j];
-        float x[6];
-        FlowLoad(rec, b, x);
-        v.lv[b] = m3V3(x[0], x[1], x[2]);
-        v.av[b] = m3V3(x[3], x[4], x[5]);
+        m3Vec3 lin, ang;
+        CellFinal(L, rec, j, b, 2u * (uint32_t)sp.substeps - 1u, &lin, &ang, &v.sc->fault);
+        v.lv[b] = lin;
+        v.av[b] = ang;
     }
 }
 
@@ -4172,6 +4279,7 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&w->dev.cellBodies, (int64_t)M3_CELL_MAX * M3_CELL_BODIES) ||
         DevAlloc(&w->dev.cellCount, M3_CELL_MAX) || DevAlloc(&w->dev.cellConsCount, M3_CELL_MAX) ||
         DevAlloc(&w->dev.slotOwner, np) ||
+        DevAlloc(&w->dev.cellCons, (int64_t)M3_CELL_MAX * M3_CELL_CONS) ||
         DevAlloc(&w->hitRecBuf, np) || DevAlloc(&v.shapeRank, ns) || DevAlloc(&v.ccdStatics, ns) ||
         DevAlloc(&w->dev.ccdXf, nb) || DevAlloc(&w->dev.ccdHit, nb))
     {
@@ -4241,10 +4349,10 @@ static int CreateImpl(m3gpuWorld* w)
         w->coopMax = maxBlocks;
         const char* flowEnv = getenv("MAUL3D_AMD_FLOW");
         w->flow = flowEnv != nullptr && atoi(flowEnv) == 0 ? 0 : 1;
-        // About 160 movers per cell (MAUL3D_AMD_CELLS overrides), as a grid
+        // About 100 movers per cell (MAUL3D_AMD_CELLS overrides), as a grid
         // of x slabs by z cells, within the co-resident limit.
         const char* cellEnv = getenv("MAUL3D_AMD_CELLS");
-        int cells = cellEnv != nullptr ? atoi(cellEnv) : (nb + 159) / 160;
+        int cells = cellEnv != nullptr ? atoi(cellEnv) : (nb + 99) / 100;
         const int cellMax = maxBlocks < M3_CELL_MAX ? maxBlocks : M3_CELL_MAX;
         cells = cells < 1 ? 1 : (cells > cellMax ? cellMax : cells);
         int gx = 1;
@@ -4321,7 +4429,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
                     v.slotTable, v.slotMask, w->dev.barrier, w->dev.flowRec, w->dev.flowRank,
                     w->dev.warmAdd, w->dev.cellOf, w->dev.cellLocal, w->dev.cellBodies, w->dev.cellCount,
-                    w->dev.cellConsCount, w->dev.slotOwner, w->cubTemp, w->devDev,
+                    w->dev.cellConsCount, w->dev.slotOwner, w->dev.cellCons, w->cubTemp, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
                     v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
                     v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
