@@ -45,18 +45,26 @@ static int Fail(const char* what, hipError_t e)
 // What a kernel sees of one world: the device view plus the step's scratch.
 // Kernels take an array of these and work on entry blockIdx.y, so one
 // launch serves one world or a batch of independent worlds alike.
+// A shape's sweep key: the sortable bits of its lower x bound rounded down
+// to float (the value the sweep compares) over its index; shapes out of
+// the sweep (dead, planes) carry an all-ones high word and sort last.
+#define M3_SAP_OUT 0xFFFFFFFFu
+
 struct m3Dev
 {
     m3View v;
-    uint64_t* sapKeysIn;
-    uint64_t* sapKeysOut;
-    int32_t* sapValsIn;
-    int32_t* sapValsOut;
+    uint64_t* sapKeysIn;  // per shape: its sweep key (k_fresh)
+    uint64_t* sapKeysOut; // the keys sorted
     float4* sapXY;
     float2* sapZ;
     int4* sapAttr;
-    uint32_t* emitSorted;
     uint32_t* emitUnique;
+    // Pair-key grouping by lower shape (k_emit_pairs): per shape the count,
+    // start, fill cursor, unique count and unique start of its group, the
+    // groups too large for one thread and their count; per key its group.
+    int32_t* grp;
+    uint32_t* grpKeys;
+    int32_t* grpOf;
     uint8_t* pairFlags;
     uint8_t* bodyFlags;
     int32_t* blockCounts; // set flags per 256-thread workgroup of the last flag pass
@@ -131,8 +139,6 @@ struct m3gpuWorld
     int streamBusy; // work enqueued on the world's stream since it was last idle
     int batched;    // the world's latest device work ran in the batch stream
     int64_t stamp;  // duplicate detection in batched calls
-    void* cubTemp;
-    size_t cubBytes;
     m3Scalars* hostScalars; // pinned mirror for stats
     struct m3Profiler* prof;
     int coopBlocks;  // co-resident blocks of the substep kernel (maximum)
@@ -146,6 +152,8 @@ struct m3gpuWorld
     int cellsZ;      // and cells per slab along z
     int coopThreads; // threads per block of the substep kernel
     int wideHulls;   // some hull is larger than a box (wide hull pairs possible)
+    void* cubTemp;       // temp storage of the sweep-key radix sort
+    size_t cubBytes;
     m3HitRec* hitRecBuf; // per pair; in the view only while some shape has hit events
     struct m3EventBufs* events; // event classification and output (allocated at first read)
     int boxHulls;    // some hull is at least box-sized
@@ -183,6 +191,18 @@ static inline unsigned Blocks(int64_t n, int threads)
 {
     int64_t b = (n + threads - 1) / threads;
     return (unsigned)(b > 0 ? b : 1);
+}
+
+__device__ static inline float RoundDown(double d)
+{
+    float f = (float)d;
+    return (double)f > d ? nextafterf(f, -INFINITY) : f;
+}
+
+__device__ static inline float RoundUp(double d)
+{
+    float f = (float)d;
+    return (double)f < d ? nextafterf(f, INFINITY) : f;
 }
 
 __device__ static inline uint64_t SortableDouble(double d)
@@ -320,6 +340,17 @@ __global__ void k_open(const m3Dev* __restrict__ ds)
     {
         v.emit[i] = M3_SENTINEL32;
     }
+    // The pair-key groups' counters start at zero (k_emit_pairs).
+    const int32_t stride = v.maxShape + 1;
+    if (i < v.maxShape)
+    {
+        d.grp[i] = 0;
+        d.grp[2 * stride + i] = 0;
+    }
+    if (i == 0)
+    {
+        d.grp[6 * stride] = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -408,7 +439,7 @@ __global__ void k_fresh(const m3Dev* __restrict__ ds)
     {
         return;
     }
-    uint64_t key = M3_SENTINEL64;
+    uint32_t u = M3_SAP_OUT;
     if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE)
     {
         double box[6];
@@ -417,14 +448,11 @@ __global__ void k_fresh(const m3Dev* __restrict__ ds)
         {
             v.fresh[6 * s + k] = box[k];
         }
-        key = SortableDouble(box[0]);
-        if (key == M3_SENTINEL64)
-        {
-            key -= 1;
-        }
+        u = __float_as_uint(RoundDown(box[0]));
+        u ^= (u & 0x80000000u) != 0u ? 0xFFFFFFFFu : 0x80000000u;
+        u = u < M3_SAP_OUT ? u : M3_SAP_OUT - 1u;
     }
-    d.sapKeysIn[s] = key;
-    d.sapValsIn[s] = s;
+    d.sapKeysIn[s] = ((uint64_t)u << 32) | (uint32_t)s;
 }
 
 __global__ void k_reset_sleep(const m3Dev* __restrict__ ds)
@@ -485,17 +513,6 @@ __global__ void k_refresh(const m3Dev* __restrict__ ds)
 // Outward-rounded float copies of the sorted fresh boxes: a float test that
 // fails implies the exact double test fails, so the sweep filters on floats
 // and confirms survivors in double.
-__device__ static inline float RoundDown(double d)
-{
-    float f = (float)d;
-    return (double)f > d ? nextafterf(f, -INFINITY) : f;
-}
-
-__device__ static inline float RoundUp(double d)
-{
-    float f = (float)d;
-    return (double)f < d ? nextafterf(f, INFINITY) : f;
-}
 
 // Per-shape pair facts packed at gather time so the sweep's filters are
 // register operations (src/broad_phase.c:182-251 read these per pair).
@@ -513,8 +530,7 @@ enum
 __global__ void k_sap_gather(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    const uint64_t* keys = d.sapKeysOut;
-    const int32_t* vals = d.sapValsOut;
+    const uint64_t* sorted = d.sapKeysOut;
     float4* xy = d.sapXY;
     float2* zz = d.sapZ;
     int4* attr = d.sapAttr;
@@ -523,14 +539,14 @@ __global__ void k_sap_gather(const m3Dev* __restrict__ ds)
     {
         return;
     }
-    if (keys[i] == M3_SENTINEL64)
+    if ((uint32_t)(sorted[i] >> 32) == M3_SAP_OUT)
     {
         xy[i] = make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
         zz[i] = make_float2(INFINITY, -INFINITY);
         attr[i] = make_int4(-1, -1, 0, 0);
         return;
     }
-    int32_t s = vals[i];
+    int32_t s = (int32_t)(sorted[i] & 0xFFFFFFFFu);
     const double* f = &v.fresh[6 * s];
     xy[i] = make_float4(RoundDown(f[0]), RoundUp(f[3]), RoundDown(f[1]), RoundUp(f[4]));
     zz[i] = make_float2(RoundDown(f[2]), RoundUp(f[5]));
@@ -4070,38 +4086,242 @@ __device__ static inline int32_t NextPow2(int32_t n)
     return p;
 }
 
-// Sweep keys ordered by (key, shape): the order a stable sort by key gives.
-struct m3SweepItem
-{
-    uint64_t key;
-    int32_t shape;
-    int32_t pad;
-    __device__ bool operator<(const m3SweepItem& o) const
-    {
-        return key < o.key || (key == o.key && shape < o.shape);
-    }
-};
 
 __global__ void __launch_bounds__(M3_BATCH_THREADS) k_sap_sort_block(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    __shared__ m3SweepItem items[M3_BATCH_SHAPES_MAX];
+    __shared__ uint64_t keys[M3_BATCH_SHAPES_MAX];
     const int32_t n = v.maxShape;
     const int32_t n2 = NextPow2(n);
     for (int32_t i = threadIdx.x; i < n2; i += blockDim.x)
     {
-        m3SweepItem it;
-        it.key = i < n ? d.sapKeysIn[i] : M3_SENTINEL64;
-        it.shape = i < n ? d.sapValsIn[i] : 0x7FFFFFFF;
-        it.pad = 0;
-        items[i] = it;
+        keys[i] = i < n ? d.sapKeysIn[i] : M3_SENTINEL64;
     }
     __syncthreads();
-    BitonicSortLds(items, n2);
+    BitonicSortLds(keys, n2);
     for (int32_t i = threadIdx.x; i < n; i += blockDim.x)
     {
-        d.sapKeysOut[i] = items[i].key;
-        d.sapValsOut[i] = items[i].shape;
+        d.sapKeysOut[i] = keys[i];
+    }
+}
+
+#define M3_SORT_THREADS 1024
+#define M3_SORT_GRID 32 // workgroups of the pair-key launch
+
+// ---------------------------------------------------------------------------
+// The pair list's candidate keys sorted and made unique (src/broad_phase.c:636
+// sorts the unique keys) in one cooperative launch.  A compact key is
+// lower shape << 16 | higher shape, so keys group by their lower shape:
+// count per group, scan, scatter; then each group is ordered and its
+// duplicates dropped (up to M3_GROUP_SMALL keys in one thread's registers,
+// larger groups by a workgroup through an LDS bitmap over the higher
+// shape); a second scan places the groups' unique keys.
+// ---------------------------------------------------------------------------
+#define M3_GROUP_SMALL 16
+#define M3_GROUP_BITMAP_WORDS 2048 // higher shapes < 65536
+
+// Exclusive scan of in[0, n) into out[0, n] (out[n] = total) by one
+// workgroup of M3_SORT_THREADS; groups above `big` keys are listed.
+__device__ static inline void GroupScan(const int32_t* in, int32_t* out, int32_t n, int32_t big,
+                                        int32_t* list, int32_t* listCount)
+{
+    int32_t carry = 0;
+    for (int32_t c = 0; c < n; c += blockDim.x)
+    {
+        const int32_t i = c + threadIdx.x;
+        const int32_t x = i < n ? in[i] : 0;
+        int32_t total = 0;
+        const int32_t pos = BlockExclusiveScan(x, &total);
+        if (i < n)
+        {
+            out[i] = carry + pos;
+            if (big >= 0 && x > big)
+            {
+                list[atomicAdd(listCount, 1)] = i;
+            }
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0)
+    {
+        out[n] = carry;
+    }
+}
+
+__global__ void __launch_bounds__(M3_SORT_THREADS) k_emit_pairs(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    __shared__ uint32_t bits[M3_GROUP_BITMAP_WORDS];
+    const m3GridSync sync{d.barrier, &v.sc->fault};
+    const int32_t ns = v.maxShape;
+    const int32_t stride = ns + 1;
+    int32_t* cnt = d.grp;
+    int32_t* start = d.grp + stride;
+    int32_t* fill = d.grp + 2 * stride;
+    int32_t* ucnt = d.grp + 3 * stride;
+    int32_t* ustart = d.grp + 4 * stride;
+    int32_t* large = d.grp + 5 * stride;
+    int32_t* largeCount = d.grp + 6 * stride;
+    int32_t n = v.sc->emitCount;
+    n = n < 0 ? 0 : (n < v.emitCapacity ? n : v.emitCapacity);
+    const int32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t gsz = gridDim.x * blockDim.x;
+    // Emission writes each wavefront's keys together, so runs of one lower
+    // shape (a plane's pairs) share a wavefront: their lanes add once.
+    const int32_t lane = threadIdx.x & 63;
+    for (int32_t i0 = gid - lane; i0 < n; i0 += gsz)
+    {
+        const int32_t i = i0 + lane;
+        const uint32_t lo = i < n ? v.emit[i] >> 16 : 0xFFFFFFFFu;
+        const bool live = lo < (uint32_t)ns;
+        const uint32_t lead = (uint32_t)__shfl((int32_t)lo, 0);
+        const uint64_t same = __ballot(live && lo == lead);
+        if (live && lo != lead)
+        {
+            atomicAdd(&cnt[lo], 1);
+        }
+        if (lane == 0 && same != 0ull)
+        {
+            atomicAdd(&cnt[lead], (int32_t)__popcll(same));
+        }
+    }
+    sync();
+    if (blockIdx.x == 0)
+    {
+        GroupScan(cnt, start, ns, M3_GROUP_SMALL, large, largeCount);
+    }
+    sync();
+    for (int32_t i0 = gid - lane; i0 < n; i0 += gsz)
+    {
+        const int32_t i = i0 + lane;
+        const uint32_t key = i < n ? v.emit[i] : M3_SENTINEL32;
+        const uint32_t lo = key >> 16;
+        const bool live = lo < (uint32_t)ns;
+        const uint32_t lead = (uint32_t)__shfl((int32_t)lo, 0);
+        const uint64_t same = __ballot(live && lo == lead);
+        int32_t base = 0;
+        if (lane == 0 && same != 0ull)
+        {
+            base = atomicAdd(&fill[lead], (int32_t)__popcll(same));
+        }
+        base = __shfl(base, 0);
+        if (live)
+        {
+            const int32_t at = lo == lead ? start[lo] + base + (int32_t)__popcll(same & ((1ull << lane) - 1ull))
+                                          : start[lo] + atomicAdd(&fill[lo], 1);
+            d.grpKeys[at] = key;
+            d.grpOf[at] = (int32_t)lo;
+        }
+    }
+    sync();
+    // Large groups: a workgroup each, through a bitmap over the higher shape.
+    const int32_t words = (ns + 31) >> 5;
+    const int32_t nLarge = *largeCount;
+    for (int32_t g = blockIdx.x; g < nLarge; g += gridDim.x)
+    {
+        const int32_t lo = large[g];
+        const int32_t s0 = start[lo];
+        const int32_t c = cnt[lo];
+        for (int32_t w = threadIdx.x; w < words; w += blockDim.x)
+        {
+            bits[w] = 0u;
+        }
+        __syncthreads();
+        for (int32_t j = threadIdx.x; j < c; j += blockDim.x)
+        {
+            const uint32_t hi = d.grpKeys[s0 + j] & 0xFFFFu;
+            atomicOr(&bits[hi >> 5], 1u << (hi & 31u));
+        }
+        __syncthreads();
+        int32_t carry = 0;
+        for (int32_t w0 = 0; w0 < words; w0 += blockDim.x)
+        {
+            const int32_t w = w0 + threadIdx.x;
+            const uint32_t m = w < words ? bits[w] : 0u;
+            int32_t total = 0;
+            int32_t pos = carry + BlockExclusiveScan(__popc(m), &total);
+            for (uint32_t r = m; r != 0u; r &= r - 1u)
+            {
+                const uint32_t hi = ((uint32_t)w << 5) | (uint32_t)__builtin_ctz(r);
+                d.grpKeys[s0 + pos++] = ((uint32_t)lo << 16) | hi;
+            }
+            carry += total;
+        }
+        if (threadIdx.x == 0)
+        {
+            ucnt[lo] = carry;
+        }
+        __syncthreads();
+    }
+    // Small groups: one thread each, in registers.
+    for (int32_t lo = gid; lo < ns; lo += gsz)
+    {
+        const int32_t c = cnt[lo];
+        if (c > M3_GROUP_SMALL)
+        {
+            continue;
+        }
+        const int32_t s0 = start[lo];
+        uint32_t k[M3_GROUP_SMALL];
+        M3_UNROLL
+        for (int32_t j = 0; j < M3_GROUP_SMALL; ++j)
+        {
+            k[j] = j < c ? d.grpKeys[s0 + j] : M3_SENTINEL32;
+        }
+        // Bitonic network over the 16 registers (ascending).
+        M3_UNROLL
+        for (int32_t size = 2; size <= M3_GROUP_SMALL; size <<= 1)
+        {
+            M3_UNROLL
+            for (int32_t half = size >> 1; half > 0; half >>= 1)
+            {
+                M3_UNROLL
+                for (int32_t a = 0; a < M3_GROUP_SMALL; ++a)
+                {
+                    const int32_t b = a ^ half;
+                    if (b > a)
+                    {
+                        const bool up = (a & size) == 0;
+                        const uint32_t x = k[a];
+                        const uint32_t y = k[b];
+                        const bool swap = up ? x > y : x < y;
+                        k[a] = swap ? y : x;
+                        k[b] = swap ? x : y;
+                    }
+                }
+            }
+        }
+        int32_t u = 0;
+        M3_UNROLL
+        for (int32_t j = 0; j < M3_GROUP_SMALL; ++j)
+        {
+            if (j < c && (j == 0 || k[j] != k[j - 1]))
+            {
+                d.grpKeys[s0 + u] = k[j];
+                ++u;
+            }
+        }
+        ucnt[lo] = u;
+    }
+    sync();
+    if (blockIdx.x == 0)
+    {
+        GroupScan(ucnt, ustart, ns, -1, nullptr, nullptr);
+    }
+    sync();
+    const int32_t placed = start[ns];
+    for (int32_t p = gid; p < placed; p += gsz)
+    {
+        const int32_t lo = d.grpOf[p];
+        const int32_t j = p - start[lo];
+        if (j < ucnt[lo])
+        {
+            d.emitUnique[ustart[lo] + j] = d.grpKeys[p];
+        }
+    }
+    if (gid == 0)
+    {
+        v.sc->uniqueCount = ustart[ns];
     }
 }
 
@@ -4256,7 +4476,7 @@ static int CreateImpl(m3gpuWorld* w)
         }
     }
     if (DevAlloc(&v.sleepKeys, np) || DevAlloc(&v.emit, v.emitCapacity) ||
-        DevAlloc(&w->dev.emitSorted, v.emitCapacity) || DevAlloc(&w->dev.emitUnique, v.emitCapacity) ||
+        DevAlloc(&w->dev.emitUnique, v.emitCapacity) ||
         DevAlloc(&v.satQueries, np) || DevAlloc(&v.satList, np) ||
         DevAlloc(&v.cons, np) || DevAlloc(&v.consPair, np) || DevAlloc(&v.colors, np) ||
         DevAlloc(&v.colorLists, np) || DevAlloc(&v.movers, nb) || DevAlloc(&v.deltaPos, nb) ||
@@ -4264,7 +4484,8 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&v.parent, nb) || DevAlloc(&v.forced, nb) || DevAlloc(&v.flag, nb) ||
         DevAlloc(&v.pending, nb) ||
         DevAlloc(&v.sc, 1) || DevAlloc(&w->dev.sapKeysIn, ns) || DevAlloc(&w->dev.sapKeysOut, ns) ||
-        DevAlloc(&w->dev.sapValsIn, ns) || DevAlloc(&w->dev.sapValsOut, ns) ||
+        DevAlloc(&w->dev.grp, 6 * ((int64_t)ns + 1) + 1) || DevAlloc(&w->dev.grpKeys, 2 * (int64_t)np) ||
+        DevAlloc(&w->dev.grpOf, 2 * (int64_t)np) ||
         DevAlloc(&w->dev.sapXY, ns) || DevAlloc(&w->dev.sapZ, ns) || DevAlloc(&w->dev.sapAttr, ns) ||
         DevAlloc(&w->dev.pairFlags, np) || DevAlloc(&w->dev.bodyFlags, nb) ||
         DevAlloc(&w->dev.blockCounts, Blocks(np > nb ? np : nb, M3_COMPACT_THREADS)) || DevAlloc(&w->dev.root, nb) ||
@@ -4306,22 +4527,17 @@ static int CreateImpl(m3gpuWorld* w)
     {
         return -1;
     }
+    // Temp storage of the sweep-key radix sort.
+    {
+        size_t bytes = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, w->dev.sapKeysIn, w->dev.sapKeysOut, ns,
+                                                  0, 64, w->stream));
+        w->cubBytes = bytes > 0 ? bytes : 1;
+        HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
+    }
     HIP_TRY(hipHostMalloc((void**)&w->hostScalars, sizeof(m3Scalars), hipHostMallocDefault));
     memset(w->hostScalars, 0, sizeof(m3Scalars));
 
-    // One temp buffer sized for the largest primitive call.
-    size_t need = 0;
-    size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, w->dev.sapKeysIn, w->dev.sapKeysOut,
-                                               w->dev.sapValsIn, w->dev.sapValsOut, ns, 0, 64, w->stream));
-    need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, v.emit, w->dev.emitSorted,
-                                              v.emitCapacity, 0, 32, w->stream));
-    need = bytes > need ? bytes : need;
-    HIP_TRY(hipcub::DeviceSelect::Unique(nullptr, bytes, w->dev.emitSorted, w->dev.emitUnique,
-                                         &v.sc->uniqueCount, v.emitCapacity, w->stream));
-    need = bytes > need ? bytes : need;
-    w->cubBytes = need > 0 ? need : 1;
     {
         int dev = 0;
         HIP_TRY(hipGetDevice(&dev));
@@ -4363,7 +4579,6 @@ static int CreateImpl(m3gpuWorld* w)
         w->cellsX = gx;
         w->cellsZ = cells / gx;
     }
-    HIP_TRY(hipMalloc(&w->cubTemp, w->cubBytes));
     HIP_TRY(hipMalloc(&w->devDev, 2 * sizeof(m3Dev)));
     w->devDirty = 1;
     w->cur = 0;
@@ -4422,14 +4637,14 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     v.hasOffset, v.localPos, v.localRot, v.surfaceVel, v.category, v.mask,
                     v.group, v.shapeNext, v.hullIndex, v.leaf, v.fresh, v.planes,
                     (void*)v.hulls, w->keysBuf[0], w->keysBuf[1], w->mansBuf[0], w->mansBuf[1],
-                    v.sleepKeys, v.emit, v.satQueries, v.satList, w->dev.emitSorted, w->dev.emitUnique, v.cons, v.consPair,
+                    v.sleepKeys, v.emit, v.satQueries, v.satList, w->dev.emitUnique, v.cons, v.consPair,
                     v.colors, v.colorLists, v.movers, v.deltaPos, v.deltaRot, v.com0, v.rot0,
-                    v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut, w->dev.sapValsIn,
-                    w->dev.sapValsOut, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.blockCounts, w->dev.root, w->dev.narrowBins,
+                    v.parent, v.forced, v.flag, v.pending, v.sc, w->dev.sapKeysIn, w->dev.sapKeysOut,
+                    w->cubTemp, w->dev.grp, w->dev.grpKeys, w->dev.grpOf, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.blockCounts, w->dev.root, w->dev.narrowBins,
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
                     v.slotTable, v.slotMask, w->dev.barrier, w->dev.flowRec, w->dev.flowRank,
                     w->dev.warmAdd, w->dev.cellOf, w->dev.cellLocal, w->dev.cellBodies, w->dev.cellCount,
-                    w->dev.cellConsCount, w->dev.slotOwner, w->dev.cellCons, w->cubTemp, w->devDev,
+                    w->dev.cellConsCount, w->dev.slotOwner, w->dev.cellCons, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
                     v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
                     v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
@@ -5132,8 +5347,6 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     int32_t ns = v.maxShape;
     int32_t np = v.pairCapacity;
     int32_t ne = v.emitCapacity;
-    size_t tb = w->cubBytes;
-    void* ct = w->cubTemp;
 
     STAGE(ST_FILL, KL(k_open, dim3(Blocks(ne, T)), dim3(T), 0, st, D));
     // Broadphase.
@@ -5146,9 +5359,10 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
         STAGE(ST_DISCOVER_ALL, KL(k_pending_all, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
         w->fullQuery = 0;
     }
-    STAGE(ST_SAP_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortPairs(
-                           ct, tb, hd.sapKeysIn, hd.sapKeysOut, hd.sapValsIn, hd.sapValsOut, ns, 0,
-                           64, st)));
+    // rocPRIM's radix sort over the packed sweep keys: for ten thousand keys
+    // it beat every single-workgroup form tried here.
+    STAGE(ST_SAP_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(w->cubTemp, w->cubBytes, hd.sapKeysIn,
+                                                                 hd.sapKeysOut, ns, 0, 64, st)));
     STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
     STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, M3_SAP_SHAPES)), dim3(M3_SAP_THREADS), 0, st, D));
     if (v.planeCount > 0)
@@ -5164,10 +5378,11 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
         HIP_TRY(hipEventRecord(w->leavesCopied, st));
     }
     STAGE(ST_FROZEN, KL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, D));
-    STAGE(ST_EMIT_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(ct, tb, v.emit, hd.emitSorted,
-                                                                   ne, 0, 32, st)));
-    STAGE(ST_UNIQUE, HIP_TRY(hipcub::DeviceSelect::Unique(ct, tb, hd.emitSorted, hd.emitUnique,
-                                                          &v.sc->uniqueCount, ne, st)));
+    {
+        void* args[] = {(void*)&D};
+        STAGE(ST_EMIT_SORT, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_emit_pairs, dim3(M3_SORT_GRID),
+                                                               dim3(M3_SORT_THREADS), args, 0, st)));
+    }
     const int CT = M3_COMPACT_THREADS;
     STAGE(ST_EXPAND, KL(k_pairs_settle, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
     STAGE(ST_SLEEP_SELECT, KL(k_compact_frozen, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
@@ -5205,8 +5420,6 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
     const int T = 256;
     const int32_t nb = v.maxBody;
     const int32_t np = v.pairCapacity;
-    size_t tb = w->cubBytes;
-    void* ct = w->cubTemp;
     const bool vetoes = vetoCount > 0;
     if (vetoes != (w->view.veto != nullptr))
     {
