@@ -183,6 +183,7 @@ typedef const __attribute__((address_space(4))) m3Dev m3ConstDev;
 // Rank of a constraint side that is not an awake dynamic body (barrier-free
 // substep passes).
 #define M3_FLOW_NONE 0xFFu
+#define M3_FLOW_STALE 0xFFFFFFFEu // a record version no pass writes
 // Solve and relax passes of one step the versions cover (two per substep;
 // more substeps take the colored path).
 #define M3_FLOW_PASSES 16
@@ -317,6 +318,7 @@ __device__ static inline void StepBegin(const m3Dev& d)
     sc->jointActive = 0;
     sc->satCount = 0;
     sc->flowVeto = 0;
+    sc->flowKinematic = 0;
     if (d.narrowBins != nullptr)
     {
         for (int c = 0; c < 8; ++c)
@@ -2900,6 +2902,13 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
         const int32_t b = v.movers[m];
         const int32_t cell = xOf[CellBin(q.x, box[0], box[1])] * gz + zOf[CellBin(q.y, box[2], box[3])];
         const int32_t local = atomicAdd(&cnt[cell], 1);
+        if (d.flowRec != nullptr)
+        {
+            // No pass writes this version, so a record left by an earlier
+            // step (or never written) cannot pass for this step's warm start
+            // before its home cell stores it.
+            d.flowRec[6 * (int64_t)b + 5] = (uint64_t)M3_FLOW_STALE << 32;
+        }
         d.cellOf[b] = (uint8_t)cell;
         d.cellLocal[b] = (uint16_t)(local < M3_CELL_BODIES ? local : 0);
         if (local < M3_CELL_BODIES)
@@ -2933,6 +2942,10 @@ __global__ void k_cell_owner(const m3Dev* __restrict__ ds, int32_t cells)
     const int32_t body = c->dynA ? c->bodyA : c->bodyB;
     const uint8_t owner = (c->dynA || c->dynB) ? d.cellOf[body] : 0;
     d.slotOwner[k] = owner;
+    if ((!c->dynA && v.type[c->bodyA] == M3_KINEMATIC) || (!c->dynB && v.type[c->bodyB] == M3_KINEMATIC))
+    {
+        v.sc->flowKinematic = 1;
+    }
     const bool outside = (c->dynA && d.cellOf[c->bodyA] >= cells) || (c->dynB && d.cellOf[c->bodyB] >= cells);
     if (outside || owner >= cells)
     {
@@ -3474,6 +3487,7 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
     const int32_t nHome = d.cellCount[cell];
     const int32_t* home = d.cellBodies + cell * M3_CELL_BODIES;
     uint64_t* rec = d.flowRec;
+    const bool kinematic = v.sc->flowKinematic != 0;
     CellSetup(v, d, L, cell);
     for (int32_t sub = 0; sub < sp.substeps; ++sub)
     {
@@ -3500,14 +3514,28 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             CellPut(L, j, lin, ang, solvePass << 8);
             FlowStore(rec, b, lin, ang, solvePass << 8);
         }
+        // The solve pass needs no grid barrier first: a constraint waits for
+        // its bodies' warm-start versions, and positions did not change.
         CellArm(L, solvePass);
         {
             m3ContactIn first;
             CellFirst<1>(v, L, &first);
-            sync();
+            __syncthreads();
             CellPass<1>(v, sp, L, rec, d.flowRank, d.warmAdd, false, solvePass, &first);
         }
-        sync();
+        // Nor does integrate-positions: a dynamic body waits for its last
+        // solve, after which no solve anywhere reads its position deltas.  A
+        // kinematic body has no solve count, so a contact on one keeps the
+        // grid barrier.  The workgroup barrier keeps the mailboxes armed
+        // until the whole cell is through the pass.
+        if (kinematic)
+        {
+            sync();
+        }
+        else
+        {
+            __syncthreads();
+        }
         for (int32_t j = t; j < nHome; j += blockDim.x)
         {
             const int32_t b = home[j];

@@ -125,6 +125,7 @@ typedef struct m3Scalars
     int32_t jointActive; // joints prepared this step
     int32_t satCount;    // wide hull pairs listed for the wavefront SAT queries
     int32_t flowVeto;    // a constraint does not fit the barrier-free substep passes
+    int32_t flowKinematic; // a contact touches a kinematic body, whose position deltas move
     uint64_t stepCount;
 } m3Scalars;
 

@@ -14,6 +14,7 @@ import math
 
 from .api import (
     DYNAMIC,
+    KINEMATIC,
     STATIC,
     Api,
     BodyDef,
@@ -752,6 +753,49 @@ def event_rain(api: Api, count: int = 400, seed: int = 0xE7E47, extent: float = 
     return world
 
 
+def sweepers(api: Api, count: int = 1600, seed: int = 0x5EE9, extent: float = 12.0) -> WorldId:
+    """Kinematic paddles ploughing a pile: ``count`` boxes and spheres
+    dropped from 1-9 m onto a plane over [-extent, extent]^2, four
+    kinematic slabs sliding along x at 3 m/s while they spin about y, and a
+    kinematic post turning in place at the centre.  Every contact with a
+    kinematic body reads position deltas that the body's own substeps move
+    (src/solver.c:3938-4203 integrates kinematic movers with the rest)."""
+    world = _world(api, count + 16, count + 16)
+    _ground(api, world)
+    sd = api.m3DefaultShapeDef()
+    sd.density = 1000.0
+    sd.friction = 0.6
+    kd = api.m3DefaultBodyDef()
+    kd.type = KINEMATIC
+    for k in range(4):
+        kd.position = Pos3(-extent + 1.0, 0.7, -9.0 + 6.0 * k)
+        kd.linearVelocity = Vec3(3.0, 0.0, 0.0)
+        kd.angularVelocity = Vec3(0.0, 0.8 if k % 2 == 0 else -0.8, 0.0)
+        paddle = api.m3CreateBody(world, C.byref(kd))
+        _check(api.m3CreateBoxShape(paddle, C.byref(sd), Vec3(2.5, 0.6, 0.3)), "paddle")
+    kd.position = Pos3(0.0, 1.5, 0.0)
+    kd.linearVelocity = Vec3(0.0, 0.0, 0.0)
+    kd.angularVelocity = Vec3(0.0, 2.0, 0.0)
+    post = api.m3CreateBody(world, C.byref(kd))
+    _check(api.m3CreateBoxShape(post, C.byref(sd), Vec3(3.0, 1.5, 0.4)), "post")
+    bd = api.m3DefaultBodyDef()
+    bd.type = DYNAMIC
+    rng = SplitMix64(seed)
+    for k in range(count):
+        x = rng.range(-extent, extent)
+        y = rng.range(1.0, 9.0)
+        z = rng.range(-extent, extent)
+        bd.position = Pos3(x, y, z)
+        body = api.m3CreateBody(world, C.byref(bd))
+        if k % 2 == 0:
+            he = Vec3(rng.range(0.15, 0.4), rng.range(0.15, 0.4), rng.range(0.15, 0.4))
+            _check(api.m3CreateBoxShape(body, C.byref(sd), he), "box")
+        else:
+            ball = Sphere(Vec3(0.0, 0.0, 0.0), rng.range(0.15, 0.4))
+            _check(api.m3CreateSphereShape(body, C.byref(sd), C.byref(ball)), "sphere")
+    return world
+
+
 SCENES = {
     "bench_pyramid": bench_pyramid,
     "smoke10k": smoke10k,
@@ -765,4 +809,5 @@ SCENES = {
     "hulljam": hulljam,
     "bounce": bounce,
     "ccd_arena": ccd_arena,
+    "sweepers": sweepers,
 }

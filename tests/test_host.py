@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
     assert missing == []
 
 
-@pytest.mark.parametrize("scene", ["bench_pyramid", "box_pyramid", "rain", "smoke10k"])
+@pytest.mark.parametrize("scene", ["bench_pyramid", "box_pyramid", "rain", "smoke10k", "sweepers"])
 def test_initial_world_hash_matches_reference(scene):
     """Creation-time parity: ids, mass properties, box hulls, shape records
     and the curated hash all agree with the reference before any step."""
