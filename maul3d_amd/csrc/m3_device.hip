@@ -78,6 +78,7 @@ struct m3Dev
     uint64_t* flowRec;
     uint32_t* flowRank;
     m3WarmAdd* warmAdd; // per slot and side: the next warm start's addends (barrier-free form)
+    uint64_t* deltaRec; // per body: versioned position and rotation deltas
     // Cells of the barrier-free form: per mover its cell and index there,
     // per cell its movers and counts, per slot the solving cell.
     uint8_t* cellOf;
@@ -184,6 +185,10 @@ typedef const __attribute__((address_space(4))) m3Dev m3ConstDev;
 // substep passes).
 #define M3_FLOW_NONE 0xFFu
 #define M3_FLOW_STALE 0xFFFFFFFEu // a record version no pass writes
+// Per body, the position and rotation deltas integrate-positions left
+// (seven versioned words, one unused): the passes that follow read them
+// there, since no grid barrier separates them from the writer.
+#define M3_DELTA_WORDS 8
 // Solve and relax passes of one step the versions cover (two per substep;
 // more substeps take the colored path).
 #define M3_FLOW_PASSES 16
@@ -2094,6 +2099,20 @@ __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_mover_flags(const m3Dev*
     {
         flags[i] = f;
     }
+    if (f != 0 && d.flowRec != nullptr)
+    {
+        // The barrier-free passes' records of a mover: no pass writes this
+        // version, so no word left by an earlier step (or never written)
+        // can pass for one of this step's.
+        for (int w = 0; w < 6; ++w)
+        {
+            d.flowRec[6 * (int64_t)i + w] = (uint64_t)M3_FLOW_STALE << 32;
+        }
+        for (int w = 0; w < M3_DELTA_WORDS; ++w)
+        {
+            d.deltaRec[M3_DELTA_WORDS * (int64_t)i + w] = (uint64_t)M3_FLOW_STALE << 32;
+        }
+    }
     FlagCount(f, d.blockCounts);
 }
 
@@ -2714,6 +2733,46 @@ __device__ static inline uint32_t FlowLoad(const uint64_t* rec, int32_t b, float
     return same ? ver : 0xFFFFFFFFu;
 }
 
+// A pass whose position deltas have not moved since the step began reads
+// them directly.
+#define M3_DELTA_FIXED 0xFFFFFFFFu
+
+__device__ static inline void DeltaStore(uint64_t* rec, int32_t b, m3Vec3 dp, m3Quat dq, uint32_t ver)
+{
+    uint64_t* r = rec + M3_DELTA_WORDS * (int64_t)b;
+    const float x[7] = {dp.x, dp.y, dp.z, dq.x, dq.y, dq.z, dq.w};
+    M3_UNROLL
+    for (int j = 0; j < 7; ++j)
+    {
+        __hip_atomic_store(&r[j], FlowWord(x[j], ver), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// True when all seven words carry version ver; then dp and dq hold them.
+__device__ static inline bool DeltaLoad(const uint64_t* rec, int32_t b, uint32_t ver, m3Vec3* dp, m3Quat* dq)
+{
+    const uint64_t* r = rec + M3_DELTA_WORDS * (int64_t)b;
+    uint64_t w[7];
+    M3_UNROLL
+    for (int j = 0; j < 7; ++j)
+    {
+        w[j] = __hip_atomic_load(&r[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    bool same = true;
+    M3_UNROLL
+    for (int j = 0; j < 7; ++j)
+    {
+        same = same && (uint32_t)(w[j] >> 32) == ver;
+    }
+    *dp = m3V3(__uint_as_float((uint32_t)w[0]), __uint_as_float((uint32_t)w[1]), __uint_as_float((uint32_t)w[2]));
+    dq->x = __uint_as_float((uint32_t)w[3]);
+    dq->y = __uint_as_float((uint32_t)w[4]);
+    dq->z = __uint_as_float((uint32_t)w[5]);
+    dq->w = __uint_as_float((uint32_t)w[6]);
+    return same;
+}
+
+
 // Quantile cells over the movers: gx slabs of equal mover count along x by
 // gz of equal count along z.  Any partition gives the same results (the
 // per-body operation order does not depend on it); a compact one keeps
@@ -2902,13 +2961,6 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
         const int32_t b = v.movers[m];
         const int32_t cell = xOf[CellBin(q.x, box[0], box[1])] * gz + zOf[CellBin(q.y, box[2], box[3])];
         const int32_t local = atomicAdd(&cnt[cell], 1);
-        if (d.flowRec != nullptr)
-        {
-            // No pass writes this version, so a record left by an earlier
-            // step (or never written) cannot pass for this step's warm start
-            // before its home cell stores it.
-            d.flowRec[6 * (int64_t)b + 5] = (uint64_t)M3_FLOW_STALE << 32;
-        }
         d.cellOf[b] = (uint8_t)cell;
         d.cellLocal[b] = (uint16_t)(local < M3_CELL_BODIES ? local : 0);
         if (local < M3_CELL_BODIES)
@@ -3117,12 +3169,15 @@ __device__ static inline bool FlowGiveUp(uint32_t* spins, uint64_t* since, int32
 
 // One pass (kMode 1 solve, 2 relax) of one cell.  Solver threads take the
 // cell's constraints in slot order; the mail wavefront fills the
-// mailboxes.  A relax pass that precedes another substep (`addends`) also
-// leaves each constraint's warm-start addends for that substep.
+// mailboxes.  Dynamic sides read their position deltas from the delta
+// records at version deltaVer (M3_DELTA_FIXED: directly).  A relax pass
+// that precedes another substep (`addends`) also leaves each constraint's
+// warm-start addends for that substep.
 template <int kMode>
 __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& sp, m3CellLds& L,
-                                       uint64_t* rec, const uint32_t* rank, m3WarmAdd* warm,
-                                       bool addends, uint32_t pass, const m3ContactIn* first)
+                                       uint64_t* rec, const uint32_t* rank, const uint64_t* drec,
+                                       m3WarmAdd* warm, uint32_t deltaVer, bool addends, uint32_t pass,
+                                       const m3ContactIn* first)
 {
     const uint32_t base = pass << 8;
     int32_t* fault = &v.sc->fault;
@@ -3213,6 +3268,41 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
     uint32_t wantA = 0, wantB = 0;
     uint32_t fl = 0;
     int32_t locA = 0, locB = 0, ownA = 0, ownB = 0;
+    bool biased = false;
+    // The bias terms, once both sides' deltas are in.
+    auto bias = [&]() -> bool {
+        m3Vec3 dpA, dpB;
+        m3Quat dqA, dqB;
+        if (deltaVer == M3_DELTA_FIXED || !in.dynA)
+        {
+            dpA = v.deltaPos[in.a];
+            dqA = v.deltaRot[in.a];
+        }
+        else if (!DeltaLoad(drec, in.a, deltaVer, &dpA, &dqA))
+        {
+            return false;
+        }
+        if (deltaVer == M3_DELTA_FIXED || !in.dynB)
+        {
+            dpB = v.deltaPos[in.b];
+            dqB = v.deltaRot[in.b];
+        }
+        else if (!DeltaLoad(drec, in.b, deltaVer, &dpB, &dqB))
+        {
+            return false;
+        }
+        M3_UNROLL
+        for (int32_t q = 0; q < M3_MANIFOLD_MAX_POINTS; ++q)
+        {
+            cb.bias[q] = 0.0f;
+            cb.massScale[q] = 1.0f;
+            cb.impulseScale[q] = 0.0f;
+        }
+        const int32_t slots =
+            in.pointCount < M3_MANIFOLD_MAX_POINTS ? in.pointCount : M3_MANIFOLD_MAX_POINTS;
+        m3ContactBiasTerms(&in, dpA, dqA, dpB, dqB, sp.invH, sp.push, kMode == 1, slots, &cb);
+        return true;
+    };
     auto take = [&](bool loaded) {
         k = L.slot[i];
         if (!loaded)
@@ -3232,17 +3322,7 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
         sAngA = v.av[in.a];
         sLinB = v.lv[in.b];
         sAngB = v.av[in.b];
-        M3_UNROLL
-        for (int32_t q = 0; q < M3_MANIFOLD_MAX_POINTS; ++q)
-        {
-            cb.bias[q] = 0.0f;
-            cb.massScale[q] = 1.0f;
-            cb.impulseScale[q] = 0.0f;
-        }
-        const int32_t slots =
-            in.pointCount < M3_MANIFOLD_MAX_POINTS ? in.pointCount : M3_MANIFOLD_MAX_POINTS;
-        m3ContactBiasTerms(&in, v.deltaPos[in.a], v.deltaRot[in.a], v.deltaPos[in.b], v.deltaRot[in.b],
-                           sp.invH, sp.push, kMode == 1, slots, &cb);
+        biased = bias();
     };
     if (have)
     {
@@ -3254,10 +3334,14 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
     while (__any(have))
     {
         bool progressed = false;
+        if (have && !biased)
+        {
+            biased = bias();
+        }
         if (have)
         {
             // A side reads its cell body (bit 0 / 2) or its mailbox.
-            bool ready = true;
+            bool ready = biased;
             m3PairVelocity s;
             s.vA = sLinA;
             s.wA = sAngA;
@@ -3487,6 +3571,10 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
     const int32_t nHome = d.cellCount[cell];
     const int32_t* home = d.cellBodies + cell * M3_CELL_BODIES;
     uint64_t* rec = d.flowRec;
+    int32_t* fault = &v.sc->fault;
+    // A kinematic body has no solve count, and contacts read its deltas
+    // directly: a contact on one keeps the grid barriers around
+    // integrate-positions.
     const bool kinematic = v.sc->flowKinematic != 0;
     CellSetup(v, d, L, cell);
     for (int32_t sub = 0; sub < sp.substeps; ++sub)
@@ -3498,7 +3586,7 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             if (sub > 0)
             {
                 m3Vec3 lin, ang;
-                CellFinal(L, rec, j, b, solvePass - 1u, &lin, &ang, &v.sc->fault);
+                CellFinal(L, rec, j, b, solvePass - 1u, &lin, &ang, fault);
                 v.lv[b] = lin;
                 v.av[b] = ang;
             }
@@ -3514,20 +3602,19 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             CellPut(L, j, lin, ang, solvePass << 8);
             FlowStore(rec, b, lin, ang, solvePass << 8);
         }
-        // The solve pass needs no grid barrier first: a constraint waits for
-        // its bodies' warm-start versions, and positions did not change.
+        // No grid barrier opens a pass: a constraint waits for its bodies'
+        // versions.  The solve pass reads the deltas directly (the last
+        // grid barrier followed their writes); the relax pass reads them
+        // from the delta records.  The workgroup barriers keep the
+        // mailboxes armed until the whole cell is through a pass.
         CellArm(L, solvePass);
         {
             m3ContactIn first;
             CellFirst<1>(v, L, &first);
             __syncthreads();
-            CellPass<1>(v, sp, L, rec, d.flowRank, d.warmAdd, false, solvePass, &first);
+            CellPass<1>(v, sp, L, rec, d.flowRank, d.deltaRec, d.warmAdd, M3_DELTA_FIXED, false, solvePass,
+                        &first);
         }
-        // Nor does integrate-positions: a dynamic body waits for its last
-        // solve, after which no solve anywhere reads its position deltas.  A
-        // kinematic body has no solve count, so a contact on one keeps the
-        // grid barrier.  The workgroup barrier keeps the mailboxes armed
-        // until the whole cell is through the pass.
         if (kinematic)
         {
             sync();
@@ -3536,14 +3623,17 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
         {
             __syncthreads();
         }
+        // A dynamic body waits for its last solve, after which no solve
+        // anywhere reads its deltas.
         for (int32_t j = t; j < nHome; j += blockDim.x)
         {
             const int32_t b = home[j];
             m3Vec3 lin, ang;
-            CellFinal(L, rec, j, b, solvePass, &lin, &ang, &v.sc->fault);
+            CellFinal(L, rec, j, b, solvePass, &lin, &ang, fault);
             v.lv[b] = lin;
             v.av[b] = ang;
             m3IntegratePosition(&v, b, sp.h);
+            DeltaStore(d.deltaRec, b, v.deltaPos[b], v.deltaRot[b], (solvePass + 1u) << 8);
             CellPut(L, j, v.lv[b], v.av[b], (solvePass + 1u) << 8);
             FlowStore(rec, b, v.lv[b], v.av[b], (solvePass + 1u) << 8);
         }
@@ -3551,17 +3641,25 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
         {
             m3ContactIn first;
             CellFirst<2>(v, L, &first);
-            sync();
-            CellPass<2>(v, sp, L, rec, d.flowRank, d.warmAdd, sub + 1 < sp.substeps, solvePass + 1u,
-                        &first);
+            if (kinematic)
+            {
+                sync();
+            }
+            else
+            {
+                __syncthreads();
+            }
+            CellPass<2>(v, sp, L, rec, d.flowRank, d.deltaRec, d.warmAdd, (solvePass + 1u) << 8,
+                        sub + 1 < sp.substeps, solvePass + 1u, &first);
         }
+        // The next warm start reads the addends the relax pass left.
         sync();
     }
     for (int32_t j = t; j < nHome; j += blockDim.x)
     {
         const int32_t b = home[j];
         m3Vec3 lin, ang;
-        CellFinal(L, rec, j, b, 2u * (uint32_t)sp.substeps - 1u, &lin, &ang, &v.sc->fault);
+        CellFinal(L, rec, j, b, 2u * (uint32_t)sp.substeps - 1u, &lin, &ang, fault);
         v.lv[b] = lin;
         v.av[b] = ang;
     }
@@ -4524,6 +4622,7 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&w->dev.barrier, 4) || DevAlloc(&w->dev.narrowBins, 8 + 8 * (int64_t)np) ||
         DevAlloc(&w->dev.flowRec, 6 * (int64_t)nb) || DevAlloc(&w->dev.flowRank, np) ||
         DevAlloc(&w->dev.warmAdd, 2 * (int64_t)np) ||
+        DevAlloc(&w->dev.deltaRec, M3_DELTA_WORDS * (int64_t)nb) ||
         DevAlloc(&w->dev.cellOf, nb) || DevAlloc(&w->dev.cellLocal, nb) ||
         DevAlloc(&w->dev.cellBodies, (int64_t)M3_CELL_MAX * M3_CELL_BODIES) ||
         DevAlloc(&w->dev.cellCount, M3_CELL_MAX) || DevAlloc(&w->dev.cellConsCount, M3_CELL_MAX) ||
@@ -4671,7 +4770,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     w->cubTemp, w->dev.grp, w->dev.grpKeys, w->dev.grpOf, w->dev.sapXY, w->dev.sapZ, w->dev.sapAttr, w->dev.pairFlags, w->dev.bodyFlags, w->dev.blockCounts, w->dev.root, w->dev.narrowBins,
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
                     v.slotTable, v.slotMask, w->dev.barrier, w->dev.flowRec, w->dev.flowRank,
-                    w->dev.warmAdd, w->dev.cellOf, w->dev.cellLocal, w->dev.cellBodies, w->dev.cellCount,
+                    w->dev.warmAdd, w->dev.deltaRec, w->dev.cellOf, w->dev.cellLocal, w->dev.cellBodies, w->dev.cellCount,
                     w->dev.cellConsCount, w->dev.slotOwner, w->dev.cellCons, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
                     v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
