@@ -53,7 +53,7 @@ static int Fail(const char* what, hipError_t e)
 struct m3Dev
 {
     m3View v;
-    uint64_t* sapKeysIn;  // per shape: its sweep key (k_fresh)
+    uint64_t* sapKeysIn;  // per shape: its sweep key (k_open)
     uint64_t* sapKeysOut; // the keys sorted
     float4* sapXY;
     float2* sapZ;
@@ -88,6 +88,7 @@ struct m3Dev
     int32_t* cellConsCount;
     int32_t* cellCons; // per cell: its slots (any order)
     uint8_t* slotOwner;
+    int2* slotBodies; // per slot: its awake dynamic bodies (or -1), for the cell owners
     m3Tape* tape;      // intermediate-value record of this step (adjoint replays), or null
     // Pairs binned by collision routine: 8 counts, then one bin of
     // pairCapacity entries per routine.
@@ -143,6 +144,7 @@ struct m3gpuWorld
     m3Scalars* hostScalars; // pinned mirror for stats
     struct m3Profiler* prof;
     int coopBlocks;  // co-resident blocks of the substep kernel (maximum)
+    int coopCompact; // co-resident 256-thread blocks of k_islands, k_constraints and k_movers
     int coopFixed;   // coopBlocks set from the environment or tuning: launch exactly that many
     int coopDefault; // coopBlocks as created (restored when tuning is cleared)
     int coopMax;     // co-resident limit of the cooperative launch
@@ -333,8 +335,11 @@ __device__ static inline void StepBegin(const m3Dev& d)
     }
 }
 
+__device__ static inline void FreshShape(const m3Dev& d, int32_t s);
+
 // The step's opening: thread 0 resets the step's scalars, every thread
-// fills its slot of the emission buffer with the sentinel.
+// fills its slot of the emission buffer with the sentinel, and each shape's
+// fresh box and sweep key are computed (none of these reads another).
 __global__ void k_open(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
@@ -342,6 +347,10 @@ __global__ void k_open(const m3Dev* __restrict__ ds)
     if (i == 0)
     {
         StepBegin(d);
+    }
+    if (i < v.maxShape)
+    {
+        FreshShape(d, i);
     }
     if (i < v.emitCapacity)
     {
@@ -369,9 +378,12 @@ __global__ void k_open(const m3Dev* __restrict__ ds)
 // ---------------------------------------------------------------------------
 
 #define M3_COMPACT_THREADS 256
+// Workgroups of a cooperative compaction (k_islands, k_constraints,
+// k_movers): the grid barrier's arrivals serialise on one counter.
+#define M3_COOP_COMPACT_BLOCKS 64
 
 // Every thread of the workgroup calls it.
-__device__ static inline void FlagCount(int flag, int32_t* blockCounts)
+__device__ static inline void FlagCount(int flag, int32_t* blockCounts, int32_t chunk)
 {
     __shared__ int32_t waveCount[M3_COMPACT_THREADS / 64];
     const uint64_t mask = __ballot(flag);
@@ -389,19 +401,19 @@ __device__ static inline void FlagCount(int flag, int32_t* blockCounts)
         {
             n += waveCount[k];
         }
-        blockCounts[blockIdx.x] = n;
+        blockCounts[chunk] = n;
     }
 }
 
 // This thread's output position if its flag is set (every thread calls it);
 // *total receives the count over all workgroups.
-__device__ static inline int32_t FlagRank(int flag, const int32_t* blockCounts, int32_t* total)
+__device__ static inline int32_t FlagRank(int flag, const int32_t* blockCounts, int32_t chunk, int32_t* total)
 {
     __shared__ int32_t partial[M3_COMPACT_THREADS];
     __shared__ int32_t waveCount[M3_COMPACT_THREADS / 64];
     const int32_t t = threadIdx.x;
     int32_t sum = 0;
-    for (int32_t k = t; k < (int32_t)blockIdx.x; k += M3_COMPACT_THREADS)
+    for (int32_t k = t; k < chunk; k += M3_COMPACT_THREADS)
     {
         sum += blockCounts[k];
     }
@@ -438,14 +450,9 @@ __device__ static inline int32_t FlagRank(int flag, const int32_t* blockCounts, 
 }
 
 // Fresh bounds of every alive non-plane shape and its sweep key.
-__global__ void k_fresh(const m3Dev* __restrict__ ds)
+__device__ static inline void FreshShape(const m3Dev& d, int32_t s)
 {
-    M3_WORLD;
-    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= v.maxShape)
-    {
-        return;
-    }
+    const m3View& v = d.v;
     uint32_t u = M3_SAP_OUT;
     if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE)
     {
@@ -842,7 +849,7 @@ __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_pairs_settle(const m3Dev
         }
         d.pairFlags[i] = cold;
     }
-    FlagCount(cold, d.blockCounts);
+    FlagCount(cold, d.blockCounts, blockIdx.x);
 }
 
 // The cold keys of the pair list, in order, become the frozen buffer.
@@ -852,7 +859,7 @@ __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_frozen(const m3D
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int flag = i < v.pairCapacity && d.pairFlags[i] != 0;
     int32_t total;
-    const int32_t pos = FlagRank(flag, d.blockCounts, &total);
+    const int32_t pos = FlagRank(flag, d.blockCounts, blockIdx.x, &total);
     if (flag)
     {
         v.sleepKeys[pos] = v.keys[i];
@@ -1540,48 +1547,51 @@ __global__ void k_union_joints(const m3Dev* __restrict__ ds)
     }
 }
 
+// Body i's island root; an awake or forced dynamic body marks its island
+// for waking (the mark needs only the body's own root).
+__device__ static inline void RootMark(const m3Dev& d, int32_t i)
+{
+    const m3View& v = d.v;
+    const int32_t r = Find(v.parent, i);
+    d.root[i] = r;
+    v.slotMask[i] = 0u;
+    if (v.bodyAlive[i] != 0 && v.type[i] == M3_DYNAMIC && (v.awake[i] != 0 || v.forced[i] != 0))
+    {
+        v.flag[r] = 1;
+    }
+}
+
+// Every dynamic body of a marked island wakes.
+__device__ static inline void WakeApply(const m3Dev& d, int32_t i)
+{
+    const m3View& v = d.v;
+    if (v.bodyAlive[i] != 0 && v.type[i] == M3_DYNAMIC && v.awake[i] == 0 && v.flag[d.root[i]] != 0)
+    {
+        v.awake[i] = 1;
+        v.sleepTime[i] = 0.0f;
+    }
+}
+
 __global__ void k_root(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= v.maxBody || v.sc->stall != 0)
     {
         return;
     }
-    root[i] = Find(v.parent, i);
-    v.slotMask[i] = 0u;
-}
-
-__global__ void k_wake_mark(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    const int32_t* root = d.root;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC)
-    {
-        return;
-    }
-    if (v.awake[i] != 0 || v.forced[i] != 0)
-    {
-        v.flag[root[i]] = 1;
-    }
+    RootMark(d, i);
 }
 
 __global__ void k_wake_apply(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC)
+    if (i >= v.maxBody || v.sc->stall != 0)
     {
         return;
     }
-    if (v.awake[i] == 0 && v.flag[root[i]] != 0)
-    {
-        v.awake[i] = 1;
-        v.sleepTime[i] = 0.0f;
-    }
+    WakeApply(d, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -1656,29 +1666,36 @@ __global__ void k_veto_mark(const m3Dev* __restrict__ ds, const uint64_t* keys, 
     flags[i] = hit;
 }
 
-__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_constraint_flags(const m3Dev* __restrict__ ds)
+__device__ static inline uint8_t ConstraintFlag(const m3Dev& d, int32_t i)
 {
-    M3_WORLD;
-    uint8_t* flags = d.pairFlags;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const m3View& v = d.v;
     uint8_t f = 0;
     if (i < v.pairCapacity)
     {
         f = (v.sc->stall == 0 && i < v.sc->pairCount && m3PairIsConstraint(&v, i)) ? 1 : 0;
-        flags[i] = f;
+        d.pairFlags[i] = f;
     }
-    FlagCount(f, d.blockCounts);
+    return f;
+}
+
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_constraint_flags(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    const uint8_t f = ConstraintFlag(d, blockIdx.x * blockDim.x + threadIdx.x);
+    FlagCount(f, d.blockCounts, blockIdx.x);
 }
 
 // The constraint list in pair order, with the awake dynamic bodies of each
 // constraint, the only bodies coloring tracks (src/solver.c:2741-2822).
-__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_constraints(const m3Dev* __restrict__ ds)
+// Chunk `chunk` of the constraint list (constraint c: its pair and its
+// awake dynamic bodies, or -1); the last chunk's thread 0 writes the count.
+__device__ static inline void CompactConstraints(const m3Dev& d, int32_t chunk, int32_t chunks)
 {
-    M3_WORLD;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const m3View& v = d.v;
+    const int32_t i = chunk * M3_COMPACT_THREADS + threadIdx.x;
     const int flag = i < v.pairCapacity && d.pairFlags[i] != 0;
     int32_t total;
-    const int32_t c = FlagRank(flag, d.blockCounts, &total);
+    const int32_t c = FlagRank(flag, d.blockCounts, chunk, &total);
     if (flag)
     {
         v.consPair[c] = i;
@@ -1688,10 +1705,16 @@ __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_constraints(cons
         v.consBodies[2 * c] = (v.type[a] == M3_DYNAMIC && v.awake[a] != 0) ? a : -1;
         v.consBodies[2 * c + 1] = (v.type[b] == M3_DYNAMIC && v.awake[b] != 0) ? b : -1;
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    if (chunk == chunks - 1 && threadIdx.x == 0)
     {
         v.sc->constraintCount = total;
     }
+}
+
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_constraints(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    CompactConstraints(d, blockIdx.x, gridDim.x);
 }
 
 // The awake dynamic bodies of each constraint, the only bodies coloring
@@ -1763,12 +1786,21 @@ __global__ void k_prepare(const m3Dev* __restrict__ ds, float hertz, float zeta,
     {
         // The barrier-free passes track awake dynamic bodies only, and the
         // rolling row writes a non-dynamic side's angular velocity.
-        const bool awakeA = v.consBodies[2 * c] >= 0;
-        const bool awakeB = v.consBodies[2 * c + 1] >= 0;
+        const int2 ab = make_int2(v.consBodies[2 * c], v.consBodies[2 * c + 1]);
+        d.slotBodies[k] = ab;
+        const bool awakeA = ab.x >= 0;
+        const bool awakeB = ab.y >= 0;
         if ((con->dynA != 0) != awakeA || (con->dynB != 0) != awakeB ||
             (con->rollingResistance > 0.0f && (!con->dynA || !con->dynB)))
         {
             v.sc->flowVeto = 1;
+        }
+        // A kinematic side's position deltas move during the substeps
+        // without a solve count of their own.
+        if ((!con->dynA && v.type[con->bodyA] == M3_KINEMATIC) ||
+            (!con->dynB && v.type[con->bodyB] == M3_KINEMATIC))
+        {
+            v.sc->flowKinematic = 1;
         }
         m3Vec3 rA[M3_MANIFOLD_MAX_POINTS];
         m3Vec3 rB[M3_MANIFOLD_MAX_POINTS];
@@ -2077,11 +2109,12 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
 // Movers and the substep loop (src/solver.c:3938-4203)
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_mover_flags(const m3Dev* __restrict__ ds, float dt)
+// Body i's mover flag (kinematic, or dynamic and awake); a kinematic body
+// with a target gets its servo velocities.
+__device__ static inline uint8_t MoverFlag(const m3Dev& d, int32_t i, float dt)
 {
-    M3_WORLD;
+    const m3View& v = d.v;
     uint8_t* flags = d.bodyFlags;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint8_t f = 0;
     if (i < v.maxBody && v.sc->stall == 0 && v.bodyAlive[i] != 0 && v.enabled[i] != 0)
     {
@@ -2113,25 +2146,38 @@ __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_mover_flags(const m3Dev*
             d.deltaRec[M3_DELTA_WORDS * (int64_t)i + w] = (uint64_t)M3_FLOW_STALE << 32;
         }
     }
-    FlagCount(f, d.blockCounts);
+    return f;
+}
+
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_mover_flags(const m3Dev* __restrict__ ds, float dt)
+{
+    M3_WORLD;
+    const uint8_t f = MoverFlag(d, blockIdx.x * blockDim.x + threadIdx.x, dt);
+    FlagCount(f, d.blockCounts, blockIdx.x);
 }
 
 // The movers (solved bodies) in slot order.
-__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_movers(const m3Dev* __restrict__ ds)
+__device__ static inline void CompactMovers(const m3Dev& d, int32_t chunk, int32_t chunks)
 {
-    M3_WORLD;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const m3View& v = d.v;
+    const int32_t i = chunk * M3_COMPACT_THREADS + threadIdx.x;
     const int flag = i < v.maxBody && d.bodyFlags[i] != 0;
     int32_t total;
-    const int32_t pos = FlagRank(flag, d.blockCounts, &total);
+    const int32_t pos = FlagRank(flag, d.blockCounts, chunk, &total);
     if (flag)
     {
         v.movers[pos] = i;
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0)
+    if (chunk == chunks - 1 && threadIdx.x == 0)
     {
         v.sc->moverCount = total;
     }
+}
+
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_movers(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    CompactMovers(d, blockIdx.x, gridDim.x);
 }
 
 struct m3SubstepParams
@@ -2317,6 +2363,76 @@ struct m3BlockSync
 {
     __device__ void operator()() const { __syncthreads(); }
 };
+
+// ---------------------------------------------------------------------------
+// Cooperative forms of the island pass and the two flag-and-compact pairs of
+// the single-world step: one launch each, a grid barrier between the
+// passes.  Chunk c (items 256 c .. 256 c + 255) belongs to workgroup
+// c % gridDim.x, so any co-resident grid computes the same result.
+// ---------------------------------------------------------------------------
+
+__device__ static inline int32_t Chunks(int32_t n)
+{
+    return n > 0 ? (n + M3_COMPACT_THREADS - 1) / M3_COMPACT_THREADS : 1;
+}
+
+// Roots and wake marks, then the wake (src/solver.c:3532-3580).
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_islands(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    const m3GridSync sync = {d.barrier, &v.sc->fault};
+    const bool live = v.sc->stall == 0;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; live && i < v.maxBody; i += stride)
+    {
+        RootMark(d, i);
+    }
+    sync();
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; live && i < v.maxBody; i += stride)
+    {
+        WakeApply(d, i);
+    }
+}
+
+// The step's constraints: flags, then the ordered compaction.
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_constraints(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    const m3GridSync sync = {d.barrier, &v.sc->fault};
+    const int32_t chunks = Chunks(v.pairCapacity);
+    for (int32_t c = blockIdx.x; c < chunks; c += gridDim.x)
+    {
+        const uint8_t f = ConstraintFlag(d, c * M3_COMPACT_THREADS + threadIdx.x);
+        FlagCount(f, d.blockCounts, c);
+        __syncthreads();
+    }
+    sync();
+    for (int32_t c = blockIdx.x; c < chunks; c += gridDim.x)
+    {
+        CompactConstraints(d, c, chunks);
+        __syncthreads();
+    }
+}
+
+// The step's movers: flags (and kinematic servos), then the compaction.
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_movers(const m3Dev* __restrict__ ds, float dt)
+{
+    M3_WORLD;
+    const m3GridSync sync = {d.barrier, &v.sc->fault};
+    const int32_t chunks = Chunks(v.maxBody);
+    for (int32_t c = blockIdx.x; c < chunks; c += gridDim.x)
+    {
+        const uint8_t f = MoverFlag(d, c * M3_COMPACT_THREADS + threadIdx.x, dt);
+        FlagCount(f, d.blockCounts, c);
+        __syncthreads();
+    }
+    sync();
+    for (int32_t c = blockIdx.x; c < chunks; c += gridDim.x)
+    {
+        CompactMovers(d, c, chunks);
+        __syncthreads();
+    }
+}
 
 // On a grid, the last colors of a pass are small (the greedy coloring's
 // sizes fall off quickly): the trailing colors holding at most
@@ -2667,7 +2783,7 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
 // Where a body's velocity travels decides the cost of that wait, so the
 // island is cut into cells, one workgroup each (k_cells: a quantile grid
 // over the movers' x and z).  A body lives in its cell's LDS; a constraint
-// belongs to the cell of one of its dynamic bodies (k_cell_owner).  When
+// belongs to the cell of one of its dynamic bodies (k_cells).  When
 // the previous constraint on a body was solved in the same cell, the wait
 // is on an LDS version.  Otherwise the body's velocity comes from its six
 // global words: each holds one float and the version, written and read
@@ -2684,9 +2800,10 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
 // co-resident (cooperative launch), so the lowest unsolved slot is always
 // ready and the passes end.  Waits are bounded all the same: one that
 // gives up marks the world faulted, and every wait ends once it sees the
-// fault.  The grid still meets between phases (integrate velocities and
-// warm start, the solve pass, integrate positions, the relax pass), where
-// positions change.
+// fault.  The grid meets once per substep, after the relax pass, whose
+// warm-start addends the next substep reads; the relax pass takes the
+// position deltas that integrate-positions left from versioned per-body
+// records.
 // ---------------------------------------------------------------------------
 // A wait gives up after this much wall time (s_memrealtime runs at
 // 100 MHz): far beyond any step, so only a broken wait reaches it.
@@ -2783,6 +2900,7 @@ __device__ static inline bool DeltaLoad(const uint64_t* rec, int32_t b, uint32_t
 #define M3_CELL_BINS 1024   // histogram bins per axis (one per thread of k_cells)
 #define M3_CELL_GX_MAX 16
 #define M3_CELL_CACHE 12288 // movers whose x, z k_cells keeps in LDS
+#define M3_CELL_LDS_BODIES 32768 // bodies whose cell k_cells keeps in LDS
 
 __device__ static inline int32_t CellBin(float x, float lo, float scale)
 {
@@ -2855,14 +2973,16 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
     __shared__ uint8_t zOf[M3_CELL_BINS];
     __shared__ int32_t totals[M3_CELL_BINS / 64];
     __shared__ int32_t cnt[M3_CELL_MAX];
+    __shared__ int32_t consCnt[M3_CELL_MAX];
     __shared__ float2 pos[M3_CELL_CACHE];
+    __shared__ uint8_t cellOfL[M3_CELL_LDS_BODIES];
     const int32_t t = threadIdx.x;
     const int32_t cells = gx * gz;
     const int32_t n = v.sc->stall != 0 ? 0 : v.sc->moverCount;
     if (t < M3_CELL_MAX)
     {
         cnt[t] = 0;
-        d.cellConsCount[t] = 0;
+        consCnt[t] = 0;
     }
     hx[t] = 0;
     hz[t] = 0;
@@ -2870,6 +2990,10 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
     for (int32_t b = t; b < v.maxBody; b += blockDim.x)
     {
         d.cellOf[b] = 0xFF;
+        if (b < M3_CELL_LDS_BODIES)
+        {
+            cellOfL[b] = 0xFF;
+        }
     }
     auto at = [&](int32_t m) -> float2 {
         if (m < M3_CELL_CACHE)
@@ -2962,12 +3086,20 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
         const int32_t cell = xOf[CellBin(q.x, box[0], box[1])] * gz + zOf[CellBin(q.y, box[2], box[3])];
         const int32_t local = atomicAdd(&cnt[cell], 1);
         d.cellOf[b] = (uint8_t)cell;
+        if (b < M3_CELL_LDS_BODIES)
+        {
+            cellOfL[b] = (uint8_t)cell;
+        }
         d.cellLocal[b] = (uint16_t)(local < M3_CELL_BODIES ? local : 0);
         if (local < M3_CELL_BODIES)
         {
             d.cellBodies[cell * M3_CELL_BODIES + local] = b;
         }
     }
+    // Bodies past the LDS copy are read back from global memory: every
+    // wavefront's stores complete before the barrier, and the reads are
+    // device-coherent.
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     if (t < cells)
     {
@@ -2977,40 +3109,68 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
             v.sc->flowVeto = 1;
         }
     }
-}
-
-// The cell that solves each slot: the cell of its body A when A is dynamic,
-// else of body B.  A dynamic side outside every cell, or a cell over its
-// capacity, sends the step to the colored form.
-__global__ void k_cell_owner(const m3Dev* __restrict__ ds, int32_t cells)
-{
-    M3_WORLD;
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || k >= v.sc->colorStarts[M3_GRAPH_COLORS])
+    // The cell that solves each slot: the cell of its body A when A is
+    // dynamic, else of body B.  A dynamic side outside every cell, or a
+    // cell over its constraint capacity, sends the step to the colored
+    // form.  A cell's list is in no particular order (the substep kernel
+    // sorts it).
+    auto cellOfBody = [&](int32_t b) -> uint32_t {
+        if (b < 0)
+        {
+            return 0u;
+        }
+        if (b < M3_CELL_LDS_BODIES)
+        {
+            return cellOfL[b];
+        }
+        return __hip_atomic_load(&d.cellOf[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    const int32_t nc = v.sc->stall != 0 ? 0 : v.sc->colorStarts[M3_GRAPH_COLORS];
+    bool veto = false;
+    for (int32_t k0 = t; k0 < nc; k0 += 4 * M3_CELL_BINS)
     {
-        return;
+        int2 ab[4];
+        M3_UNROLL
+        for (int q = 0; q < 4; ++q)
+        {
+            const int32_t k = k0 + q * M3_CELL_BINS;
+            ab[q] = k < nc ? d.slotBodies[k] : make_int2(-1, -1);
+        }
+        M3_UNROLL
+        for (int q = 0; q < 4; ++q)
+        {
+            const int32_t k = k0 + q * M3_CELL_BINS;
+            if (k >= nc)
+            {
+                continue;
+            }
+            const uint32_t ca = cellOfBody(ab[q].x);
+            const uint32_t cb = cellOfBody(ab[q].y);
+            const uint32_t owner = ab[q].x >= 0 ? ca : cb;
+            d.slotOwner[k] = (uint8_t)owner;
+            if ((ab[q].x >= 0 && ca >= (uint32_t)cells) || (ab[q].y >= 0 && cb >= (uint32_t)cells))
+            {
+                veto = true;
+                continue;
+            }
+            const int32_t at = atomicAdd(&consCnt[owner], 1);
+            if (at >= M3_CELL_CONS)
+            {
+                veto = true;
+                continue;
+            }
+            d.cellCons[owner * M3_CELL_CONS + at] = k;
+        }
     }
-    const m3Constraint* c = &v.cons[k];
-    const int32_t body = c->dynA ? c->bodyA : c->bodyB;
-    const uint8_t owner = (c->dynA || c->dynB) ? d.cellOf[body] : 0;
-    d.slotOwner[k] = owner;
-    if ((!c->dynA && v.type[c->bodyA] == M3_KINEMATIC) || (!c->dynB && v.type[c->bodyB] == M3_KINEMATIC))
-    {
-        v.sc->flowKinematic = 1;
-    }
-    const bool outside = (c->dynA && d.cellOf[c->bodyA] >= cells) || (c->dynB && d.cellOf[c->bodyB] >= cells);
-    if (outside || owner >= cells)
+    if (veto)
     {
         v.sc->flowVeto = 1;
-        return;
     }
-    const int32_t at = atomicAdd(&d.cellConsCount[owner], 1);
-    if (at >= M3_CELL_CONS)
+    __syncthreads();
+    if (t < cells)
     {
-        v.sc->flowVeto = 1;
-        return;
+        d.cellConsCount[t] = consCnt[t];
     }
-    d.cellCons[owner * M3_CELL_CONS + at] = k;
 }
 
 // Per-cell LDS of the substep kernel.
@@ -3725,25 +3885,6 @@ __global__ void k_restitution(const m3Dev* __restrict__ ds, float threshold)
 
 // The impulse store, and the applied forces of every mover cleared (the
 // grid covers the larger of the pair and body capacities).
-__global__ void k_store(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0)
-    {
-        return;
-    }
-    if (c < v.sc->moverCount)
-    {
-        const int32_t i = v.movers[c];
-        v.force[i] = m3V3(0.0f, 0.0f, 0.0f);
-        v.torque[i] = m3V3(0.0f, 0.0f, 0.0f);
-    }
-    if (c < v.sc->constraintCount)
-    {
-        m3StoreOne(&v, &v.cons[c]);
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Continuous collision and sleep (src/solver.c:3349-3430, 3582-3660)
@@ -3752,15 +3893,39 @@ __global__ void k_store(const m3Dev* __restrict__ ds)
 // Continuous collision (src/solver.c:3349-3430).  Without bullets a fast
 // body sweeps only against static shapes and planes, which the pass never
 // moves, so bodies run in parallel and apply their own pull-back.  With
-// bullets, k_ccd computes the other bodies into ccdXf / ccdHit,
+// bullets, k_finish computes the other bodies into ccdXf / ccdHit,
 // k_ccd_bullets the bullets (each sees the targets of lower slots pulled
 // back, as the reference's slot-order loop does; bullets never sweep against
 // bullets), and k_ccd_apply applies both.
-__global__ void k_ccd(const m3Dev* __restrict__ ds)
+// The step's closing work, one item per thread, none reading another's
+// output: constraint c's accumulators back into its manifold and mover c's
+// force cleared (src/solver.c:4210-4240), body i's continuous pass when
+// `ccd` is set, and body i's island readiness flag raised for the sleep
+// pass (cleared there by any slow member).
+__global__ void __launch_bounds__(64) k_finish(const m3Dev* __restrict__ ds, int ccd)
 {
     M3_WORLD;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0)
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v.sc->stall != 0)
+    {
+        return;
+    }
+    if (i < v.sc->moverCount)
+    {
+        const int32_t b = v.movers[i];
+        v.force[b] = m3V3(0.0f, 0.0f, 0.0f);
+        v.torque[b] = m3V3(0.0f, 0.0f, 0.0f);
+    }
+    if (i < v.sc->constraintCount)
+    {
+        m3StoreOne(&v, &v.cons[i]);
+    }
+    if (i >= v.maxBody)
+    {
+        return;
+    }
+    v.flag[i] = 1;
+    if (ccd == 0)
     {
         return;
     }
@@ -3830,17 +3995,15 @@ __global__ void k_pair_table(const m3Dev* __restrict__ ds, uint64_t* tableKeys, 
     }
 }
 
+// Island count and sleep timers; a body still too young to sleep clears
+// its island's readiness flag (raised by k_finish).
 __global__ void k_sleep_timers(const m3Dev* __restrict__ ds, float dt, int sleepEnabled)
 {
     M3_WORLD;
     const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0)
-    {
-        return;
-    }
-    v.flag[i] = 1; // island readiness, cleared below by any slow member
-    if (v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC || v.awake[i] == 0)
+    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
+        v.awake[i] == 0)
     {
         return;
     }
@@ -3851,32 +4014,35 @@ __global__ void k_sleep_timers(const m3Dev* __restrict__ ds, float dt, int sleep
     if (sleepEnabled)
     {
         m3SleepTimer(&v, i, dt);
+        if (v.sleepTime[i] < 0.5f)
+        {
+            v.flag[root[i]] = 0;
+        }
     }
 }
 
-__global__ void k_sleep_ready(const m3Dev* __restrict__ ds)
+// The step's close: thread 0 counts the step and ends the full query;
+// with sleep on, every body of a ready island freezes.
+__global__ void k_close(const m3Dev* __restrict__ ds, int sleepEnabled)
 {
     M3_WORLD;
     const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
-        v.awake[i] == 0)
+    if (i == 0)
     {
-        return;
+        m3Scalars* sc = v.sc;
+        if (sc->stall == 0)
+        {
+            sc->stepCount += 1;
+        }
+        sc->fullQuery = 0;
+        if (sc->sleepCount > v.pairCapacity)
+        {
+            sc->sleepCount = v.pairCapacity;
+        }
     }
-    if (v.sleepTime[i] < 0.5f)
-    {
-        v.flag[root[i]] = 0;
-    }
-}
-
-__global__ void k_sleep_freeze(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    const int32_t* root = d.root;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
-        v.awake[i] == 0)
+    if (sleepEnabled == 0 || i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 ||
+        v.type[i] != M3_DYNAMIC || v.awake[i] == 0)
     {
         return;
     }
@@ -3891,21 +4057,6 @@ __global__ void k_sleep_freeze(const m3Dev* __restrict__ ds)
         {
             d.tape->slept[i] = 1;
         }
-    }
-}
-
-__global__ void k_end(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    m3Scalars* sc = v.sc;
-    if (sc->stall == 0)
-    {
-        sc->stepCount += 1;
-    }
-    sc->fullQuery = 0;
-    if (sc->sleepCount > v.pairCapacity)
-    {
-        sc->sleepCount = v.pairCapacity;
     }
 }
 
@@ -4626,7 +4777,7 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&w->dev.cellOf, nb) || DevAlloc(&w->dev.cellLocal, nb) ||
         DevAlloc(&w->dev.cellBodies, (int64_t)M3_CELL_MAX * M3_CELL_BODIES) ||
         DevAlloc(&w->dev.cellCount, M3_CELL_MAX) || DevAlloc(&w->dev.cellConsCount, M3_CELL_MAX) ||
-        DevAlloc(&w->dev.slotOwner, np) ||
+        DevAlloc(&w->dev.slotOwner, np) || DevAlloc(&w->dev.slotBodies, np) ||
         DevAlloc(&w->dev.cellCons, (int64_t)M3_CELL_MAX * M3_CELL_CONS) ||
         DevAlloc(&w->hitRecBuf, np) || DevAlloc(&v.shapeRank, ns) || DevAlloc(&v.ccdStatics, ns) ||
         DevAlloc(&w->dev.ccdXf, nb) || DevAlloc(&w->dev.ccdHit, nb))
@@ -4679,6 +4830,15 @@ static int CreateImpl(m3gpuWorld* w)
         w->coopThreads = M3_SUBSTEP_THREADS;
         HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_substeps, w->coopThreads, 0));
         int maxBlocks = perCu * prop.multiProcessorCount;
+        {
+            int a = 0, b = 0, c = 0;
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_islands, M3_COMPACT_THREADS, 0));
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_constraints, M3_COMPACT_THREADS, 0));
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_movers, M3_COMPACT_THREADS, 0));
+            const int m = a < b ? (a < c ? a : c) : (b < c ? b : c);
+            const int most = (m > 0 ? m : 1) * prop.multiProcessorCount;
+            w->coopCompact = most < M3_COOP_COMPACT_BLOCKS ? most : M3_COOP_COMPACT_BLOCKS;
+        }
         // Up to 40 workgroups of 256: one body per thread in the
         // integrate-and-warm-start phase for worlds up to 10,240 bodies,
         // while the grid barrier stays near its few-microsecond floor.  A
@@ -4771,7 +4931,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
                     v.slotTable, v.slotMask, w->dev.barrier, w->dev.flowRec, w->dev.flowRank,
                     w->dev.warmAdd, w->dev.deltaRec, w->dev.cellOf, w->dev.cellLocal, w->dev.cellBodies, w->dev.cellCount,
-                    w->dev.cellConsCount, w->dev.slotOwner, w->dev.cellCons, w->devDev,
+                    w->dev.cellConsCount, w->dev.slotOwner, w->dev.slotBodies, w->dev.cellCons, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
                     v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
                     v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
@@ -5447,6 +5607,40 @@ extern "C" int m3gpu_events_noalloc(m3gpuWorld* w, float hitThreshold, m3gpuEven
     return 0;
 }
 
+// Launches a kernel whose workgroups meet at grid barriers (GridBarrier),
+// on a grid within the kernel's occupancy limit.  The barrier needs every
+// workgroup resident at once, which such a grid gets on an in-order stream
+// (the previous launch has drained the device); its waits are bounded, so
+// a grid that was somehow not all resident faults the world rather than
+// hanging.  hipLaunchCooperativeKernel guarantees residency but idles the
+// device around each launch (about 11 us before and after on MI355X), so it
+// is taken only when MAUL3D_AMD_COOP_LAUNCH=1.
+static int GridLaunch(const void* fn, dim3 grid, dim3 block, void** args, hipStream_t st)
+{
+    static const int coop = [] {
+        const char* env = getenv("MAUL3D_AMD_COOP_LAUNCH");
+        return env != nullptr && atoi(env) != 0 ? 1 : 0;
+    }();
+    if (coop)
+    {
+        HIP_TRY(hipLaunchCooperativeKernel(fn, grid, block, args, 0, st));
+    }
+    else
+    {
+        HIP_TRY(hipLaunchKernel(fn, grid, block, args, 0, st));
+    }
+    return 0;
+}
+
+// A grid-barrier launch of 256-thread workgroups over `items`, at most
+// `limit` workgroups (the kernels stride over chunks of the items).
+static int CoopLaunch(const void* fn, int64_t items, int limit, void** args, hipStream_t st)
+{
+    int blocks = (int)Blocks(items, M3_COMPACT_THREADS);
+    blocks = blocks > limit ? limit : blocks;
+    return GridLaunch(fn, dim3(blocks), dim3(M3_COMPACT_THREADS), args, st);
+}
+
 // Enqueue one step.  Launch geometry only depends on host-known capacities
 // and high-water marks; every data-dependent count is read on the device.
 // The step through the island wake pass (pair list, narrowphase, islands);
@@ -5475,9 +5669,8 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     int32_t np = v.pairCapacity;
     int32_t ne = v.emitCapacity;
 
-    STAGE(ST_FILL, KL(k_open, dim3(Blocks(ne, T)), dim3(T), 0, st, D));
-    // Broadphase.
-    STAGE(ST_FRESH, KL(k_fresh, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
+    // Step scalars, the emission sentinel and the broadphase's fresh boxes.
+    STAGE(ST_FILL, KL(k_open, dim3(Blocks(ne > ns ? ne : ns, T)), dim3(T), 0, st, D));
     if (w->fullQuery)
     {
         // After a restore the frozen buffer is rebuilt from every sleeping
@@ -5507,8 +5700,8 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     STAGE(ST_FROZEN, KL(k_frozen, dim3(Blocks(np, T)), dim3(T), 0, st, D));
     {
         void* args[] = {(void*)&D};
-        STAGE(ST_EMIT_SORT, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_emit_pairs, dim3(M3_SORT_GRID),
-                                                               dim3(M3_SORT_THREADS), args, 0, st)));
+        STAGE(ST_EMIT_SORT, if (GridLaunch((const void*)k_emit_pairs, dim3(M3_SORT_GRID), dim3(M3_SORT_THREADS),
+                                            args, st) != 0) { return -1; });
     }
     const int CT = M3_COMPACT_THREADS;
     STAGE(ST_EXPAND, KL(k_pairs_settle, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
@@ -5528,9 +5721,10 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     {
         STAGE(ST_UNION, KL(k_union_joints, dim3(Blocks(v.maxJoint, T)), dim3(T), 0, st, D));
     }
-    STAGE(ST_ROOT, KL(k_root, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
-    STAGE(ST_WAKE_MARK, KL(k_wake_mark, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
-    STAGE(ST_WAKE_APPLY, KL(k_wake_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
+    {
+        void* args[] = {(void*)&D};
+        STAGE(ST_ROOT, if (CoopLaunch((const void*)k_islands, nb, w->coopCompact, args, st) != 0) { return -1; });
+    }
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -5574,8 +5768,10 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
     // Prepare and color.
     const m3SubstepParams sp = SubstepParams(p);
     const int CT = M3_COMPACT_THREADS;
-    STAGE(ST_CONS_FLAGS, KL(k_constraint_flags, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
-    STAGE(ST_CONS_SELECT, KL(k_compact_constraints, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
+    {
+        void* args[] = {(void*)&D};
+        STAGE(ST_CONS_FLAGS, if (CoopLaunch((const void*)k_constraints, np, w->coopCompact, args, st) != 0) { return -1; });
+    }
     if (nb <= M3_COLOR_LDS_BODIES)
     {
         STAGE(ST_COLOR, KL(k_color<true>, dim3(1), dim3(M3_COLOR_THREADS), 0, st, D, 1));
@@ -5591,8 +5787,11 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
         STAGE(ST_PREPARE,
               KL(k_prepare_joints, dim3(Blocks(v.maxJoint, 64)), dim3(64), 0, st, D, sp.h));
     }
-    STAGE(ST_MOVER_FLAGS, KL(k_mover_flags, dim3(Blocks(nb, CT)), dim3(CT), 0, st, D, p->dt));
-    STAGE(ST_MOVER_SELECT, KL(k_compact_movers, dim3(Blocks(nb, CT)), dim3(CT), 0, st, D));
+    {
+        float dt = p->dt;
+        void* args[] = {(void*)&D, (void*)&dt};
+        STAGE(ST_MOVER_FLAGS, if (CoopLaunch((const void*)k_movers, nb, w->coopCompact, args, st) != 0) { return -1; });
+    }
     // Substeps: one cooperative launch.
     {
         m3SubstepParams spArg = sp;
@@ -5614,16 +5813,14 @@ static int EnqueueStepB(m3gpuWorld* w, const m3gpuStepParams* p, const uint64_t*
             // does not fit the cells, runs on the same grid.
             blocks = w->cellsX * w->cellsZ;
             STAGE(ST_CELLS, KL(k_cells, dim3(1), dim3(M3_CELL_BINS), 0, st, D, w->cellsX, w->cellsZ));
-            STAGE(ST_CELLS, KL(k_cell_owner, dim3(Blocks(np, T)), dim3(T), 0, st, D, blocks));
         }
-        STAGE(ST_SUBSTEPS, HIP_TRY(hipLaunchCooperativeKernel((const void*)k_substeps, dim3(blocks),
-                                                              dim3(w->coopThreads), args, 0, st)));
+        STAGE(ST_SUBSTEPS, if (GridLaunch((const void*)k_substeps, dim3(blocks), dim3(w->coopThreads), args,
+                                          st) != 0) { return -1; });
     }
     if (w->anyRestitution)
     {
         STAGE(ST_RESTITUTION, KL(k_restitution, dim3(1), dim3(1), 0, st, D, p->restitutionThreshold));
     }
-    STAGE(ST_STORE, KL(k_store, dim3(Blocks(np > nb ? np : nb, T)), dim3(T), 0, st, D));
     if (v.maxJoint > 0)
     {
         STAGE(ST_STORE, KL(k_store_joints, dim3(Blocks(v.maxJoint, T)), dim3(T), 0, st, D));
@@ -5668,9 +5865,12 @@ static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t
         HIP_TRY(hipEventRecord(w->ranksCopied, st));
         w->rankCopyPending = 1;
     }
+    // Impulse store, the continuous pass and the sleep flags' reset.
+    const int32_t np = v.pairCapacity;
+    STAGE(ST_STORE, KL(k_finish, dim3(Blocks(np > nb ? np : nb, 64)), dim3(64), 0, st, D,
+                       p->continuousEnabled ? 1 : 0));
     if (p->continuousEnabled)
     {
-        STAGE(ST_CCD, KL(k_ccd, dim3(Blocks(nb, 64)), dim3(64), 0, st, D));
         if (w->dev.ccdBullets)
         {
             STAGE(ST_CCD, KL(k_ccd_bullets, dim3(Blocks(nb, 64)), dim3(64), 0, st, D));
@@ -5679,12 +5879,7 @@ static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t
     }
     STAGE(ST_SLEEP_TIMERS,
           KL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt, p->sleepEnabled));
-    if (p->sleepEnabled)
-    {
-        STAGE(ST_SLEEP_READY, KL(k_sleep_ready, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
-        STAGE(ST_SLEEP_FREEZE, KL(k_sleep_freeze, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
-    }
-    STAGE(ST_END, KL(k_end, dim3(1), dim3(1), 0, st, D));
+    STAGE(ST_END, KL(k_close, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->sleepEnabled));
     if (w->pairTable)
     {
         KL(k_pair_table, dim3(Blocks(v.pairCapacity, T)), dim3(T), 0, st, D, w->tableKeys, w->tableMans);
@@ -6274,8 +6469,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     {
         KL(k_tape_start, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     }
-    KL(k_open, dim3(Blocks(ne, T), W), dim3(T), 0, st, D);
-    KL(k_fresh, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
+    KL(k_open, dim3(Blocks(ne > ns ? ne : ns, T), W), dim3(T), 0, st, D);
     KL(k_reset_sleep, dim3(1, W), dim3(1), 0, st, D);
     KL(k_pending_all, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_sap_sort_block, dim3(1, W), dim3(BT), 0, st, D);
@@ -6308,7 +6502,6 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
         KL(k_union_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);
     }
     KL(k_root, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
-    KL(k_wake_mark, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_wake_apply, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
     KL(k_constraint_flags, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 1);
@@ -6328,22 +6521,13 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     }
     KL(k_substeps_block, dim3(1, W), dim3(s_batch.threads), 0, st, D, sp);
     KL(k_restitution, dim3(1, W), dim3(1), 0, st, D, p->restitutionThreshold);
-    KL(k_store, dim3(Blocks(np > nb ? np : nb, T), W), dim3(T), 0, st, D);
     if (s_batch.maxJoint > 0)
     {
         KL(k_store_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);
     }
-    if (p->continuousEnabled)
-    {
-        KL(k_ccd, dim3(Blocks(nb, 64), W), dim3(64), 0, st, D);
-    }
+    KL(k_finish, dim3(Blocks(np > nb ? np : nb, 64), W), dim3(64), 0, st, D, p->continuousEnabled ? 1 : 0);
     KL(k_sleep_timers, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->dt, p->sleepEnabled);
-    if (p->sleepEnabled)
-    {
-        KL(k_sleep_ready, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
-        KL(k_sleep_freeze, dim3(Blocks(nb, T), W), dim3(T), 0, st, D);
-    }
-    KL(k_end, dim3(1, W), dim3(1), 0, st, D);
+    KL(k_close, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->sleepEnabled);
     HIP_TRY(hipGetLastError());
     return 0;
 }
