@@ -53,7 +53,7 @@ static int Fail(const char* what, hipError_t e)
 struct m3Dev
 {
     m3View v;
-    uint64_t* sapKeysIn;  // per shape: its sweep key (k_open)
+    uint64_t* sapKeysIn;  // per shape: its sweep key (k_open); k_sap_sort leaves its sorted runs here
     uint64_t* sapKeysOut; // the keys sorted
     float4* sapXY;
     float2* sapZ;
@@ -4382,6 +4382,250 @@ __global__ void __launch_bounds__(M3_BATCH_THREADS) k_sap_sort_block(const m3Dev
     }
 }
 
+// The sweep keys of one world, sorted in one launch of 1024-thread
+// workgroups.  Every key is distinct (its low word is the shape).
+//   1. Workgroup r < runs sorts run r (keys [r R, r R + R), R at most
+//      M3_RSORT_RUN) in LDS and writes it back in place: a stable LSD radix
+//      sort of the high words, 4 bits a pass, each thread holding `per`
+//      consecutive keys.  A thread counts its digits in registers (16
+//      counters of 16 bits packed in four 64-bit words), DPP scans those
+//      counters across each wavefront and wavefront 0 across the
+//      wavefronts, and a key's place is its digit's base + the counts of
+//      that digit in earlier threads + its rank among the thread's own, so
+//      equal digits keep their order.  The keys arrive in shape order, so
+//      the high words' stable order is the full keys' order.
+//   2. After the grid barrier every workgroup loads the sorted runs into
+//      LDS, and each key goes to its place in its own run plus, for every
+//      other run, the number of that run's keys below it (binary search).
+// Worlds over M3_RSORT_MAX shapes take the library sort.
+#define M3_RSORT_THREADS 1024
+#define M3_RSORT_ITEMS 2
+#define M3_RSORT_RUN (M3_RSORT_THREADS * M3_RSORT_ITEMS)
+#define M3_RSORT_MAX 14336
+#define M3_RSORT_BITS 4
+
+// Inclusive sum over a wavefront by DPP: shifts within each row of 16
+// lanes, then each row's total broadcast into the rows above it.
+__device__ static inline uint32_t WaveInclusiveSum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false); // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false); // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false); // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false); // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false); // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false); // row_bcast:31
+    return x;
+}
+
+// The same over packed 16-bit counters (no field exceeds 65535, so the
+// 32-bit sums carry nothing between fields).
+__device__ static inline uint64_t WaveInclusiveSum64(uint64_t x)
+{
+    const uint32_t lo = WaveInclusiveSum((uint32_t)x);
+    const uint32_t hi = WaveInclusiveSum((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// 16-bit field d (0-15) of four packed words.
+__device__ static inline uint32_t RsortField(const uint64_t* p, uint32_t d)
+{
+    const uint32_t q = d >> 2;
+    const uint64_t w = q == 0 ? p[0] : (q == 1 ? p[1] : (q == 2 ? p[2] : p[3]));
+    return (uint32_t)(w >> (16 * (d & 3))) & 0xFFFFu;
+}
+
+// Stable sort of keys[0, per * 1024) by their high words, in LDS.
+__device__ static inline void RunSort(uint64_t* keys, int32_t per, uint64_t (*waveTot)[4], uint64_t* grand)
+{
+    const int32_t t = threadIdx.x;
+    const int32_t lane = t & 63;
+    const int32_t wave = t >> 6;
+    uint64_t k[M3_RSORT_ITEMS];
+    M3_UNROLL
+    for (int32_t j = 0; j < M3_RSORT_ITEMS; ++j)
+    {
+        k[j] = j < per ? keys[t * per + j] : 0ull;
+    }
+    const uint64_t ones = 0x0001000100010001ull;
+    for (int32_t bit = 32; bit < 64; bit += M3_RSORT_BITS)
+    {
+        uint64_t c[4] = {0ull, 0ull, 0ull, 0ull};
+        uint32_t rank[M3_RSORT_ITEMS];
+        M3_UNROLL
+        for (int32_t j = 0; j < M3_RSORT_ITEMS; ++j)
+        {
+            if (j < per)
+            {
+                const uint32_t dg = (uint32_t)(k[j] >> bit) & 15u;
+                rank[j] = RsortField(c, dg);
+                const uint64_t inc = 1ull << (16 * (dg & 3));
+                const uint32_t q = dg >> 2;
+                c[0] += q == 0 ? inc : 0ull;
+                c[1] += q == 1 ? inc : 0ull;
+                c[2] += q == 2 ? inc : 0ull;
+                c[3] += q == 3 ? inc : 0ull;
+            }
+        }
+        uint64_t sc[4];
+        M3_UNROLL
+        for (int r = 0; r < 4; ++r)
+        {
+            sc[r] = WaveInclusiveSum64(c[r]);
+        }
+        if (lane == 63)
+        {
+            M3_UNROLL
+            for (int r = 0; r < 4; ++r)
+            {
+                waveTot[wave][r] = sc[r];
+            }
+        }
+        __syncthreads();
+        if (wave == 0)
+        {
+            uint64_t x[4];
+            uint64_t in[4];
+            M3_UNROLL
+            for (int r = 0; r < 4; ++r)
+            {
+                x[r] = lane < M3_RSORT_THREADS / 64 ? waveTot[lane][r] : 0ull;
+                in[r] = WaveInclusiveSum64(x[r]);
+            }
+            if (lane < M3_RSORT_THREADS / 64)
+            {
+                M3_UNROLL
+                for (int r = 0; r < 4; ++r)
+                {
+                    waveTot[lane][r] = in[r] - x[r];
+                }
+            }
+            if (lane == M3_RSORT_THREADS / 64 - 1)
+            {
+                M3_UNROLL
+                for (int r = 0; r < 4; ++r)
+                {
+                    grand[r] = in[r];
+                }
+            }
+        }
+        __syncthreads();
+        // Each field of `at`: the digit's base (keys of lower digits) + the
+        // digit's keys in earlier threads.
+        uint64_t at[4];
+        uint64_t carry = 0ull;
+        M3_UNROLL
+        for (int r = 0; r < 4; ++r)
+        {
+            const uint64_t g = grand[r];
+            const uint64_t before = (g << 16) + (g << 32) + (g << 48); // earlier fields of this word
+            at[r] = before + carry * ones + waveTot[wave][r] + sc[r] - c[r];
+            carry += (g * ones) >> 48; // the word's total
+        }
+        M3_UNROLL
+        for (int32_t j = 0; j < M3_RSORT_ITEMS; ++j)
+        {
+            if (j < per)
+            {
+                const uint32_t dg = (uint32_t)(k[j] >> bit) & 15u;
+                keys[RsortField(at, dg) + rank[j]] = k[j];
+            }
+        }
+        __syncthreads();
+        M3_UNROLL
+        for (int32_t j = 0; j < M3_RSORT_ITEMS; ++j)
+        {
+            if (j < per)
+            {
+                k[j] = keys[t * per + j];
+            }
+        }
+    }
+}
+
+// Keys of a sorted run below `key`.
+__device__ static inline int32_t RunBelow(const uint64_t* run, int32_t len, uint64_t key)
+{
+    int32_t lo = 0;
+    int32_t hi = len;
+    while (lo < hi)
+    {
+        const int32_t mid = (lo + hi) >> 1;
+        if (run[mid] < key)
+        {
+            lo = mid + 1;
+        }
+        else
+        {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(M3_RSORT_THREADS) k_sap_sort(const m3Dev* __restrict__ ds, int32_t runs)
+{
+    M3_WORLD;
+    __shared__ uint64_t keys[M3_RSORT_MAX];
+    __shared__ uint64_t waveTot[M3_RSORT_THREADS / 64][4];
+    __shared__ uint64_t grand[4];
+    const m3GridSync sync = {d.barrier, &v.sc->fault};
+    const int32_t t = threadIdx.x;
+    const int32_t n = v.maxShape;
+    const int32_t runLen = (n + runs - 1) / runs;
+    uint64_t* buf = d.sapKeysIn;
+    if ((int32_t)blockIdx.x < runs)
+    {
+        const int32_t base = blockIdx.x * runLen;
+        const int32_t len = n - base < runLen ? (n - base > 0 ? n - base : 0) : runLen;
+        const int32_t per = (len + M3_RSORT_THREADS - 1) / M3_RSORT_THREADS;
+        const int32_t m = per * M3_RSORT_THREADS;
+        // Padding keys sort after every real key and are never written out.
+        M3_UNROLL
+        for (int32_t j = 0; j < M3_RSORT_ITEMS; ++j)
+        {
+            const int32_t i = t + j * M3_RSORT_THREADS;
+            if (i < m)
+            {
+                keys[i] = i < len ? buf[base + i] : ~0ull;
+            }
+        }
+        __syncthreads();
+        RunSort(keys, per, waveTot, grand);
+        M3_UNROLL
+        for (int32_t j = 0; j < M3_RSORT_ITEMS; ++j)
+        {
+            const int32_t i = t + j * M3_RSORT_THREADS;
+            if (i < len)
+            {
+                buf[base + i] = keys[i];
+            }
+        }
+    }
+    sync();
+    for (int32_t i = t; i < n; i += M3_RSORT_THREADS)
+    {
+        keys[i] = buf[i];
+    }
+    __syncthreads();
+    const int32_t i = blockIdx.x * M3_RSORT_THREADS + t;
+    if (i < n)
+    {
+        const int32_t own = i / runLen;
+        const uint64_t key = keys[i];
+        int32_t at = i - own * runLen;
+        for (int32_t r = 0; r < runs; ++r)
+        {
+            if (r != own)
+            {
+                const int32_t base = r * runLen;
+                const int32_t len = n - base < runLen ? (n - base > 0 ? n - base : 0) : runLen;
+                at += RunBelow(keys + base, len, key);
+            }
+        }
+        d.sapKeysOut[at] = key;
+    }
+}
+
 #define M3_SORT_THREADS 1024
 #define M3_SORT_GRID 32 // workgroups of the pair-key launch
 
@@ -5679,10 +5923,19 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
         STAGE(ST_DISCOVER_ALL, KL(k_pending_all, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
         w->fullQuery = 0;
     }
-    // rocPRIM's radix sort over the packed sweep keys: for ten thousand keys
-    // it beat every single-workgroup form tried here.
-    STAGE(ST_SAP_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(w->cubTemp, w->cubBytes, hd.sapKeysIn,
-                                                                 hd.sapKeysOut, ns, 0, 64, st)));
+    if (ns <= M3_RSORT_MAX)
+    {
+        // Runs of at most M3_RSORT_RUN keys; one thread per key to merge.
+        int32_t runs = (int32_t)Blocks(ns, M3_RSORT_RUN);
+        void* args[] = {(void*)&D, (void*)&runs};
+        STAGE(ST_SAP_SORT, if (GridLaunch((const void*)k_sap_sort, dim3(Blocks(ns, M3_RSORT_THREADS)),
+                                          dim3(M3_RSORT_THREADS), args, st) != 0) { return -1; });
+    }
+    else
+    {
+        STAGE(ST_SAP_SORT, HIP_TRY(hipcub::DeviceRadixSort::SortKeys(w->cubTemp, w->cubBytes, hd.sapKeysIn,
+                                                                     hd.sapKeysOut, ns, 0, 64, st)));
+    }
     STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
     STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, M3_SAP_SHAPES)), dim3(M3_SAP_THREADS), 0, st, D));
     if (v.planeCount > 0)
