@@ -501,28 +501,6 @@ __device__ static inline void AppendFrozen(const m3View& v, int32_t i, int32_t j
 
 // A proxy moves only when its fresh box escapes the stored box; it is then
 // re-stored inflated by the margin (src/broad_phase.c:505-539).
-__global__ void k_refresh(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= v.maxShape || v.shapeAlive[s] == 0 || v.shapeType[s] == M3_PLANE ||
-        v.awake[v.shapeBody[s]] == 0)
-    {
-        return;
-    }
-    const double* f = &v.fresh[6 * s];
-    double* leaf = &v.leaf[6 * s];
-    if (leaf[0] <= f[0] && leaf[1] <= f[1] && leaf[2] <= f[2] && f[3] <= leaf[3] &&
-        f[4] <= leaf[4] && f[5] <= leaf[5])
-    {
-        return;
-    }
-    for (int k = 0; k < 3; ++k)
-    {
-        leaf[k] = f[k] - (double)M3_AABB_MARGIN;
-        leaf[3 + k] = f[3 + k] + (double)M3_AABB_MARGIN;
-    }
-}
 
 // Outward-rounded float copies of the sorted fresh boxes: a float test that
 // fails implies the exact double test fails, so the sweep filters on floats
@@ -761,28 +739,6 @@ __global__ void __launch_bounds__(M3_SAP_THREADS) k_sap(const m3Dev* __restrict_
 }
 
 // Planes pair with every allowed hot shape (src/broad_phase.c:541-564).
-__global__ void k_planes(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    bool hot = false;
-    uint32_t key = 0;
-    if (idx < (int64_t)v.maxShape * v.planeCount)
-    {
-        int32_t s = (int32_t)(idx % v.maxShape);
-        int32_t p = v.planes[idx / v.maxShape];
-        if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE && m3PairAllowed(&v, p, s))
-        {
-            if (v.pending[v.shapeBody[s]])
-            {
-                AppendFrozen(v, p, s); // planes pair unconditionally with a frozen body
-            }
-            hot = m3PairHot(&v, p, s) != 0;
-            key = m3CompactKey(p, s);
-        }
-    }
-    EmitWave(v, hot, key);
-}
 
 // Still-cold frozen pairs ride along (src/broad_phase.c:607-632).
 __global__ void k_frozen(const m3Dev* __restrict__ ds)
@@ -809,48 +765,6 @@ __global__ void k_frozen(const m3Dev* __restrict__ ds)
 // count and the stall from scalars that no thread of this pass changes
 // (thread 0 only publishes them), then expands its key or restores its old
 // pair, flags the cold keys and counts them per workgroup.
-__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_pairs_settle(const m3Dev* __restrict__ ds)
-{
-    M3_WORLD;
-    const uint32_t* unique = d.emitUnique;
-    m3Scalars* sc = v.sc;
-    int32_t n = sc->uniqueCount;
-    if (n > 0 && unique[n - 1] == M3_SENTINEL32)
-    {
-        n -= 1;
-    }
-    const int stall = sc->stall != 0 || sc->emitCount > v.emitCapacity || n > v.pairCapacity;
-    const int32_t oldCount = sc->oldPairCount;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0)
-    {
-        if (stall)
-        {
-            sc->stall = 1;
-        }
-        sc->pairCount = stall ? oldCount : n;
-    }
-    uint8_t cold = 0;
-    if (i < v.pairCapacity)
-    {
-        if (stall)
-        {
-            if (i < oldCount)
-            {
-                v.keys[i] = v.oldKeys[i];
-                v.manifolds[i] = v.oldManifolds[i];
-            }
-        }
-        else if (i < n)
-        {
-            const uint64_t key = m3ExpandKey(unique[i]);
-            v.keys[i] = key;
-            cold = m3PairHot(&v, (int32_t)(key >> 32), (int32_t)(key & 0xFFFFFFFFu)) ? 0 : 1;
-        }
-        d.pairFlags[i] = cold;
-    }
-    FlagCount(cold, d.blockCounts, blockIdx.x);
-}
 
 // The cold keys of the pair list, in order, become the frozen buffer.
 __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_frozen(const m3Dev* __restrict__ ds)
@@ -893,33 +807,57 @@ __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_compact_frozen(const m3D
 // identically.  Every candidate value comes from the same operations as the
 // serial scan, so the selected separation and axis are bit-identical.
 
-__global__ void k_sat_list(const m3Dev* __restrict__ ds)
+__device__ static inline void RefreshShape(const m3View& v, int32_t s)
 {
-    M3_WORLD;
-    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v.sc->stall != 0 || i >= v.sc->pairCount)
+    if (s >= v.maxShape || v.shapeAlive[s] == 0 || v.shapeType[s] == M3_PLANE ||
+        v.awake[v.shapeBody[s]] == 0)
     {
         return;
     }
-    uint64_t key = v.keys[i];
-    int32_t shapeA = (int32_t)(key >> 32);
-    int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
-    const uint8_t typeA = v.shapeType[shapeA];
-    const uint8_t typeB = v.shapeType[shapeB];
-    int wide = 0;
-    if (typeA == M3_HULL && typeB == M3_HULL)
+    const double* f = &v.fresh[6 * s];
+    double* leaf = &v.leaf[6 * s];
+    if (leaf[0] <= f[0] && leaf[1] <= f[1] && leaf[2] <= f[2] && f[3] <= leaf[3] &&
+        f[4] <= leaf[4] && f[5] <= leaf[5])
     {
-        wide = m3WideHullPair(&v.hulls[v.hullIndex[shapeA]], &v.hulls[v.hullIndex[shapeB]], v.satBoxes);
+        return;
     }
-    else if ((typeA == M3_HULL && typeB == M3_CAPSULE) || (typeA == M3_CAPSULE && typeB == M3_HULL))
+    for (int k = 0; k < 3; ++k)
     {
-        wide = m3WideSegmentHull(&v.hulls[v.hullIndex[typeA == M3_HULL ? shapeA : shapeB]]);
-    }
-    if (wide)
-    {
-        v.satList[atomicAdd(&v.sc->satCount, 1)] = i;
+        leaf[k] = f[k] - (double)M3_AABB_MARGIN;
+        leaf[3 + k] = f[3 + k] + (double)M3_AABB_MARGIN;
     }
 }
+
+// Shape-plane pairs (idx = plane slot * maxShape + shape) and, for
+// idx < maxShape, the shape's proxy refresh: neither reads the other's
+// output.
+__global__ void k_planes(const m3Dev* __restrict__ ds)
+{
+    M3_WORLD;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool hot = false;
+    uint32_t key = 0;
+    if (idx < (int64_t)v.maxShape * v.planeCount)
+    {
+        int32_t s = (int32_t)(idx % v.maxShape);
+        int32_t p = v.planes[idx / v.maxShape];
+        if (v.shapeAlive[s] != 0 && v.shapeType[s] != M3_PLANE && m3PairAllowed(&v, p, s))
+        {
+            if (v.pending[v.shapeBody[s]])
+            {
+                AppendFrozen(v, p, s); // planes pair unconditionally with a frozen body
+            }
+            hot = m3PairHot(&v, p, s) != 0;
+            key = m3CompactKey(p, s);
+        }
+    }
+    EmitWave(v, hot, key);
+    if (idx < v.maxShape)
+    {
+        RefreshShape(v, (int32_t)idx);
+    }
+}
+
 
 #define M3_HULL_MAX_EDGES (M3_HULL_MAX_HALF_EDGES / 2)
 
@@ -1419,46 +1357,110 @@ __device__ static inline void CaptureBody(const m3View& v, int32_t i)
     v.deltaRot[i] = m3IdentityQuat();
 }
 
-// The pairs are first binned by collision routine (the order inside a bin
-// is immaterial: every pair writes its own manifold slot), then each
-// wavefront of k_narrow_binned takes 64 pairs of one bin, so no wavefront
-// runs more than one routine (a wavefront mixing routines runs them one
-// after another).  Batched worlds bin per world (grid.y).  The same pass
-// captures every body's step-start state first (the grid covers the larger
-// of the pair and body capacities); the narrowphase that follows unites
-// bodies over the reset union-find.
+// The pairs are first binned by collision routine (k_pairs_settle; the
+// order inside a bin is immaterial: every pair writes its own manifold
+// slot), then each wavefront of k_narrow_binned takes 64 pairs of one bin,
+// so no wavefront runs more than one routine (a wavefront mixing routines
+// runs them one after another).  Batched worlds bin per world (grid.y).
+// The same launch captures every body's step-start state; the narrowphase
+// that follows unites bodies over the reset union-find.
 #define M3_NARROW_CLASSES 7
 
-__global__ void k_narrow_bin(const m3Dev* __restrict__ ds)
+// Pair i listed for the wavefront SAT queries when it is a wide hull pair.
+__device__ static inline void SatList(const m3View& v, int32_t i, uint64_t key)
+{
+    int32_t shapeA = (int32_t)(key >> 32);
+    int32_t shapeB = (int32_t)(key & 0xFFFFFFFFu);
+    const uint8_t typeA = v.shapeType[shapeA];
+    const uint8_t typeB = v.shapeType[shapeB];
+    int wide = 0;
+    if (typeA == M3_HULL && typeB == M3_HULL)
+    {
+        wide = m3WideHullPair(&v.hulls[v.hullIndex[shapeA]], &v.hulls[v.hullIndex[shapeB]], v.satBoxes);
+    }
+    else if ((typeA == M3_HULL && typeB == M3_CAPSULE) || (typeA == M3_CAPSULE && typeB == M3_HULL))
+    {
+        wide = m3WideSegmentHull(&v.hulls[v.hullIndex[typeA == M3_HULL ? shapeA : shapeB]]);
+    }
+    if (wide)
+    {
+        v.satList[atomicAdd(&v.sc->satCount, 1)] = i;
+    }
+}
+
+// The step's pair list from the unique keys (or, on a stall, the old list
+// restored) and each pair's sleep flag (FlagCount for k_compact_frozen);
+// then, with the key in hand, the wide hull pairs listed for the SAT
+// queries (`sat`) and each pair binned by collision routine; and each
+// body's step-start state and union-find reset.  The grid covers
+// max(pairCapacity, maxBody).
+__global__ void __launch_bounds__(M3_COMPACT_THREADS) k_pairs_settle(const m3Dev* __restrict__ ds, int sat)
 {
     M3_WORLD;
     __shared__ int32_t cnt[8];
     __shared__ int32_t base[8];
-    const int32_t t = threadIdx.x;
+    const uint32_t* unique = d.emitUnique;
+    m3Scalars* sc = v.sc;
+    int32_t n = sc->uniqueCount;
+    if (n > 0 && unique[n - 1] == M3_SENTINEL32)
     {
-        const int32_t bi = blockIdx.x * blockDim.x + t;
-        if (bi < v.maxBody && v.sc->stall == 0)
-        {
-            CaptureBody(v, bi);
-        }
+        n -= 1;
     }
-    if (t < 8)
+    const int stall = sc->stall != 0 || sc->emitCount > v.emitCapacity || n > v.pairCapacity;
+    const int32_t oldCount = sc->oldPairCount;
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0)
     {
-        cnt[t] = 0;
+        if (stall)
+        {
+            sc->stall = 1;
+        }
+        sc->pairCount = stall ? oldCount : n;
+    }
+    uint8_t cold = 0;
+    if (i < v.pairCapacity)
+    {
+        if (stall)
+        {
+            if (i < oldCount)
+            {
+                v.keys[i] = v.oldKeys[i];
+                v.manifolds[i] = v.oldManifolds[i];
+            }
+        }
+        else if (i < n)
+        {
+            const uint64_t key = m3ExpandKey(unique[i]);
+            v.keys[i] = key;
+            cold = m3PairHot(&v, (int32_t)(key >> 32), (int32_t)(key & 0xFFFFFFFFu)) ? 0 : 1;
+        }
+        d.pairFlags[i] = cold;
+    }
+    FlagCount(cold, d.blockCounts, blockIdx.x);
+    if (i < v.maxBody && !stall)
+    {
+        CaptureBody(v, i);
+    }
+    const bool live = !stall && i < n;
+    if (sat != 0 && live)
+    {
+        SatList(v, i, v.keys[i]);
+    }
+    if (threadIdx.x < 8)
+    {
+        cnt[threadIdx.x] = 0;
     }
     __syncthreads();
-    const int32_t n = v.sc->stall != 0 ? 0 : v.sc->pairCount;
-    const int32_t i = blockIdx.x * blockDim.x + t;
-    const int cls = i < n ? NarrowClass(v, i) : M3_NARROW_CLASSES;
+    const int cls = live ? NarrowClass(v, i) : M3_NARROW_CLASSES;
     int32_t rank = 0;
     if (cls < M3_NARROW_CLASSES)
     {
         rank = atomicAdd(&cnt[cls], 1);
     }
     __syncthreads();
-    if (t < M3_NARROW_CLASSES && cnt[t] > 0)
+    if (threadIdx.x < M3_NARROW_CLASSES && cnt[threadIdx.x] > 0)
     {
-        base[t] = atomicAdd(&d.narrowBins[t], cnt[t]);
+        base[threadIdx.x] = atomicAdd(&d.narrowBins[threadIdx.x], cnt[threadIdx.x]);
     }
     __syncthreads();
     if (cls < M3_NARROW_CLASSES)
@@ -1466,6 +1468,7 @@ __global__ void k_narrow_bin(const m3Dev* __restrict__ ds)
         d.narrowBins[8 + (int64_t)cls * v.pairCapacity + base[cls] + rank] = i;
     }
 }
+
 
 // Grid: ceil(pairCapacity / 64) + M3_NARROW_CLASSES workgroups of 64, enough
 // for every bin rounded up to whole wavefronts.  Workgroup b walks the bins'
@@ -1502,7 +1505,7 @@ __global__ void __launch_bounds__(64) k_narrow_binned(const m3Dev* __restrict__ 
         v.hitRecs[i].flag = 0;
     }
     // The island union of a touching pair follows its manifold here (the
-    // bodies' union-find entries were reset by k_narrow_bin before this launch).
+    // bodies' union-find entries were reset by k_pairs_settle before this launch).
     if (m3NarrowPair(&v, i, v.sc->oldPairCount) > 0)
     {
         UnionTouching(v, i);
@@ -5938,12 +5941,10 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     }
     STAGE(ST_SAP_GATHER, KL(k_sap_gather, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
     STAGE(ST_SAP, KL(k_sap, dim3(Blocks(ns, M3_SAP_SHAPES)), dim3(M3_SAP_THREADS), 0, st, D));
-    if (v.planeCount > 0)
     {
-        STAGE(ST_PLANES,
-              KL(k_planes, dim3(Blocks((int64_t)ns * v.planeCount, T)), dim3(T), 0, st, D));
+        const int64_t work = (int64_t)ns * (v.planeCount > 0 ? v.planeCount : 1);
+        STAGE(ST_PLANES, KL(k_planes, dim3(Blocks(work, T)), dim3(T), 0, st, D));
     }
-    STAGE(ST_REFRESH, KL(k_refresh, dim3(Blocks(ns, T)), dim3(T), 0, st, D));
     if (leaves)
     {
         HIP_TRY(hipMemcpyAsync(w->leafStage, v.leaf, (size_t)ns * 6 * sizeof(double),
@@ -5957,17 +5958,17 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
                                             args, st) != 0) { return -1; });
     }
     const int CT = M3_COMPACT_THREADS;
-    STAGE(ST_EXPAND, KL(k_pairs_settle, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
+    // The pair list, its SAT listing and narrowphase bins, and the bodies'
+    // step-start state.
+    STAGE(ST_EXPAND, KL(k_pairs_settle, dim3(Blocks(np > nb ? np : nb, CT)), dim3(CT), 0, st, D,
+                        (w->wideHulls || w->boxHulls) ? 1 : 0));
     STAGE(ST_SLEEP_SELECT, KL(k_compact_frozen, dim3(Blocks(np, CT)), dim3(CT), 0, st, D));
     // Narrowphase.
     if (w->wideHulls || w->boxHulls)
     {
-        STAGE(ST_SAT_LIST, KL(k_sat_list, dim3(Blocks(np, T)), dim3(T), 0, st, D));
         STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
     }
-    // Step-start body state and union-find reset, then the narrowphase,
-    // which also unites the bodies of each touching pair.
-    STAGE(ST_NARROW, KL(k_narrow_bin, dim3(Blocks(np > nb ? np : nb, T)), dim3(T), 0, st, D));
+    // The narrowphase, which also unites the bodies of each touching pair.
     STAGE(ST_NARROW, KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES), dim3(64), 0, st, D));
     // Islands and wake.
     if (v.maxJoint > 0)
@@ -6728,23 +6729,18 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_sap_sort_block, dim3(1, W), dim3(BT), 0, st, D);
     KL(k_sap_gather, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
     KL(k_sap, dim3(Blocks(ns, M3_SAP_SHAPES), W), dim3(M3_SAP_THREADS), 0, st, D);
-    if (s_batch.maxPlaneWork > 0)
-    {
-        KL(k_planes, dim3(Blocks(s_batch.maxPlaneWork, T), W), dim3(T), 0, st, D);
-    }
-    KL(k_refresh, dim3(Blocks(ns, T), W), dim3(T), 0, st, D);
+    KL(k_planes, dim3(Blocks(s_batch.maxPlaneWork > ns ? s_batch.maxPlaneWork : ns, T), W), dim3(T), 0, st, D);
     KL(k_frozen, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
     KL(k_emit_sort_block, dim3(1, W), dim3(BT), 0, st, D);
-    KL(k_pairs_settle, dim3(Blocks(np, M3_COMPACT_THREADS), W), dim3(M3_COMPACT_THREADS), 0, st, D);
+    KL(k_pairs_settle, dim3(Blocks(np > nb ? np : nb, M3_COMPACT_THREADS), W), dim3(M3_COMPACT_THREADS), 0, st,
+       D, s_batch.wideHulls ? 1 : 0);
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 0);
     if (s_batch.wideHulls)
     {
-        KL(k_sat_list, dim3(Blocks(np, T), W), dim3(T), 0, st, D);
         KL(k_sat, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
     }
-    // Step-start reset, then the narrowphase binned by collision routine
-    // per world, which also unites the bodies of each touching pair.
-    KL(k_narrow_bin, dim3(Blocks(np > nb ? np : nb, T), W), dim3(T), 0, st, D);
+    // The narrowphase binned by collision routine per world, which also
+    // unites the bodies of each touching pair.
     KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES, W), dim3(64), 0, st, D);
     if (taped)
     {
