@@ -2134,6 +2134,10 @@ __device__ static inline uint8_t MoverFlag(const m3Dev& d, int32_t i, float dt)
     if (i < v.maxBody)
     {
         flags[i] = f;
+        if (v.sc->stall == 0)
+        {
+            v.flag[i] = 1; // island readiness for the sleep pass, cleared by any slow member
+        }
     }
     if (f != 0 && d.flowRec != nullptr)
     {
@@ -3900,12 +3904,37 @@ __global__ void k_restitution(const m3Dev* __restrict__ ds, float threshold)
 // k_ccd_bullets the bullets (each sees the targets of lower slots pulled
 // back, as the reference's slot-order loop does; bullets never sweep against
 // bullets), and k_ccd_apply applies both.
-// The step's closing work, one item per thread, none reading another's
-// output: constraint c's accumulators back into its manifold and mover c's
-// force cleared (src/solver.c:4210-4240), body i's continuous pass when
-// `ccd` is set, and body i's island readiness flag raised for the sleep
-// pass (cleared there by any slow member).
-__global__ void __launch_bounds__(64) k_finish(const m3Dev* __restrict__ ds, int ccd)
+// Island count and sleep timer of body i; a body still too young to sleep
+// clears its island's readiness flag (raised by the mover pass).
+__device__ static inline void SleepTimer(const m3Dev& d, int32_t i, float dt, int sleepEnabled)
+{
+    const m3View& v = d.v;
+    if (v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC || v.awake[i] == 0)
+    {
+        return;
+    }
+    const int32_t r = d.root[i];
+    if (r == i)
+    {
+        atomicAdd(&v.sc->islandCount, 1);
+    }
+    if (sleepEnabled)
+    {
+        m3SleepTimer(&v, i, dt);
+        if (v.sleepTime[i] < 0.5f)
+        {
+            v.flag[r] = 0;
+        }
+    }
+}
+
+// The step's closing work, one item per thread: constraint c's
+// accumulators back into its manifold and mover c's force cleared
+// (src/solver.c:4210-4240); body i's continuous pass when `ccd` is set;
+// and, with `timers`, body i's sleep timer after its own continuous pass
+// (without bullets a body's pass moves only that body).
+__global__ void __launch_bounds__(64) k_finish(const m3Dev* __restrict__ ds, int ccd, int timers, float dt,
+                                               int sleepEnabled)
 {
     M3_WORLD;
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -3927,26 +3956,30 @@ __global__ void __launch_bounds__(64) k_finish(const m3Dev* __restrict__ ds, int
     {
         return;
     }
-    v.flag[i] = 1;
-    if (ccd == 0)
+    if (ccd != 0)
     {
-        return;
-    }
-    if (d.ccdBullets == 0)
-    {
-        int pulled = m3ContinuousBody(&v, i);
-        if (d.tape != nullptr)
+        if (d.ccdBullets == 0)
         {
-            d.tape->ccd[i] = (uint8_t)pulled;
+            int pulled = m3ContinuousBody(&v, i);
+            if (d.tape != nullptr)
+            {
+                d.tape->ccd[i] = (uint8_t)pulled;
+            }
         }
-        return;
+        else
+        {
+            d.ccdHit[i] = 0;
+            m3Transform xf;
+            if (v.bullet[i] == 0 && m3ContinuousSweep(&v, nullptr, i, &xf))
+            {
+                d.ccdXf[i] = xf;
+                d.ccdHit[i] = 1;
+            }
+        }
     }
-    d.ccdHit[i] = 0;
-    m3Transform xf;
-    if (v.bullet[i] == 0 && m3ContinuousSweep(&v, nullptr, i, &xf))
+    if (timers != 0)
     {
-        d.ccdXf[i] = xf;
-        d.ccdHit[i] = 1;
+        SleepTimer(d, i, dt, sleepEnabled);
     }
 }
 
@@ -3998,30 +4031,17 @@ __global__ void k_pair_table(const m3Dev* __restrict__ ds, uint64_t* tableKeys, 
     }
 }
 
-// Island count and sleep timers; a body still too young to sleep clears
-// its island's readiness flag (raised by k_finish).
+// The sleep timers in a launch of their own (worlds with bullets, whose
+// pulled-back poses land after every body's continuous pass).
 __global__ void k_sleep_timers(const m3Dev* __restrict__ ds, float dt, int sleepEnabled)
 {
     M3_WORLD;
-    const int32_t* root = d.root;
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= v.maxBody || v.sc->stall != 0 || v.bodyAlive[i] == 0 || v.type[i] != M3_DYNAMIC ||
-        v.awake[i] == 0)
+    if (i >= v.maxBody || v.sc->stall != 0)
     {
         return;
     }
-    if (root[i] == i)
-    {
-        atomicAdd(&v.sc->islandCount, 1);
-    }
-    if (sleepEnabled)
-    {
-        m3SleepTimer(&v, i, dt);
-        if (v.sleepTime[i] < 0.5f)
-        {
-            v.flag[root[i]] = 0;
-        }
-    }
+    SleepTimer(d, i, dt, sleepEnabled);
 }
 
 // The step's close: thread 0 counts the step and ends the full query;
@@ -6121,8 +6141,9 @@ static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t
     }
     // Impulse store, the continuous pass and the sleep flags' reset.
     const int32_t np = v.pairCapacity;
+    const int bullets = p->continuousEnabled && w->dev.ccdBullets;
     STAGE(ST_STORE, KL(k_finish, dim3(Blocks(np > nb ? np : nb, 64)), dim3(64), 0, st, D,
-                       p->continuousEnabled ? 1 : 0));
+                       p->continuousEnabled ? 1 : 0, bullets ? 0 : 1, p->dt, p->sleepEnabled));
     if (p->continuousEnabled)
     {
         if (w->dev.ccdBullets)
@@ -6131,8 +6152,11 @@ static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t
             STAGE(ST_CCD, KL(k_ccd_apply, dim3(Blocks(nb, T)), dim3(T), 0, st, D));
         }
     }
-    STAGE(ST_SLEEP_TIMERS,
-          KL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt, p->sleepEnabled));
+    if (bullets)
+    {
+        STAGE(ST_SLEEP_TIMERS,
+              KL(k_sleep_timers, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->dt, p->sleepEnabled));
+    }
     STAGE(ST_END, KL(k_close, dim3(Blocks(nb, T)), dim3(T), 0, st, D, p->sleepEnabled));
     if (w->pairTable)
     {
@@ -6774,8 +6798,8 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     {
         KL(k_store_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);
     }
-    KL(k_finish, dim3(Blocks(np > nb ? np : nb, 64), W), dim3(64), 0, st, D, p->continuousEnabled ? 1 : 0);
-    KL(k_sleep_timers, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->dt, p->sleepEnabled);
+    KL(k_finish, dim3(Blocks(np > nb ? np : nb, 64), W), dim3(64), 0, st, D, p->continuousEnabled ? 1 : 0, 1,
+       p->dt, p->sleepEnabled);
     KL(k_close, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->sleepEnabled);
     HIP_TRY(hipGetLastError());
     return 0;
