@@ -3551,21 +3551,26 @@ __device__ static inline void CellPass(const m3View& v, const m3SubstepParams& s
             {
                 const int32_t slots = m3ContactSlots(in.pointCount);
                 m3ContactImpulses(&v.cons[k], &in, &cb, kMode == 1, slots, &s);
+                // Global words first: once the LDS version lets another
+                // wavefront of this cell write the body's next version,
+                // this one's stores to the same words are already issued
+                // ahead of it, so the next version cannot be overwritten by
+                // this older one landing late.
                 if (in.dynA)
                 {
-                    if (fl & 2u)
-                    {
-                        CellPut(L, ownA, s.vA, s.wA, wantA + 1u);
-                    }
                     FlowStore(rec, in.a, s.vA, s.wA, wantA + 1u);
                 }
                 if (in.dynB)
                 {
-                    if (fl & 8u)
-                    {
-                        CellPut(L, ownB, s.vB, s.wB, wantB + 1u);
-                    }
                     FlowStore(rec, in.b, s.vB, s.wB, wantB + 1u);
+                }
+                if (in.dynA && (fl & 2u))
+                {
+                    CellPut(L, ownA, s.vA, s.wA, wantA + 1u);
+                }
+                if (in.dynB && (fl & 8u))
+                {
+                    CellPut(L, ownB, s.vB, s.wB, wantB + 1u);
                 }
                 progressed = true;
                 spins = 0;
@@ -3766,8 +3771,8 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             }
             v.lv[b] = lin;
             v.av[b] = ang;
+            FlowStore(rec, b, lin, ang, solvePass << 8); // global before LDS, as in CellPass
             CellPut(L, j, lin, ang, solvePass << 8);
-            FlowStore(rec, b, lin, ang, solvePass << 8);
         }
         // No grid barrier opens a pass: a constraint waits for its bodies'
         // versions.  The solve pass reads the deltas directly (the last
@@ -3801,8 +3806,8 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             v.av[b] = ang;
             m3IntegratePosition(&v, b, sp.h);
             DeltaStore(d.deltaRec, b, v.deltaPos[b], v.deltaRot[b], (solvePass + 1u) << 8);
-            CellPut(L, j, v.lv[b], v.av[b], (solvePass + 1u) << 8);
             FlowStore(rec, b, v.lv[b], v.av[b], (solvePass + 1u) << 8);
+            CellPut(L, j, v.lv[b], v.av[b], (solvePass + 1u) << 8);
         }
         CellArm(L, solvePass + 1u);
         {
@@ -5119,10 +5124,11 @@ static int CreateImpl(m3gpuWorld* w)
         w->coopMax = maxBlocks;
         const char* flowEnv = getenv("MAUL3D_AMD_FLOW");
         w->flow = flowEnv != nullptr && atoi(flowEnv) == 0 ? 0 : 1;
-        // About 100 movers per cell (MAUL3D_AMD_CELLS overrides), as a grid
+        // About 63 movers per cell (160 cells for the 10k rain, the fastest
+        // of 48-256 measured there; MAUL3D_AMD_CELLS overrides), as a grid
         // of x slabs by z cells, within the co-resident limit.
         const char* cellEnv = getenv("MAUL3D_AMD_CELLS");
-        int cells = cellEnv != nullptr ? atoi(cellEnv) : (nb + 99) / 100;
+        int cells = cellEnv != nullptr ? atoi(cellEnv) : (nb + 62) / 63;
         const int cellMax = maxBlocks < M3_CELL_MAX ? maxBlocks : M3_CELL_MAX;
         cells = cells < 1 ? 1 : (cells > cellMax ? cellMax : cells);
         int gx = 1;
