@@ -2401,12 +2401,14 @@ __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_islands(const m3Dev* __r
     }
 }
 
-// The step's constraints: flags, then the ordered compaction.
+// The step's constraints: flags, then the ordered compaction, over the
+// live pairs only (the pair count is the previous launch's).
 __global__ void __launch_bounds__(M3_COMPACT_THREADS) k_constraints(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
     const m3GridSync sync = {d.barrier, &v.sc->fault};
-    const int32_t chunks = Chunks(v.pairCapacity);
+    const int32_t live = v.sc->pairCount < v.pairCapacity ? v.sc->pairCount : v.pairCapacity;
+    const int32_t chunks = Chunks(live);
     for (int32_t c = blockIdx.x; c < chunks; c += gridDim.x)
     {
         const uint8_t f = ConstraintFlag(d, c * M3_COMPACT_THREADS + threadIdx.x);
@@ -3134,17 +3136,17 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
     };
     const int32_t nc = v.sc->stall != 0 ? 0 : v.sc->colorStarts[M3_GRAPH_COLORS];
     bool veto = false;
-    for (int32_t k0 = t; k0 < nc; k0 += 4 * M3_CELL_BINS)
+    for (int32_t k0 = t; k0 < nc; k0 += 8 * M3_CELL_BINS)
     {
-        int2 ab[4];
+        int2 ab[8];
         M3_UNROLL
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 8; ++q)
         {
             const int32_t k = k0 + q * M3_CELL_BINS;
             ab[q] = k < nc ? d.slotBodies[k] : make_int2(-1, -1);
         }
         M3_UNROLL
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 8; ++q)
         {
             const int32_t k = k0 + q * M3_CELL_BINS;
             if (k >= nc)
