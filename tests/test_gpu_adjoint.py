@@ -156,6 +156,76 @@ def test_trajectory_loss_on_device_equals_float_twin():
         assert torch.allclose(a, b, rtol=1e-5, atol=1e-5)
 
 
+def test_device_gradients_track_fp64_differences_on_all_64_worlds():
+    """The device's gradients (precision 32, the default) on the full toss
+    configuration, all 64 worlds and 120 steps, against FP64 central
+    differences of the FP64 twin's loss on two random parameters per world
+    (contact-stable samples: steps eps and 4 eps agree to 1e-5).  The device
+    differentiates its own float trajectory, which drifts from the double
+    one over 120 contact-rich steps, so a sample may miss the FP64 slope by
+    more than 1e-3: at least two thirds of the stable samples must match to
+    1e-3 and every one to 5e-2 (the float twin, bit-equal to the device,
+    gives 7 of 9 within 1e-3 and a worst of 2.7e-2 on these samples)."""
+    import random
+
+    from tests.test_adjoint import EPS, KINDS, TWIN64
+    from tests.test_adjoint import adjoint as twin_adjoint
+
+    if not os.path.exists(TWIN64):
+        pytest.skip("oracle twin not built (make -C oracle/twin)")
+    T64 = api.Api(C.CDLL(TWIN64))
+    gpu = product_api()
+    seeds = list(range(64))
+    wg = [scenes.toss(gpu, s) for s in seeds]
+    n = len(seeds)
+    defs = (api.AdjointDef * n)()
+    res = (api.AdjointResult * n)()
+    keep = []
+    for k in range(n):
+        d = gpu.m3DefaultAdjointDef()
+        d.steps = 120
+        d.checkpointEvery = 10
+        d.target = api.Vec3(1.0, 0.5, -2.0)
+        defs[k] = d
+        arrs = {}
+        for name in NAMES[:3]:
+            a = (api.Vec3 * NB)()
+            arrs[name] = a
+            setattr(res[k], name, C.cast(a, C.POINTER(api.Vec3)))
+        for name in NAMES[3:]:
+            a = (C.c_float * NB)()
+            arrs[name] = a
+            setattr(res[k], name, C.cast(a, C.POINTER(C.c_float)))
+        keep.append(arrs)
+    ids = (api.WorldId * n)(*wg)
+    assert gpu.m3World_Adjoint(ids, n, defs, res) == n
+    rng = random.Random(0xF32)
+    errs = []
+    for k, s in enumerate(seeds):
+        assert res[k].unsupported == 0
+        w64 = scenes.toss(T64, s)
+        for _ in range(2):
+            kind = rng.choice([1, 2, 3, 4, 5])
+            body = rng.randrange(1, 33)
+            comp = rng.randrange(3) if kind in (1, 2, 5) else 0
+            eps = EPS[kind]
+            L = lambda e: twin_adjoint(T64, w64, 120, (kind, body, comp, e), True, every=10)[0].loss  # noqa: E731
+            fd = (L(eps) - L(-eps)) / (2 * eps)
+            fd4 = (L(4 * eps) - L(-4 * eps)) / (8 * eps)
+            scale = max(abs(fd), 1e-6)
+            if abs(fd - fd4) / scale > 1e-5:
+                continue
+            a = keep[k][KINDS[kind]][body]
+            dev = (a.x, a.y, a.z)[comp] if kind in (1, 2, 5) else a
+            errs.append((abs(dev - fd) / scale, s, KINDS[kind], body, comp, dev, fd))
+        T64.m3DestroyWorld(w64)
+    for w in wg:
+        gpu.m3DestroyWorld(w)
+    assert len(errs) >= 6, f"only {len(errs)} contact-stable samples"
+    assert max(e[0] for e in errs) < 5e-2, max(errs)
+    assert 3 * sum(e[0] < 1e-3 for e in errs) >= 2 * len(errs), sorted(errs, reverse=True)[:4]
+
+
 @pytest.mark.skip(reason="precision 64 is switched off on the device: its first MI355X run ended in a "
                   "GPU memory fault whose cause is not yet shown (its driver runs clean on the CPU)")
 def test_fp64_device_gradients_match_fp64_differences_on_all_64_worlds():
