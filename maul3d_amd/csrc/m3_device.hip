@@ -4672,30 +4672,60 @@ __global__ void __launch_bounds__(M3_RSORT_THREADS) k_sap_sort(const m3Dev* __re
 #define M3_GROUP_BITMAP_WORDS 2048 // higher shapes < 65536
 
 // Exclusive scan of in[0, n) into out[0, n] (out[n] = total) by one
-// workgroup of M3_SORT_THREADS; groups above `big` keys are listed.
+// workgroup of M3_SORT_THREADS; groups above `big` keys are listed.  Each
+// thread takes a contiguous run of the entries (its loads issued eight at
+// a time), one workgroup scan places the runs, and each thread writes its
+// run's prefixes.
 __device__ static inline void GroupScan(const int32_t* in, int32_t* out, int32_t n, int32_t big,
                                         int32_t* list, int32_t* listCount)
 {
-    int32_t carry = 0;
-    for (int32_t c = 0; c < n; c += blockDim.x)
+    const int32_t per = (n + (int32_t)blockDim.x - 1) / (int32_t)blockDim.x;
+    const int32_t b = (int32_t)threadIdx.x * per;
+    int32_t sum = 0;
+    for (int32_t j = 0; j < per; j += 8)
     {
-        const int32_t i = c + threadIdx.x;
-        const int32_t x = i < n ? in[i] : 0;
-        int32_t total = 0;
-        const int32_t pos = BlockExclusiveScan(x, &total);
-        if (i < n)
+        int32_t x[8];
+        M3_UNROLL
+        for (int32_t q = 0; q < 8; ++q)
         {
-            out[i] = carry + pos;
-            if (big >= 0 && x > big)
+            const int32_t i = b + j + q;
+            x[q] = (j + q < per && i < n) ? in[i] : 0;
+        }
+        M3_UNROLL
+        for (int32_t q = 0; q < 8; ++q)
+        {
+            sum += x[q];
+        }
+    }
+    int32_t total = 0;
+    int32_t pos = BlockExclusiveScan(sum, &total);
+    for (int32_t j = 0; j < per; j += 8)
+    {
+        int32_t x[8];
+        M3_UNROLL
+        for (int32_t q = 0; q < 8; ++q)
+        {
+            const int32_t i = b + j + q;
+            x[q] = (j + q < per && i < n) ? in[i] : 0;
+        }
+        M3_UNROLL
+        for (int32_t q = 0; q < 8; ++q)
+        {
+            const int32_t i = b + j + q;
+            if (j + q < per && i < n)
             {
-                list[atomicAdd(listCount, 1)] = i;
+                out[i] = pos;
+                pos += x[q];
+                if (big >= 0 && x[q] > big)
+                {
+                    list[atomicAdd(listCount, 1)] = i;
+                }
             }
         }
-        carry += total;
     }
     if (threadIdx.x == 0)
     {
-        out[n] = carry;
+        out[n] = total;
     }
 }
 
