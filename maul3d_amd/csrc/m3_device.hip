@@ -89,6 +89,7 @@ struct m3Dev
     int32_t* cellCons; // per cell: its slots (any order)
     uint8_t* slotOwner;
     int2* slotBodies; // per slot: its awake dynamic bodies (or -1), for the cell owners
+    uint32_t* cellDone; // per cell: the substeps whose relax pass (and addends) it has finished
     m3Tape* tape;      // intermediate-value record of this step (adjoint replays), or null
     // Pairs binned by collision routine: 8 counts, then one bin of
     // pairCapacity entries per routine.
@@ -2992,6 +2993,7 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
     {
         cnt[t] = 0;
         consCnt[t] = 0;
+        d.cellDone[t] = 0u;
     }
     hx[t] = 0;
     hz[t] = 0;
@@ -3195,6 +3197,7 @@ struct m3CellLds
     uint16_t loc[M3_CELL_CONS][2]; // per side: where it reads (LDS body or mailbox index)
     uint16_t own[M3_CELL_CONS][2]; // per side: the LDS body it writes (home sides)
     uint8_t flags[M3_CELL_CONS];   // per side: bit 0 / 2 read LDS body, bit 1 / 3 write LDS body
+    uint32_t nbr[M3_CELL_MAX / 32]; // cells whose finished substeps this one waits for
     int32_t count;
     int32_t imports;
 };
@@ -3627,11 +3630,31 @@ __device__ static inline void CellSetup(const m3View& v, const m3Dev& d, m3CellL
         L.count = count;
         L.imports = 0;
     }
+    if (t < M3_CELL_MAX / 32)
+    {
+        L.nbr[t] = 0u;
+    }
     for (int32_t i = t; i < count; i += blockDim.x)
     {
         L.slot[i] = d.cellCons[cell * M3_CELL_CONS + i];
     }
     __syncthreads();
+    // Neighbours: the owners of the constraints on this cell's bodies (their
+    // warm-start addends) and, below, the home cells of the bodies of this
+    // cell's constraints (their position deltas).
+    for (int32_t j = t; j < nHome; j += blockDim.x)
+    {
+        const int32_t b = d.cellBodies[cell * M3_CELL_BODIES + j];
+        for (uint32_t m = v.slotMask[b] & 0xFFFFu; m != 0u; m &= m - 1u)
+        {
+            const int32_t slot = v.slotTable[(int64_t)b * M3_GRAPH_COLORS + __builtin_ctz(m)] >> 1;
+            const uint32_t o = d.slotOwner[slot];
+            if (o != (uint32_t)cell && o < M3_CELL_MAX)
+            {
+                atomicOr(&L.nbr[o >> 5], 1u << (o & 31u));
+            }
+        }
+    }
     CellSort(L, count);
     for (int32_t i = t; i < count; i += blockDim.x)
     {
@@ -3650,6 +3673,11 @@ __device__ static inline void CellSetup(const m3View& v, const m3Dev& d, m3CellL
             }
             const int32_t b = side == 0 ? c->bodyA : c->bodyB;
             const bool home = d.cellOf[b] == (uint8_t)cell;
+            if (!home && d.cellOf[b] < M3_CELL_MAX)
+            {
+                const uint32_t hc = d.cellOf[b];
+                atomicOr(&L.nbr[hc >> 5], 1u << (hc & 31u));
+            }
             const uint32_t below = v.slotMask[b] & lower;
             bool prevHere = below == 0u;
             if (!prevHere)
@@ -3735,6 +3763,45 @@ __device__ static inline void CellArm(m3CellLds& L, uint32_t pass)
     }
 }
 
+// After its relax pass (and the addends it leaves) a cell publishes the
+// substep as done: every wavefront's stores are acknowledged, then one
+// agent-scope release makes them visible to any XCD that acquires.
+__device__ static inline void CellSignal(uint32_t* done, int32_t cell, uint32_t substeps)
+{
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&done[cell], substeps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Before substep `sub` a cell waits until each neighbour has finished
+// substep sub - 1 (lanes of wavefront 0 poll four cells each), then
+// acquires, so the neighbours' addends and position deltas are visible.
+__device__ static inline void CellWait(const m3CellLds& L, const uint32_t* done, uint32_t sub, int32_t* fault)
+{
+    if (threadIdx.x < 64)
+    {
+        for (int32_t c = (int32_t)threadIdx.x; c < M3_CELL_MAX; c += 64)
+        {
+            if ((L.nbr[c >> 5] >> (c & 31)) & 1u)
+            {
+                uint32_t spins = 0;
+                uint64_t since = 0;
+                while (__hip_atomic_load(&done[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < sub &&
+                       !FlowGiveUp(&spins, &since, fault))
+                {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
 // The substep loop of src/solver.c:4085-4203 over cells.  Body phases run
 // in each body's home cell.
 __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, const m3SubstepParams& sp,
@@ -3754,6 +3821,10 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
     for (int32_t sub = 0; sub < sp.substeps; ++sub)
     {
         const uint32_t solvePass = 2u * (uint32_t)sub;
+        if (sub > 0 && !kinematic)
+        {
+            CellWait(L, d.cellDone, (uint32_t)sub, fault);
+        }
         for (int32_t j = t; j < nHome; j += blockDim.x)
         {
             const int32_t b = home[j];
@@ -3826,8 +3897,19 @@ __device__ static inline void SubstepCells(const m3View& v, const m3Dev& d, cons
             CellPass<2>(v, sp, L, rec, d.flowRank, d.deltaRec, d.warmAdd, (solvePass + 1u) << 8,
                         sub + 1 < sp.substeps, solvePass + 1u, &first);
         }
-        // The next warm start reads the addends the relax pass left.
-        sync();
+        // The next warm start reads the addends the relax pass left, and the
+        // next solve pass the position deltas: the neighbours that wrote
+        // them (owners of this cell's bodies' constraints, homes of its
+        // constraints' bodies) are waited for at the top of the loop.  A
+        // contact on a kinematic body keeps the grid barrier.
+        if (kinematic)
+        {
+            sync();
+        }
+        else
+        {
+            CellSignal(d.cellDone, cell, (uint32_t)sub + 1u);
+        }
     }
     for (int32_t j = t; j < nHome; j += blockDim.x)
     {
@@ -5082,6 +5164,7 @@ static int CreateImpl(m3gpuWorld* w)
         DevAlloc(&w->dev.cellBodies, (int64_t)M3_CELL_MAX * M3_CELL_BODIES) ||
         DevAlloc(&w->dev.cellCount, M3_CELL_MAX) || DevAlloc(&w->dev.cellConsCount, M3_CELL_MAX) ||
         DevAlloc(&w->dev.slotOwner, np) || DevAlloc(&w->dev.slotBodies, np) ||
+        DevAlloc(&w->dev.cellDone, M3_CELL_MAX) ||
         DevAlloc(&w->dev.cellCons, (int64_t)M3_CELL_MAX * M3_CELL_CONS) ||
         DevAlloc(&w->hitRecBuf, np) || DevAlloc(&v.shapeRank, ns) || DevAlloc(&v.ccdStatics, ns) ||
         DevAlloc(&w->dev.ccdXf, nb) || DevAlloc(&w->dev.ccdHit, nb))
@@ -5236,7 +5319,7 @@ extern "C" void m3gpu_world_destroy(m3gpuWorld* w)
                     w->dev.bodyMasks, w->dev.bodyOwner, v.slotColors, v.consBodies, v.slotOf,
                     v.slotTable, v.slotMask, w->dev.barrier, w->dev.flowRec, w->dev.flowRank,
                     w->dev.warmAdd, w->dev.deltaRec, w->dev.cellOf, w->dev.cellLocal, w->dev.cellBodies, w->dev.cellCount,
-                    w->dev.cellConsCount, w->dev.slotOwner, w->dev.slotBodies, w->dev.cellCons, w->devDev,
+                    w->dev.cellConsCount, w->dev.slotOwner, w->dev.slotBodies, w->dev.cellDone, w->dev.cellCons, w->devDev,
                     v.jointAlive, v.jointType, v.jointBodyA, v.jointBodyB, v.jointLocalA,
                     v.jointLocalB, v.jointCollide, v.jointFrameQA, v.jointFrameQB, v.jointFlags,
                     v.jointLimits, v.jointImpulse, v.jointPerpImpulse, v.jointLimitImpulse,
