@@ -2807,13 +2807,15 @@ __device__ static inline void SubstepLoop(const m3View& v, const m3SubstepParams
 //
 // Every participant takes its cell's constraints in slot order, every
 // mail lane polls all its imports each round, and all workgroups are
-// co-resident (cooperative launch), so the lowest unsolved slot is always
-// ready and the passes end.  Waits are bounded all the same: one that
-// gives up marks the world faulted, and every wait ends once it sees the
-// fault.  The grid meets once per substep, after the relax pass, whose
-// warm-start addends the next substep reads; the relax pass takes the
-// position deltas that integrate-positions left from versioned per-body
-// records.
+// co-resident (a grid within the occupancy limit), so the lowest unsolved
+// slot is always ready and the passes end.  Waits are bounded all the
+// same: one that gives up marks the world faulted, and every wait ends once
+// it sees the fault.  The relax pass takes the position deltas that
+// integrate-positions left from versioned per-body records.  Between
+// substeps no grid barrier: each cell publishes its finished substep
+// (release) and the next substep of a cell waits (acquire) only for its
+// neighbours, the cells whose plain stores it reads (warm-start addends,
+// position deltas).
 // ---------------------------------------------------------------------------
 // A wait gives up after this much wall time (s_memrealtime runs at
 // 100 MHz): far beyond any step, so only a broken wait reaches it.
