@@ -3092,10 +3092,24 @@ __global__ void __launch_bounds__(M3_CELL_BINS) k_cells(const m3Dev* __restrict_
     xOf[t] = (uint8_t)(sx < gx ? sx : gx - 1);
     zOf[t] = (uint8_t)(sz < gz ? sz : gz - 1);
     __syncthreads();
+    // The cached movers' bodies are still in registers (mb); the rest are
+    // loaded again.
     for (int32_t m = t; m < n; m += blockDim.x)
     {
         const float2 q = at(m);
-        const int32_t b = v.movers[m];
+        int32_t b = 0;
+        if (m < M3_CELL_CACHE)
+        {
+            M3_UNROLL
+            for (int32_t j = 0; j < kPer; ++j)
+            {
+                b = t + j * M3_CELL_BINS == m ? mb[j] : b;
+            }
+        }
+        else
+        {
+            b = v.movers[m];
+        }
         const int32_t cell = xOf[CellBin(q.x, box[0], box[1])] * gz + zOf[CellBin(q.y, box[2], box[3])];
         const int32_t local = atomicAdd(&cnt[cell], 1);
         d.cellOf[b] = (uint8_t)cell;
