@@ -1792,6 +1792,13 @@ __global__ void k_prepare(const m3Dev* __restrict__ ds, float hertz, float zeta,
         // rolling row writes a non-dynamic side's angular velocity.
         const int2 ab = make_int2(v.consBodies[2 * c], v.consBodies[2 * c + 1]);
         d.slotBodies[k] = ab;
+        // The constraint's rank on each awake dynamic body: the colors
+        // below its own in which the body has a constraint (the final masks
+        // k_color left).
+        const uint32_t lower = (1u << color) - 1u;
+        const uint32_t rankA = ab.x >= 0 ? (uint32_t)__popc((uint32_t)d.bodyMasks[ab.x] & lower) : M3_FLOW_NONE;
+        const uint32_t rankB = ab.y >= 0 ? (uint32_t)__popc((uint32_t)d.bodyMasks[ab.y] & lower) : M3_FLOW_NONE;
+        d.flowRank[k] = rankA | (rankB << 8);
         const bool awakeA = ab.x >= 0;
         const bool awakeB = ab.y >= 0;
         if ((con->dynA != 0) != awakeA || (con->dynB != 0) != awakeB ||
@@ -2030,6 +2037,14 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
     }
     // The list build reads every participant's colors from global memory.
     __syncthreads();
+    if (kLds && flowPrep)
+    {
+        // The final body masks, for each constraint's ranks (k_prepare).
+        for (int32_t i = t; i < v.maxBody; i += M3_COLOR_THREADS)
+        {
+            gMasks[i] = sMask[i];
+        }
+    }
     __shared__ int32_t starts[M3_GRAPH_COLORS + 2];
     if (t == 0)
     {
@@ -2056,10 +2071,27 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
     __syncthreads();
     const int32_t wave = t >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
+    // Eight chunks' colors are loaded at a time, ahead of their passes.
+    int32_t ahead[8];
     for (int32_t base = 0; base < n; base += M3_COLOR_THREADS)
     {
+        const int32_t q = (base / M3_COLOR_THREADS) & 7;
+        if (q == 0)
+        {
+            M3_UNROLL
+            for (int32_t j = 0; j < 8; ++j)
+            {
+                const int32_t cj = base + j * M3_COLOR_THREADS + t;
+                ahead[j] = cj < n ? (int32_t)v.colors[cj] : -1;
+            }
+        }
         int32_t c = base + t;
-        int32_t color = c < n ? (int32_t)v.colors[c] : -1;
+        int32_t color = -1;
+        M3_UNROLL
+        for (int32_t j = 0; j < 8; ++j)
+        {
+            color = j == q ? ahead[j] : color;
+        }
         int32_t rank = 0;
         for (int32_t k = 0; k <= M3_GRAPH_COLORS; ++k)
         {
@@ -2091,18 +2123,6 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
             int32_t slot = starts[color] + waveBase[color][wave] + rank;
             v.colorLists[slot] = c;
             v.slotColors[slot] = (uint8_t)color;
-            if (flowPrep)
-            {
-                // The constraint's rank on each awake dynamic body: the
-                // colors below its own in which the body has a constraint
-                // (the masks are final here).
-                const uint32_t lower = (1u << color) - 1u;
-                const int32_t A = v.consBodies[2 * c];
-                const int32_t B = v.consBodies[2 * c + 1];
-                const uint32_t rA = A >= 0 ? (uint32_t)__popc((uint32_t)masks[A] & lower) : M3_FLOW_NONE;
-                const uint32_t rB = B >= 0 ? (uint32_t)__popc((uint32_t)masks[B] & lower) : M3_FLOW_NONE;
-                d.flowRank[slot] = rA | (rB << 8);
-            }
         }
         __syncthreads();
     }
