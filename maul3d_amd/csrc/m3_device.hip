@@ -4055,28 +4055,35 @@ __device__ static inline void SleepTimer(const m3Dev& d, int32_t i, float dt, in
 
 // The step's closing work, one item per thread: constraint c's
 // accumulators back into its manifold and mover c's force cleared
-// (src/solver.c:4210-4240); body i's continuous pass when `ccd` is set;
-// and, with `timers`, body i's sleep timer after its own continuous pass
-// (without bullets a body's pass moves only that body).
+// (src/solver.c:4210-4240) in the workgroups below `bodyBase`; from
+// `bodyBase` on, body i's continuous pass when `ccd` is set and, with
+// `timers`, body i's sleep timer after its own continuous pass (without
+// bullets a body's pass moves only that body).  The two halves touch
+// disjoint data, and separate wavefronts run their load chains side by side.
 __global__ void __launch_bounds__(64) k_finish(const m3Dev* __restrict__ ds, int ccd, int timers, float dt,
-                                               int sleepEnabled)
+                                               int sleepEnabled, int32_t bodyBase)
 {
     M3_WORLD;
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (v.sc->stall != 0)
     {
         return;
     }
-    if (i < v.sc->moverCount)
+    if ((int32_t)blockIdx.x < bodyBase)
     {
-        const int32_t b = v.movers[i];
-        v.force[b] = m3V3(0.0f, 0.0f, 0.0f);
-        v.torque[b] = m3V3(0.0f, 0.0f, 0.0f);
+        const int32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c < v.sc->moverCount)
+        {
+            const int32_t b = v.movers[c];
+            v.force[b] = m3V3(0.0f, 0.0f, 0.0f);
+            v.torque[b] = m3V3(0.0f, 0.0f, 0.0f);
+        }
+        if (c < v.sc->constraintCount)
+        {
+            m3StoreOne(&v, &v.cons[c]);
+        }
+        return;
     }
-    if (i < v.sc->constraintCount)
-    {
-        m3StoreOne(&v, &v.cons[i]);
-    }
+    const int32_t i = ((int32_t)blockIdx.x - bodyBase) * blockDim.x + threadIdx.x;
     if (i >= v.maxBody)
     {
         return;
@@ -6299,8 +6306,9 @@ static int EnqueueStepEnd(m3gpuWorld* w, const m3gpuStepParams* p, const int32_t
     // Impulse store, the continuous pass and the sleep flags' reset.
     const int32_t np = v.pairCapacity;
     const int bullets = p->continuousEnabled && w->dev.ccdBullets;
-    STAGE(ST_STORE, KL(k_finish, dim3(Blocks(np > nb ? np : nb, 64)), dim3(64), 0, st, D,
-                       p->continuousEnabled ? 1 : 0, bullets ? 0 : 1, p->dt, p->sleepEnabled));
+    const int32_t finishBase = Blocks(np > nb ? np : nb, 64);
+    STAGE(ST_STORE, KL(k_finish, dim3(finishBase + Blocks(nb, 64)), dim3(64), 0, st, D,
+                       p->continuousEnabled ? 1 : 0, bullets ? 0 : 1, p->dt, p->sleepEnabled, finishBase));
     if (p->continuousEnabled)
     {
         if (w->dev.ccdBullets)
@@ -6955,8 +6963,9 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     {
         KL(k_store_joints, dim3(Blocks(s_batch.maxJoint, T), W), dim3(T), 0, st, D);
     }
-    KL(k_finish, dim3(Blocks(np > nb ? np : nb, 64), W), dim3(64), 0, st, D, p->continuousEnabled ? 1 : 0, 1,
-       p->dt, p->sleepEnabled);
+    const int32_t finishBase = Blocks(np > nb ? np : nb, 64);
+    KL(k_finish, dim3(finishBase + Blocks(nb, 64), W), dim3(64), 0, st, D, p->continuousEnabled ? 1 : 0, 1,
+       p->dt, p->sleepEnabled, finishBase);
     KL(k_close, dim3(Blocks(nb, T), W), dim3(T), 0, st, D, p->sleepEnabled);
     HIP_TRY(hipGetLastError());
     return 0;
