@@ -1892,6 +1892,19 @@ __device__ static inline void ColorBarrier()
     }
 }
 
+// Inclusive sum within each row of 16 lanes (DPP row shifts).
+__device__ static inline uint32_t RowInclusiveSum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false); // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false); // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false); // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false); // row_shr:8
+    return x;
+}
+
+static_assert(M3_COLOR_THREADS / 64 == 16, "k_color scans one DPP row of wave counts per color");
+static_assert((M3_GRAPH_COLORS + 1) * 16 <= M3_COLOR_THREADS, "k_color's color scan needs a row per color");
+
 template <bool kLds>
 __global__ void __launch_bounds__(M3_COLOR_THREADS)
     k_color(const m3Dev* __restrict__ ds, int flowPrep)
@@ -2106,16 +2119,20 @@ __global__ void __launch_bounds__(M3_COLOR_THREADS)
             }
         }
         __syncthreads();
-        if (t <= M3_GRAPH_COLORS)
+        // Each color's 16 wave counts are one DPP row: an exclusive scan
+        // across the row, on top of the color's running count.
+        if (t < (M3_GRAPH_COLORS + 1) * 16)
         {
-            int32_t acc = running[t];
-            for (int32_t w = 0; w < M3_COLOR_THREADS / 64; ++w)
+            const int32_t k = t >> 4;
+            const int32_t w = t & 15;
+            const uint32_t x = (uint32_t)waveBase[k][w];
+            const uint32_t incl = RowInclusiveSum(x);
+            const int32_t run = running[k];
+            waveBase[k][w] = run + (int32_t)(incl - x);
+            if (w == 15)
             {
-                int32_t x = waveBase[t][w];
-                waveBase[t][w] = acc;
-                acc += x;
+                running[k] = run + (int32_t)incl;
             }
-            running[t] = acc;
         }
         __syncthreads();
         if (color >= 0)
