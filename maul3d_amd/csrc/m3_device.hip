@@ -161,6 +161,7 @@ struct m3gpuWorld
     m3HitRec* hitRecBuf; // per pair; in the view only while some shape has hit events
     struct m3EventBufs* events; // event classification and output (allocated at first read)
     int boxHulls;    // some hull is at least box-sized
+    int smallHulls; // every hull within SatLdsBox (k_sat's small LDS layout)
 };
 
 // The entry is read through the constant address space: its fields load as
@@ -862,18 +863,25 @@ __global__ void k_planes(const m3Dev* __restrict__ ds)
 
 #define M3_HULL_MAX_EDGES (M3_HULL_MAX_HALF_EDGES / 2)
 
-struct SatLds
+template <int kVerts, int kEdges>
+struct SatLdsT
 {
-    m3Vec3 w[M3_HULL_MAX_VERTS]; // the other hull's vertices in the reference hull's frame
-    m3Vec3 aP[M3_HULL_MAX_EDGES]; // A's edges: origin, direction, the two face normals
-    m3Vec3 aE[M3_HULL_MAX_EDGES];
-    m3Vec3 aU[M3_HULL_MAX_EDGES];
-    m3Vec3 aV[M3_HULL_MAX_EDGES];
-    m3Vec3 bP[M3_HULL_MAX_EDGES]; // B's edges, moved into A's frame
-    m3Vec3 bE[M3_HULL_MAX_EDGES];
-    m3Vec3 bU[M3_HULL_MAX_EDGES];
-    m3Vec3 bV[M3_HULL_MAX_EDGES];
+    m3Vec3 w[kVerts]; // the other hull's vertices in the reference hull's frame
+    m3Vec3 aP[kEdges]; // A's edges: origin, direction, the two face normals
+    m3Vec3 aE[kEdges];
+    m3Vec3 aU[kEdges];
+    m3Vec3 aV[kEdges];
+    m3Vec3 bP[kEdges]; // B's edges, moved into A's frame
+    m3Vec3 bE[kEdges];
+    m3Vec3 bU[kEdges];
+    m3Vec3 bV[kEdges];
 };
+// Any hull (18.6 KB: eight workgroups per CU), and worlds whose hulls all
+// have at most 8 vertices and 12 edges, boxes (1.2 KB).
+typedef SatLdsT<M3_HULL_MAX_VERTS, M3_HULL_MAX_EDGES> SatLds;
+typedef SatLdsT<8, 12> SatLdsBox;
+#define M3_SAT_BOX_VERTS 8
+#define M3_SAT_BOX_HALF_EDGES 24
 
 // (separation, index) of the serial scan's choice among this lane's and the
 // other lanes' candidates: first exceeding index if any, else first maximum.
@@ -933,8 +941,9 @@ __device__ inline SatPick SatPickInit()
 }
 
 // m3QueryFaces over the lanes (same values, same choice).
+template <class Lds>
 __device__ m3FaceQuery WaveQueryFaces(const m3HullData* ref, const m3HullData* other, m3Quat q,
-                                      m3Vec3 p, int refIsA, SatLds& L, int lane)
+                                      m3Vec3 p, int refIsA, Lds& L, int lane)
 {
     int32_t nv = other->vertexCount;
     for (int32_t k = lane; k < nv; k += 64)
@@ -979,7 +988,8 @@ __device__ m3FaceQuery WaveQueryFaces(const m3HullData* ref, const m3HullData* o
 
 // Separation of edge pair (ja, jb) as m3QueryEdges computes it; false when
 // the Gauss-map test or the degenerate-axis test rejects the pair.
-__device__ inline bool SatEdgePair(const SatLds& L, int32_t ja, int32_t jb, m3Vec3 centerA,
+template <class Lds>
+__device__ inline bool SatEdgePair(const Lds& L, int32_t ja, int32_t jb, m3Vec3 centerA,
                                    m3real* sepOut, m3Vec3* axisOut)
 {
     m3Vec3 eA = L.aE[ja];
@@ -1011,8 +1021,9 @@ __device__ inline bool SatEdgePair(const SatLds& L, int32_t ja, int32_t jb, m3Ve
 
 // m3QueryEdges over the lanes: candidates in the serial scan order
 // t = (B edge) * (A edges) + (A edge).
+template <class Lds>
 __device__ m3EdgeQuery WaveQueryEdges(const m3HullData* hullA, const m3HullData* hullB, m3Quat q,
-                                      m3Vec3 p, SatLds& L, int lane)
+                                      m3Vec3 p, Lds& L, int lane)
 {
     int32_t nA = hullA->edgeCount >> 1;
     int32_t nB = hullB->edgeCount >> 1;
@@ -1088,7 +1099,8 @@ __device__ m3EdgeQuery WaveQueryEdges(const m3HullData* hullA, const m3HullData*
 // serial scan's choice, the first index of the largest separation (a lane
 // offers its candidates in ascending index, with the same strict test), so
 // the queries equal m3CollideSegmentHullQ's own loops bit for bit.
-__device__ m3SatQuery WaveSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 p2, SatLds& L, int lane)
+template <class Lds>
+__device__ m3SatQuery WaveSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 p2, Lds& L, int lane)
 {
     const int32_t nv = hull->vertexCount;
     for (int32_t k = lane; k < nv; k += 64)
@@ -1170,10 +1182,11 @@ __device__ m3SatQuery WaveSegmentHull(const m3HullData* hull, m3Vec3 p1, m3Vec3 
 }
 
 // One 64-lane workgroup per listed pair, grid-stride over the list.
+template <class Lds>
 __global__ __launch_bounds__(64) void k_sat(const m3Dev* __restrict__ ds)
 {
     M3_WORLD;
-    __shared__ SatLds L;
+    __shared__ Lds L;
     if (v.sc->stall != 0)
     {
         return;
@@ -5507,11 +5520,17 @@ extern "C" int m3gpu_upload(m3gpuWorld* w, const m3gpuHostState* h, uint32_t fie
         UP((void*)v.hulls, h->hullData, h->maxHull, m3HullData);
         w->wideHulls = 0;
         w->boxHulls = 0;
+        w->smallHulls = 1;
         for (int32_t k = 0; k < h->maxHull; ++k)
         {
-            int32_t e = ((const m3HullData*)h->hullData)[k].edgeCount;
+            const m3HullData* hull = &((const m3HullData*)h->hullData)[k];
+            int32_t e = hull->edgeCount;
             w->wideHulls |= e > 24;
             w->boxHulls |= e >= 24;
+            if (hull->vertexCount > M3_SAT_BOX_VERTS || e > M3_SAT_BOX_HALF_EDGES)
+            {
+                w->smallHulls = 0;
+            }
         }
         if (h->maxJoint > w->caps.jointCapacity)
         {
@@ -6167,7 +6186,16 @@ static int EnqueueStepA(m3gpuWorld* w, const m3gpuStepParams* p, bool leaves)
     // Narrowphase.
     if (w->wideHulls || w->boxHulls)
     {
-        STAGE(ST_SAT, KL(k_sat, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
+        if (w->smallHulls)
+        {
+            // The small layout leaves the registers, not LDS, to bound the
+            // resident workgroups: one per listed pair, up to 8192.
+            STAGE(ST_SAT, KL(k_sat<SatLdsBox>, dim3(SatBlocks(np, 1) * 4), dim3(64), 0, st, D));
+        }
+        else
+        {
+            STAGE(ST_SAT, KL(k_sat<SatLds>, dim3(SatBlocks(np, 1)), dim3(64), 0, st, D));
+        }
     }
     // The narrowphase, which also unites the bodies of each touching pair.
     STAGE(ST_NARROW, KL(k_narrow_binned, dim3(Blocks(np, 64) + M3_NARROW_CLASSES), dim3(64), 0, st, D));
@@ -6943,7 +6971,7 @@ static int BatchLaunches(hipStream_t st, const m3Dev* D, const m3gpuStepParams* 
     KL(k_select_block, dim3(1, W), dim3(BT), 0, st, D, 0);
     if (s_batch.wideHulls)
     {
-        KL(k_sat, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
+        KL(k_sat<SatLds>, dim3(SatBlocks(np, W), W), dim3(64), 0, st, D);
     }
     // The narrowphase binned by collision routine per world, which also
     // unites the bodies of each touching pair.
